@@ -1,0 +1,192 @@
+"""Benchmark: forward+adjoint mass-spring rollout throughput on one MI355X.
+
+Workload (one "step" of this harness = one full training rollout of the
+batch): the 3D quadruped (N=54 vertices, S=323 springs, 4 actuator groups,
+MLP 179-64-4 tanh) with 1024 target tasks, FP32, horizon 1000 advances,
+checkpoints every 50 steps recomputed in the backward pass, windowed COM
+loss over the last 500 states, gradients of every MLP weight.  Inputs and
+weights are synthetic (seeded), resident in HBM before the timed region.
+
+Printed as ONE JSON line (rank 0):
+  value            batched persistent robot-steps/s (forward + adjoint)
+  plain            one-robot-per-launch form (reference launch structure,
+                   per-step kernels + tape replay) on a sample of robots
+  roofline         HBM roofline of the dominant kernel (adjoint rollout)
+  cpu_baseline     FP64 CPU restatement (oracle), forward+backward, on a
+                   bounded sample of the same workload
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "robot-steps/s fwd+bwd batched vs plain; MPM particle-steps/s; p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(N, S, I, H, O, K, R_d, dtype_bytes=4):
+    """Bytes per robot-step by kernel (state round-trip model): forward reads
+    and writes x,v (48N) plus a checkpoint every K steps (24N/K) plus one pass
+    over springs (20 B) and weights; the adjoint recomputes the forward (48N)
+    and runs the adjoint (read x,v, read/write xbar,vbar: 72N) with two passes
+    over springs and weights.  Weight/topology bytes amortise over the R_d
+    robots sharing a design."""
+    P = I * H + H * O + H + O
+    s = dtype_bytes / 4.0
+    fwd = (48 * N + 24 * N / K + (20 * S + 4 * P) / R_d) * s
+    bwd = (120 * N + (40 * S + 8 * P) / R_d) * s
+    return fwd, bwd
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--robots", type=int, default=1024)
+    ap.add_argument("--horizon", type=int, default=1000)
+    ap.add_argument("--ckpt", type=int, default=50)
+    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--plain-robots", type=int, default=2)
+    ap.add_argument("--cpu-robots", type=int, default=32)
+    ap.add_argument("--cpu-horizon", type=int, default=1000)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are not supported "
+              f"(asked for --gpus {args.gpus})", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from __graft_entry__ import load_package
+    dl = load_package()
+    from diffloco.rollout import MassSpringBatch
+    from diffloco.solver_mass_spring import _physics
+    from diffloco.taichi_env import sensor_layout
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    R, T, K, W = args.robots, args.horizon, args.ckpt, 500
+    design = dl.build_quadruped()
+    qc = dl.quadruped_config(batch_size=R, steps=T, window=W).get_config()
+    lay = sensor_layout(qc)
+    N, S, O, H = design.n_vert, design.n_spring, design.n_act, 64
+    I = qc["nn"]["n_input_states"]
+    rng = np.random.default_rng(1)
+    tv = np.stack([rng.uniform(-0.08, 0.08, R), np.zeros(R), rng.uniform(-0.08, 0.08, R)], 1)
+    th = rng.uniform(0.1, 0.25, R)
+    tg = np.concatenate([tv, th[:, None], np.zeros((R, 1))], 1)
+    x0 = design.objects + np.array([0.4, 0.0, 0.0])
+    batch = MassSpringBatch([design], [0] * R, T, _physics(qc), lay, tg, [x0] * R, act="tanh",
+                            ckpt_every=K, dtype=torch.float32, block=args.block, loss_window=W)
+    from diffloco.rollout import init_weights
+    w = init_weights(I, H, O, seed=0)
+    batch.set_weights(0, *w)
+    lg = np.full(R, 1.0 / R)
+
+    def one_step():
+        batch.reset()
+        batch.forward()
+        batch.backward(loss_grad=lg)
+
+    for _ in range(args.warmup):
+        one_step()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    fwd_ms, bwd_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.reset()
+        ev[0].record()
+        batch.forward()
+        ev[1].record()
+        ev[2].record()
+        batch.backward(loss_grad=lg)
+        ev[3].record()
+        torch.cuda.synchronize()
+        fwd_ms.append(ev[0].elapsed_time(ev[1]))
+        bwd_ms.append(ev[2].elapsed_time(ev[3]))
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    robot_steps = R * T
+    value = robot_steps * args.steps / wall
+    f_b, b_b = algorithmic_bytes(N, S, I, H, O, K, R)
+    bwd_avg_ms = float(np.mean(bwd_ms))
+    fwd_avg_ms = float(np.mean(fwd_ms))
+    achieved_bwd = b_b * robot_steps / (bwd_avg_ms * 1e-3) / 1e9
+    achieved_fwd = f_b * robot_steps / (fwd_avg_ms * 1e-3) / 1e9
+    loss = float(batch.loss.double().mean().item())
+    gnorm = float(batch.grad_out.double().norm().item())
+
+    # ---- plain form: per-step kernels, one robot per launch
+    plain = None
+    if not args.skip_plain and args.plain_robots > 0:
+        qc1 = dl.quadruped_config(batch_size=1, steps=T, window=W)
+        tr = dl.DiffPhyTrainer(qc1)
+        tr.taichi_env.setup_robot()
+        tr.nn.set_weights(*[x.float() for x in w])
+        tr.simulate_task(T, tv[:1], th[:1])  # warm-up
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for r in range(args.plain_robots):
+            tr.simulate_task(T, tv[r:r + 1], th[r:r + 1])
+        torch.cuda.synchronize()
+        pw = time.perf_counter() - t1
+        pv = args.plain_robots * T / pw
+        plain = {"value": pv, "unit": "robot-steps/s", "sample": f"{args.plain_robots} robots x {T} steps, "
+                 "per-step kernels (center, sensors, MLP, advance) + tape adjoints, B=1 per launch",
+                 "ms_per_robot_rollout": pw * 1e3 / args.plain_robots}
+
+    # ---- CPU baseline: FP64 oracle forward+backward on a bounded sample
+    cpu = None
+    if not args.skip_cpu:
+        from oracle import builders as ob
+        from oracle import mass_spring as om
+        Bc, Tc = args.cpu_robots, args.cpu_horizon
+        objects, springs, groups = ob.build_quadruped()
+        rob = om.make_robot(objects, springs, groups)
+        cfg = om.TaskConfig(steps=Tc, window=min(W, Tc))
+        w64 = [x.double().cpu().clone().requires_grad_() for x in w]
+        x0o, v0o = om.initial_state(rob, Bc)
+        t2 = time.perf_counter()
+        lo, _, _ = om.rollout_task(rob, cfg, w64, torch.tensor(tv[:Bc]), torch.tensor(th[:Bc]), x0o, v0o)
+        lo.backward()
+        cw = time.perf_counter() - t2
+        cpu = {"value": Bc * Tc / cw, "unit": "robot-steps/s", "cores": torch.get_num_threads(),
+               "kind": "port", "sample": f"FP64 torch-CPU oracle, {Bc} robots x {Tc} steps, "
+               "forward + autograd backward (no checkpointing)"}
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "robot-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded targets, N(0,1/fan_in) weights)",
+        "config": {"workload": "quadruped mass-spring rollout fwd+adjoint, 1024 tasks x 1000 steps, "
+                   "ckpt 50, MLP 179-64-4 tanh", "robots": R, "horizon": T, "ckpt_every": K,
+                   "n_vert": N, "n_spring": S, "mlp": [I, H, O], "block": args.block,
+                   "form": "batched persistent (one launch forward, one launch adjoint)"},
+        "roofline": {"bound": "hbm", "achieved": achieved_bwd, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved_bwd / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_rollout_bwd", "algorithmic_bytes_per_robot_step": b_b,
+                     "avg_launch_ms": bwd_avg_ms,
+                     "forward": {"kernel": "k_rollout_fwd", "achieved": achieved_fwd,
+                                 "avg_launch_ms": fwd_avg_ms,
+                                 "algorithmic_bytes_per_robot_step": f_b}},
+        "plain": plain,
+        "speedup_vs_plain": (value / plain["value"]) if plain else None,
+        "cpu_baseline": cpu,
+        "check": {"mean_loss": loss, "grad_norm": gnorm},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

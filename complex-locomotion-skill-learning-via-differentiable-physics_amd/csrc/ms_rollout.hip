@@ -1,0 +1,739 @@
+// Persistent batched mass-spring rollout: one workgroup owns one robot and
+// advances it through many steps with its state resident in LDS.  Robots of
+// different size, design and horizon share one launch through the offset
+// table (dl_robot).  The backward kernel walks each robot's checkpoint
+// segments in reverse: recompute the segment forward from its checkpoint into
+// a per-robot scratch buffer, then run the hand-written adjoint back through
+// it, and fold the segment's MLP weight gradients in once per segment.
+//
+// Per step and robot (forward):
+//   COM (block reduction) -> sensors -> MLP (2 layers) -> spring impulses
+//   -> per-vertex gather (CSR, fixed order) + integrate.
+
+#include "dl_common.hpp"
+
+namespace dl {
+
+constexpr int HMAX = 64;
+
+template <typename T>
+struct RArgs {
+    const dl_robot *robots;
+    const int32_t *sa, *sb, *sact;
+    const T *L0, *k, *coef;
+    const int32_t *vptr, *vidx, *aptr, *aidx;
+    const T *W;
+    T *Wg;
+    T *state, *adj, *ckpt, *scratch;
+    const T *targets;
+    T *loss;
+    const T *loss_grad;
+    T *traj;
+    const T *traj_grad;
+    Phys<T> P;
+    Layout L;
+    int H, K, max_steps, act, loss_kind, window, store_traj;
+    int max_n, max_s, max_i, max_o;
+    int springs_in_lds;
+};
+
+// Per-robot view (all pointers resolved from the offset table).
+template <typename T>
+struct Rob {
+    int N, S, O, I, Th;
+    const int32_t *sa, *sb, *sact, *vptr, *vidx, *aptr, *aidx;
+    const T *L0, *k, *coef;
+    const T *W1, *W1T, *b1, *W2, *W2T, *b2;
+    T *Wg1, *bg1, *Wg2, *bg2;
+    const T *tgt;
+    int tgt_stride;
+    T *state, *adj, *ckpt;
+    T *traj;
+    const T *trajg;
+    // scratch (segment of K steps)
+    T *sx, *sin, *shid, *sdhid, *sout, *sdout, *scv, *sz1, *sz2, *spill;
+};
+
+template <typename T>
+__device__ __forceinline__ Rob<T> make_rob(const RArgs<T> &A, int r) {
+    const dl_robot R = A.robots[r];
+    Rob<T> q;
+    q.N = R.n_vert; q.S = R.n_spring; q.O = R.n_act; q.I = R.n_in; q.Th = R.horizon;
+    q.sa = A.sa + R.spring0; q.sb = A.sb + R.spring0; q.sact = A.sact + R.spring0;
+    q.L0 = A.L0 + R.spring0; q.k = A.k + R.spring0; q.coef = A.coef + R.spring0;
+    q.vptr = A.vptr + R.vptr0; q.vidx = A.vidx;   // vptr rows hold absolute vidx positions
+    q.aptr = A.aptr + R.aptr0; q.aidx = A.aidx;
+    const int H = A.H, I = q.I, O = q.O;
+    const T *w = A.W + R.weights;
+    q.W1 = w; q.W1T = w + (size_t)H * I; q.b1 = q.W1T + (size_t)I * H;
+    q.W2 = q.b1 + H; q.W2T = q.W2 + (size_t)O * H; q.b2 = q.W2T + (size_t)H * O;
+    T *g = A.Wg + R.wgrad;
+    q.Wg1 = g; q.bg1 = g + (size_t)H * I; q.Wg2 = q.bg1 + H; q.bg2 = q.Wg2 + (size_t)O * H;
+    q.tgt = A.targets + R.targets;
+    q.tgt_stride = R.target_stride;
+    q.state = A.state + R.state;
+    q.adj = A.adj ? A.adj + R.state : nullptr;
+    q.ckpt = A.ckpt + R.ckpt;
+    q.traj = (A.traj && R.traj >= 0) ? A.traj + R.traj : nullptr;
+    q.trajg = (A.traj_grad && R.traj >= 0) ? A.traj_grad + R.traj : nullptr;
+    const int K = A.K, N = q.N;
+    T *s = A.scratch ? A.scratch + R.scratch : nullptr;
+    if (s) {
+        q.sx = s; s += (size_t)K * 6 * N;
+        q.sin = s; s += (size_t)K * I;
+        q.shid = s; s += (size_t)K * H;
+        q.sdhid = s; s += (size_t)K * H;
+        q.sz1 = s; s += (size_t)K * H;
+        q.sout = s; s += (size_t)K * O;
+        q.sdout = s; s += (size_t)K * O;
+        q.sz2 = s; s += (size_t)K * O;
+        q.scv = s; s += (size_t)K * 6;
+        q.spill = s;
+    } else {
+        q.sx = q.sin = q.shid = q.sdhid = q.sz1 = q.sout = q.sdout = q.sz2 = q.scv = q.spill = nullptr;
+    }
+    return q;
+}
+
+// Shared-memory carve-up (sized for the largest robot of the launch).
+template <typename T>
+struct Sm {
+    T *xs, *vs;        // [3][N] each, stride N
+    T *xb, *vb, *vib;  // adjoint state / v_inc adjoint
+    T *J;              // [3][S] impulses, then [7][S] adjoint terms (after J)
+    T *inb, *inbar;    // [I]
+    T *hid, *dhid, *z1b;   // [HMAX]
+    T *outb, *dout, *z2b;  // [O]
+    T *red;            // reductions: 16*HMAX + 96
+    T *cv;             // center[3], vcom[3], cbar[3], vcbar[3]
+};
+
+template <typename T>
+__device__ __forceinline__ Sm<T> carve(const RArgs<T> &A, unsigned char *raw, T *spill, bool bwd) {
+    Sm<T> m;
+    T *p = reinterpret_cast<T *>(raw);
+    const int n3 = 3 * A.max_n;
+    m.xs = p; p += n3;
+    m.vs = p; p += n3;
+    if (bwd) {
+        m.xb = p; p += n3;
+        m.vb = p; p += n3;
+        m.vib = p; p += n3;
+    } else {
+        m.xb = m.vb = m.vib = nullptr;
+    }
+    if (A.springs_in_lds) {
+        m.J = p;
+        p += (bwd ? 10 : 3) * A.max_s;
+    } else {
+        m.J = spill;
+    }
+    m.inb = p; p += A.max_i;
+    m.inbar = p; p += bwd ? A.max_i : 0;
+    m.hid = p; p += HMAX;
+    m.dhid = p; p += HMAX;
+    m.z1b = p; p += HMAX;
+    m.outb = p; p += A.max_o;
+    m.dout = p; p += A.max_o;
+    m.z2b = p; p += A.max_o;
+    m.cv = p; p += 16;
+    m.red = p;
+    return m;
+}
+
+template <typename T>
+__host__ size_t smem_bytes(int max_n, int max_s, int max_i, int max_o, bool bwd, bool springs) {
+    size_t n = 6 * (size_t)max_n + (bwd ? 9 * (size_t)max_n : 0);
+    if (springs) n += (bwd ? 10 : 3) * (size_t)max_s;
+    n += (bwd ? 2 : 1) * (size_t)max_i + 3 * HMAX + 3 * (size_t)max_o + 16 + 16 * HMAX + 96;
+    return n * sizeof(T);
+}
+
+// COM position and velocity of the resident state -> m.cv[0..5]
+template <typename T>
+__device__ __forceinline__ void com(const Rob<T> &R, Sm<T> &m) {
+    T s[6] = {0, 0, 0, 0, 0, 0};
+    const T invN = T(1) / T(R.N);
+    for (int i = threadIdx.x; i < R.N; i += blockDim.x) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            s[d] += m.xs[d * R.N + i] * invN;
+            s[3 + d] += m.vs[d * R.N + i] * invN;
+        }
+    }
+    block_sum<T, 6>(s, m.red);
+    if (threadIdx.x < 6) m.cv[threadIdx.x] = s[threadIdx.x];
+    __syncthreads();
+}
+
+template <typename T>
+__device__ __forceinline__ const T *tgt_at(const Rob<T> &R, int t) {
+    if (R.tgt_stride == 0) return R.tgt;
+    const int tt = t < R.Th ? t : R.Th - 1;
+    return R.tgt + (size_t)tt * R.tgt_stride;
+}
+
+// Windowed COM loss term |vc_xz - tv_xz|^2 + (c_y - th)^2
+template <typename T>
+__device__ __forceinline__ T task_term(const T *cv, const T *tg) {
+    const T a = cv[3] - tg[0], b = cv[5] - tg[2], c = cv[1] - tg[3];
+    return a * a + b * b + c * c;
+}
+
+enum { MODE_FWD = 0, MODE_REC = 1 };
+
+// One forward step from the resident state at step t.  In MODE_REC the step
+// inputs and intermediates are written to scratch slot `slot`.
+template <typename T>
+__device__ void fwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int t, int mode, int slot,
+                         T &lsum) {
+    const int N = R.N, S = R.S, I = R.I, O = R.O, H = A.H;
+    const int tid = threadIdx.x, BT = blockDim.x;
+    com(R, m);
+    const T *tg = tgt_at(R, t);
+    if (mode == MODE_FWD) {
+        if (t % A.K == 0) {
+            T *ck = R.ckpt + (size_t)(t / A.K) * 6 * N;
+            for (int i = tid; i < 3 * N; i += BT) {
+                ck[i] = m.xs[i];
+                ck[3 * N + i] = m.vs[i];
+            }
+        }
+        if (A.store_traj && R.traj) {
+            T *row = R.traj + (size_t)t * (3 * N + 3);
+            for (int i = tid; i < 3 * N; i += BT) row[i] = m.xs[i];
+            if (tid < 3) row[3 * N + tid] = m.cv[tid];
+        }
+        if (A.loss_kind == 1 && t > R.Th - A.window && tid == 0) lsum += task_term(m.cv, tg);
+    } else {
+        T *sx = R.sx + (size_t)slot * 6 * N;
+        for (int i = tid; i < 3 * N; i += BT) {
+            sx[i] = m.xs[i];
+            sx[3 * N + i] = m.vs[i];
+        }
+        if (tid < 6) R.scv[slot * 6 + tid] = m.cv[tid];
+    }
+    // sensors
+    for (int q = tid; q < I; q += BT) {
+        const T val = sensor_value<T>(A.L, q, t, 0, N, m.xs, m.cv, m.cv + 3, tg);
+        m.inb[q] = val;
+        if (mode == MODE_REC) R.sin[(size_t)slot * I + q] = val;
+    }
+    __syncthreads();
+    // MLP layer 1: thread (j, chunk c); W1T rows are coalesced across j
+    {
+        const int j = tid & (HMAX - 1), c = tid / HMAX, C = BT / HMAX;
+        const int Ic = (I + C - 1) / C;
+        const int i0 = c * Ic, i1 = min(I, i0 + Ic);
+        T acc = T(0);
+        if (j < H)
+            for (int i = i0; i < i1; ++i) acc += R.W1T[(size_t)i * H + j] * m.inb[i];
+        m.red[c * HMAX + j] = acc;
+        __syncthreads();
+        if (tid < H) {
+            T z = R.b1[tid];
+            for (int cc = 0; cc < C; ++cc) z += m.red[cc * HMAX + tid];
+            T y, dy;
+            activate<T>(A.act, z, y, dy);
+            m.hid[tid] = y;
+            m.dhid[tid] = dy;
+            if (mode == MODE_REC) {
+                R.shid[(size_t)slot * H + tid] = y;
+                R.sdhid[(size_t)slot * H + tid] = dy;
+            }
+        }
+        __syncthreads();
+    }
+    // MLP layer 2: P lanes per output
+    {
+        int P = 64;
+        while (P > 1 && P * O > BT) P >>= 1;
+        const int p = tid % P;
+        for (int o0 = tid / P; o0 < ((O + BT / P - 1) / (BT / P)) * (BT / P); o0 += BT / P) {
+            T acc = T(0);
+            if (o0 < O)
+                for (int jj = p; jj < H; jj += P) acc += R.W2[(size_t)o0 * H + jj] * m.hid[jj];
+            for (int off = P >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+            if (p == 0 && o0 < O) {
+                T y, dy;
+                activate<T>(A.act, acc + R.b2[o0], y, dy);
+                m.outb[o0] = y;
+                m.dout[o0] = dy;
+                if (mode == MODE_REC) {
+                    R.sout[(size_t)slot * O + o0] = y;
+                    R.sdout[(size_t)slot * O + o0] = dy;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // spring impulses
+    for (int s = tid; s < S; s += BT) {
+        const int a = R.sa[s], b = R.sb[s];
+        const T d[3] = {m.xs[a] - m.xs[b], m.xs[N + a] - m.xs[N + b], m.xs[2 * N + a] - m.xs[2 * N + b]};
+        const T w[3] = {m.vs[a] - m.vs[b], m.vs[N + a] - m.vs[N + b], m.vs[2 * N + a] - m.vs[2 * N + b]};
+        T J[3];
+        spring_impulse<T>(d, w, R.L0[s], R.k[s], R.coef[s], m.outb[R.sact[s]], A.P, J);
+        m.J[s] = J[0]; m.J[S + s] = J[1]; m.J[2 * S + s] = J[2];
+    }
+    __syncthreads();
+    // gather + integrate (vertex owned by one thread, in place)
+    for (int i = tid; i < N; i += BT) {
+        T vi[3] = {0, 0, 0};
+        for (int e = R.vptr[i]; e < R.vptr[i + 1]; ++e) {
+            const int ent = R.vidx[e], s = ent >> 1;
+            const T sg = (ent & 1) ? T(1) : T(-1);
+            vi[0] += sg * m.J[s]; vi[1] += sg * m.J[S + s]; vi[2] += sg * m.J[2 * S + s];
+        }
+        const T x[3] = {m.xs[i], m.xs[N + i], m.xs[2 * N + i]};
+        const T v[3] = {m.vs[i], m.vs[N + i], m.vs[2 * N + i]};
+        T xn[3], vn[3];
+        integrate<T>(x, v, vi, A.P, xn, vn);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            m.xs[d * N + i] = xn[d];
+            m.vs[d * N + i] = vn[d];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_rollout_fwd(RArgs<T> A) {
+    extern __shared__ unsigned char smem_raw[];
+    const int r = blockIdx.x;
+    const Rob<T> R = make_rob(A, r);
+    Sm<T> m = carve(A, smem_raw, R.spill, false);
+    const int N = R.N, tid = threadIdx.x, BT = blockDim.x;
+    const int t0 = A.robots[r].t_fwd;
+    int t1 = R.Th;
+    if (A.max_steps > 0 && t0 + A.max_steps < t1) t1 = t0 + A.max_steps;
+    if (t0 >= t1) return;
+    for (int i = tid; i < 3 * N; i += BT) {
+        m.xs[i] = R.state[i];
+        m.vs[i] = R.state[3 * N + i];
+    }
+    __syncthreads();
+    T lsum = T(0);
+    for (int t = t0; t < t1; ++t) fwd_step(A, R, m, t, MODE_FWD, 0, lsum);
+    if (t1 == R.Th) {
+        // final state: loss term and trajectory row (the center of the final
+        // state is not part of the trajectory, matching the reference field)
+        com(R, m);
+        if (A.loss_kind == 1 && tid == 0) lsum += task_term(m.cv, tgt_at(R, R.Th));
+        if (A.store_traj && R.traj) {
+            T *row = R.traj + (size_t)R.Th * (3 * N + 3);
+            for (int i = tid; i < 3 * N; i += BT) row[i] = m.xs[i];
+        }
+    }
+    for (int i = tid; i < 3 * N; i += BT) {
+        R.state[i] = m.xs[i];
+        R.state[3 * N + i] = m.vs[i];
+    }
+    if (A.loss_kind == 1 && tid == 0) A.loss[r] += lsum / T(A.window);
+}
+
+// Adjoint of one step t.  On entry m.xb/m.vb hold dL/d(state t+1); on exit
+// dL/d(state t).  Intermediates come from scratch slot `slot`.
+template <typename T>
+__device__ void bwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int r, int t, int slot) {
+    const int N = R.N, S = R.S, I = R.I, O = R.O, H = A.H;
+    const int tid = threadIdx.x, BT = blockDim.x;
+    const T *sx = R.sx + (size_t)slot * 6 * N;
+    for (int i = tid; i < 3 * N; i += BT) {
+        m.xs[i] = sx[i];
+        m.vs[i] = sx[3 * N + i];
+    }
+    for (int q = tid; q < I; q += BT) m.inb[q] = R.sin[(size_t)slot * I + q];
+    for (int j = tid; j < H; j += BT) m.dhid[j] = R.sdhid[(size_t)slot * H + j];
+    for (int o = tid; o < O; o += BT) {
+        m.outb[o] = R.sout[(size_t)slot * O + o];
+        m.dout[o] = R.sdout[(size_t)slot * O + o];
+    }
+    if (tid < 6) m.cv[tid] = R.scv[slot * 6 + tid];
+    __syncthreads();
+    T *G = m.J + 3 * S;  // [7][S]
+    // recompute impulses
+    for (int s = tid; s < S; s += BT) {
+        const int a = R.sa[s], b = R.sb[s];
+        const T d[3] = {m.xs[a] - m.xs[b], m.xs[N + a] - m.xs[N + b], m.xs[2 * N + a] - m.xs[2 * N + b]};
+        const T w[3] = {m.vs[a] - m.vs[b], m.vs[N + a] - m.vs[N + b], m.vs[2 * N + a] - m.vs[2 * N + b]};
+        T J[3];
+        spring_impulse<T>(d, w, R.L0[s], R.k[s], R.coef[s], m.outb[R.sact[s]], A.P, J);
+        m.J[s] = J[0]; m.J[S + s] = J[1]; m.J[2 * S + s] = J[2];
+    }
+    __syncthreads();
+    // integrate adjoint
+    for (int i = tid; i < N; i += BT) {
+        T vi[3] = {0, 0, 0};
+        for (int e = R.vptr[i]; e < R.vptr[i + 1]; ++e) {
+            const int ent = R.vidx[e], s = ent >> 1;
+            const T sg = (ent & 1) ? T(1) : T(-1);
+            vi[0] += sg * m.J[s]; vi[1] += sg * m.J[S + s]; vi[2] += sg * m.J[2 * S + s];
+        }
+        const T x[3] = {m.xs[i], m.xs[N + i], m.xs[2 * N + i]};
+        const T v[3] = {m.vs[i], m.vs[N + i], m.vs[2 * N + i]};
+        const T xnb[3] = {m.xb[i], m.xb[N + i], m.xb[2 * N + i]};
+        const T vnb[3] = {m.vb[i], m.vb[N + i], m.vb[2 * N + i]};
+        T xb[3], vb[3], vib[3];
+        integrate_adj<T>(x, v, vi, A.P, xnb, vnb, xb, vb, vib);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            m.xb[d * N + i] = xb[d];
+            m.vb[d * N + i] = vb[d];
+            m.vib[d * N + i] = vib[d];
+        }
+    }
+    __syncthreads();
+    // spring adjoint
+    for (int s = tid; s < S; s += BT) {
+        const int a = R.sa[s], b = R.sb[s];
+        const T d[3] = {m.xs[a] - m.xs[b], m.xs[N + a] - m.xs[N + b], m.xs[2 * N + a] - m.xs[2 * N + b]};
+        const T w[3] = {m.vs[a] - m.vs[b], m.vs[N + a] - m.vs[N + b], m.vs[2 * N + a] - m.vs[2 * N + b]};
+        const T Jb[3] = {m.vib[b] - m.vib[a], m.vib[N + b] - m.vib[N + a], m.vib[2 * N + b] - m.vib[2 * N + a]};
+        T gd[3], gw[3], ga;
+        spring_impulse_adj<T>(d, w, R.L0[s], R.k[s], R.coef[s], m.outb[R.sact[s]], A.P, Jb, gd, gw, ga);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            G[c * S + s] = gd[c];
+            G[(3 + c) * S + s] = gw[c];
+        }
+        G[6 * S + s] = ga;
+    }
+    __syncthreads();
+    for (int i = tid; i < N; i += BT) {
+        T gx[3] = {0, 0, 0}, gv[3] = {0, 0, 0};
+        for (int e = R.vptr[i]; e < R.vptr[i + 1]; ++e) {
+            const int ent = R.vidx[e], s = ent >> 1;
+            const T sg = (ent & 1) ? T(-1) : T(1);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                gx[c] += sg * G[c * S + s];
+                gv[c] += sg * G[(3 + c) * S + s];
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            m.xb[d * N + i] += gx[d];
+            m.vb[d * N + i] += gv[d];
+        }
+    }
+    for (int o = tid; o < O; o += BT) {
+        T ab = T(0);
+        for (int e = R.aptr[o]; e < R.aptr[o + 1]; ++e) ab += G[6 * S + R.aidx[e]];
+        const T z = ab * m.dout[o];
+        m.z2b[o] = z;
+        R.sz2[(size_t)slot * O + o] = z;
+    }
+    __syncthreads();
+    // MLP adjoint: hidden
+    for (int j = tid; j < H; j += BT) {
+        T acc = T(0);
+        for (int o = 0; o < O; ++o) acc += R.W2[(size_t)o * H + j] * m.z2b[o];
+        const T z = acc * m.dhid[j];
+        m.z1b[j] = z;
+        R.sz1[(size_t)slot * H + j] = z;
+    }
+    __syncthreads();
+    // input adjoint: in_b = W1^T z1b (W1 [H][I] rows coalesced across i)
+    for (int i = tid; i < I; i += BT) {
+        T acc = T(0);
+        for (int j = 0; j < H; ++j) acc += R.W1[(size_t)j * I + i] * m.z1b[j];
+        m.inbar[i] = acc;
+    }
+    __syncthreads();
+    // sensor adjoint, loss, external trajectory adjoint
+    const Layout &L = A.L;
+    const int base = L.vert_base(), vs = L.vstride();
+    const T inv = T(1) / T(L.offset_scale);
+    T cb[3] = {0, 0, 0};
+    const T *eg = R.trajg ? R.trajg + (size_t)t * (3 * N + 3) : nullptr;
+    for (int i = tid; i < N; i += BT) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const T g = m.inbar[base + i * vs + d] * inv;
+            T add = g;
+            if (eg) add += eg[d * N + i];
+            m.xb[d * N + i] += add;
+            cb[d] -= g;
+        }
+    }
+    block_sum<T, 3>(cb, m.red);
+    T vcb[3] = {0, 0, 0};
+    if (L.vcom)
+        for (int d = 0; d < 3; ++d) vcb[d] = m.inbar[L.n_sin + d];
+    if (eg)
+        for (int d = 0; d < 3; ++d) cb[d] += eg[3 * N + d];
+    if (A.loss_kind == 1 && t > R.Th - A.window) {
+        const T lg = A.loss_grad[r] / T(A.window);
+        const T *tg = tgt_at(R, t);
+        vcb[0] += T(2) * lg * (m.cv[3] - tg[0]);
+        vcb[2] += T(2) * lg * (m.cv[5] - tg[2]);
+        cb[1] += T(2) * lg * (m.cv[1] - tg[3]);
+    }
+    const T invN = T(1) / T(N);
+    for (int i = tid; i < N; i += BT) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            m.xb[d * N + i] += cb[d] * invN;
+            m.vb[d * N + i] += vcb[d] * invN;
+        }
+    }
+    __syncthreads();
+}
+
+// Fold the segment's weight gradients into the robot's partial:
+// W1g += sum_t z1b_t in_t^T, b1g += sum_t z1b_t, W2g += sum_t z2b_t h_t^T, ...
+template <typename T>
+__device__ void seg_wgrad(const RArgs<T> &A, const Rob<T> &R, int n) {
+    const int I = R.I, O = R.O, H = A.H, tid = threadIdx.x, BT = blockDim.x;
+    for (int i0 = 0; i0 < I; i0 += BT) {
+        const int i = i0 + tid;
+        const bool ok = i < I;
+        T acc[HMAX];
+#pragma unroll
+        for (int j = 0; j < HMAX; ++j) acc[j] = T(0);
+        for (int s = 0; s < n; ++s) {
+            const T xin = ok ? R.sin[(size_t)s * I + i] : T(0);
+            const T *z = R.sz1 + (size_t)s * H;
+#pragma unroll
+            for (int j = 0; j < HMAX; ++j)
+                if (j < H) acc[j] += z[j] * xin;
+        }
+        if (ok) {
+#pragma unroll
+            for (int j = 0; j < HMAX; ++j)
+                if (j < H) R.Wg1[(size_t)j * I + i] += acc[j];
+        }
+    }
+    for (int j = tid; j < H; j += BT) {
+        T acc = T(0);
+        for (int s = 0; s < n; ++s) acc += R.sz1[(size_t)s * H + j];
+        R.bg1[j] += acc;
+    }
+    for (int q = tid; q < O * H; q += BT) {
+        const int o = q / H, j = q - o * H;
+        T acc = T(0);
+        for (int s = 0; s < n; ++s) acc += R.sz2[(size_t)s * O + o] * R.shid[(size_t)s * H + j];
+        R.Wg2[q] += acc;
+    }
+    for (int o = tid; o < O; o += BT) {
+        T acc = T(0);
+        for (int s = 0; s < n; ++s) acc += R.sz2[(size_t)s * O + o];
+        R.bg2[o] += acc;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_rollout_bwd(RArgs<T> A) {
+    extern __shared__ unsigned char smem_raw[];
+    const int r = blockIdx.x;
+    const Rob<T> R = make_rob(A, r);
+    Sm<T> m = carve(A, smem_raw, R.spill, true);
+    const int N = R.N, tid = threadIdx.x, BT = blockDim.x, K = A.K;
+    int tb = A.robots[r].t_bwd;
+    if (tb <= 0) return;
+    for (int i = tid; i < 3 * N; i += BT) {
+        m.xb[i] = R.adj[i];
+        m.vb[i] = R.adj[3 * N + i];
+    }
+    __syncthreads();
+    if (tb == R.Th) {
+        // seed from the final state's loss term and external trajectory row
+        for (int i = tid; i < 3 * N; i += BT) {
+            m.xs[i] = R.state[i];
+            m.vs[i] = R.state[3 * N + i];
+        }
+        __syncthreads();
+        com(R, m);
+        T cb[3] = {0, 0, 0}, vcb[3] = {0, 0, 0};
+        if (A.loss_kind == 1) {
+            const T lg = A.loss_grad[r] / T(A.window);
+            const T *tg = tgt_at(R, R.Th);
+            vcb[0] = T(2) * lg * (m.cv[3] - tg[0]);
+            vcb[2] = T(2) * lg * (m.cv[5] - tg[2]);
+            cb[1] = T(2) * lg * (m.cv[1] - tg[3]);
+        }
+        const T *eg = R.trajg ? R.trajg + (size_t)R.Th * (3 * N + 3) : nullptr;
+        const T invN = T(1) / T(N);
+        for (int i = tid; i < N; i += BT) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                m.xb[d * N + i] += cb[d] * invN + (eg ? eg[d * N + i] : T(0));
+                m.vb[d * N + i] += vcb[d] * invN;
+            }
+        }
+        __syncthreads();
+    }
+    int budget = A.max_steps > 0 ? A.max_steps : 0x7fffffff;
+    T dummy = T(0);
+    while (tb > 0 && budget > 0) {
+        const int seg = (tb - 1) / K, s0 = seg * K;
+        const int lo = max(s0, tb - budget);
+        const T *ck = R.ckpt + (size_t)seg * 6 * N;
+        for (int i = tid; i < 3 * N; i += BT) {
+            m.xs[i] = ck[i];
+            m.vs[i] = ck[3 * N + i];
+        }
+        __syncthreads();
+        for (int t = s0; t < tb; ++t) fwd_step(A, R, m, t, MODE_REC, t - s0, dummy);
+        for (int t = tb - 1; t >= lo; --t) bwd_step(A, R, m, r, t, t - s0);
+        // the weight-gradient fold reads slots lo - s0 .. tb - s0 - 1
+        {
+            Rob<T> Rs = R;
+            const int off = lo - s0;
+            Rs.sin = R.sin + (size_t)off * R.I;
+            Rs.sz1 = R.sz1 + (size_t)off * A.H;
+            Rs.sz2 = R.sz2 + (size_t)off * R.O;
+            Rs.shid = R.shid + (size_t)off * A.H;
+            seg_wgrad(A, Rs, tb - lo);
+        }
+        __syncthreads();
+        budget -= tb - lo;
+        tb = lo;
+    }
+    for (int i = tid; i < 3 * N; i += BT) {
+        R.adj[i] = m.xb[i];
+        R.adj[3 * N + i] = m.vb[i];
+    }
+}
+
+// Fixed-order sum of per-robot gradient partials into per-design gradients.
+template <typename T>
+__global__ void k_wgrad_reduce(const int32_t *lptr, const int32_t *list, const int64_t *roff,
+                               const int64_t *doff, const int64_t *dsize, const T *wg, T *out) {
+    const int d = blockIdx.y;
+    const int64_t n = dsize[d];
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        T s = T(0);
+        for (int q = lptr[d]; q < lptr[d + 1]; ++q) s += wg[roff[list[q]] + e];
+        out[doff[d] + e] = s;
+    }
+}
+
+}  // namespace dl
+
+using namespace dl;
+
+template <typename T>
+static RArgs<T> make_rargs(const dl_rollout *r) {
+    RArgs<T> A;
+    A.robots = r->robots;
+    A.sa = r->spring_a; A.sb = r->spring_b; A.sact = r->spring_act;
+    A.L0 = (const T *)r->spring_L0; A.k = (const T *)r->spring_k; A.coef = (const T *)r->spring_coeff;
+    A.vptr = r->vert_ptr; A.vidx = r->vert_idx; A.aptr = r->act_ptr; A.aidx = r->act_idx;
+    A.W = (const T *)r->weights; A.Wg = (T *)r->wgrad;
+    A.state = (T *)r->state; A.adj = (T *)r->adjoint; A.ckpt = (T *)r->ckpt;
+    A.scratch = (T *)r->scratch;
+    A.targets = (const T *)r->targets;
+    A.loss = (T *)r->loss; A.loss_grad = (const T *)r->loss_grad;
+    A.traj = (T *)r->traj; A.traj_grad = (const T *)r->traj_grad;
+    A.P = make_phys<T>(r->phys);
+    A.L = make_layout(r->lay);
+    A.H = r->n_hidden; A.K = r->ckpt_every; A.max_steps = r->max_steps; A.act = r->act;
+    A.loss_kind = r->loss_kind; A.window = r->loss_window; A.store_traj = r->store_traj;
+    A.max_n = r->max_n; A.max_s = r->max_s; A.max_i = r->max_i; A.max_o = r->max_o;
+    A.springs_in_lds = 1;
+    return A;
+}
+
+static int check_rollout(const dl_rollout *r, bool bwd) {
+    DL_CHECK_ARG(r != nullptr, "rollout descriptor is NULL");
+    DL_CHECK_ARG(r->dtype == DL_F32 || r->dtype == DL_F64, "rollout: bad dtype %d", r->dtype);
+    DL_CHECK_ARG(r->act == DL_ACT_SIN || r->act == DL_ACT_TANH, "rollout: bad activation");
+    DL_CHECK_ARG(r->n_robots > 0 && r->n_robots <= 65535 * 1024, "rollout: bad robot count");
+    DL_CHECK_ARG(r->n_hidden > 0 && r->n_hidden <= HMAX, "rollout: hidden width must be 1..%d", HMAX);
+    DL_CHECK_ARG(r->ckpt_every > 0, "rollout: checkpoint interval must be positive");
+    DL_CHECK_ARG(r->block >= 64 && r->block <= 256 && r->block % 64 == 0,
+                 "rollout: block must be a multiple of 64 in [64, 256]");
+    DL_CHECK_ARG(r->max_n > 0 && r->max_s >= 0 && r->max_i > 0 && r->max_o > 0,
+                 "rollout: bad size maxima");
+    DL_CHECK_ARG(r->lay.n_tv >= 0 && r->lay.n_tv <= 3 && r->lay.run_period > 0, "rollout: bad layout");
+    DL_CHECK_ARG(r->loss_kind == 0 || (r->loss_kind == 1 && r->loss_window > 0),
+                 "rollout: bad loss configuration");
+    DL_CHECK_ARG(r->robots && r->spring_a && r->spring_b && r->spring_act && r->spring_L0 &&
+                     r->spring_k && r->spring_coeff && r->vert_ptr && r->vert_idx && r->act_ptr &&
+                     r->act_idx && r->weights && r->state && r->ckpt && r->targets,
+                 "rollout: NULL buffer");
+    DL_CHECK_ARG(r->loss_kind == 0 || r->loss, "rollout: loss buffer required");
+    if (bwd)
+        DL_CHECK_ARG(r->adjoint && r->scratch && r->wgrad && (r->loss_kind == 0 || r->loss_grad),
+                     "rollout backward: adjoint, scratch, wgrad and loss_grad buffers required");
+    return 0;
+}
+
+static size_t g_lds_limit = 64 * 1024;
+
+template <typename T>
+static int launch_rollout(const dl_rollout *r, bool bwd, hipStream_t st) {
+    RArgs<T> A = make_rargs<T>(r);
+    size_t sh = smem_bytes<T>(r->max_n, r->max_s, r->max_i, r->max_o, bwd, true);
+    if (sh > g_lds_limit) {
+        A.springs_in_lds = 0;
+        sh = smem_bytes<T>(r->max_n, r->max_s, r->max_i, r->max_o, bwd, false);
+        DL_CHECK_ARG(r->scratch != nullptr, "rollout: robot spills springs to scratch; scratch required");
+    }
+    DL_CHECK_ARG(sh <= g_lds_limit, "rollout: robot too large for LDS (%zu bytes)", sh);
+    if (bwd) {
+        if (sh > 65536)
+            DL_CHECK_HIP(hipFuncSetAttribute((const void *)k_rollout_bwd<T>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+        hipLaunchKernelGGL(k_rollout_bwd<T>, dim3(r->n_robots), dim3(r->block), sh, st, A);
+    } else {
+        if (sh > 65536)
+            DL_CHECK_HIP(hipFuncSetAttribute((const void *)k_rollout_fwd<T>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+        hipLaunchKernelGGL(k_rollout_fwd<T>, dim3(r->n_robots), dim3(r->block), sh, st, A);
+    }
+    DL_CHECK_LAUNCH();
+    return 0;
+}
+
+extern "C" {
+
+int64_t dl_weights_size(int32_t I, int32_t H, int32_t O) {
+    return 2 * (int64_t)H * I + H + 2 * (int64_t)O * H + O;
+}
+
+int64_t dl_wgrad_size(int32_t I, int32_t H, int32_t O) {
+    return (int64_t)H * I + H + (int64_t)O * H + O;
+}
+
+int64_t dl_scratch_size(int32_t N, int32_t S, int32_t I, int32_t H, int32_t O, int32_t K) {
+    return (int64_t)K * (6 * (int64_t)N + I + 3 * (int64_t)H + 3 * (int64_t)O + 6) + 10 * (int64_t)S;
+}
+
+int32_t dl_ms_rollout_forward(const dl_rollout *r, void *stream) {
+    if (int e = check_rollout(r, false)) return e;
+    if (r->dtype == DL_F32) return launch_rollout<float>(r, false, (hipStream_t)stream);
+    return launch_rollout<double>(r, false, (hipStream_t)stream);
+}
+
+int32_t dl_ms_rollout_backward(const dl_rollout *r, void *stream) {
+    if (int e = check_rollout(r, true)) return e;
+    if (r->dtype == DL_F32) return launch_rollout<float>(r, true, (hipStream_t)stream);
+    return launch_rollout<double>(r, true, (hipStream_t)stream);
+}
+
+int32_t dl_wgrad_reduce(int32_t dtype, int32_t n_robots, const int32_t *lptr, const int32_t *list,
+                        int32_t n_designs, const int64_t *roff, const int64_t *doff,
+                        const int64_t *dsize, const void *wg, void *out, void *stream) {
+    DL_CHECK_ARG(n_robots > 0 && n_designs > 0 && n_designs <= 65535 && lptr && list && roff &&
+                     doff && dsize && wg && out,
+                 "dl_wgrad_reduce: bad arguments");
+    if (dtype == DL_F32)
+        hipLaunchKernelGGL(k_wgrad_reduce<float>, dim3(64, n_designs), dim3(256), 0,
+                           (hipStream_t)stream, lptr, list, roff, doff, dsize, (const float *)wg,
+                           (float *)out);
+    else if (dtype == DL_F64)
+        hipLaunchKernelGGL(k_wgrad_reduce<double>, dim3(64, n_designs), dim3(256), 0,
+                           (hipStream_t)stream, lptr, list, roff, doff, dsize, (const double *)wg,
+                           (double *)out);
+    else
+        DL_CHECK_ARG(false, "dl_wgrad_reduce: bad dtype");
+    DL_CHECK_LAUNCH();
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,262 @@
+/*
+ * diffloco.h -- C ABI of the MI355X (gfx950) differentiable locomotion rollout.
+ *
+ * Every entry point is extern "C", noexcept, takes plain pointers and sizes,
+ * and returns 0 on success or a negative DL_E* code; dl_last_error() returns
+ * the message of the calling thread's last failure.  Device buffers are
+ * caller-allocated (torch) device pointers; `stream` is a hipStream_t (NULL =
+ * default stream).  Nothing here allocates device memory.  `dtype` selects
+ * the arithmetic type of every real-valued buffer of the call (DL_F32 or
+ * DL_F64); integer buffers are int32.
+ *
+ * Field layout of the per-step API (one design, B robots): positions and
+ * velocities are SoA per robot and step, x[t][k][d][i] (d = 0..2, i = vertex);
+ * step slices are passed as pointers to x[t].
+ *
+ * Each per-step entry point replaces one reference kernel (file:line in the
+ * reference repository, erizmr/Complex-locomotion-skill-learning-via-
+ * differentiable-physics); the *_grad entry points replace the adjoint that
+ * Taichi generates for that kernel under ti.ad.Tape
+ * (multitask/diffphy_trainer.py:246).  Gradients ACCUMULATE (+=) into the
+ * *_grad outputs, like Taichi's .grad fields.
+ */
+#ifndef DIFFLOCO_H
+#define DIFFLOCO_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DL_ABI_VERSION 1
+
+enum { DL_F32 = 0, DL_F64 = 1 };
+enum { DL_ACT_SIN = 0, DL_ACT_TANH = 1 };
+enum { DL_OK = 0, DL_E_ARG = -1, DL_E_HIP = -2, DL_E_SIZE = -3, DL_E_NUMERIC = -4 };
+
+/* Physics constants of the mass-spring step
+ * (multitask/solver_mass_spring.py:8-23, multitask/config_sim.py:112-121). */
+typedef struct dl_ms_physics {
+    double dt;
+    double gravity;          /* along +y */
+    double ground;           /* ground height */
+    double drag;             /* velocity factor exp(-dt*drag) (:120) */
+    double dashpot;          /* dashpot impulse coefficient, no dt (:103) */
+    double friction;         /* per-axis Coulomb clamp (:136-144) */
+} dl_ms_physics;
+
+/* Sensor vector layout (multitask/taichi_env.py:149-183, generalised):
+ *   [n_sin sinusoids | vcom? 3 COM-velocity slots | per vertex (x-c)/offset_scale
+ *    followed by vert_pad zeros | dup_v copies of target_v[tv_comp[..]]/max_speed
+ *    | dup_h copies of target_h | dup_c copies of target_c] */
+typedef struct dl_sensor_layout {
+    int32_t n_sin;
+    int32_t run_period;      /* sinusoid period in steps */
+    int32_t vcom;            /* 0/1 */
+    int32_t vert_pad;
+    int32_t n_tv;            /* components per target_v copy (1..3) */
+    int32_t tv_comp[3];
+    int32_t dup_v, dup_h, dup_c;
+    int32_t pad_;
+    double offset_scale;
+    double max_speed;
+} dl_sensor_layout;
+
+/* Topology of one design (multitask/solver_mass_spring.py:47-67): springs
+ * (endpoints, rest length, stiffness ALREADY multiplied by the solver's
+ * stiffness scale, actuation coefficient, driving MLP output) plus two CSR
+ * maps the builder derives: vertex -> incident springs (entry = 2*s + side,
+ * side 0 when the vertex is endpoint a, 1 when b) and MLP output -> springs. */
+typedef struct dl_ms_topology {
+    int32_t n_vert, n_spring, n_act;
+    int32_t pad_;
+    const int32_t *spring_a, *spring_b, *spring_act;    /* [S] */
+    const void *spring_L0, *spring_k, *spring_coeff;    /* [S] real */
+    const int32_t *vert_ptr, *vert_idx;                 /* [N+1], [2S] */
+    const int32_t *act_ptr, *act_idx;                   /* [O+1], [<=S] */
+} dl_ms_topology;
+
+/* ---------------------------------------------------------------- generic */
+int32_t dl_abi_version(void);
+int32_t dl_last_error(char *buf, size_t len);
+int32_t dl_device_count(int32_t *count);
+
+/* ------------------------------------------------- per-step mass-spring API */
+
+/* compute_center (solver_mass_spring.py:150-156): center[k] = mean_i x[k][:,i].
+ * Also writes the COM velocity when vcom != NULL (quadruped sensor/loss). */
+int32_t dl_ms_center(int32_t dtype, int32_t B, int32_t N, const void *x_t, const void *v_t,
+                     void *center_t, void *vcom_t, void *stream);
+int32_t dl_ms_center_grad(int32_t dtype, int32_t B, int32_t N, const void *center_grad,
+                          const void *vcom_grad, void *x_grad, void *v_grad, void *stream);
+
+/* compute_height (:158-173): height[t] = windowed max of min_i y, upper = max_i y.
+ * height_prev may be NULL when t % jump_period == 0. */
+int32_t dl_ms_height(int32_t dtype, int32_t B, int32_t N, int32_t t, int32_t jump_period,
+                     const void *x_t, const void *height_prev, void *height_t, void *upper_t,
+                     void *stream);
+int32_t dl_ms_height_grad(int32_t dtype, int32_t B, int32_t N, int32_t t, int32_t jump_period,
+                          const void *x_t, const void *height_prev, const void *height_grad,
+                          const void *upper_grad, void *x_grad, void *height_prev_grad,
+                          void *stream);
+
+/* compute_rotation (:175-192).  robot_id 100 uses vertices 45,46,34,36;
+ * otherwise head/tail split by x_0 < center_0 along x. */
+int32_t dl_ms_rotation(int32_t dtype, int32_t B, int32_t N, int32_t robot_id, const void *x_t,
+                       const void *x_0, const void *center_0, void *rotation_t, void *stream);
+int32_t dl_ms_rotation_grad(int32_t dtype, int32_t B, int32_t N, int32_t robot_id,
+                            const void *x_t, const void *x_0, const void *center_0,
+                            const void *rotation_grad, void *x_grad, void *stream);
+
+/* nn_input (taichi_env.py:149-183).  tv_t [B][3], th_t/tc_t [B]; in_t [B][I]. */
+int32_t dl_nn_input(int32_t dtype, const dl_sensor_layout *lay, int32_t B, int32_t N,
+                    int32_t t, int32_t offset, const void *x_t, const void *v_t,
+                    const void *center_t, const void *tv_t, const void *th_t,
+                    const void *tc_t, void *in_t, void *stream);
+int32_t dl_nn_input_grad(int32_t dtype, const dl_sensor_layout *lay, int32_t B, int32_t N,
+                         const void *in_grad, void *x_grad, void *v_grad, void *center_grad,
+                         void *stream);
+
+/* Model.nn1 + nn2 (nn.py:130-150): hidden = act(W1 in + b1), out = act(W2 hidden + b2).
+ * W1 [H][I], W2 [O][H].  hidden_act [B][H], out_act [B][O]. */
+int32_t dl_mlp_forward(int32_t dtype, int32_t act, int32_t B, int32_t I, int32_t H, int32_t O,
+                       const void *in, const void *W1, const void *b1, const void *W2,
+                       const void *b2, void *hidden_act, void *out_act, void *stream);
+/* Adjoint of dl_mlp_forward given out_grad; accumulates into in_grad (may be
+ * NULL), W1_grad, b1_grad, W2_grad, b2_grad.  work: caller workspace of
+ * B*(2H+O) reals.  Weight gradients are summed over the batch in order. */
+int32_t dl_mlp_backward(int32_t dtype, int32_t act, int32_t B, int32_t I, int32_t H, int32_t O,
+                        const void *in, const void *W1, const void *b1, const void *W2,
+                        const void *b2, const void *out_grad, void *in_grad, void *W1_grad,
+                        void *b1_grad, void *W2_grad, void *b2_grad, void *work, void *stream);
+
+/* advance(t) = apply_spring_force(t) + advance_toi(t+1)
+ * (solver_mass_spring.py:86-148, :200-202), fused: reads x[t], v[t], act[t]
+ * ([B][O]), writes x[t+1], v[t+1].  Spring impulses are summed per vertex in
+ * CSR order (deterministic). */
+int32_t dl_ms_advance(int32_t dtype, const dl_ms_topology *topo, const dl_ms_physics *phys,
+                      int32_t B, const void *x_t, const void *v_t, const void *act_t,
+                      void *x_t1, void *v_t1, void *stream);
+int32_t dl_ms_advance_grad(int32_t dtype, const dl_ms_topology *topo,
+                           const dl_ms_physics *phys, int32_t B, const void *x_t,
+                           const void *v_t, const void *act_t, const void *x_t1_grad,
+                           const void *v_t1_grad, void *x_t_grad, void *v_t_grad,
+                           void *act_t_grad, void *stream);
+
+/* compute_loss_velocity / compute_loss_rotation (taichi_env.py:185-211, 235-252)
+ * over a stored trajectory.  center [T][B][3], x [T][B][3][N], tv [T][B][3],
+ * th [T][B].  The rollout wrote center rows t < T_written and x rows
+ * t <= T_written (x also holds the state after the last advance); unwritten
+ * rows read as zero.  loss_batch [B] receives loss*B per robot like
+ * loss_velocity_batch; loss [1] the sum. */
+int32_t dl_loss_velocity(int32_t dtype, int32_t B, int32_t T, int32_t T_written, int32_t steps,
+                         int32_t run_period, int32_t turn_period, const void *center,
+                         const void *tv, void *loss, void *loss_batch, void *stream);
+int32_t dl_loss_velocity_grad(int32_t dtype, int32_t B, int32_t T, int32_t T_written,
+                              int32_t steps, int32_t run_period, int32_t turn_period,
+                              const void *center, const void *tv, double loss_grad,
+                              void *center_grad, void *stream);
+/* work: caller workspace of B reals. */
+int32_t dl_loss_rotation(int32_t dtype, int32_t B, int32_t N, int32_t T, int32_t T_written,
+                         int32_t steps, int32_t run_period, int32_t turn_period, const void *x,
+                         const void *center, const void *th, void *loss, void *loss_batch,
+                         void *work, void *stream);
+int32_t dl_loss_rotation_grad(int32_t dtype, int32_t B, int32_t N, int32_t T, int32_t T_written,
+                              int32_t steps, int32_t run_period, int32_t turn_period,
+                              const void *x, const void *center, const void *th,
+                              double loss_grad, void *x_grad, void *center_grad, void *stream);
+
+/* Windowed COM loss of the quadruped task over stored center/vcom [T+1][B][3]
+ * with per-robot targets tv [B][3], th [B]:
+ *   loss = mean_k mean_{t in (T-W, T]} |vc_xz - tv_xz|^2 + (c_y - th)^2.
+ * No reference counterpart (the reference trains velocity/rotation losses,
+ * taichi_env.py:185-252); it is the objective of the quadruped workload. */
+int32_t dl_loss_task(int32_t dtype, int32_t B, int32_t T, int32_t W, const void *center,
+                     const void *vcom, const void *tv, const void *th, void *loss,
+                     void *loss_batch, void *stream);
+int32_t dl_loss_task_grad(int32_t dtype, int32_t B, int32_t T, int32_t W, const void *center,
+                          const void *vcom, const void *tv, const void *th, double loss_grad,
+                          void *center_grad, void *vcom_grad, void *stream);
+
+/* ------------------------------------------- persistent batched rollout API */
+
+/* One row of the robot offset table (device memory).  Robots of different
+ * size, design and horizon share one launch; one workgroup advances one
+ * robot through many steps with its state resident in LDS. */
+typedef struct dl_robot {
+    int32_t n_vert, n_spring, n_act, n_in;
+    int32_t horizon;         /* number of advances T */
+    int32_t t_fwd;           /* forward: next step to take (0 initially) */
+    int32_t t_bwd;           /* backward: adjoint has reached this step (T initially) */
+    int32_t spring0;         /* first spring of the design in the spring arrays */
+    int32_t vptr0;           /* first row of the design's vertex CSR (N+1 rows) */
+    int32_t aptr0;           /* first row of the design's actuator CSR (O+1 rows) */
+    int64_t weights;         /* element offset of the design's weight block */
+    int64_t wgrad;           /* element offset of this robot's gradient partial */
+    int64_t state;           /* element offset of the [6][N] state / adjoint slot */
+    int64_t ckpt;            /* element offset of [T/K+1][6][N] checkpoints */
+    int64_t scratch;         /* element offset of the per-segment recompute buffer */
+    int64_t targets;         /* element offset of targets: [1 or T][5] = tv xyz, th, tc */
+    int64_t traj;            /* element offset of [T][3N+3] x/center trajectory, -1 = none */
+    int32_t target_stride;   /* 0 = constant per robot, 5 = per step */
+    int32_t pad_;
+} dl_robot;
+
+/* Weight block of a design: W1 [H][I], W1T [I][H], b1 [H], W2 [O][H],
+ * W2T [H][O], b2 [O] -- dl_weights_size() elements.  Gradient partial
+ * block: W1 [H][I], b1 [H], W2 [O][H], b2 [O] -- dl_wgrad_size() elements. */
+int64_t dl_weights_size(int32_t I, int32_t H, int32_t O);
+int64_t dl_wgrad_size(int32_t I, int32_t H, int32_t O);
+/* Elements of per-robot scratch for checkpoint interval K. */
+int64_t dl_scratch_size(int32_t N, int32_t S, int32_t I, int32_t H, int32_t O, int32_t K);
+
+typedef struct dl_rollout {
+    int32_t dtype, act;
+    int32_t n_robots;
+    int32_t n_hidden;        /* H, shared by every design */
+    int32_t ckpt_every;      /* K */
+    int32_t max_steps;       /* steps this launch may take per robot (<= 0: to the end) */
+    int32_t block;           /* threads per workgroup (64..256, multiple of 64) */
+    int32_t loss_kind;       /* 0: none in-kernel, 1: windowed COM loss */
+    int32_t loss_window;     /* states t in (T - window, T] */
+    int32_t store_traj;      /* forward writes x/center trajectory rows */
+    int32_t max_n, max_s, max_i, max_o;   /* maxima over the table (LDS sizing) */
+    dl_ms_physics phys;
+    dl_sensor_layout lay;
+    const dl_robot *robots;  /* device [n_robots] */
+    const int32_t *spring_a, *spring_b, *spring_act;
+    const void *spring_L0, *spring_k, *spring_coeff;
+    const int32_t *vert_ptr, *vert_idx, *act_ptr, *act_idx;
+    const void *weights;
+    void *wgrad;             /* per-robot gradient partials (accumulated) */
+    void *state;             /* x,v slots (forward state) */
+    void *adjoint;           /* xbar,vbar slots (backward state) */
+    void *ckpt;
+    void *scratch;
+    const void *targets;
+    void *loss;              /* [n_robots], accumulated by the forward */
+    const void *loss_grad;   /* [n_robots] upstream dL/dloss_r for the backward */
+    void *traj;              /* forward trajectory rows */
+    const void *traj_grad;   /* backward: external dL/dx_t, dL/dc_t rows (NULL = none) */
+} dl_rollout;
+
+/* Advance every robot of the table from its t_fwd by up to max_steps steps
+ * (one persistent launch), writing checkpoints every K steps. */
+int32_t dl_ms_rollout_forward(const dl_rollout *r, void *stream);
+/* Adjoint of the rollout: from each robot's t_bwd down by up to max_steps
+ * steps, segment by segment (recompute from checkpoint, then reverse). */
+int32_t dl_ms_rollout_backward(const dl_rollout *r, void *stream);
+/* Sum per-robot gradient partials into per-design gradients in a fixed order:
+ * out[d] = sum over robots r with design_of[r] == d of wgrad[r]. */
+int32_t dl_wgrad_reduce(int32_t dtype, int32_t n_robots, const int32_t *robot_list_ptr,
+                        const int32_t *robot_list, int32_t n_designs,
+                        const int64_t *robot_wgrad_off, const int64_t *design_out_off,
+                        const int64_t *design_size, const void *wgrad, void *out,
+                        void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DIFFLOCO_H */

@@ -1,0 +1,315 @@
+"""HIP mass-spring path against the FP64 oracle (per-step kernels and the
+persistent batched rollout).  Tolerances: FP64 kernels agree with the oracle
+to rounding (1e-10 relative); FP32 positions/velocities within 1e-5 relative
+per step, loss within 1e-4, gradients within 1e-3 (relative, L2)."""
+
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import diffloco as dl
+from diffloco import _native as nat
+from diffloco.rollout import MassSpringBatch, rollout_loss
+from diffloco.solver_mass_spring import DeviceTopology, _physics
+from diffloco.taichi_env import sensor_layout
+from oracle import mass_spring as om
+
+from helpers import (from_soa, max_rel, oracle_quadruped, oracle_reference_robot, perturbed_state,
+                     rel_err, task_targets, to_soa)
+
+pytestmark = pytest.mark.gpu
+DT = {"f64": torch.float64, "f32": torch.float32}
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _phys_ref():
+    return om.Physics()
+
+
+def _phys_quad():
+    return om.TaskConfig().physics
+
+
+def _nat_phys(p):
+    return nat.MSPhysics(dt=p.dt, gravity=p.gravity, ground=p.ground, drag=p.drag,
+                         dashpot=p.dashpot, friction=p.friction)
+
+
+CASES = [("ref", 100), ("quad", None)]
+
+
+def _design(kind):
+    if kind == "ref":
+        return dl.build_reference(100), oracle_reference_robot(100), _phys_ref()
+    return dl.build_quadruped(), oracle_quadruped(), _phys_quad()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("kind", ["ref", "quad"])
+def test_advance_matches_oracle(kind, prec):
+    _say(f"advance {kind} {prec}")
+    design, rob, phys = _design(kind)
+    B, dt = 6, DT[prec]
+    x, v = perturbed_state(rob, B, seed=3, ground=phys.ground)
+    act = torch.rand(B, design.n_act, dtype=torch.float64, generator=torch.Generator().manual_seed(4)) * 2 - 1
+    topo = DeviceTopology(design, dt, "cuda", 0.1)
+    xd, vd, ad = to_soa(x, dt), to_soa(v, dt), act.to("cuda", dt)
+    x1, v1 = torch.empty_like(xd), torch.empty_like(vd)
+    nat.call("dl_ms_advance", nat.dtype_code(dt), ctypes.byref(topo.c), ctypes.byref(_nat_phys(phys)),
+             B, nat.ptr(xd), nat.ptr(vd), nat.ptr(ad), nat.ptr(x1), nat.ptr(v1), nat.stream())
+    torch.cuda.synchronize()
+    # oracle fed with the same (rounded) inputs
+    xo, vo = from_soa(xd), from_soa(vd)
+    ex, ev = om.step(xo, vo, ad.double().cpu(), rob, phys)
+    tol = 1e-11 if prec == "f64" else 1e-5
+    assert max_rel(from_soa(x1), ex, 1.0) < tol
+    assert max_rel(from_soa(v1), ev, 1.0) < tol * 10
+
+
+@pytest.mark.parametrize("kind", ["ref", "quad"])
+def test_advance_grad_matches_autograd(kind):
+    _say(f"advance_grad {kind}")
+    design, rob, phys = _design(kind)
+    B, dt = 4, torch.float64
+    x, v = perturbed_state(rob, B, seed=5, ground=phys.ground)
+    g = torch.Generator().manual_seed(6)
+    act = torch.rand(B, design.n_act, dtype=dt, generator=g) * 2 - 1
+    xb1 = torch.randn(x.shape, dtype=dt, generator=g)
+    vb1 = torch.randn(x.shape, dtype=dt, generator=g)
+    topo = DeviceTopology(design, dt, "cuda", 0.1)
+    xg, vg = torch.zeros(B, 3, design.n_vert, dtype=dt, device="cuda"), torch.zeros(B, 3, design.n_vert, dtype=dt, device="cuda")
+    ag = torch.zeros(B, design.n_act, dtype=dt, device="cuda")
+    keep = [to_soa(x), to_soa(v), act.cuda(), to_soa(xb1), to_soa(vb1)]   # alive across the launch
+    nat.call("dl_ms_advance_grad", nat.F64, ctypes.byref(topo.c), ctypes.byref(_nat_phys(phys)), B,
+             *[nat.ptr(t) for t in keep], nat.ptr(xg), nat.ptr(vg), nat.ptr(ag), nat.stream())
+    torch.cuda.synchronize()
+    xr, vr, ar = x.clone().requires_grad_(), v.clone().requires_grad_(), act.clone().requires_grad_()
+    ex, ev = om.step(xr, vr, ar, rob, phys)
+    (ex * xb1).sum().add((ev * vb1).sum()).backward()
+    assert rel_err(from_soa(xg), xr.grad) < 1e-10
+    assert rel_err(from_soa(vg), vr.grad) < 1e-10
+    assert rel_err(ag.cpu(), ar.grad) < 1e-10
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("act", ["sin", "tanh"])
+def test_mlp_forward_backward(prec, act):
+    _say(f"mlp {prec} {act}")
+    dt = DT[prec]
+    B, I, H, O = 5, 179, 64, 4
+    g = torch.Generator().manual_seed(7)
+    inp = torch.randn(B, I, dtype=torch.float64, generator=g)
+    W = om.init_weights(I, H, O, seed=8)
+    W[1] = 0.1 * torch.randn(H, dtype=torch.float64, generator=g)
+    W[3] = 0.1 * torch.randn(O, dtype=torch.float64, generator=g)
+    og = torch.randn(B, O, dtype=torch.float64, generator=g)
+    Wd = [w.to("cuda", dt).contiguous() for w in W]
+    hid = torch.zeros(B, H, dtype=dt, device="cuda")
+    out = torch.zeros(B, O, dtype=dt, device="cuda")
+    code = nat.ACT_SIN if act == "sin" else nat.ACT_TANH
+    ind = inp.to("cuda", dt)
+    nat.call("dl_mlp_forward", nat.dtype_code(dt), code, B, I, H, O, nat.ptr(ind),
+             *[nat.ptr(w) for w in Wd], nat.ptr(hid), nat.ptr(out), nat.stream())
+    Gs = [torch.zeros_like(w) for w in Wd]
+    ing = torch.zeros_like(ind)
+    work = torch.zeros(B * (2 * H + O), dtype=dt, device="cuda")
+    ogd = og.to("cuda", dt)
+    nat.call("dl_mlp_backward", nat.dtype_code(dt), code, B, I, H, O, nat.ptr(ind),
+             *[nat.ptr(w) for w in Wd], nat.ptr(ogd), nat.ptr(ing),
+             *[nat.ptr(x) for x in Gs], nat.ptr(work), nat.stream())
+    torch.cuda.synchronize()
+    Wr = [w.clone().requires_grad_() for w in W]
+    ir = inp.clone().requires_grad_()
+    eh, eo = om.mlp_forward(ir, *Wr, act=act)
+    (eo * og).sum().backward()
+    tol = 1e-12 if prec == "f64" else 2e-5
+    assert rel_err(out.cpu(), eo.detach()) < tol
+    assert rel_err(hid.cpu(), eh.detach()) < tol
+    gtol = 1e-11 if prec == "f64" else 1e-4
+    assert rel_err(ing.cpu(), ir.grad) < gtol
+    for a, b in zip(Gs, Wr):
+        assert rel_err(a.cpu(), b.grad) < gtol
+
+
+@pytest.mark.parametrize("layout", ["ref", "quad"])
+def test_nn_input_and_grad(layout):
+    _say(f"nn_input {layout}")
+    if layout == "ref":
+        cfg = dl.ConfigSim.from_file("tests/golden/sim_quad.json").get_config()
+        lay_o = om.SensorLayout()
+        rob = oracle_reference_robot()
+    else:
+        cfg = dl.quadruped_config(batch_size=4).get_config()
+        lay_o = om.TaskConfig().layout
+        rob = oracle_quadruped()
+    lay = sensor_layout(cfg)
+    B, N, t = 4, rob.n_vert, 37
+    dt = torch.float64
+    x, v = perturbed_state(rob, B, seed=9)
+    c = (x / N).sum(1)
+    g = torch.Generator().manual_seed(10)
+    tv = torch.randn(B, 3, dtype=dt, generator=g) * 0.05
+    th = torch.rand(B, dtype=dt, generator=g)
+    tc = torch.zeros(B, dtype=dt)
+    I = lay_o.n_input(N)
+    inp = torch.zeros(B, I, dtype=dt, device="cuda")
+    keep = [to_soa(x), to_soa(v), c.cuda(), tv.cuda(), th.cuda(), tc.cuda()]
+    nat.call("dl_nn_input", nat.F64, ctypes.byref(lay), B, N, t, 0, *[nat.ptr(u) for u in keep],
+             nat.ptr(inp), nat.stream())
+    torch.cuda.synchronize()
+    ref = om.nn_input(t, x, v, c, tv, th, tc, lay_o)
+    assert max_rel(inp.cpu(), ref, 1.0) < 1e-12
+    # adjoint
+    ig = torch.randn(B, I, dtype=dt, generator=g)
+    xg = torch.zeros(B, 3, N, dtype=dt, device="cuda")
+    vg = torch.zeros_like(xg)
+    cg = torch.zeros(B, 3, dtype=dt, device="cuda")
+    igd = ig.cuda()
+    nat.call("dl_nn_input_grad", nat.F64, ctypes.byref(lay), B, N, nat.ptr(igd), nat.ptr(xg),
+             nat.ptr(vg), nat.ptr(cg), nat.stream())
+    torch.cuda.synchronize()
+    xr, vr, cr = x.clone().requires_grad_(), v.clone().requires_grad_(), c.clone().requires_grad_()
+    (om.nn_input(t, xr, vr, cr, tv, th, tc, lay_o) * ig).sum().backward()
+    assert rel_err(from_soa(xg), xr.grad) < 1e-12
+    assert rel_err(cg.cpu(), cr.grad) < 1e-12
+    if layout == "quad":
+        assert rel_err(from_soa(vg), vr.grad) < 1e-12
+
+
+def _ref_small_config(steps, batch, prec):
+    import json
+    with open("tests/golden/sim_quad.json") as fh:
+        raw = json.load(fh)
+    raw["process"].update(max_steps=steps, run_period=20, turn_period=60, jump_period=60)
+    raw["nn"]["batch_size"] = batch
+    raw["precision"] = prec
+    return dl.ConfigSim(raw)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_per_step_trainer_matches_oracle(prec):
+    """The reference-shaped per-step loop with Tape replay (ref preset)."""
+    _say(f"trainer ref {prec}")
+    T, B = 130, 3
+    config = _ref_small_config(T, B, prec)
+    trainer = dl.DiffPhyTrainer(config)
+    env = trainer.taichi_env
+    env.setup_robot()
+    trainer.nn.weights_init(seed=11)
+    W = [w.data.double().cpu().clone() for w in (trainer.nn.weights1, trainer.nn.bias1,
+                                                  trainer.nn.weights2, trainer.nn.bias2)]
+    pool = torch.rand(B * (T // 60 + 1) * 3, dtype=torch.float64,
+                      generator=torch.Generator().manual_seed(12))
+    trainer.simulate(T, iter=0, max_speed=0.1, max_height=0.1, pool=pool)
+    torch.cuda.synchronize()
+    loss = env.loss[None]
+    # oracle
+    rob = oracle_reference_robot()
+    cfg = om.RefConfig(steps=T, run_period=20, turn_period=60, jump_period=60,
+                       layout=om.SensorLayout(run_period=20))
+    tv, th, tc = om.train_targets(pool.numpy(), T, B, 60, 0.1)
+    Wr = [w.clone().requires_grad_() for w in W]
+    x0, v0 = om.initial_state(rob, B)
+    lo, _, _ = om.rollout_ref(rob, cfg, Wr, tv, th, tc, x0, v0)
+    lo.backward()
+    ltol, gtol = (1e-10, 1e-8) if prec == "f64" else (1e-4, 1e-3)
+    assert abs(loss - lo.item()) / abs(lo.item()) < ltol
+    G = [trainer.nn.weights1.grad, trainer.nn.bias1.grad, trainer.nn.weights2.grad,
+         trainer.nn.bias2.grad]
+    for a, b in zip(G, Wr):
+        assert rel_err(a.data.cpu(), b.grad) < gtol
+
+
+def _quad_batch(B, T, W, K, prec, tv, th, store_traj=False, loss_window=None):
+    design = dl.build_quadruped()
+    qc = dl.quadruped_config(batch_size=B, steps=T, window=W).get_config()
+    x0 = design.objects + np.array([0.4, 0.0, 0.0])
+    tg = np.concatenate([tv, th[:, None], np.zeros((B, 1))], 1)
+    return MassSpringBatch([design], [0] * B, T, _physics(qc), sensor_layout(qc), tg, [x0] * B,
+                           act="tanh", ckpt_every=K, dtype=DT[prec],
+                           loss_window=W if loss_window is None else loss_window,
+                           store_traj=store_traj)
+
+
+def _oracle_task(B, T, W, tv, th, w64):
+    rob = oracle_quadruped()
+    cfg = om.TaskConfig(steps=T, window=W)
+    Wr = [w.clone().requires_grad_() for w in w64]
+    x0, v0 = om.initial_state(rob, B)
+    lo, lb, _ = om.rollout_task(rob, cfg, Wr, torch.tensor(tv), torch.tensor(th), x0, v0)
+    lo.backward()
+    return lo.item(), lb.detach(), [w.grad for w in Wr]
+
+
+@pytest.mark.parametrize("prec,K", [("f64", 16), ("f64", 7), ("f32", 16)])
+def test_batched_rollout_matches_oracle(prec, K):
+    _say(f"batched quad {prec} K={K}")
+    B, T, W = 5, 90, 40
+    tv, th = task_targets(B, 13)
+    design = dl.build_quadruped()
+    I = om.TaskConfig().layout.n_input(design.n_vert)
+    w64 = om.init_weights(I, 64, 4, seed=14)
+    batch = _quad_batch(B, T, W, K, prec, tv, th)
+    wd = [w.to("cuda", DT[prec]).requires_grad_() for w in w64]
+    lossv = rollout_loss(batch, [wd])
+    lossv.mean().backward()
+    torch.cuda.synchronize()
+    lo, lb, go = _oracle_task(B, T, W, tv, th, w64)
+    ltol, gtol = (1e-10, 1e-8) if prec == "f64" else (1e-4, 1e-3)
+    assert abs(lossv.mean().item() - lo) / abs(lo) < ltol
+    assert rel_err(lossv.detach().cpu(), lb) < ltol
+    for a, b in zip(wd, go):
+        assert rel_err(a.grad.cpu(), b) < gtol
+
+
+def test_batched_chunked_equals_single_launch():
+    """Forward in 3 chunks and backward in chunks give the same numbers as one launch."""
+    _say("batched chunked")
+    B, T, W, K = 3, 70, 30, 10
+    tv, th = task_targets(B, 15)
+    design = dl.build_quadruped()
+    I = om.TaskConfig().layout.n_input(design.n_vert)
+    w = [x.to("cuda") for x in om.init_weights(I, 64, 4, seed=16)]
+    a = _quad_batch(B, T, W, K, "f64", tv, th)
+    b = _quad_batch(B, T, W, K, "f64", tv, th)
+    for bt in (a, b):
+        bt.set_weights(0, *w)
+        bt.reset()
+    a.forward()
+    a.backward(loss_grad=np.full(B, 1.0 / B))
+    for _ in range(3):
+        b.forward(max_steps=25)
+    while np.any(b.table["t_bwd"] > 0):
+        b.backward(loss_grad=np.full(B, 1.0 / B), max_steps=13, reduce=False)
+    b.reduce_grads()
+    torch.cuda.synchronize()
+    # chunking only changes floating-point summation order
+    assert rel_err(b.loss.cpu(), a.loss.cpu()) < 1e-14
+    assert rel_err(b.grad_out.cpu(), a.grad_out.cpu()) < 1e-12
+
+
+def test_batched_deterministic():
+    _say("batched determinism")
+    B, T, W, K = 4, 60, 20, 16
+    tv, th = task_targets(B, 17)
+    design = dl.build_quadruped()
+    I = om.TaskConfig().layout.n_input(design.n_vert)
+    w = [x.to("cuda", torch.float32) for x in om.init_weights(I, 64, 4, seed=18)]
+    outs = []
+    for _ in range(2):
+        bt = _quad_batch(B, T, W, K, "f32", tv, th)
+        bt.set_weights(0, *w)
+        bt.reset()
+        bt.forward()
+        bt.backward(loss_grad=np.full(B, 1.0 / B))
+        torch.cuda.synchronize()
+        outs.append((bt.loss.clone(), bt.grad_out.clone()))
+    assert torch.equal(outs[0][0], outs[1][0])
+    assert torch.equal(outs[0][1], outs[1][1])
