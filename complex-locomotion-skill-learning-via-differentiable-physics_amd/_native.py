@@ -70,7 +70,8 @@ assert ctypes.sizeof(RobotC) == ROBOT_DTYPE.itemsize, "dl_robot layout mismatch"
 class Rollout(Structure):
     _fields_ = [(n, c_int32) for n in ("dtype", "act", "n_robots", "n_hidden", "ckpt_every",
                                        "max_steps", "block", "loss_kind", "loss_window",
-                                       "store_traj", "max_n", "max_s", "max_i", "max_o")] + \
+                                       "store_traj", "max_n", "max_s", "max_i", "max_o",
+                                       "lds_policy", "pad_")] + \
                [("phys", MSPhysics), ("lay", SensorLayoutC)] + \
                [(n, c_void_p) for n in ("robots", "spring_a", "spring_b", "spring_act",
                                         "spring_L0", "spring_k", "spring_coeff", "vert_ptr",

@@ -34,7 +34,9 @@ struct RArgs {
     Layout L;
     int H, K, max_steps, act, loss_kind, window, store_traj;
     int max_n, max_s, max_i, max_o;
-    int springs_in_lds;
+    int springs_in_lds;   // per-step spring buffers in LDS (else scratch)
+    int topo_in_lds;      // design topology staged in LDS at launch start
+    int w_in_lds;         // W1 staged in LDS as [I][H+1] (serves W1 x and W1^T x)
 };
 
 // Per-robot view (all pointers resolved from the offset table).
@@ -44,6 +46,7 @@ struct Rob {
     const int32_t *sa, *sb, *sact, *vptr, *vidx, *aptr, *aidx;
     const T *L0, *k, *coef;
     const T *W1, *W1T, *b1, *W2, *W2T, *b2;
+    const T *WL;       // LDS copy of W1^T with row pitch H+1 (nullptr when not staged)
     T *Wg1, *bg1, *Wg2, *bg2;
     const T *tgt;
     int tgt_stride;
@@ -67,6 +70,7 @@ __device__ __forceinline__ Rob<T> make_rob(const RArgs<T> &A, int r) {
     const T *w = A.W + R.weights;
     q.W1 = w; q.W1T = w + (size_t)H * I; q.b1 = q.W1T + (size_t)I * H;
     q.W2 = q.b1 + H; q.W2T = q.W2 + (size_t)O * H; q.b2 = q.W2T + (size_t)H * O;
+    q.WL = nullptr;
     T *g = A.Wg + R.wgrad;
     q.Wg1 = g; q.bg1 = g + (size_t)H * I; q.Wg2 = q.bg1 + H; q.bg2 = q.Wg2 + (size_t)O * H;
     q.tgt = A.targets + R.targets;
@@ -100,12 +104,15 @@ template <typename T>
 struct Sm {
     T *xs, *vs;        // [3][N] each, stride N
     T *xb, *vb, *vib;  // adjoint state / v_inc adjoint
-    T *J;              // [3][S] impulses, then [7][S] adjoint terms (after J)
+    T *J;              // [3][S] impulses; in the adjoint reused for [7][S] terms
     T *inb, *inbar;    // [I]
     T *hid, *dhid, *z1b;   // [HMAX]
     T *outb, *dout, *z2b;  // [O]
-    T *red;            // reductions: 16*HMAX + 96
+    T *red;            // reductions: 4*HMAX + 96
     T *cv;             // center[3], vcom[3], cbar[3], vcbar[3]
+    T *topo_r;         // staged topology reals [3][S]
+    T *wl;             // staged W1^T [I][H+1]
+    int *topo_i;       // staged topology ints
 };
 
 template <typename T>
@@ -124,7 +131,7 @@ __device__ __forceinline__ Sm<T> carve(const RArgs<T> &A, unsigned char *raw, T 
     }
     if (A.springs_in_lds) {
         m.J = p;
-        p += (bwd ? 10 : 3) * A.max_s;
+        p += (bwd ? 7 : 3) * A.max_s;
     } else {
         m.J = spill;
     }
@@ -137,16 +144,59 @@ __device__ __forceinline__ Sm<T> carve(const RArgs<T> &A, unsigned char *raw, T 
     m.dout = p; p += A.max_o;
     m.z2b = p; p += A.max_o;
     m.cv = p; p += 16;
-    m.red = p;
+    m.red = p; p += 4 * HMAX + 96;   // MLP partials (block <= 256) and block sums
+    m.topo_r = p; p += A.topo_in_lds ? 3 * A.max_s : 0;
+    m.wl = p; p += A.w_in_lds ? (size_t)A.max_i * (HMAX + 1) : 0;
+    m.topo_i = reinterpret_cast<int *>(p);
     return m;
 }
 
 template <typename T>
-__host__ size_t smem_bytes(int max_n, int max_s, int max_i, int max_o, bool bwd, bool springs) {
+__host__ size_t smem_bytes(int max_n, int max_s, int max_i, int max_o, bool bwd, bool springs,
+                           bool topo, bool wl) {
     size_t n = 6 * (size_t)max_n + (bwd ? 9 * (size_t)max_n : 0);
-    if (springs) n += (bwd ? 10 : 3) * (size_t)max_s;
-    n += (bwd ? 2 : 1) * (size_t)max_i + 3 * HMAX + 3 * (size_t)max_o + 16 + 16 * HMAX + 96;
-    return n * sizeof(T);
+    if (springs) n += (bwd ? 7 : 3) * (size_t)max_s;
+    n += (bwd ? 2 : 1) * (size_t)max_i + 3 * HMAX + 3 * (size_t)max_o + 16 + 4 * HMAX + 96;
+    if (topo) n += 3 * (size_t)max_s;
+    if (wl) n += (size_t)max_i * (HMAX + 1);
+    size_t b = n * sizeof(T);
+    if (topo) b += sizeof(int) * (6 * (size_t)max_s + max_n + max_o + 2);
+    return b;
+}
+
+// Copy the robot's design topology (and W1^T when requested) into LDS and
+// repoint the robot view at the copies.  CSR rows are rebased to the copies.
+template <typename T>
+__device__ void stage(const RArgs<T> &A, Rob<T> &R, Sm<T> &m) {
+    const int tid = threadIdx.x, BT = blockDim.x, S = R.S, N = R.N, O = R.O;
+    if (A.topo_in_lds) {
+        T *tr = m.topo_r;
+        int *ti = m.topo_i;
+        for (int q = tid; q < S; q += BT) {
+            tr[q] = R.L0[q]; tr[S + q] = R.k[q]; tr[2 * S + q] = R.coef[q];
+            ti[q] = R.sa[q]; ti[S + q] = R.sb[q]; ti[2 * S + q] = R.sact[q];
+        }
+        const int v0 = R.vptr[0], nv = R.vptr[N] - v0;
+        int *vp = ti + 3 * S, *vx = vp + N + 1;
+        for (int q = tid; q <= N; q += BT) vp[q] = R.vptr[q] - v0;
+        for (int q = tid; q < nv; q += BT) vx[q] = R.vidx[v0 + q];
+        const int a0 = R.aptr[0], na = R.aptr[O] - a0;
+        int *ap = vx + nv, *ax = ap + O + 1;
+        for (int q = tid; q <= O; q += BT) ap[q] = R.aptr[q] - a0;
+        for (int q = tid; q < na; q += BT) ax[q] = R.aidx[a0 + q];
+        R.L0 = tr; R.k = tr + S; R.coef = tr + 2 * S;
+        R.sa = ti; R.sb = ti + S; R.sact = ti + 2 * S;
+        R.vptr = vp; R.vidx = vx; R.aptr = ap; R.aidx = ax;
+    }
+    if (A.w_in_lds) {
+        const int H = A.H, I = R.I;
+        for (int q = tid; q < I * H; q += BT) {
+            const int i = q / H, j = q - i * H;
+            m.wl[i * (HMAX + 1) + j] = R.W1T[q];
+        }
+        R.WL = m.wl;
+    }
+    __syncthreads();
 }
 
 // COM position and velocity of the resident state -> m.cv[0..5]
@@ -226,8 +276,15 @@ __device__ void fwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int t, in
         const int Ic = (I + C - 1) / C;
         const int i0 = c * Ic, i1 = min(I, i0 + Ic);
         T acc = T(0);
-        if (j < H)
-            for (int i = i0; i < i1; ++i) acc += R.W1T[(size_t)i * H + j] * m.inb[i];
+        if (j < H) {
+            if (R.WL) {
+#pragma unroll 8
+                for (int i = i0; i < i1; ++i) acc += R.WL[i * (HMAX + 1) + j] * m.inb[i];
+            } else {
+#pragma unroll 8
+                for (int i = i0; i < i1; ++i) acc += R.W1T[(size_t)i * H + j] * m.inb[i];
+            }
+        }
         m.red[c * HMAX + j] = acc;
         __syncthreads();
         if (tid < H) {
@@ -302,8 +359,9 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_rollout_fwd(RArgs<T> A) {
     extern __shared__ unsigned char smem_raw[];
     const int r = blockIdx.x;
-    const Rob<T> R = make_rob(A, r);
+    Rob<T> R = make_rob(A, r);
     Sm<T> m = carve(A, smem_raw, R.spill, false);
+    stage(A, R, m);
     const int N = R.N, tid = threadIdx.x, BT = blockDim.x;
     const int t0 = A.robots[r].t_fwd;
     int t1 = R.Th;
@@ -352,7 +410,7 @@ __device__ void bwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int r, in
     }
     if (tid < 6) m.cv[tid] = R.scv[slot * 6 + tid];
     __syncthreads();
-    T *G = m.J + 3 * S;  // [7][S]
+    T *G = m.J;  // [7][S], overwrites the impulses once the integrator adjoint consumed them
     // recompute impulses
     for (int s = tid; s < S; s += BT) {
         const int a = R.sa[s], b = R.sb[s];
@@ -438,7 +496,14 @@ __device__ void bwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int r, in
     // input adjoint: in_b = W1^T z1b (W1 [H][I] rows coalesced across i)
     for (int i = tid; i < I; i += BT) {
         T acc = T(0);
-        for (int j = 0; j < H; ++j) acc += R.W1[(size_t)j * I + i] * m.z1b[j];
+        if (R.WL) {
+            const T *row = R.WL + i * (HMAX + 1);
+#pragma unroll 16
+            for (int j = 0; j < H; ++j) acc += row[j] * m.z1b[j];
+        } else {
+#pragma unroll 16
+            for (int j = 0; j < H; ++j) acc += R.W1[(size_t)j * I + i] * m.z1b[j];
+        }
         m.inbar[i] = acc;
     }
     __syncthreads();
@@ -528,8 +593,9 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_rollout_bwd(RArgs<T> A) {
     extern __shared__ unsigned char smem_raw[];
     const int r = blockIdx.x;
-    const Rob<T> R = make_rob(A, r);
+    Rob<T> R = make_rob(A, r);
     Sm<T> m = carve(A, smem_raw, R.spill, true);
+    stage(A, R, m);
     const int N = R.N, tid = threadIdx.x, BT = blockDim.x, K = A.K;
     int tb = A.robots[r].t_bwd;
     if (tb <= 0) return;
@@ -635,6 +701,8 @@ static RArgs<T> make_rargs(const dl_rollout *r) {
     A.loss_kind = r->loss_kind; A.window = r->loss_window; A.store_traj = r->store_traj;
     A.max_n = r->max_n; A.max_s = r->max_s; A.max_i = r->max_i; A.max_o = r->max_o;
     A.springs_in_lds = 1;
+    A.topo_in_lds = 0;
+    A.w_in_lds = 0;
     return A;
 }
 
@@ -649,6 +717,7 @@ static int check_rollout(const dl_rollout *r, bool bwd) {
                  "rollout: block must be a multiple of 64 in [64, 256]");
     DL_CHECK_ARG(r->max_n > 0 && r->max_s >= 0 && r->max_i > 0 && r->max_o > 0,
                  "rollout: bad size maxima");
+    DL_CHECK_ARG(r->lds_policy >= 0 && r->lds_policy <= 3, "rollout: bad LDS policy");
     DL_CHECK_ARG(r->lay.n_tv >= 0 && r->lay.n_tv <= 3 && r->lay.run_period > 0, "rollout: bad layout");
     DL_CHECK_ARG(r->loss_kind == 0 || (r->loss_kind == 1 && r->loss_window > 0),
                  "rollout: bad loss configuration");
@@ -663,18 +732,31 @@ static int check_rollout(const dl_rollout *r, bool bwd) {
     return 0;
 }
 
-static size_t g_lds_limit = 64 * 1024;
+static const size_t kLdsLimit = 160 * 1024;
 
 template <typename T>
 static int launch_rollout(const dl_rollout *r, bool bwd, hipStream_t st) {
     RArgs<T> A = make_rargs<T>(r);
-    size_t sh = smem_bytes<T>(r->max_n, r->max_s, r->max_i, r->max_o, bwd, true);
-    if (sh > g_lds_limit) {
-        A.springs_in_lds = 0;
-        sh = smem_bytes<T>(r->max_n, r->max_s, r->max_i, r->max_o, bwd, false);
-        DL_CHECK_ARG(r->scratch != nullptr, "rollout: robot spills springs to scratch; scratch required");
+    // LDS placement: policy 0 = auto (topology when it fits in 64 KiB, W1 too
+    // when the total stays within 80 KiB), 1 = topology only, 2 = topology +
+    // W1, 3 = nothing staged.
+    const int pol = r->lds_policy;
+    auto bytes = [&](bool springs, bool topo, bool wl) {
+        return smem_bytes<T>(r->max_n, r->max_s, r->max_i, r->max_o, bwd, springs, topo, wl);
+    };
+    bool springs = bytes(true, false, false) <= 64 * 1024;
+    bool topo = pol != 3, wl = pol == 2 || pol == 0;
+    if (pol == 0) {
+        topo = bytes(springs, true, false) <= 64 * 1024;
+        wl = topo && r->n_hidden == HMAX && bytes(springs, true, true) <= 80 * 1024;
     }
-    DL_CHECK_ARG(sh <= g_lds_limit, "rollout: robot too large for LDS (%zu bytes)", sh);
+    size_t sh = bytes(springs, topo, wl);
+    DL_CHECK_ARG(sh <= kLdsLimit, "rollout: robot too large for LDS (%zu bytes)", sh);
+    DL_CHECK_ARG(springs || r->scratch != nullptr,
+                 "rollout: robot spills springs to scratch; scratch required");
+    A.springs_in_lds = springs;
+    A.topo_in_lds = topo;
+    A.w_in_lds = wl;
     if (bwd) {
         if (sh > 65536)
             DL_CHECK_HIP(hipFuncSetAttribute((const void *)k_rollout_bwd<T>,

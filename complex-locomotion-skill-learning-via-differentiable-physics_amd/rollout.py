@@ -59,13 +59,15 @@ class MassSpringBatch:
 
     def __init__(self, designs, robot_design, horizons, physics, layout, targets, x0,
                  n_hidden=64, act="tanh", ckpt_every=50, dtype=torch.float32, block=256,
-                 loss_window=500, store_traj=False, stiffness_scale=0.1, device="cuda"):
+                 loss_window=500, store_traj=False, stiffness_scale=0.1, device="cuda",
+                 lds_policy=0):
         self.designs = designs
         self.robot_design = np.asarray(robot_design, np.int64)
         R = self.R = len(self.robot_design)
         self.horizons = np.broadcast_to(np.asarray(horizons, np.int64), (R,)).copy()
         self.dtype, self.device = dtype, torch.device(device)
         self.H, self.K, self.block = n_hidden, ckpt_every, block
+        self.lds_policy = lds_policy
         self.act = nat.ACT_SIN if act == "sin" else nat.ACT_TANH
         self.phys, self.layout = physics, layout
         self.loss_window = loss_window
@@ -232,6 +234,7 @@ class MassSpringBatch:
         D.loss_window = max(self.loss_window, 0)
         D.store_traj = int(self.store_traj)
         D.max_n, D.max_s, D.max_i, D.max_o = self.max_n, self.max_s, self.max_i, self.max_o
+        D.lds_policy = self.lds_policy
         D.phys = self.phys
         D.lay = self.layout
         P = nat.ptr
