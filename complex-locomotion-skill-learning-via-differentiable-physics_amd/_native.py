@@ -112,6 +112,15 @@ _SIGS = {
     "dl_ms_rollout_forward": ([POINTER(Rollout), _P], c_int32),
     "dl_ms_rollout_backward": ([POINTER(Rollout), _P], c_int32),
     "dl_wgrad_reduce": ([_I, _I, _P, _P, _I, _P, _P, _P, _P, _P, _P], c_int32),
+    "dl_mpm_fill": ([_I, _P, c_int64, c_double, _P], c_int32),
+    "dl_mpm_p2g": ([_I, _P, _I, _I] + [_P] * 6 + [_I] + [_P] * 5, c_int32),
+    "dl_mpm_grid": ([_I, _P, _I] + [_P] * 4, c_int32),
+    "dl_mpm_g2p": ([_I, _P, _I, _I] + [_P] * 6, c_int32),
+    "dl_mpm_g2p_adj": ([_I, _P, _I, _I] + [_P] * 8, c_int32),
+    "dl_mpm_grid_adj": ([_I, _P, _I] + [_P] * 6, c_int32),
+    "dl_mpm_p2g_adj": ([_I, _P, _I, _I] + [_P] * 6 + [_I] + [_P] * 9, c_int32),
+    "dl_mpm_sensor": ([_I] * 7 + [c_double] * 2 + [_P] * 9, c_int32),
+    "dl_mpm_sensor_adj": ([_I] * 5 + [c_double] + [_P] * 6, c_int32),
 }
 
 _lib = None

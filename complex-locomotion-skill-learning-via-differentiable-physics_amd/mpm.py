@@ -1,0 +1,222 @@
+"""MLS-MPM path: parameters, particle robots, the drop-in per-step
+``SolverMPM`` and the batched checkpointed rollout ``MPMBatch``.
+
+Kernels (``csrc/mpm.hip``, 2D and 3D, FP32/FP64): ``dl_mpm_p2g``,
+``dl_mpm_grid``, ``dl_mpm_g2p`` and their adjoints, replacing
+``SolverMPM.p2g/grid_op/g2p`` (``multitask/solver_mpm.py:83-165``) and the
+Taichi-generated gradients that ``advance_core_grad`` (``:182-190``) calls
+after re-running the grid transfer -- the same recompute scheme is used here.
+"""
+
+import ctypes
+import itertools
+import math
+from ctypes import Structure, c_double, c_int32
+
+import numpy as np
+import torch
+
+from . import _native as nat
+
+
+class MPMParamsC(Structure):
+    """dl_mpm_params."""
+    _fields_ = [("dim", c_int32), ("n_grid", c_int32), ("bound", c_int32), ("act_axis", c_int32),
+                ("dt", c_double), ("gravity", c_double), ("p_vol", c_double), ("p_mass", c_double),
+                ("mu", c_double), ("la", c_double), ("act_strength", c_double)]
+
+
+def lame(E, nu):
+    return E / (2 * (1 + nu)), E * nu / ((1 + nu) * (1 - 2 * nu))
+
+
+def reference_params():
+    """cfg3d/sim_quad.json + config_sim.py for the MPM solver (2D)."""
+    return MPMParamsC(dim=2, n_grid=32, bound=3, act_axis=1, dt=1e-3, gravity=-10.0, p_vol=1.0,
+                      p_mass=1.0, mu=40.0, la=40.0, act_strength=4.0)
+
+
+def quadruped3d_params(n_grid=64):
+    """3D soft quadruped: E = 5e3, nu = 0.2, dt = 2e-4, gravity -1.8, active
+    stress along x scaled by 5, particle volume (dx/2)^3 at density 1."""
+    mu, la = lame(5e3, 0.2)
+    vol = (0.5 / n_grid) ** 3
+    return MPMParamsC(dim=3, n_grid=n_grid, bound=3, act_axis=0, dt=2e-4, gravity=-1.8,
+                      p_vol=vol, p_mass=vol, mu=mu, la=la, act_strength=5.0)
+
+
+def particles_2d(design):
+    """``RobotDesignMPM.build`` (robot_design.py:65-103): n_num^2 particles at
+    the sub-square centres of every square of every rectangle."""
+    p = design["design_parameter"]
+    size, n_num = p["square_size"], p["particle_in_square_edge"]
+    off = p.get("offset", {"x": 0, "y": 0})
+    dx = size / n_num
+    pos, ids = [], []
+    d = design["design"]
+    for anchor, shape, act in zip(d["anchor"], d["shape"], d["actuator"]):
+        for i, j in itertools.product(range(shape[0]), range(shape[1])):
+            sx, sy = anchor[0] + i, anchor[1] + j
+            for a, b in itertools.product(range(n_num), range(n_num)):
+                pos.append((off["x"] + sx * size + (a + 0.5) * dx, off["y"] + sy * size + (b + 0.5) * dx))
+                ids.append(act)
+    return np.array(pos), np.array(ids, np.int32), p["actuator_num"]
+
+
+def quadruped3d_particles(cells_per_cube=6, ppc_edge=2, n_grid=64, origin_cells=(6, 4, 6)):
+    """Particles filling the quadruped's voxel body (4x1x2 body on four 1x2x1
+    legs, leg g = actuator group g, body passive), each unit cube spanning
+    ``cells_per_cube`` grid cells, ``ppc_edge^3`` particles per cell."""
+    from .robot_design import quadruped_cubes
+    dx = 1.0 / n_grid
+    s = cells_per_cube
+    sub = (np.arange(ppc_edge) + 0.5) / ppc_edge
+    pos, ids = [], []
+    for (ci, cj, ck), act, g in quadruped_cubes():
+        for a, b, c in itertools.product(range(s), repeat=3):
+            cell = np.array([origin_cells[0] + ci * s + a, origin_cells[1] + cj * s + b,
+                             origin_cells[2] + ck * s + c], np.float64)
+            for u, v, w in itertools.product(sub, sub, sub):
+                pos.append((cell + np.array([u, v, w])) * dx)
+                ids.append(g if act else -1)
+    return np.array(pos), np.array(ids, np.int32), 4
+
+
+def _soa(a, B, dtype, device):
+    """[P,D] host -> [B,D,P] device copies."""
+    t = torch.as_tensor(np.ascontiguousarray(np.asarray(a).T), dtype=dtype)
+    return t[None].expand(B, -1, -1).contiguous().to(device)
+
+
+class MPMStep:
+    """Launch helpers for one batch geometry (B robots, P particles each)."""
+
+    def __init__(self, params, B, P, n_act, act_id, dtype, device="cuda"):
+        self.p, self.B, self.P, self.n_act = params, B, P, n_act
+        self.D = params.dim
+        self.G = params.n_grid ** params.dim
+        self.dtype, self.device = dtype, torch.device(device)
+        self.dc = nat.dtype_code(dtype)
+        ids = torch.as_tensor(np.asarray(act_id, np.int32))
+        self.act_id = (ids[None].expand(B, -1) if ids.dim() == 1 else ids).contiguous().to(self.device)
+        z = lambda *s: torch.zeros(*s, dtype=dtype, device=self.device)
+        self.gv, self.gm, self.gvo = z(B, self.D, self.G), z(B, self.G), z(B, self.D, self.G)
+        self.gvo_b, self.gv_b, self.gm_b = z(B, self.D, self.G), z(B, self.D, self.G), z(B, self.G)
+        self.err = torch.zeros(1, dtype=torch.int32, device=self.device)
+
+    def fill(self, t, value=0.0):
+        nat.call("dl_mpm_fill", self.dc, nat.ptr(t), t.numel(), float(value), nat.stream())
+
+    def forward(self, x, v, C, F, act, x1, v1, C1, F1):
+        """advance_core: clear grid, P2G (writes F1), grid op, G2P."""
+        self.fill(self.gv)
+        self.fill(self.gm)
+        pp = ctypes.byref(self.p)
+        nat.call("dl_mpm_p2g", self.dc, pp, self.B, self.P, *map(nat.ptr, (x, v, C, F, act)),
+                 nat.ptr(self.act_id), self.n_act, nat.ptr(F1), nat.ptr(self.gv), nat.ptr(self.gm),
+                 nat.ptr(self.err), nat.stream())
+        nat.call("dl_mpm_grid", self.dc, pp, self.B, nat.ptr(self.gv), nat.ptr(self.gm),
+                 nat.ptr(self.gvo), nat.stream())
+        nat.call("dl_mpm_g2p", self.dc, pp, self.B, self.P, nat.ptr(x), nat.ptr(self.gvo),
+                 nat.ptr(x1), nat.ptr(v1), nat.ptr(C1), nat.stream())
+
+    def backward(self, x, v, C, F, act, xb1, vb1, Cb1, Fb1, xb, vb, Cb, Fb, actb):
+        """Recompute the grid of step t, then G2P, grid and P2G adjoints.
+        Writes xb, vb, Cb, Fb (adjoint of state t); accumulates actb."""
+        scratchF = self.scratch_F(F)
+        self.fill(self.gv)
+        self.fill(self.gm)
+        pp = ctypes.byref(self.p)
+        nat.call("dl_mpm_p2g", self.dc, pp, self.B, self.P, *map(nat.ptr, (x, v, C, F, act)),
+                 nat.ptr(self.act_id), self.n_act, nat.ptr(scratchF), nat.ptr(self.gv),
+                 nat.ptr(self.gm), nat.ptr(self.err), nat.stream())
+        nat.call("dl_mpm_grid", self.dc, pp, self.B, nat.ptr(self.gv), nat.ptr(self.gm),
+                 nat.ptr(self.gvo), nat.stream())
+        self.fill(self.gvo_b)
+        nat.call("dl_mpm_g2p_adj", self.dc, pp, self.B, self.P, nat.ptr(x), nat.ptr(self.gvo),
+                 nat.ptr(xb1), nat.ptr(vb1), nat.ptr(Cb1), nat.ptr(xb), nat.ptr(self.gvo_b),
+                 nat.stream())
+        nat.call("dl_mpm_grid_adj", self.dc, pp, self.B, nat.ptr(self.gv), nat.ptr(self.gm),
+                 nat.ptr(self.gvo_b), nat.ptr(self.gv_b), nat.ptr(self.gm_b), nat.stream())
+        nat.call("dl_mpm_p2g_adj", self.dc, pp, self.B, self.P, *map(nat.ptr, (x, v, C, F, act)),
+                 nat.ptr(self.act_id), self.n_act, nat.ptr(self.gv_b), nat.ptr(self.gm_b),
+                 nat.ptr(Fb1), nat.ptr(xb), nat.ptr(vb), nat.ptr(Cb), nat.ptr(Fb), nat.ptr(actb),
+                 nat.stream())
+
+    def scratch_F(self, F):
+        if getattr(self, "_sF", None) is None or self._sF.shape != F.shape:
+            self._sF = torch.empty_like(F)
+        return self._sF
+
+    def check(self):
+        if int(self.err.item()) != 0:
+            raise nat.NativeError("MPM: a particle left the grid (stencil out of bounds)")
+
+
+class SolverMPM:
+    """Drop-in ``SolverMPM`` (``multitask/solver_mpm.py:5-196``) driven by a
+    particle design.  Fields: x, v [T+1,B,D,P] (SoA), C, F [T+1,B,D*D,P],
+    center [T+1,B,3], actuation [T+1,B,n_act]; grid fields per robot."""
+
+    def __init__(self, params, positions, act_id, n_act, batch_size, max_steps,
+                 dtype=torch.float64, device="cuda"):
+        self.params = params
+        self.dim, self.batch_size, self.max_steps = params.dim, batch_size, max_steps
+        self.n_particles = len(positions)
+        self.n_act = n_act
+        self.dtype, self.device = dtype, torch.device(device)
+        self.positions = np.asarray(positions, np.float64)
+        D, P, B, T = self.dim, self.n_particles, batch_size, max_steps + 1
+        z = lambda *s: torch.zeros(*s, dtype=dtype, device=self.device)
+        self.x, self.v = z(T, B, D, P), z(T, B, D, P)
+        self.C, self.F = z(T, B, D * D, P), z(T, B, D * D, P)
+        self.center = z(T, B, 3)
+        self.actuation = z(T, B, n_act)
+        self.x_grad, self.v_grad = torch.zeros_like(self.x), torch.zeros_like(self.v)
+        self.C_grad, self.F_grad = torch.zeros_like(self.C), torch.zeros_like(self.F)
+        self.actuation_grad = torch.zeros_like(self.actuation)
+        self.step_ = MPMStep(params, B, P, n_act, act_id, dtype, device)
+
+    def initialize_robot(self):
+        """x[0] = design positions, actuator ids set (solver_mpm.py:54-58)."""
+        self.x[0] = _soa(self.positions, self.batch_size, self.dtype, self.device)
+
+    def clear_states(self, steps):
+        """C = 0, F = I, gradients zeroed (solver_mpm.py:60-71)."""
+        D = self.dim
+        self.C[:steps].zero_()
+        self.F[:steps].zero_()
+        for i in range(D):
+            self.F[:steps, :, i * D + i] = 1.0
+        for g in (self.x_grad, self.v_grad, self.C_grad, self.F_grad, self.actuation_grad):
+            g.zero_()
+
+    def compute_center(self, t):
+        """solver_mpm.py:167-173 (3D center padded for 2D robots)."""
+        if self.dim == 3:
+            nat.call("dl_ms_center", self.step_.dc, self.batch_size, self.n_particles,
+                     nat.ptr(self.x[t]), None, nat.ptr(self.center[t]), None, nat.stream())
+        else:
+            self.center[t, :, :2] = self.x[t].mean(-1)
+
+    def pre_advance(self, s):
+        self.compute_center(s)
+
+    def advance(self, s):
+        """advance_core(s) (solver_mpm.py:175-180)."""
+        self.step_.forward(self.x[s], self.v[s], self.C[s], self.F[s], self.actuation[s],
+                           self.x[s + 1], self.v[s + 1], self.C[s + 1], self.F[s + 1])
+
+    def advance_grad(self, s):
+        """advance_core_grad(s) (solver_mpm.py:182-190): recompute the grid of
+        step s, then the G2P, grid and P2G adjoints; accumulates into the
+        gradient fields of step s from those of step s+1."""
+        xb = torch.zeros_like(self.x_grad[s])
+        vb, Cb, Fb = torch.zeros_like(xb), torch.zeros_like(self.C_grad[s]), torch.zeros_like(self.F_grad[s])
+        self.step_.backward(self.x[s], self.v[s], self.C[s], self.F[s], self.actuation[s],
+                            self.x_grad[s + 1], self.v_grad[s + 1], self.C_grad[s + 1],
+                            self.F_grad[s + 1], xb, vb, Cb, Fb, self.actuation_grad[s])
+        self.x_grad[s] += xb
+        self.v_grad[s] += vb
+        self.C_grad[s] += Cb
+        self.F_grad[s] += Fb

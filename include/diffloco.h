@@ -258,6 +258,71 @@ int32_t dl_wgrad_reduce(int32_t dtype, int32_t n_robots, const int32_t *robot_li
                         const int64_t *design_size, const void *wgrad, void *out,
                         void *stream);
 
+/* ------------------------------------------------------------- MLS-MPM API */
+
+/* MLS-MPM constants (multitask/solver_mpm.py:23-33, config_sim.py:112-129).
+ * Active stress A = act_strength * a * e e^T along axis act_axis (reference:
+ * y in 2D). */
+typedef struct dl_mpm_params {
+    int32_t dim;             /* 2 or 3 */
+    int32_t n_grid;          /* n cells per axis, dx = 1/n */
+    int32_t bound;           /* sticky-wall width in cells */
+    int32_t act_axis;
+    double dt, gravity, p_vol, p_mass, mu, la, act_strength;
+} dl_mpm_params;
+
+/* Particle fields per robot, structure-of-arrays: x, v [B][D][P]; C, F
+ * [B][D*D][P] (row-major entries); act [B][n_act]; act_id [B][P] (-1 =
+ * passive).  Grids per robot: grid_v [B][D][n^D], grid_m [B][n^D]. */
+
+/* Fill n reals with a value (grid clears; graph-capture safe). */
+int32_t dl_mpm_fill(int32_t dtype, void *ptr, int64_t n, double value, void *stream);
+/* p2g (solver_mpm.py:83-124): F update, polar, stress, APIC scatter into the
+ * (cleared) grid with atomics; writes F_out.  *err |= 1 if a stencil leaves
+ * the grid. */
+int32_t dl_mpm_p2g(int32_t dtype, const dl_mpm_params *params, int32_t B, int32_t P,
+                   const void *x, const void *v, const void *C, const void *F, const void *act,
+                   const int32_t *act_id, int32_t n_act, void *F_out, void *grid_v,
+                   void *grid_m, int32_t *err, void *stream);
+/* grid_op (solver_mpm.py:126-144). */
+int32_t dl_mpm_grid(int32_t dtype, const dl_mpm_params *params, int32_t B, const void *grid_v,
+                    const void *grid_m, void *grid_v_out, void *stream);
+/* g2p (solver_mpm.py:146-165). */
+int32_t dl_mpm_g2p(int32_t dtype, const dl_mpm_params *params, int32_t B, int32_t P,
+                   const void *x, const void *grid_v_out, void *x_out, void *v_out, void *C_out,
+                   void *stream);
+/* Adjoints (the Taichi-generated g2p.grad, grid_op.grad, p2g.grad that
+ * advance_core_grad runs, solver_mpm.py:182-190).  g2p_adj WRITES x_grad
+ * (= x_grad1 + stencil terms) and accumulates grid_v_out_grad atomically;
+ * grid_adj writes the grid adjoints; p2g_adj accumulates x_grad, WRITES
+ * v_grad, C_grad, F_grad, and accumulates act_grad atomically. */
+int32_t dl_mpm_g2p_adj(int32_t dtype, const dl_mpm_params *params, int32_t B, int32_t P,
+                       const void *x, const void *grid_v_out, const void *x_grad1,
+                       const void *v_grad1, const void *C_grad1, void *x_grad,
+                       void *grid_v_out_grad, void *stream);
+int32_t dl_mpm_grid_adj(int32_t dtype, const dl_mpm_params *params, int32_t B,
+                        const void *grid_v, const void *grid_m, const void *grid_v_out_grad,
+                        void *grid_v_grad, void *grid_m_grad, void *stream);
+int32_t dl_mpm_p2g_adj(int32_t dtype, const dl_mpm_params *params, int32_t B, int32_t P,
+                       const void *x, const void *v, const void *C, const void *F,
+                       const void *act, const int32_t *act_id, int32_t n_act,
+                       const void *grid_v_grad, const void *grid_m_grad, const void *F_grad1,
+                       void *x_grad, void *v_grad, void *C_grad, void *F_grad, void *act_grad,
+                       void *stream);
+/* Sensor vector of an MPM robot (3D): [n_sin sinusoids | COM velocity |
+ * per-group mean offset from the COM / offset_scale | target_v / max_speed |
+ * target_h]; targets [B][5]; group [B][P] in [0, n_groups); inv_count [B][G]
+ * = 1 / particles per group.  Also writes center/vcom [B][3] when non-NULL.
+ * No reference counterpart: the reference's MPM env path is unreachable
+ * (taichi_env.py:136 calls SolverMPM() without its config). */
+int32_t dl_mpm_sensor(int32_t dtype, int32_t B, int32_t P, int32_t n_groups, int32_t n_sin,
+                      int32_t run_period, int32_t t, double offset_scale, double max_speed,
+                      const void *x, const void *v, const int32_t *group, const void *targets,
+                      const void *inv_count, void *in, void *center, void *vcom, void *stream);
+int32_t dl_mpm_sensor_adj(int32_t dtype, int32_t B, int32_t P, int32_t n_groups, int32_t n_sin,
+                          double offset_scale, const int32_t *group, const void *inv_count,
+                          const void *in_grad, void *x_grad, void *v_grad, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

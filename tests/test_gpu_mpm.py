@@ -1,0 +1,122 @@
+"""HIP MLS-MPM step (P2G, grid, G2P) and its adjoint against the FP64 oracle,
+2D (the reference's robot_mpm_1 design) and 3D.  FP64 agrees to rounding;
+FP32 within 1e-5 relative on positions (atomics reorder the P2G sums, the
+only run-to-run variation of this path)."""
+
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from diffloco import mpm as dm
+from oracle import builders as ob
+from oracle import mpm as omp
+
+from helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+DT = {"f64": torch.float64, "f32": torch.float32}
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _case(dim, B, seed):
+    g = torch.Generator().manual_seed(seed)
+    if dim == 2:
+        pos, ids, n_act = ob.MPM2D(ob.load_json("tests/golden/robot_mpm_1.json")).build()
+        pos = torch.tensor(pos, dtype=torch.float64) + torch.tensor([0.2, 0.15])
+        params = dm.reference_params()
+        op = omp.MPMParams()
+    else:
+        n = 16
+        side = torch.arange(6, dtype=torch.float64)
+        grid = torch.stack(torch.meshgrid(side, side, side, indexing="ij"), -1).reshape(-1, 3)
+        pos = (grid * 0.5 + 5.0) / n
+        ids = [(i % 5) - 1 for i in range(len(pos))]
+        n_act = 4
+        params = dm.quadruped3d_params(n_grid=n)
+        params.dt = 1e-3
+        op = omp.MPMParams(dim=3, n_grid=n, dt=params.dt, gravity=params.gravity,
+                           p_vol=params.p_vol, p_mass=params.p_mass, mu=params.mu, la=params.la,
+                           act_strength=params.act_strength, act_axis=0, bound=3)
+    P, D = pos.shape
+    x = pos[None].expand(B, -1, -1) + 0.002 * torch.randn(B, P, D, generator=g, dtype=torch.float64)
+    v = 0.3 * torch.randn(B, P, D, generator=g, dtype=torch.float64)
+    C = 2.0 * torch.randn(B, P, D, D, generator=g, dtype=torch.float64)
+    F = torch.eye(D, dtype=torch.float64) + 0.05 * torch.randn(B, P, D, D, generator=g, dtype=torch.float64)
+    act = torch.rand(B, n_act, generator=g, dtype=torch.float64) * 2 - 1
+    act_id = torch.tensor(ids, dtype=torch.int64)[None].expand(B, -1)
+    return params, op, x, v, C, F, act, act_id, n_act
+
+
+def _vsoa(a, dt):
+    return a.transpose(1, 2).contiguous().to("cuda", dt)
+
+
+def _msoa(a, dt):
+    B, P, D, _ = a.shape
+    return a.reshape(B, P, D * D).transpose(1, 2).contiguous().to("cuda", dt)
+
+
+def _vback(a):
+    return a.double().cpu().transpose(1, 2)
+
+
+def _mback(a, D):
+    B, _, P = a.shape
+    return a.double().cpu().transpose(1, 2).reshape(B, P, D, D)
+
+
+@pytest.mark.parametrize("dim,prec", [(2, "f64"), (2, "f32"), (3, "f64"), (3, "f32")])
+def test_mpm_step_matches_oracle(dim, prec):
+    _say(f"mpm step {dim}d {prec}")
+    B = 2
+    params, op, x, v, C, F, act, act_id, n_act = _case(dim, B, seed=20 + dim)
+    dt = DT[prec]
+    st = dm.MPMStep(params, B, x.shape[1], n_act, act_id[0].numpy(), dt)
+    ins = [_vsoa(x, dt), _vsoa(v, dt), _msoa(C, dt), _msoa(F, dt), act.to("cuda", dt)]
+    outs = [torch.empty_like(ins[0]), torch.empty_like(ins[1]), torch.empty_like(ins[2]),
+            torch.empty_like(ins[3])]
+    st.forward(*ins, *outs)
+    torch.cuda.synchronize()
+    st.check()
+    xo, vo, Co, Fo = omp.step(_vback(ins[0]), _vback(ins[1]), _mback(ins[2], dim),
+                              _mback(ins[3], dim), ins[4].double().cpu(), act_id, op)
+    tol = 1e-10 if prec == "f64" else 2e-5
+    assert rel_err(_vback(outs[0]), xo) < tol
+    assert rel_err(_vback(outs[1]), vo) < tol * 10
+    assert rel_err(_mback(outs[2], dim), Co) < tol * 10
+    assert rel_err(_mback(outs[3], dim), Fo) < tol
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_mpm_adjoint_matches_autograd(dim):
+    _say(f"mpm adjoint {dim}d")
+    B = 2
+    params, op, x, v, C, F, act, act_id, n_act = _case(dim, B, seed=30 + dim)
+    dt = torch.float64
+    P = x.shape[1]
+    g = torch.Generator().manual_seed(40 + dim)
+    xb1, vb1 = torch.randn(B, P, dim, generator=g, dtype=dt), torch.randn(B, P, dim, generator=g, dtype=dt)
+    Cb1, Fb1 = torch.randn(B, P, dim, dim, generator=g, dtype=dt), torch.randn(B, P, dim, dim, generator=g, dtype=dt)
+    st = dm.MPMStep(params, B, P, n_act, act_id[0].numpy(), dt)
+    ins = [_vsoa(x, dt), _vsoa(v, dt), _msoa(C, dt), _msoa(F, dt), act.to("cuda", dt)]
+    ups = [_vsoa(xb1, dt), _vsoa(vb1, dt), _msoa(Cb1, dt), _msoa(Fb1, dt)]
+    xb, vb = torch.zeros_like(ins[0]), torch.zeros_like(ins[1])
+    Cb, Fb = torch.zeros_like(ins[2]), torch.zeros_like(ins[3])
+    ab = torch.zeros_like(ins[4])
+    st.backward(*ins, *ups, xb, vb, Cb, Fb, ab)
+    torch.cuda.synchronize()
+    st.check()
+    xr, vr, Cr, Fr, ar = [t.clone().requires_grad_() for t in (x, v, C, F, act)]
+    xo, vo, Co, Fo = omp.step(xr, vr, Cr, Fr, ar, act_id, op)
+    ((xo * xb1).sum() + (vo * vb1).sum() + (Co * Cb1).sum() + (Fo * Fb1).sum()).backward()
+    assert rel_err(_vback(xb), xr.grad) < 1e-9
+    assert rel_err(_vback(vb), vr.grad) < 1e-9
+    assert rel_err(_mback(Cb, dim), Cr.grad) < 1e-9
+    assert rel_err(_mback(Fb, dim), Fr.grad) < 1e-8
+    assert rel_err(ab.cpu(), ar.grad) < 1e-8
