@@ -71,7 +71,7 @@ class Rollout(Structure):
     _fields_ = [(n, c_int32) for n in ("dtype", "act", "n_robots", "n_hidden", "ckpt_every",
                                        "max_steps", "block", "loss_kind", "loss_window",
                                        "store_traj", "max_n", "max_s", "max_i", "max_o",
-                                       "lds_policy", "pad_")] + \
+                                       "lds_policy", "variant")] + \
                [("phys", MSPhysics), ("lay", SensorLayoutC)] + \
                [(n, c_void_p) for n in ("robots", "spring_a", "spring_b", "spring_act",
                                         "spring_L0", "spring_k", "spring_coeff", "vert_ptr",

@@ -49,6 +49,17 @@ template <> struct M<double> {
     static __device__ __forceinline__ double atan2(double y, double x) { return ::atan2(y, x); }
 };
 
+// Square root and reciprocal: IEEE by default; with FAST (FP32 fast path) the
+// hardware v_sqrt_f32 / v_rcp_f32 instructions (about 1 ulp).
+template <bool FAST, typename T>
+__device__ __forceinline__ T fsqrt(T x) { return M<T>::sqrt(x); }
+template <bool FAST, typename T>
+__device__ __forceinline__ T frcp(T x) { return T(1) / x; }
+template <>
+__device__ __forceinline__ float fsqrt<true, float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+template <>
+__device__ __forceinline__ float frcp<true, float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // activation value and derivative (nn.py:139,146 use sin; the quadruped MLP tanh)
 template <typename T>
 __device__ __forceinline__ void activate(int kind, T z, T &y, T &dy) {
@@ -113,14 +124,14 @@ __host__ Phys<T> make_phys(const dl_ms_physics &p) {
 //   L = sqrt(|d|^2 + 1e-8) + 1e-4, L* = L0 (1 + coeff a)
 //   J = dt k (L - L*) / L d + c ((va - vb) . d^) d^,   d = xa - xb
 // The impulse is subtracted from vertex a and added to vertex b.
-template <typename T>
+template <typename T, bool FAST = false>
 __device__ __forceinline__ void spring_impulse(const T d[3], const T w[3], T L0, T k, T coeff,
                                                T act, const Phys<T> &P, T J[3]) {
     const T q = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-    const T len = M<T>::sqrt(q + T(1e-8)) + T(1e-4);
+    const T len = fsqrt<FAST>(q + T(1e-8)) + T(1e-4);
     const T target = L0 * (T(1) + coeff * act);
-    const T A = P.dt * (len - target) * k / len;
-    const T inv = T(1) / M<T>::sqrt(q);
+    const T A = FAST ? P.dt * (len - target) * k * frcp<FAST>(len) : P.dt * (len - target) * k / len;
+    const T inv = frcp<FAST>(fsqrt<FAST>(q));
     const T dh[3] = {d[0] * inv, d[1] * inv, d[2] * inv};
     const T vrel = w[0] * dh[0] + w[1] * dh[1] + w[2] * dh[2];
     const T C = P.dashpot * vrel;
@@ -130,26 +141,27 @@ __device__ __forceinline__ void spring_impulse(const T d[3], const T w[3], T L0,
 
 // Adjoint of spring_impulse: given Jb = dL/dJ returns gd = dL/dd (add to xa,
 // subtract from xb), gw = dL/dw (add to va, subtract from vb), ga = dL/dact.
-template <typename T>
+template <typename T, bool FAST = false>
 __device__ __forceinline__ void spring_impulse_adj(const T d[3], const T w[3], T L0, T k, T coeff,
                                                    T act, const Phys<T> &P, const T Jb[3],
                                                    T gd[3], T gw[3], T &ga) {
     const T q = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-    const T rq = M<T>::sqrt(q + T(1e-8));
+    const T rq = fsqrt<FAST>(q + T(1e-8));
     const T len = rq + T(1e-4);
+    const T ilen = frcp<FAST>(len);
     const T target = L0 * (T(1) + coeff * act);
-    const T A = P.dt * (len - target) * k / len;
-    const T n = M<T>::sqrt(q);
-    const T inv = T(1) / n;
+    const T A = FAST ? P.dt * (len - target) * k * ilen : P.dt * (len - target) * k / len;
+    const T n = fsqrt<FAST>(q);
+    const T inv = frcp<FAST>(n);
     const T dh[3] = {d[0] * inv, d[1] * inv, d[2] * inv};
     const T vrel = w[0] * dh[0] + w[1] * dh[1] + w[2] * dh[2];
     // term 1: A d
     const T Ab = Jb[0] * d[0] + Jb[1] * d[1] + Jb[2] * d[2];
-    const T dkl = P.dt * k / len;
-    const T Lb = Ab * dkl * target / len;        // dA/dL = dt k L* / L^2
+    const T dkl = FAST ? P.dt * k * ilen : P.dt * k / len;
+    const T Lb = FAST ? Ab * dkl * target * ilen : Ab * dkl * target / len;   // dA/dL = dt k L* / L^2
     const T Tb = -Ab * dkl;                      // dA/dL* = -dt k / L
     ga = Tb * L0 * coeff;
-    const T lq = Lb / rq;                        // dL/dd = d / sqrt(q + eps)
+    const T lq = FAST ? Lb * frcp<FAST>(rq) : Lb / rq;   // dL/dd = d / sqrt(q + eps)
     // term 2: c vrel dh
     const T Jdh = Jb[0] * dh[0] + Jb[1] * dh[1] + Jb[2] * dh[2];
     const T vb = P.dashpot * Jdh;                // d/dvrel
@@ -275,14 +287,15 @@ __device__ __forceinline__ T sensor_value(const Layout &L, int q, int t, int off
     }
     q -= L.n_sin;
     if (L.vcom) {
-        if (q < 3) return vc[q];
+        if (q < 3) return q == 0 ? vc[0] : (q == 1 ? vc[1] : vc[2]);   // no dynamic private index
         q -= 3;
     }
     const int vs = L.vstride();
     if (q < N * vs) {
         const int i = q / vs, d = q - i * vs;
         if (d >= 3) return T(0);
-        return (xs[d * N + i] - c[d]) / T(L.offset_scale);
+        const T cd = d == 0 ? c[0] : (d == 1 ? c[1] : c[2]);
+        return (xs[d * N + i] - cd) / T(L.offset_scale);
     }
     q -= N * vs;
     if (q < L.dup_v * L.n_tv) {

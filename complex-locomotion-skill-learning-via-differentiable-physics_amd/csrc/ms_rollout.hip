@@ -337,6 +337,7 @@ __device__ void fwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int t, in
     // gather + integrate (vertex owned by one thread, in place)
     for (int i = tid; i < N; i += BT) {
         T vi[3] = {0, 0, 0};
+#pragma unroll 4
         for (int e = R.vptr[i]; e < R.vptr[i + 1]; ++e) {
             const int ent = R.vidx[e], s = ent >> 1;
             const T sg = (ent & 1) ? T(1) : T(-1);
@@ -644,6 +645,7 @@ __global__ void __launch_bounds__(256) k_rollout_bwd(RArgs<T> A) {
         __syncthreads();
         for (int t = s0; t < tb; ++t) fwd_step(A, R, m, t, MODE_REC, t - s0, dummy);
         for (int t = tb - 1; t >= lo; --t) bwd_step(A, R, m, r, t, t - s0);
+        __syncthreads();
         // the weight-gradient fold reads slots lo - s0 .. tb - s0 - 1
         {
             Rob<T> Rs = R;
@@ -659,6 +661,687 @@ __global__ void __launch_bounds__(256) k_rollout_bwd(RArgs<T> A) {
         tb = lo;
     }
     for (int i = tid; i < 3 * N; i += BT) {
+        R.adj[i] = m.xb[i];
+        R.adj[3 * N + i] = m.vb[i];
+    }
+}
+
+// ===================================================================
+// Wave-per-robot rollout (FP32, MLP input width <= IMAX).
+//
+// A workgroup holds WAVES robots, one wavefront each, so every SIMD of a CU
+// advances its own robot with no workgroup barrier inside the step loop.
+// Lane j keeps column j of W1^T (the weights of hidden unit j) in VGPRs for
+// the forward GEMV; the adjoint GEMV W1^T z reads one LDS copy of W1 shared
+// by the workgroup's robots (robots of one design are packed together).  The
+// design topology is staged into LDS once per workgroup.
+// ===================================================================
+
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T>
+struct WSm {
+    T *xs, *vs, *xb, *vb, *vib, *J, *inb, *inbar, *z1b, *outb, *dout, *z2b;
+};
+
+__host__ __device__ inline size_t wave_region(int max_n, int max_s, int max_i, int max_o, bool bwd) {
+    const size_t ni = (size_t)(max_i > 192 ? max_i : 192) + 4;   // GEMV reads up to 192 inputs
+    size_t n = (bwd ? 15 : 6) * (size_t)max_n + (bwd ? 7 : 3) * (size_t)max_s +
+               (bwd ? 2 : 1) * ni + HMAX + 3 * (size_t)max_o;
+    return (n + 3) & ~(size_t)3;
+}
+
+__host__ inline size_t wave_smem_bytes(int max_n, int max_s, int max_i, int max_o, bool bwd,
+                                       int n_tab, int waves) {
+    size_t n = waves * wave_region(max_n, max_s, max_i, max_o, bwd) + 3 * (size_t)max_s + n_tab;
+    if (bwd) n += (size_t)max_i * (HMAX + 1);
+    return n * sizeof(float) + sizeof(int) * (6 * (size_t)max_s + max_n + max_o + 2);
+}
+
+template <typename T>
+__device__ __forceinline__ WSm<T> wave_carve(const RArgs<T> &A, T *base, int wid, bool bwd) {
+    T *p = base + (size_t)wid * wave_region(A.max_n, A.max_s, A.max_i, A.max_o, bwd);
+    WSm<T> m;
+    const int n3 = 3 * A.max_n;
+    m.xs = p; p += n3;
+    m.vs = p; p += n3;
+    if (bwd) {
+        m.xb = p; p += n3;
+        m.vb = p; p += n3;
+        m.vib = p; p += n3;
+    } else {
+        m.xb = m.vb = m.vib = nullptr;
+    }
+    m.J = p; p += (bwd ? 7 : 3) * A.max_s;
+    const int ni = (A.max_i > 192 ? A.max_i : 192) + 4;
+    m.inb = p; p += ni;
+    if (bwd) { m.inbar = p; p += ni; } else { m.inbar = nullptr; }
+    m.z1b = p; p += HMAX;
+    m.outb = p; p += A.max_o;
+    m.dout = p; p += A.max_o;
+    m.z2b = p;
+    return m;
+}
+
+// Block-wide: stage the topology of robot r0's design (and W1 for the
+// adjoint) into the shared region; returns pointers through `sh`.
+template <typename T>
+struct WShared {
+    T *topo_r, *wl, *sin_tab;
+    int *topo_i;
+};
+
+template <typename T>
+__device__ WShared<T> wave_stage(const RArgs<T> &A, const Rob<T> &R0, unsigned char *raw, bool bwd) {
+    WShared<T> sh;
+    T *p = reinterpret_cast<T *>(raw) + (blockDim.x >> 6) * wave_region(A.max_n, A.max_s, A.max_i, A.max_o, bwd);
+    sh.topo_r = p; p += 3 * A.max_s;
+    sh.wl = bwd ? p : nullptr;
+    if (bwd) p += (size_t)A.max_i * (HMAX + 1);
+    // sinusoid table over one period: phase 2 pi (t mod P) / P + 2 pi j / n_sin
+    // evaluated in double precision (periodic in t, so equal to the direct form)
+    const int n_tab = A.L.run_period * A.L.n_sin;
+    sh.sin_tab = p; p += n_tab;
+    sh.topo_i = reinterpret_cast<int *>(p);
+    const int tid = threadIdx.x, BT = blockDim.x, S = R0.S, N = R0.N, O = R0.O;
+    for (int q = tid; q < n_tab; q += BT) {
+        const int tt = q / A.L.n_sin, j = q - tt * A.L.n_sin;
+        sh.sin_tab[q] = T(::sin(2.0 * M_PI / A.L.run_period * (double)tt +
+                                2.0 * M_PI / A.L.n_sin * (double)j));
+    }
+    for (int q = tid; q < S; q += BT) {
+        sh.topo_r[q] = R0.L0[q]; sh.topo_r[S + q] = R0.k[q]; sh.topo_r[2 * S + q] = R0.coef[q];
+        sh.topo_i[q] = R0.sa[q]; sh.topo_i[S + q] = R0.sb[q]; sh.topo_i[2 * S + q] = R0.sact[q];
+    }
+    const int v0 = R0.vptr[0], nv = R0.vptr[N] - v0;
+    int *vp = sh.topo_i + 3 * S, *vx = vp + N + 1;
+    for (int q = tid; q <= N; q += BT) vp[q] = R0.vptr[q] - v0;
+    for (int q = tid; q < nv; q += BT) vx[q] = R0.vidx[v0 + q];
+    const int a0 = R0.aptr[0], na = R0.aptr[O] - a0;
+    int *ap = vx + nv, *ax = ap + O + 1;
+    for (int q = tid; q <= O; q += BT) ap[q] = R0.aptr[q] - a0;
+    for (int q = tid; q < na; q += BT) ax[q] = R0.aidx[a0 + q];
+    if (bwd) {
+        const int H = A.H, I = R0.I;
+        for (int q = tid; q < I * H; q += BT) {
+            const int i = q / H, j = q - i * H;
+            sh.wl[i * (HMAX + 1) + j] = R0.W1T[q];
+        }
+    }
+    return sh;
+}
+
+template <typename T>
+__device__ __forceinline__ void wave_point_at_shared(Rob<T> &R, const WShared<T> &sh, bool bwd) {
+    const int S = R.S, N = R.N, O = R.O;
+    const int nv = 2 * S;
+    R.L0 = sh.topo_r; R.k = sh.topo_r + S; R.coef = sh.topo_r + 2 * S;
+    R.sa = sh.topo_i; R.sb = sh.topo_i + S; R.sact = sh.topo_i + 2 * S;
+    R.vptr = sh.topo_i + 3 * S;
+    R.vidx = R.vptr + N + 1;
+    R.aptr = R.vidx + nv;
+    R.aidx = R.aptr + O + 1;
+    if (bwd) R.WL = sh.wl;
+}
+
+template <typename T, int IMAX, int OMAX>
+struct WRegs {
+    T w1[IMAX];   // W1^T column of hidden unit `lane`
+    T w2[OMAX];   // W2 column of hidden unit `lane`
+    T b1;
+};
+
+template <typename T, int IMAX, int OMAX>
+__device__ __forceinline__ void wave_load_weights(const Rob<T> &R, int H, int lane,
+                                                  WRegs<T, IMAX, OMAX> &w) {
+    const bool on = lane < H;
+#pragma unroll
+    for (int i = 0; i < IMAX; ++i) w.w1[i] = (on && i < R.I) ? R.W1T[(size_t)i * H + lane] : T(0);
+#pragma unroll
+    for (int o = 0; o < OMAX; ++o) w.w2[o] = (on && o < R.O) ? R.W2[(size_t)o * H + lane] : T(0);
+    w.b1 = on ? R.b1[lane] : T(0);
+}
+
+// one forward step of the wave's robot (MODE_FWD / MODE_REC like fwd_step).
+// WREG: layer-1 weights from registers (forward kernel) or from the shared
+// LDS copy of W1 (adjoint kernel, whose adjoint phase needs the registers).
+template <typename T, int IMAX, int OMAX, bool WREG>
+__device__ __forceinline__ void wave_fwd_step(const RArgs<T> &A, const Rob<T> &R, WSm<T> &m,
+                              const WRegs<T, IMAX, OMAX> &w, const T *sin_tab, int lane, int t,
+                              int mode, int slot, T &lsum) {
+    const int N = R.N, S = R.S, I = R.I, O = R.O, H = A.H;
+    T s6[6] = {0, 0, 0, 0, 0, 0};
+    const T invN = T(1) / T(N);
+    for (int i = lane; i < N; i += 64) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            s6[d] += m.xs[d * N + i] * invN;
+            s6[3 + d] += m.vs[d * N + i] * invN;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) s6[q] = wave_sum(s6[q]);
+    const T *tg = tgt_at(R, t);
+    if (mode == MODE_FWD) {
+        if (t % A.K == 0) {
+            T *ck = R.ckpt + (size_t)(t / A.K) * 6 * N;
+            for (int i = lane; i < 3 * N; i += 64) {
+                ck[i] = m.xs[i];
+                ck[3 * N + i] = m.vs[i];
+            }
+        }
+        if (A.store_traj && R.traj) {
+            T *row = R.traj + (size_t)t * (3 * N + 3);
+            for (int i = lane; i < 3 * N; i += 64) row[i] = m.xs[i];
+            if (lane < 3) row[3 * N + lane] = lane == 0 ? s6[0] : (lane == 1 ? s6[1] : s6[2]);
+        }
+        if (A.loss_kind == 1 && t > R.Th - A.window) lsum += task_term(s6, tg);
+    } else {
+        T *sx = R.sx + (size_t)slot * 6 * N;
+        for (int i = lane; i < 3 * N; i += 64) {
+            sx[i] = m.xs[i];
+            sx[3 * N + i] = m.vs[i];
+        }
+        if (lane < 6) {
+            T val = s6[0];
+#pragma unroll
+            for (int q = 1; q < 6; ++q) val = lane == q ? s6[q] : val;
+            R.scv[slot * 6 + lane] = val;
+        }
+    }
+    // sensor vector, assembled per block of slots (no per-slot division)
+    {
+        const Layout &L = A.L;
+        const int tmod = t % L.run_period;
+        for (int j = lane; j < L.n_sin; j += 64) m.inb[j] = sin_tab[tmod * L.n_sin + j];
+        if (L.vcom && lane < 3) m.inb[L.n_sin + lane] = lane == 0 ? s6[3] : (lane == 1 ? s6[4] : s6[5]);
+        const int base = L.vert_base(), vs = L.vstride();
+        const T isc = T(1) / T(L.offset_scale);
+        for (int i = lane; i < N; i += 64) {
+            T *o = m.inb + base + i * vs;
+            o[0] = (m.xs[i] - s6[0]) * isc;
+            o[1] = (m.xs[N + i] - s6[1]) * isc;
+            o[2] = (m.xs[2 * N + i] - s6[2]) * isc;
+            for (int d = 3; d < vs; ++d) o[d] = T(0);
+        }
+        const int tail = base + N * vs, nv = L.dup_v * L.n_tv;
+        for (int q = lane; q < I - tail; q += 64) {
+            T val;
+            if (q < nv) {
+                const int mm = q % L.n_tv;
+                val = tg[mm == 0 ? L.tv0 : (mm == 1 ? L.tv1 : L.tv2)] / T(L.max_speed);
+            } else {
+                val = q < nv + L.dup_h ? tg[3] : tg[4];
+            }
+            m.inb[tail + q] = val;
+        }
+    }
+    wave_sync();
+    if (mode == MODE_REC)
+        for (int q = lane; q < I; q += 64) R.sin[(size_t)slot * I + q] = m.inb[q];
+    // layer 1: lane j = hidden unit j, weights in registers, inputs broadcast from LDS
+    T acc = w.b1;
+    if constexpr (WREG) {
+        // weights beyond I are zero and the input tail is zero-filled: no guards
+        T a1 = T(0), a2 = T(0), a3 = T(0);
+#pragma unroll
+        for (int i = 0; i < IMAX; i += 4) {
+            acc += w.w1[i] * m.inb[i];
+            a1 += w.w1[i + 1] * m.inb[i + 1];
+            a2 += w.w1[i + 2] * m.inb[i + 2];
+            a3 += w.w1[i + 3] * m.inb[i + 3];
+        }
+        acc = (acc + a1) + (a2 + a3);
+    } else if (R.WL) {
+        if (lane < H) {
+#pragma unroll 8
+            for (int i = 0; i < I; ++i) acc += R.WL[i * (HMAX + 1) + lane] * m.inb[i];
+        }
+    } else if (lane < H) {
+#pragma unroll 8
+        for (int i = 0; i < I; ++i) acc += R.W1T[(size_t)i * H + lane] * m.inb[i];
+    }
+    T hj, dhj;
+    activate<T>(A.act, acc, hj, dhj);
+    if (lane >= H) hj = dhj = T(0);
+    if (mode == MODE_REC && lane < H) {
+        R.shid[(size_t)slot * H + lane] = hj;
+        R.sdhid[(size_t)slot * H + lane] = dhj;
+    }
+    // layer 2: one wave reduction per output, lane o then activates output o
+    {
+        T zsel = T(0);
+#pragma unroll
+        for (int o = 0; o < OMAX; ++o) {
+            if (o < O) {
+                const T z = wave_sum(w.w2[o] * hj) + R.b2[o];
+                zsel = lane == o ? z : zsel;
+            }
+        }
+        T y, dy;
+        activate<T>(A.act, zsel, y, dy);
+        if (lane < O) {
+            m.outb[lane] = y;
+            m.dout[lane] = dy;
+            if (mode == MODE_REC) {
+                R.sout[(size_t)slot * O + lane] = y;
+                R.sdout[(size_t)slot * O + lane] = dy;
+            }
+        }
+    }
+    wave_sync();
+#pragma unroll 2
+    for (int s = lane; s < S; s += 64) {
+        const int a = R.sa[s], b = R.sb[s];
+        const T d[3] = {m.xs[a] - m.xs[b], m.xs[N + a] - m.xs[N + b], m.xs[2 * N + a] - m.xs[2 * N + b]};
+        const T wv[3] = {m.vs[a] - m.vs[b], m.vs[N + a] - m.vs[N + b], m.vs[2 * N + a] - m.vs[2 * N + b]};
+        T J[3];
+        spring_impulse<T, true>(d, wv, R.L0[s], R.k[s], R.coef[s], m.outb[R.sact[s]], A.P, J);
+        m.J[s] = J[0]; m.J[S + s] = J[1]; m.J[2 * S + s] = J[2];
+    }
+    wave_sync();
+    for (int i = lane; i < N; i += 64) {
+        T vi[3] = {0, 0, 0};
+#pragma unroll 4
+        for (int e = R.vptr[i]; e < R.vptr[i + 1]; ++e) {
+            const int ent = R.vidx[e], s = ent >> 1;
+            const T sg = (ent & 1) ? T(1) : T(-1);
+            vi[0] += sg * m.J[s]; vi[1] += sg * m.J[S + s]; vi[2] += sg * m.J[2 * S + s];
+        }
+        const T x[3] = {m.xs[i], m.xs[N + i], m.xs[2 * N + i]};
+        const T v[3] = {m.vs[i], m.vs[N + i], m.vs[2 * N + i]};
+        T xn[3], vn[3];
+        integrate<T>(x, v, vi, A.P, xn, vn);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            m.xs[d * N + i] = xn[d];
+            m.vs[d * N + i] = vn[d];
+        }
+    }
+    wave_sync();
+}
+
+template <typename T, int IMAX, int OMAX>
+__global__ void __launch_bounds__(256) k_wave_fwd(RArgs<T> A, int n_robots) {
+    extern __shared__ unsigned char smem_raw[];
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int r0 = blockIdx.x * (blockDim.x >> 6);
+    const Rob<T> R0 = make_rob(A, r0);
+    const WShared<T> sh = wave_stage(A, R0, smem_raw, false);
+    __syncthreads();
+    const int r = r0 + wid;
+    if (r >= n_robots) return;
+    Rob<T> R = make_rob(A, r);
+    WSm<T> m = wave_carve(A, reinterpret_cast<T *>(smem_raw), wid, false);
+    if (A.robots[r].spring0 != A.robots[r0].spring0 || A.robots[r].weights != A.robots[r0].weights) {
+        // robots of one workgroup must share a design (the host packs them so)
+        if (lane == 0 && A.loss) A.loss[r] = T(NAN);
+        return;
+    }
+    wave_point_at_shared(R, sh, false);
+    for (int q = lane; q < (A.max_i > 192 ? A.max_i : 192) + 4; q += 64) m.inb[q] = T(0);
+    const int N = R.N;
+    const int t0 = A.robots[r].t_fwd;
+    int t1 = R.Th;
+    if (A.max_steps > 0 && t0 + A.max_steps < t1) t1 = t0 + A.max_steps;
+    if (t0 >= t1) return;
+    WRegs<T, IMAX, OMAX> w;
+    wave_load_weights(R, A.H, lane, w);
+    for (int i = lane; i < 3 * N; i += 64) {
+        m.xs[i] = R.state[i];
+        m.vs[i] = R.state[3 * N + i];
+    }
+    wave_sync();
+    T lsum = T(0);
+    for (int t = t0; t < t1; ++t)
+        wave_fwd_step<T, IMAX, OMAX, true>(A, R, m, w, sh.sin_tab, lane, t, MODE_FWD, 0, lsum);
+    if (t1 == R.Th) {
+        T s6[6] = {0, 0, 0, 0, 0, 0};
+        const T invN = T(1) / T(N);
+        for (int i = lane; i < N; i += 64)
+            for (int d = 0; d < 3; ++d) {
+                s6[d] += m.xs[d * N + i] * invN;
+                s6[3 + d] += m.vs[d * N + i] * invN;
+            }
+        for (int q = 0; q < 6; ++q) s6[q] = wave_sum(s6[q]);
+        if (A.loss_kind == 1) lsum += task_term(s6, tgt_at(R, R.Th));
+        if (A.store_traj && R.traj) {
+            T *row = R.traj + (size_t)R.Th * (3 * N + 3);
+            for (int i = lane; i < 3 * N; i += 64) row[i] = m.xs[i];
+        }
+    }
+    for (int i = lane; i < 3 * N; i += 64) {
+        R.state[i] = m.xs[i];
+        R.state[3 * N + i] = m.vs[i];
+    }
+    if (A.loss_kind == 1 && lane == 0) A.loss[r] += lsum / T(A.window);
+}
+
+// Per-lane copy of one step's recomputed intermediates, loaded one step ahead
+// of its use so the global-memory latency overlaps the previous adjoint step.
+// Holds 6N <= 6*64 state values (the wave kernel is limited to N <= 64).
+// Each lane reads back exactly the elements it stored during the recompute.
+template <typename T>
+struct Pref {
+    T x[3], v[3];
+    T out, dout, dh, cv;
+};
+
+template <typename T>
+__device__ __forceinline__ void pref_load(const Rob<T> &R, int H, int lane, int slot, Pref<T> &p) {
+    const int N = R.N, O = R.O;
+    const T *sx = R.sx + (size_t)slot * 6 * N;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int i = lane + 64 * q;
+        p.x[q] = i < 3 * N ? sx[i] : T(0);
+        p.v[q] = i < 3 * N ? sx[3 * N + i] : T(0);
+    }
+    p.out = lane < O ? R.sout[(size_t)slot * O + lane] : T(0);
+    p.dout = lane < O ? R.sdout[(size_t)slot * O + lane] : T(0);
+    p.dh = lane < H ? R.sdhid[(size_t)slot * H + lane] : T(0);
+    p.cv = lane < 6 ? R.scv[slot * 6 + lane] : T(0);
+}
+
+template <typename T, int IMAX, int OMAX>
+__device__ __forceinline__ void wave_bwd_step(const RArgs<T> &A, const Rob<T> &R, WSm<T> &m, int r, int lane,
+                              int t, int slot, Pref<T> &pf, const WRegs<T, 1, OMAX> &w, int next_slot) {
+    const int N = R.N, S = R.S, I = R.I, O = R.O, H = A.H;
+    // consume the prefetched step
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int i = lane + 64 * q;
+        if (i < 3 * N) {
+            m.xs[i] = pf.x[q];
+            m.vs[i] = pf.v[q];
+        }
+    }
+    if (lane < O) {
+        m.outb[lane] = pf.out;
+        m.dout[lane] = pf.dout;
+    }
+    T cvr[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) cvr[q] = __shfl(pf.cv, q, 64);
+    const T dhj = pf.dh;
+    if (next_slot >= 0) pref_load(R, H, lane, next_slot, pf);   // in flight during this step
+    wave_sync();
+#pragma unroll 2
+    for (int s = lane; s < S; s += 64) {
+        const int a = R.sa[s], b = R.sb[s];
+        const T d[3] = {m.xs[a] - m.xs[b], m.xs[N + a] - m.xs[N + b], m.xs[2 * N + a] - m.xs[2 * N + b]};
+        const T wv[3] = {m.vs[a] - m.vs[b], m.vs[N + a] - m.vs[N + b], m.vs[2 * N + a] - m.vs[2 * N + b]};
+        T J[3];
+        spring_impulse<T, true>(d, wv, R.L0[s], R.k[s], R.coef[s], m.outb[R.sact[s]], A.P, J);
+        m.J[s] = J[0]; m.J[S + s] = J[1]; m.J[2 * S + s] = J[2];
+    }
+    wave_sync();
+    for (int i = lane; i < N; i += 64) {
+        T vi[3] = {0, 0, 0};
+        for (int e = R.vptr[i]; e < R.vptr[i + 1]; ++e) {
+            const int ent = R.vidx[e], s = ent >> 1;
+            const T sg = (ent & 1) ? T(1) : T(-1);
+            vi[0] += sg * m.J[s]; vi[1] += sg * m.J[S + s]; vi[2] += sg * m.J[2 * S + s];
+        }
+        const T x[3] = {m.xs[i], m.xs[N + i], m.xs[2 * N + i]};
+        const T v[3] = {m.vs[i], m.vs[N + i], m.vs[2 * N + i]};
+        const T xnb[3] = {m.xb[i], m.xb[N + i], m.xb[2 * N + i]};
+        const T vnb[3] = {m.vb[i], m.vb[N + i], m.vb[2 * N + i]};
+        T xb[3], vb[3], vib[3];
+        integrate_adj<T>(x, v, vi, A.P, xnb, vnb, xb, vb, vib);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            m.xb[d * N + i] = xb[d];
+            m.vb[d * N + i] = vb[d];
+            m.vib[d * N + i] = vib[d];
+        }
+    }
+    wave_sync();
+    T *G = m.J;
+#pragma unroll 2
+    for (int s = lane; s < S; s += 64) {
+        const int a = R.sa[s], b = R.sb[s];
+        const T d[3] = {m.xs[a] - m.xs[b], m.xs[N + a] - m.xs[N + b], m.xs[2 * N + a] - m.xs[2 * N + b]};
+        const T wv[3] = {m.vs[a] - m.vs[b], m.vs[N + a] - m.vs[N + b], m.vs[2 * N + a] - m.vs[2 * N + b]};
+        const T Jb[3] = {m.vib[b] - m.vib[a], m.vib[N + b] - m.vib[N + a], m.vib[2 * N + b] - m.vib[2 * N + a]};
+        T gd[3], gw[3], ga;
+        spring_impulse_adj<T, true>(d, wv, R.L0[s], R.k[s], R.coef[s], m.outb[R.sact[s]], A.P, Jb, gd, gw, ga);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            G[c * S + s] = gd[c];
+            G[(3 + c) * S + s] = gw[c];
+        }
+        G[6 * S + s] = ga;
+    }
+    wave_sync();
+    for (int i = lane; i < N; i += 64) {
+        T gx[3] = {0, 0, 0}, gv[3] = {0, 0, 0};
+        for (int e = R.vptr[i]; e < R.vptr[i + 1]; ++e) {
+            const int ent = R.vidx[e], s = ent >> 1;
+            const T sg = (ent & 1) ? T(-1) : T(1);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                gx[c] += sg * G[c * S + s];
+                gv[c] += sg * G[(3 + c) * S + s];
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            m.xb[d * N + i] += gx[d];
+            m.vb[d * N + i] += gv[d];
+        }
+    }
+    for (int o = lane; o < O; o += 64) {
+        T ab = T(0);
+        for (int e = R.aptr[o]; e < R.aptr[o + 1]; ++e) ab += G[6 * S + R.aidx[e]];
+        const T z = ab * m.dout[o];
+        m.z2b[o] = z;
+        R.sz2[(size_t)slot * O + o] = z;
+    }
+    wave_sync();
+    {
+        T hb = T(0);
+#pragma unroll
+        for (int o = 0; o < OMAX; ++o)
+            if (o < O) hb += w.w2[o] * m.z2b[o];
+        const T z1 = hb * dhj;
+        m.z1b[lane] = z1;
+        if (lane < H) R.sz1[(size_t)slot * H + lane] = z1;
+    }
+    wave_sync();
+    for (int i = lane; i < I; i += 64) {
+        T acc = T(0);
+        if (R.WL) {
+            const T *row = R.WL + i * (HMAX + 1);
+#pragma unroll 16
+            for (int j = 0; j < H; ++j) acc += row[j] * m.z1b[j];
+        } else {
+#pragma unroll 16
+            for (int j = 0; j < H; ++j) acc += R.W1[(size_t)j * I + i] * m.z1b[j];
+        }
+        m.inbar[i] = acc;
+    }
+    wave_sync();
+    const Layout &L = A.L;
+    const int base = L.vert_base(), vs = L.vstride();
+    const T inv = T(1) / T(L.offset_scale);
+    T cb[3] = {0, 0, 0};
+    const T *eg = R.trajg ? R.trajg + (size_t)t * (3 * N + 3) : nullptr;
+    for (int i = lane; i < N; i += 64) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const T g = m.inbar[base + i * vs + d] * inv;
+            m.xb[d * N + i] += g + (eg ? eg[d * N + i] : T(0));
+            cb[d] -= g;
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) cb[d] = wave_sum(cb[d]);
+    T vcb[3] = {0, 0, 0};
+    if (L.vcom)
+        for (int d = 0; d < 3; ++d) vcb[d] = m.inbar[L.n_sin + d];
+    if (eg)
+        for (int d = 0; d < 3; ++d) cb[d] += eg[3 * N + d];
+    if (A.loss_kind == 1 && t > R.Th - A.window) {
+        const T lg = A.loss_grad[r] / T(A.window);
+        const T *tg = tgt_at(R, t);
+        vcb[0] += T(2) * lg * (cvr[3] - tg[0]);
+        vcb[2] += T(2) * lg * (cvr[5] - tg[2]);
+        cb[1] += T(2) * lg * (cvr[1] - tg[3]);
+    }
+    const T invN = T(1) / T(N);
+    for (int i = lane; i < N; i += 64) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            m.xb[d * N + i] += cb[d] * invN;
+            m.vb[d * N + i] += vcb[d] * invN;
+        }
+    }
+    wave_sync();
+}
+
+// Weight-gradient fold of n steps starting at scratch slot `off`:
+// W1g[j][i] += sum_s z1[s][j] in[s][i].  Lanes run over inputs i with 64
+// accumulators (one per hidden unit); z1 rows are staged through LDS (`zl`,
+// the free spring buffer) in chunks so the inner loop reads LDS broadcasts.
+template <typename T>
+__device__ __forceinline__ void wave_seg_wgrad(const RArgs<T> &A, const Rob<T> &R, T *zl, int zcap,
+                                               int lane, int off, int n) {
+    const int I = R.I, O = R.O, H = A.H;
+    const int chunk = zcap / HMAX;
+    for (int i0 = 0; i0 < I; i0 += 64) {
+        const int i = i0 + lane;
+        const bool ok = i < I;
+        T acc[HMAX];
+#pragma unroll
+        for (int j = 0; j < HMAX; ++j) acc[j] = T(0);
+        for (int c0 = 0; c0 < n; c0 += chunk) {
+            const int cn = min(chunk, n - c0);
+            wave_sync();
+            for (int q = lane; q < cn * HMAX; q += 64) {
+                const int ss = q / HMAX, j = q - ss * HMAX;
+                zl[q] = j < H ? R.sz1[(size_t)(off + c0 + ss) * H + j] : T(0);
+            }
+            wave_sync();
+            for (int ss = 0; ss < cn; ++ss) {
+                const T xin = ok ? R.sin[(size_t)(off + c0 + ss) * I + i] : T(0);
+                const T *z = zl + ss * HMAX;
+#pragma unroll
+                for (int j = 0; j < HMAX; ++j) acc[j] += z[j] * xin;
+            }
+        }
+        if (ok) {
+#pragma unroll
+            for (int j = 0; j < HMAX; ++j)
+                if (j < H) R.Wg1[(size_t)j * I + i] += acc[j];
+        }
+    }
+    wave_sync();
+    if (lane < H) {
+        T b = T(0);
+        for (int ss = 0; ss < n; ++ss) b += R.sz1[(size_t)(off + ss) * H + lane];
+        R.bg1[lane] += b;
+        for (int o = 0; o < O; ++o) {
+            T a = T(0);
+            for (int ss = 0; ss < n; ++ss)
+                a += R.sz2[(size_t)(off + ss) * O + o] * R.shid[(size_t)(off + ss) * H + lane];
+            R.Wg2[(size_t)o * H + lane] += a;
+        }
+    }
+    for (int o = lane; o < O; o += 64) {
+        T a = T(0);
+        for (int ss = 0; ss < n; ++ss) a += R.sz2[(size_t)(off + ss) * O + o];
+        R.bg2[o] += a;
+    }
+}
+
+template <typename T, int IMAX, int OMAX>
+__global__ void __launch_bounds__(256) k_wave_bwd(RArgs<T> A, int n_robots) {
+    extern __shared__ unsigned char smem_raw[];
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int r0 = blockIdx.x * (blockDim.x >> 6);
+    const Rob<T> R0 = make_rob(A, r0);
+    const WShared<T> sh = wave_stage(A, R0, smem_raw, true);
+    __syncthreads();
+    const int r = r0 + wid;
+    if (r >= n_robots) return;
+    Rob<T> R = make_rob(A, r);
+    WSm<T> m = wave_carve(A, reinterpret_cast<T *>(smem_raw), wid, true);
+    if (A.robots[r].spring0 != A.robots[r0].spring0 || A.robots[r].weights != A.robots[r0].weights) {
+        if (lane == 0) R.Wg1[0] = T(NAN);   // robots of one workgroup must share a design
+        return;
+    }
+    wave_point_at_shared(R, sh, true);
+    for (int q = lane; q < (A.max_i > 192 ? A.max_i : 192) + 4; q += 64) m.inb[q] = T(0);
+    const int N = R.N, K = A.K;
+    int tb = A.robots[r].t_bwd;
+    if (tb <= 0) return;
+    WRegs<T, 1, OMAX> w;
+    wave_load_weights(R, A.H, lane, w);
+    for (int i = lane; i < 3 * N; i += 64) {
+        m.xb[i] = R.adj[i];
+        m.vb[i] = R.adj[3 * N + i];
+    }
+    wave_sync();
+    if (tb == R.Th) {
+        T s6[6] = {0, 0, 0, 0, 0, 0};
+        const T invN = T(1) / T(N);
+        for (int i = lane; i < N; i += 64)
+            for (int d = 0; d < 3; ++d) {
+                s6[d] += R.state[d * N + i] * invN;
+                s6[3 + d] += R.state[3 * N + d * N + i] * invN;
+            }
+        for (int q = 0; q < 6; ++q) s6[q] = wave_sum(s6[q]);
+        T cb[3] = {0, 0, 0}, vcb[3] = {0, 0, 0};
+        if (A.loss_kind == 1) {
+            const T lg = A.loss_grad[r] / T(A.window);
+            const T *tg = tgt_at(R, R.Th);
+            vcb[0] = T(2) * lg * (s6[3] - tg[0]);
+            vcb[2] = T(2) * lg * (s6[5] - tg[2]);
+            cb[1] = T(2) * lg * (s6[1] - tg[3]);
+        }
+        const T *eg = R.trajg ? R.trajg + (size_t)R.Th * (3 * N + 3) : nullptr;
+        for (int i = lane; i < N; i += 64)
+            for (int d = 0; d < 3; ++d) {
+                m.xb[d * N + i] += cb[d] * invN + (eg ? eg[d * N + i] : T(0));
+                m.vb[d * N + i] += vcb[d] * invN;
+            }
+        wave_sync();
+    }
+    int budget = A.max_steps > 0 ? A.max_steps : 0x7fffffff;
+    T dummy = T(0);
+    while (tb > 0 && budget > 0) {
+        const int seg = (tb - 1) / K, s0 = seg * K;
+        const int lo = max(s0, tb - budget);
+        const T *ck = R.ckpt + (size_t)seg * 6 * N;
+        for (int i = lane; i < 3 * N; i += 64) {
+            m.xs[i] = ck[i];
+            m.vs[i] = ck[3 * N + i];
+        }
+        wave_sync();
+        for (int t = s0; t < tb; ++t)
+            wave_fwd_step<T, 1, OMAX, false>(A, R, m, w, sh.sin_tab, lane, t, MODE_REC, t - s0, dummy);
+        // the recompute's scratch stores complete before any lane reads them back
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+        Pref<T> pf;
+        pref_load(R, A.H, lane, tb - 1 - s0, pf);
+        for (int t = tb - 1; t >= lo; --t)
+            wave_bwd_step<T, IMAX, OMAX>(A, R, m, r, lane, t, t - s0, pf, w, t - 1 >= lo ? t - 1 - s0 : -1);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+        wave_sync();
+        wave_seg_wgrad(A, R, m.J, 7 * A.max_s, lane, lo - s0, tb - lo);
+        wave_sync();
+        budget -= tb - lo;
+        tb = lo;
+    }
+    for (int i = lane; i < 3 * N; i += 64) {
         R.adj[i] = m.xb[i];
         R.adj[3 * N + i] = m.vb[i];
     }
@@ -718,6 +1401,7 @@ static int check_rollout(const dl_rollout *r, bool bwd) {
     DL_CHECK_ARG(r->max_n > 0 && r->max_s >= 0 && r->max_i > 0 && r->max_o > 0,
                  "rollout: bad size maxima");
     DL_CHECK_ARG(r->lds_policy >= 0 && r->lds_policy <= 3, "rollout: bad LDS policy");
+    DL_CHECK_ARG(r->variant >= 0 && r->variant <= 2, "rollout: bad kernel variant");
     DL_CHECK_ARG(r->lay.n_tv >= 0 && r->lay.n_tv <= 3 && r->lay.run_period > 0, "rollout: bad layout");
     DL_CHECK_ARG(r->loss_kind == 0 || (r->loss_kind == 1 && r->loss_window > 0),
                  "rollout: bad loss configuration");
@@ -735,6 +1419,37 @@ static int check_rollout(const dl_rollout *r, bool bwd) {
 static const size_t kLdsLimit = 160 * 1024;
 
 template <typename T>
+static int launch_wave(const dl_rollout *r, RArgs<T> A, bool bwd, hipStream_t st) {
+    if constexpr (sizeof(T) != 4) {
+        DL_CHECK_ARG(false, "rollout: wave kernel is FP32 only");
+    } else {
+        const int waves = r->block / 64;   // robots per workgroup (all of one design)
+        const size_t sh = wave_smem_bytes(r->max_n, r->max_s, r->max_i, r->max_o, bwd,
+                                          r->lay.run_period * r->lay.n_sin, waves);
+        const dim3 grid((r->n_robots + waves - 1) / waves), block(64 * waves);
+#define DL_WAVE_LAUNCH(IM)                                                                      \
+    do {                                                                                        \
+        auto kf = bwd ? (const void *)k_wave_bwd<float, IM, 8> : (const void *)k_wave_fwd<float, IM, 8>; \
+        if (sh > 65536)                                                                         \
+            DL_CHECK_HIP(hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh)); \
+        if (bwd)                                                                                \
+            hipLaunchKernelGGL((k_wave_bwd<float, IM, 8>), grid, block, sh, st, A, r->n_robots); \
+        else                                                                                    \
+            hipLaunchKernelGGL((k_wave_fwd<float, IM, 8>), grid, block, sh, st, A, r->n_robots); \
+    } while (0)
+        if (r->max_i <= 64)
+            DL_WAVE_LAUNCH(64);
+        else if (r->max_i <= 128)
+            DL_WAVE_LAUNCH(128);
+        else
+            DL_WAVE_LAUNCH(192);
+#undef DL_WAVE_LAUNCH
+        DL_CHECK_LAUNCH();
+    }
+    return 0;
+}
+
+template <typename T>
 static int launch_rollout(const dl_rollout *r, bool bwd, hipStream_t st) {
     RArgs<T> A = make_rargs<T>(r);
     // LDS placement: policy 0 = auto (topology when it fits in 64 KiB, W1 too
@@ -750,6 +1465,17 @@ static int launch_rollout(const dl_rollout *r, bool bwd, hipStream_t st) {
         topo = bytes(springs, true, false) <= 64 * 1024;
         wl = topo && r->n_hidden == HMAX && bytes(springs, true, true) <= 80 * 1024;
     }
+    // wave-per-robot kernel: FP32, hidden width 64, inputs <= 192, outputs <= 8
+    const int variant = r->variant;
+    const int n_tab = r->lay.run_period * r->lay.n_sin;
+    const bool wave_ok = sizeof(T) == 4 && r->n_hidden == HMAX && r->max_i <= 192 &&
+                         r->max_o <= 8 && r->max_n <= 64 && 7 * r->max_s >= HMAX && r->lay.n_tv >= 1 &&
+                         (r->block == 64 || r->block == 256) &&
+                         wave_smem_bytes(r->max_n, r->max_s, r->max_i, r->max_o, bwd, n_tab,
+                                         r->block / 64) <= kLdsLimit;
+    DL_CHECK_ARG(variant != 2 || wave_ok,
+                 "rollout: the wave-per-robot kernel needs FP32, H=64, I<=192, O<=8, block 64 or 256");
+    if (variant == 2 || (variant == 0 && wave_ok)) return launch_wave<T>(r, A, bwd, st);
     size_t sh = bytes(springs, topo, wl);
     DL_CHECK_ARG(sh <= kLdsLimit, "rollout: robot too large for LDS (%zu bytes)", sh);
     DL_CHECK_ARG(springs || r->scratch != nullptr,

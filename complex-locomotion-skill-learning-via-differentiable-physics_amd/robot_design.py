@@ -13,6 +13,7 @@ reference drives every spring by its own output (identity map); voxel robots
 with actuator groups map every actuated spring of a cube to the cube's group.
 """
 
+import math
 from dataclasses import dataclass
 
 import numpy as np
@@ -201,3 +202,54 @@ def build_reference(robot_id, stiffness=1e4, actuation=0.25):
     b = VoxelRobot3D(spring_stiffness=stiffness, spring_actuation=actuation)
     objects, springs, faces = b.build_reference(robot_id)
     return make_design(objects, springs, faces=faces)
+
+
+_DIRS = ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1))
+
+
+def random_voxel_cubes(rng, n_cubes, n_groups):
+    """Connected random accretion: start from one cube and repeatedly attach
+    a new cube to a face of a uniformly chosen existing cube (never below
+    y = 0).  Actuator groups are spatial clusters: ``n_groups`` seed cubes,
+    every cube joins its nearest seed (ties to the lower seed index)."""
+    cells = [(0, 0, 0)]
+    have = {(0, 0, 0)}
+    while len(cells) < n_cubes:
+        b = cells[int(rng.integers(len(cells)))]
+        d = _DIRS[int(rng.integers(6))]
+        c = (b[0] + d[0], b[1] + d[1], b[2] + d[2])
+        if c[1] < 0 or c in have:
+            continue
+        have.add(c)
+        cells.append(c)
+    g = min(n_groups, n_cubes)
+    seeds = np.array([cells[i] for i in rng.choice(n_cubes, size=g, replace=False)], np.float64)
+    pos = np.array(cells, np.float64)
+    dist = ((pos[:, None, :] - seeds[None]) ** 2).sum(-1)
+    group = dist.argmin(1)
+    return [(c, True, int(k)) for c, k in zip(cells, group)], g
+
+
+def random_voxel_robot(seed, n_cubes=None, n_groups=None, stiffness=3e4, actuation=0.1):
+    """One random robot: ``n_cubes`` log-uniform in [8, 400] and ``n_groups``
+    uniform in [4, 64] unless given."""
+    rng = np.random.default_rng(seed)
+    if n_cubes is None:
+        n_cubes = int(round(math.exp(rng.uniform(math.log(8), math.log(400)))))
+    if n_groups is None:
+        n_groups = int(rng.integers(4, 65))
+    cubes, g = random_voxel_cubes(rng, n_cubes, n_groups)
+    return build_voxels(cubes, stiffness, actuation, n_act=g)
+
+
+def random_robot_batch(n, seed, horizon_range=(200, 1000), cube_range=(8, 400), group_range=(4, 64)):
+    """``n`` random designs with horizons uniform in ``horizon_range``."""
+    rng = np.random.default_rng(seed)
+    designs, horizons = [], []
+    for _ in range(n):
+        nc = int(round(math.exp(rng.uniform(math.log(cube_range[0]), math.log(cube_range[1])))))
+        ng = int(rng.integers(group_range[0], group_range[1] + 1))
+        cubes, g = random_voxel_cubes(rng, nc, ng)
+        designs.append(build_voxels(cubes, n_act=g))
+        horizons.append(int(rng.integers(horizon_range[0], horizon_range[1] + 1)))
+    return designs, horizons

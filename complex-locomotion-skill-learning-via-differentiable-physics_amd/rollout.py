@@ -60,7 +60,7 @@ class MassSpringBatch:
     def __init__(self, designs, robot_design, horizons, physics, layout, targets, x0,
                  n_hidden=64, act="tanh", ckpt_every=50, dtype=torch.float32, block=256,
                  loss_window=500, store_traj=False, stiffness_scale=0.1, device="cuda",
-                 lds_policy=0):
+                 lds_policy=0, variant=0):
         self.designs = designs
         self.robot_design = np.asarray(robot_design, np.int64)
         R = self.R = len(self.robot_design)
@@ -68,6 +68,7 @@ class MassSpringBatch:
         self.dtype, self.device = dtype, torch.device(device)
         self.H, self.K, self.block = n_hidden, ckpt_every, block
         self.lds_policy = lds_policy
+        self.variant = variant
         self.act = nat.ACT_SIN if act == "sin" else nat.ACT_TANH
         self.phys, self.layout = physics, layout
         self.loss_window = loss_window
@@ -235,6 +236,11 @@ class MassSpringBatch:
         D.store_traj = int(self.store_traj)
         D.max_n, D.max_s, D.max_i, D.max_o = self.max_n, self.max_s, self.max_i, self.max_o
         D.lds_policy = self.lds_policy
+        D.variant = self.variant
+        if self.variant == 2 or (self.variant == 0 and self.wave_eligible()):
+            # one wavefront per robot; 4 per workgroup when each aligned group of
+            # 4 robots shares one design (topology and W1 are staged per workgroup)
+            D.block = 256 if self.packed4() else 64
         D.phys = self.phys
         D.lay = self.layout
         P = nat.ptr
@@ -250,6 +256,17 @@ class MassSpringBatch:
         D.traj = P(self.traj) if self.traj is not None else None
         D.traj_grad = P(self.traj_grad) if self.traj_grad is not None else None
         return D
+
+    def wave_eligible(self):
+        """The wave-per-robot kernel covers FP32, hidden 64, inputs <= 192, outputs <= 8."""
+        return self.dtype == torch.float32 and self.H == 64 and self.max_i <= 192 and self.max_o <= 8
+
+    def packed4(self):
+        rd = self.robot_design
+        n = len(rd) - len(rd) % 4
+        groups_ok = np.all(rd[:n].reshape(-1, 4) == rd[:n].reshape(-1, 4)[:, :1])
+        tail_ok = len(rd) % 4 == 0 or np.all(rd[n:] == rd[n])
+        return bool(groups_ok and tail_ok)
 
     def forward(self, max_steps=0):
         """Advance every robot from its clock by up to max_steps (0: to the end)."""

@@ -224,7 +224,8 @@ typedef struct dl_rollout {
     int32_t store_traj;      /* forward writes x/center trajectory rows */
     int32_t max_n, max_s, max_i, max_o;   /* maxima over the table (LDS sizing) */
     int32_t lds_policy;      /* 0 auto, 1 stage topology, 2 topology + W1, 3 none */
-    int32_t pad_;
+    int32_t variant;         /* 0 auto, 1 workgroup per robot, 2 wavefront per robot (FP32,
+                                H = 64, I <= 192, O <= 8; robots of one design packed by 4) */
     dl_ms_physics phys;
     dl_sensor_layout lay;
     const dl_robot *robots;  /* device [n_robots] */

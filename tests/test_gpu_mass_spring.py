@@ -313,3 +313,147 @@ def test_batched_deterministic():
         outs.append((bt.loss.clone(), bt.grad_out.clone()))
     assert torch.equal(outs[0][0], outs[1][0])
     assert torch.equal(outs[0][1], outs[1][1])
+
+
+@pytest.mark.parametrize("K", [16, 7])
+def test_wave_kernel_matches_oracle(K):
+    """Wave-per-robot kernel (FP32, W1 column in VGPRs) against the FP64 oracle."""
+    _say(f"wave quad K={K}")
+    B, T, W = 6, 90, 40
+    tv, th = task_targets(B, 21)
+    design = dl.build_quadruped()
+    I = om.TaskConfig().layout.n_input(design.n_vert)
+    w64 = om.init_weights(I, 64, 4, seed=22)
+    qc = dl.quadruped_config(batch_size=B, steps=T, window=W).get_config()
+    x0 = design.objects + np.array([0.4, 0.0, 0.0])
+    tg = np.concatenate([tv, th[:, None], np.zeros((B, 1))], 1)
+    batch = MassSpringBatch([design], [0] * B, T, _physics(qc), sensor_layout(qc), tg, [x0] * B,
+                            act="tanh", ckpt_every=K, dtype=torch.float32, loss_window=W, variant=2)
+    wd = [w.to("cuda", torch.float32).requires_grad_() for w in w64]
+    lossv = rollout_loss(batch, [wd])
+    lossv.mean().backward()
+    torch.cuda.synchronize()
+    lo, lb, go = _oracle_task(B, T, W, tv, th, w64)
+    assert abs(lossv.mean().item() - lo) / abs(lo) < 1e-4
+    assert rel_err(lossv.detach().cpu(), lb) < 1e-4
+    for a, b in zip(wd, go):
+        assert rel_err(a.grad.cpu(), b) < 1e-3
+
+
+def _uneven(prec, variant, seed=31):
+    from diffloco import robot_design as rd
+    rng = np.random.default_rng(seed)
+    cubes_list, designs = [], []
+    for n_cubes, n_groups in ((8, 4), (11, 5), (9, 4)):
+        cubes, g = rd.random_voxel_cubes(rng, n_cubes, n_groups)
+        cubes_list.append((cubes, g))
+        designs.append(rd.build_voxels(cubes, n_act=g))
+    horizons = [100, 130, 115]
+    W = 50
+    qc = dl.quadruped_config(batch_size=3, steps=130, window=W).get_config()
+    tv, th = task_targets(3, seed + 1)
+    tg = np.concatenate([tv, th[:, None], np.zeros((3, 1))], 1)
+    x0 = [d.objects + np.array([0.4, 0.0, 0.0]) for d in designs]
+    batch = MassSpringBatch(designs, [0, 1, 2], horizons, _physics(qc), sensor_layout(qc), tg, x0,
+                            act="tanh", ckpt_every=16, dtype=DT[prec], loss_window=W,
+                            variant=variant)
+    return batch, designs, cubes_list, horizons, W, tv, th
+
+
+@pytest.mark.parametrize("prec,variant", [("f64", 1), ("f32", 1), ("f32", 2)])
+def test_uneven_batch_matches_oracle(prec, variant):
+    """Three different random designs (own sizes, MLPs and horizons) in one launch."""
+    _say(f"uneven {prec} variant={variant}")
+    from oracle import builders as ob
+    batch, designs, cubes_list, horizons, W, tv, th = _uneven(prec, variant)
+    lay = om.TaskConfig().layout
+    weights64 = [om.init_weights(lay.n_input(d.n_vert), 64, d.n_act, seed=40 + i)
+                 for i, d in enumerate(designs)]
+    wd = [[w.to("cuda", DT[prec]).requires_grad_() for w in ws] for ws in weights64]
+    lossv = rollout_loss(batch, wd)
+    lossv.sum().backward()
+    torch.cuda.synchronize()
+    ltol, gtol = (1e-10, 1e-8) if prec == "f64" else (1e-4, 1e-3)
+    for r, ((cubes, g), T) in enumerate(zip(cubes_list, horizons)):
+        objects, springs, groups = ob.build_voxels(cubes)
+        rob = om.make_robot(objects, springs, groups)
+        cfg = om.TaskConfig(steps=T, window=W)
+        Wr = [w.clone().requires_grad_() for w in weights64[r]]
+        x0, v0 = om.initial_state(rob, 1)
+        lo, _, _ = om.rollout_task(rob, cfg, Wr, torch.tensor(tv[r:r + 1]), torch.tensor(th[r:r + 1]), x0, v0)
+        lo.backward()
+        assert abs(lossv[r].item() - lo.item()) / abs(lo.item()) < ltol
+        for a, b in zip(wd[r], Wr):
+            assert rel_err(a.grad.cpu(), b.grad) < gtol
+
+
+def test_reference_preset_through_batched_trajectory():
+    """sim_quad semantics through the persistent kernel: T-1 advances,
+    trajectory rows stored, velocity + rotation losses evaluated on them by
+    the loss kernels, their adjoint fed back as external trajectory gradients."""
+    _say("ref preset batched")
+    T, B = 130, 2
+    cfg = om.RefConfig(steps=T, run_period=20, turn_period=60, jump_period=60,
+                       layout=om.SensorLayout(run_period=20))
+    rob = oracle_reference_robot()
+    design = dl.build_reference(100)
+    pool = torch.rand(B * (T // 60 + 1) * 3, dtype=torch.float64, generator=torch.Generator().manual_seed(50))
+    tv, th, tc = om.train_targets(pool.numpy(), T, B, 60, 0.1)
+    W = om.init_weights(838, 64, 272, seed=51)
+    W = [(torch.rand(w.shape, dtype=torch.float64, generator=torch.Generator().manual_seed(52 + i)) * 2 - 1) * 0.05
+         if i in (0, 2) else w for i, w in enumerate(W)]
+    Wr = [w.clone().requires_grad_() for w in W]
+    x0, v0 = om.initial_state(rob, B)
+    lo, _, _ = om.rollout_ref(rob, cfg, Wr, tv, th, tc, x0, v0)
+    lo.backward()
+    # product: per-step targets [T, 5] per robot
+    raw = dl.ConfigSim.from_file("tests/golden/sim_quad.json").get_config()
+    raw["process"]["run_period"] = 20
+    from diffloco import taichi_env as te
+    lay = te.sensor_layout(raw)
+    tg = [np.concatenate([tv[:, k].numpy(), th[:, k, None].numpy(), tc[:, k, None].numpy()], 1)
+          for k in range(B)]
+    xs0 = [design.objects + np.array([0.4, 0.0, 0.0])] * B
+    batch = MassSpringBatch([design], [0] * B, T - 1, _physics(raw), lay, tg, xs0, act="sin",
+                            ckpt_every=16, dtype=torch.float64, loss_window=0, store_traj=True)
+    for i, w in enumerate((W[0], W[1], W[2], W[3])):
+        pass
+    batch.set_weights(0, *[w.cuda() for w in W])
+    batch.reset()
+    batch.forward()
+    # external losses on the stored trajectory, with the loss kernels
+    N = design.n_vert
+    xs = torch.zeros(T, B, 3, N, dtype=torch.float64, device="cuda")
+    cs = torch.zeros(T, B, 3, dtype=torch.float64, device="cuda")
+    for k in range(B):
+        xk, ck = batch.trajectory(k)
+        xs[:, k] = xk.transpose(1, 2)
+        cs[:T - 1, k] = ck[:T - 1]
+    tvd, thd = tv.cuda().contiguous(), th.cuda().contiguous()
+    loss = torch.zeros(2, dtype=torch.float64, device="cuda")
+    lb = torch.zeros(B, dtype=torch.float64, device="cuda")
+    work = torch.zeros(B, dtype=torch.float64, device="cuda")
+    nat.call("dl_loss_velocity", nat.F64, B, T, T - 1, T, 20, 60, nat.ptr(cs), nat.ptr(tvd),
+             nat.ptr(loss[0:1]), nat.ptr(lb), nat.stream())
+    nat.call("dl_loss_rotation", nat.F64, B, N, T, T - 1, T, 20, 60, nat.ptr(xs), nat.ptr(cs),
+             nat.ptr(thd), nat.ptr(loss[1:2]), nat.ptr(lb), nat.ptr(work), nat.stream())
+    xg = torch.zeros_like(xs)
+    cg = torch.zeros_like(cs)
+    nat.call("dl_loss_velocity_grad", nat.F64, B, T, T - 1, T, 20, 60, nat.ptr(cs), nat.ptr(tvd),
+             1.0, nat.ptr(cg), nat.stream())
+    nat.call("dl_loss_rotation_grad", nat.F64, B, N, T, T - 1, T, 20, 60, nat.ptr(xs), nat.ptr(cs),
+             nat.ptr(thd), 1.0, nat.ptr(xg), nat.ptr(cg), nat.stream())
+    torch.cuda.synchronize()
+    assert abs(loss.sum().item() - lo.item()) / abs(lo.item()) < 1e-10
+    tgrad = torch.zeros_like(batch.traj)
+    for k in range(B):
+        o = int(batch.table[k]["traj"])
+        rows = tgrad[o:o + T * (3 * N + 3)].view(T, 3 * N + 3)
+        rows[:, :3 * N] = xg[:, k].reshape(T, 3 * N)
+        rows[:, 3 * N:] = cg[:, k]
+    batch.traj_grad = tgrad
+    batch.backward(loss_grad=np.zeros(B))
+    torch.cuda.synchronize()
+    G = batch.weight_grads(0)
+    for a, b in zip(G, Wr):
+        assert rel_err(a.cpu(), b.grad) < 1e-8
