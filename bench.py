@@ -58,17 +58,34 @@ def main():
     ap.add_argument("--cpu-horizon", type=int, default=1000)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--skip-mpm", action="store_true")
+    ap.add_argument("--mpm-robots", type=int, default=8)
+    ap.add_argument("--mpm-horizon", type=int, default=2000)
+    ap.add_argument("--mpm-ckpt", type=int, default=100)
+    ap.add_argument("--mpm-steps", type=int, default=1)
+    ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-sized device buffer 4 times (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not supported "
               f"(asked for --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibrate:
+        import torch
+        n = args.copy_calibrate << 18          # float32 elements in MIB mebibytes
+        src = torch.ones(n, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(4):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": n * 4, "copies": 4}))
+        return
 
     import numpy as np
     import torch
     from __graft_entry__ import load_package
     dl = load_package()
-    from diffloco.rollout import MassSpringBatch
+    from diffloco.rollout import MassSpringBatch, init_weights
     from diffloco.solver_mass_spring import _physics
     from diffloco.taichi_env import sensor_layout
 
@@ -87,7 +104,6 @@ def main():
     x0 = design.objects + np.array([0.4, 0.0, 0.0])
     batch = MassSpringBatch([design], [0] * R, T, _physics(qc), lay, tg, [x0] * R, act="tanh",
                             ckpt_every=K, dtype=torch.float32, block=args.block, loss_window=W)
-    from diffloco.rollout import init_weights
     w = init_weights(I, H, O, seed=0)
     batch.set_weights(0, *w)
     lg = np.full(R, 1.0 / R)
@@ -123,6 +139,8 @@ def main():
     fwd_avg_ms = float(np.mean(fwd_ms))
     achieved_bwd = b_b * robot_steps / (bwd_avg_ms * 1e-3) / 1e9
     achieved_fwd = f_b * robot_steps / (fwd_avg_ms * 1e-3) / 1e9
+    wave = batch.variant == 2 or (batch.variant == 0 and batch.wave_eligible())
+    kname = "k_wave" if wave else "k_rollout"
     loss = float(batch.loss.double().mean().item())
     gnorm = float(batch.grad_out.double().norm().item())
 
@@ -144,6 +162,49 @@ def main():
         plain = {"value": pv, "unit": "robot-steps/s", "sample": f"{args.plain_robots} robots x {T} steps, "
                  "per-step kernels (center, sensors, MLP, advance) + tape adjoints, B=1 per launch",
                  "ms_per_robot_rollout": pw * 1e3 / args.plain_robots}
+
+    # ---- MLS-MPM: 8 soft quadrupeds, each on its own 64^3 grid
+    mpm = None
+    if not args.skip_mpm:
+        from diffloco import mpm as dm
+        Bm, Tm, Km = args.mpm_robots, args.mpm_horizon, args.mpm_ckpt
+        pos, ids, n_act = dm.quadruped3d_particles(cells_per_cube=6, n_grid=64)
+        mb = dm.MPMBatch(dm.quadruped3d_params(64), pos, ids, n_act, Bm, Tm, K=Km,
+                         window=Tm // 2, dtype=torch.float32)
+        rngm = np.random.default_rng(7)
+        mb.set_targets(np.stack([rngm.uniform(-0.08, 0.08, Bm), np.zeros(Bm),
+                                 rngm.uniform(-0.08, 0.08, Bm)], 1), rngm.uniform(0.1, 0.25, Bm))
+        mw = init_weights(mb.I, 64, n_act, seed=8)
+        mb.set_weights(*mw)
+        mb.forward()
+        mb.backward()
+        torch.cuda.synchronize()
+        mb.step_.check()
+        active = int((mb.step_.gm > 0).sum().item()) / Bm
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        mf, mbk = [], []
+        for _ in range(args.mpm_steps):
+            e0.record()
+            mb.forward()
+            e1.record()
+            mb.backward()
+            e2.record()
+            torch.cuda.synchronize()
+            mf.append(e0.elapsed_time(e1))
+            mbk.append(e1.elapsed_time(e2))
+        P = len(pos)
+        ms_rollout = float(np.mean(mf) + np.mean(mbk))
+        pps = Bm * P * Tm / (ms_rollout * 1e-3)
+        gp = active / P
+        bpp = 684 + 224 * gp + 96 / Km          # algorithmic bytes per particle-step, fwd+bwd
+        mpm = {"value": pps, "unit": "particle-steps/s", "robots": Bm, "particles": P,
+               "grid": 64, "horizon": Tm, "ckpt_every": Km, "dtype": "f32",
+               "ms_forward": float(np.mean(mf)), "ms_backward": float(np.mean(mbk)),
+               "active_nodes_per_particle": gp,
+               "roofline": {"bound": "hbm", "achieved": bpp * pps / 1e9, "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": bpp * pps / 1e9 / HBM_PEAK_GBS,
+                            "algorithmic_bytes_per_particle_step": bpp, "traffic": None},
+               "mean_loss": float(mb.loss.item())}
 
     # ---- CPU baseline: FP64 oracle forward+backward on a bounded sample
     cpu = None
@@ -175,14 +236,15 @@ def main():
                    "form": "batched persistent (one launch forward, one launch adjoint)"},
         "roofline": {"bound": "hbm", "achieved": achieved_bwd, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_bwd / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_rollout_bwd", "algorithmic_bytes_per_robot_step": b_b,
+                     "kernel": kname + "_bwd", "algorithmic_bytes_per_robot_step": b_b,
                      "avg_launch_ms": bwd_avg_ms,
-                     "forward": {"kernel": "k_rollout_fwd", "achieved": achieved_fwd,
+                     "forward": {"kernel": kname + "_fwd", "achieved": achieved_fwd,
                                  "avg_launch_ms": fwd_avg_ms,
                                  "algorithmic_bytes_per_robot_step": f_b}},
         "plain": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "cpu_baseline": cpu,
+        "mpm": mpm,
         "check": {"mean_loss": loss, "grad_norm": gnorm},
     }
     print(json.dumps(out))

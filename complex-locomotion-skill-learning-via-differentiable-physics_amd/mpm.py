@@ -220,3 +220,157 @@ class SolverMPM:
         self.v_grad[s] += vb
         self.C_grad[s] += Cb
         self.F_grad[s] += Fb
+
+
+class MPMBatch:
+    """Batched, checkpointed MLS-MPM training rollout.
+
+    B robots of one particle design, each on its own grid, driven by the
+    sensor -> MLP -> actuation loop of the quadruped workload (sinusoids,
+    COM velocity, per-group mean offsets from the COM, targets), trained on
+    the windowed COM loss.  ``forward`` advances all robots ``T`` steps,
+    checkpointing particle state every ``K`` steps; ``backward`` recomputes
+    each segment from its checkpoint and runs the hand-written adjoints
+    (P2G/grid/G2P, MLP, sensors, COM) in reverse.  Weight gradients land in
+    ``grads`` (W1, b1, W2, b2)."""
+
+    def __init__(self, params, positions, act_id, n_act, B, T, K=100, window=None,
+                 dtype=torch.float32, n_sin=10, run_period=100, offset_scale=0.2,
+                 max_speed=0.08, activation="tanh", n_hidden=64, device="cuda"):
+        self.p, self.B, self.T, self.K = params, B, T, K
+        self.D = D = params.dim
+        self.P = P = len(positions)
+        self.n_act = n_act
+        self.window = window if window is not None else T // 2
+        self.dtype, self.device = dtype, torch.device(device)
+        self.dc = nat.dtype_code(dtype)
+        self.n_sin, self.run_period = n_sin, run_period
+        self.offset_scale, self.max_speed = offset_scale, max_speed
+        self.act = nat.ACT_SIN if activation == "sin" else nat.ACT_TANH
+        self.G = n_act + 1                                  # actuator groups + passive body
+        self.I = n_sin + 3 + 3 * self.G + 4
+        self.H, self.O = n_hidden, n_act
+        ids = np.asarray(act_id, np.int32)
+        grp = np.where(ids >= 0, ids, n_act).astype(np.int32)
+        counts = np.bincount(grp, minlength=self.G).astype(np.float64)
+        z = lambda *s: torch.zeros(*s, dtype=dtype, device=self.device)
+        self.group = torch.as_tensor(grp)[None].expand(B, -1).contiguous().to(self.device)
+        self.inv_count = torch.as_tensor(np.where(counts > 0, 1.0 / np.maximum(counts, 1), 0.0),
+                                         dtype=dtype)[None].expand(B, -1).contiguous().to(self.device)
+        self.step_ = MPMStep(params, B, P, n_act, ids, dtype, device)
+        self.x0 = _soa(positions, B, dtype, self.device)
+        nck = (T + K - 1) // K
+        self.ck = [z(nck, B, D, P), z(nck, B, D, P), z(nck, B, D * D, P), z(nck, B, D * D, P)]
+        self.seg = [z(K + 1, B, D, P), z(K + 1, B, D, P), z(K + 1, B, D * D, P), z(K + 1, B, D * D, P)]
+        self.inp = z(K, B, self.I)
+        self.act_t = z(K, B, self.O)
+        self.hid = z(K, B, self.H)
+        self.center, self.vcom = z(T + 1, B, 3), z(T + 1, B, 3)
+        self.targets = z(B, 5)
+        self.tv, self.th = z(B, 3), z(B)
+        self.loss = z(1)
+        self.loss_b = z(B)
+        self.adj = [[z(B, D, P), z(B, D, P), z(B, D * D, P), z(B, D * D, P)] for _ in range(2)]
+        self.act_b, self.in_b = z(B, self.O), z(B, self.I)
+        self.cg, self.vg = z(T + 1, B, 3), z(T + 1, B, 3)
+        self.W = [z(self.H, self.I), z(self.H), z(self.O, self.H), z(self.O)]
+        self.grads = [torch.zeros_like(w) for w in self.W]
+        self.work = z(B * (2 * self.H + self.O))
+        self.cur = [z(B, D, P), z(B, D, P), z(B, D * D, P), z(B, D * D, P)]
+        self.nxt = [torch.zeros_like(t) for t in self.cur]
+
+    def set_targets(self, tv, th):
+        t = np.zeros((self.B, 5))
+        t[:, :3] = np.asarray(tv)
+        t[:, 3] = np.asarray(th)
+        self.targets.copy_(torch.as_tensor(t, dtype=self.dtype))
+        # contiguous copies held for the loss kernels (a temporary would be
+        # freed, and its memory reused, before the launch reads it)
+        self.tv = self.targets[:, 0:3].contiguous()
+        self.th = self.targets[:, 3].contiguous()
+
+    def set_weights(self, W1, b1, W2, b2):
+        for d, s in zip(self.W, (W1, b1, W2, b2)):
+            d.copy_(torch.as_tensor(s, dtype=self.dtype))
+
+    def _init_state(self, st):
+        D = self.D
+        st[0].copy_(self.x0)
+        st[1].zero_()
+        st[2].zero_()
+        st[3].zero_()
+        for i in range(D):
+            st[3][:, i * D + i] = 1.0
+
+    def _controller(self, t, x, v, inp, act, hid):
+        nat.call("dl_mpm_sensor", self.dc, self.B, self.P, self.G, self.n_sin, self.run_period, t,
+                 self.offset_scale, self.max_speed, nat.ptr(x), nat.ptr(v), nat.ptr(self.group),
+                 nat.ptr(self.targets), nat.ptr(self.inv_count), nat.ptr(inp),
+                 nat.ptr(self.center[t]), nat.ptr(self.vcom[t]), nat.stream())
+        nat.call("dl_mlp_forward", self.dc, self.act, self.B, self.I, self.H, self.O, nat.ptr(inp),
+                 *map(nat.ptr, self.W), nat.ptr(hid), nat.ptr(act), nat.stream())
+
+    def forward(self):
+        st, nx = self.cur, self.nxt
+        self._init_state(st)
+        self.step_.err.zero_()
+        for t in range(self.T):
+            k = t % self.K
+            if k == 0:
+                for c, s in zip(self.ck, st):
+                    c[t // self.K].copy_(s)
+            self._controller(t, st[0], st[1], self.inp[k], self.act_t[k], self.hid[k])
+            self.step_.forward(*st, self.act_t[k], *nx)
+            st, nx = nx, st
+        # COM of the final state (loss term at T)
+        nat.call("dl_ms_center", self.dc, self.B, self.P, nat.ptr(st[0]), nat.ptr(st[1]),
+                 nat.ptr(self.center[self.T]), nat.ptr(self.vcom[self.T]), nat.stream())
+        self.final = st
+        nat.call("dl_loss_task", self.dc, self.B, self.T, self.window, nat.ptr(self.center),
+                 nat.ptr(self.vcom), nat.ptr(self.tv), nat.ptr(self.th), nat.ptr(self.loss),
+                 nat.ptr(self.loss_b), nat.stream())
+        return self.loss
+
+    def backward(self, loss_grad=1.0):
+        B, P, T, K = self.B, self.P, self.T, self.K
+        for g in self.grads:
+            g.zero_()
+        self.cg.zero_()
+        self.vg.zero_()
+        nat.call("dl_loss_task_grad", self.dc, B, T, self.window, nat.ptr(self.center),
+                 nat.ptr(self.vcom), nat.ptr(self.tv), nat.ptr(self.th), float(loss_grad), nat.ptr(self.cg),
+                 nat.ptr(self.vg), nat.stream())
+        a1, a0 = self.adj
+        for t_ in a1:
+            t_.zero_()
+        nat.call("dl_ms_center_grad", self.dc, B, P, nat.ptr(self.cg[T]), nat.ptr(self.vg[T]),
+                 nat.ptr(a1[0]), nat.ptr(a1[1]), nat.stream())
+        nseg = (T + K - 1) // K
+        for seg in reversed(range(nseg)):
+            s0, s1 = seg * K, min(T, (seg + 1) * K)
+            for d, c in zip(self.seg, self.ck):
+                d[0].copy_(c[seg])
+            for t in range(s0, s1):
+                k = t - s0
+                st = [b[k] for b in self.seg]
+                nx = [b[k + 1] for b in self.seg]
+                self._controller(t, st[0], st[1], self.inp[k], self.act_t[k], self.hid[k])
+                self.step_.forward(*st, self.act_t[k], *nx)
+            for t in reversed(range(s0, s1)):
+                k = t - s0
+                st = [b[k] for b in self.seg]
+                self.act_b.zero_()
+                self.step_.backward(*st, self.act_t[k], *a1, *a0, self.act_b)
+                self.in_b.zero_()
+                nat.call("dl_mlp_backward", self.dc, self.act, B, self.I, self.H, self.O,
+                         nat.ptr(self.inp[k]), *map(nat.ptr, self.W), nat.ptr(self.act_b),
+                         nat.ptr(self.in_b), *map(nat.ptr, self.grads), nat.ptr(self.work),
+                         nat.stream())
+                nat.call("dl_mpm_sensor_adj", self.dc, B, P, self.G, self.n_sin, self.offset_scale,
+                         nat.ptr(self.group), nat.ptr(self.inv_count), nat.ptr(self.in_b),
+                         nat.ptr(a0[0]), nat.ptr(a0[1]), nat.stream())
+                nat.call("dl_ms_center_grad", self.dc, B, P, nat.ptr(self.cg[t]), nat.ptr(self.vg[t]),
+                         nat.ptr(a0[0]), nat.ptr(a0[1]), nat.stream())
+                a1, a0 = a0, a1
+        self.adj = [a1, a0]
+        return self.grads

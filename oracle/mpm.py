@@ -13,10 +13,11 @@ same rules:
 * ``g2p``      -- ``SolverMPM.g2p`` (``:146-165``).
 * ``center``   -- ``SolverMPM.compute_center`` (``:167-173``).
 
-The polar factor comes from ``torch.linalg.svd`` (R = U V^T with the sign of
-the smallest singular direction fixed so det R = +1): an independent route
-from the kernels' closed form (2D) and Newton iteration (3D).  Gradients
-come from ``torch.autograd`` over this forward.
+The polar factor comes from an unscaled Newton iteration differentiated by
+autograd (finite at F = I, where the SVD's backward is not), pinned against
+``torch.linalg.svd``; the kernels use a closed form (2D) and a scaled Newton
+iteration with a closed-form adjoint (3D).  Gradients come from
+``torch.autograd`` over this forward.
 
 Pinned by tests/test_oracle_mpm.py: B-spline partition of unity, mass
 conservation, exact transfer of an affine velocity field, zero stress at
@@ -82,12 +83,24 @@ def _node_index(base, off, n, dim):
     return flat
 
 
-def polar(Fm):
+def polar_svd(Fm):
+    """Rotation factor from the SVD, R = U V^T with det R = +1."""
     U, S, Vh = torch.linalg.svd(Fm)
     det = torch.det(U @ Vh)
     fix = torch.ones_like(S)
     fix[..., -1] = torch.sign(det)
     return (U * fix[..., None, :]) @ Vh
+
+
+def polar(Fm, iters=25):
+    """Rotation factor by the Newton iteration X <- (X + X^{-T}) / 2, which
+    autograd differentiates through; unlike the SVD's backward it stays
+    finite at repeated singular values (F = I at the start of a rollout).
+    Its value is pinned against ``polar_svd``."""
+    X = Fm
+    for _ in range(iters):
+        X = 0.5 * (X + torch.linalg.inv(X).transpose(-1, -2))
+    return X
 
 
 def p2g(x, v, C, Fm, act, act_id, p):
@@ -153,3 +166,45 @@ def step(x, v, C, Fm, act, act_id, p):
 
 def center(x):
     return x.mean(1)
+
+
+def sensors(t, x, v, group, n_groups, n_sin, run_period, offset_scale, max_speed, tv, th):
+    """Sensor vector of the MPM quadruped: [sinusoids | COM velocity | per
+    group mean offset from the COM / scale | tv / max_speed | th]."""
+    import math
+    B, P, D = x.shape
+    c = x.mean(1)
+    vc = v.mean(1)
+    j = torch.arange(n_sin, dtype=F64)
+    s = torch.sin(2 * math.pi / run_period * t + 2 * math.pi / n_sin * j).expand(B, -1)
+    offs = []
+    for g in range(n_groups):
+        m = group == g
+        gm = x[:, m].mean(1) if m.any() else torch.zeros(B, D, dtype=F64)
+        offs.append((gm - c) / offset_scale)
+    return torch.cat([s, vc, torch.cat(offs, -1), tv / max_speed, th[:, None]], -1), c, vc
+
+
+def rollout_task(x0, act_id, n_act, p, weights, tv, th, T, window, n_sin=10, run_period=100,
+                 offset_scale=0.2, max_speed=0.08):
+    """T MPM steps under the MLP controller; loss = mean over robots and over
+    states t in (T - window, T] of |v_com,xz - tv_xz|^2 + (c_y - th)^2."""
+    W1, b1, W2, b2 = weights
+    B, P, D = x0.shape
+    G = n_act + 1
+    group = torch.where(act_id[0] >= 0, act_id[0], torch.tensor(n_act))
+    aid = act_id
+    x, v = x0, torch.zeros_like(x0)
+    C = torch.zeros(B, P, D, D, dtype=F64)
+    Fm = torch.eye(D, dtype=F64).expand(B, P, D, D).clone()
+    total = torch.zeros(B, dtype=F64)
+    for t in range(T + 1):
+        inp, c, vc = sensors(t, x, v, group, G, n_sin, run_period, offset_scale, max_speed, tv, th)
+        if t > T - window:
+            total = total + (vc[:, 0] - tv[:, 0]) ** 2 + (vc[:, 2] - tv[:, 2]) ** 2 + (c[:, 1] - th) ** 2
+        if t == T:
+            break
+        h = torch.tanh(inp @ W1.T + b1)
+        act = torch.tanh(h @ W2.T + b2)
+        x, v, C, Fm = step(x, v, C, Fm, act, aid, p)
+    return (total / window).mean()

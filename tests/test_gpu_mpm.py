@@ -120,3 +120,44 @@ def test_mpm_adjoint_matches_autograd(dim):
     assert rel_err(_mback(Cb, dim), Cr.grad) < 1e-9
     assert rel_err(_mback(Fb, dim), Fr.grad) < 1e-8
     assert rel_err(ab.cpu(), ar.grad) < 1e-8
+
+
+def test_mpm_batched_rollout_matches_oracle():
+    """Checkpointed MPM training rollout (sensors, MLP, P2G/grid/G2P, windowed
+    COM loss) and its adjoint through every segment, FP64, against autograd."""
+    _say("mpm batched rollout")
+    n = 16
+    params = dm.quadruped3d_params(n_grid=n)
+    params.dt = 1e-3
+    side = torch.arange(5, dtype=torch.float64)
+    grid = torch.stack(torch.meshgrid(side, side, side, indexing="ij"), -1).reshape(-1, 3)
+    pos = (grid * 0.5 + torch.tensor([5.0, 3.2, 5.0])) / n
+    ids = torch.tensor([(i % 5) - 1 for i in range(len(pos))])
+    B, T, K, W = 2, 24, 7, 12
+    batch = dm.MPMBatch(params, pos.numpy(), ids.numpy(), 4, B, T, K=K, window=W,
+                        dtype=torch.float64)
+    rng = np.random.default_rng(5)
+    tv = np.stack([rng.uniform(-0.08, 0.08, B), np.zeros(B), rng.uniform(-0.08, 0.08, B)], 1)
+    th = rng.uniform(0.1, 0.25, B)
+    batch.set_targets(tv, th)
+    I = batch.I
+    g = torch.Generator().manual_seed(6)
+    Wt = [torch.randn(64, I, generator=g, dtype=torch.float64) / I ** 0.5,
+          0.1 * torch.randn(64, generator=g, dtype=torch.float64),
+          torch.randn(4, 64, generator=g, dtype=torch.float64) / 8.0,
+          0.1 * torch.randn(4, generator=g, dtype=torch.float64)]
+    batch.set_weights(*[w.cuda() for w in Wt])
+    loss = batch.forward()
+    grads = batch.backward()
+    torch.cuda.synchronize()
+    batch.step_.check()
+    op = omp.MPMParams(dim=3, n_grid=n, dt=params.dt, gravity=params.gravity, p_vol=params.p_vol,
+                       p_mass=params.p_mass, mu=params.mu, la=params.la,
+                       act_strength=params.act_strength, act_axis=0, bound=3)
+    Wr = [w.clone().requires_grad_() for w in Wt]
+    x0 = pos[None].expand(B, -1, -1).clone()
+    lo = omp.rollout_task(x0, ids[None].expand(B, -1), 4, op, Wr, torch.tensor(tv), torch.tensor(th), T, W)
+    lo.backward()
+    assert abs(loss.item() - lo.item()) / abs(lo.item()) < 1e-10
+    for a, b in zip(grads, Wr):
+        assert rel_err(a.cpu(), b.grad) < 1e-7

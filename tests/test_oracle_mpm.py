@@ -81,6 +81,7 @@ def test_polar_is_rotation_times_spd():
     Fm = torch.eye(3, dtype=F64) + 0.3 * torch.randn(50, 3, 3, generator=g, dtype=F64)
     Fm = Fm[torch.det(Fm) > 0]
     R = omp.polar(Fm)
+    assert torch.allclose(R, omp.polar_svd(Fm), atol=1e-12)
     assert torch.allclose(R @ R.transpose(-1, -2), torch.eye(3, dtype=F64).expand_as(R), atol=1e-12)
     assert torch.allclose(torch.det(R), torch.ones(R.shape[0], dtype=F64), atol=1e-12)
     S = R.transpose(-1, -2) @ Fm
@@ -136,3 +137,21 @@ def test_step_gradient_matches_central_differences():
         fd = (f(*hi) - f(*lo)).item() / (2 * eps)
         ad = args[ai].grad[idx].item()
         assert abs(fd - ad) <= 1e-5 * max(1.0, abs(ad)), (ai, idx, fd, ad)
+
+
+def test_polar_gradient_finite_differences_at_identity():
+    """d R / d F at F = I (repeated singular values) from the Newton oracle
+    against central differences: R(I + eps E) for skew E rotates, for
+    symmetric E stays I to first order."""
+    g = torch.Generator().manual_seed(4)
+    Fm = torch.eye(3, dtype=F64)[None].clone().requires_grad_()
+    Rb = torch.randn(1, 3, 3, generator=g, dtype=F64)
+    (omp.polar(Fm) * Rb).sum().backward()
+    eps = 1e-6
+    for i in range(3):
+        for j in range(3):
+            E = torch.zeros(1, 3, 3, dtype=F64)
+            E[0, i, j] = eps
+            fd = ((omp.polar_svd(torch.eye(3, dtype=F64)[None] + E) * Rb).sum() -
+                  (omp.polar_svd(torch.eye(3, dtype=F64)[None] - E) * Rb).sum()) / (2 * eps)
+            assert abs(fd.item() - Fm.grad[0, i, j].item()) < 1e-8
