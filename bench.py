@@ -179,8 +179,8 @@ def main():
         mb.forward()
         mb.backward()
         torch.cuda.synchronize()
-        mb.step_.check()
-        active = int((mb.step_.gm > 0).sum().item()) / Bm
+        mb.check()
+        active = mb.active_nodes()
         e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
         mf, mbk = [], []
         for _ in range(args.mpm_steps):
@@ -190,6 +190,7 @@ def main():
             mb.backward()
             e2.record()
             torch.cuda.synchronize()
+            mb.check()
             mf.append(e0.elapsed_time(e1))
             mbk.append(e1.elapsed_time(e2))
         P = len(pos)

@@ -121,6 +121,8 @@ _SIGS = {
     "dl_mpm_p2g_adj": ([_I, _P, _I, _I] + [_P] * 6 + [_I] + [_P] * 9, c_int32),
     "dl_mpm_sensor": ([_I] * 7 + [c_double] * 2 + [_P] * 9, c_int32),
     "dl_mpm_sensor_adj": ([_I] * 5 + [c_double] + [_P] * 6, c_int32),
+    "dl_mpm_forward": ([_P, _P, _I, _I, _I, _P], c_int32),
+    "dl_mpm_backward": ([_P, _P, _P, _I, _I, _P], c_int32),
 }
 
 _lib = None

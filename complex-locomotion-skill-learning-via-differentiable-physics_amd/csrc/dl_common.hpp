@@ -40,6 +40,7 @@ template <> struct M<float> {
     static __device__ __forceinline__ float cos(float x) { return cosf(x); }
     static __device__ __forceinline__ float tanh(float x) { return tanhf(x); }
     static __device__ __forceinline__ float atan2(float y, float x) { return atan2f(y, x); }
+    static __device__ __forceinline__ float rcbrt(float x) { return rcbrtf(x); }
 };
 template <> struct M<double> {
     static __device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
@@ -47,6 +48,7 @@ template <> struct M<double> {
     static __device__ __forceinline__ double cos(double x) { return ::cos(x); }
     static __device__ __forceinline__ double tanh(double x) { return ::tanh(x); }
     static __device__ __forceinline__ double atan2(double y, double x) { return ::atan2(y, x); }
+    static __device__ __forceinline__ double rcbrt(double x) { return ::rcbrt(x); }
 };
 
 // Square root and reciprocal: IEEE by default; with FAST (FP32 fast path) the

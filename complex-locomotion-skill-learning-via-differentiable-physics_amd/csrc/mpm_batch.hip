@@ -1,0 +1,1151 @@
+// Batched MLS-MPM training rollout: B robots, each on its own grid, advanced
+// and differentiated a segment of steps per host call (dl_mpm_forward /
+// dl_mpm_backward), so no per-step work crosses the C ABI.
+//
+// Follows multitask/solver_mpm.py: p2g (:83-124), grid_op (:126-144), g2p
+// (:146-165), compute_center (:167-173) and advance_core(_grad) (:175-190),
+// with the quadruped controller (COM and per-group sensors, MLP) in front.
+//
+// Layouts (per robot k):
+//   particle slots   x, v [S][B][D][P];  C, F [S][B][D*D][P]
+//   grids            [B][n^D][4] records: grid = (momentum (D), mass),
+//                    grid_vo = velocity, grid_vo_grad = its adjoint,
+//                    grid_grad = adjoint of (momentum, mass)
+//   active blocks    flag/list [B][nblk], count [B]; a block is 4^3 nodes
+//                    (3D) or 8^2 (2D) -- one wavefront per block.
+//
+// Grid sparsity: P2G appends every block its particles touch to the robot's
+// active list; the grid passes visit only listed blocks and clear what they
+// consume, so grid memory is zero between steps and no dense clear or dense
+// pass runs.  Invariants between calls: grid, grid_vo_grad, flags, count and
+// act_grad are zero.
+//
+// P2G and the G2P adjoint accumulate a workgroup's particles into an LDS
+// tile covering their stencils' bounding box, then commit each touched node
+// with one global atomic per component (direct global atomics when the box
+// exceeds the tile).  Atomic summation order is the run-to-run variation.
+
+#include "mpm_core.hpp"
+
+#include <algorithm>
+#include <climits>
+
+namespace dl {
+
+constexpr int TILE_NODES = 1024;     // LDS tile capacity, nodes
+constexpr int PB = 256;              // particles per workgroup
+constexpr int NWAVE = PB / 64;
+
+template <int D> struct BlkE { static constexpr int E = D == 3 ? 4 : 8; };
+
+struct Blk {
+    int32_t *flag, *list, *count;
+    int nb, nblk;
+};
+
+template <int D>
+__host__ __device__ __forceinline__ size_t gsize(int n) {
+    size_t g = 1;
+    for (int d = 0; d < D; ++d) g *= (size_t)n;
+    return g;
+}
+
+// ---------------------------------------------------------------- AoS grid io
+template <typename T> __device__ __forceinline__ void ld4(const T *p, T o[4]);
+template <> __device__ __forceinline__ void ld4<float>(const float *p, float o[4]) {
+    const float4 v = *reinterpret_cast<const float4 *>(p);
+    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+}
+template <> __device__ __forceinline__ void ld4<double>(const double *p, double o[4]) {
+    const double2 a = *reinterpret_cast<const double2 *>(p);
+    const double2 b = *reinterpret_cast<const double2 *>(p + 2);
+    o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+}
+template <typename T> __device__ __forceinline__ void st4(T *p, const T o[4]);
+template <> __device__ __forceinline__ void st4<float>(float *p, const float o[4]) {
+    *reinterpret_cast<float4 *>(p) = make_float4(o[0], o[1], o[2], o[3]);
+}
+template <> __device__ __forceinline__ void st4<double>(double *p, const double o[4]) {
+    *reinterpret_cast<double2 *>(p) = make_double2(o[0], o[1]);
+    *reinterpret_cast<double2 *>(p + 2) = make_double2(o[2], o[3]);
+}
+
+template <typename T>
+__device__ __forceinline__ T wsel(int i, T a, T b, T c) { return i == 0 ? a : (i == 1 ? b : c); }
+
+template <typename T>
+__device__ __forceinline__ void gadd(T *p, T v) { unsafeAtomicAdd(p, v); }
+
+// Bounding box of the base cells of the workgroup's in-grid particles.
+// Returns false (uniformly) when the workgroup has none.
+template <int D>
+__device__ __forceinline__ bool block_bbox(bool valid, const int base[D], int lo[D], int ext[D],
+                                           int *red) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        int mn = valid ? base[d] : INT_MAX, mx = valid ? base[d] : INT_MIN;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            mn = min(mn, __shfl_xor(mn, off, 64));
+            mx = max(mx, __shfl_xor(mx, off, 64));
+        }
+        if (lane == 0) {
+            red[(2 * d) * NWAVE + wid] = mn;
+            red[(2 * d + 1) * NWAVE + wid] = mx;
+        }
+    }
+    __syncthreads();
+    bool any = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        int mn = INT_MAX, mx = INT_MIN;
+#pragma unroll
+        for (int w = 0; w < NWAVE; ++w) {
+            mn = min(mn, red[(2 * d) * NWAVE + w]);
+            mx = max(mx, red[(2 * d + 1) * NWAVE + w]);
+        }
+        any = any && mn <= mx;
+        lo[d] = mn;
+        ext[d] = mx - mn + 3;
+    }
+    __syncthreads();
+    return any;
+}
+
+// Append every block overlapped by the node box [lo, lo + ext) to robot k's
+// active list (first marker wins; marking a block with no mass is harmless).
+template <int D>
+__device__ __forceinline__ void mark_box(const Blk &bk, int k, const int lo[D], const int ext[D]) {
+    constexpr int E = BlkE<D>::E;
+    int b0[D], nbt[D], tot = 1;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        b0[d] = lo[d] / E;
+        nbt[d] = (lo[d] + ext[d] - 1) / E - b0[d] + 1;
+        tot *= nbt[d];
+    }
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+        int rem = i, c[D];
+#pragma unroll
+        for (int d = D - 1; d >= 0; --d) {
+            c[d] = b0[d] + rem % nbt[d];
+            rem /= nbt[d];
+        }
+        int bid = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) bid = bid * bk.nb + c[d];
+        int32_t *f = bk.flag + (size_t)k * bk.nblk + bid;
+        if (*reinterpret_cast<volatile int32_t *>(f) == 0 && atomicCAS(f, 0, 1) == 0) {
+            const int slot = atomicAdd(bk.count + k, 1);
+            bk.list[(size_t)k * bk.nblk + slot] = bid;
+        }
+    }
+}
+
+// Node of lane `lane` in block `bid`; sets bnd for sticky boundary nodes.
+template <int D, typename T>
+__device__ __forceinline__ int block_node(int bid, int lane, int nb, const MP<T> &p, bool &bnd) {
+    constexpr int E = BlkE<D>::E;
+    int rem = bid, lr = lane, c[D];
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) {
+        c[d] = (rem % nb) * E + lr % E;
+        rem /= nb;
+        lr /= E;
+    }
+    int node = 0;
+    bnd = false;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        node = node * p.n + c[d];
+        bnd = bnd || c[d] < p.bound || c[d] > p.n - p.bound;
+    }
+    return node;
+}
+
+// Wave-aggregated accumulation of per-particle values into LDS slots keyed
+// by a small integer (sensor group, actuator id): one atomic per distinct
+// key in the wave.  All lanes of the wave must call it.
+template <typename T, int NV>
+__device__ __forceinline__ void keyed_wave_add(bool valid, int key, const T (&v)[NV], T *dst,
+                                               int stride) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long pend = __ballot(valid);
+    while (pend) {
+        const int leader = __ffsll((long long)pend) - 1;
+        const int kl = __shfl(key, leader, 64);
+        const bool mine = valid && key == kl;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            const T s = wave_sum(mine ? v[i] : T(0));
+            if (lane == leader) atomicAdd(dst + kl * stride + i, s);
+        }
+        pend &= ~__ballot(mine);
+    }
+}
+
+// ------------------------------------------------------------ DPP row shifts
+// Lane i reads lane i-N (shr) or i+N (shl) of its 16-lane row; lanes without
+// a source read 0.  VALU only (no LDS traffic).
+template <int N> __device__ __forceinline__ int dpp_shr_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x110 + N, 0xF, 0xF, true);
+}
+template <int N> __device__ __forceinline__ int dpp_shl_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x100 + N, 0xF, 0xF, true);
+}
+template <int N> __device__ __forceinline__ float dpp_shr(float v) {
+    return __int_as_float(dpp_shr_i<N>(__float_as_int(v)));
+}
+template <int N> __device__ __forceinline__ double dpp_shr(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_shr_i<N>((int)(b & 0xffffffffll)), hi = dpp_shr_i<N>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Runs of consecutive lanes (within a 16-lane row) holding the same key --
+// particles of one grid cell, since particles are sorted by cell.  dist =
+// lanes before this one in its run; tail = last lane of its run.
+struct Seg {
+    int dist;
+    bool tail;
+};
+// The lane shifts must run with every lane active (a disabled source lane
+// reads 0), so they are evaluated unconditionally, never under a
+// short-circuit condition.
+__device__ __forceinline__ Seg lane_runs(int key) {
+    const int r = threadIdx.x & 15;
+    const int prev = dpp_shr_i<1>(key), next = dpp_shl_i<1>(key);
+    const bool head = (r == 0) | (prev != key);
+    Seg sg;
+    sg.tail = (r == 15) | (next != key);
+    int s = head ? r : 0;
+    s = max(s, dpp_shr_i<1>(s));
+    s = max(s, dpp_shr_i<2>(s));
+    s = max(s, dpp_shr_i<4>(s));
+    s = max(s, dpp_shr_i<8>(s));
+    sg.dist = r - s;
+    return sg;
+}
+
+// Inclusive segmented sum over a run; the tail lane ends with the run total.
+template <typename T>
+__device__ __forceinline__ T run_sum(T v, int dist) {
+    T t = dpp_shr<1>(v);
+    v += dist >= 1 ? t : T(0);
+    t = dpp_shr<2>(v);
+    v += dist >= 2 ? t : T(0);
+    t = dpp_shr<4>(v);
+    v += dist >= 4 ? t : T(0);
+    t = dpp_shr<8>(v);
+    v += dist >= 8 ? t : T(0);
+    return v;
+}
+
+// Scatter-add NC values per stencil node of the workgroup's particles into
+// robot grid gk ([n^D][4] records).  For each stencil offset the particles
+// of one cell (a run of lanes) are summed in registers first, and only the
+// run's tail lane adds to memory: LDS atomics are paid per lane, so this
+// divides their cost by the particles per cell.  The sums go through the
+// LDS tile: the node box (lo, ext) is cut into x-slabs of at most
+// TILE_NODES nodes, each accumulated in LDS and committed with one global
+// atomic per nonzero component; a box whose single x-plane exceeds the tile
+// is scattered with direct global atomics.  val(off, v) gives the values of
+// stencil offset off.  Every thread of the workgroup must call it
+// (uniform control flow, all lanes active for the lane shifts).
+template <int D, typename T, int NC, typename F>
+__device__ __forceinline__ void tile_scatter(bool valid, const Stencil<D, T> &s, const int lo[D],
+                                             const int ext[D], T *tile, T *gk, int n, F &&val) {
+    constexpr int K = D == 2 ? 9 : 27;
+    int key = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) key = key * n + s.base[d];
+    if (!valid) key = -1 - (int)(threadIdx.x & 63);
+    const Seg sg = lane_runs(key);
+    int plane = 1;
+#pragma unroll
+    for (int d = 1; d < D; ++d) plane *= ext[d];
+    const bool direct = plane > TILE_NODES;
+    const int sx = direct ? ext[0] : min(ext[0], TILE_NODES / plane);
+    for (int x0 = 0; x0 < ext[0]; x0 += sx) {
+        const int nx = min(sx, ext[0] - x0), nodes = nx * plane;
+        if (!direct) {
+            for (int i = threadIdx.x; i < nodes * 4; i += blockDim.x) tile[i] = T(0);
+            __syncthreads();
+        }
+#pragma unroll
+        for (int o = 0; o < K; ++o) {
+            int off[D];
+            unflat<D>(o, off);
+            const int l0 = s.base[0] - lo[0] + off[0] - x0;
+            const bool mine = valid && (direct || (l0 >= 0 && l0 < nx));
+            T v[NC];
+            val(off, v);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) v[c] = run_sum(mine ? v[c] : T(0), sg.dist);
+            if (mine && sg.tail) {
+                if (direct) {
+                    int node = 0;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) node = node * n + s.base[d] + off[d];
+#pragma unroll
+                    for (int c = 0; c < NC; ++c)
+                        if (v[c] != T(0)) gadd(gk + (size_t)node * 4 + c, v[c]);
+                } else {
+                    int li = l0;
+#pragma unroll
+                    for (int d = 1; d < D; ++d) li = li * ext[d] + s.base[d] - lo[d] + off[d];
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) atomicAdd(tile + li * 4 + c, v[c]);
+                }
+            }
+        }
+        if (direct) continue;
+        __syncthreads();
+        for (int i = threadIdx.x; i < nodes; i += blockDim.x) {
+            int rem = i, c[D];
+#pragma unroll
+            for (int d = D - 1; d >= 1; --d) {
+                c[d] = lo[d] + rem % ext[d];
+                rem /= ext[d];
+            }
+            c[0] = lo[0] + x0 + rem;
+            int node = 0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) node = node * n + c[d];
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) {
+                const T r = tile[i * 4 + cc];
+                if (r != T(0)) gadd(gk + (size_t)node * 4 + cc, r);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Sensor sums of one workgroup's particles (3D): per group sum of x (3 per
+// group), then sum x (3), sum v (3).  acc: LDS [3*Gs + 6], committed to the
+// robot's sens row with one global atomic per nonzero slot.
+template <typename T>
+__device__ __forceinline__ void sense_block(bool valid, int g, const T nx[3], const T nv[3], int Gs,
+                                            T *acc, T *sens_k) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const T a = wave_sum(valid ? nx[d] : T(0));
+        const T b = wave_sum(valid ? nv[d] : T(0));
+        if (lane == 0) {
+            atomicAdd(acc + 3 * Gs + d, a);
+            atomicAdd(acc + 3 * Gs + 3 + d, b);
+        }
+    }
+    const T xv[3] = {nx[0], nx[1], nx[2]};
+    keyed_wave_add<T, 3>(valid, g, xv, acc, 3);
+    __syncthreads();
+    for (int i = threadIdx.x; i < 3 * Gs + 6; i += blockDim.x)
+        if (acc[i] != T(0)) gadd(sens_k + i, acc[i]);
+}
+
+// ---------------------------------------------------------------- P2G
+template <int D, typename T, bool STORE_F>
+__global__ __launch_bounds__(PB) void k_p2g_b(MP<T> p, PView<D, T> pv, const T *act,
+                                              const int32_t *act_id, int n_act, T *F_out,
+                                              T *grid, Blk bk, int32_t *err) {
+    __shared__ T tile[TILE_NODES * 4];
+    __shared__ int red[2 * D * NWAVE];
+    const int k = blockIdx.y, q = blockIdx.x * PB + threadIdx.x, P = pv.P;
+    bool valid = q < P;
+    T mv[D];
+    Mat<D, T> aff;
+    Stencil<D, T> s;
+#pragma unroll
+    for (int d = 0; d < D; ++d) s.base[d] = 0;
+    if (valid) {
+        T xs[D], vs[D];
+        Mat<D, T> Cm, Fm;
+        pv.load(k, q, xs, vs, Cm, Fm);
+        const int id = act_id[(size_t)k * P + q];
+        const T a = id >= 0 ? act[(size_t)k * n_act + id] : T(0);
+        const Consti<D, T> c = constitutive<D, T>(Cm, Fm, a, p);
+        if (STORE_F) store_mat<D, T>(F_out, k, q, P, c.Fn, false);
+        aff = c.affine;
+#pragma unroll
+        for (int d = 0; d < D; ++d) mv[d] = p.p_mass * vs[d];
+        s = stencil<D, T>(xs, p);
+        if (!s.ok) {
+            atomicOr(err, 1);
+            valid = false;
+        }
+    }
+    int lo[D], ext[D];
+    if (!block_bbox<D>(valid, s.base, lo, ext, red)) return;
+    T *gk = grid + (size_t)k * gsize<D>(p.n) * 4;
+    tile_scatter<D, T, 4>(valid, s, lo, ext, tile, gk, p.n, [&](const int off[D], T v[4]) {
+        T W = T(1), dpos[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            W *= wsel(off[d], s.w[0][d], s.w[1][d], s.w[2][d]);
+            dpos[d] = (T(off[d]) - s.fx[d]) * p.dx;
+        }
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            T m = mv[i];
+#pragma unroll
+            for (int j = 0; j < D; ++j) m += aff.a[i][j] * dpos[j];
+            v[i] = W * m;
+        }
+#pragma unroll
+        for (int i = D; i < 3; ++i) v[i] = T(0);
+        v[3] = W * p.p_mass;
+    });
+    mark_box<D>(bk, k, lo, ext);
+}
+
+// ---------------------------------------------------------------- grid
+// Velocity of every node of the active blocks; with `clear`, the momentum /
+// mass records are zeroed after use (forward); flags are always reset.
+template <int D, typename T>
+__global__ __launch_bounds__(PB) void k_grid_b(MP<T> p, T *grid, T *gvo, Blk bk, int clear) {
+    const int k = blockIdx.y, lane = threadIdx.x & 63;
+    const int nw = gridDim.x * NWAVE;
+    const int cnt = bk.count[k];
+    const int32_t *lst = bk.list + (size_t)k * bk.nblk;
+    const size_t G = gsize<D>(p.n);
+    T *gk = grid + (size_t)k * G * 4, *ok = gvo + (size_t)k * G * 4;
+    for (int e = blockIdx.x * NWAVE + (threadIdx.x >> 6); e < cnt; e += nw) {
+        const int bid = lst[e];
+        bool bnd;
+        const int node = block_node<D, T>(bid, lane, bk.nb, p, bnd);
+        T r[4], o[4] = {T(0), T(0), T(0), T(0)};
+        ld4(gk + (size_t)node * 4, r);
+        const T inv_m = T(1) / (r[3] + T(1e-10));
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            T vo = inv_m * r[d];
+            if (d == 1) vo += p.g_dt;
+            o[d] = bnd ? T(0) : vo;
+        }
+        st4(ok + (size_t)node * 4, o);
+        if (clear) {
+            const T z[4] = {T(0), T(0), T(0), T(0)};
+            st4(gk + (size_t)node * 4, z);
+        }
+        if (lane == 0) bk.flag[(size_t)k * bk.nblk + bid] = 0;
+    }
+}
+
+// ---------------------------------------------------------------- G2P
+template <typename T> struct SenseOut {
+    T *sens;               // row t+1: [B][3*Gs + 6]
+    const int32_t *group;  // [B][P]
+    int Gs;
+};
+
+template <int D, typename T, bool SENSE>
+__global__ __launch_bounds__(PB) void k_g2p_b(MP<T> p, int P, const T *x, const T *gvo, T *x_out,
+                                              T *v_out, T *C_out, int32_t *count,
+                                              SenseOut<T> so) {
+    extern __shared__ unsigned char smem_raw[];
+    T *acc = reinterpret_cast<T *>(smem_raw);
+    const int k = blockIdx.y, q = blockIdx.x * PB + threadIdx.x;
+    if (SENSE) {
+        for (int i = threadIdx.x; i < 3 * so.Gs + 6; i += PB) acc[i] = T(0);
+        __syncthreads();
+    }
+    // the grid pass that read the active list has finished: empty it
+    if (blockIdx.x == 0 && threadIdx.x == 0) count[k] = 0;
+    const bool valid = q < P;
+    T nx[D], nv[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) nx[d] = nv[d] = T(0);
+    if (valid) {
+        const size_t bv = (size_t)k * D * P, bm = (size_t)k * D * D * P;
+        T xs[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) xs[d] = x[bv + (size_t)d * P + q];
+        const Stencil<D, T> s = stencil<D, T>(xs, p);
+        T nC[D][D];
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int j = 0; j < D; ++j) nC[i][j] = T(0);
+        if (s.ok) {
+            const T *g = gvo + (size_t)k * gsize<D>(p.n) * 4;
+            constexpr int K = D == 2 ? 9 : 27;
+#pragma unroll
+            for (int o = 0; o < K; ++o) {
+                int off[D];
+                unflat<D>(o, off);
+                T W = T(1), dp[D];
+                int node = 0;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    W *= s.w[off[d]][d];
+                    dp[d] = T(off[d]) - s.fx[d];
+                    node = node * p.n + s.base[d] + off[d];
+                }
+                T gv4[4];
+                ld4(g + (size_t)node * 4, gv4);
+#pragma unroll
+                for (int i = 0; i < D; ++i) {
+                    nv[i] += W * gv4[i];
+#pragma unroll
+                    for (int j = 0; j < D; ++j) nC[i][j] += T(4) * p.inv_dx * W * gv4[i] * dp[j];
+                }
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            nx[d] = xs[d] + p.dt * nv[d];
+            v_out[bv + (size_t)d * P + q] = nv[d];
+            x_out[bv + (size_t)d * P + q] = nx[d];
+        }
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int j = 0; j < D; ++j) C_out[bm + (size_t)(i * D + j) * P + q] = nC[i][j];
+    }
+    if constexpr (SENSE && D == 3) {
+        const int g = valid ? so.group[(size_t)k * P + q] : 0;
+        sense_block<T>(valid, g, nx, nv, so.Gs, acc, so.sens + (size_t)k * (3 * so.Gs + 6));
+    }
+}
+
+// Sensor sums of a stored state (the first step of a rollout).
+template <typename T>
+__global__ __launch_bounds__(PB) void k_sense(int P, const T *x, const T *v, SenseOut<T> so) {
+    extern __shared__ unsigned char smem_raw[];
+    T *acc = reinterpret_cast<T *>(smem_raw);
+    const int k = blockIdx.y, q = blockIdx.x * PB + threadIdx.x;
+    for (int i = threadIdx.x; i < 3 * so.Gs + 6; i += PB) acc[i] = T(0);
+    __syncthreads();
+    const bool valid = q < P;
+    T nx[3] = {T(0), T(0), T(0)}, nv[3] = {T(0), T(0), T(0)};
+    int g = 0;
+    if (valid) {
+        const size_t bv = (size_t)k * 3 * P;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            nx[d] = x[bv + (size_t)d * P + q];
+            nv[d] = v[bv + (size_t)d * P + q];
+        }
+        g = so.group[(size_t)k * P + q];
+    }
+    sense_block<T>(valid, g, nx, nv, so.Gs, acc, so.sens + (size_t)k * (3 * so.Gs + 6));
+}
+
+// ---------------------------------------------------------------- controller
+template <typename T> struct Ctrl {
+    int P, Gs, n_sin, run_period, I, H, O, act;
+    T scale, max_speed;
+    const T *inv_count, *targets, *W1, *b1, *W2, *b2;
+};
+
+// Block per robot: COM / COM velocity of step t from the sensor sums, and
+// with `mlp` the network input, hidden and output rows (taichi_env.py:149-183
+// sensor order for the MPM quadruped; nn.py:130-150).
+template <typename T>
+__global__ __launch_bounds__(64) void k_ctrl(Ctrl<T> c, int t, const T *sens, T *center, T *vcom,
+                                             T *inp, T *hid, T *out, int mlp) {
+    extern __shared__ unsigned char smem_raw[];
+    T *si = reinterpret_cast<T *>(smem_raw);
+    T *shd = si + c.I;
+    const int k = blockIdx.x, tid = threadIdx.x;
+    const int NS = 3 * c.Gs + 6;
+    const T *sk = sens + (size_t)k * NS;
+    const T invP = T(1) / T(c.P);
+    if (tid < 3) {
+        center[k * 3 + tid] = sk[3 * c.Gs + tid] * invP;
+        vcom[k * 3 + tid] = sk[3 * c.Gs + 3 + tid] * invP;
+    }
+    if (!mlp) return;
+    const T *tg = c.targets + (size_t)k * 5;
+    for (int j = tid; j < c.I; j += 64) {
+        T val;
+        if (j < c.n_sin) {
+            val = T(::sin(2.0 * M_PI / c.run_period * (double)t + 2.0 * M_PI / c.n_sin * (double)j));
+        } else if (j < c.n_sin + 3) {
+            val = sk[3 * c.Gs + 3 + (j - c.n_sin)] * invP;
+        } else if (j < c.n_sin + 3 + 3 * c.Gs) {
+            const int q = j - c.n_sin - 3, g = q / 3, d = q - 3 * g;
+            val = (sk[q] * c.inv_count[(size_t)k * c.Gs + g] - sk[3 * c.Gs + d] * invP) / c.scale;
+        } else if (j < c.n_sin + 6 + 3 * c.Gs) {
+            val = tg[j - c.n_sin - 3 - 3 * c.Gs] / c.max_speed;
+        } else {
+            val = tg[3];
+        }
+        si[j] = val;
+        inp[(size_t)k * c.I + j] = val;
+    }
+    __syncthreads();
+    for (int h = tid; h < c.H; h += 64) {
+        T z = c.b1[h];
+        const T *w = c.W1 + (size_t)h * c.I;
+        for (int i = 0; i < c.I; ++i) z += w[i] * si[i];
+        T y, dy;
+        activate<T>(c.act, z, y, dy);
+        shd[h] = y;
+        hid[(size_t)k * c.H + h] = y;
+    }
+    __syncthreads();
+    for (int o = tid; o < c.O; o += 64) {
+        T z = c.b2[o];
+        const T *w = c.W2 + (size_t)o * c.H;
+        for (int h = 0; h < c.H; ++h) z += w[h] * shd[h];
+        T y, dy;
+        activate<T>(c.act, z, y, dy);
+        out[(size_t)k * c.O + o] = y;
+    }
+}
+
+// Adjoint of k_ctrl (with mlp) at step t: consumes (and zeroes) act_b,
+// writes in_b row t, accumulates the robot's weight-gradient slice
+// [W1 (H*I) | b1 (H) | W2 (O*H) | b2 (O)].
+template <typename T>
+__global__ __launch_bounds__(64) void k_ctrl_adj(Ctrl<T> c, const T *inp, T *act_b, T *in_b,
+                                                 T *wg) {
+    extern __shared__ unsigned char smem_raw[];
+    T *si = reinterpret_cast<T *>(smem_raw);
+    T *shd = si + c.I;
+    T *sd1 = shd + c.H;
+    T *sz1b = sd1 + c.H;
+    T *sab = sz1b + c.H;
+    const int k = blockIdx.x, tid = threadIdx.x;
+    for (int i = tid; i < c.I; i += 64) si[i] = inp[(size_t)k * c.I + i];
+    __syncthreads();
+    for (int h = tid; h < c.H; h += 64) {
+        T z = c.b1[h];
+        const T *w = c.W1 + (size_t)h * c.I;
+        for (int i = 0; i < c.I; ++i) z += w[i] * si[i];
+        T y, dy;
+        activate<T>(c.act, z, y, dy);
+        shd[h] = y;
+        sd1[h] = dy;
+    }
+    __syncthreads();
+    for (int o = tid; o < c.O; o += 64) {
+        T z = c.b2[o];
+        const T *w = c.W2 + (size_t)o * c.H;
+        for (int h = 0; h < c.H; ++h) z += w[h] * shd[h];
+        T y, dy;
+        activate<T>(c.act, z, y, dy);
+        T *ab = act_b + (size_t)k * c.O + o;
+        sab[o] = *ab * dy;
+        *ab = T(0);
+    }
+    __syncthreads();
+    for (int h = tid; h < c.H; h += 64) {
+        T hb = T(0);
+        for (int o = 0; o < c.O; ++o) hb += c.W2[(size_t)o * c.H + h] * sab[o];
+        sz1b[h] = hb * sd1[h];
+    }
+    __syncthreads();
+    for (int i = tid; i < c.I; i += 64) {
+        T ib = T(0);
+        for (int h = 0; h < c.H; ++h) ib += c.W1[(size_t)h * c.I + i] * sz1b[h];
+        in_b[(size_t)k * c.I + i] = ib;
+    }
+    T *g = wg + (size_t)k * ((size_t)c.H * c.I + c.H + (size_t)c.O * c.H + c.O);
+    for (int idx = tid; idx < c.H * c.I; idx += 64) {
+        const int h = idx / c.I, i = idx - h * c.I;
+        g[idx] += sz1b[h] * si[i];
+    }
+    g += (size_t)c.H * c.I;
+    for (int h = tid; h < c.H; h += 64) g[h] += sz1b[h];
+    g += c.H;
+    for (int idx = tid; idx < c.O * c.H; idx += 64) {
+        const int o = idx / c.H, h = idx - o * c.H;
+        g[idx] += sab[o] * shd[h];
+    }
+    g += (size_t)c.O * c.H;
+    for (int o = tid; o < c.O; o += 64) g[o] += sab[o];
+}
+
+// ---------------------------------------------------------------- adjoints
+template <typename T> struct SenseAdj {
+    const T *in_b;         // row t+1: [B][I]; null: no sensor terms
+    const T *cg, *vg;      // rows t+1: [B][3]
+    const int32_t *group;  // [B][P]
+    const T *inv_count;    // [B][Gs]
+    int Gs, n_sin, I;
+    T scale;
+};
+
+// G2P adjoint with the sensor / COM adjoint of state t+1 folded into the
+// incoming (x, v) adjoints: writes xb (state t), scatters dL/d grid_vo.
+template <int D, typename T>
+__global__ __launch_bounds__(PB) void k_g2p_adj_b(MP<T> p, int P, const T *x, const T *gvo,
+                                                  const T *xb1, const T *vb1, const T *Cb1, T *xb,
+                                                  T *gvo_b, SenseAdj<T> sa) {
+    __shared__ T tile[TILE_NODES * 4];
+    __shared__ int red[2 * D * NWAVE];
+    const int k = blockIdx.y, q = blockIdx.x * PB + threadIdx.x;
+    const bool valid = q < P;
+    const size_t bv = (size_t)k * D * P, bm = (size_t)k * D * D * P;
+    T xs[D], xe[D], vnb[D], Cb[D][D];
+    Stencil<D, T> s;
+#pragma unroll
+    for (int d = 0; d < D; ++d) s.base[d] = 0;
+    bool inside = false;
+    if (valid) {
+        T ve[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            xs[d] = x[bv + (size_t)d * P + q];
+            xe[d] = xb1[bv + (size_t)d * P + q];
+            ve[d] = vb1[bv + (size_t)d * P + q];
+        }
+        if constexpr (D == 3) {
+            if (sa.in_b) {
+                const T *ib = sa.in_b + (size_t)k * sa.I;
+                const int g = sa.group[(size_t)k * P + q];
+                const T invP = T(1) / T(P);
+                const T icg = sa.inv_count[(size_t)k * sa.Gs + g];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    T S = T(0);
+                    for (int h = 0; h < sa.Gs; ++h) S += ib[sa.n_sin + 3 + 3 * h + d];
+                    xe[d] += (ib[sa.n_sin + 3 + 3 * g + d] * icg - S * invP) / sa.scale +
+                             sa.cg[k * 3 + d] * invP;
+                    ve[d] += (ib[sa.n_sin + d] + sa.vg[k * 3 + d]) * invP;
+                }
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) vnb[d] = ve[d] + p.dt * xe[d];
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int j = 0; j < D; ++j) Cb[i][j] = Cb1[bm + (size_t)(i * D + j) * P + q];
+        s = stencil<D, T>(xs, p);
+        inside = s.ok;
+    }
+    // gather: stencil-weight derivatives against grid_vo
+    T fxb[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) fxb[d] = T(0);
+    const size_t G = gsize<D>(p.n);
+    const T c4 = T(4) * p.inv_dx;
+    if (inside) {
+        const T *g = gvo + (size_t)k * G * 4;
+        constexpr int K = D == 2 ? 9 : 27;
+#pragma unroll
+        for (int o = 0; o < K; ++o) {
+            int off[D];
+            unflat<D>(o, off);
+            T W = T(1), dp[D];
+            int node = 0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                W *= s.w[off[d]][d];
+                dp[d] = T(off[d]) - s.fx[d];
+                node = node * p.n + s.base[d] + off[d];
+            }
+            T vo4[4];
+            ld4(g + (size_t)node * 4, vo4);
+            T Wb = T(0);
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                T acc = T(0);
+#pragma unroll
+                for (int j = 0; j < D; ++j) acc += Cb[i][j] * dp[j];
+                Wb += vo4[i] * (vnb[i] + c4 * acc);
+            }
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                T CtV = T(0);
+#pragma unroll
+                for (int i = 0; i < D; ++i) CtV += Cb[i][j] * vo4[i];
+                fxb[j] -= c4 * W * CtV;
+            }
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                T dW = s.dw[off[d]][d];
+#pragma unroll
+                for (int e = 0; e < D; ++e)
+                    if (e != d) dW *= s.w[off[e]][e];
+                fxb[d] += Wb * dW;
+            }
+        }
+    }
+    // scatter: dL/d grid_vo += W (vnb + 4/dx Cb dpos)
+    int lo[D], ext[D];
+    if (block_bbox<D>(inside, s.base, lo, ext, red)) {
+        tile_scatter<D, T, D>(inside, s, lo, ext, tile, gvo_b + (size_t)k * G * 4, p.n,
+                              [&](const int off[D], T v[D]) {
+            T W = T(1), dp[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                W *= wsel(off[d], s.w[0][d], s.w[1][d], s.w[2][d]);
+                dp[d] = T(off[d]) - s.fx[d];
+            }
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                T acc = T(0);
+#pragma unroll
+                for (int j = 0; j < D; ++j) acc += Cb[i][j] * dp[j];
+                v[i] = W * (vnb[i] + c4 * acc);
+            }
+        });
+    }
+    if (valid) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) xb[bv + (size_t)d * P + q] = xe[d] + p.inv_dx * fxb[d];
+    }
+}
+
+// Grid adjoint over the active blocks: dL/d(momentum, mass) from dL/d vo;
+// clears the grid and vo-adjoint records it consumed.
+template <int D, typename T>
+__global__ __launch_bounds__(PB) void k_grid_adj_b(MP<T> p, T *grid, T *gvo_b, T *grid_b, Blk bk) {
+    const int k = blockIdx.y, lane = threadIdx.x & 63;
+    const int nw = gridDim.x * NWAVE;
+    const int cnt = bk.count[k];
+    const int32_t *lst = bk.list + (size_t)k * bk.nblk;
+    const size_t G = gsize<D>(p.n);
+    T *gk = grid + (size_t)k * G * 4, *obk = gvo_b + (size_t)k * G * 4;
+    T *ibk = grid_b + (size_t)k * G * 4;
+    const T z[4] = {T(0), T(0), T(0), T(0)};
+    for (int e = blockIdx.x * NWAVE + (threadIdx.x >> 6); e < cnt; e += nw) {
+        bool bnd;
+        const int node = block_node<D, T>(lst[e], lane, bk.nb, p, bnd);
+        T r[4], ob[4], o[4] = {T(0), T(0), T(0), T(0)};
+        ld4(gk + (size_t)node * 4, r);
+        ld4(obk + (size_t)node * 4, ob);
+        const T m = r[3] + T(1e-10);
+        T mb = T(0);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const T b = bnd ? T(0) : ob[d];
+            o[d] = b / m;
+            mb -= b * r[d] / (m * m);
+        }
+        o[3] = mb;
+        st4(ibk + (size_t)node * 4, o);
+        st4(gk + (size_t)node * 4, z);
+        st4(obk + (size_t)node * 4, z);
+    }
+}
+
+// P2G adjoint: dL/d(momentum, mass) and dL/dF[t+1] -> xb (+=), vb, Cb, Fb
+// (written), act_b (wave-aggregated atomics).  Empties the active list.
+template <int D, typename T>
+__global__ __launch_bounds__(PB) void k_p2g_adj_b(MP<T> p, PView<D, T> pv, const T *act,
+                                                  const int32_t *act_id, int n_act,
+                                                  const T *grid_b, const T *Fb1, T *xb, T *vb,
+                                                  T *Cb, T *Fb, T *act_b, int32_t *count) {
+    extern __shared__ unsigned char smem_raw[];
+    T *sab = reinterpret_cast<T *>(smem_raw);     // [n_act]
+    const int k = blockIdx.y, q = blockIdx.x * PB + threadIdx.x, P = pv.P;
+    for (int i = threadIdx.x; i < n_act; i += PB) sab[i] = T(0);
+    __syncthreads();
+    // the grid adjoint pass that read the active list has finished
+    if (blockIdx.x == 0 && threadIdx.x == 0) count[k] = 0;
+    const bool valid = q < P;
+    int id = -1;
+    T abar[1] = {T(0)};
+    if (valid) {
+        T xs[D], vs[D];
+        Mat<D, T> Cm, Fm;
+        pv.load(k, q, xs, vs, Cm, Fm);
+        id = act_id[(size_t)k * P + q];
+        const T a = id >= 0 ? act[(size_t)k * n_act + id] : T(0);
+        const Consti<D, T> c = constitutive<D, T>(Cm, Fm, a, p);
+        const Stencil<D, T> s = stencil<D, T>(xs, p);
+        const size_t bv = (size_t)k * D * P, bm = (size_t)k * D * D * P;
+        Mat<D, T> affb, Fnb;
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                affb.a[i][j] = T(0);
+                Fnb.a[i][j] = Fb1[bm + (size_t)(i * D + j) * P + q];
+            }
+        T vacc[D], fxb[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) vacc[d] = fxb[d] = T(0);
+        if (s.ok) {
+            const T *gb = grid_b + (size_t)k * gsize<D>(p.n) * 4;
+            constexpr int K = D == 2 ? 9 : 27;
+#pragma unroll
+            for (int o = 0; o < K; ++o) {
+                int off[D];
+                unflat<D>(o, off);
+                T W = T(1), dpos[D];
+                int node = 0;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    W *= s.w[off[d]][d];
+                    dpos[d] = (T(off[d]) - s.fx[d]) * p.dx;
+                    node = node * p.n + s.base[d] + off[d];
+                }
+                T g4[4];
+                ld4(gb + (size_t)node * 4, g4);
+                T Wb = g4[3] * p.p_mass;
+#pragma unroll
+                for (int i = 0; i < D; ++i) {
+                    T cv = p.p_mass * vs[i];
+#pragma unroll
+                    for (int j = 0; j < D; ++j) cv += c.affine.a[i][j] * dpos[j];
+                    Wb += g4[i] * cv;
+                    vacc[i] += W * p.p_mass * g4[i];
+#pragma unroll
+                    for (int j = 0; j < D; ++j) affb.a[i][j] += W * g4[i] * dpos[j];
+                }
+#pragma unroll
+                for (int j = 0; j < D; ++j) {
+                    T atg = T(0);
+#pragma unroll
+                    for (int i = 0; i < D; ++i) atg += c.affine.a[i][j] * g4[i];
+                    fxb[j] -= p.dx * W * atg;
+                }
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    T dWd = s.dw[off[d]][d];
+#pragma unroll
+                    for (int e = 0; e < D; ++e)
+                        if (e != d) dWd *= s.w[off[e]][e];
+                    fxb[d] += Wb * dWd;
+                }
+            }
+        }
+        Mat<D, T> Fbm, Cbm;
+        abar[0] = consti_adj<D, T>(p, c, Cm, Fm, affb, Fnb, Fbm, Cbm);
+        store_mat<D, T>(Fb, k, q, P, Fbm, false);
+        store_mat<D, T>(Cb, k, q, P, Cbm, false);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            vb[bv + (size_t)d * P + q] = vacc[d];
+            xb[bv + (size_t)d * P + q] += p.inv_dx * fxb[d];
+        }
+    }
+    keyed_wave_add<T, 1>(valid && id >= 0, id, abar, sab, 1);
+    __syncthreads();
+    for (int i = threadIdx.x; i < n_act; i += PB)
+        if (sab[i] != T(0)) gadd(act_b + (size_t)k * n_act + i, sab[i]);
+}
+
+template <typename T>
+__global__ void k_copy(const T *src, T *dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+}  // namespace dl
+
+// ================================================================== C ABI
+using namespace dl;
+
+namespace {
+
+int check_batch(const dl_mpm_batch *b) {
+    DL_CHECK_ARG(b != nullptr, "MPM batch descriptor is NULL");
+    const dl_mpm_params &c = b->params;
+    DL_CHECK_ARG(c.dim == 2 || c.dim == 3, "MPM batch: dim must be 2 or 3");
+    const int E = c.dim == 3 ? 4 : 8;
+    DL_CHECK_ARG(c.n_grid >= 8 && c.n_grid <= 1024 && c.n_grid % E == 0,
+                 "MPM batch: n_grid must be a multiple of %d in [8, 1024]", E);
+    DL_CHECK_ARG(c.act_axis >= 0 && c.act_axis < c.dim, "MPM batch: bad actuation axis");
+    DL_CHECK_ARG(b->dtype == DL_F32 || b->dtype == DL_F64, "MPM batch: bad dtype");
+    DL_CHECK_ARG(b->B > 0 && b->P > 0 && b->n_act > 0 && b->horizon > 0, "MPM batch: bad sizes");
+    DL_CHECK_ARG(b->act_id && b->grid && b->grid_vo && b->blk_flag && b->blk_list &&
+                     b->blk_count && b->err && b->act_out,
+                 "MPM batch: NULL buffer");
+    return 0;
+}
+
+int check_control(const dl_mpm_batch *b) {
+    DL_CHECK_ARG(b->params.dim == 3, "MPM controller: 3D robots only");
+    DL_CHECK_ARG(b->n_groups > 0 && b->n_sin >= 0 && b->run_period > 0 && b->n_hidden > 0 &&
+                     b->offset_scale != 0 && b->max_speed != 0,
+                 "MPM controller: bad layout");
+    DL_CHECK_ARG(b->group && b->inv_count && b->targets && b->W1 && b->b1 && b->W2 && b->b2 &&
+                     b->sens && b->center && b->vcom && b->inp && b->hid,
+                 "MPM controller: NULL buffer");
+    return 0;
+}
+
+int n_in(const dl_mpm_batch *b) { return b->n_sin + 3 + 3 * b->n_groups + 4; }
+
+template <typename T> Ctrl<T> make_ctrl(const dl_mpm_batch *b) {
+    Ctrl<T> c;
+    c.P = b->P; c.Gs = b->n_groups; c.n_sin = b->n_sin; c.run_period = b->run_period;
+    c.I = n_in(b); c.H = b->n_hidden; c.O = b->n_act; c.act = b->act;
+    c.scale = T(b->offset_scale); c.max_speed = T(b->max_speed);
+    c.inv_count = (const T *)b->inv_count; c.targets = (const T *)b->targets;
+    c.W1 = (const T *)b->W1; c.b1 = (const T *)b->b1; c.W2 = (const T *)b->W2;
+    c.b2 = (const T *)b->b2;
+    return c;
+}
+
+template <int D> Blk make_blk(const dl_mpm_batch *b) {
+    Blk k;
+    k.flag = b->blk_flag; k.list = b->blk_list; k.count = b->blk_count;
+    k.nb = b->params.n_grid / BlkE<D>::E;
+    k.nblk = 1;
+    for (int d = 0; d < D; ++d) k.nblk *= k.nb;
+    return k;
+}
+
+template <int D, typename T> struct Slot {
+    T *x, *v, *C, *F;
+};
+template <int D, typename T> Slot<D, T> slot(const dl_mpm_slots *s, int i, int B, int P) {
+    const size_t sv = (size_t)B * D * P, sm = (size_t)B * D * D * P;
+    return {(T *)s->x + i * sv, (T *)s->v + i * sv, (T *)s->C + i * sm, (T *)s->F + i * sm};
+}
+
+int grid_wgs(const Blk &k) { return std::max(1, std::min(64, (k.nblk + NWAVE - 1) / NWAVE)); }
+
+template <int D, typename T>
+int forward_impl(const dl_mpm_batch *b, const dl_mpm_slots *st, int t0, int n, int flags,
+                 hipStream_t s) {
+    const int B = b->B, P = b->P, O = b->n_act, NS = 3 * b->n_groups + 6;
+    const bool control = flags & DL_MPM_CONTROL;
+    const MP<T> mp = make_mp<T>(b->params);
+    const Blk bk = make_blk<D>(b);
+    const dim3 pg((P + PB - 1) / PB, B), gg(grid_wgs(bk), B);
+    const int I = n_in(b), H = b->n_hidden;
+    const T *act = (const T *)b->act_out;
+    if (control && (flags & DL_MPM_SENSE_FIRST)) {
+        const Slot<D, T> s0 = slot<D, T>(st, 0, B, P);
+        if constexpr (D == 3)
+            hipLaunchKernelGGL(k_sense<T>, pg, dim3(PB), NS * sizeof(T), s, P, s0.x, s0.v,
+                               SenseOut<T>{(T *)b->sens + (size_t)t0 * B * NS, b->group,
+                                           b->n_groups});
+    }
+    for (int j = 0; j < n; ++j) {
+        const int t = t0 + j;
+        const Slot<D, T> a = slot<D, T>(st, j % st->n_slots, B, P);
+        const Slot<D, T> c = slot<D, T>(st, (j + 1) % st->n_slots, B, P);
+        if (control)
+            hipLaunchKernelGGL(k_ctrl<T>, dim3(B), dim3(64), (I + H) * sizeof(T), s,
+                               make_ctrl<T>(b), t, (const T *)b->sens + (size_t)t * B * NS,
+                               (T *)b->center + (size_t)t * B * 3, (T *)b->vcom + (size_t)t * B * 3,
+                               (T *)b->inp + (size_t)t * B * I, (T *)b->hid + (size_t)t * B * H,
+                               (T *)b->act_out + (size_t)t * B * O, 1);
+        PView<D, T> pv{a.x, a.v, a.C, a.F, P};
+        hipLaunchKernelGGL((k_p2g_b<D, T, true>), pg, dim3(PB), 0, s, mp, pv,
+                           act + (size_t)t * B * O, b->act_id, O, c.F, (T *)b->grid, bk, b->err);
+        hipLaunchKernelGGL((k_grid_b<D, T>), gg, dim3(PB), 0, s, mp, (T *)b->grid, (T *)b->grid_vo,
+                           bk, 1);
+        if (control && D == 3)
+            hipLaunchKernelGGL((k_g2p_b<D, T, true>), pg, dim3(PB), NS * sizeof(T), s, mp, P, a.x,
+                               (const T *)b->grid_vo, c.x, c.v, c.C, b->blk_count,
+                               SenseOut<T>{(T *)b->sens + (size_t)(t + 1) * B * NS, b->group,
+                                           b->n_groups});
+        else
+            hipLaunchKernelGGL((k_g2p_b<D, T, false>), pg, dim3(PB), 0, s, mp, P, a.x,
+                               (const T *)b->grid_vo, c.x, c.v, c.C, b->blk_count,
+                               SenseOut<T>{nullptr, nullptr, 0});
+    }
+    if (control && t0 + n == b->horizon)
+        hipLaunchKernelGGL(k_ctrl<T>, dim3(B), dim3(64), (I + H) * sizeof(T), s, make_ctrl<T>(b),
+                           t0 + n, (const T *)b->sens + (size_t)(t0 + n) * B * NS,
+                           (T *)b->center + (size_t)(t0 + n) * B * 3,
+                           (T *)b->vcom + (size_t)(t0 + n) * B * 3, (T *)nullptr, (T *)nullptr,
+                           (T *)nullptr, 0);
+    DL_CHECK_LAUNCH();
+    return 0;
+}
+
+template <int D, typename T>
+int backward_impl(const dl_mpm_batch *b, const dl_mpm_slots *seg, const dl_mpm_slots *adj, int t0,
+                  int n, hipStream_t s) {
+    const int B = b->B, P = b->P, O = b->n_act;
+    const MP<T> mp = make_mp<T>(b->params);
+    const Blk bk = make_blk<D>(b);
+    const dim3 pg((P + PB - 1) / PB, B), gg(grid_wgs(bk), B);
+    const int I = n_in(b), H = b->n_hidden;
+    const Ctrl<T> ctrl = make_ctrl<T>(b);
+    const T *act = (const T *)b->act_out;
+    int cur = 0;
+    for (int j = n - 1; j >= 0; --j) {
+        const int t = t0 + j, nxt = 1 - cur;
+        const Slot<D, T> a = slot<D, T>(seg, j, B, P);
+        const Slot<D, T> A1 = slot<D, T>(adj, cur, B, P), A0 = slot<D, T>(adj, nxt, B, P);
+        PView<D, T> pv{a.x, a.v, a.C, a.F, P};
+        hipLaunchKernelGGL((k_p2g_b<D, T, false>), pg, dim3(PB), 0, s, mp, pv,
+                           act + (size_t)t * B * O, b->act_id, O, (T *)nullptr, (T *)b->grid, bk,
+                           b->err);
+        hipLaunchKernelGGL((k_grid_b<D, T>), gg, dim3(PB), 0, s, mp, (T *)b->grid, (T *)b->grid_vo,
+                           bk, 0);
+        SenseAdj<T> sa{};
+        if (D == 3 && b->in_grad) {
+            sa.in_b = (const T *)b->in_grad + (size_t)(t + 1) * B * I;
+            sa.cg = (const T *)b->center_grad + (size_t)(t + 1) * B * 3;
+            sa.vg = (const T *)b->vcom_grad + (size_t)(t + 1) * B * 3;
+            sa.group = b->group;
+            sa.inv_count = (const T *)b->inv_count;
+            sa.Gs = b->n_groups;
+            sa.n_sin = b->n_sin;
+            sa.I = I;
+            sa.scale = T(b->offset_scale);
+        }
+        hipLaunchKernelGGL((k_g2p_adj_b<D, T>), pg, dim3(PB), 0, s, mp, P, a.x,
+                           (const T *)b->grid_vo, A1.x, A1.v, A1.C, A0.x, (T *)b->grid_vo_grad, sa);
+        hipLaunchKernelGGL((k_grid_adj_b<D, T>), gg, dim3(PB), 0, s, mp, (T *)b->grid,
+                           (T *)b->grid_vo_grad, (T *)b->grid_grad, bk);
+        hipLaunchKernelGGL((k_p2g_adj_b<D, T>), pg, dim3(PB), O * sizeof(T), s, mp, pv,
+                           act + (size_t)t * B * O, b->act_id, O, (const T *)b->grid_grad, A1.F,
+                           A0.x, A0.v, A0.C, A0.F, (T *)b->act_grad, b->blk_count);
+        if (b->in_grad)
+            hipLaunchKernelGGL(k_ctrl_adj<T>, dim3(B), dim3(64), (I + 3 * H + O) * sizeof(T), s,
+                               ctrl, (const T *)b->inp + (size_t)t * B * I, (T *)b->act_grad,
+                               (T *)b->in_grad + (size_t)t * B * I, (T *)b->wgrad);
+        cur = nxt;
+    }
+    if (cur != 0) {
+        const Slot<D, T> from = slot<D, T>(adj, 1, B, P), to = slot<D, T>(adj, 0, B, P);
+        const size_t nv = (size_t)B * D * P, nm = (size_t)B * D * D * P;
+        hipLaunchKernelGGL(k_copy<T>, dim3(1024), dim3(256), 0, s, from.x, to.x, nv);
+        hipLaunchKernelGGL(k_copy<T>, dim3(1024), dim3(256), 0, s, from.v, to.v, nv);
+        hipLaunchKernelGGL(k_copy<T>, dim3(1024), dim3(256), 0, s, from.C, to.C, nm);
+        hipLaunchKernelGGL(k_copy<T>, dim3(1024), dim3(256), 0, s, from.F, to.F, nm);
+    }
+    DL_CHECK_LAUNCH();
+    return 0;
+}
+
+}  // namespace
+
+#define MPMB_DISPATCH(b, FN, ...)                                                          \
+    do {                                                                                   \
+        if ((b)->params.dim == 3 && (b)->dtype == DL_F32) return FN<3, float>(__VA_ARGS__);  \
+        if ((b)->params.dim == 3 && (b)->dtype == DL_F64) return FN<3, double>(__VA_ARGS__); \
+        if ((b)->params.dim == 2 && (b)->dtype == DL_F32) return FN<2, float>(__VA_ARGS__);  \
+        return FN<2, double>(__VA_ARGS__);                                                 \
+    } while (0)
+
+extern "C" {
+
+int32_t dl_mpm_forward(const dl_mpm_batch *b, const dl_mpm_slots *st, int32_t t0, int32_t n,
+                       int32_t flags, void *stream) {
+    if (int e = check_batch(b)) return e;
+    DL_CHECK_ARG(st && st->x && st->v && st->C && st->F && st->n_slots >= 2,
+                 "dl_mpm_forward: bad state slots");
+    DL_CHECK_ARG(n > 0 && t0 >= 0 && t0 + n <= b->horizon, "dl_mpm_forward: steps [%d, %d) "
+                 "outside the horizon %d", t0, t0 + n, b->horizon);
+    if (flags & DL_MPM_CONTROL)
+        if (int e = check_control(b)) return e;
+    MPMB_DISPATCH(b, forward_impl, b, st, t0, n, flags, (hipStream_t)stream);
+}
+
+int32_t dl_mpm_backward(const dl_mpm_batch *b, const dl_mpm_slots *seg, const dl_mpm_slots *adj,
+                        int32_t t0, int32_t n, void *stream) {
+    if (int e = check_batch(b)) return e;
+    DL_CHECK_ARG(seg && seg->x && seg->v && seg->C && seg->F && seg->n_slots >= n + 1,
+                 "dl_mpm_backward: the segment needs n + 1 state slots");
+    DL_CHECK_ARG(adj && adj->x && adj->v && adj->C && adj->F && adj->n_slots == 2,
+                 "dl_mpm_backward: the adjoint needs 2 slots");
+    DL_CHECK_ARG(n > 0 && t0 >= 0 && t0 + n <= b->horizon, "dl_mpm_backward: bad step range");
+    DL_CHECK_ARG(b->grid_vo_grad && b->grid_grad && b->act_grad, "dl_mpm_backward: NULL buffer");
+    if (b->in_grad) {
+        if (int e = check_control(b)) return e;
+        DL_CHECK_ARG(b->center_grad && b->vcom_grad && b->wgrad, "dl_mpm_backward: NULL buffer");
+    }
+    MPMB_DISPATCH(b, backward_impl, b, seg, adj, t0, n, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -222,24 +222,89 @@ class SolverMPM:
         self.F_grad[s] += Fb
 
 
+class MPMBatchC(Structure):
+    """dl_mpm_batch."""
+    _fields_ = ([(n, c_int32) for n in ("dtype", "B", "P", "n_act", "n_groups", "n_sin",
+                                        "run_period", "n_hidden", "act", "horizon")] +
+                [("offset_scale", c_double), ("max_speed", c_double), ("params", MPMParamsC)] +
+                [(n, ctypes.c_void_p) for n in (
+                    "act_id", "group", "inv_count", "targets", "W1", "b1", "W2", "b2",
+                    "grid", "grid_vo", "grid_vo_grad", "grid_grad", "blk_flag", "blk_list",
+                    "blk_count", "err", "sens", "center", "vcom", "inp", "hid", "act_out",
+                    "act_grad", "in_grad", "center_grad", "vcom_grad", "wgrad")])
+
+
+class MPMSlotsC(Structure):
+    """dl_mpm_slots."""
+    _fields_ = [("x", ctypes.c_void_p), ("v", ctypes.c_void_p), ("C", ctypes.c_void_p),
+                ("F", ctypes.c_void_p), ("n_slots", c_int32), ("pad_", c_int32)]
+
+
+MPM_CONTROL, MPM_SENSE_FIRST = 1, 2
+
+
+def block_order(positions, n_grid, dim=3):
+    """Permutation sorting particles by grid block (4^3 nodes in 3D, 8^2 in
+    2D) of their stencil base cell, then by cell: consecutive particles then
+    share stencil nodes, which keeps the LDS tiles of P2G and of the G2P
+    adjoint small."""
+    E = 4 if dim == 3 else 8
+    base = np.floor(np.asarray(positions) * n_grid - 0.5).astype(np.int64)
+    nb = n_grid // E
+    blk = np.zeros(len(base), np.int64)
+    cell = np.zeros(len(base), np.int64)
+    for d in range(dim):
+        blk = blk * nb + base[:, d] // E
+        cell = cell * E + base[:, d] % E
+    return np.argsort(blk * E ** dim + cell, kind="stable")
+
+
+class _Slots:
+    """S state slots: x, v [S,B,D,P]; C, F [S,B,D*D,P]."""
+
+    def __init__(self, S, B, D, P, dtype, device):
+        z = lambda *s: torch.zeros(*s, dtype=dtype, device=device)
+        self.x, self.v, self.C, self.F = z(S, B, D, P), z(S, B, D, P), z(S, B, D * D, P), z(S, B, D * D, P)
+        self.c = MPMSlotsC(nat.ptr(self.x), nat.ptr(self.v), nat.ptr(self.C), nat.ptr(self.F), S, 0)
+
+    def fields(self):
+        return (self.x, self.v, self.C, self.F)
+
+    def copy_slot(self, dst, src, other=None):
+        for a, b in zip(self.fields(), (other or self).fields()):
+            a[dst].copy_(b[src])
+
+
 class MPMBatch:
-    """Batched, checkpointed MLS-MPM training rollout.
+    """Batched, checkpointed MLS-MPM training rollout (3D).
 
     B robots of one particle design, each on its own grid, driven by the
     sensor -> MLP -> actuation loop of the quadruped workload (sinusoids,
-    COM velocity, per-group mean offsets from the COM, targets), trained on
-    the windowed COM loss.  ``forward`` advances all robots ``T`` steps,
-    checkpointing particle state every ``K`` steps; ``backward`` recomputes
-    each segment from its checkpoint and runs the hand-written adjoints
-    (P2G/grid/G2P, MLP, sensors, COM) in reverse.  Weight gradients land in
-    ``grads`` (W1, b1, W2, b2)."""
+    COM velocity, per-group mean offsets from the COM, targets) and trained
+    on the windowed COM loss.  ``forward`` advances all robots ``T`` steps,
+    checkpointing particle state every ``K`` steps, one native call per
+    segment; ``backward`` recomputes each segment from its checkpoint and runs
+    the hand-written adjoints in reverse.  Weight gradients land in ``grads``
+    (W1, b1, W2, b2).  Particles are reordered by grid block (see
+    ``block_order``; ``order`` maps internal to given indices) -- the sensors,
+    COM and loss do not depend on particle order."""
 
     def __init__(self, params, positions, act_id, n_act, B, T, K=100, window=None,
                  dtype=torch.float32, n_sin=10, run_period=100, offset_scale=0.2,
-                 max_speed=0.08, activation="tanh", n_hidden=64, device="cuda"):
+                 max_speed=0.08, activation="tanh", n_hidden=64, device="cuda",
+                 sort_particles=True):
+        if params.dim != 3:
+            raise ValueError("MPMBatch drives 3D robots (the controller's sensors are 3D)")
+        if params.n_grid % 4:
+            raise ValueError("MPMBatch: n_grid must be a multiple of 4")
         self.p, self.B, self.T, self.K = params, B, T, K
-        self.D = D = params.dim
-        self.P = P = len(positions)
+        self.D = D = 3
+        pos = np.asarray(positions, np.float64)
+        ids = np.asarray(act_id, np.int32)
+        self.order = block_order(pos, params.n_grid) if sort_particles else np.arange(len(pos))
+        pos, ids = pos[self.order], ids[self.order]
+        self.positions = pos
+        self.P = P = len(pos)
         self.n_act = n_act
         self.window = window if window is not None else T // 2
         self.dtype, self.device = dtype, torch.device(device)
@@ -250,127 +315,122 @@ class MPMBatch:
         self.G = n_act + 1                                  # actuator groups + passive body
         self.I = n_sin + 3 + 3 * self.G + 4
         self.H, self.O = n_hidden, n_act
-        ids = np.asarray(act_id, np.int32)
         grp = np.where(ids >= 0, ids, n_act).astype(np.int32)
         counts = np.bincount(grp, minlength=self.G).astype(np.float64)
-        z = lambda *s: torch.zeros(*s, dtype=dtype, device=self.device)
-        self.group = torch.as_tensor(grp)[None].expand(B, -1).contiguous().to(self.device)
-        self.inv_count = torch.as_tensor(np.where(counts > 0, 1.0 / np.maximum(counts, 1), 0.0),
-                                         dtype=dtype)[None].expand(B, -1).contiguous().to(self.device)
-        self.step_ = MPMStep(params, B, P, n_act, ids, dtype, device)
-        self.x0 = _soa(positions, B, dtype, self.device)
-        nck = (T + K - 1) // K
-        self.ck = [z(nck, B, D, P), z(nck, B, D, P), z(nck, B, D * D, P), z(nck, B, D * D, P)]
-        self.seg = [z(K + 1, B, D, P), z(K + 1, B, D, P), z(K + 1, B, D * D, P), z(K + 1, B, D * D, P)]
-        self.inp = z(K, B, self.I)
-        self.act_t = z(K, B, self.O)
-        self.hid = z(K, B, self.H)
+        dev = self.device
+        z = lambda *s: torch.zeros(*s, dtype=dtype, device=dev)
+        zi = lambda *s: torch.zeros(*s, dtype=torch.int32, device=dev)
+        bt = lambda a, dt_: torch.as_tensor(a, dtype=dt_)[None].expand(B, *np.shape(a)).contiguous().to(dev)
+        self.act_id = bt(ids, torch.int32)
+        self.group = bt(grp, torch.int32)
+        self.inv_count = bt(np.where(counts > 0, 1.0 / np.maximum(counts, 1), 0.0), dtype)
+        ng = params.n_grid ** 3
+        nblk = (params.n_grid // 4) ** 3
+        self.grid, self.grid_vo, self.grid_vo_grad, self.grid_grad = (z(B, ng, 4) for _ in range(4))
+        self.blk_flag, self.blk_list, self.blk_count = zi(B, nblk), zi(B, nblk), zi(B)
+        self.err = zi(1)
+        NS = 3 * self.G + 6
+        self.sens = z(T + 1, B, NS)
         self.center, self.vcom = z(T + 1, B, 3), z(T + 1, B, 3)
-        self.targets = z(B, 5)
-        self.tv, self.th = z(B, 3), z(B)
-        self.loss = z(1)
-        self.loss_b = z(B)
-        self.adj = [[z(B, D, P), z(B, D, P), z(B, D * D, P), z(B, D * D, P)] for _ in range(2)]
-        self.act_b, self.in_b = z(B, self.O), z(B, self.I)
+        self.inp, self.hid, self.act_out = z(T, B, self.I), z(T, B, self.H), z(T, B, self.O)
+        self.act_grad, self.in_grad = z(B, self.O), z(T + 1, B, self.I)
         self.cg, self.vg = z(T + 1, B, 3), z(T + 1, B, 3)
-        self.W = [z(self.H, self.I), z(self.H), z(self.O, self.H), z(self.O)]
-        self.grads = [torch.zeros_like(w) for w in self.W]
-        self.work = z(B * (2 * self.H + self.O))
-        self.cur = [z(B, D, P), z(B, D, P), z(B, D * D, P), z(B, D * D, P)]
-        self.nxt = [torch.zeros_like(t) for t in self.cur]
+        H, I, O = self.H, self.I, self.O
+        self.nw = H * I + H + O * H + O
+        self.wgrad = z(B, self.nw)
+        self.W = [z(H, I), z(H), z(O, H), z(O)]
+        self.targets, self.tv, self.th = z(B, 5), z(B, 3), z(B)
+        self.loss, self.loss_b = z(1), z(B)
+        self.x0 = torch.as_tensor(np.ascontiguousarray(pos.T), dtype=dtype).to(dev)   # [3,P]
+        self.nck = (T + K - 1) // K
+        self.ring = _Slots(2, B, D, P, dtype, dev)
+        self.ck = _Slots(self.nck, B, D, P, dtype, dev)
+        self.seg = _Slots(K + 1, B, D, P, dtype, dev)
+        self.adj = _Slots(2, B, D, P, dtype, dev)
+        P_ = lambda t: nat.ptr(t)
+        self.desc = MPMBatchC(
+            dtype=self.dc, B=B, P=P, n_act=n_act, n_groups=self.G, n_sin=n_sin,
+            run_period=run_period, n_hidden=H, act=self.act, horizon=T,
+            offset_scale=offset_scale, max_speed=max_speed, params=params,
+            act_id=P_(self.act_id), group=P_(self.group), inv_count=P_(self.inv_count),
+            targets=P_(self.targets), W1=P_(self.W[0]), b1=P_(self.W[1]), W2=P_(self.W[2]),
+            b2=P_(self.W[3]), grid=P_(self.grid), grid_vo=P_(self.grid_vo),
+            grid_vo_grad=P_(self.grid_vo_grad), grid_grad=P_(self.grid_grad),
+            blk_flag=P_(self.blk_flag), blk_list=P_(self.blk_list), blk_count=P_(self.blk_count),
+            err=P_(self.err), sens=P_(self.sens), center=P_(self.center), vcom=P_(self.vcom),
+            inp=P_(self.inp), hid=P_(self.hid), act_out=P_(self.act_out),
+            act_grad=P_(self.act_grad), in_grad=P_(self.in_grad), center_grad=P_(self.cg),
+            vcom_grad=P_(self.vg), wgrad=P_(self.wgrad))
+        self.grads = None
 
     def set_targets(self, tv, th):
         t = np.zeros((self.B, 5))
         t[:, :3] = np.asarray(tv)
         t[:, 3] = np.asarray(th)
         self.targets.copy_(torch.as_tensor(t, dtype=self.dtype))
-        # contiguous copies held for the loss kernels (a temporary would be
-        # freed, and its memory reused, before the launch reads it)
-        self.tv = self.targets[:, 0:3].contiguous()
-        self.th = self.targets[:, 3].contiguous()
+        self.tv.copy_(self.targets[:, 0:3])
+        self.th.copy_(self.targets[:, 3])
 
     def set_weights(self, W1, b1, W2, b2):
         for d, s in zip(self.W, (W1, b1, W2, b2)):
             d.copy_(torch.as_tensor(s, dtype=self.dtype))
 
-    def _init_state(self, st):
-        D = self.D
-        st[0].copy_(self.x0)
-        st[1].zero_()
-        st[2].zero_()
-        st[3].zero_()
-        for i in range(D):
-            st[3][:, i * D + i] = 1.0
+    def active_nodes(self):
+        """Grid nodes inside some particle's stencil in the initial state (per robot)."""
+        base = np.floor(self.positions * self.p.n_grid - 0.5).astype(np.int64)
+        offs = np.stack(np.meshgrid(*[np.arange(3)] * 3, indexing="ij"), -1).reshape(-1, 3)
+        nodes = (base[:, None, :] + offs[None]).reshape(-1, 3)
+        n = self.p.n_grid
+        return len(np.unique((nodes[:, 0] * n + nodes[:, 1]) * n + nodes[:, 2]))
 
-    def _controller(self, t, x, v, inp, act, hid):
-        nat.call("dl_mpm_sensor", self.dc, self.B, self.P, self.G, self.n_sin, self.run_period, t,
-                 self.offset_scale, self.max_speed, nat.ptr(x), nat.ptr(v), nat.ptr(self.group),
-                 nat.ptr(self.targets), nat.ptr(self.inv_count), nat.ptr(inp),
-                 nat.ptr(self.center[t]), nat.ptr(self.vcom[t]), nat.stream())
-        nat.call("dl_mlp_forward", self.dc, self.act, self.B, self.I, self.H, self.O, nat.ptr(inp),
-                 *map(nat.ptr, self.W), nat.ptr(hid), nat.ptr(act), nat.stream())
+    def _segments(self):
+        return [(s * self.K, min(self.K, self.T - s * self.K)) for s in range(self.nck)]
 
     def forward(self):
-        st, nx = self.cur, self.nxt
-        self._init_state(st)
-        self.step_.err.zero_()
-        for t in range(self.T):
-            k = t % self.K
-            if k == 0:
-                for c, s in zip(self.ck, st):
-                    c[t // self.K].copy_(s)
-            self._controller(t, st[0], st[1], self.inp[k], self.act_t[k], self.hid[k])
-            self.step_.forward(*st, self.act_t[k], *nx)
-            st, nx = nx, st
-        # COM of the final state (loss term at T)
-        nat.call("dl_ms_center", self.dc, self.B, self.P, nat.ptr(st[0]), nat.ptr(st[1]),
-                 nat.ptr(self.center[self.T]), nat.ptr(self.vcom[self.T]), nat.stream())
-        self.final = st
+        r = self.ring
+        r.x[0].copy_(self.x0[None].expand(self.B, -1, -1))
+        r.v[0].zero_()
+        r.C[0].zero_()
+        r.F[0].zero_()
+        for i in range(3):
+            r.F[0][:, i * 3 + i] = 1.0
+        self.sens.zero_()
+        for s, (t0, n) in enumerate(self._segments()):
+            self.ck.copy_slot(s, 0, r)
+            flags = MPM_CONTROL | (MPM_SENSE_FIRST if s == 0 else 0)
+            nat.call("dl_mpm_forward", ctypes.byref(self.desc), ctypes.byref(r.c), t0, n, flags,
+                     nat.stream())
+            if n % 2:
+                r.copy_slot(0, 1)
         nat.call("dl_loss_task", self.dc, self.B, self.T, self.window, nat.ptr(self.center),
                  nat.ptr(self.vcom), nat.ptr(self.tv), nat.ptr(self.th), nat.ptr(self.loss),
                  nat.ptr(self.loss_b), nat.stream())
         return self.loss
 
     def backward(self, loss_grad=1.0):
-        B, P, T, K = self.B, self.P, self.T, self.K
-        for g in self.grads:
-            g.zero_()
+        B, T = self.B, self.T
         self.cg.zero_()
         self.vg.zero_()
+        self.in_grad.zero_()
+        self.wgrad.zero_()
+        for f in self.adj.fields():
+            f[0].zero_()
         nat.call("dl_loss_task_grad", self.dc, B, T, self.window, nat.ptr(self.center),
-                 nat.ptr(self.vcom), nat.ptr(self.tv), nat.ptr(self.th), float(loss_grad), nat.ptr(self.cg),
-                 nat.ptr(self.vg), nat.stream())
-        a1, a0 = self.adj
-        for t_ in a1:
-            t_.zero_()
-        nat.call("dl_ms_center_grad", self.dc, B, P, nat.ptr(self.cg[T]), nat.ptr(self.vg[T]),
-                 nat.ptr(a1[0]), nat.ptr(a1[1]), nat.stream())
-        nseg = (T + K - 1) // K
-        for seg in reversed(range(nseg)):
-            s0, s1 = seg * K, min(T, (seg + 1) * K)
-            for d, c in zip(self.seg, self.ck):
-                d[0].copy_(c[seg])
-            for t in range(s0, s1):
-                k = t - s0
-                st = [b[k] for b in self.seg]
-                nx = [b[k + 1] for b in self.seg]
-                self._controller(t, st[0], st[1], self.inp[k], self.act_t[k], self.hid[k])
-                self.step_.forward(*st, self.act_t[k], *nx)
-            for t in reversed(range(s0, s1)):
-                k = t - s0
-                st = [b[k] for b in self.seg]
-                self.act_b.zero_()
-                self.step_.backward(*st, self.act_t[k], *a1, *a0, self.act_b)
-                self.in_b.zero_()
-                nat.call("dl_mlp_backward", self.dc, self.act, B, self.I, self.H, self.O,
-                         nat.ptr(self.inp[k]), *map(nat.ptr, self.W), nat.ptr(self.act_b),
-                         nat.ptr(self.in_b), *map(nat.ptr, self.grads), nat.ptr(self.work),
-                         nat.stream())
-                nat.call("dl_mpm_sensor_adj", self.dc, B, P, self.G, self.n_sin, self.offset_scale,
-                         nat.ptr(self.group), nat.ptr(self.inv_count), nat.ptr(self.in_b),
-                         nat.ptr(a0[0]), nat.ptr(a0[1]), nat.stream())
-                nat.call("dl_ms_center_grad", self.dc, B, P, nat.ptr(self.cg[t]), nat.ptr(self.vg[t]),
-                         nat.ptr(a0[0]), nat.ptr(a0[1]), nat.stream())
-                a1, a0 = a0, a1
-        self.adj = [a1, a0]
+                 nat.ptr(self.vcom), nat.ptr(self.tv), nat.ptr(self.th), float(loss_grad),
+                 nat.ptr(self.cg), nat.ptr(self.vg), nat.stream())
+        for s, (t0, n) in reversed(list(enumerate(self._segments()))):
+            self.seg.copy_slot(0, s, self.ck)
+            nat.call("dl_mpm_forward", ctypes.byref(self.desc), ctypes.byref(self.seg.c), t0, n, 0,
+                     nat.stream())
+            nat.call("dl_mpm_backward", ctypes.byref(self.desc), ctypes.byref(self.seg.c),
+                     ctypes.byref(self.adj.c), t0, n, nat.stream())
+        g = self.wgrad.sum(0)
+        H, I, O = self.H, self.I, self.O
+        sizes = [H * I, H, O * H, O]
+        parts = torch.split(g, sizes)
+        self.grads = [parts[0].view(H, I), parts[1], parts[2].view(O, H), parts[3]]
         return self.grads
+
+    def check(self):
+        if int(self.err.item()) != 0:
+            raise nat.NativeError("MPM: a particle left the grid (stencil out of bounds)")

@@ -324,6 +324,72 @@ int32_t dl_mpm_sensor_adj(int32_t dtype, int32_t B, int32_t P, int32_t n_groups,
                           double offset_scale, const int32_t *group, const void *inv_count,
                           const void *in_grad, void *x_grad, void *v_grad, void *stream);
 
+/* ---------------------------------------------------------------- batched MPM rollout
+ * B robots of one particle design, each on its own grid, advanced (and
+ * differentiated) a segment of steps per call.  Replaces the per-step loop
+ * of SolverMPM.advance_core / advance_core_grad (solver_mpm.py:175-190) plus
+ * the controller (nn.py:130-150) and COM/sensor reductions
+ * (solver_mpm.py:167-173) of a training rollout, with the Taichi tape's
+ * recompute scheme (checkpoint -> recompute the segment -> adjoint).
+ *
+ * Grids: [B][n^D][4] records.  grid = (momentum (D), mass), grid_vo =
+ * velocity, grid_vo_grad its adjoint, grid_grad = adjoint of grid.  Active
+ * blocks (4^3 nodes in 3D, 8^2 in 2D): blk_flag, blk_list [B][(n/E)^D],
+ * blk_count [B].  Between calls grid, grid_vo_grad, blk_flag, blk_count and
+ * act_grad must be zero (allocate them zeroed; the calls keep them so).
+ * Per-step rows (index t = 0..horizon): sens [T+1][B][3*n_groups+6] (sums:
+ * per-group x, x, v; zeroed by the caller before a forward rollout),
+ * center/vcom [T+1][B][3], inp [T][B][I], hid [T][B][H], act_out [T][B][O],
+ * in_grad [T+1][B][I] (row T zero), center_grad/vcom_grad [T+1][B][3] (loss
+ * adjoints, e.g. from dl_loss_task_grad), wgrad [B][H*I + H + O*H + O]
+ * (per-robot slices, accumulated).  I = n_sin + 3 + 3*n_groups + 4. */
+enum { DL_MPM_CONTROL = 1, DL_MPM_SENSE_FIRST = 2 };
+
+typedef struct dl_mpm_batch {
+    int32_t dtype, B, P, n_act;
+    int32_t n_groups;        /* sensor groups (actuators + passive body) */
+    int32_t n_sin, run_period, n_hidden;
+    int32_t act;             /* DL_ACT_* of both MLP layers */
+    int32_t horizon;         /* T */
+    double offset_scale, max_speed;
+    dl_mpm_params params;
+    const int32_t *act_id;   /* [B][P], -1 = passive */
+    const int32_t *group;    /* [B][P] in [0, n_groups) */
+    const void *inv_count;   /* [B][n_groups] */
+    const void *targets;     /* [B][5]: tv (3), th, unused */
+    const void *W1, *b1, *W2, *b2;
+    void *grid, *grid_vo, *grid_vo_grad, *grid_grad;
+    int32_t *blk_flag, *blk_list, *blk_count;
+    int32_t *err;            /* |= 1 when a particle's stencil leaves the grid */
+    void *sens, *center, *vcom, *inp, *hid, *act_out;
+    void *act_grad;          /* [B][O] */
+    void *in_grad;           /* NULL: no controller adjoint (physics only) */
+    void *center_grad, *vcom_grad, *wgrad;
+} dl_mpm_batch;
+
+/* Ring of particle-state slots: slot s of x/v at x + s*B*D*P reals, of C/F
+ * at C + s*B*D*D*P reals. */
+typedef struct dl_mpm_slots {
+    void *x, *v, *C, *F;
+    int32_t n_slots, pad_;
+} dl_mpm_slots;
+
+/* Steps t0 .. t0+n-1: state t0 in slot 0, state t0+j in slot j % n_slots.
+ * DL_MPM_CONTROL: run the controller (center/vcom/inp/hid/act_out rows t,
+ * sensor sums of row t+1; center/vcom of row T when t0+n == T); without it
+ * act_out rows are read (segment recompute).  DL_MPM_SENSE_FIRST: first
+ * accumulate the sensor sums of state t0 (row t0). */
+int32_t dl_mpm_forward(const dl_mpm_batch *b, const dl_mpm_slots *states, int32_t t0,
+                       int32_t n, int32_t flags, void *stream);
+/* Adjoint of steps t0 .. t0+n-1 given the segment's states (n+1 slots) and
+ * the adjoint of state t0+n in slot 0 of `adj` (2 slots), without the
+ * sensor/COM terms of that state, which this call adds from in_grad,
+ * center_grad and vcom_grad rows t0+n.  On return slot 0 holds the adjoint
+ * of state t0 in the same convention; in_grad rows t0..t0+n-1 and wgrad are
+ * updated. */
+int32_t dl_mpm_backward(const dl_mpm_batch *b, const dl_mpm_slots *segment,
+                        const dl_mpm_slots *adj, int32_t t0, int32_t n, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

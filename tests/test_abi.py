@@ -9,6 +9,7 @@ from pathlib import Path
 import pytest
 
 from diffloco import _native as nat
+from diffloco import mpm as dm
 
 ROOT = Path(__file__).resolve().parents[1]
 HEADER = ROOT / "include" / "diffloco.h"
@@ -45,6 +46,9 @@ int main(void) {
   printf("%zu %zu %zu %zu\n", offsetof(dl_rollout, phys), offsetof(dl_rollout, lay),
          offsetof(dl_rollout, robots), offsetof(dl_rollout, traj_grad));
   printf("%zu %zu\n", offsetof(dl_robot, weights), offsetof(dl_robot, target_stride));
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(dl_mpm_params), sizeof(dl_mpm_batch),
+         sizeof(dl_mpm_slots), offsetof(dl_mpm_batch, params), offsetof(dl_mpm_batch, act_id),
+         offsetof(dl_mpm_batch, wgrad));
   return 0;
 }
 ''')
@@ -56,7 +60,9 @@ int main(void) {
            ctypes.sizeof(nat.MSTopology), ctypes.sizeof(nat.RobotC), ctypes.sizeof(nat.Rollout),
            nat.Rollout.phys.offset, nat.Rollout.lay.offset, nat.Rollout.robots.offset,
            nat.Rollout.traj_grad.offset, nat.RobotC.weights.offset,
-           nat.RobotC.target_stride.offset]
+           nat.RobotC.target_stride.offset,
+           ctypes.sizeof(dm.MPMParamsC), ctypes.sizeof(dm.MPMBatchC), ctypes.sizeof(dm.MPMSlotsC),
+           dm.MPMBatchC.params.offset, dm.MPMBatchC.act_id.offset, dm.MPMBatchC.wgrad.offset]
     assert got == exp
     assert nat.ROBOT_DTYPE.itemsize == got[3]
 
@@ -82,3 +88,9 @@ def test_invalid_arguments_are_rejected_without_a_gpu():
     assert "dtype" in nat.last_error()
     with pytest.raises(nat.NativeError):
         nat.call("dl_ms_center", 0, 0, 10, None, None, None, None, None)
+    # batched MPM: descriptor checks come first
+    assert L.dl_mpm_forward(None, None, 0, 1, 0, None) == -1
+    mb = dm.MPMBatchC(dtype=0, B=1, P=1, n_act=1, horizon=10,
+                      params=dm.quadruped3d_params(n_grid=30))
+    assert L.dl_mpm_forward(ctypes.byref(mb), None, 0, 1, 0, None) == -1
+    assert "multiple of 4" in nat.last_error()

@@ -122,42 +122,91 @@ def test_mpm_adjoint_matches_autograd(dim):
     assert rel_err(ab.cpu(), ar.grad) < 1e-8
 
 
-def test_mpm_batched_rollout_matches_oracle():
-    """Checkpointed MPM training rollout (sensors, MLP, P2G/grid/G2P, windowed
-    COM loss) and its adjoint through every segment, FP64, against autograd."""
-    _say("mpm batched rollout")
-    n = 16
+def _batched_case(offsets=((5.0, 3.2, 5.0),), n=16, side=5, perm_seed=None):
+    """Blobs of side^3 particles, half a cell apart, at the given cell
+    offsets; the 3D quadruped material at dt = 2e-4 (stable for these
+    blobs; the state stays well inside the grid)."""
     params = dm.quadruped3d_params(n_grid=n)
-    params.dt = 1e-3
-    side = torch.arange(5, dtype=torch.float64)
-    grid = torch.stack(torch.meshgrid(side, side, side, indexing="ij"), -1).reshape(-1, 3)
-    pos = (grid * 0.5 + torch.tensor([5.0, 3.2, 5.0])) / n
+    params.dt = 2e-4
+    ax = torch.arange(side, dtype=torch.float64)
+    grid = torch.stack(torch.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3)
+    pos = torch.cat([(grid * 0.5 + torch.tensor(o)) / n for o in offsets])
     ids = torch.tensor([(i % 5) - 1 for i in range(len(pos))])
-    B, T, K, W = 2, 24, 7, 12
-    batch = dm.MPMBatch(params, pos.numpy(), ids.numpy(), 4, B, T, K=K, window=W,
-                        dtype=torch.float64)
+    if perm_seed is not None:
+        perm = torch.randperm(len(pos), generator=torch.Generator().manual_seed(perm_seed))
+        pos, ids = pos[perm], ids[perm]
+    op = omp.MPMParams(dim=3, n_grid=n, dt=params.dt, gravity=params.gravity, p_vol=params.p_vol,
+                       p_mass=params.p_mass, mu=params.mu, la=params.la,
+                       act_strength=params.act_strength, act_axis=0, bound=3)
+    return params, op, pos, ids
+
+
+def _weights(I, seed=6):
+    g = torch.Generator().manual_seed(seed)
+    return [torch.randn(64, I, generator=g, dtype=torch.float64) / I ** 0.5,
+            0.1 * torch.randn(64, generator=g, dtype=torch.float64),
+            torch.randn(4, 64, generator=g, dtype=torch.float64) / 8.0,
+            0.1 * torch.randn(4, generator=g, dtype=torch.float64)]
+
+
+def _run_batched(params, op, pos, ids, B, T, K, W, dtype, sort=True):
+    batch = dm.MPMBatch(params, pos.numpy(), ids.numpy(), 4, B, T, K=K, window=W, dtype=dtype,
+                        sort_particles=sort)
     rng = np.random.default_rng(5)
     tv = np.stack([rng.uniform(-0.08, 0.08, B), np.zeros(B), rng.uniform(-0.08, 0.08, B)], 1)
     th = rng.uniform(0.1, 0.25, B)
     batch.set_targets(tv, th)
-    I = batch.I
-    g = torch.Generator().manual_seed(6)
-    Wt = [torch.randn(64, I, generator=g, dtype=torch.float64) / I ** 0.5,
-          0.1 * torch.randn(64, generator=g, dtype=torch.float64),
-          torch.randn(4, 64, generator=g, dtype=torch.float64) / 8.0,
-          0.1 * torch.randn(4, generator=g, dtype=torch.float64)]
+    Wt = _weights(batch.I)
     batch.set_weights(*[w.cuda() for w in Wt])
     loss = batch.forward()
     grads = batch.backward()
     torch.cuda.synchronize()
-    batch.step_.check()
-    op = omp.MPMParams(dim=3, n_grid=n, dt=params.dt, gravity=params.gravity, p_vol=params.p_vol,
-                       p_mass=params.p_mass, mu=params.mu, la=params.la,
-                       act_strength=params.act_strength, act_axis=0, bound=3)
+    batch.check()
     Wr = [w.clone().requires_grad_() for w in Wt]
     x0 = pos[None].expand(B, -1, -1).clone()
-    lo = omp.rollout_task(x0, ids[None].expand(B, -1), 4, op, Wr, torch.tensor(tv), torch.tensor(th), T, W)
+    lo = omp.rollout_task(x0, ids[None].expand(B, -1), 4, op, Wr, torch.tensor(tv),
+                          torch.tensor(th), T, W)
     lo.backward()
-    assert abs(loss.item() - lo.item()) / abs(lo.item()) < 1e-10
-    for a, b in zip(grads, Wr):
-        assert rel_err(a.cpu(), b.grad) < 1e-7
+    return batch, loss.item(), grads, lo.item(), [w.grad for w in Wr]
+
+
+@pytest.mark.parametrize("prec,K", [("f64", 7), ("f64", 24), ("f32", 7)])
+def test_mpm_batched_rollout_matches_oracle(prec, K):
+    """Checkpointed MPM training rollout (sensors, MLP, P2G/grid/G2P, windowed
+    COM loss) and its adjoint through every segment, against autograd of the
+    FP64 oracle.  Particles are block-sorted internally (order-independent
+    results); K = 7 exercises partial segments and odd segment lengths."""
+    _say(f"mpm batched rollout {prec} K={K}")
+    params, op, pos, ids = _batched_case()
+    B, T, W = 2, 24, 12
+    batch, loss, grads, lo, gref = _run_batched(params, op, pos, ids, B, T, K, W, DT[prec])
+    tl, tg = (1e-10, 1e-7) if prec == "f64" else (1e-4, 1e-3)
+    assert abs(loss - lo) / abs(lo) < tl
+    for a, b in zip(grads, gref):
+        assert rel_err(a.double().cpu(), b) < tg
+
+
+@pytest.mark.parametrize("case", ["slabs", "direct"])
+def test_mpm_batched_scatter_paths_and_invariants(case):
+    """Two blobs far apart in one workgroup (shuffled, unsorted), so the
+    workgroup's node box exceeds the LDS tile: "slabs" (box 21 x 5 x 21)
+    scatters in several x-slab passes through the tile, "direct" (a y-z plane
+    of 45 x 45 nodes) with direct global atomics.  Results match the oracle,
+    and the grid, active-block and actuation-adjoint buffers are zero again
+    afterwards."""
+    _say(f"mpm batched scatter path {case}")
+    if case == "slabs":
+        params, op, pos, ids = _batched_case(offsets=((5.0, 3.2, 5.0), (20.0, 3.2, 20.0)), n=32,
+                                             perm_seed=3)
+    else:
+        params, op, pos, ids = _batched_case(offsets=((5.0, 4.0, 5.0), (5.0, 44.0, 44.0)), n=64,
+                                             perm_seed=3)
+    B, T, K, W = 2, 12, 5, 6
+    batch, loss, grads, lo, gref = _run_batched(params, op, pos, ids, B, T, K, W, torch.float64,
+                                                sort=False)
+    assert abs(loss - lo) / abs(lo) < 1e-10
+    for a, b in zip(grads, gref):
+        assert rel_err(a.double().cpu(), b) < 1e-7
+    for t in (batch.grid, batch.grid_vo_grad, batch.act_grad):
+        assert t.abs().max().item() == 0.0
+    assert batch.blk_flag.abs().max().item() == 0 and batch.blk_count.abs().max().item() == 0
