@@ -54,7 +54,7 @@ def main():
     ap.add_argument("--ckpt", type=int, default=50)
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--plain-robots", type=int, default=2)
-    ap.add_argument("--cpu-robots", type=int, default=32)
+    ap.add_argument("--cpu-robots", type=int, default=96)
     ap.add_argument("--cpu-horizon", type=int, default=1000)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
