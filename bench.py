@@ -63,6 +63,8 @@ def main():
     ap.add_argument("--mpm-horizon", type=int, default=2000)
     ap.add_argument("--mpm-ckpt", type=int, default=100)
     ap.add_argument("--mpm-steps", type=int, default=1)
+    ap.add_argument("--mpm-scatter", type=int, default=0, help="0: tile atomics, 1: store + gather")
+    ap.add_argument("--mpm-tile", type=int, default=4096, help="LDS scatter tile, nodes")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer 4 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -170,7 +172,8 @@ def main():
         Bm, Tm, Km = args.mpm_robots, args.mpm_horizon, args.mpm_ckpt
         pos, ids, n_act = dm.quadruped3d_particles(cells_per_cube=6, n_grid=64)
         mb = dm.MPMBatch(dm.quadruped3d_params(64), pos, ids, n_act, Bm, Tm, K=Km,
-                         window=Tm // 2, dtype=torch.float32)
+                         window=Tm // 2, dtype=torch.float32, scatter=args.mpm_scatter,
+                         tile_nodes=args.mpm_tile)
         rngm = np.random.default_rng(7)
         mb.set_targets(np.stack([rngm.uniform(-0.08, 0.08, Bm), np.zeros(Bm),
                                  rngm.uniform(-0.08, 0.08, Bm)], 1), rngm.uniform(0.1, 0.25, Bm))
@@ -200,6 +203,8 @@ def main():
         bpp = 684 + 224 * gp + 96 / Km          # algorithmic bytes per particle-step, fwd+bwd
         mpm = {"value": pps, "unit": "particle-steps/s", "robots": Bm, "particles": P,
                "grid": 64, "horizon": Tm, "ckpt_every": Km, "dtype": "f32",
+               "scatter": ["tile atomics", "tile store + gather"][args.mpm_scatter],
+               "tile_nodes": args.mpm_tile,
                "ms_forward": float(np.mean(mf)), "ms_backward": float(np.mean(mbk)),
                "active_nodes_per_particle": gp,
                "roofline": {"bound": "hbm", "achieved": bpp * pps / 1e9, "peak": HBM_PEAK_GBS,

@@ -328,7 +328,7 @@ template <typename T>
 __global__ void k_mpm_sensor(int P, int Gs, int n_sin, int run_period, int t, T scale, T max_speed,
                              const T *x, const T *v, const int32_t *group, const T *tgt,
                              const T *inv_count, T *in, T *center, T *vcom) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     T *acc = reinterpret_cast<T *>(smem_raw);      // [3*Gs + 6]
     T *red = acc + 3 * Gs + 6;
     const int k = blockIdx.x;

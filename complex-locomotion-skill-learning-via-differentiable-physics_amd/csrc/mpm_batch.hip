@@ -32,9 +32,26 @@
 
 namespace dl {
 
-constexpr int TILE_NODES = 1024;     // LDS tile capacity, nodes
-constexpr int PB = 256;              // particles per workgroup
+constexpr int TILE_MAX = DL_MPM_TILE_MAX;       // largest LDS tile, nodes
+constexpr int TILE_CAP = DL_MPM_TILE_CAP;       // tiles listed per block
+constexpr int PB = DL_MPM_PB;                   // particles per workgroup
 constexpr int NWAVE = PB / 64;
+constexpr int MAX_TILE_BLOCKS = 256;            // blocks a stored tile may overlap
+
+// Stored scatter tiles of one robot: tile w at tiles + w*tn*4 with
+// its node box (lo[3], ext[3]) at hdr + 8w; per block the ids of the tiles
+// overlapping it (count, ids[TILE_CAP]).
+template <typename T> struct TileBuf {
+    T *tiles;
+    int32_t *hdr, *cnt, *ids;
+    int wpr;                                    // workgroups (tiles) per robot
+    int tn;                                     // tile capacity, nodes
+    int store;                                  // commit tiles by store + gather
+    __device__ TileBuf robot(int k, int nblk) const {
+        return {tiles + (size_t)k * wpr * tn * 4, hdr + (size_t)k * wpr * 8,
+                cnt + (size_t)k * nblk, ids + (size_t)k * nblk * TILE_CAP, wpr, tn, store};
+    }
+};
 
 template <int D> struct BlkE { static constexpr int E = D == 3 ? 4 : 8; };
 
@@ -145,9 +162,10 @@ __device__ __forceinline__ void mark_box(const Blk &bk, int k, const int lo[D], 
 
 // Node of lane `lane` in block `bid`; sets bnd for sticky boundary nodes.
 template <int D, typename T>
-__device__ __forceinline__ int block_node(int bid, int lane, int nb, const MP<T> &p, bool &bnd) {
+__device__ __forceinline__ int block_node(int bid, int lane, int nb, const MP<T> &p, bool &bnd,
+                                          int c[D]) {
     constexpr int E = BlkE<D>::E;
-    int rem = bid, lr = lane, c[D];
+    int rem = bid, lr = lane;
 #pragma unroll
     for (int d = D - 1; d >= 0; --d) {
         c[d] = (rem % nb) * E + lr % E;
@@ -182,6 +200,106 @@ __device__ __forceinline__ void keyed_wave_add(bool valid, int key, const T (&v)
             if (lane == leader) atomicAdd(dst + kl * stride + i, s);
         }
         pend &= ~__ballot(mine);
+    }
+}
+
+// Store a workgroup's whole LDS tile (nodes <= tile capacity) with plain
+// coalesced writes and list it under every block its box overlaps; the
+// grid pass then gathers the listed tiles, so no global atomics are needed.
+// A block whose list is full gets this tile's nodes by global atomics
+// instead (into gk).  Returns false, storing nothing, when the box overlaps
+// more than MAX_TILE_BLOCKS blocks (the caller then commits by atomics).
+// Workgroup-uniform; called after the tile is complete (post barrier).
+template <int D, typename T>
+__device__ __forceinline__ bool store_tile(const T *tile, int nodes, const int lo[D],
+                                           const int ext[D], const TileBuf<T> &tb, int nb,
+                                           T *gk, int n) {
+    constexpr int E = BlkE<D>::E;
+    int b0[D], nbt[D], tot = 1;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        b0[d] = lo[d] / E;
+        nbt[d] = (lo[d] + ext[d] - 1) / E - b0[d] + 1;
+        tot *= nbt[d];
+    }
+    if (tot > MAX_TILE_BLOCKS) return false;
+    __shared__ unsigned int full_mask[MAX_TILE_BLOCKS / 32];
+    __shared__ int any_full;
+    const int w = blockIdx.x;
+    T *dst = tb.tiles + (size_t)w * tb.tn * 4;
+    for (int i = threadIdx.x; i < nodes; i += blockDim.x) st4(dst + (size_t)i * 4, tile + i * 4);
+    if (threadIdx.x < D) {
+        tb.hdr[w * 8 + threadIdx.x] = lo[threadIdx.x];
+        tb.hdr[w * 8 + 3 + threadIdx.x] = ext[threadIdx.x];
+    }
+    if (threadIdx.x < MAX_TILE_BLOCKS / 32) full_mask[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) any_full = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+        int rem = i, bid = 0, c[D];
+#pragma unroll
+        for (int d = D - 1; d >= 0; --d) {
+            c[d] = b0[d] + rem % nbt[d];
+            rem /= nbt[d];
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) bid = bid * nb + c[d];
+        const int slot = atomicAdd(tb.cnt + bid, 1);
+        if (slot < TILE_CAP) {
+            tb.ids[(size_t)bid * TILE_CAP + slot] = w;
+        } else {
+            atomicOr(full_mask + (i >> 5), 1u << (i & 31));
+            any_full = 1;
+        }
+    }
+    __syncthreads();
+    if (any_full) {
+        for (int i = threadIdx.x; i < nodes; i += blockDim.x) {
+            int rem = i, node = 0, c[D], bl = 0;
+#pragma unroll
+            for (int d = D - 1; d >= 0; --d) {
+                c[d] = lo[d] + rem % ext[d];
+                rem /= ext[d];
+            }
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                node = node * n + c[d];
+                bl = bl * nbt[d] + c[d] / E - b0[d];
+            }
+            if (full_mask[bl >> 5] & (1u << (bl & 31))) {
+#pragma unroll
+                for (int cc = 0; cc < 4; ++cc) {
+                    const T r = tile[i * 4 + cc];
+                    if (r != T(0)) gadd(gk + (size_t)node * 4 + cc, r);
+                }
+            }
+        }
+    }
+    return true;
+}
+
+// Sum of the stored tiles listed under block `bid` at node coordinates c.
+template <int D, typename T>
+__device__ __forceinline__ void gather_tiles(const TileBuf<T> &tb, int bid, const int c[D],
+                                             T r[4]) {
+    const int nt = min(tb.cnt[bid], TILE_CAP);
+    for (int s = 0; s < nt; ++s) {
+        const int w = tb.ids[(size_t)bid * TILE_CAP + s];
+        const int32_t *h = tb.hdr + w * 8;
+        int li = 0;
+        bool in = true;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int q = c[d] - h[d];
+            in = in && q >= 0 && q < h[3 + d];
+            li = li * h[3 + d] + q;
+        }
+        if (in) {
+            T t4[4];
+            ld4(tb.tiles + (size_t)w * tb.tn * 4 + (size_t)li * 4, t4);
+#pragma unroll
+            for (int cc = 0; cc < 4; ++cc) r[cc] += t4[cc];
+        }
     }
 }
 
@@ -248,14 +366,15 @@ __device__ __forceinline__ T run_sum(T v, int dist) {
 // run's tail lane adds to memory: LDS atomics are paid per lane, so this
 // divides their cost by the particles per cell.  The sums go through the
 // LDS tile: the node box (lo, ext) is cut into x-slabs of at most
-// TILE_NODES nodes, each accumulated in LDS and committed with one global
+// tb.tn nodes, each accumulated in LDS and committed with one global
 // atomic per nonzero component; a box whose single x-plane exceeds the tile
 // is scattered with direct global atomics.  val(off, v) gives the values of
 // stencil offset off.  Every thread of the workgroup must call it
 // (uniform control flow, all lanes active for the lane shifts).
 template <int D, typename T, int NC, typename F>
 __device__ __forceinline__ void tile_scatter(bool valid, const Stencil<D, T> &s, const int lo[D],
-                                             const int ext[D], T *tile, T *gk, int n, F &&val) {
+                                             const int ext[D], T *tile, T *gk, int n,
+                                             const TileBuf<T> &tb, int nb, F &&val) {
     constexpr int K = D == 2 ? 9 : 27;
     int key = 0;
 #pragma unroll
@@ -265,8 +384,8 @@ __device__ __forceinline__ void tile_scatter(bool valid, const Stencil<D, T> &s,
     int plane = 1;
 #pragma unroll
     for (int d = 1; d < D; ++d) plane *= ext[d];
-    const bool direct = plane > TILE_NODES;
-    const int sx = direct ? ext[0] : min(ext[0], TILE_NODES / plane);
+    const bool direct = plane > tb.tn;
+    const int sx = direct ? ext[0] : min(ext[0], tb.tn / plane);
     for (int x0 = 0; x0 < ext[0]; x0 += sx) {
         const int nx = min(sx, ext[0] - x0), nodes = nx * plane;
         if (!direct) {
@@ -302,6 +421,8 @@ __device__ __forceinline__ void tile_scatter(bool valid, const Stencil<D, T> &s,
         }
         if (direct) continue;
         __syncthreads();
+        if (tb.store && nx == ext[0] && store_tile<D, T>(tile, nodes, lo, ext, tb, nb, gk, n))
+            break;
         for (int i = threadIdx.x; i < nodes; i += blockDim.x) {
             int rem = i, c[D];
 #pragma unroll
@@ -350,8 +471,9 @@ __device__ __forceinline__ void sense_block(bool valid, int g, const T nx[3], co
 template <int D, typename T, bool STORE_F>
 __global__ __launch_bounds__(PB) void k_p2g_b(MP<T> p, PView<D, T> pv, const T *act,
                                               const int32_t *act_id, int n_act, T *F_out,
-                                              T *grid, Blk bk, int32_t *err) {
-    __shared__ T tile[TILE_NODES * 4];
+                                              T *grid, Blk bk, int32_t *err, TileBuf<T> tb) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *tile = reinterpret_cast<T *>(smem_raw);     // [tb.tn][4]
     __shared__ int red[2 * D * NWAVE];
     const int k = blockIdx.y, q = blockIdx.x * PB + threadIdx.x, P = pv.P;
     bool valid = q < P;
@@ -380,7 +502,8 @@ __global__ __launch_bounds__(PB) void k_p2g_b(MP<T> p, PView<D, T> pv, const T *
     int lo[D], ext[D];
     if (!block_bbox<D>(valid, s.base, lo, ext, red)) return;
     T *gk = grid + (size_t)k * gsize<D>(p.n) * 4;
-    tile_scatter<D, T, 4>(valid, s, lo, ext, tile, gk, p.n, [&](const int off[D], T v[4]) {
+    tile_scatter<D, T, 4>(valid, s, lo, ext, tile, gk, p.n, tb.robot(k, bk.nblk), bk.nb,
+                          [&](const int off[D], T v[4]) {
         T W = T(1), dpos[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
@@ -405,8 +528,10 @@ __global__ __launch_bounds__(PB) void k_p2g_b(MP<T> p, PView<D, T> pv, const T *
 // Velocity of every node of the active blocks; with `clear`, the momentum /
 // mass records are zeroed after use (forward); flags are always reset.
 template <int D, typename T>
-__global__ __launch_bounds__(PB) void k_grid_b(MP<T> p, T *grid, T *gvo, Blk bk, int clear) {
+__global__ __launch_bounds__(PB) void k_grid_b(MP<T> p, T *grid, T *gvo, Blk bk, int clear,
+                                               TileBuf<T> tb) {
     const int k = blockIdx.y, lane = threadIdx.x & 63;
+    const TileBuf<T> tbk = tb.robot(k, bk.nblk);
     const int nw = gridDim.x * NWAVE;
     const int cnt = bk.count[k];
     const int32_t *lst = bk.list + (size_t)k * bk.nblk;
@@ -415,9 +540,11 @@ __global__ __launch_bounds__(PB) void k_grid_b(MP<T> p, T *grid, T *gvo, Blk bk,
     for (int e = blockIdx.x * NWAVE + (threadIdx.x >> 6); e < cnt; e += nw) {
         const int bid = lst[e];
         bool bnd;
-        const int node = block_node<D, T>(bid, lane, bk.nb, p, bnd);
+        int c[D];
+        const int node = block_node<D, T>(bid, lane, bk.nb, p, bnd, c);
         T r[4], o[4] = {T(0), T(0), T(0), T(0)};
         ld4(gk + (size_t)node * 4, r);
+        gather_tiles<D, T>(tbk, bid, c, r);
         const T inv_m = T(1) / (r[3] + T(1e-10));
 #pragma unroll
         for (int d = 0; d < D; ++d) {
@@ -429,8 +556,13 @@ __global__ __launch_bounds__(PB) void k_grid_b(MP<T> p, T *grid, T *gvo, Blk bk,
         if (clear) {
             const T z[4] = {T(0), T(0), T(0), T(0)};
             st4(gk + (size_t)node * 4, z);
+        } else {
+            st4(gk + (size_t)node * 4, r);      // totals, read by the grid adjoint
         }
-        if (lane == 0) bk.flag[(size_t)k * bk.nblk + bid] = 0;
+        if (lane == 0) {
+            bk.flag[(size_t)k * bk.nblk + bid] = 0;
+            tbk.cnt[bid] = 0;
+        }
     }
 }
 
@@ -445,7 +577,7 @@ template <int D, typename T, bool SENSE>
 __global__ __launch_bounds__(PB) void k_g2p_b(MP<T> p, int P, const T *x, const T *gvo, T *x_out,
                                               T *v_out, T *C_out, int32_t *count,
                                               SenseOut<T> so) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     T *acc = reinterpret_cast<T *>(smem_raw);
     const int k = blockIdx.y, q = blockIdx.x * PB + threadIdx.x;
     if (SENSE) {
@@ -514,7 +646,7 @@ __global__ __launch_bounds__(PB) void k_g2p_b(MP<T> p, int P, const T *x, const 
 // Sensor sums of a stored state (the first step of a rollout).
 template <typename T>
 __global__ __launch_bounds__(PB) void k_sense(int P, const T *x, const T *v, SenseOut<T> so) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     T *acc = reinterpret_cast<T *>(smem_raw);
     const int k = blockIdx.y, q = blockIdx.x * PB + threadIdx.x;
     for (int i = threadIdx.x; i < 3 * so.Gs + 6; i += PB) acc[i] = T(0);
@@ -547,7 +679,7 @@ template <typename T> struct Ctrl {
 template <typename T>
 __global__ __launch_bounds__(64) void k_ctrl(Ctrl<T> c, int t, const T *sens, T *center, T *vcom,
                                              T *inp, T *hid, T *out, int mlp) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     T *si = reinterpret_cast<T *>(smem_raw);
     T *shd = si + c.I;
     const int k = blockIdx.x, tid = threadIdx.x;
@@ -604,7 +736,7 @@ __global__ __launch_bounds__(64) void k_ctrl(Ctrl<T> c, int t, const T *sens, T 
 template <typename T>
 __global__ __launch_bounds__(64) void k_ctrl_adj(Ctrl<T> c, const T *inp, T *act_b, T *in_b,
                                                  T *wg) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     T *si = reinterpret_cast<T *>(smem_raw);
     T *shd = si + c.I;
     T *sd1 = shd + c.H;
@@ -676,8 +808,9 @@ template <typename T> struct SenseAdj {
 template <int D, typename T>
 __global__ __launch_bounds__(PB) void k_g2p_adj_b(MP<T> p, int P, const T *x, const T *gvo,
                                                   const T *xb1, const T *vb1, const T *Cb1, T *xb,
-                                                  T *gvo_b, SenseAdj<T> sa) {
-    __shared__ T tile[TILE_NODES * 4];
+                                                  T *gvo_b, SenseAdj<T> sa, Blk bk, TileBuf<T> tb) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *tile = reinterpret_cast<T *>(smem_raw);     // [tb.tn][4]
     __shared__ int red[2 * D * NWAVE];
     const int k = blockIdx.y, q = blockIdx.x * PB + threadIdx.x;
     const bool valid = q < P;
@@ -772,7 +905,7 @@ __global__ __launch_bounds__(PB) void k_g2p_adj_b(MP<T> p, int P, const T *x, co
     int lo[D], ext[D];
     if (block_bbox<D>(inside, s.base, lo, ext, red)) {
         tile_scatter<D, T, D>(inside, s, lo, ext, tile, gvo_b + (size_t)k * G * 4, p.n,
-                              [&](const int off[D], T v[D]) {
+                              tb.robot(k, bk.nblk), bk.nb, [&](const int off[D], T v[D]) {
             T W = T(1), dp[D];
 #pragma unroll
             for (int d = 0; d < D; ++d) {
@@ -797,8 +930,10 @@ __global__ __launch_bounds__(PB) void k_g2p_adj_b(MP<T> p, int P, const T *x, co
 // Grid adjoint over the active blocks: dL/d(momentum, mass) from dL/d vo;
 // clears the grid and vo-adjoint records it consumed.
 template <int D, typename T>
-__global__ __launch_bounds__(PB) void k_grid_adj_b(MP<T> p, T *grid, T *gvo_b, T *grid_b, Blk bk) {
+__global__ __launch_bounds__(PB) void k_grid_adj_b(MP<T> p, T *grid, T *gvo_b, T *grid_b, Blk bk,
+                                                   TileBuf<T> tb) {
     const int k = blockIdx.y, lane = threadIdx.x & 63;
+    const TileBuf<T> tbk = tb.robot(k, bk.nblk);
     const int nw = gridDim.x * NWAVE;
     const int cnt = bk.count[k];
     const int32_t *lst = bk.list + (size_t)k * bk.nblk;
@@ -808,10 +943,13 @@ __global__ __launch_bounds__(PB) void k_grid_adj_b(MP<T> p, T *grid, T *gvo_b, T
     const T z[4] = {T(0), T(0), T(0), T(0)};
     for (int e = blockIdx.x * NWAVE + (threadIdx.x >> 6); e < cnt; e += nw) {
         bool bnd;
-        const int node = block_node<D, T>(lst[e], lane, bk.nb, p, bnd);
+        int c[D];
+        const int bid = lst[e];
+        const int node = block_node<D, T>(bid, lane, bk.nb, p, bnd, c);
         T r[4], ob[4], o[4] = {T(0), T(0), T(0), T(0)};
         ld4(gk + (size_t)node * 4, r);
         ld4(obk + (size_t)node * 4, ob);
+        gather_tiles<D, T>(tbk, bid, c, ob);
         const T m = r[3] + T(1e-10);
         T mb = T(0);
 #pragma unroll
@@ -824,6 +962,7 @@ __global__ __launch_bounds__(PB) void k_grid_adj_b(MP<T> p, T *grid, T *gvo_b, T
         st4(ibk + (size_t)node * 4, o);
         st4(gk + (size_t)node * 4, z);
         st4(obk + (size_t)node * 4, z);
+        if (lane == 0) tbk.cnt[bid] = 0;
     }
 }
 
@@ -834,7 +973,7 @@ __global__ __launch_bounds__(PB) void k_p2g_adj_b(MP<T> p, PView<D, T> pv, const
                                                   const int32_t *act_id, int n_act,
                                                   const T *grid_b, const T *Fb1, T *xb, T *vb,
                                                   T *Cb, T *Fb, T *act_b, int32_t *count) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     T *sab = reinterpret_cast<T *>(smem_raw);     // [n_act]
     const int k = blockIdx.y, q = blockIdx.x * PB + threadIdx.x, P = pv.P;
     for (int i = threadIdx.x; i < n_act; i += PB) sab[i] = T(0);
@@ -948,9 +1087,16 @@ int check_batch(const dl_mpm_batch *b) {
                  "MPM batch: n_grid must be a multiple of %d in [8, 1024]", E);
     DL_CHECK_ARG(c.act_axis >= 0 && c.act_axis < c.dim, "MPM batch: bad actuation axis");
     DL_CHECK_ARG(b->dtype == DL_F32 || b->dtype == DL_F64, "MPM batch: bad dtype");
+    DL_CHECK_ARG(b->tile_nodes >= 64 && b->tile_nodes <= DL_MPM_TILE_MAX &&
+                     b->tile_nodes * 4 * (b->dtype == DL_F64 ? 8 : 4) <= 128 * 1024,
+                 "MPM batch: tile_nodes %d outside [64, %d] or over 128 KiB of LDS",
+                 b->tile_nodes, (int)DL_MPM_TILE_MAX);
+    DL_CHECK_ARG(b->scatter == DL_MPM_SCATTER_ATOMIC || b->scatter == DL_MPM_SCATTER_STORE,
+                 "MPM batch: bad scatter mode %d", b->scatter);
     DL_CHECK_ARG(b->B > 0 && b->P > 0 && b->n_act > 0 && b->horizon > 0, "MPM batch: bad sizes");
     DL_CHECK_ARG(b->act_id && b->grid && b->grid_vo && b->blk_flag && b->blk_list &&
-                     b->blk_count && b->err && b->act_out,
+                     b->blk_count && b->err && b->act_out && b->tiles && b->tile_hdr &&
+                     b->blk_tile_count && b->blk_tiles,
                  "MPM batch: NULL buffer");
     return 0;
 }
@@ -996,6 +1142,11 @@ template <int D, typename T> Slot<D, T> slot(const dl_mpm_slots *s, int i, int B
     return {(T *)s->x + i * sv, (T *)s->v + i * sv, (T *)s->C + i * sm, (T *)s->F + i * sm};
 }
 
+template <typename T> TileBuf<T> make_tiles(const dl_mpm_batch *b) {
+    return {(T *)b->tiles, b->tile_hdr, b->blk_tile_count, b->blk_tiles, (b->P + PB - 1) / PB,
+            b->tile_nodes, b->scatter == DL_MPM_SCATTER_STORE};
+}
+
 int grid_wgs(const Blk &k) { return std::max(1, std::min(64, (k.nblk + NWAVE - 1) / NWAVE)); }
 
 template <int D, typename T>
@@ -1008,6 +1159,10 @@ int forward_impl(const dl_mpm_batch *b, const dl_mpm_slots *st, int t0, int n, i
     const dim3 pg((P + PB - 1) / PB, B), gg(grid_wgs(bk), B);
     const int I = n_in(b), H = b->n_hidden;
     const T *act = (const T *)b->act_out;
+    const TileBuf<T> tb = make_tiles<T>(b);
+    const size_t tile_bytes = (size_t)tb.tn * 4 * sizeof(T);
+    DL_CHECK_HIP(hipFuncSetAttribute((const void *)k_p2g_b<D, T, true>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)tile_bytes));
     if (control && (flags & DL_MPM_SENSE_FIRST)) {
         const Slot<D, T> s0 = slot<D, T>(st, 0, B, P);
         if constexpr (D == 3)
@@ -1026,10 +1181,11 @@ int forward_impl(const dl_mpm_batch *b, const dl_mpm_slots *st, int t0, int n, i
                                (T *)b->inp + (size_t)t * B * I, (T *)b->hid + (size_t)t * B * H,
                                (T *)b->act_out + (size_t)t * B * O, 1);
         PView<D, T> pv{a.x, a.v, a.C, a.F, P};
-        hipLaunchKernelGGL((k_p2g_b<D, T, true>), pg, dim3(PB), 0, s, mp, pv,
-                           act + (size_t)t * B * O, b->act_id, O, c.F, (T *)b->grid, bk, b->err);
+        hipLaunchKernelGGL((k_p2g_b<D, T, true>), pg, dim3(PB), tile_bytes, s, mp, pv,
+                           act + (size_t)t * B * O, b->act_id, O, c.F, (T *)b->grid, bk, b->err,
+                           tb);
         hipLaunchKernelGGL((k_grid_b<D, T>), gg, dim3(PB), 0, s, mp, (T *)b->grid, (T *)b->grid_vo,
-                           bk, 1);
+                           bk, 1, tb);
         if (control && D == 3)
             hipLaunchKernelGGL((k_g2p_b<D, T, true>), pg, dim3(PB), NS * sizeof(T), s, mp, P, a.x,
                                (const T *)b->grid_vo, c.x, c.v, c.C, b->blk_count,
@@ -1060,17 +1216,23 @@ int backward_impl(const dl_mpm_batch *b, const dl_mpm_slots *seg, const dl_mpm_s
     const int I = n_in(b), H = b->n_hidden;
     const Ctrl<T> ctrl = make_ctrl<T>(b);
     const T *act = (const T *)b->act_out;
+    const TileBuf<T> tb = make_tiles<T>(b);
+    const size_t tile_bytes = (size_t)tb.tn * 4 * sizeof(T);
+    DL_CHECK_HIP(hipFuncSetAttribute((const void *)k_p2g_b<D, T, false>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)tile_bytes));
+    DL_CHECK_HIP(hipFuncSetAttribute((const void *)k_g2p_adj_b<D, T>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)tile_bytes));
     int cur = 0;
     for (int j = n - 1; j >= 0; --j) {
         const int t = t0 + j, nxt = 1 - cur;
         const Slot<D, T> a = slot<D, T>(seg, j, B, P);
         const Slot<D, T> A1 = slot<D, T>(adj, cur, B, P), A0 = slot<D, T>(adj, nxt, B, P);
         PView<D, T> pv{a.x, a.v, a.C, a.F, P};
-        hipLaunchKernelGGL((k_p2g_b<D, T, false>), pg, dim3(PB), 0, s, mp, pv,
+        hipLaunchKernelGGL((k_p2g_b<D, T, false>), pg, dim3(PB), tile_bytes, s, mp, pv,
                            act + (size_t)t * B * O, b->act_id, O, (T *)nullptr, (T *)b->grid, bk,
-                           b->err);
+                           b->err, tb);
         hipLaunchKernelGGL((k_grid_b<D, T>), gg, dim3(PB), 0, s, mp, (T *)b->grid, (T *)b->grid_vo,
-                           bk, 0);
+                           bk, 0, tb);
         SenseAdj<T> sa{};
         if (D == 3 && b->in_grad) {
             sa.in_b = (const T *)b->in_grad + (size_t)(t + 1) * B * I;
@@ -1083,10 +1245,11 @@ int backward_impl(const dl_mpm_batch *b, const dl_mpm_slots *seg, const dl_mpm_s
             sa.I = I;
             sa.scale = T(b->offset_scale);
         }
-        hipLaunchKernelGGL((k_g2p_adj_b<D, T>), pg, dim3(PB), 0, s, mp, P, a.x,
-                           (const T *)b->grid_vo, A1.x, A1.v, A1.C, A0.x, (T *)b->grid_vo_grad, sa);
+        hipLaunchKernelGGL((k_g2p_adj_b<D, T>), pg, dim3(PB), tile_bytes, s, mp, P, a.x,
+                           (const T *)b->grid_vo, A1.x, A1.v, A1.C, A0.x, (T *)b->grid_vo_grad, sa,
+                           bk, tb);
         hipLaunchKernelGGL((k_grid_adj_b<D, T>), gg, dim3(PB), 0, s, mp, (T *)b->grid,
-                           (T *)b->grid_vo_grad, (T *)b->grid_grad, bk);
+                           (T *)b->grid_vo_grad, (T *)b->grid_grad, bk, tb);
         hipLaunchKernelGGL((k_p2g_adj_b<D, T>), pg, dim3(PB), O * sizeof(T), s, mp, pv,
                            act + (size_t)t * B * O, b->act_id, O, (const T *)b->grid_grad, A1.F,
                            A0.x, A0.v, A0.C, A0.F, (T *)b->act_grad, b->blk_count);

@@ -358,7 +358,7 @@ __device__ void fwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int t, in
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_rollout_fwd(RArgs<T> A) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     const int r = blockIdx.x;
     Rob<T> R = make_rob(A, r);
     Sm<T> m = carve(A, smem_raw, R.spill, false);
@@ -592,7 +592,7 @@ __device__ void seg_wgrad(const RArgs<T> &A, const Rob<T> &R, int n) {
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_rollout_bwd(RArgs<T> A) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     const int r = blockIdx.x;
     Rob<T> R = make_rob(A, r);
     Sm<T> m = carve(A, smem_raw, R.spill, true);
@@ -968,7 +968,7 @@ __device__ __forceinline__ void wave_fwd_step(const RArgs<T> &A, const Rob<T> &R
 
 template <typename T, int IMAX, int OMAX>
 __global__ void __launch_bounds__(256) k_wave_fwd(RArgs<T> A, int n_robots) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int r0 = blockIdx.x * (blockDim.x >> 6);
@@ -1263,7 +1263,7 @@ __device__ __forceinline__ void wave_seg_wgrad(const RArgs<T> &A, const Rob<T> &
 
 template <typename T, int IMAX, int OMAX>
 __global__ void __launch_bounds__(256) k_wave_bwd(RArgs<T> A, int n_robots) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int r0 = blockIdx.x * (blockDim.x >> 6);

@@ -175,7 +175,7 @@ template <typename T>
 __global__ void k_nn_input(Layout L, int N, int I, int t, int offset, const T *x, const T *v,
                            const T *center, const T *tv, const T *th, const T *tc, T *in) {
     const int k = blockIdx.y;
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     T *vc = reinterpret_cast<T *>(smem_raw);
     const T *xk = x + (size_t)k * 3 * N;
     if (L.vcom) {
@@ -198,7 +198,7 @@ template <typename T>
 __global__ void k_nn_input_grad(Layout L, int N, int I, const T *ing, T *xg, T *vg, T *cg) {
     // one block per robot: vertex slots -> x, center; vcom slots -> v
     const int k = blockIdx.x;
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     T *red = reinterpret_cast<T *>(smem_raw);
     const T *g = ing + (size_t)k * I;
     const int base = L.vert_base(), vs = L.vstride();
@@ -226,7 +226,7 @@ __global__ void k_nn_input_grad(Layout L, int N, int I, const T *ing, T *xg, T *
 template <typename T>
 __global__ void k_mlp_fwd(int act, int I, int H, int O, const T *in, const T *W1, const T *b1,
                           const T *W2, const T *b2, T *hid, T *out, T *z1save, T *z2save) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     T *xin = reinterpret_cast<T *>(smem_raw);   // [I]
     T *h = xin + I;                             // [H]
     const int k = blockIdx.x;
@@ -268,7 +268,7 @@ __global__ void k_mlp_fwd(int act, int I, int H, int O, const T *in, const T *W1
 template <typename T>
 __global__ void k_mlp_bwd1(int act, int I, int H, int O, const T *in, const T *W1, const T *b1,
                            const T *W2, const T *b2, const T *og, T *ing, T *work) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     T *xin = reinterpret_cast<T *>(smem_raw);   // [I]
     T *h = xin + I;                             // [H]
     T *dh = h + H;                              // [H]
@@ -406,7 +406,7 @@ __global__ void k_advance(Topo<T> tp, Phys<T> P, int B, const T *x, const T *v, 
 template <typename T>
 __global__ void k_advance_grad(Topo<T> tp, Phys<T> P, const T *x, const T *v, const T *act,
                                const T *xg1, const T *vg1, T *xg, T *vg, T *ag) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     const int N = tp.N, S = tp.S;
     T *vib = reinterpret_cast<T *>(smem_raw);  // [3][N]
     T *gs = vib + 3 * N;                       // [7][S]: gd, gw, ga
@@ -480,7 +480,7 @@ __device__ __forceinline__ T cat(const T *c, int Tw, int B, int t, int k, int d)
 template <typename T>
 __global__ void k_loss_velocity(int B, int Tw, int steps, int rp, int tp, const T *c, const T *tv,
                                 T *loss, T *loss_b) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     T *red = reinterpret_cast<T *>(smem_raw);
     T tot[1] = {T(0)};
     for (int k = threadIdx.x; k < B; k += blockDim.x) {
@@ -523,7 +523,7 @@ __global__ void k_loss_velocity_grad(int B, int Tw, int steps, int rp, int tp, c
 template <typename T>
 __global__ void k_loss_rotation(int B, int N, int Tw, int Tx, int steps, int rp, int tp,
                                 const T *x, const T *c, const T *th, T *lossb_tmp) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     T *red = reinterpret_cast<T *>(smem_raw);
     const int k = blockIdx.x;
     T s[1] = {T(0)};
@@ -550,7 +550,7 @@ __global__ void k_loss_rotation(int B, int N, int Tw, int Tx, int steps, int rp,
 
 template <typename T>
 __global__ void k_loss_sum(int B, const T *part, T *loss, T *loss_b) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     T *red = reinterpret_cast<T *>(smem_raw);
     T s[1] = {T(0)};
     for (int k = threadIdx.x; k < B; k += blockDim.x) {
@@ -564,7 +564,7 @@ __global__ void k_loss_sum(int B, const T *part, T *loss, T *loss_b) {
 template <typename T>
 __global__ void k_loss_rotation_grad(int B, int N, int Tw, int Tx, int steps, int rp, int tp,
                                      const T *x, const T *c, const T *th, T g, T *xg, T *cg) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     T *red = reinterpret_cast<T *>(smem_raw);
     const int k = blockIdx.x;
     for (int t = 1; t <= steps; ++t) {
@@ -607,7 +607,7 @@ __global__ void k_loss_rotation_grad(int B, int N, int Tw, int Tx, int steps, in
 template <typename T>
 __global__ void k_loss_task(int B, int Tn, int W, const T *c, const T *vc, const T *tv,
                             const T *th, T *loss, T *loss_b) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     T *red = reinterpret_cast<T *>(smem_raw);
     T tot[1] = {T(0)};
     for (int k = threadIdx.x; k < B; k += blockDim.x) {

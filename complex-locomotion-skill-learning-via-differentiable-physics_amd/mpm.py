@@ -225,13 +225,15 @@ class SolverMPM:
 class MPMBatchC(Structure):
     """dl_mpm_batch."""
     _fields_ = ([(n, c_int32) for n in ("dtype", "B", "P", "n_act", "n_groups", "n_sin",
-                                        "run_period", "n_hidden", "act", "horizon")] +
+                                        "run_period", "n_hidden", "act", "horizon",
+                                        "tile_nodes", "scatter")] +
                 [("offset_scale", c_double), ("max_speed", c_double), ("params", MPMParamsC)] +
                 [(n, ctypes.c_void_p) for n in (
                     "act_id", "group", "inv_count", "targets", "W1", "b1", "W2", "b2",
                     "grid", "grid_vo", "grid_vo_grad", "grid_grad", "blk_flag", "blk_list",
                     "blk_count", "err", "sens", "center", "vcom", "inp", "hid", "act_out",
-                    "act_grad", "in_grad", "center_grad", "vcom_grad", "wgrad")])
+                    "act_grad", "in_grad", "center_grad", "vcom_grad", "wgrad", "tiles",
+                    "tile_hdr", "blk_tile_count", "blk_tiles")])
 
 
 class MPMSlotsC(Structure):
@@ -241,6 +243,8 @@ class MPMSlotsC(Structure):
 
 
 MPM_CONTROL, MPM_SENSE_FIRST = 1, 2
+MPM_PB, MPM_TILE_MAX, MPM_TILE_CAP = 256, 8192, 32
+SCATTER_ATOMIC, SCATTER_STORE = 0, 1
 
 
 def block_order(positions, n_grid, dim=3):
@@ -292,7 +296,7 @@ class MPMBatch:
     def __init__(self, params, positions, act_id, n_act, B, T, K=100, window=None,
                  dtype=torch.float32, n_sin=10, run_period=100, offset_scale=0.2,
                  max_speed=0.08, activation="tanh", n_hidden=64, device="cuda",
-                 sort_particles=True):
+                 sort_particles=True, tile_nodes=None, scatter=SCATTER_ATOMIC):
         if params.dim != 3:
             raise ValueError("MPMBatch drives 3D robots (the controller's sensors are 3D)")
         if params.n_grid % 4:
@@ -329,6 +333,13 @@ class MPMBatch:
         self.grid, self.grid_vo, self.grid_vo_grad, self.grid_grad = (z(B, ng, 4) for _ in range(4))
         self.blk_flag, self.blk_list, self.blk_count = zi(B, nblk), zi(B, nblk), zi(B)
         self.err = zi(1)
+        wpr = (P + MPM_PB - 1) // MPM_PB
+        if tile_nodes is None:      # 64 KiB of LDS in FP32, 128 KiB in FP64
+            tile_nodes = 4096
+        self.tile_nodes, self.scatter = tile_nodes, scatter
+        self.tiles = z(B, wpr, tile_nodes if scatter == SCATTER_STORE else 1, 4)
+        self.tile_hdr = zi(B, wpr, 8)
+        self.blk_tile_count, self.blk_tiles = zi(B, nblk), zi(B, nblk, MPM_TILE_CAP)
         NS = 3 * self.G + 6
         self.sens = z(T + 1, B, NS)
         self.center, self.vcom = z(T + 1, B, 3), z(T + 1, B, 3)
@@ -351,6 +362,7 @@ class MPMBatch:
         self.desc = MPMBatchC(
             dtype=self.dc, B=B, P=P, n_act=n_act, n_groups=self.G, n_sin=n_sin,
             run_period=run_period, n_hidden=H, act=self.act, horizon=T,
+            tile_nodes=tile_nodes, scatter=scatter,
             offset_scale=offset_scale, max_speed=max_speed, params=params,
             act_id=P_(self.act_id), group=P_(self.group), inv_count=P_(self.inv_count),
             targets=P_(self.targets), W1=P_(self.W[0]), b1=P_(self.W[1]), W2=P_(self.W[2]),
@@ -360,7 +372,9 @@ class MPMBatch:
             err=P_(self.err), sens=P_(self.sens), center=P_(self.center), vcom=P_(self.vcom),
             inp=P_(self.inp), hid=P_(self.hid), act_out=P_(self.act_out),
             act_grad=P_(self.act_grad), in_grad=P_(self.in_grad), center_grad=P_(self.cg),
-            vcom_grad=P_(self.vg), wgrad=P_(self.wgrad))
+            vcom_grad=P_(self.vg), wgrad=P_(self.wgrad), tiles=P_(self.tiles),
+            tile_hdr=P_(self.tile_hdr), blk_tile_count=P_(self.blk_tile_count),
+            blk_tiles=P_(self.blk_tiles))
         self.grads = None
 
     def set_targets(self, tv, th):

@@ -344,6 +344,16 @@ int32_t dl_mpm_sensor_adj(int32_t dtype, int32_t B, int32_t P, int32_t n_groups,
  * adjoints, e.g. from dl_loss_task_grad), wgrad [B][H*I + H + O*H + O]
  * (per-robot slices, accumulated).  I = n_sin + 3 + 3*n_groups + 4. */
 enum { DL_MPM_CONTROL = 1, DL_MPM_SENSE_FIRST = 2 };
+/* Scatter tiles: particles are processed DL_MPM_PB per workgroup
+ * (W = ceil(P / DL_MPM_PB) workgroups per robot), each accumulating its
+ * stencil box in an LDS tile of tile_nodes nodes (a box larger than that is
+ * done in slabs).  scatter = DL_MPM_SCATTER_ATOMIC commits tiles to the grid
+ * with global atomics; DL_MPM_SCATTER_STORE stores them to tiles
+ * [B][W][tile_nodes][4] (box in tile_hdr [B][W][8]), lists them under every
+ * block they overlap (blk_tile_count [B][nblk], zero between calls;
+ * blk_tiles [B][nblk][DL_MPM_TILE_CAP]) and the grid pass gathers them. */
+enum { DL_MPM_PB = 256, DL_MPM_TILE_MAX = 8192, DL_MPM_TILE_CAP = 32 };
+enum { DL_MPM_SCATTER_ATOMIC = 0, DL_MPM_SCATTER_STORE = 1 };
 
 typedef struct dl_mpm_batch {
     int32_t dtype, B, P, n_act;
@@ -351,6 +361,8 @@ typedef struct dl_mpm_batch {
     int32_t n_sin, run_period, n_hidden;
     int32_t act;             /* DL_ACT_* of both MLP layers */
     int32_t horizon;         /* T */
+    int32_t tile_nodes;      /* LDS scatter tile capacity, nodes */
+    int32_t scatter;         /* DL_MPM_SCATTER_* */
     double offset_scale, max_speed;
     dl_mpm_params params;
     const int32_t *act_id;   /* [B][P], -1 = passive */
@@ -365,6 +377,8 @@ typedef struct dl_mpm_batch {
     void *act_grad;          /* [B][O] */
     void *in_grad;           /* NULL: no controller adjoint (physics only) */
     void *center_grad, *vcom_grad, *wgrad;
+    void *tiles;
+    int32_t *tile_hdr, *blk_tile_count, *blk_tiles;
 } dl_mpm_batch;
 
 /* Ring of particle-state slots: slot s of x/v at x + s*B*D*P reals, of C/F

@@ -149,9 +149,9 @@ def _weights(I, seed=6):
             0.1 * torch.randn(4, generator=g, dtype=torch.float64)]
 
 
-def _run_batched(params, op, pos, ids, B, T, K, W, dtype, sort=True):
+def _run_batched(params, op, pos, ids, B, T, K, W, dtype, sort=True, **kw):
     batch = dm.MPMBatch(params, pos.numpy(), ids.numpy(), 4, B, T, K=K, window=W, dtype=dtype,
-                        sort_particles=sort)
+                        sort_particles=sort, **kw)
     rng = np.random.default_rng(5)
     tv = np.stack([rng.uniform(-0.08, 0.08, B), np.zeros(B), rng.uniform(-0.08, 0.08, B)], 1)
     th = rng.uniform(0.1, 0.25, B)
@@ -170,31 +170,38 @@ def _run_batched(params, op, pos, ids, B, T, K, W, dtype, sort=True):
     return batch, loss.item(), grads, lo.item(), [w.grad for w in Wr]
 
 
-@pytest.mark.parametrize("prec,K", [("f64", 7), ("f64", 24), ("f32", 7)])
-def test_mpm_batched_rollout_matches_oracle(prec, K):
+@pytest.mark.parametrize("prec,K,scatter,tile", [
+    ("f64", 7, dm.SCATTER_ATOMIC, None), ("f64", 24, dm.SCATTER_STORE, None),
+    ("f32", 7, dm.SCATTER_ATOMIC, None), ("f32", 7, dm.SCATTER_STORE, None),
+    ("f64", 7, dm.SCATTER_STORE, 64)])
+def test_mpm_batched_rollout_matches_oracle(prec, K, scatter, tile):
     """Checkpointed MPM training rollout (sensors, MLP, P2G/grid/G2P, windowed
     COM loss) and its adjoint through every segment, against autograd of the
     FP64 oracle.  Particles are block-sorted internally (order-independent
-    results); K = 7 exercises partial segments and odd segment lengths."""
-    _say(f"mpm batched rollout {prec} K={K}")
+    results); K = 7 exercises partial segments and odd segment lengths; both
+    tile commit modes (global atomics, store + gather); a 64-node tile forces
+    the slab path."""
+    _say(f"mpm batched rollout {prec} K={K} scatter={scatter} tile={tile}")
     params, op, pos, ids = _batched_case()
     B, T, W = 2, 24, 12
-    batch, loss, grads, lo, gref = _run_batched(params, op, pos, ids, B, T, K, W, DT[prec])
+    batch, loss, grads, lo, gref = _run_batched(params, op, pos, ids, B, T, K, W, DT[prec],
+                                                scatter=scatter, tile_nodes=tile)
     tl, tg = (1e-10, 1e-7) if prec == "f64" else (1e-4, 1e-3)
     assert abs(loss - lo) / abs(lo) < tl
     for a, b in zip(grads, gref):
         assert rel_err(a.double().cpu(), b) < tg
 
 
-@pytest.mark.parametrize("case", ["slabs", "direct"])
-def test_mpm_batched_scatter_paths_and_invariants(case):
+@pytest.mark.parametrize("case,scatter", [("slabs", dm.SCATTER_ATOMIC), ("direct", dm.SCATTER_ATOMIC),
+                                          ("slabs", dm.SCATTER_STORE)])
+def test_mpm_batched_scatter_paths_and_invariants(case, scatter):
     """Two blobs far apart in one workgroup (shuffled, unsorted), so the
     workgroup's node box exceeds the LDS tile: "slabs" (box 21 x 5 x 21)
     scatters in several x-slab passes through the tile, "direct" (a y-z plane
     of 45 x 45 nodes) with direct global atomics.  Results match the oracle,
     and the grid, active-block and actuation-adjoint buffers are zero again
     afterwards."""
-    _say(f"mpm batched scatter path {case}")
+    _say(f"mpm batched scatter path {case} scatter={scatter}")
     if case == "slabs":
         params, op, pos, ids = _batched_case(offsets=((5.0, 3.2, 5.0), (20.0, 3.2, 20.0)), n=32,
                                              perm_seed=3)
@@ -203,10 +210,11 @@ def test_mpm_batched_scatter_paths_and_invariants(case):
                                              perm_seed=3)
     B, T, K, W = 2, 12, 5, 6
     batch, loss, grads, lo, gref = _run_batched(params, op, pos, ids, B, T, K, W, torch.float64,
-                                                sort=False)
+                                                sort=False, scatter=scatter, tile_nodes=1024)
     assert abs(loss - lo) / abs(lo) < 1e-10
     for a, b in zip(grads, gref):
         assert rel_err(a.double().cpu(), b) < 1e-7
     for t in (batch.grid, batch.grid_vo_grad, batch.act_grad):
         assert t.abs().max().item() == 0.0
     assert batch.blk_flag.abs().max().item() == 0 and batch.blk_count.abs().max().item() == 0
+    assert batch.blk_tile_count.abs().max().item() == 0
