@@ -28,6 +28,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "robot-steps/s fwd+bwd batched vs plain; MPM particle-steps/s; p99 latency"
 HBM_PEAK_GBS = 8000.0
+UNEVEN_DASHPOT = 0.03
 
 
 def algorithmic_bytes(N, S, I, H, O, K, R_d, dtype_bytes=4):
@@ -65,6 +66,9 @@ def main():
     ap.add_argument("--mpm-steps", type=int, default=1)
     ap.add_argument("--mpm-scatter", type=int, default=0, help="0: tile atomics, 1: store + gather")
     ap.add_argument("--mpm-tile", type=int, default=4096, help="LDS scatter tile, nodes")
+    ap.add_argument("--skip-uneven", action="store_true")
+    ap.add_argument("--uneven-robots", type=int, default=512)
+    ap.add_argument("--uneven-iters", type=int, default=3)
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer 4 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -212,6 +216,67 @@ def main():
                             "algorithmic_bytes_per_particle_step": bpp, "traffic": None},
                "mean_loss": float(mb.loss.item())}
 
+    # ---- uneven batch: 512 seeded random voxel robots, size classes, HIP graphs
+    uneven = None
+    if not args.skip_uneven:
+        from diffloco import robot_design as rdm
+        from diffloco.uneven import UnevenBatch, n_inputs
+        designs_u, hz = rdm.random_robot_batch(args.uneven_robots, seed=3)
+        Ru = len(designs_u)
+        rngu = np.random.default_rng(4)
+        tgu = np.zeros((Ru, 5))
+        tgu[:, 0] = rngu.uniform(-0.08, 0.08, Ru)
+        tgu[:, 2] = rngu.uniform(-0.08, 0.08, Ru)
+        tgu[:, 3] = rngu.uniform(0.1, 0.25, Ru)
+        # dashpot impulse 0.03: the no-dt impulse c (va - vb).d d of the
+        # reference integrator diverges at c = 0.12 for vertices with many
+        # springs (dense voxel bodies), also in the FP64 oracle
+        phys_u = _physics(qc)
+        phys_u.dashpot = UNEVEN_DASHPOT
+        ub = UnevenBatch(designs_u, hz, phys_u, lay, tgu, ckpt_every=50, loss_window=100)
+        for r, d in enumerate(designs_u):
+            ub.set_weights(r, *init_weights(n_inputs(lay, d.n_vert), 64, d.n_act, seed=1000 + r))
+        t_tune = time.perf_counter()
+        ub.tune()
+        t_tune = time.perf_counter() - t_tune
+        ub.prepare(np.full(Ru, 1.0 / Ru))
+        ub.run_iteration()
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        ub.run_iteration()
+        torch.cuda.synchronize()
+        eager_ms = (time.perf_counter() - t3) * 1e3
+        ub.capture()
+        ub.replay()
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        for _ in range(args.uneven_iters):
+            ub.replay()
+        torch.cuda.synchronize()
+        graph_ms = (time.perf_counter() - t3) * 1e3 / args.uneven_iters
+        rs = ub.robot_steps
+        ubytes = 0.0
+        for d, T_r in zip(designs_u, hz):
+            I_r = n_inputs(lay, d.n_vert)
+            f_r, b_r = algorithmic_bytes(d.n_vert, d.n_spring, I_r, 64, d.n_act, 50, 1)
+            ubytes += (f_r + b_r) * T_r
+        uval = rs / (graph_ms * 1e-3)
+        nv = np.array([d.n_vert for d in designs_u])
+        uneven = {"value": uval, "unit": "robot-steps/s", "robots": Ru, "robot_steps": rs,
+                  "n_vert": [int(nv.min()), float(nv.mean()), int(nv.max())],
+                  "horizon": [int(min(hz)), int(max(hz))], "dtype": "f32",
+                  "ms_per_iteration_graph": graph_ms, "ms_per_iteration_eager": eager_ms,
+                  "autotune_s": t_tune,
+                  "classes": [{"name": bk["name"], "robots": len(bk["ids"]), "launch": bk["config"]}
+                              for bk in ub.buckets],
+                  "roofline": {"bound": "hbm", "achieved": ubytes / (graph_ms * 1e-3) / 1e9,
+                               "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": ubytes / (graph_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                               "algorithmic_bytes_per_iteration": ubytes, "traffic": None},
+                  "dashpot_impulse": UNEVEN_DASHPOT,
+                  "nonfinite_robots": int((~torch.isfinite(ub.losses())).sum().item()),
+                  "mean_loss": float(ub.losses().double().mean().item())}
+
     # ---- CPU baseline: FP64 oracle forward+backward on a bounded sample
     cpu = None
     if not args.skip_cpu:
@@ -251,6 +316,7 @@ def main():
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "cpu_baseline": cpu,
         "mpm": mpm,
+        "uneven": uneven,
         "check": {"mean_loss": loss, "grad_norm": gnorm},
     }
     print(json.dumps(out))

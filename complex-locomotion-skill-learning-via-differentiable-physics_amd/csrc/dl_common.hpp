@@ -6,6 +6,9 @@
 #include <string.h>
 #include <stdio.h>
 
+#include <mutex>
+#include <unordered_map>
+
 #include "diffloco.h"
 
 namespace dl {
@@ -31,6 +34,23 @@ void set_error(const char *fmt, ...);
     } while (0)
 
 #define DL_CHECK_LAUNCH() DL_CHECK_HIP(hipGetLastError())
+
+// Raise a kernel's dynamic-LDS limit to at least `bytes`, calling the
+// runtime only when the limit grows: after the first launch of a shape no
+// attribute call is made, so launches issued under stream capture (HIP
+// graphs) are pure launches.
+inline hipError_t ensure_dynamic_lds(const void *kernel, size_t bytes) {
+    static std::mutex mu;
+    static std::unordered_map<const void *, size_t> done;
+    if (bytes <= 65536) return hipSuccess;
+    std::lock_guard<std::mutex> lock(mu);
+    size_t &have = done[kernel];
+    if (bytes <= have) return hipSuccess;
+    const hipError_t e =
+        hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess) have = bytes;
+    return e;
+}
 
 // ------------------------------------------------------------------ math
 template <typename T> struct M;

@@ -1161,8 +1161,7 @@ int forward_impl(const dl_mpm_batch *b, const dl_mpm_slots *st, int t0, int n, i
     const T *act = (const T *)b->act_out;
     const TileBuf<T> tb = make_tiles<T>(b);
     const size_t tile_bytes = (size_t)tb.tn * 4 * sizeof(T);
-    DL_CHECK_HIP(hipFuncSetAttribute((const void *)k_p2g_b<D, T, true>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)tile_bytes));
+    DL_CHECK_HIP(ensure_dynamic_lds((const void *)k_p2g_b<D, T, true>, tile_bytes));
     if (control && (flags & DL_MPM_SENSE_FIRST)) {
         const Slot<D, T> s0 = slot<D, T>(st, 0, B, P);
         if constexpr (D == 3)
@@ -1218,10 +1217,8 @@ int backward_impl(const dl_mpm_batch *b, const dl_mpm_slots *seg, const dl_mpm_s
     const T *act = (const T *)b->act_out;
     const TileBuf<T> tb = make_tiles<T>(b);
     const size_t tile_bytes = (size_t)tb.tn * 4 * sizeof(T);
-    DL_CHECK_HIP(hipFuncSetAttribute((const void *)k_p2g_b<D, T, false>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)tile_bytes));
-    DL_CHECK_HIP(hipFuncSetAttribute((const void *)k_g2p_adj_b<D, T>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)tile_bytes));
+    DL_CHECK_HIP(ensure_dynamic_lds((const void *)k_p2g_b<D, T, false>, tile_bytes));
+    DL_CHECK_HIP(ensure_dynamic_lds((const void *)k_g2p_adj_b<D, T>, tile_bytes));
     int cur = 0;
     for (int j = n - 1; j >= 0; --j) {
         const int t = t0 + j, nxt = 1 - cur;

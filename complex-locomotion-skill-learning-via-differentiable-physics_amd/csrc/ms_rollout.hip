@@ -1430,8 +1430,7 @@ static int launch_wave(const dl_rollout *r, RArgs<T> A, bool bwd, hipStream_t st
 #define DL_WAVE_LAUNCH(IM)                                                                      \
     do {                                                                                        \
         auto kf = bwd ? (const void *)k_wave_bwd<float, IM, 8> : (const void *)k_wave_fwd<float, IM, 8>; \
-        if (sh > 65536)                                                                         \
-            DL_CHECK_HIP(hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh)); \
+        DL_CHECK_HIP(ensure_dynamic_lds(kf, sh));                                               \
         if (bwd)                                                                                \
             hipLaunchKernelGGL((k_wave_bwd<float, IM, 8>), grid, block, sh, st, A, r->n_robots); \
         else                                                                                    \
@@ -1484,14 +1483,10 @@ static int launch_rollout(const dl_rollout *r, bool bwd, hipStream_t st) {
     A.topo_in_lds = topo;
     A.w_in_lds = wl;
     if (bwd) {
-        if (sh > 65536)
-            DL_CHECK_HIP(hipFuncSetAttribute((const void *)k_rollout_bwd<T>,
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+        DL_CHECK_HIP(ensure_dynamic_lds((const void *)k_rollout_bwd<T>, sh));
         hipLaunchKernelGGL(k_rollout_bwd<T>, dim3(r->n_robots), dim3(r->block), sh, st, A);
     } else {
-        if (sh > 65536)
-            DL_CHECK_HIP(hipFuncSetAttribute((const void *)k_rollout_fwd<T>,
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+        DL_CHECK_HIP(ensure_dynamic_lds((const void *)k_rollout_fwd<T>, sh));
         hipLaunchKernelGGL(k_rollout_fwd<T>, dim3(r->n_robots), dim3(r->block), sh, st, A);
     }
     DL_CHECK_LAUNCH();

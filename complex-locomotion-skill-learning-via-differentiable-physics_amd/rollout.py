@@ -292,6 +292,36 @@ class MassSpringBatch:
         if reduce:
             self.reduce_grads()
 
+    # ------------------------------------------------ whole training iterations
+    def prepare_iteration(self, loss_grad=None):
+        """Set up full-horizon training iterations (forward to every robot's
+        horizon, adjoint back to 0): the initial state, the reset clock table
+        (the forward reads only t_fwd, the adjoint only t_bwd) and the loss
+        adjoint stay on the device, so ``run_iteration`` issues device work
+        only and can be captured in a HIP graph."""
+        self.reset()
+        tab = self.table.copy()
+        tab["t_fwd"] = 0
+        tab["t_bwd"] = self.horizons
+        self.t_table_iter = torch.from_numpy(tab.view(np.uint8).copy()).to(self.device)
+        if loss_grad is not None:
+            self.loss_grad.copy_(torch.as_tensor(loss_grad, dtype=self.dtype).reshape(-1))
+
+    def run_iteration(self):
+        """Reset state, forward rollout, adjoint and per-design gradient
+        reduction -- launches and device copies only."""
+        self.state.copy_(self._state0)
+        self.loss.zero_()
+        self.adjoint.zero_()
+        self.wgrad.zero_()
+        if self.traj is not None:
+            self.traj.zero_()
+        D = self._desc()
+        D.robots = nat.ptr(self.t_table_iter)
+        nat.call("dl_ms_rollout_forward", ctypes.byref(D), nat.stream())
+        nat.call("dl_ms_rollout_backward", ctypes.byref(D), nat.stream())
+        self.reduce_grads()
+
     def reduce_grads(self):
         nat.call("dl_wgrad_reduce", nat.dtype_code(self.dtype), self.R, nat.ptr(self.t_lptr),
                  nat.ptr(self.t_list), len(self.designs), nat.ptr(self.t_roff),
@@ -332,3 +362,27 @@ def rollout_loss(batch, weights):
     (``weights``: list over designs of [W1, b1, W2, b2])."""
     flat = [t for w in weights for t in w]
     return _RolloutFn.apply(batch, *flat)
+
+
+class GraphedIteration:
+    """A device-only callable (e.g. ``MassSpringBatch.run_iteration``)
+    captured once in a HIP graph (torch's graph API over the HIP runtime) and
+    replayed: one launch of the whole recorded sequence instead of one host
+    call per kernel.  The callable is run ``warmup`` times eagerly first on a
+    side stream, so that one-time setup (LDS limits, allocator growth)
+    happens outside the capture."""
+
+    def __init__(self, fn, warmup=1):
+        self.fn = fn
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            for _ in range(warmup):
+                fn()
+        torch.cuda.current_stream().wait_stream(side)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            fn()
+
+    def replay(self):
+        self.graph.replay()

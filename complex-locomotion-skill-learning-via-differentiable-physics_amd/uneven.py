@@ -1,0 +1,166 @@
+"""Uneven mass-spring batches: robots of different designs, sizes, actuator
+counts and horizons, each with its own MLP, trained together.
+
+The reference trains one design at a time, the batch axis holding tasks of
+that design (``diffphy_trainer.py:219-268``).  Here every robot may be a
+different design.  Robots are grouped into size classes; each class runs as
+one persistent forward launch and one adjoint launch (``MassSpringBatch``,
+driven by the per-robot offset table) with a launch shape chosen by the
+autotuner, and the classes run concurrently on separate streams.  A whole
+iteration can be captured in HIP graphs and replayed.
+
+Size classes: robots that fit the wave-per-robot kernel (FP32, hidden 64,
+network input <= 192, outputs <= 8, <= 64 vertices) form one class; the rest
+are split by vertex count (the block-per-robot kernel, whose best block size
+grows with the robot).
+"""
+
+import numpy as np
+import torch
+
+from .autotune import Autotuner
+from .rollout import GraphedIteration, MassSpringBatch
+
+WAVE_MAX_IN, WAVE_MAX_OUT, WAVE_MAX_VERT = 192, 8, 64
+
+
+def n_inputs(layout, n_vert):
+    """Network input width of a robot with n_vert vertices (taichi_env.py:149-183 order)."""
+    lay = layout
+    return (lay.n_sin + 3 * lay.vcom + n_vert * (3 + lay.vert_pad) + lay.dup_v * lay.n_tv
+            + lay.dup_h + lay.dup_c)
+
+
+def size_classes(designs, layout, dtype, n_hidden=64, bounds=(128, 400)):
+    """Ordered list of (name, robot indices)."""
+    out = {}
+    for r, d in enumerate(designs):
+        if (dtype == torch.float32 and n_hidden == 64 and n_inputs(layout, d.n_vert) <= WAVE_MAX_IN
+                and d.n_act <= WAVE_MAX_OUT and d.n_vert <= WAVE_MAX_VERT):
+            name = "wave"
+        else:
+            name = next((f"n<={b}" for b in bounds if d.n_vert <= b), f"n>{bounds[-1]}")
+        out.setdefault(name, []).append(r)
+    order = ["wave"] + [f"n<={b}" for b in bounds] + [f"n>{bounds[-1]}"]
+    return [(k, out[k]) for k in order if k in out]
+
+
+class UnevenBatch:
+    """R robots, robot r = design designs[r] with horizon horizons[r].
+
+    targets: [R, 5] (tv (3), th, unused); x0: list of [N_r, 3] (default: the
+    design's rest positions shifted +0.4 in x, taichi_env.py:500-501)."""
+
+    def __init__(self, designs, horizons, physics, layout, targets, x0=None, ckpt_every=50,
+                 loss_window=100, dtype=torch.float32, act="tanh", n_hidden=64, streams=True,
+                 device="cuda", bounds=(128, 400)):
+        self.R = len(designs)
+        self.designs, self.horizons = designs, np.asarray(horizons, np.int64)
+        if x0 is None:
+            x0 = [d.objects + np.array([0.4, 0.0, 0.0]) for d in designs]
+        targets = np.asarray(targets, np.float64)
+        self.dtype, self.streams_on = dtype, streams
+        self.buckets = []
+        self.slot = {}
+        for name, ids in size_classes(designs, layout, dtype, n_hidden, bounds):
+            b = MassSpringBatch([designs[r] for r in ids], list(range(len(ids))),
+                                self.horizons[ids], physics, layout, targets[ids],
+                                [x0[r] for r in ids], n_hidden=n_hidden, act=act,
+                                ckpt_every=ckpt_every, dtype=dtype, loss_window=loss_window,
+                                device=device, variant=2 if name == "wave" else 1,
+                                block=64 if name == "wave" else 256)
+            for j, r in enumerate(ids):
+                self.slot[r] = (len(self.buckets), j)
+            self.buckets.append({"name": name, "ids": np.asarray(ids), "batch": b, "config": None})
+        self.graphs = None
+        self._streams = [torch.cuda.Stream(device=device) for _ in self.buckets] if streams else None
+
+    # ------------------------------------------------------------ weights
+    def set_weights(self, r, W1, b1, W2, b2):
+        bi, j = self.slot[r]
+        self.buckets[bi]["batch"].set_weights(j, W1, b1, W2, b2)
+
+    def weight_grads(self, r):
+        bi, j = self.slot[r]
+        return self.buckets[bi]["batch"].weight_grads(j)
+
+    def losses(self):
+        out = torch.empty(self.R, dtype=self.dtype, device=self.buckets[0]["batch"].loss.device)
+        for bk in self.buckets:
+            out[torch.as_tensor(bk["ids"], device=out.device)] = bk["batch"].loss
+        return out
+
+    @property
+    def robot_steps(self):
+        return int(self.horizons.sum())
+
+    # ------------------------------------------------------------ launch shapes
+    @staticmethod
+    def candidates(name):
+        if name == "wave":
+            return [{"variant": 2, "block": 64, "lds_policy": 0}]
+        return [{"variant": 1, "block": b, "lds_policy": p} for b in (64, 128, 256) for p in (0, 1)]
+
+    def _apply(self, bk, cfg):
+        b = bk["batch"]
+        b.variant, b.block, b.lds_policy = cfg["variant"], cfg["block"], cfg["lds_policy"]
+        bk["config"] = cfg
+
+    def tune(self, tuner=None):
+        """Pick each size class's launch shape by timing a full iteration of
+        that class with every candidate (cached by the autotuner)."""
+        tuner = tuner or Autotuner()
+        for bk in self.buckets:
+            b = bk["batch"]
+            nv = np.array([b.designs[int(d)].n_vert for d in b.robot_design])
+            key = Autotuner.key(kernel="ms_rollout", cls=bk["name"], dtype=str(self.dtype),
+                                robots=len(bk["ids"]), mean_n=int(nv.mean()),
+                                mean_T=int(b.horizons.mean()))
+
+            def measure(cfg, bk=bk):
+                self._apply(bk, cfg)
+                bk["batch"].prepare_iteration(np.ones(len(bk["ids"])))
+                bk["batch"].run_iteration()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                bk["batch"].run_iteration()
+                e1.record()
+                torch.cuda.synchronize()
+                return e0.elapsed_time(e1)
+
+            self._apply(bk, tuner.pick(key, self.candidates(bk["name"]), measure))
+        return [bk["config"] for bk in self.buckets]
+
+    # ------------------------------------------------------------ iterations
+    def prepare(self, loss_grad=None):
+        """Device-side setup for full iterations; loss_grad: [R] weights of
+        the per-robot losses (default 1 each)."""
+        lg = np.ones(self.R) if loss_grad is None else np.asarray(loss_grad, np.float64)
+        for bk in self.buckets:
+            if bk["config"] is None:
+                self._apply(bk, self.candidates(bk["name"])[0])
+            bk["batch"].prepare_iteration(lg[bk["ids"]])
+
+    def _on_streams(self, fns):
+        if not self.streams_on:
+            for f in fns:
+                f()
+            return
+        cur = torch.cuda.current_stream()
+        for s, f in zip(self._streams, fns):
+            s.wait_stream(cur)
+            with torch.cuda.stream(s):
+                f()
+        for s in self._streams:
+            cur.wait_stream(s)
+
+    def run_iteration(self):
+        """One forward + adjoint of every robot (size classes concurrently)."""
+        self._on_streams([bk["batch"].run_iteration for bk in self.buckets])
+
+    def capture(self):
+        """Record each size class's iteration in a HIP graph."""
+        self.graphs = [GraphedIteration(bk["batch"].run_iteration) for bk in self.buckets]
+
+    def replay(self):
+        self._on_streams([g.replay for g in self.graphs])

@@ -1,0 +1,106 @@
+"""Uneven batches: size classes, autotuner choice/caching (CPU), and on the
+GPU the bucketed multi-stream iteration and its HIP-graph replay against a
+single-launch reference."""
+
+import numpy as np
+import pytest
+import torch
+
+import diffloco as dl
+from diffloco import robot_design as rd
+from diffloco.autotune import Autotuner
+from diffloco.solver_mass_spring import _physics
+from diffloco.taichi_env import sensor_layout
+
+from helpers import rel_err
+
+
+def _setup(n=10, seed=7, cube_range=(2, 30)):
+    designs, horizons = rd.random_robot_batch(n, seed, horizon_range=(40, 90), cube_range=cube_range,
+                                              group_range=(4, 12))
+    qc = dl.quadruped_config(batch_size=n, steps=int(max(horizons)), window=30).get_config()
+    rng = np.random.default_rng(seed + 1)
+    tg = np.zeros((n, 5))
+    tg[:, 0] = rng.uniform(-0.08, 0.08, n)
+    tg[:, 2] = rng.uniform(-0.08, 0.08, n)
+    tg[:, 3] = rng.uniform(0.1, 0.25, n)
+    return designs, horizons, _physics(qc), sensor_layout(qc), tg
+
+
+def test_size_classes_partition_every_robot_once():
+    from diffloco.uneven import size_classes, n_inputs, WAVE_MAX_IN
+    designs, horizons, phys, lay, tg = _setup(n=40, cube_range=(2, 200))
+    cls = size_classes(designs, lay, torch.float32)
+    ids = np.concatenate([np.asarray(i) for _, i in cls])
+    assert sorted(ids.tolist()) == list(range(40))
+    for name, rs in cls:
+        for r in rs:
+            d = designs[r]
+            if name == "wave":
+                assert n_inputs(lay, d.n_vert) <= WAVE_MAX_IN and d.n_act <= 8 and d.n_vert <= 64
+            elif name.startswith("n<="):
+                assert d.n_vert <= int(name[3:])
+    assert size_classes(designs, lay, torch.float64)[0][0] != "wave"
+
+
+def test_autotuner_picks_fastest_and_caches(tmp_path):
+    t = Autotuner(tmp_path / "tune.json")
+    cands = [{"block": 64}, {"block": 128}, {"block": 256}]
+    cost = {64: 3.0, 128: 1.0, 256: 2.0}
+    calls = []
+
+    def measure(c):
+        calls.append(c["block"])
+        if c["block"] == 256:
+            raise RuntimeError("shape not supported")
+        return cost[c["block"]]
+
+    assert t.pick("k", cands, measure) == {"block": 128}
+    n = len(calls)
+    t2 = Autotuner(tmp_path / "tune.json")
+    assert t2.pick("k", cands, measure) == {"block": 128} and len(calls) == n   # cached
+    assert "error" in str(t2.cache["k"]["timings"])
+
+
+@pytest.mark.gpu
+def test_uneven_iteration_and_graph_replay_match_single_launch(tmp_path):
+    from diffloco.rollout import MassSpringBatch, init_weights
+    from diffloco.uneven import UnevenBatch, n_inputs
+    print("uneven batch: tune, streams, graph", flush=True)
+    designs, horizons, phys, lay, tg = _setup()
+    R = len(designs)
+    ub = UnevenBatch(designs, horizons, phys, lay, tg, ckpt_every=16, loss_window=30)
+    assert len(ub.buckets) >= 2 and ub.buckets[0]["name"] == "wave"
+    W = [init_weights(n_inputs(lay, d.n_vert), 64, d.n_act, seed=100 + r) for r, d in enumerate(designs)]
+    for r in range(R):
+        ub.set_weights(r, *W[r])
+    ub.tune(Autotuner(tmp_path / "tune.json"))
+    ub.prepare()
+    ub.run_iteration()
+    torch.cuda.synchronize()
+    loss_e = ub.losses().clone()
+    grads_e = [[g.clone() for g in ub.weight_grads(r)] for r in range(R)]
+
+    ref = MassSpringBatch(designs, list(range(R)), horizons, phys, lay, tg,
+                          [d.objects + np.array([0.4, 0.0, 0.0]) for d in designs], ckpt_every=16,
+                          dtype=torch.float32, loss_window=30, variant=1)
+    for r in range(R):
+        ref.set_weights(r, *W[r])
+    ref.prepare_iteration(np.ones(R))
+    ref.run_iteration()
+    torch.cuda.synchronize()
+    assert rel_err(loss_e.double().cpu(), ref.loss.double().cpu()) < 1e-4
+    for r in range(R):
+        for a, b in zip(grads_e[r], ref.weight_grads(r)):
+            assert rel_err(a.double().cpu(), b.double().cpu()) < 1e-3, r
+
+    ub.capture()
+    for rep in range(3):
+        ref.run_iteration()                 # unrelated eager work between replays
+        torch.cuda.synchronize()
+        ub.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(ub.losses(), loss_e), rep
+        for r in range(R):
+            for a, b in zip(ub.weight_grads(r), grads_e[r]):
+                assert torch.equal(a, b), (rep, r)
