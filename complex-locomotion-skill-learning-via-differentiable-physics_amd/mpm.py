@@ -400,7 +400,9 @@ class MPMBatch:
     def _segments(self):
         return [(s * self.K, min(self.K, self.T - s * self.K)) for s in range(self.nck)]
 
-    def forward(self):
+    # A rollout as resumable pieces (a serving loop advances one segment per
+    # tick); forward() / backward() run them all.
+    def start_forward(self):
         r = self.ring
         r.x[0].copy_(self.x0[None].expand(self.B, -1, -1))
         r.v[0].zero_()
@@ -409,41 +411,69 @@ class MPMBatch:
         for i in range(3):
             r.F[0][:, i * 3 + i] = 1.0
         self.sens.zero_()
-        for s, (t0, n) in enumerate(self._segments()):
-            self.ck.copy_slot(s, 0, r)
-            flags = MPM_CONTROL | (MPM_SENSE_FIRST if s == 0 else 0)
-            nat.call("dl_mpm_forward", ctypes.byref(self.desc), ctypes.byref(r.c), t0, n, flags,
-                     nat.stream())
-            if n % 2:
-                r.copy_slot(0, 1)
+
+    def forward_segment(self, s):
+        r = self.ring
+        t0, n = self._segments()[s]
+        self.ck.copy_slot(s, 0, r)
+        flags = MPM_CONTROL | (MPM_SENSE_FIRST if s == 0 else 0)
+        nat.call("dl_mpm_forward", ctypes.byref(self.desc), ctypes.byref(r.c), t0, n, flags,
+                 nat.stream())
+        if n % 2:
+            r.copy_slot(0, 1)
+
+    def finish_forward(self):
         nat.call("dl_loss_task", self.dc, self.B, self.T, self.window, nat.ptr(self.center),
                  nat.ptr(self.vcom), nat.ptr(self.tv), nat.ptr(self.th), nat.ptr(self.loss),
                  nat.ptr(self.loss_b), nat.stream())
         return self.loss
 
-    def backward(self, loss_grad=1.0):
-        B, T = self.B, self.T
+    def forward(self):
+        self.start_forward()
+        for s in range(self.nck):
+            self.forward_segment(s)
+        return self.finish_forward()
+
+    def start_backward(self, loss_grad=1.0):
         self.cg.zero_()
         self.vg.zero_()
         self.in_grad.zero_()
         self.wgrad.zero_()
         for f in self.adj.fields():
             f[0].zero_()
-        nat.call("dl_loss_task_grad", self.dc, B, T, self.window, nat.ptr(self.center),
+        nat.call("dl_loss_task_grad", self.dc, self.B, self.T, self.window, nat.ptr(self.center),
                  nat.ptr(self.vcom), nat.ptr(self.tv), nat.ptr(self.th), float(loss_grad),
                  nat.ptr(self.cg), nat.ptr(self.vg), nat.stream())
-        for s, (t0, n) in reversed(list(enumerate(self._segments()))):
-            self.seg.copy_slot(0, s, self.ck)
-            nat.call("dl_mpm_forward", ctypes.byref(self.desc), ctypes.byref(self.seg.c), t0, n, 0,
-                     nat.stream())
-            nat.call("dl_mpm_backward", ctypes.byref(self.desc), ctypes.byref(self.seg.c),
-                     ctypes.byref(self.adj.c), t0, n, nat.stream())
+
+    def backward_segment(self, s):
+        """Adjoint of segment s (segments run last to first)."""
+        t0, n = self._segments()[s]
+        self.seg.copy_slot(0, s, self.ck)
+        nat.call("dl_mpm_forward", ctypes.byref(self.desc), ctypes.byref(self.seg.c), t0, n, 0,
+                 nat.stream())
+        nat.call("dl_mpm_backward", ctypes.byref(self.desc), ctypes.byref(self.seg.c),
+                 ctypes.byref(self.adj.c), t0, n, nat.stream())
+
+    def finish_backward(self):
         g = self.wgrad.sum(0)
         H, I, O = self.H, self.I, self.O
-        sizes = [H * I, H, O * H, O]
-        parts = torch.split(g, sizes)
+        parts = torch.split(g, [H * I, H, O * H, O])
         self.grads = [parts[0].view(H, I), parts[1], parts[2].view(O, H), parts[3]]
         return self.grads
+
+    def backward(self, loss_grad=1.0):
+        self.start_backward(loss_grad)
+        for s in reversed(range(self.nck)):
+            self.backward_segment(s)
+        return self.finish_backward()
+
+    def nbytes(self):
+        """Device bytes held by this rollout's buffers."""
+        own = sum(t.numel() * t.element_size() for t in vars(self).values()
+                  if isinstance(t, torch.Tensor))
+        own += sum(t.numel() * t.element_size() for t in self.W)
+        return own + sum(f.numel() * f.element_size()
+                         for sl in (self.ring, self.ck, self.seg, self.adj) for f in sl.fields())
 
     def check(self):
         if int(self.err.item()) != 0:
