@@ -69,6 +69,11 @@ def main():
     ap.add_argument("--skip-uneven", action="store_true")
     ap.add_argument("--uneven-robots", type=int, default=512)
     ap.add_argument("--uneven-iters", type=int, default=3)
+    ap.add_argument("--skip-serve", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-duration", type=float, default=4.0, help="seconds of arrivals")
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--skip-serial", action="store_true")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer 4 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -277,6 +282,34 @@ def main():
                   "nonfinite_robots": int((~torch.isfinite(ub.losses())).sum().item()),
                   "mean_loss": float(ub.losses().double().mean().item())}
 
+    # ---- request loop: Poisson arrivals, 70 % random mass-spring robots, 30 % MPM bodies
+    serve = None
+    if not args.skip_serve:
+        from diffloco import serving as sv
+        phys_s = _physics(qc)
+        phys_s.dashpot = UNEVEN_DASHPOT
+
+        def serve_run(max_active):
+            reqs = sv.make_requests(args.serve_rate, args.serve_duration, seed=4, layout=lay)
+            ex = sv.default_executors(phys_s, lay)
+            srv = sv.Server(ex, budget_bytes=int(args.serve_budget_gb * 2 ** 30), chunk=50,
+                            max_active=max_active)
+            t4 = time.perf_counter()
+            done = srv.run(reqs)
+            wall_s = time.perf_counter() - t4
+            st = sv.latency_stats(done, wall_s)
+            st.update({"wall_s": wall_s, "ticks": srv.ticks,
+                       "mpm_requests": sum(r.kind == "mpm" for r in done),
+                       "grad_requests": sum(r.grad for r in done),
+                       "robot_steps_per_s": st["robot_steps"] / wall_s})
+            return st
+
+        serve = {"arrival_rate_per_s": args.serve_rate, "arrival_window_s": args.serve_duration,
+                 "hbm_budget_gb": args.serve_budget_gb, "tick_steps": 50,
+                 "continuous": serve_run(None)}
+        if not args.skip_serial:
+            serve["one_at_a_time"] = serve_run(1)
+
     # ---- CPU baseline: FP64 oracle forward+backward on a bounded sample
     cpu = None
     if not args.skip_cpu:
@@ -317,6 +350,7 @@ def main():
         "cpu_baseline": cpu,
         "mpm": mpm,
         "uneven": uneven,
+        "serve": serve,
         "check": {"mean_loss": loss, "grad_norm": gnorm},
     }
     print(json.dumps(out))
