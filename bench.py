@@ -70,8 +70,8 @@ def main():
     ap.add_argument("--uneven-robots", type=int, default=512)
     ap.add_argument("--uneven-iters", type=int, default=3)
     ap.add_argument("--skip-serve", action="store_true")
-    ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrivals per second")
-    ap.add_argument("--serve-duration", type=float, default=4.0, help="seconds of arrivals")
+    ap.add_argument("--serve-rate", type=float, default=25.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-duration", type=float, default=2.0, help="seconds of arrivals")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--skip-serial", action="store_true")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",

@@ -203,6 +203,9 @@ class MSPool:
     def active(self):
         return [k for k in range(self.C) if self.req[k] is not None]
 
+    def busy(self):
+        return any(p in ("fwd", "bwd") for p in self.phase)
+
     # ------------------------------------------------------------ ticks
     def _desc(self, chunk):
         D = nat.Rollout()
@@ -291,14 +294,15 @@ class MSPool:
 
 
 class MPMRunner:
-    """MPM requests, each its own one-robot MPMBatch advanced one checkpoint
-    segment per tick on a side stream."""
+    """MPM requests, each its own one-robot MPMBatch on its own stream,
+    advanced one checkpoint segment per tick (jobs overlap on the device)."""
 
     def __init__(self, max_active=8, ckpt_every=25, dtype=torch.float32, device="cuda"):
         self.max_active, self.K = max_active, ckpt_every
         self.dtype, self.device = dtype, device
         self.jobs = []
-        self.stream = torch.cuda.Stream(device=device)
+        self.stream = torch.cuda.Stream(device=device)          # joins the job streams
+        self.free_streams = [torch.cuda.Stream(device=device) for _ in range(max_active)]
 
     def fits(self, req):
         return req.kind == "mpm"
@@ -306,8 +310,14 @@ class MPMRunner:
     def has_slot(self):
         return len(self.jobs) < self.max_active
 
+    def busy(self):
+        return any(j["phase"] in ("fwd", "bwd") for j in self.jobs)
+
     @staticmethod
     def estimate_bytes(req, ckpt_every=25, es=4):
+        """Device bytes of a one-robot MPMBatch: state slots (ring, checkpoints,
+        segment, adjoint) of 24 reals per particle, four 16-byte grid records
+        per node, block tables."""
         m = req.mpm
         P, n = len(m["positions"]), m["params"].n_grid
         nck = (req.horizon + ckpt_every - 1) // ckpt_every
@@ -320,25 +330,29 @@ class MPMRunner:
     def admit(self, req):
         from .mpm import MPMBatch
         m = req.mpm
-        self.stream.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(self.stream):
+        st = self.free_streams.pop()
+        st.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(st):
             b = MPMBatch(m["params"], m["positions"], m["act_id"], m["n_act"], 1, req.horizon,
                          K=self.K, window=min(100, req.horizon), dtype=self.dtype,
                          device=self.device)
             b.set_targets(np.asarray(req.target)[None, :3], np.asarray(req.target)[3:4])
             b.set_weights(*req.weights)
             b.start_forward()
-        self.jobs.append({"req": req, "b": b, "phase": "fwd", "seg": 0})
+        self.jobs.append({"req": req, "b": b, "phase": "fwd", "seg": 0, "stream": st})
 
     def active(self):
         return self.jobs
 
     def tick(self, chunk=None):
         finished = []
-        self.stream.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(self.stream):
-            for j in self.jobs:
-                b = j["b"]
+        cur = torch.cuda.current_stream()
+        for j in self.jobs:
+            if j["phase"] not in ("fwd", "bwd"):
+                continue
+            st, b = j["stream"], j["b"]
+            st.wait_stream(cur)
+            with torch.cuda.stream(st):
                 if j["phase"] == "fwd":
                     b.forward_segment(j["seg"])
                     j["seg"] += 1
@@ -350,13 +364,14 @@ class MPMRunner:
                         else:
                             j["phase"] = "done"
                             finished.append(j)
-                elif j["phase"] == "bwd":
+                else:
                     b.backward_segment(j["seg"])
                     j["seg"] -= 1
                     if j["seg"] < 0:
                         b.finish_backward()
                         j["phase"] = "done"
                         finished.append(j)
+            self.stream.wait_stream(st)
         return finished
 
     def collect(self, j):
@@ -366,22 +381,39 @@ class MPMRunner:
         if req.grad:
             req.grads = [g.clone() for g in b.grads]
         self.jobs.remove(j)
+        self.free_streams.append(j["stream"])
         return req
+
+
+class CudaEvents:
+    """Completion of an executor's launched tick: an event on its stream."""
+
+    def record(self, ex):
+        ev = torch.cuda.Event()
+        ev.record(ex.stream)
+        return ev
+
+    def done(self, ev):
+        return ev.query()
 
 
 class Server:
     """Continuous-batching loop over executors (MSPool, MPMRunner) with an
-    HBM admission budget.  ``run(requests)`` plays the arrivals in real time
-    and returns the requests with admission / completion times and results."""
+    HBM admission budget.  Each executor ticks at its own pace: a tick is
+    launched as soon as the executor's previous one has completed on the
+    device, so mass-spring slots are not held back by MPM segments.
+    ``run(requests)`` plays the arrivals in real time and returns the
+    requests with admission / completion times and results."""
 
     def __init__(self, executors, budget_bytes, chunk=50, clock=time.perf_counter,
-                 sleep=time.sleep, sync=None, max_active=None):
+                 sleep=time.sleep, events=None, max_active=None, poll_s=2e-5):
         self.ex = executors
         self.budget = budget_bytes
         self.chunk = chunk
         self.clock, self.sleep = clock, sleep
-        self.sync = sync or torch.cuda.synchronize
+        self.events = events or CudaEvents()
         self.max_active = max_active
+        self.poll_s = poll_s
         self.ticks = 0
 
     def _executor(self, req):
@@ -394,14 +426,16 @@ class Server:
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         queue = collections.deque()
         used, n_active = 0, 0
+        inflight = {}
         for e in self.ex:
             if hasattr(e, "start"):
                 e.start()
         t0 = self.clock()
+        now = lambda: self.clock() - t0
         done = []
         while pending or queue or n_active:
-            now = self.clock() - t0
-            while pending and pending[0].arrival <= now:
+            t = now()
+            while pending and pending[0].arrival <= t:
                 queue.append(pending.popleft())
             # admit in arrival order while the head fits (no overtaking)
             while queue:
@@ -414,25 +448,36 @@ class Server:
                         or (self.max_active is not None and n_active >= self.max_active)):
                     break
                 queue.popleft()
-                r.nbytes, r.admitted = need, self.clock() - t0
+                r.nbytes, r.admitted = need, now()
                 e.admit(r)
                 used += need
                 n_active += 1
-            if n_active == 0:
-                if pending:
-                    self.sleep(max(0.0, pending[0].arrival - (self.clock() - t0)))
-                continue
-            fin = [(e, e.tick(self.chunk)) for e in self.ex]
-            self.sync()
-            self.ticks += 1
-            now = self.clock() - t0
-            for e, ks in fin:
-                for k in ks:
-                    r = e.collect(k)
-                    r.done = now
+            # launch the next tick of every idle executor with work
+            for i, e in enumerate(self.ex):
+                if i not in inflight and e.busy():
+                    fin = e.tick(self.chunk)
+                    inflight[i] = (self.events.record(e), fin)
+                    self.ticks += 1
+            progressed = False
+            for i in list(inflight):
+                ev, fin = inflight[i]
+                if not self.events.done(ev):
+                    continue
+                del inflight[i]
+                progressed = True
+                t = now()
+                for k in fin:
+                    r = self.ex[i].collect(k)
+                    r.done = t
                     used -= r.nbytes
                     n_active -= 1
                     done.append(r)
+            if progressed:
+                continue
+            if inflight:
+                self.sleep(self.poll_s)
+            elif pending and n_active == 0:
+                self.sleep(max(0.0, pending[0].arrival - now()))
         return done
 
 

@@ -32,6 +32,9 @@ class FakeExec:
         self.active[r.rid] = [r, 0]
         self.log.append(("admit", r.rid))
 
+    def busy(self):
+        return len(self.active) > 0
+
     def tick(self, chunk):
         fin = []
         for k, (r, n) in list(self.active.items()):
@@ -48,6 +51,8 @@ class FakeExec:
 
 
 class Clock:
+    """Fake time: a tick of device work takes 1 s; waiting advances time."""
+
     def __init__(self):
         self.t = 0.0
 
@@ -55,10 +60,14 @@ class Clock:
         return self.t
 
     def sleep(self, dt):
-        self.t += dt
+        self.t += max(dt, 1e-3)
 
-    def tick(self):
-        self.t += 1.0          # one tick of device work = 1 s
+    # events: a tick launched now completes one second later
+    def record(self, ex):
+        return self.t + 1.0
+
+    def done(self, ev):
+        return self.t >= ev - 1e-9
 
 
 def _reqs(spec):
@@ -72,7 +81,8 @@ def test_fifo_admission_budget_and_latency():
     ex = [FakeExec("ms", 2, log), FakeExec("mpm", 1, log)]
     reqs = _reqs([("ms", 100, True, 0.0), ("ms", 50, False, 0.1), ("mpm", 200, False, 0.2),
                   ("ms", 300, True, 0.3), ("ms", 40, False, 5.0), ("mpm", 50, True, 5.5)])
-    srv = sv.Server(ex, budget_bytes=400, chunk=50, clock=ck, sleep=ck.sleep, sync=ck.tick)
+    srv = sv.Server(ex, budget_bytes=400, chunk=50, clock=ck, sleep=ck.sleep, events=ck,
+                    poll_s=0.05)
     done = srv.run(reqs)
     assert sorted(r.rid for r in done) == list(range(6))
     # admission strictly in arrival order
@@ -91,7 +101,7 @@ def test_fifo_admission_budget_and_latency():
 def test_request_over_the_whole_budget_is_rejected():
     ck = Clock()
     srv = sv.Server([FakeExec("ms", 4, [])], budget_bytes=100, chunk=50, clock=ck, sleep=ck.sleep,
-                    sync=ck.tick)
+                    events=ck)
     with pytest.raises(ValueError):
         srv.run(_reqs([("ms", 500, False, 0.0)]))
 
