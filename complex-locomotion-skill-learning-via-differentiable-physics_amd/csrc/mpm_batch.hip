@@ -371,7 +371,7 @@ __device__ __forceinline__ T run_sum(T v, int dist) {
 // is scattered with direct global atomics.  val(off, v) gives the values of
 // stencil offset off.  Every thread of the workgroup must call it
 // (uniform control flow, all lanes active for the lane shifts).
-template <int D, typename T, int NC, typename F>
+template <int D, typename T, int NC, int U, typename F>
 __device__ __forceinline__ void tile_scatter(bool valid, const Stencil<D, T> &s, const int lo[D],
                                              const int ext[D], T *tile, T *gk, int n,
                                              const TileBuf<T> &tb, int nb, F &&val) {
@@ -392,7 +392,7 @@ __device__ __forceinline__ void tile_scatter(bool valid, const Stencil<D, T> &s,
             for (int i = threadIdx.x; i < nodes * 4; i += blockDim.x) tile[i] = T(0);
             __syncthreads();
         }
-#pragma unroll
+#pragma unroll U
         for (int o = 0; o < K; ++o) {
             int off[D];
             unflat<D>(o, off);
@@ -468,7 +468,7 @@ __device__ __forceinline__ void sense_block(bool valid, int g, const T nx[3], co
 }
 
 // ---------------------------------------------------------------- P2G
-template <int D, typename T, bool STORE_F>
+template <int D, typename T, bool STORE_F, int U>
 __global__ __launch_bounds__(PB) void k_p2g_b(MP<T> p, PView<D, T> pv, const T *act,
                                               const int32_t *act_id, int n_act, T *F_out,
                                               T *grid, Blk bk, int32_t *err, TileBuf<T> tb) {
@@ -502,7 +502,7 @@ __global__ __launch_bounds__(PB) void k_p2g_b(MP<T> p, PView<D, T> pv, const T *
     int lo[D], ext[D];
     if (!block_bbox<D>(valid, s.base, lo, ext, red)) return;
     T *gk = grid + (size_t)k * gsize<D>(p.n) * 4;
-    tile_scatter<D, T, 4>(valid, s, lo, ext, tile, gk, p.n, tb.robot(k, bk.nblk), bk.nb,
+    tile_scatter<D, T, 4, U>(valid, s, lo, ext, tile, gk, p.n, tb.robot(k, bk.nblk), bk.nb,
                           [&](const int off[D], T v[4]) {
         T W = T(1), dpos[D];
 #pragma unroll
@@ -525,13 +525,21 @@ __global__ __launch_bounds__(PB) void k_p2g_b(MP<T> p, PView<D, T> pv, const T *
 }
 
 // ---------------------------------------------------------------- grid
+// One step's saved grid (segment step j): records by active-list position.
+template <typename T> struct GridSave {
+    T *g, *v;              // [B][nblk][64][4] grid totals, velocities
+    int32_t *list, *count; // [B][nblk], [B]
+};
+
 // Velocity of every node of the active blocks; with `clear`, the momentum /
-// mass records are zeroed after use (forward); flags are always reset.
+// mass records are zeroed after use (forward); flags are always reset.  With
+// sv.g set, the totals, velocities and the active list are saved too.
 template <int D, typename T>
 __global__ __launch_bounds__(PB) void k_grid_b(MP<T> p, T *grid, T *gvo, Blk bk, int clear,
-                                               TileBuf<T> tb) {
+                                               TileBuf<T> tb, GridSave<T> sv) {
     const int k = blockIdx.y, lane = threadIdx.x & 63;
     const TileBuf<T> tbk = tb.robot(k, bk.nblk);
+    if (sv.g && blockIdx.x == 0 && threadIdx.x == 0) sv.count[k] = bk.count[k];
     const int nw = gridDim.x * NWAVE;
     const int cnt = bk.count[k];
     const int32_t *lst = bk.list + (size_t)k * bk.nblk;
@@ -553,6 +561,12 @@ __global__ __launch_bounds__(PB) void k_grid_b(MP<T> p, T *grid, T *gvo, Blk bk,
             o[d] = bnd ? T(0) : vo;
         }
         st4(ok + (size_t)node * 4, o);
+        if (sv.g) {
+            const size_t si = (((size_t)k * bk.nblk + e) * 64 + lane) * 4;
+            st4(sv.g + si, r);
+            st4(sv.v + si, o);
+            if (lane == 0) sv.list[(size_t)k * bk.nblk + e] = bid;
+        }
         if (clear) {
             const T z[4] = {T(0), T(0), T(0), T(0)};
             st4(gk + (size_t)node * 4, z);
@@ -563,6 +577,32 @@ __global__ __launch_bounds__(PB) void k_grid_b(MP<T> p, T *grid, T *gvo, Blk bk,
             bk.flag[(size_t)k * bk.nblk + bid] = 0;
             tbk.cnt[bid] = 0;
         }
+    }
+}
+
+// Restore a saved step: grid totals and velocities of its active blocks,
+// and the active list itself, as the P2G + grid passes left them.
+template <int D, typename T>
+__global__ __launch_bounds__(PB) void k_grid_restore(MP<T> p, T *grid, T *gvo, Blk bk,
+                                                     GridSave<T> sv) {
+    const int k = blockIdx.y, lane = threadIdx.x & 63;
+    const int nw = gridDim.x * NWAVE;
+    const int cnt = sv.count[k];
+    const size_t G = gsize<D>(p.n);
+    T *gk = grid + (size_t)k * G * 4, *ok = gvo + (size_t)k * G * 4;
+    if (blockIdx.x == 0 && threadIdx.x == 0) bk.count[k] = cnt;
+    for (int e = blockIdx.x * NWAVE + (threadIdx.x >> 6); e < cnt; e += nw) {
+        const int bid = sv.list[(size_t)k * bk.nblk + e];
+        bool bnd;
+        int c[D];
+        const int node = block_node<D, T>(bid, lane, bk.nb, p, bnd, c);
+        const size_t si = (((size_t)k * bk.nblk + e) * 64 + lane) * 4;
+        T r[4], o[4];
+        ld4(sv.g + si, r);
+        ld4(sv.v + si, o);
+        st4(gk + (size_t)node * 4, r);
+        st4(ok + (size_t)node * 4, o);
+        if (lane == 0) bk.list[(size_t)k * bk.nblk + e] = bid;
     }
 }
 
@@ -805,7 +845,7 @@ template <typename T> struct SenseAdj {
 
 // G2P adjoint with the sensor / COM adjoint of state t+1 folded into the
 // incoming (x, v) adjoints: writes xb (state t), scatters dL/d grid_vo.
-template <int D, typename T>
+template <int D, typename T, int U>
 __global__ __launch_bounds__(PB) void k_g2p_adj_b(MP<T> p, int P, const T *x, const T *gvo,
                                                   const T *xb1, const T *vb1, const T *Cb1, T *xb,
                                                   T *gvo_b, SenseAdj<T> sa, Blk bk, TileBuf<T> tb) {
@@ -904,7 +944,7 @@ __global__ __launch_bounds__(PB) void k_g2p_adj_b(MP<T> p, int P, const T *x, co
     // scatter: dL/d grid_vo += W (vnb + 4/dx Cb dpos)
     int lo[D], ext[D];
     if (block_bbox<D>(inside, s.base, lo, ext, red)) {
-        tile_scatter<D, T, D>(inside, s, lo, ext, tile, gvo_b + (size_t)k * G * 4, p.n,
+        tile_scatter<D, T, D, U>(inside, s, lo, ext, tile, gvo_b + (size_t)k * G * 4, p.n,
                               tb.robot(k, bk.nblk), bk.nb, [&](const int off[D], T v[D]) {
             T W = T(1), dp[D];
 #pragma unroll
@@ -1076,6 +1116,18 @@ __global__ void k_copy(const T *src, T *dst, size_t n) {
 // ================================================================== C ABI
 using namespace dl;
 
+// Unroll factor of the scatter kernels' stencil-offset loop (registers vs
+// instruction-level parallelism), chosen at run time: 1, 3, 9 or 27.
+#define DL_UNROLL_DISPATCH(U_, ...)                 \
+    do {                                            \
+        switch (unroll) {                           \
+        case 1: { constexpr int U_ = 1; __VA_ARGS__; } break;   \
+        case 9: { constexpr int U_ = 9; __VA_ARGS__; } break;   \
+        case 27: { constexpr int U_ = 27; __VA_ARGS__; } break; \
+        default: { constexpr int U_ = 3; __VA_ARGS__; } break;  \
+        }                                           \
+    } while (0)
+
 namespace {
 
 int check_batch(const dl_mpm_batch *b) {
@@ -1142,6 +1194,14 @@ template <int D, typename T> Slot<D, T> slot(const dl_mpm_slots *s, int i, int B
     return {(T *)s->x + i * sv, (T *)s->v + i * sv, (T *)s->C + i * sm, (T *)s->F + i * sm};
 }
 
+template <int D, typename T> GridSave<T> save_at(const dl_mpm_batch *b, int j) {
+    if (!b->grid_save) return {nullptr, nullptr, nullptr, nullptr};
+    const Blk bk = make_blk<D>(b);
+    const size_t rec = (size_t)b->B * bk.nblk;
+    return {(T *)b->grid_save + j * rec * 64 * 4, (T *)b->vo_save + j * rec * 64 * 4,
+            b->list_save + j * rec, b->count_save + (size_t)j * b->B};
+}
+
 template <typename T> TileBuf<T> make_tiles(const dl_mpm_batch *b) {
     return {(T *)b->tiles, b->tile_hdr, b->blk_tile_count, b->blk_tiles, (b->P + PB - 1) / PB,
             b->tile_nodes, b->scatter == DL_MPM_SCATTER_STORE};
@@ -1161,7 +1221,9 @@ int forward_impl(const dl_mpm_batch *b, const dl_mpm_slots *st, int t0, int n, i
     const T *act = (const T *)b->act_out;
     const TileBuf<T> tb = make_tiles<T>(b);
     const size_t tile_bytes = (size_t)tb.tn * 4 * sizeof(T);
-    DL_CHECK_HIP(ensure_dynamic_lds((const void *)k_p2g_b<D, T, true>, tile_bytes));
+    const int unroll = b->scatter_unroll;
+    DL_UNROLL_DISPATCH(U_, DL_CHECK_HIP(ensure_dynamic_lds((const void *)k_p2g_b<D, T, true, U_>,
+                                                           tile_bytes)));
     if (control && (flags & DL_MPM_SENSE_FIRST)) {
         const Slot<D, T> s0 = slot<D, T>(st, 0, B, P);
         if constexpr (D == 3)
@@ -1180,11 +1242,14 @@ int forward_impl(const dl_mpm_batch *b, const dl_mpm_slots *st, int t0, int n, i
                                (T *)b->inp + (size_t)t * B * I, (T *)b->hid + (size_t)t * B * H,
                                (T *)b->act_out + (size_t)t * B * O, 1);
         PView<D, T> pv{a.x, a.v, a.C, a.F, P};
-        hipLaunchKernelGGL((k_p2g_b<D, T, true>), pg, dim3(PB), tile_bytes, s, mp, pv,
-                           act + (size_t)t * B * O, b->act_id, O, c.F, (T *)b->grid, bk, b->err,
-                           tb);
+        DL_UNROLL_DISPATCH(U_, hipLaunchKernelGGL((k_p2g_b<D, T, true, U_>), pg, dim3(PB),
+                                                  tile_bytes, s, mp, pv, act + (size_t)t * B * O,
+                                                  b->act_id, O, c.F, (T *)b->grid, bk, b->err,
+                                                  tb));
+        const GridSave<T> sv = (flags & DL_MPM_SAVE_GRID) ? save_at<D, T>(b, j)
+                                                           : GridSave<T>{nullptr, nullptr, nullptr, nullptr};
         hipLaunchKernelGGL((k_grid_b<D, T>), gg, dim3(PB), 0, s, mp, (T *)b->grid, (T *)b->grid_vo,
-                           bk, 1, tb);
+                           bk, 1, tb, sv);
         if (control && D == 3)
             hipLaunchKernelGGL((k_g2p_b<D, T, true>), pg, dim3(PB), NS * sizeof(T), s, mp, P, a.x,
                                (const T *)b->grid_vo, c.x, c.v, c.C, b->blk_count,
@@ -1217,19 +1282,29 @@ int backward_impl(const dl_mpm_batch *b, const dl_mpm_slots *seg, const dl_mpm_s
     const T *act = (const T *)b->act_out;
     const TileBuf<T> tb = make_tiles<T>(b);
     const size_t tile_bytes = (size_t)tb.tn * 4 * sizeof(T);
-    DL_CHECK_HIP(ensure_dynamic_lds((const void *)k_p2g_b<D, T, false>, tile_bytes));
-    DL_CHECK_HIP(ensure_dynamic_lds((const void *)k_g2p_adj_b<D, T>, tile_bytes));
+    const int unroll = b->scatter_unroll;
+    DL_UNROLL_DISPATCH(U_, DL_CHECK_HIP(ensure_dynamic_lds((const void *)k_p2g_b<D, T, false, U_>,
+                                                           tile_bytes)));
+    DL_UNROLL_DISPATCH(U_, DL_CHECK_HIP(ensure_dynamic_lds((const void *)k_g2p_adj_b<D, T, U_>,
+                                                           tile_bytes)));
     int cur = 0;
     for (int j = n - 1; j >= 0; --j) {
         const int t = t0 + j, nxt = 1 - cur;
         const Slot<D, T> a = slot<D, T>(seg, j, B, P);
         const Slot<D, T> A1 = slot<D, T>(adj, cur, B, P), A0 = slot<D, T>(adj, nxt, B, P);
         PView<D, T> pv{a.x, a.v, a.C, a.F, P};
-        hipLaunchKernelGGL((k_p2g_b<D, T, false>), pg, dim3(PB), tile_bytes, s, mp, pv,
-                           act + (size_t)t * B * O, b->act_id, O, (T *)nullptr, (T *)b->grid, bk,
-                           b->err, tb);
-        hipLaunchKernelGGL((k_grid_b<D, T>), gg, dim3(PB), 0, s, mp, (T *)b->grid, (T *)b->grid_vo,
-                           bk, 0, tb);
+        if (b->grid_save) {
+            hipLaunchKernelGGL((k_grid_restore<D, T>), gg, dim3(PB), 0, s, mp, (T *)b->grid,
+                               (T *)b->grid_vo, bk, save_at<D, T>(b, j));
+        } else {
+            DL_UNROLL_DISPATCH(U_, hipLaunchKernelGGL((k_p2g_b<D, T, false, U_>), pg, dim3(PB),
+                                                      tile_bytes, s, mp, pv,
+                                                      act + (size_t)t * B * O, b->act_id, O,
+                                                      (T *)nullptr, (T *)b->grid, bk, b->err, tb));
+            hipLaunchKernelGGL((k_grid_b<D, T>), gg, dim3(PB), 0, s, mp, (T *)b->grid,
+                               (T *)b->grid_vo, bk, 0, tb,
+                               GridSave<T>{nullptr, nullptr, nullptr, nullptr});
+        }
         SenseAdj<T> sa{};
         if (D == 3 && b->in_grad) {
             sa.in_b = (const T *)b->in_grad + (size_t)(t + 1) * B * I;
@@ -1242,9 +1317,10 @@ int backward_impl(const dl_mpm_batch *b, const dl_mpm_slots *seg, const dl_mpm_s
             sa.I = I;
             sa.scale = T(b->offset_scale);
         }
-        hipLaunchKernelGGL((k_g2p_adj_b<D, T>), pg, dim3(PB), tile_bytes, s, mp, P, a.x,
-                           (const T *)b->grid_vo, A1.x, A1.v, A1.C, A0.x, (T *)b->grid_vo_grad, sa,
-                           bk, tb);
+        DL_UNROLL_DISPATCH(U_, hipLaunchKernelGGL((k_g2p_adj_b<D, T, U_>), pg, dim3(PB),
+                                                  tile_bytes, s, mp, P, a.x,
+                                                  (const T *)b->grid_vo, A1.x, A1.v, A1.C, A0.x,
+                                                  (T *)b->grid_vo_grad, sa, bk, tb));
         hipLaunchKernelGGL((k_grid_adj_b<D, T>), gg, dim3(PB), 0, s, mp, (T *)b->grid,
                            (T *)b->grid_vo_grad, (T *)b->grid_grad, bk, tb);
         hipLaunchKernelGGL((k_p2g_adj_b<D, T>), pg, dim3(PB), O * sizeof(T), s, mp, pv,
@@ -1289,6 +1365,10 @@ int32_t dl_mpm_forward(const dl_mpm_batch *b, const dl_mpm_slots *st, int32_t t0
                  "outside the horizon %d", t0, t0 + n, b->horizon);
     if (flags & DL_MPM_CONTROL)
         if (int e = check_control(b)) return e;
+    if (flags & DL_MPM_SAVE_GRID)
+        DL_CHECK_ARG(b->grid_save && b->vo_save && b->list_save && b->count_save &&
+                         n <= b->save_steps,
+                     "dl_mpm_forward: DL_MPM_SAVE_GRID needs the save buffers for %d steps", n);
     MPMB_DISPATCH(b, forward_impl, b, st, t0, n, flags, (hipStream_t)stream);
 }
 
@@ -1301,6 +1381,9 @@ int32_t dl_mpm_backward(const dl_mpm_batch *b, const dl_mpm_slots *seg, const dl
                  "dl_mpm_backward: the adjoint needs 2 slots");
     DL_CHECK_ARG(n > 0 && t0 >= 0 && t0 + n <= b->horizon, "dl_mpm_backward: bad step range");
     DL_CHECK_ARG(b->grid_vo_grad && b->grid_grad && b->act_grad, "dl_mpm_backward: NULL buffer");
+    DL_CHECK_ARG(!b->grid_save || (b->vo_save && b->list_save && b->count_save &&
+                                   n <= b->save_steps),
+                 "dl_mpm_backward: saved grids cover %d steps, segment has %d", b->save_steps, n);
     if (b->in_grad) {
         if (int e = check_control(b)) return e;
         DL_CHECK_ARG(b->center_grad && b->vcom_grad && b->wgrad, "dl_mpm_backward: NULL buffer");

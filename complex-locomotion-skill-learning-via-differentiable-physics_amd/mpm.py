@@ -233,7 +233,8 @@ class MPMBatchC(Structure):
                     "grid", "grid_vo", "grid_vo_grad", "grid_grad", "blk_flag", "blk_list",
                     "blk_count", "err", "sens", "center", "vcom", "inp", "hid", "act_out",
                     "act_grad", "in_grad", "center_grad", "vcom_grad", "wgrad", "tiles",
-                    "tile_hdr", "blk_tile_count", "blk_tiles")])
+                    "tile_hdr", "blk_tile_count", "blk_tiles", "grid_save", "vo_save",
+                    "list_save", "count_save")] + [("save_steps", c_int32), ("scatter_unroll", c_int32)])
 
 
 class MPMSlotsC(Structure):
@@ -242,7 +243,7 @@ class MPMSlotsC(Structure):
                 ("F", ctypes.c_void_p), ("n_slots", c_int32), ("pad_", c_int32)]
 
 
-MPM_CONTROL, MPM_SENSE_FIRST = 1, 2
+MPM_CONTROL, MPM_SENSE_FIRST, MPM_SAVE_GRID = 1, 2, 4
 MPM_PB, MPM_TILE_MAX, MPM_TILE_CAP = 256, 8192, 32
 SCATTER_ATOMIC, SCATTER_STORE = 0, 1
 
@@ -296,7 +297,8 @@ class MPMBatch:
     def __init__(self, params, positions, act_id, n_act, B, T, K=100, window=None,
                  dtype=torch.float32, n_sin=10, run_period=100, offset_scale=0.2,
                  max_speed=0.08, activation="tanh", n_hidden=64, device="cuda",
-                 sort_particles=True, tile_nodes=None, scatter=SCATTER_ATOMIC):
+                 sort_particles=True, tile_nodes=None, scatter=SCATTER_ATOMIC, save_grid=True,
+                 unroll=3):
         if params.dim != 3:
             raise ValueError("MPMBatch drives 3D robots (the controller's sensors are 3D)")
         if params.n_grid % 4:
@@ -340,6 +342,15 @@ class MPMBatch:
         self.tiles = z(B, wpr, tile_nodes if scatter == SCATTER_STORE else 1, 4)
         self.tile_hdr = zi(B, wpr, 8)
         self.blk_tile_count, self.blk_tiles = zi(B, nblk), zi(B, nblk, MPM_TILE_CAP)
+        # per-step grids of a segment, saved by the segment recompute so the
+        # adjoint does not re-run P2G and the grid update (K x B x nblk
+        # blocks of 64 records, written only for active blocks)
+        self.save_grid = save_grid
+        emp = lambda *sh, dt=dtype: torch.empty(*sh, dtype=dt, device=dev)
+        self.grid_save = emp(K, B * nblk * 64 * 4) if save_grid else None
+        self.vo_save = emp(K, B * nblk * 64 * 4) if save_grid else None
+        self.list_save = emp(K, B * nblk, dt=torch.int32) if save_grid else None
+        self.count_save = emp(K, B, dt=torch.int32) if save_grid else None
         NS = 3 * self.G + 6
         self.sens = z(T + 1, B, NS)
         self.center, self.vcom = z(T + 1, B, 3), z(T + 1, B, 3)
@@ -374,7 +385,12 @@ class MPMBatch:
             act_grad=P_(self.act_grad), in_grad=P_(self.in_grad), center_grad=P_(self.cg),
             vcom_grad=P_(self.vg), wgrad=P_(self.wgrad), tiles=P_(self.tiles),
             tile_hdr=P_(self.tile_hdr), blk_tile_count=P_(self.blk_tile_count),
-            blk_tiles=P_(self.blk_tiles))
+            blk_tiles=P_(self.blk_tiles),
+            grid_save=P_(self.grid_save) if save_grid else None,
+            vo_save=P_(self.vo_save) if save_grid else None,
+            list_save=P_(self.list_save) if save_grid else None,
+            count_save=P_(self.count_save) if save_grid else None,
+            save_steps=K if save_grid else 0, scatter_unroll=unroll)
         self.grads = None
 
     def set_targets(self, tv, th):
@@ -449,8 +465,8 @@ class MPMBatch:
         """Adjoint of segment s (segments run last to first)."""
         t0, n = self._segments()[s]
         self.seg.copy_slot(0, s, self.ck)
-        nat.call("dl_mpm_forward", ctypes.byref(self.desc), ctypes.byref(self.seg.c), t0, n, 0,
-                 nat.stream())
+        nat.call("dl_mpm_forward", ctypes.byref(self.desc), ctypes.byref(self.seg.c), t0, n,
+                 MPM_SAVE_GRID if self.save_grid else 0, nat.stream())
         nat.call("dl_mpm_backward", ctypes.byref(self.desc), ctypes.byref(self.seg.c),
                  ctypes.byref(self.adj.c), t0, n, nat.stream())
 
@@ -472,6 +488,9 @@ class MPMBatch:
         own = sum(t.numel() * t.element_size() for t in vars(self).values()
                   if isinstance(t, torch.Tensor))
         own += sum(t.numel() * t.element_size() for t in self.W)
+        own += sum(t.numel() * t.element_size() for t in (self.grid_save, self.vo_save,
+                                                          self.list_save, self.count_save)
+                   if t is not None)
         return own + sum(f.numel() * f.element_size()
                          for sl in (self.ring, self.ck, self.seg, self.adj) for f in sl.fields())
 

@@ -322,7 +322,9 @@ class MPMRunner:
         P, n = len(m["positions"]), m["params"].n_grid
         nck = (req.horizon + ckpt_every - 1) // ckpt_every
         slots = 2 + nck + ckpt_every + 1 + 2
-        return es * (slots * 24 * P + 4 * 4 * n ** 3) + 4 * 3 * (n // 4) ** 3
+        nblk = (n // 4) ** 3
+        saved = ckpt_every * nblk * 64 * 4 * 2           # per-step grids of a segment
+        return es * (slots * 24 * P + 4 * 4 * n ** 3 + saved) + 4 * (3 + ckpt_every) * nblk
 
     def bytes_for(self, req):
         return self.estimate_bytes(req, self.K, torch.finfo(self.dtype).bits // 8)

@@ -343,7 +343,7 @@ int32_t dl_mpm_sensor_adj(int32_t dtype, int32_t B, int32_t P, int32_t n_groups,
  * in_grad [T+1][B][I] (row T zero), center_grad/vcom_grad [T+1][B][3] (loss
  * adjoints, e.g. from dl_loss_task_grad), wgrad [B][H*I + H + O*H + O]
  * (per-robot slices, accumulated).  I = n_sin + 3 + 3*n_groups + 4. */
-enum { DL_MPM_CONTROL = 1, DL_MPM_SENSE_FIRST = 2 };
+enum { DL_MPM_CONTROL = 1, DL_MPM_SENSE_FIRST = 2, DL_MPM_SAVE_GRID = 4 };
 /* Scatter tiles: particles are processed DL_MPM_PB per workgroup
  * (W = ceil(P / DL_MPM_PB) workgroups per robot), each accumulating its
  * stencil box in an LDS tile of tile_nodes nodes (a box larger than that is
@@ -379,6 +379,16 @@ typedef struct dl_mpm_batch {
     void *center_grad, *vcom_grad, *wgrad;
     void *tiles;
     int32_t *tile_hdr, *blk_tile_count, *blk_tiles;
+    /* Saved grids of a segment (optional, all NULL or none): for segment
+     * step j, the active blocks' grid records and velocities grid_save /
+     * vo_save [K][B][nblk][E^D][4] (by list position), list_save [K][B][nblk],
+     * count_save [K][B].  dl_mpm_forward with DL_MPM_SAVE_GRID fills them;
+     * dl_mpm_backward then restores each step's grid instead of re-running
+     * P2G and the grid update (same values, one pass fewer). */
+    void *grid_save, *vo_save;
+    int32_t *list_save, *count_save;
+    int32_t save_steps;      /* K: steps the save buffers hold */
+    int32_t scatter_unroll;  /* stencil-offset loop unroll of the scatter kernels: 1, 3, 9, 27 */
 } dl_mpm_batch;
 
 /* Ring of particle-state slots: slot s of x/v at x + s*B*D*P reals, of C/F

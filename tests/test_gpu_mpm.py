@@ -170,22 +170,25 @@ def _run_batched(params, op, pos, ids, B, T, K, W, dtype, sort=True, **kw):
     return batch, loss.item(), grads, lo.item(), [w.grad for w in Wr]
 
 
-@pytest.mark.parametrize("prec,K,scatter,tile", [
-    ("f64", 7, dm.SCATTER_ATOMIC, None), ("f64", 24, dm.SCATTER_STORE, None),
-    ("f32", 7, dm.SCATTER_ATOMIC, None), ("f32", 7, dm.SCATTER_STORE, None),
-    ("f64", 7, dm.SCATTER_STORE, 64)])
-def test_mpm_batched_rollout_matches_oracle(prec, K, scatter, tile):
+@pytest.mark.parametrize("prec,K,scatter,tile,save,unroll", [
+    ("f64", 7, dm.SCATTER_ATOMIC, None, True, 3), ("f64", 24, dm.SCATTER_STORE, None, True, 3),
+    ("f32", 7, dm.SCATTER_ATOMIC, None, True, 3), ("f32", 7, dm.SCATTER_STORE, None, True, 3),
+    ("f64", 7, dm.SCATTER_STORE, 64, True, 3), ("f64", 7, dm.SCATTER_ATOMIC, None, False, 3),
+    ("f64", 7, dm.SCATTER_ATOMIC, None, True, 1), ("f32", 7, dm.SCATTER_ATOMIC, None, True, 27)])
+def test_mpm_batched_rollout_matches_oracle(prec, K, scatter, tile, save, unroll):
     """Checkpointed MPM training rollout (sensors, MLP, P2G/grid/G2P, windowed
     COM loss) and its adjoint through every segment, against autograd of the
     FP64 oracle.  Particles are block-sorted internally (order-independent
     results); K = 7 exercises partial segments and odd segment lengths; both
     tile commit modes (global atomics, store + gather); a 64-node tile forces
-    the slab path."""
-    _say(f"mpm batched rollout {prec} K={K} scatter={scatter} tile={tile}")
+    the slab path; with and without the saved per-step grids."""
+    _say(f"mpm batched rollout {prec} K={K} scatter={scatter} tile={tile} save={save} "
+         f"unroll={unroll}")
     params, op, pos, ids = _batched_case()
     B, T, W = 2, 24, 12
     batch, loss, grads, lo, gref = _run_batched(params, op, pos, ids, B, T, K, W, DT[prec],
-                                                scatter=scatter, tile_nodes=tile)
+                                                scatter=scatter, tile_nodes=tile, save_grid=save,
+                                                unroll=unroll)
     tl, tg = (1e-10, 1e-7) if prec == "f64" else (1e-4, 1e-3)
     assert abs(loss - lo) / abs(lo) < tl
     for a, b in zip(grads, gref):
