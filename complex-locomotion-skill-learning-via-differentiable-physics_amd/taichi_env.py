@@ -7,8 +7,11 @@ The GUI/visualiser and evaluation-only target scripts are not part of this
 path.
 """
 
+import collections
 import ctypes
+import itertools
 
+import numpy as np
 import torch
 
 from . import _native as nat
@@ -68,6 +71,16 @@ class TaichiEnv:
         self.target_h = mk((T, B), "target_h")
         self.target_c = mk((T, B), "target_c")
         self.current_max_height = mk((), "current_max_height")
+
+        # validation matrix: every combination of the validation targets of
+        # the trained loss types (taichi_env.py:54-72)
+        self.validate_targets = {k: list(v) for k, v in cfg.get("validation", {}).items()
+                                 if k in self.task}
+        self.validate_targets_values = collections.defaultdict(list)
+        keys = list(self.validate_targets)
+        for element in itertools.product(*[self.validate_targets[k] for k in keys]):
+            for i, name in enumerate(keys):
+                self.validate_targets_values[name].append(element[i])
 
         self.solver = SolverMassSpring(config, device=device)
         s = self.solver
@@ -134,6 +147,33 @@ class TaichiEnv:
         self.target_v.data[: steps + 1] = tv[None]
         self.target_h.data[: steps + 1] = th[None]
         self.target_c.data[: steps + 1] = 0
+
+    def initialize_validate(self, steps, output_v, output_h, output_c):
+        """taichi_env.py:378-383: per-robot constant targets (x velocity,
+        height, crawl) for every step."""
+        f = lambda a: torch.as_tensor(np.asarray(a, np.float64), dtype=self.dtype, device=self.device)
+        self.target_v.data[:steps, :, 0] = f(output_v)[None]
+        self.target_h.data[:steps] = f(output_h)[None]
+        self.target_c.data[:steps] = f(output_c)[None]
+
+    def initialize_interactive(self, steps, output_v, output_h, output_c):
+        """taichi_env.py:333-338: one x velocity, height and crawl target for all."""
+        self.target_v.data[:steps, :, 0] = output_v
+        self.target_h.data[:steps] = output_h
+        self.target_c.data[:steps] = output_c
+
+    def initialize_script(self, steps, x0, y0, x1, y1, x2, y2, x3, y3):
+        """taichi_env.py:340-376: velocity targets (x, z) switching every 1000
+        steps through the four given pairs, then height -1 (crouch) for steps
+        4000-4999 and +1 (jump) after."""
+        t = torch.arange(steps, device=self.device)
+        seg = torch.clamp(t // 1000, max=5)
+        vx = torch.tensor([x0, x1, x2, x3, 0.0, 0.0], dtype=self.dtype, device=self.device)[seg]
+        vz = torch.tensor([y0, y1, y2, y3, 0.0, 0.0], dtype=self.dtype, device=self.device)[seg]
+        h = torch.tensor([0.0, 0.0, 0.0, 0.0, -1.0, 1.0], dtype=self.dtype, device=self.device)[seg]
+        self.target_v.data[:steps, :, 0] = vx[:, None]
+        self.target_v.data[:steps, :, 2] = vz[:, None]
+        self.target_h.data[:steps] = h[:, None]
 
     # ------------------------------------------------------------ losses
     def compute_loss_velocity(self, steps, written):

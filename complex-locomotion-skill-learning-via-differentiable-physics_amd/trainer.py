@@ -8,6 +8,7 @@ every launched kernel replays in reverse on exit.  ``simulate_task`` runs the
 quadruped workload's loop (``steps`` advances, windowed COM loss).
 """
 
+import numpy as np
 import torch
 
 from . import _native as nat
@@ -50,13 +51,39 @@ class DiffPhyTrainer:
         for l in env.loss_dict_batch.values():
             l.fill(0)
 
-    def simulate(self, steps, train=True, iter=0, max_speed=0.08, max_height=0.1,
-                 loss_enable=None, pool=None):
-        """diffphy_trainer.py:219-256 (training branch)."""
+    def simulate(self, steps, output_v=None, output_h=None, output_c=None, train=True, iter=0,
+                 max_speed=0.08, max_height=0.1, loss_enable=None, pool=None):
+        """diffphy_trainer.py:219-268.  Training: T-1 advances under the tape.
+        Evaluation (train=False): validation targets (2D) or the scripted
+        target schedule (3D), steps+1 advances without a tape, losses of
+        every row."""
         env, nn = self.taichi_env, self.nn
-        if not train:
-            raise NotImplementedError("evaluation rollouts are not part of this path")
         loss_enable = self.loss_enable if loss_enable is None else set(loss_enable)
+        if not train:
+            if steps + 1 > env.max_steps:
+                raise ValueError(f"evaluation writes state {steps + 1}; the fields hold "
+                                 f"{env.max_steps + 1} rows")
+            env.solver.clear_states(steps)
+            nn.clear()
+            if env.dim == 2:
+                if output_c is None:
+                    output_c = np.zeros(env.batch_size)
+                env.initialize_validate(steps, output_v, output_h, output_c)
+            else:
+                env.initialize_script(steps + 1, -max_speed, 0, 0, max_speed, max_speed, 0, 0,
+                                      -max_speed)
+            self._clear_losses()
+            with torch.no_grad():
+                for t in range(steps + 1):
+                    env.solver.pre_advance(t)
+                    env.nn_input(t, 0, max_speed, max_height)
+                    if t % self.control_length == 0:
+                        nn.forward(t)
+                    else:
+                        self.diff_copy(t)
+                    env.solver.advance(t)
+                env.get_loss(steps, loss_enable, written=steps + 1)
+            return
         env.solver.clear_states(steps)
         nn.clear()
         env.initialize_train(iter, steps, max_speed, max_height, pool=pool)
@@ -89,6 +116,58 @@ class DiffPhyTrainer:
                 env.solver.advance(t)
             env.solver.compute_center(steps)
             env.get_loss(steps, {"task"})
+
+    # ------------------------------------------------------------ evaluation
+    def evaluate_weights(self, weights=None, loss_enable=None, steps=None):
+        """One model of ``_evaluate`` (diffphy_trainer.py:319-435): set up the
+        robot, load the weights (a ``[W1, W2, b1, b2]`` pickle path or
+        tensors), run the evaluation rollout and return the per-robot losses,
+        keyed like the reference's metrics: ``task_loss{suffix}`` and
+        ``{name}_loss{suffix}`` with suffix ``_{target}_{value}...`` of the
+        validation matrix (or ``_{k}`` without one)."""
+        env = self.taichi_env
+        loss_enable = set(env.task) if loss_enable is None else set(loss_enable)
+        steps = env.max_steps - 1 if steps is None else steps
+        env.setup_robot()
+        if isinstance(weights, str):
+            self.nn.load_weights(weights)
+        elif weights is not None:
+            self.nn.set_weights(*weights)
+        vv = env.validate_targets_values
+        keys = list(vv)
+        B = env.batch_size
+        if keys and len(vv[keys[0]]) != B:
+            raise ValueError(f"validation matrix has {len(vv[keys[0]])} cases, batch is {B}")
+        suffix = ["".join(f"_{k}_{vv[k][i]}" for k in keys) if keys else f"_{i}" for i in range(B)]
+        out_v = np.asarray(vv.get("velocity", np.zeros(B)), np.float64)
+        out_h = np.asarray(vv.get("height", np.zeros(B)), np.float64)
+        out_c = np.asarray(vv.get("crawl", np.zeros(B)), np.float64)
+        self.simulate(steps, output_v=out_v, output_h=out_h, output_c=out_c, train=False,
+                      max_speed=self.max_speed, max_height=self.max_height,
+                      loss_enable=loss_enable)
+        lb = env.loss_batch.data.double().cpu().numpy()
+        res = {f"task_loss{suffix[k]}": float(lb[k]) for k in range(B)}
+        for name in keys:
+            v = env.loss_dict_batch[f"loss_{name}"].data.double().cpu().numpy()
+            res.update({f"{name}_loss{suffix[k]}": float(v[k]) for k in range(B)})
+        return res
+
+    def export_obj(self, folder, steps, every=10, robot=0):
+        """Mesh frames of one robot every ``every`` states as OBJ files
+        ``{t:06d}.obj`` (vertices and the design's triangles, 1-based), the
+        3D branch of the evaluation visualizer (diffphy_trainer.py:336-347)."""
+        import os
+        os.makedirs(folder, exist_ok=True)
+        x = self.taichi_env.x.view[: steps + 1: every, robot].double().cpu().numpy()
+        faces = self.config.get_config()["robot"].get("faces", [])
+        paths = []
+        for i, frame in enumerate(x):
+            fn = os.path.join(folder, f"{i:06d}.obj")
+            with open(fn, "w") as f:
+                f.writelines(f"v {a:.6f} {b:.6f} {c:.6f}\n" for a, b, c in frame)
+                f.writelines(f"f {p0 + 1} {p1 + 1} {p2 + 1}\n" for p0, p1, p2 in faces)
+            paths.append(fn)
+        return paths
 
     def run_step(self):
         """diffphy_trainer.py:277-302 without logging: one rollout, TNS, Adam."""

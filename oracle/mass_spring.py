@@ -348,6 +348,44 @@ def rollout_ref(robot, cfg, weights, tv, th, tc, x0, v0, record=False):
     return lv + lr, lvb + lrb, traj
 
 
+def script_targets(steps, B, x0, y0, x1, y1, x2, y2, x3, y3):
+    """initialize_script (taichi_env.py:340-376): (x, z) velocity targets from
+    the four pairs for steps [0,1000), [1000,2000), [2000,3000), [3000,4000);
+    then zero velocity with height target -1 below 5000 and +1 after."""
+    tv = torch.zeros(steps, B, 3, dtype=F64)
+    th = torch.zeros(steps, B, dtype=F64)
+    pairs = [(x0, y0), (x1, y1), (x2, y2), (x3, y3)]
+    for t in range(steps):
+        if t < 4000:
+            tv[t, :, 0], tv[t, :, 2] = pairs[t // 1000]
+        else:
+            th[t] = -1.0 if t < 5000 else 1.0
+    return tv, th, torch.zeros(steps, B, dtype=F64)
+
+
+def rollout_eval(robot, cfg, weights, tv, th, tc, x0, v0):
+    """simulate(train=False) (diffphy_trainer.py:257-268): steps+1 advances
+    without a tape, every center written; get_loss(steps) with the
+    velocity and rotation losses.  Returns (loss, loss_batch)."""
+    T = cfg.steps
+    W1, b1, W2, b2 = weights
+    B, N, _ = x0.shape
+    xs, cs = [x0], []
+    x, v = x0, v0
+    for t in range(T + 1):
+        c = (x / N).sum(1)
+        inp = nn_input(t, x, v, c, tv[t], th[t], tc[t], cfg.layout)
+        _, a = mlp_forward(inp, W1, b1, W2, b2, cfg.act)
+        x, v = step(x, v, a, robot, cfg.physics)
+        xs.append(x)
+        cs.append(c)
+    center = torch.stack(cs)                                      # rows 0..T
+    X = torch.stack(xs[: T + 1])
+    lv, lvb = loss_velocity(center, tv, T, cfg.run_period, cfg.turn_period)
+    lr, lrb = loss_rotation(X, center, th, T, cfg.run_period, cfg.turn_period)
+    return lv + lr, lvb + lrb
+
+
 @dataclass
 class TaskConfig:
     """The quadruped workload: T advances, per-task constant targets, loss =

@@ -499,3 +499,45 @@ def test_batched_gradient_matches_central_differences():
         fd = (loss_of(wp) - loss_of(wm)) / (2 * eps)
         ad = grads[pi].view(-1)[j].item()
         assert abs(fd - ad) <= 1e-4 * abs(ad) + 1e-6 * gmax, (pi, j, fd, ad)
+
+
+def test_training_iterations_match_oracle():
+    """Three training iterations of the reference-shaped loop (rollout under
+    the tape, TNS, Adam with b1 = b2 = 0.9 and iteration-based bias
+    correction) against the oracle's rollout + autograd + Adam, FP64: the
+    loss curve, TNS and final weights (diffphy_trainer.py:277-302,
+    nn.py:173-191)."""
+    _say("training iterations")
+    T, B = 130, 2
+    config = _ref_small_config(T, B, "f64")
+    trainer = dl.DiffPhyTrainer(config)
+    env, nn = trainer.taichi_env, trainer.nn
+    env.setup_robot()
+    nn.weights_init(seed=31)
+    nn.clear_adam()
+    W = [w.data.double().cpu().clone() for w in (nn.weights1, nn.bias1, nn.weights2, nn.bias2)]
+    ms = [torch.zeros_like(w) for w in W]
+    vs = [torch.zeros_like(w) for w in W]
+    rob = oracle_reference_robot()
+    cfg = om.RefConfig(steps=T, run_period=20, turn_period=60, jump_period=60,
+                       layout=om.SensorLayout(run_period=20))
+    lr = nn.learning_rate
+    for it in range(3):
+        pool = torch.rand(B * (T // 60 + 1) * 3, dtype=torch.float64,
+                          generator=torch.Generator().manual_seed(40 + it))
+        trainer.simulate(T, iter=it, max_speed=0.1, max_height=0.1, pool=pool)
+        tns = nn.get_TNS()
+        nn.gradient_update(it)
+        torch.cuda.synchronize()
+        tv, th, tc = om.train_targets(pool.numpy(), T, B, 60, 0.1)
+        Wr = [w.clone().requires_grad_() for w in W]
+        x0, v0 = om.initial_state(rob, B)
+        lo, _, _ = om.rollout_ref(rob, cfg, Wr, tv, th, tc, x0, v0)
+        lo.backward()
+        tns_o = sum(float((w.grad ** 2).sum()) for w in Wr)
+        for w, m, v, wr in zip(W, ms, vs, Wr):
+            om.adam_update(w, wr.grad, m, v, it, lr=lr)
+        assert abs(env.loss[None] - lo.item()) <= 1e-10 * abs(lo.item()), it
+        assert abs(tns - tns_o) <= 1e-8 * tns_o, it
+    for a, b in zip((nn.weights1, nn.bias1, nn.weights2, nn.bias2), W):
+        assert rel_err(a.data.double().cpu(), b) < 1e-9
