@@ -65,7 +65,7 @@ def main():
     ap.add_argument("--mpm-ckpt", type=int, default=100)
     ap.add_argument("--mpm-steps", type=int, default=1)
     ap.add_argument("--mpm-scatter", type=int, default=0, help="0: tile atomics, 1: store + gather")
-    ap.add_argument("--mpm-tile", type=int, default=4096, help="LDS scatter tile, nodes")
+    ap.add_argument("--mpm-tile", type=int, default=2048, help="LDS scatter tile, nodes")
     ap.add_argument("--skip-uneven", action="store_true")
     ap.add_argument("--uneven-robots", type=int, default=512)
     ap.add_argument("--uneven-iters", type=int, default=3)

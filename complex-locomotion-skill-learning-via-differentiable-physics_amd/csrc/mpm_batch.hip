@@ -347,83 +347,278 @@ __device__ __forceinline__ Seg lane_runs(int key) {
 }
 
 // Inclusive segmented sum over a run; the tail lane ends with the run total.
+// Step n adds the value n lanes down when the run reaches that far
+// (dist >= n); the 0/1 factors m[n] are per lane.  In FP32 each step is one
+// multiply-add whose first operand is the shifted value, which the compiler
+// folds into a single DPP instruction.
+template <typename T> struct RunMask {
+    T m1, m2, m4, m8;
+};
 template <typename T>
-__device__ __forceinline__ T run_sum(T v, int dist) {
-    T t = dpp_shr<1>(v);
-    v += dist >= 1 ? t : T(0);
-    t = dpp_shr<2>(v);
-    v += dist >= 2 ? t : T(0);
-    t = dpp_shr<4>(v);
-    v += dist >= 4 ? t : T(0);
-    t = dpp_shr<8>(v);
-    v += dist >= 8 ? t : T(0);
+__device__ __forceinline__ RunMask<T> run_mask(int dist) {
+    return {dist >= 1 ? T(1) : T(0), dist >= 2 ? T(1) : T(0), dist >= 4 ? T(1) : T(0),
+            dist >= 8 ? T(1) : T(0)};
+}
+template <typename T>
+__device__ __forceinline__ T run_sum(T v, const RunMask<T> &m) {
+    v = fma(dpp_shr<1>(v), m.m1, v);
+    v = fma(dpp_shr<2>(v), m.m2, v);
+    v = fma(dpp_shr<4>(v), m.m4, v);
+    v = fma(dpp_shr<8>(v), m.m8, v);
     return v;
+}
+
+// Values scattered to stencil node base + off: channel c is
+// W(off) * (b[c] + sum_d L[c][d] * off[d]) for c < NL and W(off) * b[c] for
+// the rest, W(off) the product of the particle's quadratic B-spline weights.
+template <int D, typename T, int NC> struct LinVal {
+    T b[NC];
+    T L[NC][D];
+};
+
+// Run totals of every stencil offset.  The offsets run as an outer loop over
+// the axes before the last (unrolled U times) and an unrolled inner loop over
+// the last axis, which advances the linear values by L[c][D-1].  For offset
+// o = 3 * oo + oz, emit(o, off, v) is called on every lane, v holding the
+// run total on the run's tail lane; mine(off) masks a lane's contribution.
+// All lanes must be active (lane shifts).
+template <int D, typename T, int NC, int NL, int U, typename Mine, typename Emit>
+__device__ __forceinline__ void run_offsets(const Stencil<D, T> &s, const T wz[3],
+                                            const LinVal<D, T, NC> &z, const RunMask<T> &rm,
+                                            Mine &&mine, Emit &&emit) {
+    constexpr int KO = D == 2 ? 3 : 9;
+#pragma unroll U
+    for (int oo = 0; oo < KO; ++oo) {
+        int off[D];
+        unflat<D - 1>(oo, off);
+        T Wo = T(1), m[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) m[c] = z.b[c];
+#pragma unroll
+        for (int d = 0; d < D - 1; ++d) {
+            Wo *= wsel(off[d], s.w[0][d], s.w[1][d], s.w[2][d]);
+#pragma unroll
+            for (int c = 0; c < NL; ++c) m[c] += z.L[c][d] * T(off[d]);
+        }
+        Wo = mine(off) ? Wo : T(0);
+#pragma unroll
+        for (int oz = 0; oz < 3; ++oz) {
+            off[D - 1] = oz;
+            const T W = Wo * wz[oz];
+            T v[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) v[c] = run_sum(W * m[c], rm);
+            emit(3 * oo + oz, off, v);
+#pragma unroll
+            for (int c = 0; c < NL; ++c) m[c] += z.L[c][D - 1];
+        }
+    }
 }
 
 // Scatter-add NC values per stencil node of the workgroup's particles into
 // robot grid gk ([n^D][4] records).  For each stencil offset the particles
-// of one cell (a run of lanes) are summed in registers first, and only the
-// run's tail lane adds to memory: LDS atomics are paid per lane, so this
-// divides their cost by the particles per cell.  The sums go through the
-// LDS tile: the node box (lo, ext) is cut into x-slabs of at most
-// tb.tn nodes, each accumulated in LDS and committed with one global
-// atomic per nonzero component; a box whose single x-plane exceeds the tile
-// is scattered with direct global atomics.  val(off, v) gives the values of
-// stencil offset off.  Every thread of the workgroup must call it
-// (uniform control flow, all lanes active for the lane shifts).
-template <int D, typename T, int NC, int U, typename F>
+// of one cell (a run of lanes) are summed in registers first (run_offsets),
+// so memory sees one contribution per run rather than per particle.  Two
+// ways to combine the runs' contributions, chosen per workgroup:
+//
+//  * staged (no LDS atomics per contribution): each run's tail lane stores
+//    its K totals to stage[run][o] and links the run into a per-cell list
+//    (one LDS exchange per run); then each thread takes nodes of the box
+//    and sums, over its 3^D neighbour cells, the staged totals of their
+//    runs, committing one global atomic per nonzero component.  Needs the
+//    stage (runs x K x 4 reals), the lists and the padded cell heads to fit
+//    the LDS buffer (tile, tb.tn x 4 reals), and atomic commits.
+//  * dense tile: the node box (lo, ext) is cut into x-slabs of at most tb.tn
+//    nodes, each accumulated in LDS with atomics and committed with one
+//    global atomic per nonzero component (or stored, tb.store); a box whose
+//    single x-plane exceeds the tile is scattered with direct global atomics.
+//
+// Every thread of the workgroup must call it (uniform control flow, all
+// lanes active for the lane shifts).
+template <int D, typename T, int NC, int NL, int U>
 __device__ __forceinline__ void tile_scatter(bool valid, const Stencil<D, T> &s, const int lo[D],
                                              const int ext[D], T *tile, T *gk, int n,
-                                             const TileBuf<T> &tb, int nb, F &&val) {
+                                             const TileBuf<T> &tb, int nb,
+                                             const LinVal<D, T, NC> &lv) {
     constexpr int K = D == 2 ? 9 : 27;
+    __shared__ int wave_runs[NWAVE];
     int key = 0;
 #pragma unroll
     for (int d = 0; d < D; ++d) key = key * n + s.base[d];
     if (!valid) key = -1 - (int)(threadIdx.x & 63);
     const Seg sg = lane_runs(key);
+    const RunMask<T> rm = run_mask<T>(sg.dist);
+    // A lane without a particle takes part in the lane sums with zeros; its
+    // registers are cleared here so that no stale value enters the sums.
+    T wz[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) wz[i] = valid ? s.w[i][D - 1] : T(0);
+    LinVal<D, T, NC> z;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        z.b[c] = valid ? lv.b[c] : T(0);
+#pragma unroll
+        for (int d = 0; d < D; ++d) z.L[c][d] = valid && c < NL ? lv.L[c][d] : T(0);
+    }
+
+    // ---- staged path
+    const bool tail = valid && sg.tail;
+    const unsigned long long tmask = __ballot(tail);
+    const int wid = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) wave_runs[wid] = __popcll(tmask);
+    __syncthreads();
+    int RW = 0;
+#pragma unroll
+    for (int w = 0; w < NWAVE; ++w) RW = max(RW, wave_runs[w]);
+    int hcells = 1, nodes = 1;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        hcells *= ext[d] + 2;
+        nodes *= ext[d];
+    }
+    const size_t RC = (size_t)NWAVE * RW;
+    const size_t stage_bytes = RC * K * 4 * sizeof(T);
+    const int nflag_words = (nodes + 3) / 4;
+    const bool staged = !tb.store && stage_bytes + (RC + hcells + nflag_words) * sizeof(int) <=
+                                         (size_t)tb.tn * 4 * sizeof(T);
+    if (staged) {
+        T *stage = tile;                                                 // [RC][K][4]
+        int *nxt = reinterpret_cast<int *>(reinterpret_cast<char *>(tile) + stage_bytes);
+        int *head = nxt + RC;                                            // padded cells
+        int *nflag_w = head + hcells;                                    // node touched
+        unsigned char *nflag = reinterpret_cast<unsigned char *>(nflag_w);
+        for (int i = threadIdx.x; i < hcells; i += blockDim.x) head[i] = -1;
+        for (int i = threadIdx.x; i < nflag_words; i += blockDim.x) nflag_w[i] = 0;
+        __syncthreads();
+        const int rid = wid * RW + (int)__builtin_amdgcn_mbcnt_hi(
+                                        (unsigned)(tmask >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((unsigned)tmask, 0u));
+        if (tail) {
+            int ci = 0, nl = 0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                ci = ci * (ext[d] + 2) + s.base[d] - lo[d] + 2;
+                nl = nl * ext[d] + s.base[d] - lo[d];
+            }
+            nxt[rid] = atomicExch(head + ci, rid);
+            // the run's 3^D nodes (the same byte may be set by several lanes)
+#pragma unroll
+            for (int o = 0; o < K; ++o) {
+                int off[D], dl = 0;
+                unflat<D>(o, off);
+#pragma unroll
+                for (int d = 0; d < D; ++d) dl = dl * ext[d] + off[d];
+                nflag[nl + dl] = 1;
+            }
+        }
+        run_offsets<D, T, NC, NL, U>(s, wz, z, rm, [&](const int *) { return valid; },
+                                     [&](int o, const int *, const T v[NC]) {
+            if (tail) {
+                T r[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) r[c] = c < NC ? v[c] : T(0);
+                st4(stage + ((size_t)rid * K + o) * 4, r);
+            }
+        });
+        __syncthreads();
+        for (int i = threadIdx.x; i < nodes; i += blockDim.x) {
+            if (!nflag[i]) continue;
+            int rem = i, c[D];
+#pragma unroll
+            for (int d = D - 1; d >= 0; --d) {
+                c[d] = rem % ext[d];
+                rem /= ext[d];
+            }
+            int node = 0, ci0 = 0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                node = node * n + lo[d] + c[d];
+                ci0 = ci0 * (ext[d] + 2) + c[d] + 2;
+            }
+            // Offsets in groups of 9: the heads of the group's cells, then
+            // (unconditionally, from run 0 for an empty cell, masked) their
+            // first runs' totals and list successors, so each step is one
+            // round of independent LDS loads; longer lists are walked last.
+            T acc[NC];
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) acc[cc] = T(0);
+#pragma unroll
+            for (int g = 0; g < K; g += 9) {
+                int h[9], h2[9];
+#pragma unroll
+                for (int j = 0; j < 9; ++j) {
+                    int off[D], dl = 0;
+                    unflat<D>(g + j, off);
+#pragma unroll
+                    for (int d = 0; d < D; ++d) dl = dl * (ext[d] + 2) + off[d];
+                    h[j] = head[ci0 - dl];
+                }
+#pragma unroll
+                for (int j = 0; j < 9; ++j) {
+                    const int r = max(h[j], 0);
+                    T v4[4];
+                    ld4(stage + ((size_t)r * K + g + j) * 4, v4);
+                    const int r2 = nxt[r];
+                    h2[j] = h[j] >= 0 ? r2 : -1;
+#pragma unroll
+                    for (int cc = 0; cc < NC; ++cc) acc[cc] += h[j] >= 0 ? v4[cc] : T(0);
+                }
+#pragma unroll
+                for (int j = 0; j < 9; ++j) {
+                    for (int r = h2[j]; r >= 0; r = nxt[r]) {
+                        T v4[4];
+                        ld4(stage + ((size_t)r * K + g + j) * 4, v4);
+#pragma unroll
+                        for (int cc = 0; cc < NC; ++cc) acc[cc] += v4[cc];
+                    }
+                }
+            }
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc)
+                if (acc[cc] != T(0)) gadd(gk + (size_t)node * 4 + cc, acc[cc]);
+        }
+        __syncthreads();
+        return;
+    }
+
+    // ---- dense tile path
     int plane = 1;
 #pragma unroll
     for (int d = 1; d < D; ++d) plane *= ext[d];
     const bool direct = plane > tb.tn;
     const int sx = direct ? ext[0] : min(ext[0], tb.tn / plane);
     for (int x0 = 0; x0 < ext[0]; x0 += sx) {
-        const int nx = min(sx, ext[0] - x0), nodes = nx * plane;
+        const int nx = min(sx, ext[0] - x0), snodes = nx * plane;
         if (!direct) {
-            for (int i = threadIdx.x; i < nodes * 4; i += blockDim.x) tile[i] = T(0);
+            for (int i = threadIdx.x; i < snodes * 4; i += blockDim.x) tile[i] = T(0);
             __syncthreads();
         }
-#pragma unroll U
-        for (int o = 0; o < K; ++o) {
-            int off[D];
-            unflat<D>(o, off);
+        run_offsets<D, T, NC, NL, U>(s, wz, z, rm, [&](const int *off) {
             const int l0 = s.base[0] - lo[0] + off[0] - x0;
-            const bool mine = valid && (direct || (l0 >= 0 && l0 < nx));
-            T v[NC];
-            val(off, v);
+            return valid && (direct || (l0 >= 0 && l0 < nx));
+        }, [&](int, const int *off, const T v[NC]) {
+            const int l0 = s.base[0] - lo[0] + off[0] - x0;
+            if (!(tail && (direct || (l0 >= 0 && l0 < nx)))) return;
+            if (direct) {
+                int node = 0;
 #pragma unroll
-            for (int c = 0; c < NC; ++c) v[c] = run_sum(mine ? v[c] : T(0), sg.dist);
-            if (mine && sg.tail) {
-                if (direct) {
-                    int node = 0;
+                for (int d = 0; d < D; ++d) node = node * n + s.base[d] + off[d];
 #pragma unroll
-                    for (int d = 0; d < D; ++d) node = node * n + s.base[d] + off[d];
+                for (int c = 0; c < NC; ++c)
+                    if (v[c] != T(0)) gadd(gk + (size_t)node * 4 + c, v[c]);
+            } else {
+                int li = l0;
 #pragma unroll
-                    for (int c = 0; c < NC; ++c)
-                        if (v[c] != T(0)) gadd(gk + (size_t)node * 4 + c, v[c]);
-                } else {
-                    int li = l0;
+                for (int d = 1; d < D; ++d) li = li * ext[d] + s.base[d] - lo[d] + off[d];
 #pragma unroll
-                    for (int d = 1; d < D; ++d) li = li * ext[d] + s.base[d] - lo[d] + off[d];
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) atomicAdd(tile + li * 4 + c, v[c]);
-                }
+                for (int c = 0; c < NC; ++c) atomicAdd(tile + li * 4 + c, v[c]);
             }
-        }
+        });
         if (direct) continue;
         __syncthreads();
-        if (tb.store && nx == ext[0] && store_tile<D, T>(tile, nodes, lo, ext, tb, nb, gk, n))
+        if (tb.store && nx == ext[0] && store_tile<D, T>(tile, snodes, lo, ext, tb, nb, gk, n))
             break;
-        for (int i = threadIdx.x; i < nodes; i += blockDim.x) {
+        for (int i = threadIdx.x; i < snodes; i += blockDim.x) {
             int rem = i, c[D];
 #pragma unroll
             for (int d = D - 1; d >= 1; --d) {
@@ -468,15 +663,28 @@ __device__ __forceinline__ void sense_block(bool valid, int g, const T nx[3], co
 }
 
 // ---------------------------------------------------------------- P2G
+// Particle range of this workgroup: window blockIdx.x of the window table,
+// or the blockIdx.x-th chunk of PB particles without one.
+struct Window {
+    int q0, q1;
+};
+__device__ __forceinline__ Window window_of(const int32_t *win, int P) {
+    if (!win) return {(int)blockIdx.x * PB, min((int)blockIdx.x * PB + PB, P)};
+    return {win[2 * blockIdx.x], min(win[2 * blockIdx.x + 1], P)};
+}
+
 template <int D, typename T, bool STORE_F, int U>
 __global__ __launch_bounds__(PB) void k_p2g_b(MP<T> p, PView<D, T> pv, const T *act,
                                               const int32_t *act_id, int n_act, T *F_out,
-                                              T *grid, Blk bk, int32_t *err, TileBuf<T> tb) {
+                                              T *grid, Blk bk, int32_t *err, TileBuf<T> tb,
+                                              const int32_t *win) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *tile = reinterpret_cast<T *>(smem_raw);     // [tb.tn][4]
     __shared__ int red[2 * D * NWAVE];
-    const int k = blockIdx.y, q = blockIdx.x * PB + threadIdx.x, P = pv.P;
-    bool valid = q < P;
+    const Window wn = window_of(win, pv.P);
+    if (threadIdx.x == 0 && wn.q1 - wn.q0 > PB) atomicOr(err, 2);
+    const int k = blockIdx.y, q = wn.q0 + (int)threadIdx.x, P = pv.P;
+    bool valid = q >= 0 && q < wn.q1;
     T mv[D];
     Mat<D, T> aff;
     Stencil<D, T> s;
@@ -502,25 +710,23 @@ __global__ __launch_bounds__(PB) void k_p2g_b(MP<T> p, PView<D, T> pv, const T *
     int lo[D], ext[D];
     if (!block_bbox<D>(valid, s.base, lo, ext, red)) return;
     T *gk = grid + (size_t)k * gsize<D>(p.n) * 4;
-    tile_scatter<D, T, 4, U>(valid, s, lo, ext, tile, gk, p.n, tb.robot(k, bk.nblk), bk.nb,
-                          [&](const int off[D], T v[4]) {
-        T W = T(1), dpos[D];
+    // momentum of node base + off: W (mv + A (off - fx) dx), mass W m_p
+    LinVal<D, T, 4> lv;
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-            W *= wsel(off[d], s.w[0][d], s.w[1][d], s.w[2][d]);
-            dpos[d] = (T(off[d]) - s.fx[d]) * p.dx;
+    for (int i = 0; i < D; ++i) {
+        T b = mv[i];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            lv.L[i][j] = aff.a[i][j] * p.dx;
+            b -= lv.L[i][j] * s.fx[j];
         }
+        lv.b[i] = b;
+    }
 #pragma unroll
-        for (int i = 0; i < D; ++i) {
-            T m = mv[i];
-#pragma unroll
-            for (int j = 0; j < D; ++j) m += aff.a[i][j] * dpos[j];
-            v[i] = W * m;
-        }
-#pragma unroll
-        for (int i = D; i < 3; ++i) v[i] = T(0);
-        v[3] = W * p.p_mass;
-    });
+    for (int i = D; i < 3; ++i) lv.b[i] = T(0);
+    lv.b[3] = p.p_mass;
+    tile_scatter<D, T, 4, D, U>(valid, s, lo, ext, tile, gk, p.n, tb.robot(k, bk.nblk), bk.nb,
+                                lv);
     mark_box<D>(bk, k, lo, ext);
 }
 
@@ -771,11 +977,12 @@ __global__ __launch_bounds__(64) void k_ctrl(Ctrl<T> c, int t, const T *sens, T 
 }
 
 // Adjoint of k_ctrl (with mlp) at step t: consumes (and zeroes) act_b,
-// writes in_b row t, accumulates the robot's weight-gradient slice
-// [W1 (H*I) | b1 (H) | W2 (O*H) | b2 (O)].
+// writes in_b row t and the pre-activation adjoints of the step (zh: hidden
+// [B][H], zo: output [B][O]); k_wgrad_time turns those rows into the weight
+// gradient.
 template <typename T>
 __global__ __launch_bounds__(64) void k_ctrl_adj(Ctrl<T> c, const T *inp, T *act_b, T *in_b,
-                                                 T *wg) {
+                                                 T *zh, T *zo) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *si = reinterpret_cast<T *>(smem_raw);
     T *shd = si + c.I;
@@ -817,20 +1024,40 @@ __global__ __launch_bounds__(64) void k_ctrl_adj(Ctrl<T> c, const T *inp, T *act
         for (int h = 0; h < c.H; ++h) ib += c.W1[(size_t)h * c.I + i] * sz1b[h];
         in_b[(size_t)k * c.I + i] = ib;
     }
-    T *g = wg + (size_t)k * ((size_t)c.H * c.I + c.H + (size_t)c.O * c.H + c.O);
-    for (int idx = tid; idx < c.H * c.I; idx += 64) {
-        const int h = idx / c.I, i = idx - h * c.I;
-        g[idx] += sz1b[h] * si[i];
+    for (int h = tid; h < c.H; h += 64) zh[(size_t)k * c.H + h] = sz1b[h];
+    for (int o = tid; o < c.O; o += 64) zo[(size_t)k * c.O + o] = sab[o];
+}
+
+// Weight gradient of n controller steps, added once to the robot's slice
+// [W1 (H*I) | b1 (H) | W2 (O*H) | b2 (O)] of wg: dW1 = sum_t zh_t inp_t^T,
+// db1 = sum_t zh_t, dW2 = sum_t zo_t hid_t^T, db2 = sum_t zo_t (nn.py:130-150
+// chain rule).  Row pointers start at the first step; rows are [B][.] per
+// step.  One thread per weight entry, grid (entries / 256, B); steps are
+// summed from the last to the first, the order the per-step adjoints run in.
+template <typename T>
+__global__ __launch_bounds__(256) void k_wgrad_time(int B, int I, int H, int O, int n,
+                                                    const T *inp, const T *hid, const T *zh,
+                                                    const T *zo, T *wg) {
+    const int k = blockIdx.y, e = blockIdx.x * 256 + threadIdx.x;
+    const int nW1 = H * I, nb1 = nW1 + H, nW2 = nb1 + O * H, nw = nW2 + O;
+    if (e >= nw) return;
+    T acc = T(0);
+    if (e < nW1) {
+        const int h = e / I, i = e - h * I;
+        for (int j = n - 1; j >= 0; --j)
+            acc += zh[((size_t)j * B + k) * H + h] * inp[((size_t)j * B + k) * I + i];
+    } else if (e < nb1) {
+        const int h = e - nW1;
+        for (int j = n - 1; j >= 0; --j) acc += zh[((size_t)j * B + k) * H + h];
+    } else if (e < nW2) {
+        const int r = e - nb1, o = r / H, h = r - o * H;
+        for (int j = n - 1; j >= 0; --j)
+            acc += zo[((size_t)j * B + k) * O + o] * hid[((size_t)j * B + k) * H + h];
+    } else {
+        const int o = e - nW2;
+        for (int j = n - 1; j >= 0; --j) acc += zo[((size_t)j * B + k) * O + o];
     }
-    g += (size_t)c.H * c.I;
-    for (int h = tid; h < c.H; h += 64) g[h] += sz1b[h];
-    g += c.H;
-    for (int idx = tid; idx < c.O * c.H; idx += 64) {
-        const int o = idx / c.H, h = idx - o * c.H;
-        g[idx] += sab[o] * shd[h];
-    }
-    g += (size_t)c.O * c.H;
-    for (int o = tid; o < c.O; o += 64) g[o] += sab[o];
+    wg[(size_t)k * nw + e] += acc;
 }
 
 // ---------------------------------------------------------------- adjoints
@@ -848,12 +1075,14 @@ template <typename T> struct SenseAdj {
 template <int D, typename T, int U>
 __global__ __launch_bounds__(PB) void k_g2p_adj_b(MP<T> p, int P, const T *x, const T *gvo,
                                                   const T *xb1, const T *vb1, const T *Cb1, T *xb,
-                                                  T *gvo_b, SenseAdj<T> sa, Blk bk, TileBuf<T> tb) {
+                                                  T *gvo_b, SenseAdj<T> sa, Blk bk, TileBuf<T> tb,
+                                                  const int32_t *win) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *tile = reinterpret_cast<T *>(smem_raw);     // [tb.tn][4]
     __shared__ int red[2 * D * NWAVE];
-    const int k = blockIdx.y, q = blockIdx.x * PB + threadIdx.x;
-    const bool valid = q < P;
+    const Window wn = window_of(win, P);
+    const int k = blockIdx.y, q = wn.q0 + (int)threadIdx.x;
+    const bool valid = q >= 0 && q < wn.q1;
     const size_t bv = (size_t)k * D * P, bm = (size_t)k * D * D * P;
     T xs[D], xe[D], vnb[D], Cb[D][D];
     Stencil<D, T> s;
@@ -944,22 +1173,20 @@ __global__ __launch_bounds__(PB) void k_g2p_adj_b(MP<T> p, int P, const T *x, co
     // scatter: dL/d grid_vo += W (vnb + 4/dx Cb dpos)
     int lo[D], ext[D];
     if (block_bbox<D>(inside, s.base, lo, ext, red)) {
-        tile_scatter<D, T, D, U>(inside, s, lo, ext, tile, gvo_b + (size_t)k * G * 4, p.n,
-                              tb.robot(k, bk.nblk), bk.nb, [&](const int off[D], T v[D]) {
-            T W = T(1), dp[D];
+        // W (vnb + 4/dx Cb (off - fx))
+        LinVal<D, T, D> lv;
 #pragma unroll
-            for (int d = 0; d < D; ++d) {
-                W *= wsel(off[d], s.w[0][d], s.w[1][d], s.w[2][d]);
-                dp[d] = T(off[d]) - s.fx[d];
+        for (int i = 0; i < D; ++i) {
+            T b = vnb[i];
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                lv.L[i][j] = c4 * Cb[i][j];
+                b -= lv.L[i][j] * s.fx[j];
             }
-#pragma unroll
-            for (int i = 0; i < D; ++i) {
-                T acc = T(0);
-#pragma unroll
-                for (int j = 0; j < D; ++j) acc += Cb[i][j] * dp[j];
-                v[i] = W * (vnb[i] + c4 * acc);
-            }
-        });
+            lv.b[i] = b;
+        }
+        tile_scatter<D, T, D, D, U>(inside, s, lo, ext, tile, gvo_b + (size_t)k * G * 4, p.n,
+                                    tb.robot(k, bk.nblk), bk.nb, lv);
     }
     if (valid) {
 #pragma unroll
@@ -1146,6 +1373,9 @@ int check_batch(const dl_mpm_batch *b) {
     DL_CHECK_ARG(b->scatter == DL_MPM_SCATTER_ATOMIC || b->scatter == DL_MPM_SCATTER_STORE,
                  "MPM batch: bad scatter mode %d", b->scatter);
     DL_CHECK_ARG(b->B > 0 && b->P > 0 && b->n_act > 0 && b->horizon > 0, "MPM batch: bad sizes");
+    DL_CHECK_ARG(!b->windows || (b->n_windows >= (b->P + PB - 1) / PB && b->n_windows <= b->P),
+                 "MPM batch: %d windows cannot partition %d particles in windows of at most %d",
+                 b->n_windows, b->P, PB);
     DL_CHECK_ARG(b->act_id && b->grid && b->grid_vo && b->blk_flag && b->blk_list &&
                      b->blk_count && b->err && b->act_out && b->tiles && b->tile_hdr &&
                      b->blk_tile_count && b->blk_tiles,
@@ -1202,8 +1432,12 @@ template <int D, typename T> GridSave<T> save_at(const dl_mpm_batch *b, int j) {
             b->list_save + j * rec, b->count_save + (size_t)j * b->B};
 }
 
+int n_windows(const dl_mpm_batch *b) {
+    return b->windows ? b->n_windows : (b->P + PB - 1) / PB;
+}
+
 template <typename T> TileBuf<T> make_tiles(const dl_mpm_batch *b) {
-    return {(T *)b->tiles, b->tile_hdr, b->blk_tile_count, b->blk_tiles, (b->P + PB - 1) / PB,
+    return {(T *)b->tiles, b->tile_hdr, b->blk_tile_count, b->blk_tiles, n_windows(b),
             b->tile_nodes, b->scatter == DL_MPM_SCATTER_STORE};
 }
 
@@ -1216,7 +1450,7 @@ int forward_impl(const dl_mpm_batch *b, const dl_mpm_slots *st, int t0, int n, i
     const bool control = flags & DL_MPM_CONTROL;
     const MP<T> mp = make_mp<T>(b->params);
     const Blk bk = make_blk<D>(b);
-    const dim3 pg((P + PB - 1) / PB, B), gg(grid_wgs(bk), B);
+    const dim3 pg((P + PB - 1) / PB, B), gg(grid_wgs(bk), B), pw(n_windows(b), B);
     const int I = n_in(b), H = b->n_hidden;
     const T *act = (const T *)b->act_out;
     const TileBuf<T> tb = make_tiles<T>(b);
@@ -1242,10 +1476,10 @@ int forward_impl(const dl_mpm_batch *b, const dl_mpm_slots *st, int t0, int n, i
                                (T *)b->inp + (size_t)t * B * I, (T *)b->hid + (size_t)t * B * H,
                                (T *)b->act_out + (size_t)t * B * O, 1);
         PView<D, T> pv{a.x, a.v, a.C, a.F, P};
-        DL_UNROLL_DISPATCH(U_, hipLaunchKernelGGL((k_p2g_b<D, T, true, U_>), pg, dim3(PB),
+        DL_UNROLL_DISPATCH(U_, hipLaunchKernelGGL((k_p2g_b<D, T, true, U_>), pw, dim3(PB),
                                                   tile_bytes, s, mp, pv, act + (size_t)t * B * O,
                                                   b->act_id, O, c.F, (T *)b->grid, bk, b->err,
-                                                  tb));
+                                                  tb, b->windows));
         const GridSave<T> sv = (flags & DL_MPM_SAVE_GRID) ? save_at<D, T>(b, j)
                                                            : GridSave<T>{nullptr, nullptr, nullptr, nullptr};
         hipLaunchKernelGGL((k_grid_b<D, T>), gg, dim3(PB), 0, s, mp, (T *)b->grid, (T *)b->grid_vo,
@@ -1276,7 +1510,7 @@ int backward_impl(const dl_mpm_batch *b, const dl_mpm_slots *seg, const dl_mpm_s
     const int B = b->B, P = b->P, O = b->n_act;
     const MP<T> mp = make_mp<T>(b->params);
     const Blk bk = make_blk<D>(b);
-    const dim3 pg((P + PB - 1) / PB, B), gg(grid_wgs(bk), B);
+    const dim3 pg((P + PB - 1) / PB, B), gg(grid_wgs(bk), B), pw(n_windows(b), B);
     const int I = n_in(b), H = b->n_hidden;
     const Ctrl<T> ctrl = make_ctrl<T>(b);
     const T *act = (const T *)b->act_out;
@@ -1297,10 +1531,11 @@ int backward_impl(const dl_mpm_batch *b, const dl_mpm_slots *seg, const dl_mpm_s
             hipLaunchKernelGGL((k_grid_restore<D, T>), gg, dim3(PB), 0, s, mp, (T *)b->grid,
                                (T *)b->grid_vo, bk, save_at<D, T>(b, j));
         } else {
-            DL_UNROLL_DISPATCH(U_, hipLaunchKernelGGL((k_p2g_b<D, T, false, U_>), pg, dim3(PB),
+            DL_UNROLL_DISPATCH(U_, hipLaunchKernelGGL((k_p2g_b<D, T, false, U_>), pw, dim3(PB),
                                                       tile_bytes, s, mp, pv,
                                                       act + (size_t)t * B * O, b->act_id, O,
-                                                      (T *)nullptr, (T *)b->grid, bk, b->err, tb));
+                                                      (T *)nullptr, (T *)b->grid, bk, b->err, tb,
+                                                      b->windows));
             hipLaunchKernelGGL((k_grid_b<D, T>), gg, dim3(PB), 0, s, mp, (T *)b->grid,
                                (T *)b->grid_vo, bk, 0, tb,
                                GridSave<T>{nullptr, nullptr, nullptr, nullptr});
@@ -1317,10 +1552,10 @@ int backward_impl(const dl_mpm_batch *b, const dl_mpm_slots *seg, const dl_mpm_s
             sa.I = I;
             sa.scale = T(b->offset_scale);
         }
-        DL_UNROLL_DISPATCH(U_, hipLaunchKernelGGL((k_g2p_adj_b<D, T, U_>), pg, dim3(PB),
+        DL_UNROLL_DISPATCH(U_, hipLaunchKernelGGL((k_g2p_adj_b<D, T, U_>), pw, dim3(PB),
                                                   tile_bytes, s, mp, P, a.x,
                                                   (const T *)b->grid_vo, A1.x, A1.v, A1.C, A0.x,
-                                                  (T *)b->grid_vo_grad, sa, bk, tb));
+                                                  (T *)b->grid_vo_grad, sa, bk, tb, b->windows));
         hipLaunchKernelGGL((k_grid_adj_b<D, T>), gg, dim3(PB), 0, s, mp, (T *)b->grid,
                            (T *)b->grid_vo_grad, (T *)b->grid_grad, bk, tb);
         hipLaunchKernelGGL((k_p2g_adj_b<D, T>), pg, dim3(PB), O * sizeof(T), s, mp, pv,
@@ -1329,8 +1564,18 @@ int backward_impl(const dl_mpm_batch *b, const dl_mpm_slots *seg, const dl_mpm_s
         if (b->in_grad)
             hipLaunchKernelGGL(k_ctrl_adj<T>, dim3(B), dim3(64), (I + 3 * H + O) * sizeof(T), s,
                                ctrl, (const T *)b->inp + (size_t)t * B * I, (T *)b->act_grad,
-                               (T *)b->in_grad + (size_t)t * B * I, (T *)b->wgrad);
+                               (T *)b->in_grad + (size_t)t * B * I,
+                               (T *)b->hid_grad + (size_t)t * B * H,
+                               (T *)b->out_grad + (size_t)t * B * O);
         cur = nxt;
+    }
+    if (b->in_grad) {
+        const int nw = H * I + H + O * H + O;
+        hipLaunchKernelGGL(k_wgrad_time<T>, dim3((nw + 255) / 256, B), dim3(256), 0, s, B, I, H,
+                           O, n, (const T *)b->inp + (size_t)t0 * B * I,
+                           (const T *)b->hid + (size_t)t0 * B * H,
+                           (const T *)b->hid_grad + (size_t)t0 * B * H,
+                           (const T *)b->out_grad + (size_t)t0 * B * O, (T *)b->wgrad);
     }
     if (cur != 0) {
         const Slot<D, T> from = slot<D, T>(adj, 1, B, P), to = slot<D, T>(adj, 0, B, P);
@@ -1386,7 +1631,8 @@ int32_t dl_mpm_backward(const dl_mpm_batch *b, const dl_mpm_slots *seg, const dl
                  "dl_mpm_backward: saved grids cover %d steps, segment has %d", b->save_steps, n);
     if (b->in_grad) {
         if (int e = check_control(b)) return e;
-        DL_CHECK_ARG(b->center_grad && b->vcom_grad && b->wgrad, "dl_mpm_backward: NULL buffer");
+        DL_CHECK_ARG(b->center_grad && b->vcom_grad && b->wgrad && b->hid_grad && b->out_grad,
+                     "dl_mpm_backward: NULL buffer");
     }
     MPMB_DISPATCH(b, backward_impl, b, seg, adj, t0, n, (hipStream_t)stream);
 }

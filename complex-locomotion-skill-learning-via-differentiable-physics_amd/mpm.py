@@ -234,7 +234,9 @@ class MPMBatchC(Structure):
                     "blk_count", "err", "sens", "center", "vcom", "inp", "hid", "act_out",
                     "act_grad", "in_grad", "center_grad", "vcom_grad", "wgrad", "tiles",
                     "tile_hdr", "blk_tile_count", "blk_tiles", "grid_save", "vo_save",
-                    "list_save", "count_save")] + [("save_steps", c_int32), ("scatter_unroll", c_int32)])
+                    "list_save", "count_save")] + [("save_steps", c_int32), ("scatter_unroll", c_int32)] +
+                [(n, ctypes.c_void_p) for n in ("hid_grad", "out_grad", "windows")] +
+                [("n_windows", c_int32), ("pad_", c_int32)])
 
 
 class MPMSlotsC(Structure):
@@ -262,6 +264,24 @@ def block_order(positions, n_grid, dim=3):
         blk = blk * nb + base[:, d] // E
         cell = cell * E + base[:, d] % E
     return np.argsort(blk * E ** dim + cell, kind="stable")
+
+
+def scatter_windows(positions, n_grid, box_limit, dim=3, pb=MPM_PB):
+    """Cut the particle order into windows (first, end) of at most pb
+    particles whose stencil box (base cells' bounding box plus the 2-node
+    stencil reach) holds at most box_limit nodes: each window grows until
+    the next particle would exceed either bound (a single particle always
+    fits).  Returns an int32 array [n_windows, 2] partitioning [0, P)."""
+    base = np.floor(np.asarray(positions, np.float64) * n_grid - 0.5).astype(np.int64)[:, :dim]
+    P, out, i = len(base), [], 0
+    while i < P:
+        seg = base[i:i + pb]
+        lo, hi = np.minimum.accumulate(seg, 0), np.maximum.accumulate(seg, 0)
+        over = np.nonzero(np.prod(hi - lo + 3, 1) > box_limit)[0]
+        n = len(seg) if len(over) == 0 else max(int(over[0]), 1)
+        out.append((i, i + n))
+        i += n
+    return np.asarray(out, np.int32).reshape(-1, 2)
 
 
 class _Slots:
@@ -292,13 +312,17 @@ class MPMBatch:
     the hand-written adjoints in reverse.  Weight gradients land in ``grads``
     (W1, b1, W2, b2).  Particles are reordered by grid block (see
     ``block_order``; ``order`` maps internal to given indices) -- the sensors,
-    COM and loss do not depend on particle order."""
+    COM and loss do not depend on particle order.  The scatter kernels run one
+    workgroup per window of at most 256 consecutive particles whose stencil
+    box holds at most ``box_limit`` nodes (``scatter_windows``; None: plain
+    256-particle chunks); ``tile_nodes`` sizes their LDS buffer (default
+    2048 nodes)."""
 
     def __init__(self, params, positions, act_id, n_act, B, T, K=100, window=None,
                  dtype=torch.float32, n_sin=10, run_period=100, offset_scale=0.2,
                  max_speed=0.08, activation="tanh", n_hidden=64, device="cuda",
                  sort_particles=True, tile_nodes=None, scatter=SCATTER_ATOMIC, save_grid=True,
-                 unroll=3):
+                 unroll=3, box_limit=512):
         if params.dim != 3:
             raise ValueError("MPMBatch drives 3D robots (the controller's sensors are 3D)")
         if params.n_grid % 4:
@@ -335,9 +359,13 @@ class MPMBatch:
         self.grid, self.grid_vo, self.grid_vo_grad, self.grid_grad = (z(B, ng, 4) for _ in range(4))
         self.blk_flag, self.blk_list, self.blk_count = zi(B, nblk), zi(B, nblk), zi(B)
         self.err = zi(1)
-        wpr = (P + MPM_PB - 1) // MPM_PB
-        if tile_nodes is None:      # 64 KiB of LDS in FP32, 128 KiB in FP64
-            tile_nodes = 4096
+        # scatter windows: box-limited cuts of the (sorted) particle order
+        win = (scatter_windows(pos, params.n_grid, box_limit) if box_limit is not None
+               else None)
+        self.windows = torch.as_tensor(win).to(dev) if win is not None else None
+        wpr = len(win) if win is not None else (P + MPM_PB - 1) // MPM_PB
+        if tile_nodes is None:      # 32 KiB of LDS in FP32, 64 KiB in FP64
+            tile_nodes = 2048
         self.tile_nodes, self.scatter = tile_nodes, scatter
         self.tiles = z(B, wpr, tile_nodes if scatter == SCATTER_STORE else 1, 4)
         self.tile_hdr = zi(B, wpr, 8)
@@ -360,6 +388,7 @@ class MPMBatch:
         H, I, O = self.H, self.I, self.O
         self.nw = H * I + H + O * H + O
         self.wgrad = z(B, self.nw)
+        self.hid_grad, self.out_grad = emp(T, B * H), emp(T, B * O)
         self.W = [z(H, I), z(H), z(O, H), z(O)]
         self.targets, self.tv, self.th = z(B, 5), z(B, 3), z(B)
         self.loss, self.loss_b = z(1), z(B)
@@ -390,7 +419,9 @@ class MPMBatch:
             vo_save=P_(self.vo_save) if save_grid else None,
             list_save=P_(self.list_save) if save_grid else None,
             count_save=P_(self.count_save) if save_grid else None,
-            save_steps=K if save_grid else 0, scatter_unroll=unroll)
+            save_steps=K if save_grid else 0, scatter_unroll=unroll,
+            hid_grad=P_(self.hid_grad), out_grad=P_(self.out_grad),
+            windows=P_(self.windows) if win is not None else None, n_windows=wpr)
         self.grads = None
 
     def set_targets(self, tv, th):
@@ -488,12 +519,13 @@ class MPMBatch:
         own = sum(t.numel() * t.element_size() for t in vars(self).values()
                   if isinstance(t, torch.Tensor))
         own += sum(t.numel() * t.element_size() for t in self.W)
-        own += sum(t.numel() * t.element_size() for t in (self.grid_save, self.vo_save,
-                                                          self.list_save, self.count_save)
-                   if t is not None)
         return own + sum(f.numel() * f.element_size()
                          for sl in (self.ring, self.ck, self.seg, self.adj) for f in sl.fields())
 
     def check(self):
-        if int(self.err.item()) != 0:
+        e = int(self.err.item())
+        if e & 1:
             raise nat.NativeError("MPM: a particle left the grid (stencil out of bounds)")
+        if e:
+            raise nat.NativeError(f"MPM: error flags {e:#x} (2: a scatter window wider than "
+                                  f"{MPM_PB} particles)")

@@ -389,6 +389,21 @@ typedef struct dl_mpm_batch {
     int32_t *list_save, *count_save;
     int32_t save_steps;      /* K: steps the save buffers hold */
     int32_t scatter_unroll;  /* stencil-offset loop unroll of the scatter kernels: 1, 3, 9, 27 */
+    /* Controller adjoint rows (required with in_grad): hid_grad [T][B][H] and
+     * out_grad [T][B][O], the adjoints of each step's hidden and output
+     * pre-activations.  dl_mpm_backward writes rows t0..t0+n-1 and adds their
+     * sums over those steps (outer products with the inp / hid rows) to
+     * wgrad once per call. */
+    void *hid_grad, *out_grad;
+    /* Scatter windows (optional): windows [n_windows][2] = (first, end)
+     * particle index ranges that partition [0, P) in order, each of at most
+     * DL_MPM_PB particles.  P2G and the G2P adjoint run one workgroup per
+     * window (NULL: ceil(P / DL_MPM_PB) windows of DL_MPM_PB particles), so
+     * cutting the particle order where a window's stencil box would grow
+     * large keeps every workgroup's box small.  Store-mode tile buffers hold
+     * one tile per window.  A window wider than DL_MPM_PB sets err bit 2. */
+    const int32_t *windows;
+    int32_t n_windows, pad_;
 } dl_mpm_batch;
 
 /* Ring of particle-state slots: slot s of x/v at x + s*B*D*P reals, of C/F

@@ -49,6 +49,8 @@ int main(void) {
   printf("%zu %zu %zu %zu %zu %zu\n", sizeof(dl_mpm_params), sizeof(dl_mpm_batch),
          sizeof(dl_mpm_slots), offsetof(dl_mpm_batch, params), offsetof(dl_mpm_batch, act_id),
          offsetof(dl_mpm_batch, wgrad));
+  printf("%zu %zu %zu\n", offsetof(dl_mpm_batch, hid_grad), offsetof(dl_mpm_batch, windows),
+         offsetof(dl_mpm_batch, n_windows));
   return 0;
 }
 ''')
@@ -62,7 +64,9 @@ int main(void) {
            nat.Rollout.traj_grad.offset, nat.RobotC.weights.offset,
            nat.RobotC.target_stride.offset,
            ctypes.sizeof(dm.MPMParamsC), ctypes.sizeof(dm.MPMBatchC), ctypes.sizeof(dm.MPMSlotsC),
-           dm.MPMBatchC.params.offset, dm.MPMBatchC.act_id.offset, dm.MPMBatchC.wgrad.offset]
+           dm.MPMBatchC.params.offset, dm.MPMBatchC.act_id.offset, dm.MPMBatchC.wgrad.offset,
+           dm.MPMBatchC.hid_grad.offset, dm.MPMBatchC.windows.offset,
+           dm.MPMBatchC.n_windows.offset]
     assert got == exp
     assert nat.ROBOT_DTYPE.itemsize == got[3]
 

@@ -195,28 +195,62 @@ def test_mpm_batched_rollout_matches_oracle(prec, K, scatter, tile, save, unroll
         assert rel_err(a.double().cpu(), b) < tg
 
 
+def test_mpm_batched_plain_chunks_match_windows():
+    """Several workgroups per robot (3 blobs, 375 particles): without a
+    window table (plain 256-particle chunks) and with box-limited windows
+    the rollout gives the same loss and gradients, and no error flag."""
+    _say("mpm batched plain chunks vs windows")
+    params, op, pos, ids = _batched_case(offsets=((5.0, 3.2, 5.0), (8.0, 3.2, 5.0), (5.0, 3.2, 8.0)))
+    out = []
+    for limit in (None, 200):
+        batch, loss, grads, lo, gref = _run_batched(params, op, pos, ids, 2, 10, 5, 5,
+                                                    torch.float64, box_limit=limit)
+        assert (batch.windows is None) == (limit is None)
+        assert abs(loss - lo) / abs(lo) < 1e-10
+        out.append((loss, grads))
+    assert abs(out[0][0] - out[1][0]) / abs(out[0][0]) < 1e-12
+    for a, b in zip(out[0][1], out[1][1]):
+        assert rel_err(a.double().cpu(), b.double().cpu()) < 1e-10
+
+
 @pytest.mark.parametrize("case,scatter", [("slabs", dm.SCATTER_ATOMIC), ("direct", dm.SCATTER_ATOMIC),
-                                          ("slabs", dm.SCATTER_STORE)])
+                                          ("slabs", dm.SCATTER_STORE), ("staged", dm.SCATTER_ATOMIC),
+                                          ("windows", dm.SCATTER_ATOMIC),
+                                          ("windows", dm.SCATTER_STORE)])
 def test_mpm_batched_scatter_paths_and_invariants(case, scatter):
     """Two blobs far apart in one workgroup (shuffled, unsorted), so the
     workgroup's node box exceeds the LDS tile: "slabs" (box 21 x 5 x 21)
     scatters in several x-slab passes through the tile, "direct" (a y-z plane
-    of 45 x 45 nodes) with direct global atomics.  Results match the oracle,
-    and the grid, active-block and actuation-adjoint buffers are zero again
-    afterwards."""
+    of 45 x 45 nodes) with direct global atomics.  "staged": the slabs case in
+    FP32 with the largest tile, where the per-run stage fits: nearly every
+    lane is its own run and each cell's runs (about 8) are chained in the
+    per-cell lists.  "windows": the slabs layout cut into box-limited scatter
+    windows (most of them a few particles wide, the shuffle alternating
+    between the blobs).  Results match the oracle, and the grid,
+    active-block and actuation-adjoint buffers are zero again afterwards."""
     _say(f"mpm batched scatter path {case} scatter={scatter}")
-    if case == "slabs":
+    dtype, tile, (tl, tg), limit = torch.float64, 1024, (1e-10, 1e-7), None
+    if case == "staged":
+        dtype, tile, (tl, tg) = torch.float32, dm.MPM_TILE_MAX, (1e-4, 1e-3)
+    if case == "windows":
+        limit = 512
+    if case in ("slabs", "staged", "windows"):
         params, op, pos, ids = _batched_case(offsets=((5.0, 3.2, 5.0), (20.0, 3.2, 20.0)), n=32,
                                              perm_seed=3)
     else:
         params, op, pos, ids = _batched_case(offsets=((5.0, 4.0, 5.0), (5.0, 44.0, 44.0)), n=64,
                                              perm_seed=3)
     B, T, K, W = 2, 12, 5, 6
-    batch, loss, grads, lo, gref = _run_batched(params, op, pos, ids, B, T, K, W, torch.float64,
-                                                sort=False, scatter=scatter, tile_nodes=1024)
-    assert abs(loss - lo) / abs(lo) < 1e-10
+    batch, loss, grads, lo, gref = _run_batched(params, op, pos, ids, B, T, K, W, dtype,
+                                                sort=False, scatter=scatter, tile_nodes=tile,
+                                                box_limit=limit)
+    if case == "windows":
+        assert len(batch.windows) > 20
+    else:
+        assert batch.windows is None
+    assert abs(loss - lo) / abs(lo) < tl
     for a, b in zip(grads, gref):
-        assert rel_err(a.double().cpu(), b) < 1e-7
+        assert rel_err(a.double().cpu(), b) < tg
     for t in (batch.grid, batch.grid_vo_grad, batch.act_grad):
         assert t.abs().max().item() == 0.0
     assert batch.blk_flag.abs().max().item() == 0 and batch.blk_count.abs().max().item() == 0
