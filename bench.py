@@ -64,7 +64,9 @@ def main():
     ap.add_argument("--mpm-horizon", type=int, default=2000)
     ap.add_argument("--mpm-ckpt", type=int, default=100)
     ap.add_argument("--mpm-steps", type=int, default=1)
-    ap.add_argument("--mpm-scatter", type=int, default=0, help="0: tile atomics, 1: store + gather")
+    ap.add_argument("--mpm-scatter", type=int, default=0,
+                    help="0: global-atomic commits (staged run sums, LDS tile fallback), "
+                         "1: tiles stored and gathered by the grid pass")
     ap.add_argument("--mpm-tile", type=int, default=2048, help="LDS scatter tile, nodes")
     ap.add_argument("--skip-uneven", action="store_true")
     ap.add_argument("--uneven-robots", type=int, default=512)
@@ -212,8 +214,10 @@ def main():
         bpp = 684 + 224 * gp + 96 / Km          # algorithmic bytes per particle-step, fwd+bwd
         mpm = {"value": pps, "unit": "particle-steps/s", "robots": Bm, "particles": P,
                "grid": 64, "horizon": Tm, "ckpt_every": Km, "dtype": "f32",
-               "scatter": ["tile atomics", "tile store + gather"][args.mpm_scatter],
-               "tile_nodes": args.mpm_tile,
+               "scatter": ["staged run sums + per-node gather, global atomics",
+                           "tile store + gather"][args.mpm_scatter],
+               "tile_nodes": args.mpm_tile, "scatter_windows": len(mb.windows),
+               "resort_every": Km,
                "ms_forward": float(np.mean(mf)), "ms_backward": float(np.mean(mbk)),
                "active_nodes_per_particle": gp,
                "roofline": {"bound": "hbm", "achieved": bpp * pps / 1e9, "peak": HBM_PEAK_GBS,

@@ -316,13 +316,16 @@ class MPMBatch:
     workgroup per window of at most 256 consecutive particles whose stencil
     box holds at most ``box_limit`` nodes (``scatter_windows``; None: plain
     256-particle chunks); ``tile_nodes`` sizes their LDS buffer (default
-    2048 nodes)."""
+    2048 nodes).  With ``resort`` each robot's particles are re-sorted by
+    block at every checkpoint (the state, actuator ids and sensor groups
+    permuted together), so the scatter keeps compact cells as the robots
+    move; the adjoint is permuted back at each checkpoint."""
 
     def __init__(self, params, positions, act_id, n_act, B, T, K=100, window=None,
                  dtype=torch.float32, n_sin=10, run_period=100, offset_scale=0.2,
                  max_speed=0.08, activation="tanh", n_hidden=64, device="cuda",
                  sort_particles=True, tile_nodes=None, scatter=SCATTER_ATOMIC, save_grid=True,
-                 unroll=3, box_limit=512):
+                 unroll=3, box_limit=512, resort=True):
         if params.dim != 3:
             raise ValueError("MPMBatch drives 3D robots (the controller's sensors are 3D)")
         if params.n_grid % 4:
@@ -353,6 +356,7 @@ class MPMBatch:
         bt = lambda a, dt_: torch.as_tensor(a, dtype=dt_)[None].expand(B, *np.shape(a)).contiguous().to(dev)
         self.act_id = bt(ids, torch.int32)
         self.group = bt(grp, torch.int32)
+        self.resort = resort
         self.inv_count = bt(np.where(counts > 0, 1.0 / np.maximum(counts, 1), 0.0), dtype)
         ng = params.n_grid ** 3
         nblk = (params.n_grid // 4) ** 3
@@ -394,6 +398,13 @@ class MPMBatch:
         self.loss, self.loss_b = z(1), z(B)
         self.x0 = torch.as_tensor(np.ascontiguousarray(pos.T), dtype=dtype).to(dev)   # [3,P]
         self.nck = (T + K - 1) // K
+        if resort:
+            # segment s runs in order orders[s] (indices into the initial
+            # order); perms[s]: state_s[i] = state_{s-1}[perms[s][i]]
+            self.act_id0, self.group0 = self.act_id.clone(), self.group.clone()
+            self.orders = torch.empty(self.nck, B, P, dtype=torch.int64, device=dev)
+            self.perms = torch.empty(self.nck, B, P, dtype=torch.int64, device=dev)
+            self.orders[0].copy_(torch.arange(P, device=dev)[None].expand(B, -1))
         self.ring = _Slots(2, B, D, P, dtype, dev)
         self.ck = _Slots(self.nck, B, D, P, dtype, dev)
         self.seg = _Slots(K + 1, B, D, P, dtype, dev)
@@ -449,8 +460,24 @@ class MPMBatch:
 
     # A rollout as resumable pieces (a serving loop advances one segment per
     # tick); forward() / backward() run them all.
+    def _block_keys(self, x):
+        """Sort keys (grid block, then cell; as ``block_order``) of [B, 3, P]
+        positions."""
+        n = self.p.n_grid
+        base = torch.floor(x * n - 0.5).to(torch.int64).clamp_(0, n - 1)
+        nb = n // 4
+        blk = ((base[:, 0] // 4) * nb + base[:, 1] // 4) * nb + base[:, 2] // 4
+        cell = ((base[:, 0] % 4) * 4 + base[:, 1] % 4) * 4 + base[:, 2] % 4
+        return blk * 64 + cell
+
+    def _set_order(self, s):
+        self.act_id.copy_(torch.gather(self.act_id0, 1, self.orders[s]))
+        self.group.copy_(torch.gather(self.group0, 1, self.orders[s]))
+
     def start_forward(self):
         r = self.ring
+        if self.resort:
+            self._set_order(0)
         r.x[0].copy_(self.x0[None].expand(self.B, -1, -1))
         r.v[0].zero_()
         r.C[0].zero_()
@@ -462,6 +489,13 @@ class MPMBatch:
     def forward_segment(self, s):
         r = self.ring
         t0, n = self._segments()[s]
+        if self.resort and s > 0:
+            pi = torch.argsort(self._block_keys(r.x[0]), dim=1)
+            for f in r.fields():
+                f[0].copy_(torch.gather(f[0], 2, pi[:, None, :].expand_as(f[0])))
+            self.perms[s].copy_(pi)
+            torch.gather(self.orders[s - 1], 1, pi, out=self.orders[s])
+            self._set_order(s)
         self.ck.copy_slot(s, 0, r)
         flags = MPM_CONTROL | (MPM_SENSE_FIRST if s == 0 else 0)
         nat.call("dl_mpm_forward", ctypes.byref(self.desc), ctypes.byref(r.c), t0, n, flags,
@@ -495,11 +529,18 @@ class MPMBatch:
     def backward_segment(self, s):
         """Adjoint of segment s (segments run last to first)."""
         t0, n = self._segments()[s]
+        if self.resort:
+            self._set_order(s)
         self.seg.copy_slot(0, s, self.ck)
         nat.call("dl_mpm_forward", ctypes.byref(self.desc), ctypes.byref(self.seg.c), t0, n,
                  MPM_SAVE_GRID if self.save_grid else 0, nat.stream())
         nat.call("dl_mpm_backward", ctypes.byref(self.desc), ctypes.byref(self.seg.c),
                  ctypes.byref(self.adj.c), t0, n, nat.stream())
+        if self.resort and s > 0:
+            # adjoint of state t0 back to the previous segment's order
+            pi = self.perms[s]
+            for f in self.adj.fields():
+                f[0].copy_(torch.empty_like(f[0]).scatter_(2, pi[:, None, :].expand_as(f[0]), f[0]))
 
     def finish_backward(self):
         g = self.wgrad.sum(0)

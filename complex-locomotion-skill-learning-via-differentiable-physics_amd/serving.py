@@ -317,14 +317,16 @@ class MPMRunner:
     def estimate_bytes(req, ckpt_every=25, es=4):
         """Device bytes of a one-robot MPMBatch: state slots (ring, checkpoints,
         segment, adjoint) of 24 reals per particle, four 16-byte grid records
-        per node, block tables."""
+        per node, block tables, per-checkpoint particle orders."""
         m = req.mpm
         P, n = len(m["positions"]), m["params"].n_grid
         nck = (req.horizon + ckpt_every - 1) // ckpt_every
         slots = 2 + nck + ckpt_every + 1 + 2
         nblk = (n // 4) ** 3
         saved = ckpt_every * nblk * 64 * 4 * 2           # per-step grids of a segment
-        return es * (slots * 24 * P + 4 * 4 * n ** 3 + saved) + 4 * (3 + ckpt_every) * nblk
+        reorder = 2 * 8 * nck * P                        # per-checkpoint particle orders
+        return (es * (slots * 24 * P + 4 * 4 * n ** 3 + saved) + 4 * (3 + ckpt_every) * nblk
+                + reorder)
 
     def bytes_for(self, req):
         return self.estimate_bytes(req, self.K, torch.finfo(self.dtype).bits // 8)

@@ -195,6 +195,30 @@ def test_mpm_batched_rollout_matches_oracle(prec, K, scatter, tile, save, unroll
         assert rel_err(a.double().cpu(), b) < tg
 
 
+def test_mpm_batched_resort_matches_fixed_order():
+    """Shuffled particles re-sorted by block at every checkpoint (state,
+    actuator ids and groups permuted; the adjoint permuted back) give the
+    loss and gradients of the fixed order and of the oracle."""
+    _say("mpm batched resort vs fixed order")
+    params, op, pos, ids = _batched_case(offsets=((5.0, 3.2, 5.0), (8.0, 3.2, 5.0)), perm_seed=5)
+    out = []
+    for resort in (False, True):
+        batch, loss, grads, lo, gref = _run_batched(params, op, pos, ids, 2, 13, 4, 6,
+                                                    torch.float64, sort=False, resort=resort)
+        assert abs(loss - lo) / abs(lo) < 1e-10
+        for a, b in zip(grads, gref):
+            assert rel_err(a.double().cpu(), b) < 1e-7
+        out.append((loss, grads))
+        if resort:
+            # the orders are permutations, and the re-sort really moved particles
+            o = batch.orders.cpu()
+            assert torch.equal(o.sort(-1).values, torch.arange(o.shape[-1]).expand_as(o))
+            assert not torch.equal(o[1], o[0])
+    assert abs(out[0][0] - out[1][0]) / abs(out[0][0]) < 1e-12
+    for a, b in zip(out[0][1], out[1][1]):
+        assert rel_err(a.double().cpu(), b.double().cpu()) < 1e-10
+
+
 def test_mpm_batched_plain_chunks_match_windows():
     """Several workgroups per robot (3 blobs, 375 particles): without a
     window table (plain 256-particle chunks) and with box-limited windows
@@ -229,7 +253,7 @@ def test_mpm_batched_scatter_paths_and_invariants(case, scatter):
     between the blobs).  Results match the oracle, and the grid,
     active-block and actuation-adjoint buffers are zero again afterwards."""
     _say(f"mpm batched scatter path {case} scatter={scatter}")
-    dtype, tile, (tl, tg), limit = torch.float64, 1024, (1e-10, 1e-7), None
+    dtype, tile, (tl, tg), limit, resort = torch.float64, 1024, (1e-10, 1e-7), None, False
     if case == "staged":
         dtype, tile, (tl, tg) = torch.float32, dm.MPM_TILE_MAX, (1e-4, 1e-3)
     if case == "windows":
@@ -243,7 +267,7 @@ def test_mpm_batched_scatter_paths_and_invariants(case, scatter):
     B, T, K, W = 2, 12, 5, 6
     batch, loss, grads, lo, gref = _run_batched(params, op, pos, ids, B, T, K, W, dtype,
                                                 sort=False, scatter=scatter, tile_nodes=tile,
-                                                box_limit=limit)
+                                                box_limit=limit, resort=resort)
     if case == "windows":
         assert len(batch.windows) > 20
     else:
