@@ -97,11 +97,46 @@ __device__ __forceinline__ void activate(int kind, T z, T &y, T &dy) {
 // -------------------------------------------------------------- reductions
 constexpr int WAVE = 64;
 
+// Lane-shifted copy of v (DPP, a VALU operand modifier: no LDS round trip).
+// Lanes whose source is outside the pattern, or whose row is not in ROWS,
+// read 0.
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, true));
+}
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ double dpp_f(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, ROWS, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float lane63(float v) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+__device__ __forceinline__ double lane63(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), 63);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the 64 lanes, returned to every lane.  Within each 16-lane row:
+// swap neighbours, swap pairs (quad permutes), then add the mirrored half
+// row and the mirrored row -- every lane then holds its row's sum; rows 1
+// and 3 add the last lane of rows 0 and 2 (row_bcast:15), rows 2 and 3 the
+// last lane of row 1 (row_bcast:31), leaving the total in lane 63, which is
+// read back as a wave-uniform value.  All DPP, no LDS traffic.  Every lane
+// of the wave must be active (a disabled lane would read as 0).
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
-    return v;
+    v += dpp_f<0xB1>(v);          // quad_perm [1,0,3,2]
+    v += dpp_f<0x4E>(v);          // quad_perm [2,3,0,1]
+    v += dpp_f<0x141>(v);         // row_half_mirror
+    v += dpp_f<0x140>(v);         // row_mirror
+    v += dpp_f<0x142, 0xA>(v);    // row_bcast:15 into rows 1, 3
+    v += dpp_f<0x143, 0xC>(v);    // row_bcast:31 into rows 2, 3
+    return lane63(v);
 }
 
 // Block-wide sum of NV values per thread; every thread gets the totals.
