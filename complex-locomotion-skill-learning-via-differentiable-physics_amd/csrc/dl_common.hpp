@@ -111,13 +111,15 @@ __device__ __forceinline__ double dpp_f(double v) {
     const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xF, true);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
-__device__ __forceinline__ float lane63(float v) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+
+// Value of lane q (wave-uniform q), to every lane.
+__device__ __forceinline__ float lane_read(float v, int q) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), q));
 }
-__device__ __forceinline__ double lane63(double v) {
+__device__ __forceinline__ double lane_read(double v, int q) {
     const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), 63);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), q);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), q);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
@@ -136,7 +138,7 @@ __device__ __forceinline__ T wave_sum(T v) {
     v += dpp_f<0x140>(v);         // row_mirror
     v += dpp_f<0x142, 0xA>(v);    // row_bcast:15 into rows 1, 3
     v += dpp_f<0x143, 0xC>(v);    // row_bcast:31 into rows 2, 3
-    return lane63(v);
+    return lane_read(v, 63);
 }
 
 // Block-wide sum of NV values per thread; every thread gets the totals.

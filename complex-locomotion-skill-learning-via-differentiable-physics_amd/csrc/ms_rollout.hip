@@ -462,6 +462,7 @@ __device__ void bwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int r, in
     __syncthreads();
     for (int i = tid; i < N; i += BT) {
         T gx[3] = {0, 0, 0}, gv[3] = {0, 0, 0};
+#pragma unroll 4
         for (int e = R.vptr[i]; e < R.vptr[i + 1]; ++e) {
             const int ent = R.vidx[e], s = ent >> 1;
             const T sg = (ent & 1) ? T(-1) : T(1);
@@ -479,6 +480,7 @@ __device__ void bwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int r, in
     }
     for (int o = tid; o < O; o += BT) {
         T ab = T(0);
+#pragma unroll 4
         for (int e = R.aptr[o]; e < R.aptr[o + 1]; ++e) ab += G[6 * S + R.aidx[e]];
         const T z = ab * m.dout[o];
         m.z2b[o] = z;
@@ -807,13 +809,111 @@ __device__ __forceinline__ void wave_load_weights(const Rob<T> &R, int H, int la
     w.b1 = on ? R.b1[lane] : T(0);
 }
 
+// Per-lane copy of the wave's robot topology, held in registers for the
+// whole rollout so the spring and vertex phases issue their state loads at
+// once instead of first fetching indices from LDS: springs lane + 64 k
+// (k < TSK) and the incident-spring entries of vertex `lane` (the first TDK;
+// the wave kernel has N <= 64).  Springs and entries beyond those are read
+// from the LDS copy.
+constexpr int TSK = 6, TDK = 24;
+#ifndef DL_FWD_SK
+#define DL_FWD_SK 6
+#endif
+#ifndef DL_FWD_DK
+#define DL_FWD_DK 24
+#endif
+#ifndef DL_BWD_SK
+#define DL_BWD_SK 0
+#endif
+#ifndef DL_BWD_DK
+#define DL_BWD_DK 0
+#endif
+constexpr int FSK = DL_FWD_SK, FDK = DL_FWD_DK;   // register depth of the forward kernel
+constexpr int BSK = DL_BWD_SK, BDK = DL_BWD_DK;   // and of the adjoint kernel
+template <typename T>
+struct LaneTopo {
+    int sp[TSK];                 // a | b << 8 | actuator << 16 (0 past the last spring)
+    T L0[TSK], k[TSK], coef[TSK];
+    int inc[TDK];                // vidx entries (spring << 1 | side), 0 past the degree
+    int deg;                     // incident springs of vertex `lane` (0 when lane >= N)
+};
+
+template <typename T>
+__device__ __forceinline__ void lane_topo_load(const Rob<T> &R, int lane, LaneTopo<T> &tp) {
+#pragma unroll
+    for (int q = 0; q < TSK; ++q) {
+        const int s = lane + 64 * q;
+        const bool on = s < R.S;
+        tp.sp[q] = on ? (R.sa[s] | (R.sb[s] << 8) | (R.sact[s] << 16)) : 0;
+        tp.L0[q] = on ? R.L0[s] : T(1);
+        tp.k[q] = on ? R.k[s] : T(0);
+        tp.coef[q] = on ? R.coef[s] : T(0);
+    }
+    const int e0 = lane < R.N ? R.vptr[lane] : 0;
+    tp.deg = lane < R.N ? R.vptr[lane + 1] - e0 : 0;
+#pragma unroll
+    for (int q = 0; q < TDK; ++q) tp.inc[q] = q < tp.deg ? R.vidx[e0 + q] : 0;
+}
+
+// Spring impulses J[c][s] (J = m.J, [3][S]) of the current state.  Spring
+// lane + 64 k from registers (loads issued unconditionally -- springs past
+// the last read vertex 0 -- and stores masked), the rest from LDS.
+template <int SK, typename T>
+__device__ __forceinline__ void spring_impulses(const RArgs<T> &A, const Rob<T> &R, WSm<T> &m,
+                                                const LaneTopo<T> &tp, int lane) {
+    const int N = R.N, S = R.S;
+    auto one = [&](int s, int a, int b, int ac, T L0, T kk, T co) {
+        const T d[3] = {m.xs[a] - m.xs[b], m.xs[N + a] - m.xs[N + b], m.xs[2 * N + a] - m.xs[2 * N + b]};
+        const T wv[3] = {m.vs[a] - m.vs[b], m.vs[N + a] - m.vs[N + b], m.vs[2 * N + a] - m.vs[2 * N + b]};
+        T J[3];
+        spring_impulse<T, true>(d, wv, L0, kk, co, m.outb[ac], A.P, J);
+        if (s < S) {
+            m.J[s] = J[0]; m.J[S + s] = J[1]; m.J[2 * S + s] = J[2];
+        }
+    };
+#pragma unroll
+    for (int q = 0; q < SK; ++q) {
+        const int p = tp.sp[q];
+        one(lane + 64 * q, p & 0xff, (p >> 8) & 0xff, p >> 16, tp.L0[q], tp.k[q], tp.coef[q]);
+    }
+#pragma unroll 2
+    for (int s = lane + 64 * SK; s < S; s += 64)
+        one(s, R.sa[s], R.sb[s], R.sact[s], R.L0[s], R.k[s], R.coef[s]);
+}
+
+// Impulse on vertex i (= lane): signed sum of J over its incident springs,
+// the first TDK entries from registers (all loads at once, entries past the
+// degree masked by selection), the rest from LDS.
+template <int DK, typename T>
+__device__ __forceinline__ void gather_J(const Rob<T> &R, const WSm<T> &m, const LaneTopo<T> &tp,
+                                         int i, T vi[3]) {
+    const int S = R.S;
+    vi[0] = vi[1] = vi[2] = T(0);
+#pragma unroll
+    for (int q = 0; q < DK; ++q) {
+        const int ent = tp.inc[q], s = ent >> 1;
+        const T j0 = m.J[s], j1 = m.J[S + s], j2 = m.J[2 * S + s];
+        const bool on = q < tp.deg;
+        const bool pos = ent & 1;
+        vi[0] += on ? (pos ? j0 : -j0) : T(0);
+        vi[1] += on ? (pos ? j1 : -j1) : T(0);
+        vi[2] += on ? (pos ? j2 : -j2) : T(0);
+    }
+#pragma unroll 4
+    for (int e = R.vptr[i] + DK; e < R.vptr[i + 1]; ++e) {
+        const int ent = R.vidx[e], s = ent >> 1;
+        const T sg = (ent & 1) ? T(1) : T(-1);
+        vi[0] += sg * m.J[s]; vi[1] += sg * m.J[S + s]; vi[2] += sg * m.J[2 * S + s];
+    }
+}
+
 // one forward step of the wave's robot (MODE_FWD / MODE_REC like fwd_step).
 // WREG: layer-1 weights from registers (forward kernel) or from the shared
 // LDS copy of W1 (adjoint kernel, whose adjoint phase needs the registers).
-template <typename T, int IMAX, int OMAX, bool WREG>
+template <typename T, int IMAX, int OMAX, bool WREG, int SK, int DK>
 __device__ __forceinline__ void wave_fwd_step(const RArgs<T> &A, const Rob<T> &R, WSm<T> &m,
-                              const WRegs<T, IMAX, OMAX> &w, const T *sin_tab, int lane, int t,
-                              int mode, int slot, T &lsum) {
+                              const WRegs<T, IMAX, OMAX> &w, const LaneTopo<T> &tp,
+                              const T *sin_tab, int lane, int t, int mode, int slot, T &lsum) {
     const int N = R.N, S = R.S, I = R.I, O = R.O, H = A.H;
     T s6[6] = {0, 0, 0, 0, 0, 0};
     const T invN = T(1) / T(N);
@@ -935,24 +1035,11 @@ __device__ __forceinline__ void wave_fwd_step(const RArgs<T> &A, const Rob<T> &R
         }
     }
     wave_sync();
-#pragma unroll 2
-    for (int s = lane; s < S; s += 64) {
-        const int a = R.sa[s], b = R.sb[s];
-        const T d[3] = {m.xs[a] - m.xs[b], m.xs[N + a] - m.xs[N + b], m.xs[2 * N + a] - m.xs[2 * N + b]};
-        const T wv[3] = {m.vs[a] - m.vs[b], m.vs[N + a] - m.vs[N + b], m.vs[2 * N + a] - m.vs[2 * N + b]};
-        T J[3];
-        spring_impulse<T, true>(d, wv, R.L0[s], R.k[s], R.coef[s], m.outb[R.sact[s]], A.P, J);
-        m.J[s] = J[0]; m.J[S + s] = J[1]; m.J[2 * S + s] = J[2];
-    }
+    spring_impulses<SK>(A, R, m, tp, lane);
     wave_sync();
     for (int i = lane; i < N; i += 64) {
-        T vi[3] = {0, 0, 0};
-#pragma unroll 4
-        for (int e = R.vptr[i]; e < R.vptr[i + 1]; ++e) {
-            const int ent = R.vidx[e], s = ent >> 1;
-            const T sg = (ent & 1) ? T(1) : T(-1);
-            vi[0] += sg * m.J[s]; vi[1] += sg * m.J[S + s]; vi[2] += sg * m.J[2 * S + s];
-        }
+        T vi[3];
+        gather_J<DK>(R, m, tp, i, vi);
         const T x[3] = {m.xs[i], m.xs[N + i], m.xs[2 * N + i]};
         const T v[3] = {m.vs[i], m.vs[N + i], m.vs[2 * N + i]};
         T xn[3], vn[3];
@@ -993,6 +1080,8 @@ __global__ void __launch_bounds__(256) k_wave_fwd(RArgs<T> A, int n_robots) {
     if (t0 >= t1) return;
     WRegs<T, IMAX, OMAX> w;
     wave_load_weights(R, A.H, lane, w);
+    LaneTopo<T> tp;
+    lane_topo_load(R, lane, tp);
     for (int i = lane; i < 3 * N; i += 64) {
         m.xs[i] = R.state[i];
         m.vs[i] = R.state[3 * N + i];
@@ -1000,7 +1089,7 @@ __global__ void __launch_bounds__(256) k_wave_fwd(RArgs<T> A, int n_robots) {
     wave_sync();
     T lsum = T(0);
     for (int t = t0; t < t1; ++t)
-        wave_fwd_step<T, IMAX, OMAX, true>(A, R, m, w, sh.sin_tab, lane, t, MODE_FWD, 0, lsum);
+        wave_fwd_step<T, IMAX, OMAX, true, FSK, FDK>(A, R, m, w, tp, sh.sin_tab, lane, t, MODE_FWD, 0, lsum);
     if (t1 == R.Th) {
         T s6[6] = {0, 0, 0, 0, 0, 0};
         const T invN = T(1) / T(N);
@@ -1049,9 +1138,10 @@ __device__ __forceinline__ void pref_load(const Rob<T> &R, int H, int lane, int 
     p.cv = lane < 6 ? R.scv[slot * 6 + lane] : T(0);
 }
 
-template <typename T, int IMAX, int OMAX>
+template <typename T, int IMAX, int OMAX, int SK, int DK>
 __device__ __forceinline__ void wave_bwd_step(const RArgs<T> &A, const Rob<T> &R, WSm<T> &m, int r, int lane,
-                              int t, int slot, Pref<T> &pf, const WRegs<T, 1, OMAX> &w, int next_slot) {
+                              int t, int slot, Pref<T> &pf, const WRegs<T, 1, OMAX> &w,
+                              const LaneTopo<T> &tp, int next_slot) {
     const int N = R.N, S = R.S, I = R.I, O = R.O, H = A.H;
     // consume the prefetched step
 #pragma unroll
@@ -1068,27 +1158,15 @@ __device__ __forceinline__ void wave_bwd_step(const RArgs<T> &A, const Rob<T> &R
     }
     T cvr[6];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) cvr[q] = __shfl(pf.cv, q, 64);
-    const T dhj = pf.dh;
+    for (int q = 0; q < 6; ++q) cvr[q] = lane_read(pf.cv, q);
+    const T dhj = pf.dh, pf_dout = pf.dout;
     if (next_slot >= 0) pref_load(R, H, lane, next_slot, pf);   // in flight during this step
     wave_sync();
-#pragma unroll 2
-    for (int s = lane; s < S; s += 64) {
-        const int a = R.sa[s], b = R.sb[s];
-        const T d[3] = {m.xs[a] - m.xs[b], m.xs[N + a] - m.xs[N + b], m.xs[2 * N + a] - m.xs[2 * N + b]};
-        const T wv[3] = {m.vs[a] - m.vs[b], m.vs[N + a] - m.vs[N + b], m.vs[2 * N + a] - m.vs[2 * N + b]};
-        T J[3];
-        spring_impulse<T, true>(d, wv, R.L0[s], R.k[s], R.coef[s], m.outb[R.sact[s]], A.P, J);
-        m.J[s] = J[0]; m.J[S + s] = J[1]; m.J[2 * S + s] = J[2];
-    }
+    spring_impulses<SK>(A, R, m, tp, lane);
     wave_sync();
     for (int i = lane; i < N; i += 64) {
-        T vi[3] = {0, 0, 0};
-        for (int e = R.vptr[i]; e < R.vptr[i + 1]; ++e) {
-            const int ent = R.vidx[e], s = ent >> 1;
-            const T sg = (ent & 1) ? T(1) : T(-1);
-            vi[0] += sg * m.J[s]; vi[1] += sg * m.J[S + s]; vi[2] += sg * m.J[2 * S + s];
-        }
+        T vi[3];
+        gather_J<DK>(R, m, tp, i, vi);
         const T x[3] = {m.xs[i], m.xs[N + i], m.xs[2 * N + i]};
         const T v[3] = {m.vs[i], m.vs[N + i], m.vs[2 * N + i]};
         const T xnb[3] = {m.xb[i], m.xb[N + i], m.xb[2 * N + i]};
@@ -1103,26 +1181,53 @@ __device__ __forceinline__ void wave_bwd_step(const RArgs<T> &A, const Rob<T> &R
         }
     }
     wave_sync();
+    // spring adjoints G[c][s] (d: c < 3, relative velocity: 3 <= c < 6); the
+    // actuation adjoints are summed per actuator in registers (lane partial
+    // sums, then one wave sum per actuator)
     T *G = m.J;
-#pragma unroll 2
-    for (int s = lane; s < S; s += 64) {
-        const int a = R.sa[s], b = R.sb[s];
+    T abp[OMAX];
+#pragma unroll
+    for (int o = 0; o < OMAX; ++o) abp[o] = T(0);
+    auto adj_one = [&](int s, int a, int b, int ac, T L0, T kk, T co) {
         const T d[3] = {m.xs[a] - m.xs[b], m.xs[N + a] - m.xs[N + b], m.xs[2 * N + a] - m.xs[2 * N + b]};
         const T wv[3] = {m.vs[a] - m.vs[b], m.vs[N + a] - m.vs[N + b], m.vs[2 * N + a] - m.vs[2 * N + b]};
         const T Jb[3] = {m.vib[b] - m.vib[a], m.vib[N + b] - m.vib[N + a], m.vib[2 * N + b] - m.vib[2 * N + a]};
         T gd[3], gw[3], ga;
-        spring_impulse_adj<T, true>(d, wv, R.L0[s], R.k[s], R.coef[s], m.outb[R.sact[s]], A.P, Jb, gd, gw, ga);
+        spring_impulse_adj<T, true>(d, wv, L0, kk, co, m.outb[ac], A.P, Jb, gd, gw, ga);
+        if (s < S) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            G[c * S + s] = gd[c];
-            G[(3 + c) * S + s] = gw[c];
+            for (int c = 0; c < 3; ++c) {
+                G[c * S + s] = gd[c];
+                G[(3 + c) * S + s] = gw[c];
+            }
+#pragma unroll
+            for (int o = 0; o < OMAX; ++o) abp[o] += ac == o ? ga : T(0);
         }
-        G[6 * S + s] = ga;
+    };
+#pragma unroll
+    for (int q = 0; q < SK; ++q) {
+        const int p = tp.sp[q];
+        adj_one(lane + 64 * q, p & 0xff, (p >> 8) & 0xff, p >> 16, tp.L0[q], tp.k[q], tp.coef[q]);
     }
+#pragma unroll 2
+    for (int s = lane + 64 * SK; s < S; s += 64)
+        adj_one(s, R.sa[s], R.sb[s], R.sact[s], R.L0[s], R.k[s], R.coef[s]);
     wave_sync();
     for (int i = lane; i < N; i += 64) {
         T gx[3] = {0, 0, 0}, gv[3] = {0, 0, 0};
-        for (int e = R.vptr[i]; e < R.vptr[i + 1]; ++e) {
+#pragma unroll
+        for (int q = 0; q < DK; ++q) {
+            const int ent = tp.inc[q], s = ent >> 1;
+            const bool on = q < tp.deg, neg = ent & 1;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const T g0 = G[c * S + s], g1 = G[(3 + c) * S + s];
+                gx[c] += on ? (neg ? -g0 : g0) : T(0);
+                gv[c] += on ? (neg ? -g1 : g1) : T(0);
+            }
+        }
+#pragma unroll 4
+        for (int e = R.vptr[i] + DK; e < R.vptr[i + 1]; ++e) {
             const int ent = R.vidx[e], s = ent >> 1;
             const T sg = (ent & 1) ? T(-1) : T(1);
 #pragma unroll
@@ -1137,12 +1242,15 @@ __device__ __forceinline__ void wave_bwd_step(const RArgs<T> &A, const Rob<T> &R
             m.vb[d * N + i] += gv[d];
         }
     }
-    for (int o = lane; o < O; o += 64) {
-        T ab = T(0);
-        for (int e = R.aptr[o]; e < R.aptr[o + 1]; ++e) ab += G[6 * S + R.aidx[e]];
-        const T z = ab * m.dout[o];
-        m.z2b[o] = z;
-        R.sz2[(size_t)slot * O + o] = z;
+#pragma unroll
+    for (int o = 0; o < OMAX; ++o) {
+        if (o < O) {
+            const T z = wave_sum(abp[o]) * lane_read(pf_dout, o);
+            if (lane == o) {
+                m.z2b[o] = z;
+                R.sz2[(size_t)slot * O + o] = z;
+            }
+        }
     }
     wave_sync();
     {
@@ -1285,6 +1393,8 @@ __global__ void __launch_bounds__(256) k_wave_bwd(RArgs<T> A, int n_robots) {
     if (tb <= 0) return;
     WRegs<T, 1, OMAX> w;
     wave_load_weights(R, A.H, lane, w);
+    LaneTopo<T> tp;
+    lane_topo_load(R, lane, tp);
     for (int i = lane; i < 3 * N; i += 64) {
         m.xb[i] = R.adj[i];
         m.vb[i] = R.adj[3 * N + i];
@@ -1327,13 +1437,15 @@ __global__ void __launch_bounds__(256) k_wave_bwd(RArgs<T> A, int n_robots) {
         }
         wave_sync();
         for (int t = s0; t < tb; ++t)
-            wave_fwd_step<T, 1, OMAX, false>(A, R, m, w, sh.sin_tab, lane, t, MODE_REC, t - s0, dummy);
+            wave_fwd_step<T, 1, OMAX, false, BSK, BDK>(A, R, m, w, tp, sh.sin_tab, lane, t, MODE_REC, t - s0,
+                                             dummy);
         // the recompute's scratch stores complete before any lane reads them back
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
         Pref<T> pf;
         pref_load(R, A.H, lane, tb - 1 - s0, pf);
         for (int t = tb - 1; t >= lo; --t)
-            wave_bwd_step<T, IMAX, OMAX>(A, R, m, r, lane, t, t - s0, pf, w, t - 1 >= lo ? t - 1 - s0 : -1);
+            wave_bwd_step<T, IMAX, OMAX, BSK, BDK>(A, R, m, r, lane, t, t - s0, pf, w, tp,
+                                         t - 1 >= lo ? t - 1 - s0 : -1);
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
         wave_sync();
         wave_seg_wgrad(A, R, m.J, 7 * A.max_s, lane, lo - s0, tb - lo);
