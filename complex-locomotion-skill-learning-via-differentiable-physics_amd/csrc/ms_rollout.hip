@@ -53,9 +53,25 @@ struct Rob {
     T *state, *adj, *ckpt;
     T *traj;
     const T *trajg;
-    // scratch (segment of K steps)
-    T *sx, *sin, *shid, *sdhid, *sout, *sdout, *scv, *sz1, *sz2, *spill;
+    // scratch: rows of a segment of K steps (buffer 0), the spring spill
+    // area, and a second buffer of K rows (scr2) for the pipelined adjoint
+    T *sx, *sin, *shid, *sdhid, *sout, *sdout, *scv, *sz1, *sz2, *spill, *scr2;
 };
+
+// Point the scratch rows of q at a buffer of K rows starting at s.
+template <typename T>
+__device__ __forceinline__ void carve_scratch(Rob<T> &q, T *s, int K, int H) {
+    const int N = q.N, I = q.I, O = q.O;
+    q.sx = s; s += (size_t)K * 6 * N;
+    q.sin = s; s += (size_t)K * I;
+    q.shid = s; s += (size_t)K * H;
+    q.sdhid = s; s += (size_t)K * H;
+    q.sz1 = s; s += (size_t)K * H;
+    q.sout = s; s += (size_t)K * O;
+    q.sdout = s; s += (size_t)K * O;
+    q.sz2 = s; s += (size_t)K * O;
+    q.scv = s;
+}
 
 template <typename T>
 __device__ __forceinline__ Rob<T> make_rob(const RArgs<T> &A, int r) {
@@ -83,18 +99,12 @@ __device__ __forceinline__ Rob<T> make_rob(const RArgs<T> &A, int r) {
     const int K = A.K, N = q.N;
     T *s = A.scratch ? A.scratch + R.scratch : nullptr;
     if (s) {
-        q.sx = s; s += (size_t)K * 6 * N;
-        q.sin = s; s += (size_t)K * I;
-        q.shid = s; s += (size_t)K * H;
-        q.sdhid = s; s += (size_t)K * H;
-        q.sz1 = s; s += (size_t)K * H;
-        q.sout = s; s += (size_t)K * O;
-        q.sdout = s; s += (size_t)K * O;
-        q.sz2 = s; s += (size_t)K * O;
-        q.scv = s; s += (size_t)K * 6;
-        q.spill = s;
+        carve_scratch(q, s, K, H);
+        q.spill = q.scv + (size_t)K * 6;
+        q.scr2 = q.spill + (size_t)10 * q.S;
     } else {
         q.sx = q.sin = q.shid = q.sdhid = q.sz1 = q.sout = q.sdout = q.sz2 = q.scv = q.spill = nullptr;
+        q.scr2 = nullptr;
     }
     return q;
 }
@@ -705,9 +715,17 @@ __host__ inline size_t wave_smem_bytes(int max_n, int max_s, int max_i, int max_
     return n * sizeof(float) + sizeof(int) * (6 * (size_t)max_s + max_n + max_o + 2);
 }
 
+// Pipelined adjoint: per robot an adjoint region and a recompute region.
+__host__ inline size_t wave_pipe_smem_bytes(int max_n, int max_s, int max_i, int max_o, int n_tab,
+                                            int robots) {
+    size_t n = robots * (wave_region(max_n, max_s, max_i, max_o, true) +
+                         wave_region(max_n, max_s, max_i, max_o, false)) +
+               3 * (size_t)max_s + n_tab + (size_t)max_i * (HMAX + 1);
+    return n * sizeof(float) + sizeof(int) * (6 * (size_t)max_s + max_n + max_o + 2);
+}
+
 template <typename T>
-__device__ __forceinline__ WSm<T> wave_carve(const RArgs<T> &A, T *base, int wid, bool bwd) {
-    T *p = base + (size_t)wid * wave_region(A.max_n, A.max_s, A.max_i, A.max_o, bwd);
+__device__ __forceinline__ WSm<T> wave_carve_at(const RArgs<T> &A, T *p, bool bwd) {
     WSm<T> m;
     const int n3 = 3 * A.max_n;
     m.xs = p; p += n3;
@@ -730,6 +748,12 @@ __device__ __forceinline__ WSm<T> wave_carve(const RArgs<T> &A, T *base, int wid
     return m;
 }
 
+template <typename T>
+__device__ __forceinline__ WSm<T> wave_carve(const RArgs<T> &A, T *base, int wid, bool bwd) {
+    return wave_carve_at(A, base + (size_t)wid * wave_region(A.max_n, A.max_s, A.max_i, A.max_o, bwd),
+                         bwd);
+}
+
 // Block-wide: stage the topology of robot r0's design (and W1 for the
 // adjoint) into the shared region; returns pointers through `sh`.
 template <typename T>
@@ -738,10 +762,12 @@ struct WShared {
     int *topo_i;
 };
 
+// `regions`: reals taken by the wavefronts' own regions at the start of LDS.
 template <typename T>
-__device__ WShared<T> wave_stage(const RArgs<T> &A, const Rob<T> &R0, unsigned char *raw, bool bwd) {
+__device__ WShared<T> wave_stage(const RArgs<T> &A, const Rob<T> &R0, unsigned char *raw, bool bwd,
+                                 size_t regions) {
     WShared<T> sh;
-    T *p = reinterpret_cast<T *>(raw) + (blockDim.x >> 6) * wave_region(A.max_n, A.max_s, A.max_i, A.max_o, bwd);
+    T *p = reinterpret_cast<T *>(raw) + regions;
     sh.topo_r = p; p += 3 * A.max_s;
     sh.wl = bwd ? p : nullptr;
     if (bwd) p += (size_t)A.max_i * (HMAX + 1);
@@ -1060,7 +1086,8 @@ __global__ void __launch_bounds__(256) k_wave_fwd(RArgs<T> A, int n_robots) {
     const int lane = threadIdx.x & 63;
     const int r0 = blockIdx.x * (blockDim.x >> 6);
     const Rob<T> R0 = make_rob(A, r0);
-    const WShared<T> sh = wave_stage(A, R0, smem_raw, false);
+    const WShared<T> sh = wave_stage(A, R0, smem_raw, false,
+                                     (blockDim.x >> 6) * wave_region(A.max_n, A.max_s, A.max_i, A.max_o, false));
     __syncthreads();
     const int r = r0 + wid;
     if (r >= n_robots) return;
@@ -1369,38 +1396,18 @@ __device__ __forceinline__ void wave_seg_wgrad(const RArgs<T> &A, const Rob<T> &
     }
 }
 
-template <typename T, int IMAX, int OMAX>
-__global__ void __launch_bounds__(256) k_wave_bwd(RArgs<T> A, int n_robots) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    const int r0 = blockIdx.x * (blockDim.x >> 6);
-    const Rob<T> R0 = make_rob(A, r0);
-    const WShared<T> sh = wave_stage(A, R0, smem_raw, true);
-    __syncthreads();
-    const int r = r0 + wid;
-    if (r >= n_robots) return;
-    Rob<T> R = make_rob(A, r);
-    WSm<T> m = wave_carve(A, reinterpret_cast<T *>(smem_raw), wid, true);
-    if (A.robots[r].spring0 != A.robots[r0].spring0 || A.robots[r].weights != A.robots[r0].weights) {
-        if (lane == 0) R.Wg1[0] = T(NAN);   // robots of one workgroup must share a design
-        return;
-    }
-    wave_point_at_shared(R, sh, true);
-    for (int q = lane; q < (A.max_i > 192 ? A.max_i : 192) + 4; q += 64) m.inb[q] = T(0);
-    const int N = R.N, K = A.K;
-    int tb = A.robots[r].t_bwd;
-    if (tb <= 0) return;
-    WRegs<T, 1, OMAX> w;
-    wave_load_weights(R, A.H, lane, w);
-    LaneTopo<T> tp;
-    lane_topo_load(R, lane, tp);
+// Adjoint state of the robot's last state (t_bwd) into LDS, with the loss
+// terms of the final state when t_bwd is the horizon.
+template <typename T>
+__device__ __forceinline__ void wave_bwd_begin(const RArgs<T> &A, const Rob<T> &R, WSm<T> &m, int r,
+                                               int lane) {
+    const int N = R.N;
     for (int i = lane; i < 3 * N; i += 64) {
         m.xb[i] = R.adj[i];
         m.vb[i] = R.adj[3 * N + i];
     }
     wave_sync();
-    if (tb == R.Th) {
+    if (A.robots[r].t_bwd == R.Th) {
         T s6[6] = {0, 0, 0, 0, 0, 0};
         const T invN = T(1) / T(N);
         for (int i = lane; i < N; i += 64)
@@ -1425,38 +1432,189 @@ __global__ void __launch_bounds__(256) k_wave_bwd(RArgs<T> A, int n_robots) {
             }
         wave_sync();
     }
-    int budget = A.max_steps > 0 ? A.max_steps : 0x7fffffff;
-    T dummy = T(0);
-    while (tb > 0 && budget > 0) {
-        const int seg = (tb - 1) / K, s0 = seg * K;
-        const int lo = max(s0, tb - budget);
-        const T *ck = R.ckpt + (size_t)seg * 6 * N;
-        for (int i = lane; i < 3 * N; i += 64) {
-            m.xs[i] = ck[i];
-            m.vs[i] = ck[3 * N + i];
-        }
-        wave_sync();
-        for (int t = s0; t < tb; ++t)
-            wave_fwd_step<T, 1, OMAX, false, BSK, BDK>(A, R, m, w, tp, sh.sin_tab, lane, t, MODE_REC, t - s0,
-                                             dummy);
-        // the recompute's scratch stores complete before any lane reads them back
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-        Pref<T> pf;
-        pref_load(R, A.H, lane, tb - 1 - s0, pf);
-        for (int t = tb - 1; t >= lo; --t)
-            wave_bwd_step<T, IMAX, OMAX, BSK, BDK>(A, R, m, r, lane, t, t - s0, pf, w, tp,
-                                         t - 1 >= lo ? t - 1 - s0 : -1);
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-        wave_sync();
-        wave_seg_wgrad(A, R, m.J, 7 * A.max_s, lane, lo - s0, tb - lo);
-        wave_sync();
-        budget -= tb - lo;
-        tb = lo;
+}
+
+// Recompute steps [s0, t1) of the segment starting at checkpoint s0 / K into
+// the scratch rows of R (slot t - s0).
+template <typename T, int OMAX>
+__device__ __forceinline__ void wave_recompute(const RArgs<T> &A, const Rob<T> &R, WSm<T> &m,
+                                               const WRegs<T, 1, OMAX> &w, const LaneTopo<T> &tp,
+                                               const T *sin_tab, int lane, int s0, int t1) {
+    const int N = R.N;
+    const T *ck = R.ckpt + (size_t)(s0 / A.K) * 6 * N;
+    for (int i = lane; i < 3 * N; i += 64) {
+        m.xs[i] = ck[i];
+        m.vs[i] = ck[3 * N + i];
     }
+    wave_sync();
+    T dummy = T(0);
+    for (int t = s0; t < t1; ++t)
+        wave_fwd_step<T, 1, OMAX, false, BSK, BDK>(A, R, m, w, tp, sin_tab, lane, t, MODE_REC,
+                                                   t - s0, dummy);
+    // the scratch stores complete before any lane reads them back
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+}
+
+// Adjoint steps tb - 1 down to lo of the segment starting at s0 (rows of R
+// recomputed), then the segment's weight-gradient fold.
+template <typename T, int IMAX, int OMAX>
+__device__ __forceinline__ void wave_bwd_segment(const RArgs<T> &A, const Rob<T> &R, WSm<T> &m,
+                                                 const WRegs<T, 1, OMAX> &w, const LaneTopo<T> &tp,
+                                                 int r, int lane, int s0, int lo, int tb) {
+    Pref<T> pf;
+    pref_load(R, A.H, lane, tb - 1 - s0, pf);
+    for (int t = tb - 1; t >= lo; --t)
+        wave_bwd_step<T, IMAX, OMAX, BSK, BDK>(A, R, m, r, lane, t, t - s0, pf, w, tp,
+                                               t - 1 >= lo ? t - 1 - s0 : -1);
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    wave_sync();
+    wave_seg_wgrad(A, R, m.J, 7 * A.max_s, lane, lo - s0, tb - lo);
+    wave_sync();
+}
+
+template <typename T>
+__device__ __forceinline__ void wave_bwd_end(const Rob<T> &R, const WSm<T> &m, int lane) {
+    const int N = R.N;
     for (int i = lane; i < 3 * N; i += 64) {
         R.adj[i] = m.xb[i];
         R.adj[3 * N + i] = m.vb[i];
     }
+}
+
+// The whole adjoint of robot r by one wavefront: segments in turn, each
+// recomputed from its checkpoint and then run in reverse.
+template <typename T, int IMAX, int OMAX>
+__device__ void wave_bwd_robot(const RArgs<T> &A, Rob<T> &R, WSm<T> &m, const WShared<T> &sh, int r,
+                               int lane) {
+    for (int q = lane; q < (A.max_i > 192 ? A.max_i : 192) + 4; q += 64) m.inb[q] = T(0);
+    const int K = A.K;
+    int tb = A.robots[r].t_bwd;
+    if (tb <= 0) return;
+    WRegs<T, 1, OMAX> w;
+    wave_load_weights(R, A.H, lane, w);
+    LaneTopo<T> tp;
+    lane_topo_load(R, lane, tp);
+    wave_bwd_begin(A, R, m, r, lane);
+    int budget = A.max_steps > 0 ? A.max_steps : 0x7fffffff;
+    while (tb > 0 && budget > 0) {
+        const int s0 = (tb - 1) / K * K;
+        const int lo = max(s0, tb - budget);
+        wave_recompute(A, R, m, w, tp, sh.sin_tab, lane, s0, tb);
+        wave_bwd_segment<T, IMAX, OMAX>(A, R, m, w, tp, r, lane, s0, lo, tb);
+        budget -= tb - lo;
+        tb = lo;
+    }
+    wave_bwd_end(R, m, lane);
+}
+
+template <typename T, int IMAX, int OMAX>
+__global__ void __launch_bounds__(256) k_wave_bwd(RArgs<T> A, int n_robots) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int r0 = blockIdx.x * (blockDim.x >> 6);
+    const Rob<T> R0 = make_rob(A, r0);
+    const WShared<T> sh = wave_stage(A, R0, smem_raw, true,
+                                     (blockDim.x >> 6) * wave_region(A.max_n, A.max_s, A.max_i, A.max_o, true));
+    __syncthreads();
+    const int r = r0 + wid;
+    if (r >= n_robots) return;
+    Rob<T> R = make_rob(A, r);
+    WSm<T> m = wave_carve(A, reinterpret_cast<T *>(smem_raw), wid, true);
+    if (A.robots[r].spring0 != A.robots[r0].spring0 || A.robots[r].weights != A.robots[r0].weights) {
+        if (lane == 0) R.Wg1[0] = T(NAN);   // robots of one workgroup must share a design
+        return;
+    }
+    wave_point_at_shared(R, sh, true);
+    wave_bwd_robot<T, IMAX, OMAX>(A, R, m, sh, r, lane);
+}
+
+// Segments the adjoint of a robot walks from t_bwd with a step budget.
+__device__ __forceinline__ int bwd_segments(int tb, int budget, int K) {
+    int n = 0;
+    while (tb > 0 && budget > 0) {
+        const int s0 = (tb - 1) / K * K, lo = max(s0, tb - budget);
+        budget -= tb - lo;
+        tb = lo;
+        ++n;
+    }
+    return n;
+}
+
+// Adjoint with the segment recompute on a second wavefront per robot
+// (wavefronts 2j: adjoint, 2j + 1: recompute of robot j), pipelined over two
+// scratch buffers: while the adjoint wavefront reverses segment s from one
+// buffer, the recompute wavefront rebuilds segment s - 1 into the other.
+// One workgroup barrier per segment; every wavefront runs the workgroup's
+// largest segment count, idle once its robot is done (or absent), so the
+// barrier counts always agree.
+template <typename T, int IMAX, int OMAX>
+__global__ void __launch_bounds__(512) k_wave_bwd_pipe(RArgs<T> A, int n_robots) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ int iters;
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int nrob = blockDim.x >> 7, rl = wid >> 1, role = wid & 1;
+    const int r0 = blockIdx.x * nrob;
+    const int K = A.K;
+    const int budget0 = A.max_steps > 0 ? A.max_steps : 0x7fffffff;
+    const Rob<T> R0 = make_rob(A, r0);
+    const size_t Rb = wave_region(A.max_n, A.max_s, A.max_i, A.max_o, true);
+    const size_t Rf = wave_region(A.max_n, A.max_s, A.max_i, A.max_o, false);
+    const WShared<T> sh = wave_stage(A, R0, smem_raw, true, (size_t)nrob * (Rb + Rf));
+    if (threadIdx.x == 0) {
+        int mx = 0;
+        for (int j = 0; j < nrob && r0 + j < n_robots; ++j)
+            mx = max(mx, bwd_segments(A.robots[r0 + j].t_bwd, budget0, K));
+        iters = mx;
+    }
+    __syncthreads();
+    const int r = r0 + rl;
+    bool active = r < n_robots;
+    if (active && (A.robots[r].spring0 != A.robots[r0].spring0 ||
+                   A.robots[r].weights != A.robots[r0].weights)) {
+        // robots of one workgroup must share a design
+        if (role == 0 && lane == 0) make_rob(A, r).Wg1[0] = T(NAN);
+        active = false;
+    }
+    Rob<T> R = make_rob(A, active ? r : r0);
+    wave_point_at_shared(R, sh, true);
+    T *const scr0 = R.sx, *const scr1 = R.scr2;   // scratch buffers 0 and 1
+    T *region = reinterpret_cast<T *>(smem_raw) + (size_t)rl * (Rb + Rf);
+    WSm<T> m = role == 0 ? wave_carve_at(A, region, true) : wave_carve_at(A, region + Rb, false);
+    int tb = active ? A.robots[r].t_bwd : 0, budget = budget0;
+    const bool work = tb > 0;
+    WRegs<T, 1, OMAX> w;
+    LaneTopo<T> tp;
+    if (work) {
+        for (int q = lane; q < (A.max_i > 192 ? A.max_i : 192) + 4; q += 64) m.inb[q] = T(0);
+        wave_load_weights(R, A.H, lane, w);
+        lane_topo_load(R, lane, tp);
+        if (role == 0) wave_bwd_begin(A, R, m, r, lane);
+        else wave_recompute(A, R, m, w, tp, sh.sin_tab, lane, (tb - 1) / K * K, tb);
+    }
+    __syncthreads();
+    int buf = 0;
+    for (int it = 0; it < iters; ++it) {
+        const bool go = tb > 0 && budget > 0;
+        const int s0 = go ? (tb - 1) / K * K : 0;
+        const int lo = go ? max(s0, tb - budget) : 0;
+        const int rest = go ? budget - (tb - lo) : 0;
+        if (go && role == 0) {
+            carve_scratch(R, buf ? scr1 : scr0, K, A.H);
+            wave_bwd_segment<T, IMAX, OMAX>(A, R, m, w, tp, r, lane, s0, lo, tb);
+        } else if (go && lo == s0 && s0 > 0 && rest > 0) {
+            carve_scratch(R, buf ? scr0 : scr1, K, A.H);
+            wave_recompute(A, R, m, w, tp, sh.sin_tab, lane, s0 - K, s0);
+        }
+        __syncthreads();
+        if (go) {
+            budget = rest;
+            tb = lo;
+            buf ^= 1;
+        }
+    }
+    if (work && role == 0) wave_bwd_end(R, m, lane);
 }
 
 // Fixed-order sum of per-robot gradient partials into per-design gradients.
@@ -1513,7 +1671,7 @@ static int check_rollout(const dl_rollout *r, bool bwd) {
     DL_CHECK_ARG(r->max_n > 0 && r->max_s >= 0 && r->max_i > 0 && r->max_o > 0,
                  "rollout: bad size maxima");
     DL_CHECK_ARG(r->lds_policy >= 0 && r->lds_policy <= 3, "rollout: bad LDS policy");
-    DL_CHECK_ARG(r->variant >= 0 && r->variant <= 2, "rollout: bad kernel variant");
+    DL_CHECK_ARG(r->variant >= 0 && r->variant <= 3, "rollout: bad kernel variant");
     DL_CHECK_ARG(r->lay.n_tv >= 0 && r->lay.n_tv <= 3 && r->lay.run_period > 0, "rollout: bad layout");
     DL_CHECK_ARG(r->loss_kind == 0 || (r->loss_kind == 1 && r->loss_window > 0),
                  "rollout: bad loss configuration");
@@ -1531,19 +1689,28 @@ static int check_rollout(const dl_rollout *r, bool bwd) {
 static const size_t kLdsLimit = 160 * 1024;
 
 template <typename T>
-static int launch_wave(const dl_rollout *r, RArgs<T> A, bool bwd, hipStream_t st) {
+static int launch_wave(const dl_rollout *r, RArgs<T> A, bool bwd, bool pipelined, hipStream_t st) {
     if constexpr (sizeof(T) != 4) {
         DL_CHECK_ARG(false, "rollout: wave kernel is FP32 only");
     } else {
         const int waves = r->block / 64;   // robots per workgroup (all of one design)
-        const size_t sh = wave_smem_bytes(r->max_n, r->max_s, r->max_i, r->max_o, bwd,
-                                          r->lay.run_period * r->lay.n_sin, waves);
-        const dim3 grid((r->n_robots + waves - 1) / waves), block(64 * waves);
+        const int n_tab = r->lay.run_period * r->lay.n_sin;
+        const bool pipe = bwd && pipelined;
+        const size_t sh = pipe ? wave_pipe_smem_bytes(r->max_n, r->max_s, r->max_i, r->max_o, n_tab,
+                                                      waves)
+                               : wave_smem_bytes(r->max_n, r->max_s, r->max_i, r->max_o, bwd, n_tab,
+                                                 waves);
+        const dim3 grid((r->n_robots + waves - 1) / waves), block((pipe ? 128 : 64) * waves);
 #define DL_WAVE_LAUNCH(IM)                                                                      \
     do {                                                                                        \
-        auto kf = bwd ? (const void *)k_wave_bwd<float, IM, 8> : (const void *)k_wave_fwd<float, IM, 8>; \
+        auto kf = pipe ? (const void *)k_wave_bwd_pipe<float, IM, 8>                            \
+                       : bwd ? (const void *)k_wave_bwd<float, IM, 8>                           \
+                             : (const void *)k_wave_fwd<float, IM, 8>;                          \
         DL_CHECK_HIP(ensure_dynamic_lds(kf, sh));                                               \
-        if (bwd)                                                                                \
+        if (pipe)                                                                               \
+            hipLaunchKernelGGL((k_wave_bwd_pipe<float, IM, 8>), grid, block, sh, st, A,         \
+                               r->n_robots);                                                    \
+        else if (bwd)                                                                           \
             hipLaunchKernelGGL((k_wave_bwd<float, IM, 8>), grid, block, sh, st, A, r->n_robots); \
         else                                                                                    \
             hipLaunchKernelGGL((k_wave_fwd<float, IM, 8>), grid, block, sh, st, A, r->n_robots); \
@@ -1584,9 +1751,17 @@ static int launch_rollout(const dl_rollout *r, bool bwd, hipStream_t st) {
                          (r->block == 64 || r->block == 256) &&
                          wave_smem_bytes(r->max_n, r->max_s, r->max_i, r->max_o, bwd, n_tab,
                                          r->block / 64) <= kLdsLimit;
-    DL_CHECK_ARG(variant != 2 || wave_ok,
+    DL_CHECK_ARG((variant != 2 && variant != 3) || wave_ok,
                  "rollout: the wave-per-robot kernel needs FP32, H=64, I<=192, O<=8, block 64 or 256");
-    if (variant == 2 || (variant == 0 && wave_ok)) return launch_wave<T>(r, A, bwd, st);
+    // pipelined adjoint: a second wavefront per robot (the scratch holds two
+    // segment buffers, see dl_scratch_size)
+    const bool pipe_ok = wave_ok && r->scratch != nullptr &&
+                         wave_pipe_smem_bytes(r->max_n, r->max_s, r->max_i, r->max_o, n_tab,
+                                              r->block / 64) <= kLdsLimit;
+    DL_CHECK_ARG(variant != 3 || pipe_ok,
+                 "rollout: the pipelined wave adjoint does not fit in LDS or has no scratch");
+    if (variant == 3 || (variant == 0 && pipe_ok)) return launch_wave<T>(r, A, bwd, true, st);
+    if (variant == 2 || (variant == 0 && wave_ok)) return launch_wave<T>(r, A, bwd, false, st);
     size_t sh = bytes(springs, topo, wl);
     DL_CHECK_ARG(sh <= kLdsLimit, "rollout: robot too large for LDS (%zu bytes)", sh);
     DL_CHECK_ARG(springs || r->scratch != nullptr,
@@ -1616,7 +1791,8 @@ int64_t dl_wgrad_size(int32_t I, int32_t H, int32_t O) {
 }
 
 int64_t dl_scratch_size(int32_t N, int32_t S, int32_t I, int32_t H, int32_t O, int32_t K) {
-    return (int64_t)K * (6 * (int64_t)N + I + 3 * (int64_t)H + 3 * (int64_t)O + 6) + 10 * (int64_t)S;
+    return 2 * (int64_t)K * (6 * (int64_t)N + I + 3 * (int64_t)H + 3 * (int64_t)O + 6) +
+           10 * (int64_t)S;
 }
 
 int32_t dl_ms_rollout_forward(const dl_rollout *r, void *stream) {

@@ -237,7 +237,7 @@ class MassSpringBatch:
         D.max_n, D.max_s, D.max_i, D.max_o = self.max_n, self.max_s, self.max_i, self.max_o
         D.lds_policy = self.lds_policy
         D.variant = self.variant
-        if self.variant == 2 or (self.variant == 0 and self.wave_eligible()):
+        if self.variant in (2, 3) or (self.variant == 0 and self.wave_eligible()):
             # one wavefront per robot; 4 per workgroup when each aligned group of
             # 4 robots shares one design (topology and W1 are staged per workgroup)
             D.block = 256 if self.packed4() else 64

@@ -583,7 +583,7 @@ def default_executors(physics, layout, slots_small=256, slots_large=128, mpm_act
     kernel up to 800 vertices / 7000 springs / 64 actuators / 1000 steps) and
     the MPM runner."""
     small = MSPool("wave", slots_small, (WAVE_MAX_VERT, 512, WAVE_MAX_IN, WAVE_MAX_OUT, 1000),
-                   physics, layout, ckpt_every, loss_window, dtype=dtype, variant=2, block=64,
+                   physics, layout, ckpt_every, loss_window, dtype=dtype, variant=3, block=64,
                    device=device)
     large = MSPool("block", slots_large, (800, 7000, n_inputs(layout, 800), 64, 1000), physics,
                    layout, ckpt_every, loss_window, dtype=dtype, variant=1, block=256,

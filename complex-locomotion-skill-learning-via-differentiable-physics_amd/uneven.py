@@ -67,7 +67,7 @@ class UnevenBatch:
                                 self.horizons[ids], physics, layout, targets[ids],
                                 [x0[r] for r in ids], n_hidden=n_hidden, act=act,
                                 ckpt_every=ckpt_every, dtype=dtype, loss_window=loss_window,
-                                device=device, variant=2 if name == "wave" else 1,
+                                device=device, variant=3 if name == "wave" else 1,
                                 block=64 if name == "wave" else 256)
             for j, r in enumerate(ids):
                 self.slot[r] = (len(self.buckets), j)
@@ -98,7 +98,7 @@ class UnevenBatch:
     @staticmethod
     def candidates(name):
         if name == "wave":
-            return [{"variant": 2, "block": 64, "lds_policy": 0}]
+            return [{"variant": v, "block": 64, "lds_policy": 0} for v in (3, 2)]
         return [{"variant": 1, "block": b, "lds_policy": p} for b in (64, 128, 256) for p in (0, 1)]
 
     def _apply(self, bk, cfg):

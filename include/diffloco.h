@@ -209,7 +209,9 @@ typedef struct dl_robot {
  * block: W1 [H][I], b1 [H], W2 [O][H], b2 [O] -- dl_wgrad_size() elements. */
 int64_t dl_weights_size(int32_t I, int32_t H, int32_t O);
 int64_t dl_wgrad_size(int32_t I, int32_t H, int32_t O);
-/* Elements of per-robot scratch for checkpoint interval K. */
+/* Elements of per-robot scratch for checkpoint interval K: two buffers of K
+ * steps' recompute rows (the pipelined adjoint fills one while it reverses
+ * the other) and a spring spill area. */
 int64_t dl_scratch_size(int32_t N, int32_t S, int32_t I, int32_t H, int32_t O, int32_t K);
 
 typedef struct dl_rollout {
@@ -225,7 +227,9 @@ typedef struct dl_rollout {
     int32_t max_n, max_s, max_i, max_o;   /* maxima over the table (LDS sizing) */
     int32_t lds_policy;      /* 0 auto, 1 stage topology, 2 topology + W1, 3 none */
     int32_t variant;         /* 0 auto, 1 workgroup per robot, 2 wavefront per robot (FP32,
-                                H = 64, I <= 192, O <= 8; robots of one design packed by 4) */
+                                H = 64, I <= 192, O <= 8; robots of one design packed by 4),
+                                3 as 2 with the adjoint's segment recompute on a second
+                                wavefront per robot, pipelined over two scratch buffers */
     dl_ms_physics phys;
     dl_sensor_layout lay;
     const dl_robot *robots;  /* device [n_robots] */

@@ -315,10 +315,11 @@ def test_batched_deterministic():
     assert torch.equal(outs[0][1], outs[1][1])
 
 
-@pytest.mark.parametrize("K", [16, 7])
-def test_wave_kernel_matches_oracle(K):
-    """Wave-per-robot kernel (FP32, W1 column in VGPRs) against the FP64 oracle."""
-    _say(f"wave quad K={K}")
+@pytest.mark.parametrize("K,variant", [(16, 2), (7, 2), (16, 3), (7, 3)])
+def test_wave_kernel_matches_oracle(K, variant):
+    """Wave-per-robot kernel (FP32, W1 column in VGPRs) against the FP64 oracle;
+    variant 3 runs the adjoint's segment recompute on a second wavefront."""
+    _say(f"wave quad K={K} variant={variant}")
     B, T, W = 6, 90, 40
     tv, th = task_targets(B, 21)
     design = dl.build_quadruped()
@@ -328,7 +329,8 @@ def test_wave_kernel_matches_oracle(K):
     x0 = design.objects + np.array([0.4, 0.0, 0.0])
     tg = np.concatenate([tv, th[:, None], np.zeros((B, 1))], 1)
     batch = MassSpringBatch([design], [0] * B, T, _physics(qc), sensor_layout(qc), tg, [x0] * B,
-                            act="tanh", ckpt_every=K, dtype=torch.float32, loss_window=W, variant=2)
+                            act="tanh", ckpt_every=K, dtype=torch.float32, loss_window=W,
+                            variant=variant)
     wd = [w.to("cuda", torch.float32).requires_grad_() for w in w64]
     lossv = rollout_loss(batch, [wd])
     lossv.mean().backward()
@@ -338,6 +340,49 @@ def test_wave_kernel_matches_oracle(K):
     assert rel_err(lossv.detach().cpu(), lb) < 1e-4
     for a, b in zip(wd, go):
         assert rel_err(a.grad.cpu(), b) < 1e-3
+
+
+@pytest.mark.parametrize("budget", [0, 23])
+def test_pipelined_adjoint_bit_identical_to_sequential(budget):
+    """The adjoint with the segment recompute on a second wavefront per robot
+    (variant 3) gives the sequential wave adjoint's (variant 2) gradients and
+    adjoint state bit for bit: one design, horizons 45..90 mixed inside a
+    workgroup (2 to 6 segments of K = 16), a partly filled second workgroup
+    (6 robots, 4 per workgroup), whole and step-budgeted backward calls."""
+    _say(f"pipelined vs sequential adjoint budget={budget}")
+    horizons = [90, 61, 77, 90, 45, 83]
+    B, K, W = len(horizons), 16, 40
+    tv, th = task_targets(B, 23)
+    design = dl.build_quadruped()
+    I = om.TaskConfig().layout.n_input(design.n_vert)
+    w64 = [w.to("cuda", torch.float32) for w in om.init_weights(I, 64, 4, seed=24)]
+    qc = dl.quadruped_config(batch_size=B, steps=max(horizons), window=W).get_config()
+    x0 = design.objects + np.array([0.4, 0.0, 0.0])
+    tg = np.concatenate([tv, th[:, None], np.zeros((B, 1))], 1)
+    out = []
+    for variant in (2, 3):
+        batch = MassSpringBatch([design], [0] * B, horizons, _physics(qc), sensor_layout(qc), tg,
+                                [x0] * B, act="tanh", ckpt_every=K, dtype=torch.float32,
+                                loss_window=W, variant=variant)
+        assert batch.packed4()
+        batch.set_weights(0, *w64)
+        batch.forward()
+        if budget:
+            calls = 0
+            while batch.table["t_bwd"].max() > 0:
+                batch.backward(loss_grad=np.linspace(0.5, 1.5, B), max_steps=budget, reduce=False)
+                calls += 1
+            assert calls == -(-max(horizons) // budget)
+            batch.reduce_grads()
+        else:
+            batch.backward(loss_grad=np.linspace(0.5, 1.5, B))
+        torch.cuda.synchronize()
+        out.append(([g.clone() for g in batch.weight_grads(0)], batch.adjoint.clone()))
+    (ga, aa), (gb, ab) = out
+    assert torch.isfinite(aa).all()
+    for x, y in zip(ga, gb):
+        assert torch.equal(x, y)
+    assert torch.equal(aa, ab)
 
 
 def _uneven(prec, variant, seed=31):
