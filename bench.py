@@ -152,8 +152,11 @@ def main():
     fwd_avg_ms = float(np.mean(fwd_ms))
     achieved_bwd = b_b * robot_steps / (bwd_avg_ms * 1e-3) / 1e9
     achieved_fwd = f_b * robot_steps / (fwd_avg_ms * 1e-3) / 1e9
-    wave = batch.variant == 2 or (batch.variant == 0 and batch.wave_eligible())
+    # which kernels the launches used: variant 0 picks the wave kernels with the
+    # pipelined adjoint when the robot fits them (as here), else the block kernels
+    wave = batch.variant in (2, 3) or (batch.variant == 0 and batch.wave_eligible())
     kname = "k_wave" if wave else "k_rollout"
+    kname_bwd = kname + ("_bwd_pipe" if wave and batch.variant != 2 else "_bwd")
     loss = float(batch.loss.double().mean().item())
     gnorm = float(batch.grad_out.double().norm().item())
 
@@ -344,7 +347,7 @@ def main():
                    "form": "batched persistent (one launch forward, one launch adjoint)"},
         "roofline": {"bound": "hbm", "achieved": achieved_bwd, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_bwd / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": kname + "_bwd", "algorithmic_bytes_per_robot_step": b_b,
+                     "kernel": kname_bwd, "algorithmic_bytes_per_robot_step": b_b,
                      "avg_launch_ms": bwd_avg_ms,
                      "forward": {"kernel": kname + "_fwd", "achieved": achieved_fwd,
                                  "avg_launch_ms": fwd_avg_ms,
