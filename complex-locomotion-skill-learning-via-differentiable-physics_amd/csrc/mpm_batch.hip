@@ -733,8 +733,11 @@ __global__ __launch_bounds__(PB) void k_p2g_b(MP<T> p, PView<D, T> pv, const T *
 // ---------------------------------------------------------------- grid
 // One step's saved grid (segment step j): records by active-list position.
 template <typename T> struct GridSave {
-    T *g, *v;              // [B][nblk][64][4] grid totals, velocities
-    int32_t *list, *count; // [B][nblk], [B]
+    T *g, *v;              // [B][cap][64][4] grid totals, velocities
+    int32_t *list, *count; // [B][cap], [B]
+    int cap;               // records per robot (blocks by active-list position)
+    int32_t *over;         // set when a robot has more active blocks than cap (then
+                           // nothing of that robot is saved); may be null
 };
 
 // Velocity of every node of the active blocks; with `clear`, the momentum /
@@ -745,9 +748,15 @@ __global__ __launch_bounds__(PB) void k_grid_b(MP<T> p, T *grid, T *gvo, Blk bk,
                                                TileBuf<T> tb, GridSave<T> sv) {
     const int k = blockIdx.y, lane = threadIdx.x & 63;
     const TileBuf<T> tbk = tb.robot(k, bk.nblk);
-    if (sv.g && blockIdx.x == 0 && threadIdx.x == 0) sv.count[k] = bk.count[k];
     const int nw = gridDim.x * NWAVE;
     const int cnt = bk.count[k];
+    const bool save = sv.g && cnt <= sv.cap;
+    if (sv.g && blockIdx.x == 0 && threadIdx.x == 0) {
+        if (save)
+            sv.count[k] = cnt;
+        else if (sv.over)
+            atomicOr(sv.over, 1);
+    }
     const int32_t *lst = bk.list + (size_t)k * bk.nblk;
     const size_t G = gsize<D>(p.n);
     T *gk = grid + (size_t)k * G * 4, *ok = gvo + (size_t)k * G * 4;
@@ -767,11 +776,11 @@ __global__ __launch_bounds__(PB) void k_grid_b(MP<T> p, T *grid, T *gvo, Blk bk,
             o[d] = bnd ? T(0) : vo;
         }
         st4(ok + (size_t)node * 4, o);
-        if (sv.g) {
-            const size_t si = (((size_t)k * bk.nblk + e) * 64 + lane) * 4;
+        if (save) {
+            const size_t si = (((size_t)k * sv.cap + e) * 64 + lane) * 4;
             st4(sv.g + si, r);
             st4(sv.v + si, o);
-            if (lane == 0) sv.list[(size_t)k * bk.nblk + e] = bid;
+            if (lane == 0) sv.list[(size_t)k * sv.cap + e] = bid;
         }
         if (clear) {
             const T z[4] = {T(0), T(0), T(0), T(0)};
@@ -787,29 +796,53 @@ __global__ __launch_bounds__(PB) void k_grid_b(MP<T> p, T *grid, T *gvo, Blk bk,
 }
 
 // Restore a saved step: grid totals and velocities of its active blocks,
-// and the active list itself, as the P2G + grid passes left them.
+// and the active list itself, as the P2G + grid passes left them.  With
+// vo_only only the velocities are restored (the replayed recompute's G2P
+// reads nothing else); otherwise also the totals, the active list and
+// count, which the adjoint kernels consume and clear.
 template <int D, typename T>
 __global__ __launch_bounds__(PB) void k_grid_restore(MP<T> p, T *grid, T *gvo, Blk bk,
-                                                     GridSave<T> sv) {
+                                                     GridSave<T> sv, int vo_only) {
     const int k = blockIdx.y, lane = threadIdx.x & 63;
     const int nw = gridDim.x * NWAVE;
     const int cnt = sv.count[k];
     const size_t G = gsize<D>(p.n);
     T *gk = grid + (size_t)k * G * 4, *ok = gvo + (size_t)k * G * 4;
-    if (blockIdx.x == 0 && threadIdx.x == 0) bk.count[k] = cnt;
+    if (!vo_only && blockIdx.x == 0 && threadIdx.x == 0) bk.count[k] = cnt;
     for (int e = blockIdx.x * NWAVE + (threadIdx.x >> 6); e < cnt; e += nw) {
-        const int bid = sv.list[(size_t)k * bk.nblk + e];
+        const int bid = sv.list[(size_t)k * sv.cap + e];
         bool bnd;
         int c[D];
         const int node = block_node<D, T>(bid, lane, bk.nb, p, bnd, c);
-        const size_t si = (((size_t)k * bk.nblk + e) * 64 + lane) * 4;
-        T r[4], o[4];
-        ld4(sv.g + si, r);
+        const size_t si = (((size_t)k * sv.cap + e) * 64 + lane) * 4;
+        T o[4];
         ld4(sv.v + si, o);
-        st4(gk + (size_t)node * 4, r);
         st4(ok + (size_t)node * 4, o);
-        if (lane == 0) bk.list[(size_t)k * bk.nblk + e] = bid;
+        if (!vo_only) {
+            T r[4];
+            ld4(sv.g + si, r);
+            st4(gk + (size_t)node * 4, r);
+            if (lane == 0) bk.list[(size_t)k * bk.nblk + e] = bid;
+        }
     }
+}
+
+// F of the next state for every particle, (I + dt C) F: the part of P2G that
+// a replayed recompute (saved grid velocities + G2P) still needs.
+template <int D, typename T>
+__global__ __launch_bounds__(PB) void k_fupdate(MP<T> p, int P, const T *C, const T *F, T *F_out) {
+    const int k = blockIdx.y, q = blockIdx.x * PB + threadIdx.x;
+    if (q >= P) return;
+    const size_t bm = (size_t)k * D * D * P;
+    Mat<D, T> Cm, Fm;
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            Cm.a[i][j] = C[bm + (size_t)(i * D + j) * P + q];
+            Fm.a[i][j] = F[bm + (size_t)(i * D + j) * P + q];
+        }
+    store_mat<D, T>(F_out, k, q, P, f_next<D, T>(Cm, Fm, p), false);
 }
 
 // ---------------------------------------------------------------- G2P
@@ -1383,6 +1416,17 @@ int check_batch(const dl_mpm_batch *b) {
     return 0;
 }
 
+int check_horizon_save(const dl_mpm_batch *b) {
+    const int E = b->params.dim == 3 ? 4 : 8;
+    int nblk = 1;
+    for (int d = 0; d < b->params.dim; ++d) nblk *= b->params.n_grid / E;
+    DL_CHECK_ARG(b->hgrid && b->hvo && b->hlist && b->hcount && b->hover && b->hcap > 0 &&
+                     b->hcap <= nblk,
+                 "MPM batch: the horizon grid save needs hgrid, hvo, hlist, hcount, hover and "
+                 "0 < hcap <= %d", nblk);
+    return 0;
+}
+
 int check_control(const dl_mpm_batch *b) {
     DL_CHECK_ARG(b->params.dim == 3, "MPM controller: 3D robots only");
     DL_CHECK_ARG(b->n_groups > 0 && b->n_sin >= 0 && b->run_period > 0 && b->n_hidden > 0 &&
@@ -1425,11 +1469,18 @@ template <int D, typename T> Slot<D, T> slot(const dl_mpm_slots *s, int i, int B
 }
 
 template <int D, typename T> GridSave<T> save_at(const dl_mpm_batch *b, int j) {
-    if (!b->grid_save) return {nullptr, nullptr, nullptr, nullptr};
+    if (!b->grid_save) return {nullptr, nullptr, nullptr, nullptr, 0, nullptr};
     const Blk bk = make_blk<D>(b);
     const size_t rec = (size_t)b->B * bk.nblk;
     return {(T *)b->grid_save + j * rec * 64 * 4, (T *)b->vo_save + j * rec * 64 * 4,
-            b->list_save + j * rec, b->count_save + (size_t)j * b->B};
+            b->list_save + j * rec, b->count_save + (size_t)j * b->B, bk.nblk, nullptr};
+}
+
+// Horizon save of step t (hcap blocks per robot), overflow flag of step t.
+template <typename T> GridSave<T> hsave_at(const dl_mpm_batch *b, int t) {
+    const size_t rec = (size_t)b->B * b->hcap;
+    return {(T *)b->hgrid + t * rec * 64 * 4, (T *)b->hvo + t * rec * 64 * 4,
+            b->hlist + t * rec, b->hcount + (size_t)t * b->B, b->hcap, b->hover + t};
 }
 
 int n_windows(const dl_mpm_batch *b) {
@@ -1469,6 +1520,15 @@ int forward_impl(const dl_mpm_batch *b, const dl_mpm_slots *st, int t0, int n, i
         const int t = t0 + j;
         const Slot<D, T> a = slot<D, T>(st, j % st->n_slots, B, P);
         const Slot<D, T> c = slot<D, T>(st, (j + 1) % st->n_slots, B, P);
+        if (flags & DL_MPM_REPLAY) {
+            hipLaunchKernelGGL((k_grid_restore<D, T>), gg, dim3(PB), 0, s, mp, (T *)b->grid,
+                               (T *)b->grid_vo, bk, hsave_at<T>(b, t), 1);
+            hipLaunchKernelGGL((k_fupdate<D, T>), pg, dim3(PB), 0, s, mp, P, a.C, a.F, c.F);
+            hipLaunchKernelGGL((k_g2p_b<D, T, false>), pg, dim3(PB), 0, s, mp, P, a.x,
+                               (const T *)b->grid_vo, c.x, c.v, c.C, b->blk_count,
+                               SenseOut<T>{nullptr, nullptr, 0});
+            continue;
+        }
         if (control)
             hipLaunchKernelGGL(k_ctrl<T>, dim3(B), dim3(64), (I + H) * sizeof(T), s,
                                make_ctrl<T>(b), t, (const T *)b->sens + (size_t)t * B * NS,
@@ -1480,8 +1540,9 @@ int forward_impl(const dl_mpm_batch *b, const dl_mpm_slots *st, int t0, int n, i
                                                   tile_bytes, s, mp, pv, act + (size_t)t * B * O,
                                                   b->act_id, O, c.F, (T *)b->grid, bk, b->err,
                                                   tb, b->windows));
-        const GridSave<T> sv = (flags & DL_MPM_SAVE_GRID) ? save_at<D, T>(b, j)
-                                                           : GridSave<T>{nullptr, nullptr, nullptr, nullptr};
+        const GridSave<T> sv = (flags & DL_MPM_SAVE_GRID)      ? save_at<D, T>(b, j)
+                               : (flags & DL_MPM_SAVE_HORIZON) ? hsave_at<T>(b, t)
+                                                               : GridSave<T>{nullptr, nullptr, nullptr, nullptr, 0, nullptr};
         hipLaunchKernelGGL((k_grid_b<D, T>), gg, dim3(PB), 0, s, mp, (T *)b->grid, (T *)b->grid_vo,
                            bk, 1, tb, sv);
         if (control && D == 3)
@@ -1527,9 +1588,10 @@ int backward_impl(const dl_mpm_batch *b, const dl_mpm_slots *seg, const dl_mpm_s
         const Slot<D, T> a = slot<D, T>(seg, j, B, P);
         const Slot<D, T> A1 = slot<D, T>(adj, cur, B, P), A0 = slot<D, T>(adj, nxt, B, P);
         PView<D, T> pv{a.x, a.v, a.C, a.F, P};
-        if (b->grid_save) {
+        if (b->restore_from == 1 || b->grid_save) {
             hipLaunchKernelGGL((k_grid_restore<D, T>), gg, dim3(PB), 0, s, mp, (T *)b->grid,
-                               (T *)b->grid_vo, bk, save_at<D, T>(b, j));
+                               (T *)b->grid_vo, bk,
+                               b->restore_from == 1 ? hsave_at<T>(b, t) : save_at<D, T>(b, j), 0);
         } else {
             DL_UNROLL_DISPATCH(U_, hipLaunchKernelGGL((k_p2g_b<D, T, false, U_>), pw, dim3(PB),
                                                       tile_bytes, s, mp, pv,
@@ -1538,7 +1600,7 @@ int backward_impl(const dl_mpm_batch *b, const dl_mpm_slots *seg, const dl_mpm_s
                                                       b->windows));
             hipLaunchKernelGGL((k_grid_b<D, T>), gg, dim3(PB), 0, s, mp, (T *)b->grid,
                                (T *)b->grid_vo, bk, 0, tb,
-                               GridSave<T>{nullptr, nullptr, nullptr, nullptr});
+                               GridSave<T>{nullptr, nullptr, nullptr, nullptr, 0, nullptr});
         }
         SenseAdj<T> sa{};
         if (D == 3 && b->in_grad) {
@@ -1614,6 +1676,14 @@ int32_t dl_mpm_forward(const dl_mpm_batch *b, const dl_mpm_slots *st, int32_t t0
         DL_CHECK_ARG(b->grid_save && b->vo_save && b->list_save && b->count_save &&
                          n <= b->save_steps,
                      "dl_mpm_forward: DL_MPM_SAVE_GRID needs the save buffers for %d steps", n);
+    if (flags & (DL_MPM_SAVE_HORIZON | DL_MPM_REPLAY))
+        if (int e = check_horizon_save(b)) return e;
+    DL_CHECK_ARG(!((flags & DL_MPM_SAVE_GRID) && (flags & DL_MPM_SAVE_HORIZON)),
+                 "dl_mpm_forward: DL_MPM_SAVE_GRID and DL_MPM_SAVE_HORIZON exclude each other");
+    DL_CHECK_ARG(!(flags & DL_MPM_REPLAY) ||
+                     !(flags & (DL_MPM_CONTROL | DL_MPM_SENSE_FIRST | DL_MPM_SAVE_GRID |
+                                DL_MPM_SAVE_HORIZON)),
+                 "dl_mpm_forward: DL_MPM_REPLAY recomputes only (no controller, no saving)");
     MPMB_DISPATCH(b, forward_impl, b, st, t0, n, flags, (hipStream_t)stream);
 }
 
@@ -1629,6 +1699,10 @@ int32_t dl_mpm_backward(const dl_mpm_batch *b, const dl_mpm_slots *seg, const dl
     DL_CHECK_ARG(!b->grid_save || (b->vo_save && b->list_save && b->count_save &&
                                    n <= b->save_steps),
                  "dl_mpm_backward: saved grids cover %d steps, segment has %d", b->save_steps, n);
+    DL_CHECK_ARG(b->restore_from == 0 || b->restore_from == 1,
+                 "dl_mpm_backward: restore_from must be 0 or 1");
+    if (b->restore_from == 1)
+        if (int e = check_horizon_save(b)) return e;
     if (b->in_grad) {
         if (int e = check_control(b)) return e;
         DL_CHECK_ARG(b->center_grad && b->vcom_grad && b->wgrad && b->hid_grad && b->out_grad,

@@ -288,6 +288,18 @@ __device__ __forceinline__ void column(const Mat<D, T> &A, int e, T col[D]) {
     }
 }
 
+// Deformation gradient of the next state: (I + dt C) F (solver_mpm.py:89, 95).
+template <int D, typename T>
+__device__ __forceinline__ Mat<D, T> f_next(const Mat<D, T> &Cm, const Mat<D, T> &Fm,
+                                            const MP<T> &p) {
+    Mat<D, T> Mx;
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) Mx.a[i][j] = (i == j ? T(1) : T(0)) + p.dt * Cm.a[i][j];
+    return mul(Mx, Fm);
+}
+
 // Forward constitutive quantities of one particle.
 template <int D, typename T>
 struct Consti {
@@ -299,12 +311,7 @@ template <int D, typename T>
 __device__ __forceinline__ Consti<D, T> constitutive(const Mat<D, T> &Cm, const Mat<D, T> &Fm,
                                                      T a, const MP<T> &p) {
     Consti<D, T> q;
-    Mat<D, T> Mx;
-#pragma unroll
-    for (int i = 0; i < D; ++i)
-#pragma unroll
-        for (int j = 0; j < D; ++j) Mx.a[i][j] = (i == j ? T(1) : T(0)) + p.dt * Cm.a[i][j];
-    q.Fn = mul(Mx, Fm);
+    q.Fn = f_next<D, T>(Cm, Fm, p);
     q.J = det(q.Fn);
     q.R = polar_R(q.Fn);
     q.a = a;

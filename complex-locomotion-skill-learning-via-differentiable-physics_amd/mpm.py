@@ -236,7 +236,9 @@ class MPMBatchC(Structure):
                     "tile_hdr", "blk_tile_count", "blk_tiles", "grid_save", "vo_save",
                     "list_save", "count_save")] + [("save_steps", c_int32), ("scatter_unroll", c_int32)] +
                 [(n, ctypes.c_void_p) for n in ("hid_grad", "out_grad", "windows")] +
-                [("n_windows", c_int32), ("pad_", c_int32)])
+                [("n_windows", c_int32), ("pad_", c_int32)] +
+                [(n, ctypes.c_void_p) for n in ("hgrid", "hvo", "hlist", "hcount", "hover")] +
+                [("hcap", c_int32), ("restore_from", c_int32)])
 
 
 class MPMSlotsC(Structure):
@@ -245,7 +247,7 @@ class MPMSlotsC(Structure):
                 ("F", ctypes.c_void_p), ("n_slots", c_int32), ("pad_", c_int32)]
 
 
-MPM_CONTROL, MPM_SENSE_FIRST, MPM_SAVE_GRID = 1, 2, 4
+MPM_CONTROL, MPM_SENSE_FIRST, MPM_SAVE_GRID, MPM_SAVE_HORIZON, MPM_REPLAY = 1, 2, 4, 8, 16
 MPM_PB, MPM_TILE_MAX, MPM_TILE_CAP = 256, 8192, 32
 SCATTER_ATOMIC, SCATTER_STORE = 0, 1
 
@@ -319,13 +321,19 @@ class MPMBatch:
     2048 nodes).  With ``resort`` each robot's particles are re-sorted by
     block at every checkpoint (the state, actuator ids and sensor groups
     permuted together), so the scatter keeps compact cells as the robots
-    move; the adjoint is permuted back at each checkpoint."""
+    move; the adjoint is permuted back at each checkpoint.  With
+    ``save_horizon`` the forward also saves every step's active grid blocks
+    (at most ``horizon_blocks`` per robot and step; default three times the
+    rest shape's blocks), so the backward recomputes a segment from its
+    checkpoint by restoring the saved grid velocities and running only the
+    update of F and G2P; a segment where some step did not fit is recomputed
+    in full."""
 
     def __init__(self, params, positions, act_id, n_act, B, T, K=100, window=None,
                  dtype=torch.float32, n_sin=10, run_period=100, offset_scale=0.2,
                  max_speed=0.08, activation="tanh", n_hidden=64, device="cuda",
                  sort_particles=True, tile_nodes=None, scatter=SCATTER_ATOMIC, save_grid=True,
-                 unroll=3, box_limit=512, resort=True):
+                 unroll=3, box_limit=512, resort=True, save_horizon=True, horizon_blocks=None):
         if params.dim != 3:
             raise ValueError("MPMBatch drives 3D robots (the controller's sensors are 3D)")
         if params.n_grid % 4:
@@ -433,7 +441,29 @@ class MPMBatch:
             save_steps=K if save_grid else 0, scatter_unroll=unroll,
             hid_grad=P_(self.hid_grad), out_grad=P_(self.out_grad),
             windows=P_(self.windows) if win is not None else None, n_windows=wpr)
+        self.save_horizon = save_horizon
+        if save_horizon:
+            if horizon_blocks is None:
+                horizon_blocks = max(64, 3 * self.rest_blocks())
+            hcap = int(min(nblk, max(1, horizon_blocks)))
+            self.hcap = hcap
+            self.hgrid, self.hvo = emp(T, B * hcap * 64 * 4), emp(T, B * hcap * 64 * 4)
+            self.hlist = emp(T, B * hcap, dt=torch.int32)
+            self.hcount = emp(T, B, dt=torch.int32)
+            self.hover = zi(T)
+            self.desc.hgrid, self.desc.hvo = P_(self.hgrid), P_(self.hvo)
+            self.desc.hlist, self.desc.hcount = P_(self.hlist), P_(self.hcount)
+            self.desc.hover, self.desc.hcap = P_(self.hover), hcap
+        self.replay = [False] * self.nck
         self.grads = None
+
+    def rest_blocks(self):
+        """Grid blocks (4^3 nodes) under some particle's stencil at rest."""
+        base = np.floor(self.positions * self.p.n_grid - 0.5).astype(np.int64)
+        offs = np.stack(np.meshgrid(*[np.arange(3)] * 3, indexing="ij"), -1).reshape(-1, 3)
+        blk = ((base[:, None, :] + offs[None]) // 4).reshape(-1, 3)
+        nb = self.p.n_grid // 4
+        return len(np.unique((blk[:, 0] * nb + blk[:, 1]) * nb + blk[:, 2]))
 
     def set_targets(self, tv, th):
         t = np.zeros((self.B, 5))
@@ -485,6 +515,8 @@ class MPMBatch:
         for i in range(3):
             r.F[0][:, i * 3 + i] = 1.0
         self.sens.zero_()
+        if self.save_horizon:
+            self.hover.zero_()
 
     def forward_segment(self, s):
         r = self.ring
@@ -497,7 +529,8 @@ class MPMBatch:
             torch.gather(self.orders[s - 1], 1, pi, out=self.orders[s])
             self._set_order(s)
         self.ck.copy_slot(s, 0, r)
-        flags = MPM_CONTROL | (MPM_SENSE_FIRST if s == 0 else 0)
+        flags = MPM_CONTROL | (MPM_SENSE_FIRST if s == 0 else 0) | \
+            (MPM_SAVE_HORIZON if self.save_horizon else 0)
         nat.call("dl_mpm_forward", ctypes.byref(self.desc), ctypes.byref(r.c), t0, n, flags,
                  nat.stream())
         if n % 2:
@@ -516,6 +549,10 @@ class MPMBatch:
         return self.finish_forward()
 
     def start_backward(self, loss_grad=1.0):
+        if self.save_horizon:
+            # segments whose every step's grid was saved are replayed
+            over = self.hover.cpu().numpy()
+            self.replay = [not over[t0:t0 + n].any() for t0, n in self._segments()]
         self.cg.zero_()
         self.vg.zero_()
         self.in_grad.zero_()
@@ -532,10 +569,16 @@ class MPMBatch:
         if self.resort:
             self._set_order(s)
         self.seg.copy_slot(0, s, self.ck)
-        nat.call("dl_mpm_forward", ctypes.byref(self.desc), ctypes.byref(self.seg.c), t0, n,
-                 MPM_SAVE_GRID if self.save_grid else 0, nat.stream())
+        if self.replay[s]:
+            self.desc.restore_from = 1
+            nat.call("dl_mpm_forward", ctypes.byref(self.desc), ctypes.byref(self.seg.c), t0, n,
+                     MPM_REPLAY, nat.stream())
+        else:
+            nat.call("dl_mpm_forward", ctypes.byref(self.desc), ctypes.byref(self.seg.c), t0, n,
+                     MPM_SAVE_GRID if self.save_grid else 0, nat.stream())
         nat.call("dl_mpm_backward", ctypes.byref(self.desc), ctypes.byref(self.seg.c),
                  ctypes.byref(self.adj.c), t0, n, nat.stream())
+        self.desc.restore_from = 0
         if self.resort and s > 0:
             # adjoint of state t0 back to the previous segment's order
             pi = self.perms[s]

@@ -338,6 +338,7 @@ class MPMRunner:
         st.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(st):
             b = MPMBatch(m["params"], m["positions"], m["act_id"], m["n_act"], 1, req.horizon,
+                         save_horizon=False,
                          K=self.K, window=min(100, req.horizon), dtype=self.dtype,
                          device=self.device)
             b.set_targets(np.asarray(req.target)[None, :3], np.asarray(req.target)[3:4])

@@ -347,7 +347,8 @@ int32_t dl_mpm_sensor_adj(int32_t dtype, int32_t B, int32_t P, int32_t n_groups,
  * in_grad [T+1][B][I] (row T zero), center_grad/vcom_grad [T+1][B][3] (loss
  * adjoints, e.g. from dl_loss_task_grad), wgrad [B][H*I + H + O*H + O]
  * (per-robot slices, accumulated).  I = n_sin + 3 + 3*n_groups + 4. */
-enum { DL_MPM_CONTROL = 1, DL_MPM_SENSE_FIRST = 2, DL_MPM_SAVE_GRID = 4 };
+enum { DL_MPM_CONTROL = 1, DL_MPM_SENSE_FIRST = 2, DL_MPM_SAVE_GRID = 4,
+       DL_MPM_SAVE_HORIZON = 8, DL_MPM_REPLAY = 16 };
 /* Scatter tiles: particles are processed DL_MPM_PB per workgroup
  * (W = ceil(P / DL_MPM_PB) workgroups per robot), each accumulating its
  * stencil box in an LDS tile of tile_nodes nodes (a box larger than that is
@@ -408,6 +409,19 @@ typedef struct dl_mpm_batch {
      * one tile per window.  A window wider than DL_MPM_PB sets err bit 2. */
     const int32_t *windows;
     int32_t n_windows, pad_;
+    /* Horizon grid save (optional): dl_mpm_forward with DL_MPM_SAVE_HORIZON
+     * stores every step's active blocks, at most hcap per robot, in
+     * hgrid / hvo [T][B][hcap][64][4] (totals, velocities), hlist [T][B][hcap]
+     * and hcount [T][B]; hover [T] is set (nonzero) for a step where some robot
+     * had more active blocks (that robot's step is not saved).  A segment
+     * whose steps are all saved may be recomputed with DL_MPM_REPLAY: per step
+     * the saved velocities are restored and only the update of F and G2P run
+     * (no P2G, no grid update; x, v, C come out as in the forward).
+     * dl_mpm_backward with restore_from = 1 restores each step's grid from
+     * this save (0: from grid_save, or by P2G when that is NULL). */
+    void *hgrid, *hvo;
+    int32_t *hlist, *hcount, *hover;
+    int32_t hcap, restore_from;
 } dl_mpm_batch;
 
 /* Ring of particle-state slots: slot s of x/v at x + s*B*D*P reals, of C/F
@@ -421,7 +435,10 @@ typedef struct dl_mpm_slots {
  * DL_MPM_CONTROL: run the controller (center/vcom/inp/hid/act_out rows t,
  * sensor sums of row t+1; center/vcom of row T when t0+n == T); without it
  * act_out rows are read (segment recompute).  DL_MPM_SENSE_FIRST: first
- * accumulate the sensor sums of state t0 (row t0). */
+ * accumulate the sensor sums of state t0 (row t0).  DL_MPM_SAVE_GRID /
+ * DL_MPM_SAVE_HORIZON: save each step's grid (segment slots / horizon rows;
+ * not both).  DL_MPM_REPLAY: recompute from the horizon save (no
+ * DL_MPM_CONTROL, no saving). */
 int32_t dl_mpm_forward(const dl_mpm_batch *b, const dl_mpm_slots *states, int32_t t0,
                        int32_t n, int32_t flags, void *stream);
 /* Adjoint of steps t0 .. t0+n-1 given the segment's states (n+1 slots) and

@@ -195,6 +195,31 @@ def test_mpm_batched_rollout_matches_oracle(prec, K, scatter, tile, save, unroll
         assert rel_err(a.double().cpu(), b) < tg
 
 
+@pytest.mark.parametrize("hblocks", [None, 1])
+def test_mpm_batched_replay_matches_full_recompute(hblocks):
+    """Segments recomputed from the horizon grid save (saved grid velocities
+    restored, F updated, G2P) give the loss and gradients of the full
+    recompute (P2G, grid update, G2P) and of the oracle.  With a one-block
+    capacity no step fits the save and every segment falls back to the full
+    recompute."""
+    _say(f"mpm batched replay hblocks={hblocks}")
+    params, op, pos, ids = _batched_case()
+    runs = []
+    for save in (False, True):
+        batch, loss, grads, lo, gref = _run_batched(params, op, pos, ids, 2, 24, 7, 12,
+                                                    torch.float64, save_horizon=save,
+                                                    horizon_blocks=hblocks if save else None)
+        assert abs(loss - lo) / abs(lo) < 1e-10
+        for a, b in zip(grads, gref):
+            assert rel_err(a.double().cpu(), b) < 1e-7
+        if save:
+            assert batch.replay == [hblocks is None] * batch.nck
+        runs.append((loss, grads))
+    assert abs(runs[0][0] - runs[1][0]) / abs(runs[0][0]) < 1e-12
+    for a, b in zip(runs[0][1], runs[1][1]):
+        assert rel_err(a.double().cpu(), b.double().cpu()) < 1e-10
+
+
 def test_mpm_batched_resort_matches_fixed_order():
     """Shuffled particles re-sorted by block at every checkpoint (state,
     actuator ids and groups permuted; the adjoint permuted back) give the
