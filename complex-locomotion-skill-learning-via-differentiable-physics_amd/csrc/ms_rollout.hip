@@ -1456,11 +1456,11 @@ __device__ __forceinline__ void wave_recompute(const RArgs<T> &A, const Rob<T> &
 }
 
 // Adjoint steps tb - 1 down to lo of the segment starting at s0 (rows of R
-// recomputed), then the segment's weight-gradient fold.
+// recomputed); the steps' weight-gradient rows are left in R's scratch.
 template <typename T, int IMAX, int OMAX>
-__device__ __forceinline__ void wave_bwd_segment(const RArgs<T> &A, const Rob<T> &R, WSm<T> &m,
-                                                 const WRegs<T, 1, OMAX> &w, const LaneTopo<T> &tp,
-                                                 int r, int lane, int s0, int lo, int tb) {
+__device__ __forceinline__ void wave_bwd_steps(const RArgs<T> &A, const Rob<T> &R, WSm<T> &m,
+                                               const WRegs<T, 1, OMAX> &w, const LaneTopo<T> &tp,
+                                               int r, int lane, int s0, int lo, int tb) {
     Pref<T> pf;
     pref_load(R, A.H, lane, tb - 1 - s0, pf);
     for (int t = tb - 1; t >= lo; --t)
@@ -1468,6 +1468,15 @@ __device__ __forceinline__ void wave_bwd_segment(const RArgs<T> &A, const Rob<T>
                                                t - 1 >= lo ? t - 1 - s0 : -1);
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
     wave_sync();
+}
+
+// The same, then the segment's weight-gradient fold (zl: LDS scratch of
+// zcap >= HMAX reals).
+template <typename T, int IMAX, int OMAX>
+__device__ __forceinline__ void wave_bwd_segment(const RArgs<T> &A, const Rob<T> &R, WSm<T> &m,
+                                                 const WRegs<T, 1, OMAX> &w, const LaneTopo<T> &tp,
+                                                 int r, int lane, int s0, int lo, int tb) {
+    wave_bwd_steps<T, IMAX, OMAX>(A, R, m, w, tp, r, lane, s0, lo, tb);
     wave_seg_wgrad(A, R, m.J, 7 * A.max_s, lane, lo - s0, tb - lo);
     wave_sync();
 }
@@ -1544,7 +1553,8 @@ __device__ __forceinline__ int bwd_segments(int tb, int budget, int K) {
 // Adjoint with the segment recompute on a second wavefront per robot
 // (wavefronts 2j: adjoint, 2j + 1: recompute of robot j), pipelined over two
 // scratch buffers: while the adjoint wavefront reverses segment s from one
-// buffer, the recompute wavefront rebuilds segment s - 1 into the other.
+// buffer, the recompute wavefront folds the weight gradient of segment s + 1
+// from the other and then rebuilds segment s - 1 into it.
 // One workgroup barrier per segment; every wavefront runs the workgroup's
 // largest segment count, idle once its robot is done (or absent), so the
 // barrier counts always agree.
@@ -1595,18 +1605,31 @@ __global__ void __launch_bounds__(512) k_wave_bwd_pipe(RArgs<T> A, int n_robots)
     }
     __syncthreads();
     int buf = 0;
+    int fs0 = 0, flo = 0, ftb = 0;   // segment reversed last, its weight gradient not folded yet
     for (int it = 0; it < iters; ++it) {
         const bool go = tb > 0 && budget > 0;
         const int s0 = go ? (tb - 1) / K * K : 0;
         const int lo = go ? max(s0, tb - budget) : 0;
         const int rest = go ? budget - (tb - lo) : 0;
-        if (go && role == 0) {
-            carve_scratch(R, buf ? scr1 : scr0, K, A.H);
-            wave_bwd_segment<T, IMAX, OMAX>(A, R, m, w, tp, r, lane, s0, lo, tb);
-        } else if (go && lo == s0 && s0 > 0 && rest > 0) {
+        if (role == 0) {
+            if (go) {
+                carve_scratch(R, buf ? scr1 : scr0, K, A.H);
+                wave_bwd_steps<T, IMAX, OMAX>(A, R, m, w, tp, r, lane, s0, lo, tb);
+            }
+        } else {
             carve_scratch(R, buf ? scr0 : scr1, K, A.H);
-            wave_recompute(A, R, m, w, tp, sh.sin_tab, lane, s0 - K, s0);
+            if (ftb > flo) {
+                wave_seg_wgrad(A, R, m.J, 3 * A.max_s, lane, flo - fs0, ftb - flo);
+                // the fold's reads of this buffer complete before it is rebuilt
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+                wave_sync();
+            }
+            if (go && lo == s0 && s0 > 0 && rest > 0)
+                wave_recompute(A, R, m, w, tp, sh.sin_tab, lane, s0 - K, s0);
         }
+        fs0 = s0;
+        flo = lo;
+        ftb = go ? tb : lo;
         __syncthreads();
         if (go) {
             budget = rest;
@@ -1615,6 +1638,10 @@ __global__ void __launch_bounds__(512) k_wave_bwd_pipe(RArgs<T> A, int n_robots)
         }
     }
     if (work && role == 0) wave_bwd_end(R, m, lane);
+    if (role == 1 && ftb > flo) {
+        carve_scratch(R, buf ? scr0 : scr1, K, A.H);
+        wave_seg_wgrad(A, R, m.J, 3 * A.max_s, lane, flo - fs0, ftb - flo);
+    }
 }
 
 // Fixed-order sum of per-robot gradient partials into per-design gradients.
@@ -1755,7 +1782,7 @@ static int launch_rollout(const dl_rollout *r, bool bwd, hipStream_t st) {
                  "rollout: the wave-per-robot kernel needs FP32, H=64, I<=192, O<=8, block 64 or 256");
     // pipelined adjoint: a second wavefront per robot (the scratch holds two
     // segment buffers, see dl_scratch_size)
-    const bool pipe_ok = wave_ok && r->scratch != nullptr &&
+    const bool pipe_ok = wave_ok && r->scratch != nullptr && 3 * r->max_s >= HMAX &&
                          wave_pipe_smem_bytes(r->max_n, r->max_s, r->max_i, r->max_o, n_tab,
                                               r->block / 64) <= kLdsLimit;
     DL_CHECK_ARG(variant != 3 || pipe_ok,
