@@ -154,7 +154,7 @@ __device__ __forceinline__ Sm<T> carve(const RArgs<T> &A, unsigned char *raw, T 
     m.dout = p; p += A.max_o;
     m.z2b = p; p += A.max_o;
     m.cv = p; p += 16;
-    m.red = p; p += 4 * HMAX + 96;   // MLP partials (block <= 256) and block sums
+    m.red = p; p += 16 * HMAX + 96;  // MLP partials (block <= 1024) and block sums
     m.topo_r = p; p += A.topo_in_lds ? 3 * A.max_s : 0;
     m.wl = p; p += A.w_in_lds ? (size_t)A.max_i * (HMAX + 1) : 0;
     m.topo_i = reinterpret_cast<int *>(p);
@@ -166,7 +166,7 @@ __host__ size_t smem_bytes(int max_n, int max_s, int max_i, int max_o, bool bwd,
                            bool topo, bool wl) {
     size_t n = 6 * (size_t)max_n + (bwd ? 9 * (size_t)max_n : 0);
     if (springs) n += (bwd ? 7 : 3) * (size_t)max_s;
-    n += (bwd ? 2 : 1) * (size_t)max_i + 3 * HMAX + 3 * (size_t)max_o + 16 + 4 * HMAX + 96;
+    n += (bwd ? 2 : 1) * (size_t)max_i + 3 * HMAX + 3 * (size_t)max_o + 16 + 16 * HMAX + 96;
     if (topo) n += 3 * (size_t)max_s;
     if (wl) n += (size_t)max_i * (HMAX + 1);
     size_t b = n * sizeof(T);
@@ -366,8 +366,9 @@ __device__ void fwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int t, in
     __syncthreads();
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_rollout_fwd(RArgs<T> A) {
+// LB: the largest workgroup the instantiation is launched with (256 or 512).
+template <typename T, int LB>
+__global__ void __launch_bounds__(LB) k_rollout_fwd(RArgs<T> A) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int r = blockIdx.x;
     Rob<T> R = make_rob(A, r);
@@ -602,8 +603,8 @@ __device__ void seg_wgrad(const RArgs<T> &A, const Rob<T> &R, int n) {
     }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_rollout_bwd(RArgs<T> A) {
+template <typename T, int LB>
+__global__ void __launch_bounds__(LB) k_rollout_bwd(RArgs<T> A) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int r = blockIdx.x;
     Rob<T> R = make_rob(A, r);
@@ -1693,11 +1694,11 @@ static int check_rollout(const dl_rollout *r, bool bwd) {
     DL_CHECK_ARG(r->n_robots > 0 && r->n_robots <= 65535 * 1024, "rollout: bad robot count");
     DL_CHECK_ARG(r->n_hidden > 0 && r->n_hidden <= HMAX, "rollout: hidden width must be 1..%d", HMAX);
     DL_CHECK_ARG(r->ckpt_every > 0, "rollout: checkpoint interval must be positive");
-    DL_CHECK_ARG(r->block >= 64 && r->block <= 256 && r->block % 64 == 0,
-                 "rollout: block must be a multiple of 64 in [64, 256]");
+    DL_CHECK_ARG(r->block >= 64 && r->block <= 512 && r->block % 64 == 0,
+                 "rollout: block must be a multiple of 64 in [64, 512]");
     DL_CHECK_ARG(r->max_n > 0 && r->max_s >= 0 && r->max_i > 0 && r->max_o > 0,
                  "rollout: bad size maxima");
-    DL_CHECK_ARG(r->lds_policy >= 0 && r->lds_policy <= 3, "rollout: bad LDS policy");
+    DL_CHECK_ARG(r->lds_policy >= 0 && r->lds_policy <= 4, "rollout: bad LDS policy");
     DL_CHECK_ARG(r->variant >= 0 && r->variant <= 3, "rollout: bad kernel variant");
     DL_CHECK_ARG(r->lay.n_tv >= 0 && r->lay.n_tv <= 3 && r->lay.run_period > 0, "rollout: bad layout");
     DL_CHECK_ARG(r->loss_kind == 0 || (r->loss_kind == 1 && r->loss_window > 0),
@@ -1759,16 +1760,19 @@ static int launch_rollout(const dl_rollout *r, bool bwd, hipStream_t st) {
     RArgs<T> A = make_rargs<T>(r);
     // LDS placement: policy 0 = auto (topology when it fits in 64 KiB, W1 too
     // when the total stays within 80 KiB), 1 = topology only, 2 = topology +
-    // W1, 3 = nothing staged.
+    // W1, 3 = nothing staged, 4 = as much as fits in the whole LDS (spring
+    // buffers, then topology, then W1; one workgroup per CU for big robots).
     const int pol = r->lds_policy;
     auto bytes = [&](bool springs, bool topo, bool wl) {
         return smem_bytes<T>(r->max_n, r->max_s, r->max_i, r->max_o, bwd, springs, topo, wl);
     };
-    bool springs = bytes(true, false, false) <= 64 * 1024;
+    const size_t cap = pol == 4 ? kLdsLimit : 64 * 1024;
+    bool springs = bytes(true, false, false) <= cap;
     bool topo = pol != 3, wl = pol == 2 || pol == 0;
-    if (pol == 0) {
-        topo = bytes(springs, true, false) <= 64 * 1024;
-        wl = topo && r->n_hidden == HMAX && bytes(springs, true, true) <= 80 * 1024;
+    if (pol == 0 || pol == 4) {
+        topo = bytes(springs, true, false) <= cap;
+        wl = topo && r->n_hidden == HMAX &&
+             bytes(springs, true, true) <= (pol == 4 ? kLdsLimit : 80 * 1024);
     }
     // wave-per-robot kernel: FP32, hidden width 64, inputs <= 192, outputs <= 8
     const int variant = r->variant;
@@ -1796,13 +1800,19 @@ static int launch_rollout(const dl_rollout *r, bool bwd, hipStream_t st) {
     A.springs_in_lds = springs;
     A.topo_in_lds = topo;
     A.w_in_lds = wl;
-    if (bwd) {
-        DL_CHECK_HIP(ensure_dynamic_lds((const void *)k_rollout_bwd<T>, sh));
-        hipLaunchKernelGGL(k_rollout_bwd<T>, dim3(r->n_robots), dim3(r->block), sh, st, A);
-    } else {
-        DL_CHECK_HIP(ensure_dynamic_lds((const void *)k_rollout_fwd<T>, sh));
-        hipLaunchKernelGGL(k_rollout_fwd<T>, dim3(r->n_robots), dim3(r->block), sh, st, A);
-    }
+    const void *kf = bwd ? (r->block > 256 ? (const void *)k_rollout_bwd<T, 512>
+                                           : (const void *)k_rollout_bwd<T, 256>)
+                         : (r->block > 256 ? (const void *)k_rollout_fwd<T, 512>
+                                           : (const void *)k_rollout_fwd<T, 256>);
+    DL_CHECK_HIP(ensure_dynamic_lds(kf, sh));
+    if (bwd && r->block > 256)
+        hipLaunchKernelGGL((k_rollout_bwd<T, 512>), dim3(r->n_robots), dim3(r->block), sh, st, A);
+    else if (bwd)
+        hipLaunchKernelGGL((k_rollout_bwd<T, 256>), dim3(r->n_robots), dim3(r->block), sh, st, A);
+    else if (r->block > 256)
+        hipLaunchKernelGGL((k_rollout_fwd<T, 512>), dim3(r->n_robots), dim3(r->block), sh, st, A);
+    else
+        hipLaunchKernelGGL((k_rollout_fwd<T, 256>), dim3(r->n_robots), dim3(r->block), sh, st, A);
     DL_CHECK_LAUNCH();
     return 0;
 }

@@ -99,7 +99,8 @@ class UnevenBatch:
     def candidates(name):
         if name == "wave":
             return [{"variant": v, "block": 64, "lds_policy": 0} for v in (3, 2)]
-        return [{"variant": 1, "block": b, "lds_policy": p} for b in (64, 128, 256) for p in (0, 1)]
+        return [{"variant": 1, "block": b, "lds_policy": p} for b in (64, 128, 256, 512)
+                for p in (0, 1, 4)]
 
     def _apply(self, bk, cfg):
         b = bk["batch"]
@@ -142,15 +143,19 @@ class UnevenBatch:
             bk["batch"].prepare_iteration(lg[bk["ids"]])
 
     def _on_streams(self, fns):
+        """Run one function per size class, each on its own stream, issued
+        largest class first: its robots have the longest per-step chains and
+        set the iteration's length, so they take CUs before the small ones."""
+        order = list(range(len(fns)))[::-1]
         if not self.streams_on:
-            for f in fns:
-                f()
+            for i in order:
+                fns[i]()
             return
         cur = torch.cuda.current_stream()
-        for s, f in zip(self._streams, fns):
-            s.wait_stream(cur)
-            with torch.cuda.stream(s):
-                f()
+        for i in order:
+            self._streams[i].wait_stream(cur)
+            with torch.cuda.stream(self._streams[i]):
+                fns[i]()
         for s in self._streams:
             cur.wait_stream(s)
 
@@ -160,7 +165,9 @@ class UnevenBatch:
 
     def capture(self):
         """Record each size class's iteration in a HIP graph."""
-        self.graphs = [GraphedIteration(bk["batch"].run_iteration) for bk in self.buckets]
+        self.graphs = [None] * len(self.buckets)
+        for i in range(len(self.buckets))[::-1]:
+            self.graphs[i] = GraphedIteration(self.buckets[i]["batch"].run_iteration)
 
     def replay(self):
         self._on_streams([g.replay for g in self.graphs])

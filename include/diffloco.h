@@ -220,12 +220,14 @@ typedef struct dl_rollout {
     int32_t n_hidden;        /* H, shared by every design */
     int32_t ckpt_every;      /* K */
     int32_t max_steps;       /* steps this launch may take per robot (<= 0: to the end) */
-    int32_t block;           /* threads per workgroup (64..256, multiple of 64) */
+    int32_t block;           /* threads per workgroup (64..512, multiple of 64; the
+                                wavefront-per-robot variants take 64 or 256) */
     int32_t loss_kind;       /* 0: none in-kernel, 1: windowed COM loss */
     int32_t loss_window;     /* states t in (T - window, T] */
     int32_t store_traj;      /* forward writes x/center trajectory rows */
     int32_t max_n, max_s, max_i, max_o;   /* maxima over the table (LDS sizing) */
-    int32_t lds_policy;      /* 0 auto, 1 stage topology, 2 topology + W1, 3 none */
+    int32_t lds_policy;      /* 0 auto, 1 stage topology, 2 topology + W1, 3 none,
+                                4 as much as fits in the whole LDS */
     int32_t variant;         /* 0 auto, 1 workgroup per robot, 2 wavefront per robot (FP32,
                                 H = 64, I <= 192, O <= 8; robots of one design packed by 4),
                                 3 as 2 with the adjoint's segment recompute on a second

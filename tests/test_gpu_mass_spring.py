@@ -385,7 +385,7 @@ def test_pipelined_adjoint_bit_identical_to_sequential(budget):
     assert torch.equal(aa, ab)
 
 
-def _uneven(prec, variant, seed=31):
+def _uneven(prec, variant, seed=31, block=256, lds_policy=0):
     from diffloco import robot_design as rd
     rng = np.random.default_rng(seed)
     cubes_list, designs = [], []
@@ -401,16 +401,21 @@ def _uneven(prec, variant, seed=31):
     x0 = [d.objects + np.array([0.4, 0.0, 0.0]) for d in designs]
     batch = MassSpringBatch(designs, [0, 1, 2], horizons, _physics(qc), sensor_layout(qc), tg, x0,
                             act="tanh", ckpt_every=16, dtype=DT[prec], loss_window=W,
-                            variant=variant)
+                            variant=variant, block=block, lds_policy=lds_policy)
     return batch, designs, cubes_list, horizons, W, tv, th
 
 
-@pytest.mark.parametrize("prec,variant", [("f64", 1), ("f32", 1), ("f32", 2)])
-def test_uneven_batch_matches_oracle(prec, variant):
-    """Three different random designs (own sizes, MLPs and horizons) in one launch."""
-    _say(f"uneven {prec} variant={variant}")
+@pytest.mark.parametrize("prec,variant,block,pol", [("f64", 1, 256, 0), ("f32", 1, 256, 0),
+                                                    ("f32", 2, 256, 0), ("f32", 1, 512, 4),
+                                                    ("f32", 1, 128, 4)])
+def test_uneven_batch_matches_oracle(prec, variant, block, pol):
+    """Three different random designs (own sizes, MLPs and horizons) in one launch;
+    also the block kernel at 512 threads and with everything staged in LDS
+    (policy 4)."""
+    _say(f"uneven {prec} variant={variant} block={block} policy={pol}")
     from oracle import builders as ob
-    batch, designs, cubes_list, horizons, W, tv, th = _uneven(prec, variant)
+    batch, designs, cubes_list, horizons, W, tv, th = _uneven(prec, variant, block=block,
+                                                                lds_policy=pol)
     lay = om.TaskConfig().layout
     weights64 = [om.init_weights(lay.n_input(d.n_vert), 64, d.n_act, seed=40 + i)
                  for i, d in enumerate(designs)]
