@@ -112,6 +112,13 @@ __device__ __forceinline__ double dpp_f(double v) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// Lane K of each 16-lane row, to every lane of that row (DPP row_newbcast:K);
+// the move folds into the consuming VALU op as its DPP operand.
+template <int K>
+__device__ __forceinline__ float row_bcast(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x150 + K, 0xF, 0xF, true));
+}
+
 // Value of lane q (wave-uniform q), to every lane.
 __device__ __forceinline__ float lane_read(float v, int q) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), q));

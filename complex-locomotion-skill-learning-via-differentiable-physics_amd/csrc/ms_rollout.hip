@@ -11,6 +11,8 @@
 //   -> per-vertex gather (CSR, fixed order) + integrate.
 
 #include "dl_common.hpp"
+#include <type_traits>
+#include <utility>
 
 namespace dl {
 
@@ -934,6 +936,14 @@ __device__ __forceinline__ void gather_J(const Rob<T> &R, const WSm<T> &m, const
     }
 }
 
+// a[k % 4] += w1[k] * (input k of the row block x), k = 0..15, input k taken
+// from lane k of the lane's 16-lane row.
+template <int... K>
+__device__ __forceinline__ void l1_row(float (&a)[4], const float *w1, float x,
+                                       std::integer_sequence<int, K...>) {
+    ((a[K & 3] += w1[K] * row_bcast<K>(x)), ...);
+}
+
 // one forward step of the wave's robot (MODE_FWD / MODE_REC like fwd_step).
 // WREG: layer-1 weights from registers (forward kernel) or from the shared
 // LDS copy of W1 (adjoint kernel, whose adjoint phase needs the registers).
@@ -1013,8 +1023,20 @@ __device__ __forceinline__ void wave_fwd_step(const RArgs<T> &A, const Rob<T> &R
         for (int q = lane; q < I; q += 64) R.sin[(size_t)slot * I + q] = m.inb[q];
     // layer 1: lane j = hidden unit j, weights in registers, inputs broadcast from LDS
     T acc = w.b1;
-    if constexpr (WREG) {
-        // weights beyond I are zero and the input tail is zero-filled: no guards
+    if constexpr (WREG && std::is_same<T, float>::value && IMAX % 16 == 0) {
+        // weights beyond I are zero and the input tail is zero-filled: no guards.
+        // Inputs 16q .. 16q+15 sit in every 16-lane row of xin[q] (one LDS read
+        // per 16 inputs, all issued together); each FMA takes its input by a
+        // row broadcast, so the loop has no LDS round trips.
+        float xin[IMAX / 16];
+#pragma unroll
+        for (int q = 0; q < IMAX / 16; ++q) xin[q] = m.inb[16 * q + (lane & 15)];
+        float a[4] = {acc, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int q = 0; q < IMAX / 16; ++q)
+            l1_row(a, w.w1 + 16 * q, xin[q], std::make_integer_sequence<int, 16>{});
+        acc = (a[0] + a[1]) + (a[2] + a[3]);
+    } else if constexpr (WREG) {
         T a1 = T(0), a2 = T(0), a3 = T(0);
 #pragma unroll
         for (int i = 0; i < IMAX; i += 4) {
@@ -1166,6 +1188,18 @@ __device__ __forceinline__ void pref_load(const Rob<T> &R, int H, int lane, int 
     p.cv = lane < 6 ? R.scv[slot * 6 + lane] : T(0);
 }
 
+// acc[p] += rows[p][j0 + k] * z_k in order k = 0..15, z_k from lane k of the
+// lane's 16-lane row of z.
+template <int NP, int... K>
+__device__ __forceinline__ void in_adj_rows(float (&acc)[NP], const float *const (&rows)[NP], int j0,
+                                            float z, std::integer_sequence<int, K...>) {
+    auto one = [&](float zk, int k) {
+#pragma unroll
+        for (int p = 0; p < NP; ++p) acc[p] += rows[p][j0 + k] * zk;
+    };
+    (one(row_bcast<K>(z), K), ...);
+}
+
 template <typename T, int IMAX, int OMAX, int SK, int DK>
 __device__ __forceinline__ void wave_bwd_step(const RArgs<T> &A, const Rob<T> &R, WSm<T> &m, int r, int lane,
                               int t, int slot, Pref<T> &pf, const WRegs<T, 1, OMAX> &w,
@@ -1281,27 +1315,51 @@ __device__ __forceinline__ void wave_bwd_step(const RArgs<T> &A, const Rob<T> &R
         }
     }
     wave_sync();
+    T z1;
     {
         T hb = T(0);
 #pragma unroll
         for (int o = 0; o < OMAX; ++o)
             if (o < O) hb += w.w2[o] * m.z2b[o];
-        const T z1 = hb * dhj;
-        m.z1b[lane] = z1;
+        z1 = hb * dhj;
         if (lane < H) R.sz1[(size_t)slot * H + lane] = z1;
     }
-    wave_sync();
-    for (int i = lane; i < I; i += 64) {
-        T acc = T(0);
-        if (R.WL) {
-            const T *row = R.WL + i * (HMAX + 1);
-#pragma unroll 16
-            for (int j = 0; j < H; ++j) acc += row[j] * m.z1b[j];
-        } else {
+    // input adjoint inbar = W1^T z1: lane i + 64 p sums row i of the LDS copy
+    // against z1, all row blocks in one pass over j, in chunks of 16: each
+    // chunk reads z1[j0 .. j0 + 15] into every 16-lane row with one LDS read
+    // and takes each z1_j from it by a row broadcast.  A broadcast reads 0
+    // from a disabled lane, so every lane runs the loop (rows past I read row
+    // I-1, stores guarded).  The wave kernels run with H = HMAX (checked at
+    // launch).
+    if (std::is_same<T, float>::value && R.WL) {
+        constexpr int NP = (IMAX + 63) / 64;
+        m.z1b[lane] = z1;
+        wave_sync();
+        const float *rows[NP];
+        float acc[NP];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const int i = lane + 64 * p;
+            rows[p] = reinterpret_cast<const float *>(R.WL) + (i < I ? i : I - 1) * (HMAX + 1);
+            acc[p] = 0.f;
+        }
+#pragma unroll 1
+        for (int j0 = 0; j0 < HMAX; j0 += 16) {
+            const float zc = m.z1b[j0 + (lane & 15)];
+            in_adj_rows(acc, rows, j0, zc, std::make_integer_sequence<int, 16>{});
+        }
+#pragma unroll
+        for (int p = 0; p < NP; ++p)
+            if (lane + 64 * p < I) m.inbar[lane + 64 * p] = T(acc[p]);
+    } else {
+        m.z1b[lane] = z1;
+        wave_sync();
+        for (int i = lane; i < I; i += 64) {
+            T acc = T(0);
 #pragma unroll 16
             for (int j = 0; j < H; ++j) acc += R.W1[(size_t)j * I + i] * m.z1b[j];
+            m.inbar[i] = acc;
         }
-        m.inbar[i] = acc;
     }
     wave_sync();
     const Layout &L = A.L;
