@@ -706,7 +706,7 @@ struct WSm {
 
 __host__ __device__ inline size_t wave_region(int max_n, int max_s, int max_i, int max_o, bool bwd) {
     const size_t ni = (size_t)(max_i > 192 ? max_i : 192) + 4;   // GEMV reads up to 192 inputs
-    size_t n = (bwd ? 15 : 6) * (size_t)max_n + (bwd ? 7 : 3) * (size_t)max_s +
+    size_t n = (bwd ? 15 : 6) * (size_t)max_n + (bwd ? 8 : 4) * (size_t)max_s +
                (bwd ? 2 : 1) * ni + HMAX + 3 * (size_t)max_o;
     return (n + 3) & ~(size_t)3;
 }
@@ -731,6 +731,9 @@ template <typename T>
 __device__ __forceinline__ WSm<T> wave_carve_at(const RArgs<T> &A, T *p, bool bwd) {
     WSm<T> m;
     const int n3 = 3 * A.max_n;
+    // spring rows first: the region starts 16-byte aligned, and the rows
+    // are read and written 16 bytes at a time
+    m.J = p; p += (bwd ? 8 : 4) * A.max_s;
     m.xs = p; p += n3;
     m.vs = p; p += n3;
     if (bwd) {
@@ -740,7 +743,6 @@ __device__ __forceinline__ WSm<T> wave_carve_at(const RArgs<T> &A, T *p, bool bw
     } else {
         m.xb = m.vb = m.vib = nullptr;
     }
-    m.J = p; p += (bwd ? 7 : 3) * A.max_s;
     const int ni = (A.max_i > 192 ? A.max_i : 192) + 4;
     m.inb = p; p += ni;
     if (bwd) { m.inbar = p; p += ni; } else { m.inbar = nullptr; }
@@ -884,7 +886,24 @@ __device__ __forceinline__ void lane_topo_load(const Rob<T> &R, int lane, LaneTo
     for (int q = 0; q < TDK; ++q) tp.inc[q] = q < tp.deg ? R.vidx[e0 + q] : 0;
 }
 
-// Spring impulses J[c][s] (J = m.J, [3][S]) of the current state.  Spring
+// Rows of 4 reals at p + 4 r (16-byte aligned): components 0..2 of row r.
+template <typename T>
+__device__ __forceinline__ void row3_put(T *p, int r, T a, T b, T c) {
+    p[4 * r] = a; p[4 * r + 1] = b; p[4 * r + 2] = c;
+}
+__device__ __forceinline__ void row3_put(float *p, int r, float a, float b, float c) {
+    *reinterpret_cast<float4 *>(p + 4 * r) = make_float4(a, b, c, 0.f);
+}
+template <typename T>
+__device__ __forceinline__ void row3_get(const T *p, int r, T &a, T &b, T &c) {
+    a = p[4 * r]; b = p[4 * r + 1]; c = p[4 * r + 2];
+}
+__device__ __forceinline__ void row3_get(const float *p, int r, float &a, float &b, float &c) {
+    const float4 v = *reinterpret_cast<const float4 *>(p + 4 * r);
+    a = v.x; b = v.y; c = v.z;
+}
+
+// Spring impulses J (J = m.J, row s = spring s, [S][4]) of the current state.  Spring
 // lane + 64 k from registers (loads issued unconditionally -- springs past
 // the last read vertex 0 -- and stores masked), the rest from LDS.
 template <int SK, typename T>
@@ -896,9 +915,7 @@ __device__ __forceinline__ void spring_impulses(const RArgs<T> &A, const Rob<T> 
         const T wv[3] = {m.vs[a] - m.vs[b], m.vs[N + a] - m.vs[N + b], m.vs[2 * N + a] - m.vs[2 * N + b]};
         T J[3];
         spring_impulse<T, true>(d, wv, L0, kk, co, m.outb[ac], A.P, J);
-        if (s < S) {
-            m.J[s] = J[0]; m.J[S + s] = J[1]; m.J[2 * S + s] = J[2];
-        }
+        if (s < S) row3_put(m.J, s, J[0], J[1], J[2]);
     };
 #pragma unroll
     for (int q = 0; q < SK; ++q) {
@@ -916,23 +933,26 @@ __device__ __forceinline__ void spring_impulses(const RArgs<T> &A, const Rob<T> 
 template <int DK, typename T>
 __device__ __forceinline__ void gather_J(const Rob<T> &R, const WSm<T> &m, const LaneTopo<T> &tp,
                                          int i, T vi[3]) {
-    const int S = R.S;
     vi[0] = vi[1] = vi[2] = T(0);
 #pragma unroll
     for (int q = 0; q < DK; ++q) {
         const int ent = tp.inc[q], s = ent >> 1;
-        const T j0 = m.J[s], j1 = m.J[S + s], j2 = m.J[2 * S + s];
-        const bool on = q < tp.deg;
-        const bool pos = ent & 1;
-        vi[0] += on ? (pos ? j0 : -j0) : T(0);
-        vi[1] += on ? (pos ? j1 : -j1) : T(0);
-        vi[2] += on ? (pos ? j2 : -j2) : T(0);
+        T j0, j1, j2;
+        row3_get(m.J, s, j0, j1, j2);
+        // entry sign, 0 past the degree: every load's value is used, so the
+        // loads cannot be sunk into per-entry branches (products are exact)
+        const T sg = q < tp.deg ? ((ent & 1) ? T(1) : T(-1)) : T(0);
+        vi[0] += sg * j0;
+        vi[1] += sg * j1;
+        vi[2] += sg * j2;
     }
 #pragma unroll 4
     for (int e = R.vptr[i] + DK; e < R.vptr[i + 1]; ++e) {
         const int ent = R.vidx[e], s = ent >> 1;
         const T sg = (ent & 1) ? T(1) : T(-1);
-        vi[0] += sg * m.J[s]; vi[1] += sg * m.J[S + s]; vi[2] += sg * m.J[2 * S + s];
+        T j0, j1, j2;
+        row3_get(m.J, s, j0, j1, j2);
+        vi[0] += sg * j0; vi[1] += sg * j1; vi[2] += sg * j2;
     }
 }
 
@@ -1243,9 +1263,10 @@ __device__ __forceinline__ void wave_bwd_step(const RArgs<T> &A, const Rob<T> &R
         }
     }
     wave_sync();
-    // spring adjoints G[c][s] (d: c < 3, relative velocity: 3 <= c < 6); the
-    // actuation adjoints are summed per actuator in registers (lane partial
-    // sums, then one wave sum per actuator)
+    // spring adjoints G, rows of 4: row 2 s the length-vector adjoint of
+    // spring s, row 2 s + 1 its relative-velocity adjoint; the actuation
+    // adjoints are summed per actuator in registers (lane partial sums, then
+    // one wave sum per actuator)
     T *G = m.J;
     T abp[OMAX];
 #pragma unroll
@@ -1257,11 +1278,8 @@ __device__ __forceinline__ void wave_bwd_step(const RArgs<T> &A, const Rob<T> &R
         T gd[3], gw[3], ga;
         spring_impulse_adj<T, true>(d, wv, L0, kk, co, m.outb[ac], A.P, Jb, gd, gw, ga);
         if (s < S) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                G[c * S + s] = gd[c];
-                G[(3 + c) * S + s] = gw[c];
-            }
+            row3_put(G, 2 * s, gd[0], gd[1], gd[2]);
+            row3_put(G, 2 * s + 1, gw[0], gw[1], gw[2]);
 #pragma unroll
             for (int o = 0; o < OMAX; ++o) abp[o] += ac == o ? ga : T(0);
         }
@@ -1280,22 +1298,28 @@ __device__ __forceinline__ void wave_bwd_step(const RArgs<T> &A, const Rob<T> &R
 #pragma unroll
         for (int q = 0; q < DK; ++q) {
             const int ent = tp.inc[q], s = ent >> 1;
-            const bool on = q < tp.deg, neg = ent & 1;
+            T g[6];
+            row3_get(G, 2 * s, g[0], g[1], g[2]);
+            row3_get(G, 2 * s + 1, g[3], g[4], g[5]);
+            // entry sign, 0 past the degree (loads stay unconditional)
+            const T sg = q < tp.deg ? ((ent & 1) ? T(-1) : T(1)) : T(0);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const T g0 = G[c * S + s], g1 = G[(3 + c) * S + s];
-                gx[c] += on ? (neg ? -g0 : g0) : T(0);
-                gv[c] += on ? (neg ? -g1 : g1) : T(0);
+                gx[c] += sg * g[c];
+                gv[c] += sg * g[3 + c];
             }
         }
 #pragma unroll 4
         for (int e = R.vptr[i] + DK; e < R.vptr[i + 1]; ++e) {
             const int ent = R.vidx[e], s = ent >> 1;
             const T sg = (ent & 1) ? T(-1) : T(1);
+            T g[6];
+            row3_get(G, 2 * s, g[0], g[1], g[2]);
+            row3_get(G, 2 * s + 1, g[3], g[4], g[5]);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                gx[c] += sg * G[c * S + s];
-                gv[c] += sg * G[(3 + c) * S + s];
+                gx[c] += sg * g[c];
+                gv[c] += sg * g[3 + c];
             }
         }
 #pragma unroll
