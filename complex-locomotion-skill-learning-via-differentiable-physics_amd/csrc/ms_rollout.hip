@@ -1067,10 +1067,14 @@ __device__ __forceinline__ void wave_fwd_step(const RArgs<T> &A, const Rob<T> &R
         }
         acc = (acc + a1) + (a2 + a3);
     } else if (R.WL) {
-        if (lane < H) {
-#pragma unroll 8
-            for (int i = 0; i < I; ++i) acc += R.WL[i * (HMAX + 1) + lane] * m.inb[i];
-        }
+        // W1 column `lane` from the LDS copy (H = HMAX in the wave kernels),
+        // in the register form's accumulation order (input i into a[i % 4]),
+        // so the recompute reproduces the forward's hidden values bit for bit
+        T a[4] = {acc, T(0), T(0), T(0)};
+        const T *wl = R.WL + lane;
+#pragma unroll 16
+        for (int i = 0; i < I; ++i) a[i & 3] += wl[i * (HMAX + 1)] * m.inb[i];
+        acc = (a[0] + a[1]) + (a[2] + a[3]);
     } else if (lane < H) {
 #pragma unroll 8
         for (int i = 0; i < I; ++i) acc += R.W1T[(size_t)i * H + lane] * m.inb[i];

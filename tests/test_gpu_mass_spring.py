@@ -385,6 +385,40 @@ def test_pipelined_adjoint_bit_identical_to_sequential(budget):
     assert torch.equal(aa, ab)
 
 
+@pytest.mark.parametrize("variant", [2, 3])
+def test_wave_recompute_reproduces_forward(variant):
+    """The adjoint's segment recompute repeats the forward bit for bit (same
+    accumulation orders in layer 1, the spring pass and the incidence
+    gathers, with the weights and topology read from LDS instead of
+    registers): after the adjoint, the recompute rows of the first segment
+    hold exactly the positions the forward stored for steps 0 .. K-1."""
+    _say(f"recompute vs forward variant={variant}")
+    B, T, W, K = 5, 70, 30, 16
+    tv, th = task_targets(B, 25)
+    design = dl.build_quadruped()
+    N, S, O = design.n_vert, design.n_spring, design.n_act
+    I = om.TaskConfig().layout.n_input(N)
+    w64 = [w.to("cuda", torch.float32) for w in om.init_weights(I, 64, 4, seed=26)]
+    qc = dl.quadruped_config(batch_size=B, steps=T, window=W).get_config()
+    x0 = design.objects + np.array([0.4, 0.0, 0.0])
+    tg = np.concatenate([tv, th[:, None], np.zeros((B, 1))], 1)
+    batch = MassSpringBatch([design], [0] * B, T, _physics(qc), sensor_layout(qc), tg, [x0] * B,
+                            act="tanh", ckpt_every=K, dtype=torch.float32, loss_window=W,
+                            store_traj=True, variant=variant)
+    batch.set_weights(0, *w64)
+    batch.forward()
+    batch.backward(loss_grad=np.full(B, 1.0 / B))
+    torch.cuda.synchronize()
+    buf_len = K * (6 * N + I + 3 * 64 + 3 * O + 6) + 10 * S   # buffer 0 rows + spill
+    for r in range(B):
+        o = int(batch.table[r]["traj"])
+        fwd = batch.traj[o:o + K * (3 * N + 3)].view(K, 3 * N + 3)[:, :3 * N]
+        sc = int(batch.table[r]["scratch"])
+        bufs = [batch.scratch[b:b + K * 6 * N].view(K, 6 * N)[:, :3 * N] for b in (sc, sc + buf_len)]
+        hits = [torch.equal(fwd, x) for x in bufs]
+        assert hits[0] if variant == 2 else any(hits), (r, hits)
+
+
 def _uneven(prec, variant, seed=31, block=256, lds_policy=0):
     from diffloco import robot_design as rd
     rng = np.random.default_rng(seed)
