@@ -1638,7 +1638,9 @@ __device__ __forceinline__ int bwd_segments(int tb, int budget, int K) {
 }
 
 // Adjoint with the segment recompute on a second wavefront per robot
-// (wavefronts 2j: adjoint, 2j + 1: recompute of robot j), pipelined over two
+// (wavefront j: adjoint of robot j, wavefront nrob + j: its recompute; the
+// wavefronts of a workgroup go to the four SIMDs in turn, so each SIMD runs
+// one adjoint and one recompute wavefront), pipelined over two
 // scratch buffers: while the adjoint wavefront reverses segment s from one
 // buffer, the recompute wavefront folds the weight gradient of segment s + 1
 // from the other and then rebuilds segment s - 1 into it.
@@ -1651,7 +1653,7 @@ __global__ void __launch_bounds__(512) k_wave_bwd_pipe(RArgs<T> A, int n_robots)
     __shared__ int iters;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const int nrob = blockDim.x >> 7, rl = wid >> 1, role = wid & 1;
+    const int nrob = blockDim.x >> 7, rl = wid % nrob, role = wid / nrob;
     const int r0 = blockIdx.x * nrob;
     const int K = A.K;
     const int budget0 = A.max_steps > 0 ? A.max_steps : 0x7fffffff;
