@@ -706,7 +706,7 @@ struct WSm {
 
 __host__ __device__ inline size_t wave_region(int max_n, int max_s, int max_i, int max_o, bool bwd) {
     const size_t ni = (size_t)(max_i > 192 ? max_i : 192) + 4;   // GEMV reads up to 192 inputs
-    size_t n = (bwd ? 15 : 6) * (size_t)max_n + (bwd ? 8 : 4) * (size_t)max_s +
+    size_t n = (bwd ? 20 : 8) * (size_t)max_n + (bwd ? 8 : 4) * (size_t)max_s +
                (bwd ? 2 : 1) * ni + HMAX + 3 * (size_t)max_o;
     return (n + 3) & ~(size_t)3;
 }
@@ -730,7 +730,7 @@ __host__ inline size_t wave_pipe_smem_bytes(int max_n, int max_s, int max_i, int
 template <typename T>
 __device__ __forceinline__ WSm<T> wave_carve_at(const RArgs<T> &A, T *p, bool bwd) {
     WSm<T> m;
-    const int n3 = 3 * A.max_n;
+    const int n3 = 4 * A.max_n;   // vertex rows of 4 (x, y, z, 0)
     // spring rows first: the region starts 16-byte aligned, and the rows
     // are read and written 16 bytes at a time
     m.J = p; p += (bwd ? 8 : 4) * A.max_s;
@@ -903,6 +903,22 @@ __device__ __forceinline__ void row3_get(const float *p, int r, float &a, float 
     a = v.x; b = v.y; c = v.z;
 }
 
+// The wave kernels keep vertex i's position, velocity and adjoints as LDS
+// rows of 4 at 4 i (one 16-byte access per vertex); the state, checkpoints
+// and scratch rows in global memory are [3][N].
+template <typename T>
+__device__ __forceinline__ void rows_to_soa(T *g, const T *p, int N, int lane) {
+    for (int i = lane; i < N; i += 64) {
+        T a, b, c;
+        row3_get(p, i, a, b, c);
+        g[i] = a; g[N + i] = b; g[2 * N + i] = c;
+    }
+}
+template <typename T>
+__device__ __forceinline__ void soa_to_rows(T *p, const T *g, int N, int lane) {
+    for (int i = lane; i < N; i += 64) row3_put(p, i, g[i], g[N + i], g[2 * N + i]);
+}
+
 // Spring impulses J (J = m.J, row s = spring s, [S][4]) of the current state.  Spring
 // lane + 64 k from registers (loads issued unconditionally -- springs past
 // the last read vertex 0 -- and stores masked), the rest from LDS.
@@ -911,8 +927,13 @@ __device__ __forceinline__ void spring_impulses(const RArgs<T> &A, const Rob<T> 
                                                 const LaneTopo<T> &tp, int lane) {
     const int N = R.N, S = R.S;
     auto one = [&](int s, int a, int b, int ac, T L0, T kk, T co) {
-        const T d[3] = {m.xs[a] - m.xs[b], m.xs[N + a] - m.xs[N + b], m.xs[2 * N + a] - m.xs[2 * N + b]};
-        const T wv[3] = {m.vs[a] - m.vs[b], m.vs[N + a] - m.vs[N + b], m.vs[2 * N + a] - m.vs[2 * N + b]};
+        T xa[3], xb[3], va[3], vb[3];
+        row3_get(m.xs, a, xa[0], xa[1], xa[2]);
+        row3_get(m.xs, b, xb[0], xb[1], xb[2]);
+        row3_get(m.vs, a, va[0], va[1], va[2]);
+        row3_get(m.vs, b, vb[0], vb[1], vb[2]);
+        const T d[3] = {xa[0] - xb[0], xa[1] - xb[1], xa[2] - xb[2]};
+        const T wv[3] = {va[0] - vb[0], va[1] - vb[1], va[2] - vb[2]};
         T J[3];
         spring_impulse<T, true>(d, wv, L0, kk, co, m.outb[ac], A.P, J);
         if (s < S) row3_put(m.J, s, J[0], J[1], J[2]);
@@ -977,8 +998,8 @@ __device__ __forceinline__ void wave_fwd_step(const RArgs<T> &A, const Rob<T> &R
     for (int i = lane; i < N; i += 64) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            s6[d] += m.xs[d * N + i] * invN;
-            s6[3 + d] += m.vs[d * N + i] * invN;
+            s6[d] += m.xs[4 * i + d] * invN;
+            s6[3 + d] += m.vs[4 * i + d] * invN;
         }
     }
 #pragma unroll
@@ -987,23 +1008,19 @@ __device__ __forceinline__ void wave_fwd_step(const RArgs<T> &A, const Rob<T> &R
     if (mode == MODE_FWD) {
         if (t % A.K == 0) {
             T *ck = R.ckpt + (size_t)(t / A.K) * 6 * N;
-            for (int i = lane; i < 3 * N; i += 64) {
-                ck[i] = m.xs[i];
-                ck[3 * N + i] = m.vs[i];
-            }
+            rows_to_soa(ck, m.xs, N, lane);
+            rows_to_soa(ck + 3 * N, m.vs, N, lane);
         }
         if (A.store_traj && R.traj) {
             T *row = R.traj + (size_t)t * (3 * N + 3);
-            for (int i = lane; i < 3 * N; i += 64) row[i] = m.xs[i];
+            rows_to_soa(row, m.xs, N, lane);
             if (lane < 3) row[3 * N + lane] = lane == 0 ? s6[0] : (lane == 1 ? s6[1] : s6[2]);
         }
         if (A.loss_kind == 1 && t > R.Th - A.window) lsum += task_term(s6, tg);
     } else {
         T *sx = R.sx + (size_t)slot * 6 * N;
-        for (int i = lane; i < 3 * N; i += 64) {
-            sx[i] = m.xs[i];
-            sx[3 * N + i] = m.vs[i];
-        }
+        rows_to_soa(sx, m.xs, N, lane);
+        rows_to_soa(sx + 3 * N, m.vs, N, lane);
         if (lane < 6) {
             T val = s6[0];
 #pragma unroll
@@ -1021,9 +1038,9 @@ __device__ __forceinline__ void wave_fwd_step(const RArgs<T> &A, const Rob<T> &R
         const T isc = T(1) / T(L.offset_scale);
         for (int i = lane; i < N; i += 64) {
             T *o = m.inb + base + i * vs;
-            o[0] = (m.xs[i] - s6[0]) * isc;
-            o[1] = (m.xs[N + i] - s6[1]) * isc;
-            o[2] = (m.xs[2 * N + i] - s6[2]) * isc;
+            o[0] = (m.xs[4 * i] - s6[0]) * isc;
+            o[1] = (m.xs[4 * i + 1] - s6[1]) * isc;
+            o[2] = (m.xs[4 * i + 2] - s6[2]) * isc;
             for (int d = 3; d < vs; ++d) o[d] = T(0);
         }
         const int tail = base + N * vs, nv = L.dup_v * L.n_tv;
@@ -1113,15 +1130,13 @@ __device__ __forceinline__ void wave_fwd_step(const RArgs<T> &A, const Rob<T> &R
     for (int i = lane; i < N; i += 64) {
         T vi[3];
         gather_J<DK>(R, m, tp, i, vi);
-        const T x[3] = {m.xs[i], m.xs[N + i], m.xs[2 * N + i]};
-        const T v[3] = {m.vs[i], m.vs[N + i], m.vs[2 * N + i]};
+        T x[3], v[3];
+        row3_get(m.xs, i, x[0], x[1], x[2]);
+        row3_get(m.vs, i, v[0], v[1], v[2]);
         T xn[3], vn[3];
         integrate<T>(x, v, vi, A.P, xn, vn);
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            m.xs[d * N + i] = xn[d];
-            m.vs[d * N + i] = vn[d];
-        }
+        row3_put(m.xs, i, xn[0], xn[1], xn[2]);
+        row3_put(m.vs, i, vn[0], vn[1], vn[2]);
     }
     wave_sync();
 }
@@ -1156,10 +1171,8 @@ __global__ void __launch_bounds__(256) k_wave_fwd(RArgs<T> A, int n_robots) {
     wave_load_weights(R, A.H, lane, w);
     LaneTopo<T> tp;
     lane_topo_load(R, lane, tp);
-    for (int i = lane; i < 3 * N; i += 64) {
-        m.xs[i] = R.state[i];
-        m.vs[i] = R.state[3 * N + i];
-    }
+    soa_to_rows(m.xs, R.state, N, lane);
+    soa_to_rows(m.vs, R.state + 3 * N, N, lane);
     wave_sync();
     T lsum = T(0);
     for (int t = t0; t < t1; ++t)
@@ -1169,20 +1182,18 @@ __global__ void __launch_bounds__(256) k_wave_fwd(RArgs<T> A, int n_robots) {
         const T invN = T(1) / T(N);
         for (int i = lane; i < N; i += 64)
             for (int d = 0; d < 3; ++d) {
-                s6[d] += m.xs[d * N + i] * invN;
-                s6[3 + d] += m.vs[d * N + i] * invN;
+                s6[d] += m.xs[4 * i + d] * invN;
+                s6[3 + d] += m.vs[4 * i + d] * invN;
             }
         for (int q = 0; q < 6; ++q) s6[q] = wave_sum(s6[q]);
         if (A.loss_kind == 1) lsum += task_term(s6, tgt_at(R, R.Th));
         if (A.store_traj && R.traj) {
             T *row = R.traj + (size_t)R.Th * (3 * N + 3);
-            for (int i = lane; i < 3 * N; i += 64) row[i] = m.xs[i];
+            rows_to_soa(row, m.xs, N, lane);
         }
     }
-    for (int i = lane; i < 3 * N; i += 64) {
-        R.state[i] = m.xs[i];
-        R.state[3 * N + i] = m.vs[i];
-    }
+    rows_to_soa(R.state, m.xs, N, lane);
+    rows_to_soa(R.state + 3 * N, m.vs, N, lane);
     if (A.loss_kind == 1 && lane == 0) A.loss[r] += lsum / T(A.window);
 }
 
@@ -1200,11 +1211,11 @@ template <typename T>
 __device__ __forceinline__ void pref_load(const Rob<T> &R, int H, int lane, int slot, Pref<T> &p) {
     const int N = R.N, O = R.O;
     const T *sx = R.sx + (size_t)slot * 6 * N;
+    const int i = lane < N ? lane : 0;   // lane = vertex (N <= 64)
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        const int i = lane + 64 * q;
-        p.x[q] = i < 3 * N ? sx[i] : T(0);
-        p.v[q] = i < 3 * N ? sx[3 * N + i] : T(0);
+    for (int d = 0; d < 3; ++d) {
+        p.x[d] = sx[d * N + i];
+        p.v[d] = sx[3 * N + d * N + i];
     }
     p.out = lane < O ? R.sout[(size_t)slot * O + lane] : T(0);
     p.dout = lane < O ? R.sdout[(size_t)slot * O + lane] : T(0);
@@ -1229,14 +1240,10 @@ __device__ __forceinline__ void wave_bwd_step(const RArgs<T> &A, const Rob<T> &R
                               int t, int slot, Pref<T> &pf, const WRegs<T, 1, OMAX> &w,
                               const LaneTopo<T> &tp, int next_slot) {
     const int N = R.N, S = R.S, I = R.I, O = R.O, H = A.H;
-    // consume the prefetched step
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        const int i = lane + 64 * q;
-        if (i < 3 * N) {
-            m.xs[i] = pf.x[q];
-            m.vs[i] = pf.v[q];
-        }
+    // consume the prefetched step (lane = vertex)
+    if (lane < N) {
+        row3_put(m.xs, lane, pf.x[0], pf.x[1], pf.x[2]);
+        row3_put(m.vs, lane, pf.v[0], pf.v[1], pf.v[2]);
     }
     if (lane < O) {
         m.outb[lane] = pf.out;
@@ -1253,18 +1260,17 @@ __device__ __forceinline__ void wave_bwd_step(const RArgs<T> &A, const Rob<T> &R
     for (int i = lane; i < N; i += 64) {
         T vi[3];
         gather_J<DK>(R, m, tp, i, vi);
-        const T x[3] = {m.xs[i], m.xs[N + i], m.xs[2 * N + i]};
-        const T v[3] = {m.vs[i], m.vs[N + i], m.vs[2 * N + i]};
-        const T xnb[3] = {m.xb[i], m.xb[N + i], m.xb[2 * N + i]};
-        const T vnb[3] = {m.vb[i], m.vb[N + i], m.vb[2 * N + i]};
+        T x[3], v[3];
+        row3_get(m.xs, i, x[0], x[1], x[2]);
+        row3_get(m.vs, i, v[0], v[1], v[2]);
+        T xnb[3], vnb[3];
+        row3_get(m.xb, i, xnb[0], xnb[1], xnb[2]);
+        row3_get(m.vb, i, vnb[0], vnb[1], vnb[2]);
         T xb[3], vb[3], vib[3];
         integrate_adj<T>(x, v, vi, A.P, xnb, vnb, xb, vb, vib);
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            m.xb[d * N + i] = xb[d];
-            m.vb[d * N + i] = vb[d];
-            m.vib[d * N + i] = vib[d];
-        }
+        row3_put(m.xb, i, xb[0], xb[1], xb[2]);
+        row3_put(m.vb, i, vb[0], vb[1], vb[2]);
+        row3_put(m.vib, i, vib[0], vib[1], vib[2]);
     }
     wave_sync();
     // spring adjoints G, rows of 4: row 2 s the length-vector adjoint of
@@ -1276,9 +1282,16 @@ __device__ __forceinline__ void wave_bwd_step(const RArgs<T> &A, const Rob<T> &R
 #pragma unroll
     for (int o = 0; o < OMAX; ++o) abp[o] = T(0);
     auto adj_one = [&](int s, int a, int b, int ac, T L0, T kk, T co) {
-        const T d[3] = {m.xs[a] - m.xs[b], m.xs[N + a] - m.xs[N + b], m.xs[2 * N + a] - m.xs[2 * N + b]};
-        const T wv[3] = {m.vs[a] - m.vs[b], m.vs[N + a] - m.vs[N + b], m.vs[2 * N + a] - m.vs[2 * N + b]};
-        const T Jb[3] = {m.vib[b] - m.vib[a], m.vib[N + b] - m.vib[N + a], m.vib[2 * N + b] - m.vib[2 * N + a]};
+        T xa[3], xq[3], va[3], vq[3], ja[3], jq[3];
+        row3_get(m.xs, a, xa[0], xa[1], xa[2]);
+        row3_get(m.xs, b, xq[0], xq[1], xq[2]);
+        row3_get(m.vs, a, va[0], va[1], va[2]);
+        row3_get(m.vs, b, vq[0], vq[1], vq[2]);
+        row3_get(m.vib, a, ja[0], ja[1], ja[2]);
+        row3_get(m.vib, b, jq[0], jq[1], jq[2]);
+        const T d[3] = {xa[0] - xq[0], xa[1] - xq[1], xa[2] - xq[2]};
+        const T wv[3] = {va[0] - vq[0], va[1] - vq[1], va[2] - vq[2]};
+        const T Jb[3] = {jq[0] - ja[0], jq[1] - ja[1], jq[2] - ja[2]};
         T gd[3], gw[3], ga;
         spring_impulse_adj<T, true>(d, wv, L0, kk, co, m.outb[ac], A.P, Jb, gd, gw, ga);
         if (s < S) {
@@ -1328,8 +1341,8 @@ __device__ __forceinline__ void wave_bwd_step(const RArgs<T> &A, const Rob<T> &R
         }
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            m.xb[d * N + i] += gx[d];
-            m.vb[d * N + i] += gv[d];
+            m.xb[4 * i + d] += gx[d];
+            m.vb[4 * i + d] += gv[d];
         }
     }
 #pragma unroll
@@ -1399,7 +1412,7 @@ __device__ __forceinline__ void wave_bwd_step(const RArgs<T> &A, const Rob<T> &R
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const T g = m.inbar[base + i * vs + d] * inv;
-            m.xb[d * N + i] += g + (eg ? eg[d * N + i] : T(0));
+            m.xb[4 * i + d] += g + (eg ? eg[d * N + i] : T(0));
             cb[d] -= g;
         }
     }
@@ -1421,8 +1434,8 @@ __device__ __forceinline__ void wave_bwd_step(const RArgs<T> &A, const Rob<T> &R
     for (int i = lane; i < N; i += 64) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            m.xb[d * N + i] += cb[d] * invN;
-            m.vb[d * N + i] += vcb[d] * invN;
+            m.xb[4 * i + d] += cb[d] * invN;
+            m.vb[4 * i + d] += vcb[d] * invN;
         }
     }
     wave_sync();
@@ -1489,10 +1502,8 @@ template <typename T>
 __device__ __forceinline__ void wave_bwd_begin(const RArgs<T> &A, const Rob<T> &R, WSm<T> &m, int r,
                                                int lane) {
     const int N = R.N;
-    for (int i = lane; i < 3 * N; i += 64) {
-        m.xb[i] = R.adj[i];
-        m.vb[i] = R.adj[3 * N + i];
-    }
+    soa_to_rows(m.xb, R.adj, N, lane);
+    soa_to_rows(m.vb, R.adj + 3 * N, N, lane);
     wave_sync();
     if (A.robots[r].t_bwd == R.Th) {
         T s6[6] = {0, 0, 0, 0, 0, 0};
@@ -1514,8 +1525,8 @@ __device__ __forceinline__ void wave_bwd_begin(const RArgs<T> &A, const Rob<T> &
         const T *eg = R.trajg ? R.trajg + (size_t)R.Th * (3 * N + 3) : nullptr;
         for (int i = lane; i < N; i += 64)
             for (int d = 0; d < 3; ++d) {
-                m.xb[d * N + i] += cb[d] * invN + (eg ? eg[d * N + i] : T(0));
-                m.vb[d * N + i] += vcb[d] * invN;
+                m.xb[4 * i + d] += cb[d] * invN + (eg ? eg[d * N + i] : T(0));
+                m.vb[4 * i + d] += vcb[d] * invN;
             }
         wave_sync();
     }
@@ -1529,10 +1540,8 @@ __device__ __forceinline__ void wave_recompute(const RArgs<T> &A, const Rob<T> &
                                                const T *sin_tab, int lane, int s0, int t1) {
     const int N = R.N;
     const T *ck = R.ckpt + (size_t)(s0 / A.K) * 6 * N;
-    for (int i = lane; i < 3 * N; i += 64) {
-        m.xs[i] = ck[i];
-        m.vs[i] = ck[3 * N + i];
-    }
+    soa_to_rows(m.xs, ck, N, lane);
+    soa_to_rows(m.vs, ck + 3 * N, N, lane);
     wave_sync();
     T dummy = T(0);
     for (int t = s0; t < t1; ++t)
@@ -1571,10 +1580,8 @@ __device__ __forceinline__ void wave_bwd_segment(const RArgs<T> &A, const Rob<T>
 template <typename T>
 __device__ __forceinline__ void wave_bwd_end(const Rob<T> &R, const WSm<T> &m, int lane) {
     const int N = R.N;
-    for (int i = lane; i < 3 * N; i += 64) {
-        R.adj[i] = m.xb[i];
-        R.adj[3 * N + i] = m.vb[i];
-    }
+    rows_to_soa(R.adj, m.xb, N, lane);
+    rows_to_soa(R.adj + 3 * N, m.vb, N, lane);
 }
 
 // The whole adjoint of robot r by one wavefront: segments in turn, each
