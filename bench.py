@@ -31,17 +31,21 @@ HBM_PEAK_GBS = 8000.0
 UNEVEN_DASHPOT = 0.03
 
 
-def algorithmic_bytes(N, S, I, H, O, K, R_d, dtype_bytes=4):
+def algorithmic_bytes(N, S, I, H, O, K, R_d, dtype_bytes=4, stored=False):
     """Bytes per robot-step by kernel (state round-trip model): forward reads
     and writes x,v (48N) plus a checkpoint every K steps (24N/K) plus one pass
     over springs (20 B) and weights; the adjoint recomputes the forward (48N)
     and runs the adjoint (read x,v, read/write xbar,vbar: 72N) with two passes
-    over springs and weights.  Weight/topology bytes amortise over the R_d
-    robots sharing a design."""
+    over springs and weights.  With stored rows the forward also writes each
+    step's adjoint rows (x, v, network input, hidden values and derivatives,
+    outputs and derivatives, COM: 6N + I + 2H + 2O + 6 reals) and the adjoint
+    reads them instead of recomputing.  Weight/topology bytes amortise over
+    the R_d robots sharing a design."""
     P = I * H + H * O + H + O
     s = dtype_bytes / 4.0
-    fwd = (48 * N + 24 * N / K + (20 * S + 4 * P) / R_d) * s
-    bwd = (120 * N + (40 * S + 8 * P) / R_d) * s
+    rows = 4 * (6 * N + I + 2 * H + 2 * O + 6)
+    fwd = (48 * N + 24 * N / K + (20 * S + 4 * P) / R_d + (rows if stored else 0)) * s
+    bwd = ((72 * N + rows if stored else 120 * N) + (40 * S + 8 * P) / R_d) * s
     return fwd, bwd
 
 
@@ -54,6 +58,9 @@ def main():
     ap.add_argument("--horizon", type=int, default=1000)
     ap.add_argument("--ckpt", type=int, default=50)
     ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--variant", type=int, default=4,
+                    help="mass-spring kernel variant (0 auto, 3 pipelined segment recompute, "
+                         "4 every step's rows kept by the forward)")
     ap.add_argument("--plain-robots", type=int, default=2)
     ap.add_argument("--cpu-robots", type=int, default=96)
     ap.add_argument("--cpu-horizon", type=int, default=1000)
@@ -116,7 +123,8 @@ def main():
     tg = np.concatenate([tv, th[:, None], np.zeros((R, 1))], 1)
     x0 = design.objects + np.array([0.4, 0.0, 0.0])
     batch = MassSpringBatch([design], [0] * R, T, _physics(qc), lay, tg, [x0] * R, act="tanh",
-                            ckpt_every=K, dtype=torch.float32, block=args.block, loss_window=W)
+                            ckpt_every=K, dtype=torch.float32, block=args.block, loss_window=W,
+                            variant=args.variant)
     w = init_weights(I, H, O, seed=0)
     batch.set_weights(0, *w)
     lg = np.full(R, 1.0 / R)
@@ -147,14 +155,15 @@ def main():
     ms_per_step = wall * 1e3 / args.steps
     robot_steps = R * T
     value = robot_steps * args.steps / wall
-    f_b, b_b = algorithmic_bytes(N, S, I, H, O, K, R)
+    stored = batch.variant == 4
+    f_b, b_b = algorithmic_bytes(N, S, I, H, O, batch.K, R, stored=stored)
     bwd_avg_ms = float(np.mean(bwd_ms))
     fwd_avg_ms = float(np.mean(fwd_ms))
     achieved_bwd = b_b * robot_steps / (bwd_avg_ms * 1e-3) / 1e9
     achieved_fwd = f_b * robot_steps / (fwd_avg_ms * 1e-3) / 1e9
     # which kernels the launches used: variant 0 picks the wave kernels with the
     # pipelined adjoint when the robot fits them (as here), else the block kernels
-    wave = batch.variant in (2, 3) or (batch.variant == 0 and batch.wave_eligible())
+    wave = batch.variant in (2, 3, 4) or (batch.variant == 0 and batch.wave_eligible())
     kname = "k_wave" if wave else "k_rollout"
     kname_bwd = kname + ("_bwd_pipe" if wave and batch.variant != 2 else "_bwd")
     loss = float(batch.loss.double().mean().item())
@@ -342,7 +351,10 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded targets, N(0,1/fan_in) weights)",
         "config": {"workload": "quadruped mass-spring rollout fwd+adjoint, 1024 tasks x 1000 steps, "
-                   "ckpt 50, MLP 179-64-4 tanh", "robots": R, "horizon": T, "ckpt_every": K,
+                   "MLP 179-64-4 tanh, "
+                   + ("every step's adjoint rows kept by the forward (no recompute)" if stored
+                      else f"checkpoints every {batch.K} steps, segments recomputed"),
+                   "robots": R, "horizon": T, "ckpt_every": batch.K, "variant": batch.variant,
                    "n_vert": N, "n_spring": S, "mlp": [I, H, O], "block": args.block,
                    "form": "batched persistent (one launch forward, one launch adjoint)"},
         "roofline": {"bound": "hbm", "achieved": achieved_bwd, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -988,7 +988,10 @@ __device__ __forceinline__ void l1_row(float (&a)[4], const float *w1, float x,
 // one forward step of the wave's robot (MODE_FWD / MODE_REC like fwd_step).
 // WREG: layer-1 weights from registers (forward kernel) or from the shared
 // LDS copy of W1 (adjoint kernel, whose adjoint phase needs the registers).
-template <typename T, int IMAX, int OMAX, bool WREG, int SK, int DK>
+// ROWS: a forward step also writes the step's adjoint rows (as MODE_REC does)
+// at scratch slot `slot`, for the adjoint that reads them instead of
+// recomputing.
+template <typename T, int IMAX, int OMAX, bool WREG, int SK, int DK, bool ROWS = false>
 __device__ __forceinline__ void wave_fwd_step(const RArgs<T> &A, const Rob<T> &R, WSm<T> &m,
                               const WRegs<T, IMAX, OMAX> &w, const LaneTopo<T> &tp,
                               const T *sin_tab, int lane, int t, int mode, int slot, T &lsum) {
@@ -1017,7 +1020,9 @@ __device__ __forceinline__ void wave_fwd_step(const RArgs<T> &A, const Rob<T> &R
             if (lane < 3) row[3 * N + lane] = lane == 0 ? s6[0] : (lane == 1 ? s6[1] : s6[2]);
         }
         if (A.loss_kind == 1 && t > R.Th - A.window) lsum += task_term(s6, tg);
-    } else {
+    }
+    const bool rows = ROWS || mode == MODE_REC;
+    if (rows) {
         T *sx = R.sx + (size_t)slot * 6 * N;
         rows_to_soa(sx, m.xs, N, lane);
         rows_to_soa(sx + 3 * N, m.vs, N, lane);
@@ -1056,7 +1061,7 @@ __device__ __forceinline__ void wave_fwd_step(const RArgs<T> &A, const Rob<T> &R
         }
     }
     wave_sync();
-    if (mode == MODE_REC)
+    if (rows)
         for (int q = lane; q < I; q += 64) R.sin[(size_t)slot * I + q] = m.inb[q];
     // layer 1: lane j = hidden unit j, weights in registers, inputs broadcast from LDS
     T acc = w.b1;
@@ -1099,7 +1104,7 @@ __device__ __forceinline__ void wave_fwd_step(const RArgs<T> &A, const Rob<T> &R
     T hj, dhj;
     activate<T>(A.act, acc, hj, dhj);
     if (lane >= H) hj = dhj = T(0);
-    if (mode == MODE_REC && lane < H) {
+    if (rows && lane < H) {
         R.shid[(size_t)slot * H + lane] = hj;
         R.sdhid[(size_t)slot * H + lane] = dhj;
     }
@@ -1118,7 +1123,7 @@ __device__ __forceinline__ void wave_fwd_step(const RArgs<T> &A, const Rob<T> &R
         if (lane < O) {
             m.outb[lane] = y;
             m.dout[lane] = dy;
-            if (mode == MODE_REC) {
+            if (rows) {
                 R.sout[(size_t)slot * O + lane] = y;
                 R.sdout[(size_t)slot * O + lane] = dy;
             }
@@ -1141,7 +1146,9 @@ __device__ __forceinline__ void wave_fwd_step(const RArgs<T> &A, const Rob<T> &R
     wave_sync();
 }
 
-template <typename T, int IMAX, int OMAX>
+// ROWS: also write every step's adjoint rows into the robot's scratch (slot =
+// step; the scratch holds K >= horizon rows).
+template <typename T, int IMAX, int OMAX, bool ROWS>
 __global__ void __launch_bounds__(256) k_wave_fwd(RArgs<T> A, int n_robots) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1167,6 +1174,11 @@ __global__ void __launch_bounds__(256) k_wave_fwd(RArgs<T> A, int n_robots) {
     int t1 = R.Th;
     if (A.max_steps > 0 && t0 + A.max_steps < t1) t1 = t0 + A.max_steps;
     if (t0 >= t1) return;
+    if (ROWS && (R.Th > A.K || R.sx == nullptr)) {
+        // the stored rows need K >= horizon rows of scratch
+        if (lane == 0 && A.loss) A.loss[r] = T(NAN);
+        return;
+    }
     WRegs<T, IMAX, OMAX> w;
     wave_load_weights(R, A.H, lane, w);
     LaneTopo<T> tp;
@@ -1176,7 +1188,8 @@ __global__ void __launch_bounds__(256) k_wave_fwd(RArgs<T> A, int n_robots) {
     wave_sync();
     T lsum = T(0);
     for (int t = t0; t < t1; ++t)
-        wave_fwd_step<T, IMAX, OMAX, true, FSK, FDK>(A, R, m, w, tp, sh.sin_tab, lane, t, MODE_FWD, 0, lsum);
+        wave_fwd_step<T, IMAX, OMAX, true, FSK, FDK, ROWS>(A, R, m, w, tp, sh.sin_tab, lane, t, MODE_FWD,
+                                                           t, lsum);
     if (t1 == R.Th) {
         T s6[6] = {0, 0, 0, 0, 0, 0};
         const T invN = T(1) / T(N);
@@ -1654,7 +1667,13 @@ __device__ __forceinline__ int bwd_segments(int tb, int budget, int K) {
 // One workgroup barrier per segment; every wavefront runs the workgroup's
 // largest segment count, idle once its robot is done (or absent), so the
 // barrier counts always agree.
-template <typename T, int IMAX, int OMAX>
+// STORED: the forward wrote every step's rows (scratch of K >= horizon rows,
+// slot = step), so nothing is recomputed: the adjoint wavefront reverses the
+// horizon in chunks of kRowsChunk steps and the second wavefront folds the
+// weight gradient of the chunk reversed before.
+constexpr int kRowsChunk = 50;
+
+template <typename T, int IMAX, int OMAX, bool STORED>
 __global__ void __launch_bounds__(512) k_wave_bwd_pipe(RArgs<T> A, int n_robots) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int iters;
@@ -1662,7 +1681,7 @@ __global__ void __launch_bounds__(512) k_wave_bwd_pipe(RArgs<T> A, int n_robots)
     const int lane = threadIdx.x & 63;
     const int nrob = blockDim.x >> 7, rl = wid % nrob, role = wid / nrob;
     const int r0 = blockIdx.x * nrob;
-    const int K = A.K;
+    const int K = STORED ? kRowsChunk : A.K;   // steps per segment
     const int budget0 = A.max_steps > 0 ? A.max_steps : 0x7fffffff;
     const Rob<T> R0 = make_rob(A, r0);
     const size_t Rb = wave_region(A.max_n, A.max_s, A.max_i, A.max_o, true);
@@ -1678,8 +1697,10 @@ __global__ void __launch_bounds__(512) k_wave_bwd_pipe(RArgs<T> A, int n_robots)
     const int r = r0 + rl;
     bool active = r < n_robots;
     if (active && (A.robots[r].spring0 != A.robots[r0].spring0 ||
-                   A.robots[r].weights != A.robots[r0].weights)) {
-        // robots of one workgroup must share a design
+                   A.robots[r].weights != A.robots[r0].weights ||
+                   (STORED && A.robots[r].horizon > A.K))) {
+        // robots of one workgroup must share a design (and stored rows need
+        // K >= horizon rows of scratch)
         if (role == 0 && lane == 0) make_rob(A, r).Wg1[0] = T(NAN);
         active = false;
     }
@@ -1697,7 +1718,7 @@ __global__ void __launch_bounds__(512) k_wave_bwd_pipe(RArgs<T> A, int n_robots)
         wave_load_weights(R, A.H, lane, w);
         lane_topo_load(R, lane, tp);
         if (role == 0) wave_bwd_begin(A, R, m, r, lane);
-        else wave_recompute(A, R, m, w, tp, sh.sin_tab, lane, (tb - 1) / K * K, tb);
+        else if (!STORED) wave_recompute(A, R, m, w, tp, sh.sin_tab, lane, (tb - 1) / K * K, tb);
     }
     __syncthreads();
     int buf = 0;
@@ -1707,7 +1728,15 @@ __global__ void __launch_bounds__(512) k_wave_bwd_pipe(RArgs<T> A, int n_robots)
         const int s0 = go ? (tb - 1) / K * K : 0;
         const int lo = go ? max(s0, tb - budget) : 0;
         const int rest = go ? budget - (tb - lo) : 0;
-        if (role == 0) {
+        if constexpr (STORED) {
+            // one buffer of all the steps' rows, slots = steps
+            carve_scratch(R, scr0, A.K, A.H);
+            if (role == 0) {
+                if (go) wave_bwd_steps<T, IMAX, OMAX>(A, R, m, w, tp, r, lane, 0, lo, tb);
+            } else if (ftb > flo) {
+                wave_seg_wgrad(A, R, m.J, 3 * A.max_s, lane, flo, ftb - flo);
+            }
+        } else if (role == 0) {
             if (go) {
                 carve_scratch(R, buf ? scr1 : scr0, K, A.H);
                 wave_bwd_steps<T, IMAX, OMAX>(A, R, m, w, tp, r, lane, s0, lo, tb);
@@ -1735,8 +1764,13 @@ __global__ void __launch_bounds__(512) k_wave_bwd_pipe(RArgs<T> A, int n_robots)
     }
     if (work && role == 0) wave_bwd_end(R, m, lane);
     if (role == 1 && ftb > flo) {
-        carve_scratch(R, buf ? scr0 : scr1, K, A.H);
-        wave_seg_wgrad(A, R, m.J, 3 * A.max_s, lane, flo - fs0, ftb - flo);
+        if constexpr (STORED) {
+            carve_scratch(R, scr0, A.K, A.H);
+            wave_seg_wgrad(A, R, m.J, 3 * A.max_s, lane, flo, ftb - flo);
+        } else {
+            carve_scratch(R, buf ? scr0 : scr1, K, A.H);
+            wave_seg_wgrad(A, R, m.J, 3 * A.max_s, lane, flo - fs0, ftb - flo);
+        }
     }
 }
 
@@ -1794,7 +1828,7 @@ static int check_rollout(const dl_rollout *r, bool bwd) {
     DL_CHECK_ARG(r->max_n > 0 && r->max_s >= 0 && r->max_i > 0 && r->max_o > 0,
                  "rollout: bad size maxima");
     DL_CHECK_ARG(r->lds_policy >= 0 && r->lds_policy <= 4, "rollout: bad LDS policy");
-    DL_CHECK_ARG(r->variant >= 0 && r->variant <= 3, "rollout: bad kernel variant");
+    DL_CHECK_ARG(r->variant >= 0 && r->variant <= 4, "rollout: bad kernel variant");
     DL_CHECK_ARG(r->lay.n_tv >= 0 && r->lay.n_tv <= 3 && r->lay.run_period > 0, "rollout: bad layout");
     DL_CHECK_ARG(r->loss_kind == 0 || (r->loss_kind == 1 && r->loss_window > 0),
                  "rollout: bad loss configuration");
@@ -1811,8 +1845,34 @@ static int check_rollout(const dl_rollout *r, bool bwd) {
 
 static const size_t kLdsLimit = 160 * 1024;
 
+template <int IM>
+static int launch_wave_im(const dl_rollout *r, const RArgs<float> &A, bool bwd, bool pipe, bool stored,
+                          dim3 grid, dim3 block, size_t sh, hipStream_t st) {
+    const void *kf = pipe ? (stored ? (const void *)k_wave_bwd_pipe<float, IM, 8, true>
+                                    : (const void *)k_wave_bwd_pipe<float, IM, 8, false>)
+                     : bwd ? (const void *)k_wave_bwd<float, IM, 8>
+                           : (stored ? (const void *)k_wave_fwd<float, IM, 8, true>
+                                     : (const void *)k_wave_fwd<float, IM, 8, false>);
+    DL_CHECK_HIP(ensure_dynamic_lds(kf, sh));
+    if (pipe && stored)
+        hipLaunchKernelGGL((k_wave_bwd_pipe<float, IM, 8, true>), grid, block, sh, st, A, r->n_robots);
+    else if (pipe)
+        hipLaunchKernelGGL((k_wave_bwd_pipe<float, IM, 8, false>), grid, block, sh, st, A, r->n_robots);
+    else if (bwd)
+        hipLaunchKernelGGL((k_wave_bwd<float, IM, 8>), grid, block, sh, st, A, r->n_robots);
+    else if (stored)
+        hipLaunchKernelGGL((k_wave_fwd<float, IM, 8, true>), grid, block, sh, st, A, r->n_robots);
+    else
+        hipLaunchKernelGGL((k_wave_fwd<float, IM, 8, false>), grid, block, sh, st, A, r->n_robots);
+    DL_CHECK_LAUNCH();
+    return 0;
+}
+
+// stored: the forward writes every step's adjoint rows and the pipelined
+// adjoint reads them (no segment recompute).
 template <typename T>
-static int launch_wave(const dl_rollout *r, RArgs<T> A, bool bwd, bool pipelined, hipStream_t st) {
+static int launch_wave(const dl_rollout *r, RArgs<T> A, bool bwd, bool pipelined, bool stored,
+                       hipStream_t st) {
     if constexpr (sizeof(T) != 4) {
         DL_CHECK_ARG(false, "rollout: wave kernel is FP32 only");
     } else {
@@ -1824,28 +1884,9 @@ static int launch_wave(const dl_rollout *r, RArgs<T> A, bool bwd, bool pipelined
                                : wave_smem_bytes(r->max_n, r->max_s, r->max_i, r->max_o, bwd, n_tab,
                                                  waves);
         const dim3 grid((r->n_robots + waves - 1) / waves), block((pipe ? 128 : 64) * waves);
-#define DL_WAVE_LAUNCH(IM)                                                                      \
-    do {                                                                                        \
-        auto kf = pipe ? (const void *)k_wave_bwd_pipe<float, IM, 8>                            \
-                       : bwd ? (const void *)k_wave_bwd<float, IM, 8>                           \
-                             : (const void *)k_wave_fwd<float, IM, 8>;                          \
-        DL_CHECK_HIP(ensure_dynamic_lds(kf, sh));                                               \
-        if (pipe)                                                                               \
-            hipLaunchKernelGGL((k_wave_bwd_pipe<float, IM, 8>), grid, block, sh, st, A,         \
-                               r->n_robots);                                                    \
-        else if (bwd)                                                                           \
-            hipLaunchKernelGGL((k_wave_bwd<float, IM, 8>), grid, block, sh, st, A, r->n_robots); \
-        else                                                                                    \
-            hipLaunchKernelGGL((k_wave_fwd<float, IM, 8>), grid, block, sh, st, A, r->n_robots); \
-    } while (0)
-        if (r->max_i <= 64)
-            DL_WAVE_LAUNCH(64);
-        else if (r->max_i <= 128)
-            DL_WAVE_LAUNCH(128);
-        else
-            DL_WAVE_LAUNCH(192);
-#undef DL_WAVE_LAUNCH
-        DL_CHECK_LAUNCH();
+        if (r->max_i <= 64) return launch_wave_im<64>(r, A, bwd, pipe, stored, grid, block, sh, st);
+        if (r->max_i <= 128) return launch_wave_im<128>(r, A, bwd, pipe, stored, grid, block, sh, st);
+        return launch_wave_im<192>(r, A, bwd, pipe, stored, grid, block, sh, st);
     }
     return 0;
 }
@@ -1884,10 +1925,13 @@ static int launch_rollout(const dl_rollout *r, bool bwd, hipStream_t st) {
     const bool pipe_ok = wave_ok && r->scratch != nullptr && 3 * r->max_s >= HMAX &&
                          wave_pipe_smem_bytes(r->max_n, r->max_s, r->max_i, r->max_o, n_tab,
                                               r->block / 64) <= kLdsLimit;
-    DL_CHECK_ARG(variant != 3 || pipe_ok,
+    DL_CHECK_ARG((variant != 3 && variant != 4) || pipe_ok,
                  "rollout: the pipelined wave adjoint does not fit in LDS or has no scratch");
-    if (variant == 3 || (variant == 0 && pipe_ok)) return launch_wave<T>(r, A, bwd, true, st);
-    if (variant == 2 || (variant == 0 && wave_ok)) return launch_wave<T>(r, A, bwd, false, st);
+    // variant 4 keeps every step's rows: the forward needs the scratch too
+    DL_CHECK_ARG(variant != 4 || r->scratch != nullptr, "rollout: stored rows need the scratch buffer");
+    if (variant == 4) return launch_wave<T>(r, A, bwd, true, true, st);
+    if (variant == 3 || (variant == 0 && pipe_ok)) return launch_wave<T>(r, A, bwd, true, false, st);
+    if (variant == 2 || (variant == 0 && wave_ok)) return launch_wave<T>(r, A, bwd, false, false, st);
     size_t sh = bytes(springs, topo, wl);
     DL_CHECK_ARG(sh <= kLdsLimit, "rollout: robot too large for LDS (%zu bytes)", sh);
     DL_CHECK_ARG(springs || r->scratch != nullptr,

@@ -55,6 +55,11 @@ class MassSpringBatch:
                   [T_r,5] per-step arrays
     x0:           list of [N_r,3] initial positions (v0 = 0)
     loss_window:  >0 enables the in-kernel windowed COM loss
+    variant:      0 auto, 1 workgroup per robot, 2 wavefront per robot,
+                  3 as 2 with the adjoint's segment recompute on a second
+                  wavefront, 4 as 3 without recompute (the forward keeps
+                  every step's adjoint rows; ckpt_every becomes the longest
+                  horizon)
     """
 
     def __init__(self, designs, robot_design, horizons, physics, layout, targets, x0,
@@ -65,6 +70,10 @@ class MassSpringBatch:
         self.robot_design = np.asarray(robot_design, np.int64)
         R = self.R = len(self.robot_design)
         self.horizons = np.broadcast_to(np.asarray(horizons, np.int64), (R,)).copy()
+        if variant == 4:
+            # the forward keeps every step's adjoint rows: one scratch buffer
+            # (and one checkpoint interval) spans the longest horizon
+            ckpt_every = max(int(ckpt_every), int(self.horizons.max()))
         self.dtype, self.device = dtype, torch.device(device)
         self.H, self.K, self.block = n_hidden, ckpt_every, block
         self.lds_policy = lds_policy
@@ -237,7 +246,7 @@ class MassSpringBatch:
         D.max_n, D.max_s, D.max_i, D.max_o = self.max_n, self.max_s, self.max_i, self.max_o
         D.lds_policy = self.lds_policy
         D.variant = self.variant
-        if self.variant in (2, 3) or (self.variant == 0 and self.wave_eligible()):
+        if self.variant in (2, 3, 4) or (self.variant == 0 and self.wave_eligible()):
             # one wavefront per robot; 4 per workgroup when each aligned group of
             # 4 robots shares one design (topology and W1 are staged per workgroup)
             D.block = 256 if self.packed4() else 64

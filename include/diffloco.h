@@ -231,7 +231,11 @@ typedef struct dl_rollout {
     int32_t variant;         /* 0 auto, 1 workgroup per robot, 2 wavefront per robot (FP32,
                                 H = 64, I <= 192, O <= 8; robots of one design packed by 4),
                                 3 as 2 with the adjoint's segment recompute on a second
-                                wavefront per robot, pipelined over two scratch buffers */
+                                wavefront per robot, pipelined over two scratch buffers,
+                                4 as 3 with no recompute: the forward writes every step's
+                                adjoint rows into scratch buffer 0, which then needs
+                                ckpt_every >= every horizon (a robot whose horizon
+                                exceeds it gets a NaN loss / gradient) */
     dl_ms_physics phys;
     dl_sensor_layout lay;
     const dl_robot *robots;  /* device [n_robots] */

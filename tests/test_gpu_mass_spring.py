@@ -315,10 +315,11 @@ def test_batched_deterministic():
     assert torch.equal(outs[0][1], outs[1][1])
 
 
-@pytest.mark.parametrize("K,variant", [(16, 2), (7, 2), (16, 3), (7, 3)])
+@pytest.mark.parametrize("K,variant", [(16, 2), (7, 2), (16, 3), (7, 3), (16, 4)])
 def test_wave_kernel_matches_oracle(K, variant):
     """Wave-per-robot kernel (FP32, W1 column in VGPRs) against the FP64 oracle;
-    variant 3 runs the adjoint's segment recompute on a second wavefront."""
+    variant 3 runs the adjoint's segment recompute on a second wavefront,
+    variant 4 keeps the forward's rows instead of recomputing."""
     _say(f"wave quad K={K} variant={variant}")
     B, T, W = 6, 90, 40
     tv, th = task_targets(B, 21)
@@ -346,7 +347,10 @@ def test_wave_kernel_matches_oracle(K, variant):
 def test_pipelined_adjoint_bit_identical_to_sequential(budget):
     """The adjoint with the segment recompute on a second wavefront per robot
     (variant 3) gives the sequential wave adjoint's (variant 2) gradients and
-    adjoint state bit for bit: one design, horizons 45..90 mixed inside a
+    adjoint state bit for bit; the one reading the rows the forward kept
+    (variant 4, no recompute) gives the same adjoint state bit for bit and the
+    same weight gradients up to the grouping of the per-segment sums (it folds
+    every 50 steps, not every K): one design, horizons 45..90 mixed inside a
     workgroup (2 to 6 segments of K = 16), a partly filled second workgroup
     (6 robots, 4 per workgroup), whole and step-budgeted backward calls."""
     _say(f"pipelined vs sequential adjoint budget={budget}")
@@ -360,7 +364,7 @@ def test_pipelined_adjoint_bit_identical_to_sequential(budget):
     x0 = design.objects + np.array([0.4, 0.0, 0.0])
     tg = np.concatenate([tv, th[:, None], np.zeros((B, 1))], 1)
     out = []
-    for variant in (2, 3):
+    for variant in (2, 3, 4):
         batch = MassSpringBatch([design], [0] * B, horizons, _physics(qc), sensor_layout(qc), tg,
                                 [x0] * B, act="tanh", ckpt_every=K, dtype=torch.float32,
                                 loss_window=W, variant=variant)
@@ -378,11 +382,12 @@ def test_pipelined_adjoint_bit_identical_to_sequential(budget):
             batch.backward(loss_grad=np.linspace(0.5, 1.5, B))
         torch.cuda.synchronize()
         out.append(([g.clone() for g in batch.weight_grads(0)], batch.adjoint.clone()))
-    (ga, aa), (gb, ab) = out
+    ga, aa = out[0]
     assert torch.isfinite(aa).all()
-    for x, y in zip(ga, gb):
-        assert torch.equal(x, y)
-    assert torch.equal(aa, ab)
+    for v, (gb, ab) in zip((3, 4), out[1:]):
+        for x, y in zip(ga, gb):
+            assert torch.equal(x, y) if v == 3 else rel_err(y.cpu(), x.cpu()) < 1e-5
+        assert torch.equal(aa, ab)
 
 
 @pytest.mark.parametrize("variant", [2, 3])
