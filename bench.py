@@ -37,13 +37,13 @@ def algorithmic_bytes(N, S, I, H, O, K, R_d, dtype_bytes=4, stored=False):
     over springs (20 B) and weights; the adjoint recomputes the forward (48N)
     and runs the adjoint (read x,v, read/write xbar,vbar: 72N) with two passes
     over springs and weights.  With stored rows the forward also writes each
-    step's adjoint rows (x, v, network input, hidden values and derivatives,
-    outputs and derivatives, COM: 6N + I + 2H + 2O + 6 reals) and the adjoint
-    reads them instead of recomputing.  Weight/topology bytes amortise over
-    the R_d robots sharing a design."""
+    step's adjoint rows (x, v, per-vertex impulse sums, network input, hidden
+    values and derivatives, outputs and derivatives, COM: 9N + I + 2H + 2O + 6
+    reals) and the adjoint reads them instead of recomputing.  Weight/topology
+    bytes amortise over the R_d robots sharing a design."""
     P = I * H + H * O + H + O
     s = dtype_bytes / 4.0
-    rows = 4 * (6 * N + I + 2 * H + 2 * O + 6)
+    rows = 4 * (9 * N + I + 2 * H + 2 * O + 6)
     fwd = (48 * N + 24 * N / K + (20 * S + 4 * P) / R_d + (rows if stored else 0)) * s
     bwd = ((72 * N + rows if stored else 120 * N) + (40 * S + 8 * P) / R_d) * s
     return fwd, bwd

@@ -57,7 +57,7 @@ struct Rob {
     const T *trajg;
     // scratch: rows of a segment of K steps (buffer 0), the spring spill
     // area, and a second buffer of K rows (scr2) for the pipelined adjoint
-    T *sx, *sin, *shid, *sdhid, *sout, *sdout, *scv, *sz1, *sz2, *spill, *scr2;
+    T *sx, *svi, *sin, *shid, *sdhid, *sout, *sdout, *scv, *sz1, *sz2, *spill, *scr2;
 };
 
 // Point the scratch rows of q at a buffer of K rows starting at s.
@@ -65,6 +65,7 @@ template <typename T>
 __device__ __forceinline__ void carve_scratch(Rob<T> &q, T *s, int K, int H) {
     const int N = q.N, I = q.I, O = q.O;
     q.sx = s; s += (size_t)K * 6 * N;
+    q.svi = s; s += (size_t)K * 3 * N;
     q.sin = s; s += (size_t)K * I;
     q.shid = s; s += (size_t)K * H;
     q.sdhid = s; s += (size_t)K * H;
@@ -105,7 +106,7 @@ __device__ __forceinline__ Rob<T> make_rob(const RArgs<T> &A, int r) {
         q.spill = q.scv + (size_t)K * 6;
         q.scr2 = q.spill + (size_t)10 * q.S;
     } else {
-        q.sx = q.sin = q.shid = q.sdhid = q.sz1 = q.sout = q.sdout = q.sz2 = q.scv = q.spill = nullptr;
+        q.sx = q.svi = q.sin = q.shid = q.sdhid = q.sz1 = q.sout = q.sdout = q.sz2 = q.scv = q.spill = nullptr;
         q.scr2 = nullptr;
     }
     return q;
@@ -1135,6 +1136,11 @@ __device__ __forceinline__ void wave_fwd_step(const RArgs<T> &A, const Rob<T> &R
     for (int i = lane; i < N; i += 64) {
         T vi[3];
         gather_J<DK>(R, m, tp, i, vi);
+        if constexpr (ROWS) {
+            // kept for the adjoint, which then skips the spring pass
+            T *sv = R.svi + (size_t)slot * 3 * N;
+            sv[i] = vi[0]; sv[N + i] = vi[1]; sv[2 * N + i] = vi[2];
+        }
         T x[3], v[3];
         row3_get(m.xs, i, x[0], x[1], x[2]);
         row3_get(m.vs, i, v[0], v[1], v[2]);
@@ -1217,10 +1223,11 @@ __global__ void __launch_bounds__(256) k_wave_fwd(RArgs<T> A, int n_robots) {
 template <typename T>
 struct Pref {
     T x[3], v[3];
+    T vi[3];            // impulse sums (stored rows only)
     T out, dout, dh, cv;
 };
 
-template <typename T>
+template <bool VI = false, typename T>
 __device__ __forceinline__ void pref_load(const Rob<T> &R, int H, int lane, int slot, Pref<T> &p) {
     const int N = R.N, O = R.O;
     const T *sx = R.sx + (size_t)slot * 6 * N;
@@ -1229,6 +1236,11 @@ __device__ __forceinline__ void pref_load(const Rob<T> &R, int H, int lane, int 
     for (int d = 0; d < 3; ++d) {
         p.x[d] = sx[d * N + i];
         p.v[d] = sx[3 * N + d * N + i];
+    }
+    if constexpr (VI) {
+        const T *sv = R.svi + (size_t)slot * 3 * N;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) p.vi[d] = sv[d * N + i];
     }
     p.out = lane < O ? R.sout[(size_t)slot * O + lane] : T(0);
     p.dout = lane < O ? R.sdout[(size_t)slot * O + lane] : T(0);
@@ -1248,7 +1260,9 @@ __device__ __forceinline__ void in_adj_rows(float (&acc)[NP], const float *const
     (one(row_bcast<K>(z), K), ...);
 }
 
-template <typename T, int IMAX, int OMAX, int SK, int DK>
+// VI: the step's impulse sums come from the rows the forward kept (no spring
+// pass and gather here).
+template <typename T, int IMAX, int OMAX, int SK, int DK, bool VI = false>
 __device__ __forceinline__ void wave_bwd_step(const RArgs<T> &A, const Rob<T> &R, WSm<T> &m, int r, int lane,
                               int t, int slot, Pref<T> &pf, const WRegs<T, 1, OMAX> &w,
                               const LaneTopo<T> &tp, int next_slot) {
@@ -1266,13 +1280,20 @@ __device__ __forceinline__ void wave_bwd_step(const RArgs<T> &A, const Rob<T> &R
 #pragma unroll
     for (int q = 0; q < 6; ++q) cvr[q] = lane_read(pf.cv, q);
     const T dhj = pf.dh, pf_dout = pf.dout;
-    if (next_slot >= 0) pref_load(R, H, lane, next_slot, pf);   // in flight during this step
+    const T pvi[3] = {pf.vi[0], pf.vi[1], pf.vi[2]};
+    if (next_slot >= 0) pref_load<VI>(R, H, lane, next_slot, pf);   // in flight during this step
     wave_sync();
-    spring_impulses<SK>(A, R, m, tp, lane);
-    wave_sync();
+    if constexpr (!VI) {
+        spring_impulses<SK>(A, R, m, tp, lane);
+        wave_sync();
+    }
     for (int i = lane; i < N; i += 64) {
         T vi[3];
-        gather_J<DK>(R, m, tp, i, vi);
+        if constexpr (VI) {
+            vi[0] = pvi[0]; vi[1] = pvi[1]; vi[2] = pvi[2];   // lane = vertex (N <= 64)
+        } else {
+            gather_J<DK>(R, m, tp, i, vi);
+        }
         T x[3], v[3];
         row3_get(m.xs, i, x[0], x[1], x[2]);
         row3_get(m.vs, i, v[0], v[1], v[2]);
@@ -1566,15 +1587,15 @@ __device__ __forceinline__ void wave_recompute(const RArgs<T> &A, const Rob<T> &
 
 // Adjoint steps tb - 1 down to lo of the segment starting at s0 (rows of R
 // recomputed); the steps' weight-gradient rows are left in R's scratch.
-template <typename T, int IMAX, int OMAX>
+template <typename T, int IMAX, int OMAX, bool VI = false>
 __device__ __forceinline__ void wave_bwd_steps(const RArgs<T> &A, const Rob<T> &R, WSm<T> &m,
                                                const WRegs<T, 1, OMAX> &w, const LaneTopo<T> &tp,
                                                int r, int lane, int s0, int lo, int tb) {
     Pref<T> pf;
-    pref_load(R, A.H, lane, tb - 1 - s0, pf);
+    pref_load<VI>(R, A.H, lane, tb - 1 - s0, pf);
     for (int t = tb - 1; t >= lo; --t)
-        wave_bwd_step<T, IMAX, OMAX, BSK, BDK>(A, R, m, r, lane, t, t - s0, pf, w, tp,
-                                               t - 1 >= lo ? t - 1 - s0 : -1);
+        wave_bwd_step<T, IMAX, OMAX, BSK, BDK, VI>(A, R, m, r, lane, t, t - s0, pf, w, tp,
+                                                   t - 1 >= lo ? t - 1 - s0 : -1);
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
     wave_sync();
 }
@@ -1732,7 +1753,7 @@ __global__ void __launch_bounds__(512) k_wave_bwd_pipe(RArgs<T> A, int n_robots)
             // one buffer of all the steps' rows, slots = steps
             carve_scratch(R, scr0, A.K, A.H);
             if (role == 0) {
-                if (go) wave_bwd_steps<T, IMAX, OMAX>(A, R, m, w, tp, r, lane, 0, lo, tb);
+                if (go) wave_bwd_steps<T, IMAX, OMAX, true>(A, R, m, w, tp, r, lane, 0, lo, tb);
             } else if (ftb > flo) {
                 wave_seg_wgrad(A, R, m.J, 3 * A.max_s, lane, flo, ftb - flo);
             }
@@ -1967,7 +1988,7 @@ int64_t dl_wgrad_size(int32_t I, int32_t H, int32_t O) {
 }
 
 int64_t dl_scratch_size(int32_t N, int32_t S, int32_t I, int32_t H, int32_t O, int32_t K) {
-    return 2 * (int64_t)K * (6 * (int64_t)N + I + 3 * (int64_t)H + 3 * (int64_t)O + 6) +
+    return 2 * (int64_t)K * (9 * (int64_t)N + I + 3 * (int64_t)H + 3 * (int64_t)O + 6) +
            10 * (int64_t)S;
 }
 

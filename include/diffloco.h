@@ -210,8 +210,11 @@ typedef struct dl_robot {
 int64_t dl_weights_size(int32_t I, int32_t H, int32_t O);
 int64_t dl_wgrad_size(int32_t I, int32_t H, int32_t O);
 /* Elements of per-robot scratch for checkpoint interval K: two buffers of K
- * steps' recompute rows (the pipelined adjoint fills one while it reverses
- * the other) and a spring spill area. */
+ * steps' adjoint rows -- state (6N), per-vertex impulse sums (3N, written by
+ * the stored-rows forward only), network input (I), hidden values,
+ * derivatives and adjoints (3H), outputs (3O), COM row (6); the pipelined
+ * adjoint fills one buffer while it reverses the other -- and a spring spill
+ * area (10S). */
 int64_t dl_scratch_size(int32_t N, int32_t S, int32_t I, int32_t H, int32_t O, int32_t K);
 
 typedef struct dl_rollout {

@@ -75,7 +75,7 @@ def test_size_helpers():
     L = nat.lib()
     assert L.dl_weights_size(179, 64, 4) == 2 * 64 * 179 + 64 + 2 * 4 * 64 + 4
     assert L.dl_wgrad_size(179, 64, 4) == 64 * 179 + 64 + 4 * 64 + 4
-    assert L.dl_scratch_size(54, 323, 179, 64, 4, 50) == 2 * 50 * (6 * 54 + 179 + 3 * 64 + 3 * 4 + 6) + 10 * 323
+    assert L.dl_scratch_size(54, 323, 179, 64, 4, 50) == 2 * 50 * (9 * 54 + 179 + 3 * 64 + 3 * 4 + 6) + 10 * 323
 
 
 def test_invalid_arguments_are_rejected_without_a_gpu():
