@@ -714,8 +714,8 @@ __host__ __device__ inline size_t wave_region(int max_n, int max_s, int max_i, i
 
 __host__ inline size_t wave_smem_bytes(int max_n, int max_s, int max_i, int max_o, bool bwd,
                                        int n_tab, int waves) {
-    size_t n = waves * wave_region(max_n, max_s, max_i, max_o, bwd) + 3 * (size_t)max_s + n_tab;
-    if (bwd) n += (size_t)max_i * (HMAX + 1);
+    size_t n = waves * wave_region(max_n, max_s, max_i, max_o, bwd) + 3 * (size_t)max_s + n_tab +
+               (size_t)max_i * (HMAX + 1);
     return n * sizeof(float) + sizeof(int) * (6 * (size_t)max_s + max_n + max_o + 2);
 }
 
@@ -760,8 +760,8 @@ __device__ __forceinline__ WSm<T> wave_carve(const RArgs<T> &A, T *base, int wid
                          bwd);
 }
 
-// Block-wide: stage the topology of robot r0's design (and W1 for the
-// adjoint) into the shared region; returns pointers through `sh`.
+// Block-wide: stage the topology of robot r0's design and its W1 (rows of
+// pitch HMAX + 1) into the shared region; returns pointers through `sh`.
 template <typename T>
 struct WShared {
     T *topo_r, *wl, *sin_tab;
@@ -775,8 +775,8 @@ __device__ WShared<T> wave_stage(const RArgs<T> &A, const Rob<T> &R0, unsigned c
     WShared<T> sh;
     T *p = reinterpret_cast<T *>(raw) + regions;
     sh.topo_r = p; p += 3 * A.max_s;
-    sh.wl = bwd ? p : nullptr;
-    if (bwd) p += (size_t)A.max_i * (HMAX + 1);
+    sh.wl = p;
+    p += (size_t)A.max_i * (HMAX + 1);
     // sinusoid table over one period: phase 2 pi (t mod P) / P + 2 pi j / n_sin
     // evaluated in double precision (periodic in t, so equal to the direct form)
     const int n_tab = A.L.run_period * A.L.n_sin;
@@ -800,7 +800,7 @@ __device__ WShared<T> wave_stage(const RArgs<T> &A, const Rob<T> &R0, unsigned c
     int *ap = vx + nv, *ax = ap + O + 1;
     for (int q = tid; q <= O; q += BT) ap[q] = R0.aptr[q] - a0;
     for (int q = tid; q < na; q += BT) ax[q] = R0.aidx[a0 + q];
-    if (bwd) {
+    {
         const int H = A.H, I = R0.I;
         for (int q = tid; q < I * H; q += BT) {
             const int i = q / H, j = q - i * H;
@@ -820,7 +820,7 @@ __device__ __forceinline__ void wave_point_at_shared(Rob<T> &R, const WShared<T>
     R.vidx = R.vptr + N + 1;
     R.aptr = R.vidx + nv;
     R.aidx = R.aptr + O + 1;
-    if (bwd) R.WL = sh.wl;
+    R.WL = sh.wl;
 }
 
 template <typename T, int IMAX, int OMAX>
@@ -978,23 +978,13 @@ __device__ __forceinline__ void gather_J(const Rob<T> &R, const WSm<T> &m, const
     }
 }
 
-// a[k % 4] += w1[k] * (input k of the row block x), k = 0..15, input k taken
-// from lane k of the lane's 16-lane row.
-template <int... K>
-__device__ __forceinline__ void l1_row(float (&a)[4], const float *w1, float x,
-                                       std::integer_sequence<int, K...>) {
-    ((a[K & 3] += w1[K] * row_bcast<K>(x)), ...);
-}
-
 // one forward step of the wave's robot (MODE_FWD / MODE_REC like fwd_step).
-// WREG: layer-1 weights from registers (forward kernel) or from the shared
-// LDS copy of W1 (adjoint kernel, whose adjoint phase needs the registers).
 // ROWS: a forward step also writes the step's adjoint rows (as MODE_REC does)
 // at scratch slot `slot`, for the adjoint that reads them instead of
 // recomputing.
-template <typename T, int IMAX, int OMAX, bool WREG, int SK, int DK, bool ROWS = false>
+template <typename T, int OMAX, int SK, int DK, bool ROWS = false>
 __device__ __forceinline__ void wave_fwd_step(const RArgs<T> &A, const Rob<T> &R, WSm<T> &m,
-                              const WRegs<T, IMAX, OMAX> &w, const LaneTopo<T> &tp,
+                              const WRegs<T, 1, OMAX> &w, const LaneTopo<T> &tp,
                               const T *sin_tab, int lane, int t, int mode, int slot, T &lsum) {
     const int N = R.N, S = R.S, I = R.I, O = R.O, H = A.H;
     T s6[6] = {0, 0, 0, 0, 0, 0};
@@ -1064,35 +1054,12 @@ __device__ __forceinline__ void wave_fwd_step(const RArgs<T> &A, const Rob<T> &R
     wave_sync();
     if (rows)
         for (int q = lane; q < I; q += 64) R.sin[(size_t)slot * I + q] = m.inb[q];
-    // layer 1: lane j = hidden unit j, weights in registers, inputs broadcast from LDS
+    // layer 1: lane j = hidden unit j, W1 column j from the LDS copy (H = HMAX
+    // in the wave kernels), inputs broadcast from LDS, input i into a[i % 4].
+    // The forward and the segment recompute run this same code, so the
+    // recompute reproduces the forward's hidden values bit for bit.
     T acc = w.b1;
-    if constexpr (WREG && std::is_same<T, float>::value && IMAX % 16 == 0) {
-        // weights beyond I are zero and the input tail is zero-filled: no guards.
-        // Inputs 16q .. 16q+15 sit in every 16-lane row of xin[q] (one LDS read
-        // per 16 inputs, all issued together); each FMA takes its input by a
-        // row broadcast, so the loop has no LDS round trips.
-        float xin[IMAX / 16];
-#pragma unroll
-        for (int q = 0; q < IMAX / 16; ++q) xin[q] = m.inb[16 * q + (lane & 15)];
-        float a[4] = {acc, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int q = 0; q < IMAX / 16; ++q)
-            l1_row(a, w.w1 + 16 * q, xin[q], std::make_integer_sequence<int, 16>{});
-        acc = (a[0] + a[1]) + (a[2] + a[3]);
-    } else if constexpr (WREG) {
-        T a1 = T(0), a2 = T(0), a3 = T(0);
-#pragma unroll
-        for (int i = 0; i < IMAX; i += 4) {
-            acc += w.w1[i] * m.inb[i];
-            a1 += w.w1[i + 1] * m.inb[i + 1];
-            a2 += w.w1[i + 2] * m.inb[i + 2];
-            a3 += w.w1[i + 3] * m.inb[i + 3];
-        }
-        acc = (acc + a1) + (a2 + a3);
-    } else if (R.WL) {
-        // W1 column `lane` from the LDS copy (H = HMAX in the wave kernels),
-        // in the register form's accumulation order (input i into a[i % 4]),
-        // so the recompute reproduces the forward's hidden values bit for bit
+    if (R.WL) {
         T a[4] = {acc, T(0), T(0), T(0)};
         const T *wl = R.WL + lane;
 #pragma unroll 16
@@ -1154,7 +1121,7 @@ __device__ __forceinline__ void wave_fwd_step(const RArgs<T> &A, const Rob<T> &R
 
 // ROWS: also write every step's adjoint rows into the robot's scratch (slot =
 // step; the scratch holds K >= horizon rows).
-template <typename T, int IMAX, int OMAX, bool ROWS>
+template <typename T, int OMAX, bool ROWS>
 __global__ void __launch_bounds__(256) k_wave_fwd(RArgs<T> A, int n_robots) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1185,7 +1152,7 @@ __global__ void __launch_bounds__(256) k_wave_fwd(RArgs<T> A, int n_robots) {
         if (lane == 0 && A.loss) A.loss[r] = T(NAN);
         return;
     }
-    WRegs<T, IMAX, OMAX> w;
+    WRegs<T, 1, OMAX> w;
     wave_load_weights(R, A.H, lane, w);
     LaneTopo<T> tp;
     lane_topo_load(R, lane, tp);
@@ -1194,8 +1161,7 @@ __global__ void __launch_bounds__(256) k_wave_fwd(RArgs<T> A, int n_robots) {
     wave_sync();
     T lsum = T(0);
     for (int t = t0; t < t1; ++t)
-        wave_fwd_step<T, IMAX, OMAX, true, FSK, FDK, ROWS>(A, R, m, w, tp, sh.sin_tab, lane, t, MODE_FWD,
-                                                           t, lsum);
+        wave_fwd_step<T, OMAX, FSK, FDK, ROWS>(A, R, m, w, tp, sh.sin_tab, lane, t, MODE_FWD, t, lsum);
     if (t1 == R.Th) {
         T s6[6] = {0, 0, 0, 0, 0, 0};
         const T invN = T(1) / T(N);
@@ -1579,7 +1545,7 @@ __device__ __forceinline__ void wave_recompute(const RArgs<T> &A, const Rob<T> &
     wave_sync();
     T dummy = T(0);
     for (int t = s0; t < t1; ++t)
-        wave_fwd_step<T, 1, OMAX, false, BSK, BDK>(A, R, m, w, tp, sin_tab, lane, t, MODE_REC,
+        wave_fwd_step<T, OMAX, BSK, BDK>(A, R, m, w, tp, sin_tab, lane, t, MODE_REC,
                                                    t - s0, dummy);
     // the scratch stores complete before any lane reads them back
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
@@ -1872,8 +1838,8 @@ static int launch_wave_im(const dl_rollout *r, const RArgs<float> &A, bool bwd, 
     const void *kf = pipe ? (stored ? (const void *)k_wave_bwd_pipe<float, IM, 8, true>
                                     : (const void *)k_wave_bwd_pipe<float, IM, 8, false>)
                      : bwd ? (const void *)k_wave_bwd<float, IM, 8>
-                           : (stored ? (const void *)k_wave_fwd<float, IM, 8, true>
-                                     : (const void *)k_wave_fwd<float, IM, 8, false>);
+                           : (stored ? (const void *)k_wave_fwd<float, 8, true>
+                                     : (const void *)k_wave_fwd<float, 8, false>);
     DL_CHECK_HIP(ensure_dynamic_lds(kf, sh));
     if (pipe && stored)
         hipLaunchKernelGGL((k_wave_bwd_pipe<float, IM, 8, true>), grid, block, sh, st, A, r->n_robots);
@@ -1882,9 +1848,9 @@ static int launch_wave_im(const dl_rollout *r, const RArgs<float> &A, bool bwd, 
     else if (bwd)
         hipLaunchKernelGGL((k_wave_bwd<float, IM, 8>), grid, block, sh, st, A, r->n_robots);
     else if (stored)
-        hipLaunchKernelGGL((k_wave_fwd<float, IM, 8, true>), grid, block, sh, st, A, r->n_robots);
+        hipLaunchKernelGGL((k_wave_fwd<float, 8, true>), grid, block, sh, st, A, r->n_robots);
     else
-        hipLaunchKernelGGL((k_wave_fwd<float, IM, 8, false>), grid, block, sh, st, A, r->n_robots);
+        hipLaunchKernelGGL((k_wave_fwd<float, 8, false>), grid, block, sh, st, A, r->n_robots);
     DL_CHECK_LAUNCH();
     return 0;
 }

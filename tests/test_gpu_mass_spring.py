@@ -317,7 +317,7 @@ def test_batched_deterministic():
 
 @pytest.mark.parametrize("K,variant", [(16, 2), (7, 2), (16, 3), (7, 3), (16, 4)])
 def test_wave_kernel_matches_oracle(K, variant):
-    """Wave-per-robot kernel (FP32, W1 column in VGPRs) against the FP64 oracle;
+    """Wave-per-robot kernel (FP32, one wavefront per robot) against the FP64 oracle;
     variant 3 runs the adjoint's segment recompute on a second wavefront,
     variant 4 keeps the forward's rows instead of recomputing."""
     _say(f"wave quad K={K} variant={variant}")
