@@ -978,6 +978,42 @@ __device__ __forceinline__ void gather_J(const Rob<T> &R, const WSm<T> &m, const
     }
 }
 
+// The network input of step t (layout of taichi_env.py:149-183) into m.inb,
+// assembled per block of slots (no per-slot division).
+template <typename T>
+__device__ __forceinline__ void wave_sensors(const RArgs<T> &A, const Rob<T> &R, WSm<T> &m,
+                                             const T *sin_tab, int lane, int t, T c0, T c1, T c2,
+                                             T u0, T u1, T u2, const T *tg) {
+    // c: centre of mass, u: its velocity
+    const int N = R.N, I = R.I;
+    {
+        const Layout &L = A.L;
+        const int tmod = t % L.run_period;
+        for (int j = lane; j < L.n_sin; j += 64) m.inb[j] = sin_tab[tmod * L.n_sin + j];
+        if (L.vcom && lane < 3) m.inb[L.n_sin + lane] = lane == 0 ? u0 : (lane == 1 ? u1 : u2);
+        const int base = L.vert_base(), vs = L.vstride();
+        const T isc = T(1) / T(L.offset_scale);
+        for (int i = lane; i < N; i += 64) {
+            T *o = m.inb + base + i * vs;
+            o[0] = (m.xs[4 * i] - c0) * isc;
+            o[1] = (m.xs[4 * i + 1] - c1) * isc;
+            o[2] = (m.xs[4 * i + 2] - c2) * isc;
+            for (int d = 3; d < vs; ++d) o[d] = T(0);
+        }
+        const int tail = base + N * vs, nv = L.dup_v * L.n_tv;
+        for (int q = lane; q < I - tail; q += 64) {
+            T val;
+            if (q < nv) {
+                const int mm = q % L.n_tv;
+                val = tg[mm == 0 ? L.tv0 : (mm == 1 ? L.tv1 : L.tv2)] / T(L.max_speed);
+            } else {
+                val = q < nv + L.dup_h ? tg[3] : tg[4];
+            }
+            m.inb[tail + q] = val;
+        }
+    }
+}
+
 // one forward step of the wave's robot (MODE_FWD / MODE_REC like fwd_step).
 // ROWS: a forward step also writes the step's adjoint rows (as MODE_REC does)
 // at scratch slot `slot`, for the adjoint that reads them instead of
@@ -1024,33 +1060,7 @@ __device__ __forceinline__ void wave_fwd_step(const RArgs<T> &A, const Rob<T> &R
             R.scv[slot * 6 + lane] = val;
         }
     }
-    // sensor vector, assembled per block of slots (no per-slot division)
-    {
-        const Layout &L = A.L;
-        const int tmod = t % L.run_period;
-        for (int j = lane; j < L.n_sin; j += 64) m.inb[j] = sin_tab[tmod * L.n_sin + j];
-        if (L.vcom && lane < 3) m.inb[L.n_sin + lane] = lane == 0 ? s6[3] : (lane == 1 ? s6[4] : s6[5]);
-        const int base = L.vert_base(), vs = L.vstride();
-        const T isc = T(1) / T(L.offset_scale);
-        for (int i = lane; i < N; i += 64) {
-            T *o = m.inb + base + i * vs;
-            o[0] = (m.xs[4 * i] - s6[0]) * isc;
-            o[1] = (m.xs[4 * i + 1] - s6[1]) * isc;
-            o[2] = (m.xs[4 * i + 2] - s6[2]) * isc;
-            for (int d = 3; d < vs; ++d) o[d] = T(0);
-        }
-        const int tail = base + N * vs, nv = L.dup_v * L.n_tv;
-        for (int q = lane; q < I - tail; q += 64) {
-            T val;
-            if (q < nv) {
-                const int mm = q % L.n_tv;
-                val = tg[mm == 0 ? L.tv0 : (mm == 1 ? L.tv1 : L.tv2)] / T(L.max_speed);
-            } else {
-                val = q < nv + L.dup_h ? tg[3] : tg[4];
-            }
-            m.inb[tail + q] = val;
-        }
-    }
+    wave_sensors(A, R, m, sin_tab, lane, t, s6[0], s6[1], s6[2], s6[3], s6[4], s6[5], tg);
     wave_sync();
     if (rows)
         for (int q = lane; q < I; q += 64) R.sin[(size_t)slot * I + q] = m.inb[q];
