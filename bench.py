@@ -279,7 +279,8 @@ def main():
         ubytes = 0.0
         for d, T_r in zip(designs_u, hz):
             I_r = n_inputs(lay, d.n_vert)
-            f_r, b_r = algorithmic_bytes(d.n_vert, d.n_spring, I_r, 64, d.n_act, 50, 1)
+            f_r, b_r = algorithmic_bytes(d.n_vert, d.n_spring, I_r, 64, d.n_act,
+                                         T_r if ub.stored else 50, 1, stored=ub.stored)
             ubytes += (f_r + b_r) * T_r
         uval = rs / (graph_ms * 1e-3)
         nv = np.array([d.n_vert for d in designs_u])

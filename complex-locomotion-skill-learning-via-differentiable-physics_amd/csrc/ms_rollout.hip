@@ -245,9 +245,10 @@ __device__ __forceinline__ T task_term(const T *cv, const T *tg) {
 
 enum { MODE_FWD = 0, MODE_REC = 1 };
 
-// One forward step from the resident state at step t.  In MODE_REC the step
-// inputs and intermediates are written to scratch slot `slot`.
-template <typename T>
+// One forward step from the resident state at step t.  In MODE_REC (and in
+// every mode with ROWS) the step inputs and intermediates are written to
+// scratch slot `slot`.
+template <typename T, bool ROWS = false>
 __device__ void fwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int t, int mode, int slot,
                          T &lsum) {
     const int N = R.N, S = R.S, I = R.I, O = R.O, H = A.H;
@@ -268,7 +269,9 @@ __device__ void fwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int t, in
             if (tid < 3) row[3 * N + tid] = m.cv[tid];
         }
         if (A.loss_kind == 1 && t > R.Th - A.window && tid == 0) lsum += task_term(m.cv, tg);
-    } else {
+    }
+    const bool rows = ROWS || mode == MODE_REC;
+    if (rows) {
         T *sx = R.sx + (size_t)slot * 6 * N;
         for (int i = tid; i < 3 * N; i += BT) {
             sx[i] = m.xs[i];
@@ -280,7 +283,7 @@ __device__ void fwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int t, in
     for (int q = tid; q < I; q += BT) {
         const T val = sensor_value<T>(A.L, q, t, 0, N, m.xs, m.cv, m.cv + 3, tg);
         m.inb[q] = val;
-        if (mode == MODE_REC) R.sin[(size_t)slot * I + q] = val;
+        if (rows) R.sin[(size_t)slot * I + q] = val;
     }
     __syncthreads();
     // MLP layer 1: thread (j, chunk c); W1T rows are coalesced across j
@@ -307,7 +310,7 @@ __device__ void fwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int t, in
             activate<T>(A.act, z, y, dy);
             m.hid[tid] = y;
             m.dhid[tid] = dy;
-            if (mode == MODE_REC) {
+            if (rows) {
                 R.shid[(size_t)slot * H + tid] = y;
                 R.sdhid[(size_t)slot * H + tid] = dy;
             }
@@ -329,7 +332,7 @@ __device__ void fwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int t, in
                 activate<T>(A.act, acc + R.b2[o0], y, dy);
                 m.outb[o0] = y;
                 m.dout[o0] = dy;
-                if (mode == MODE_REC) {
+                if (rows) {
                     R.sout[(size_t)slot * O + o0] = y;
                     R.sdout[(size_t)slot * O + o0] = dy;
                 }
@@ -370,7 +373,8 @@ __device__ void fwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int t, in
 }
 
 // LB: the largest workgroup the instantiation is launched with (256 or 512).
-template <typename T, int LB>
+// ROWS: also write every step's adjoint rows (slot = step; K >= horizon).
+template <typename T, int LB, bool ROWS>
 __global__ void __launch_bounds__(LB) k_rollout_fwd(RArgs<T> A) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int r = blockIdx.x;
@@ -382,13 +386,18 @@ __global__ void __launch_bounds__(LB) k_rollout_fwd(RArgs<T> A) {
     int t1 = R.Th;
     if (A.max_steps > 0 && t0 + A.max_steps < t1) t1 = t0 + A.max_steps;
     if (t0 >= t1) return;
+    if (ROWS && (R.Th > A.K || R.sx == nullptr)) {
+        // the stored rows need K >= horizon rows of scratch
+        if (tid == 0 && A.loss) A.loss[r] = T(NAN);
+        return;
+    }
     for (int i = tid; i < 3 * N; i += BT) {
         m.xs[i] = R.state[i];
         m.vs[i] = R.state[3 * N + i];
     }
     __syncthreads();
     T lsum = T(0);
-    for (int t = t0; t < t1; ++t) fwd_step(A, R, m, t, MODE_FWD, 0, lsum);
+    for (int t = t0; t < t1; ++t) fwd_step<T, ROWS>(A, R, m, t, MODE_FWD, t, lsum);
     if (t1 == R.Th) {
         // final state: loss term and trajectory row (the center of the final
         // state is not part of the trajectory, matching the reference field)
@@ -606,7 +615,9 @@ __device__ void seg_wgrad(const RArgs<T> &A, const Rob<T> &R, int n) {
     }
 }
 
-template <typename T, int LB>
+// STORED: the forward kept every step's rows (K >= horizon): no segment
+// recompute.
+template <typename T, int LB, bool STORED>
 __global__ void __launch_bounds__(LB) k_rollout_bwd(RArgs<T> A) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int r = blockIdx.x;
@@ -616,6 +627,10 @@ __global__ void __launch_bounds__(LB) k_rollout_bwd(RArgs<T> A) {
     const int N = R.N, tid = threadIdx.x, BT = blockDim.x, K = A.K;
     int tb = A.robots[r].t_bwd;
     if (tb <= 0) return;
+    if (STORED && R.Th > K) {
+        if (tid == 0) R.Wg1[0] = T(NAN);
+        return;
+    }
     for (int i = tid; i < 3 * N; i += BT) {
         m.xb[i] = R.adj[i];
         m.vb[i] = R.adj[3 * N + i];
@@ -653,13 +668,15 @@ __global__ void __launch_bounds__(LB) k_rollout_bwd(RArgs<T> A) {
     while (tb > 0 && budget > 0) {
         const int seg = (tb - 1) / K, s0 = seg * K;
         const int lo = max(s0, tb - budget);
-        const T *ck = R.ckpt + (size_t)seg * 6 * N;
-        for (int i = tid; i < 3 * N; i += BT) {
-            m.xs[i] = ck[i];
-            m.vs[i] = ck[3 * N + i];
+        if (!STORED) {
+            const T *ck = R.ckpt + (size_t)seg * 6 * N;
+            for (int i = tid; i < 3 * N; i += BT) {
+                m.xs[i] = ck[i];
+                m.vs[i] = ck[3 * N + i];
+            }
+            __syncthreads();
+            for (int t = s0; t < tb; ++t) fwd_step(A, R, m, t, MODE_REC, t - s0, dummy);
         }
-        __syncthreads();
-        for (int t = s0; t < tb; ++t) fwd_step(A, R, m, t, MODE_REC, t - s0, dummy);
         for (int t = tb - 1; t >= lo; --t) bwd_step(A, R, m, r, t, t - s0);
         __syncthreads();
         // the weight-gradient fold reads slots lo - s0 .. tb - s0 - 1
@@ -1825,7 +1842,7 @@ static int check_rollout(const dl_rollout *r, bool bwd) {
     DL_CHECK_ARG(r->max_n > 0 && r->max_s >= 0 && r->max_i > 0 && r->max_o > 0,
                  "rollout: bad size maxima");
     DL_CHECK_ARG(r->lds_policy >= 0 && r->lds_policy <= 4, "rollout: bad LDS policy");
-    DL_CHECK_ARG(r->variant >= 0 && r->variant <= 4, "rollout: bad kernel variant");
+    DL_CHECK_ARG(r->variant >= 0 && r->variant <= 5, "rollout: bad kernel variant");
     DL_CHECK_ARG(r->lay.n_tv >= 0 && r->lay.n_tv <= 3 && r->lay.run_period > 0, "rollout: bad layout");
     DL_CHECK_ARG(r->loss_kind == 0 || (r->loss_kind == 1 && r->loss_window > 0),
                  "rollout: bad loss configuration");
@@ -1927,6 +1944,7 @@ static int launch_rollout(const dl_rollout *r, bool bwd, hipStream_t st) {
     // variant 4 keeps every step's rows: the forward needs the scratch too
     DL_CHECK_ARG(variant != 4 || r->scratch != nullptr, "rollout: stored rows need the scratch buffer");
     if (variant == 4) return launch_wave<T>(r, A, bwd, true, true, st);
+    DL_CHECK_ARG(variant != 5 || r->scratch != nullptr, "rollout: stored rows need the scratch buffer");
     if (variant == 3 || (variant == 0 && pipe_ok)) return launch_wave<T>(r, A, bwd, true, false, st);
     if (variant == 2 || (variant == 0 && wave_ok)) return launch_wave<T>(r, A, bwd, false, false, st);
     size_t sh = bytes(springs, topo, wl);
@@ -1936,19 +1954,22 @@ static int launch_rollout(const dl_rollout *r, bool bwd, hipStream_t st) {
     A.springs_in_lds = springs;
     A.topo_in_lds = topo;
     A.w_in_lds = wl;
-    const void *kf = bwd ? (r->block > 256 ? (const void *)k_rollout_bwd<T, 512>
-                                           : (const void *)k_rollout_bwd<T, 256>)
-                         : (r->block > 256 ? (const void *)k_rollout_fwd<T, 512>
-                                           : (const void *)k_rollout_fwd<T, 256>);
+    const bool big = r->block > 256, rows = variant == 5;
+    const void *kf =
+        bwd ? (big ? (rows ? (const void *)k_rollout_bwd<T, 512, true> : (const void *)k_rollout_bwd<T, 512, false>)
+                   : (rows ? (const void *)k_rollout_bwd<T, 256, true> : (const void *)k_rollout_bwd<T, 256, false>))
+            : (big ? (rows ? (const void *)k_rollout_fwd<T, 512, true> : (const void *)k_rollout_fwd<T, 512, false>)
+                   : (rows ? (const void *)k_rollout_fwd<T, 256, true> : (const void *)k_rollout_fwd<T, 256, false>));
     DL_CHECK_HIP(ensure_dynamic_lds(kf, sh));
-    if (bwd && r->block > 256)
-        hipLaunchKernelGGL((k_rollout_bwd<T, 512>), dim3(r->n_robots), dim3(r->block), sh, st, A);
-    else if (bwd)
-        hipLaunchKernelGGL((k_rollout_bwd<T, 256>), dim3(r->n_robots), dim3(r->block), sh, st, A);
-    else if (r->block > 256)
-        hipLaunchKernelGGL((k_rollout_fwd<T, 512>), dim3(r->n_robots), dim3(r->block), sh, st, A);
-    else
-        hipLaunchKernelGGL((k_rollout_fwd<T, 256>), dim3(r->n_robots), dim3(r->block), sh, st, A);
+    const dim3 grid(r->n_robots), block(r->block);
+    if (bwd && big && rows) hipLaunchKernelGGL((k_rollout_bwd<T, 512, true>), grid, block, sh, st, A);
+    else if (bwd && big) hipLaunchKernelGGL((k_rollout_bwd<T, 512, false>), grid, block, sh, st, A);
+    else if (bwd && rows) hipLaunchKernelGGL((k_rollout_bwd<T, 256, true>), grid, block, sh, st, A);
+    else if (bwd) hipLaunchKernelGGL((k_rollout_bwd<T, 256, false>), grid, block, sh, st, A);
+    else if (big && rows) hipLaunchKernelGGL((k_rollout_fwd<T, 512, true>), grid, block, sh, st, A);
+    else if (big) hipLaunchKernelGGL((k_rollout_fwd<T, 512, false>), grid, block, sh, st, A);
+    else if (rows) hipLaunchKernelGGL((k_rollout_fwd<T, 256, true>), grid, block, sh, st, A);
+    else hipLaunchKernelGGL((k_rollout_fwd<T, 256, false>), grid, block, sh, st, A);
     DL_CHECK_LAUNCH();
     return 0;
 }

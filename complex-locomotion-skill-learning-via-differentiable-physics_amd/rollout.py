@@ -59,7 +59,7 @@ class MassSpringBatch:
                   3 as 2 with the adjoint's segment recompute on a second
                   wavefront, 4 as 3 without recompute (the forward keeps
                   every step's adjoint rows; ckpt_every becomes the longest
-                  horizon)
+                  horizon), 5 as 1 with the forward's rows kept likewise
     """
 
     def __init__(self, designs, robot_design, horizons, physics, layout, targets, x0,
@@ -70,7 +70,7 @@ class MassSpringBatch:
         self.robot_design = np.asarray(robot_design, np.int64)
         R = self.R = len(self.robot_design)
         self.horizons = np.broadcast_to(np.asarray(horizons, np.int64), (R,)).copy()
-        if variant == 4:
+        if variant in (4, 5):
             # the forward keeps every step's adjoint rows: one scratch buffer
             # (and one checkpoint interval) spans the longest horizon
             ckpt_every = max(int(ckpt_every), int(self.horizons.max()))

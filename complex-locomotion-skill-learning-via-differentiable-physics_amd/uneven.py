@@ -49,17 +49,20 @@ class UnevenBatch:
     """R robots, robot r = design designs[r] with horizon horizons[r].
 
     targets: [R, 5] (tv (3), th, unused); x0: list of [N_r, 3] (default: the
-    design's rest positions shifted +0.4 in x, taichi_env.py:500-501)."""
+    design's rest positions shifted +0.4 in x, taichi_env.py:500-501).
+    stored: the forwards keep every step's adjoint rows and the adjoints read
+    them (variants 4 / 5, scratch for each class's longest horizon) instead
+    of recomputing checkpoint segments of ckpt_every steps."""
 
     def __init__(self, designs, horizons, physics, layout, targets, x0=None, ckpt_every=50,
                  loss_window=100, dtype=torch.float32, act="tanh", n_hidden=64, streams=True,
-                 device="cuda", bounds=(128, 400)):
+                 device="cuda", bounds=(128, 400), stored=True):
         self.R = len(designs)
         self.designs, self.horizons = designs, np.asarray(horizons, np.int64)
         if x0 is None:
             x0 = [d.objects + np.array([0.4, 0.0, 0.0]) for d in designs]
         targets = np.asarray(targets, np.float64)
-        self.dtype, self.streams_on = dtype, streams
+        self.dtype, self.streams_on, self.stored = dtype, streams, stored
         self.buckets = []
         self.slot = {}
         for name, ids in size_classes(designs, layout, dtype, n_hidden, bounds):
@@ -67,7 +70,7 @@ class UnevenBatch:
                                 self.horizons[ids], physics, layout, targets[ids],
                                 [x0[r] for r in ids], n_hidden=n_hidden, act=act,
                                 ckpt_every=ckpt_every, dtype=dtype, loss_window=loss_window,
-                                device=device, variant=3 if name == "wave" else 1,
+                                device=device, variant=self._variant(name),
                                 block=64 if name == "wave" else 256)
             for j, r in enumerate(ids):
                 self.slot[r] = (len(self.buckets), j)
@@ -95,12 +98,17 @@ class UnevenBatch:
         return int(self.horizons.sum())
 
     # ------------------------------------------------------------ launch shapes
-    @staticmethod
-    def candidates(name):
+    def _variant(self, name):
         if name == "wave":
-            return [{"variant": v, "block": 64, "lds_policy": 0} for v in (3, 2)]
-        return [{"variant": 1, "block": b, "lds_policy": p} for b in (64, 128, 256, 512)
-                for p in (0, 1, 4)]
+            return 4 if self.stored else 3
+        return 5 if self.stored else 1
+
+    def candidates(self, name):
+        if name == "wave":
+            vs = (4,) if self.stored else (3, 2)
+            return [{"variant": v, "block": 64, "lds_policy": 0} for v in vs]
+        return [{"variant": self._variant(name), "block": b, "lds_policy": p}
+                for b in (64, 128, 256, 512) for p in (0, 1, 4)]
 
     def _apply(self, bk, cfg):
         b = bk["batch"]
@@ -116,7 +124,7 @@ class UnevenBatch:
             nv = np.array([b.designs[int(d)].n_vert for d in b.robot_design])
             key = Autotuner.key(kernel="ms_rollout", cls=bk["name"], dtype=str(self.dtype),
                                 robots=len(bk["ids"]), mean_n=int(nv.mean()),
-                                mean_T=int(b.horizons.mean()))
+                                mean_T=int(b.horizons.mean()), stored=int(self.stored))
 
             def measure(cfg, bk=bk):
                 self._apply(bk, cfg)

@@ -238,7 +238,8 @@ typedef struct dl_rollout {
                                 4 as 3 with no recompute: the forward writes every step's
                                 adjoint rows into scratch buffer 0, which then needs
                                 ckpt_every >= every horizon (a robot whose horizon
-                                exceeds it gets a NaN loss / gradient) */
+                                exceeds it gets a NaN loss / gradient),
+                                5 as 1 with the forward's rows kept likewise */
     dl_ms_physics phys;
     dl_sensor_layout lay;
     const dl_robot *robots;  /* device [n_robots] */

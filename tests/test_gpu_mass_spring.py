@@ -446,11 +446,12 @@ def _uneven(prec, variant, seed=31, block=256, lds_policy=0):
 
 @pytest.mark.parametrize("prec,variant,block,pol", [("f64", 1, 256, 0), ("f32", 1, 256, 0),
                                                     ("f32", 2, 256, 0), ("f32", 1, 512, 4),
-                                                    ("f32", 1, 128, 4)])
+                                                    ("f32", 1, 128, 4), ("f64", 5, 256, 0),
+                                                    ("f32", 5, 512, 4)])
 def test_uneven_batch_matches_oracle(prec, variant, block, pol):
     """Three different random designs (own sizes, MLPs and horizons) in one launch;
     also the block kernel at 512 threads and with everything staged in LDS
-    (policy 4)."""
+    (policy 4), and with the forward keeping every step's rows (variant 5)."""
     _say(f"uneven {prec} variant={variant} block={block} policy={pol}")
     from oracle import builders as ob
     batch, designs, cubes_list, horizons, W, tv, th = _uneven(prec, variant, block=block,

@@ -63,13 +63,17 @@ def test_autotuner_picks_fastest_and_caches(tmp_path):
 
 
 @pytest.mark.gpu
-def test_uneven_iteration_and_graph_replay_match_single_launch(tmp_path):
+@pytest.mark.parametrize("stored", [True, False])
+def test_uneven_iteration_and_graph_replay_match_single_launch(tmp_path, stored):
+    """Size classes on streams (rows kept by the forwards, or checkpoint
+    segments recomputed), tuned, then captured and replayed, against one
+    block-kernel launch over all robots."""
     from diffloco.rollout import MassSpringBatch, init_weights
     from diffloco.uneven import UnevenBatch, n_inputs
-    print("uneven batch: tune, streams, graph", flush=True)
+    print(f"uneven batch: tune, streams, graph (stored={stored})", flush=True)
     designs, horizons, phys, lay, tg = _setup()
     R = len(designs)
-    ub = UnevenBatch(designs, horizons, phys, lay, tg, ckpt_every=16, loss_window=30)
+    ub = UnevenBatch(designs, horizons, phys, lay, tg, ckpt_every=16, loss_window=30, stored=stored)
     assert len(ub.buckets) >= 2 and ub.buckets[0]["name"] == "wave"
     W = [init_weights(n_inputs(lay, d.n_vert), 64, d.n_act, seed=100 + r) for r, d in enumerate(designs)]
     for r in range(R):
