@@ -359,6 +359,11 @@ __device__ void fwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int t, in
             const T sg = (ent & 1) ? T(1) : T(-1);
             vi[0] += sg * m.J[s]; vi[1] += sg * m.J[S + s]; vi[2] += sg * m.J[2 * S + s];
         }
+        if constexpr (ROWS) {
+            // kept for the adjoint, which then skips its spring pass
+            T *sv = R.svi + (size_t)slot * 3 * N;
+            sv[i] = vi[0]; sv[N + i] = vi[1]; sv[2 * N + i] = vi[2];
+        }
         const T x[3] = {m.xs[i], m.xs[N + i], m.xs[2 * N + i]};
         const T v[3] = {m.vs[i], m.vs[N + i], m.vs[2 * N + i]};
         T xn[3], vn[3];
@@ -416,8 +421,9 @@ __global__ void __launch_bounds__(LB) k_rollout_fwd(RArgs<T> A) {
 }
 
 // Adjoint of one step t.  On entry m.xb/m.vb hold dL/d(state t+1); on exit
-// dL/d(state t).  Intermediates come from scratch slot `slot`.
-template <typename T>
+// dL/d(state t).  Intermediates come from scratch slot `slot`; with STORED
+// the step's impulse sums too (kept by the forward), so no spring pass.
+template <typename T, bool STORED = false>
 __device__ void bwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int r, int t, int slot) {
     const int N = R.N, S = R.S, I = R.I, O = R.O, H = A.H;
     const int tid = threadIdx.x, BT = blockDim.x;
@@ -435,23 +441,30 @@ __device__ void bwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int r, in
     if (tid < 6) m.cv[tid] = R.scv[slot * 6 + tid];
     __syncthreads();
     T *G = m.J;  // [7][S], overwrites the impulses once the integrator adjoint consumed them
-    // recompute impulses
-    for (int s = tid; s < S; s += BT) {
-        const int a = R.sa[s], b = R.sb[s];
-        const T d[3] = {m.xs[a] - m.xs[b], m.xs[N + a] - m.xs[N + b], m.xs[2 * N + a] - m.xs[2 * N + b]};
-        const T w[3] = {m.vs[a] - m.vs[b], m.vs[N + a] - m.vs[N + b], m.vs[2 * N + a] - m.vs[2 * N + b]};
-        T J[3];
-        spring_impulse<T>(d, w, R.L0[s], R.k[s], R.coef[s], m.outb[R.sact[s]], A.P, J);
-        m.J[s] = J[0]; m.J[S + s] = J[1]; m.J[2 * S + s] = J[2];
+    if constexpr (!STORED) {
+        // recompute impulses
+        for (int s = tid; s < S; s += BT) {
+            const int a = R.sa[s], b = R.sb[s];
+            const T d[3] = {m.xs[a] - m.xs[b], m.xs[N + a] - m.xs[N + b], m.xs[2 * N + a] - m.xs[2 * N + b]};
+            const T w[3] = {m.vs[a] - m.vs[b], m.vs[N + a] - m.vs[N + b], m.vs[2 * N + a] - m.vs[2 * N + b]};
+            T J[3];
+            spring_impulse<T>(d, w, R.L0[s], R.k[s], R.coef[s], m.outb[R.sact[s]], A.P, J);
+            m.J[s] = J[0]; m.J[S + s] = J[1]; m.J[2 * S + s] = J[2];
+        }
+        __syncthreads();
     }
-    __syncthreads();
     // integrate adjoint
     for (int i = tid; i < N; i += BT) {
         T vi[3] = {0, 0, 0};
-        for (int e = R.vptr[i]; e < R.vptr[i + 1]; ++e) {
-            const int ent = R.vidx[e], s = ent >> 1;
-            const T sg = (ent & 1) ? T(1) : T(-1);
-            vi[0] += sg * m.J[s]; vi[1] += sg * m.J[S + s]; vi[2] += sg * m.J[2 * S + s];
+        if constexpr (STORED) {
+            const T *sv = R.svi + (size_t)slot * 3 * N;
+            vi[0] = sv[i]; vi[1] = sv[N + i]; vi[2] = sv[2 * N + i];
+        } else {
+            for (int e = R.vptr[i]; e < R.vptr[i + 1]; ++e) {
+                const int ent = R.vidx[e], s = ent >> 1;
+                const T sg = (ent & 1) ? T(1) : T(-1);
+                vi[0] += sg * m.J[s]; vi[1] += sg * m.J[S + s]; vi[2] += sg * m.J[2 * S + s];
+            }
         }
         const T x[3] = {m.xs[i], m.xs[N + i], m.xs[2 * N + i]};
         const T v[3] = {m.vs[i], m.vs[N + i], m.vs[2 * N + i]};
@@ -677,7 +690,7 @@ __global__ void __launch_bounds__(LB) k_rollout_bwd(RArgs<T> A) {
             __syncthreads();
             for (int t = s0; t < tb; ++t) fwd_step(A, R, m, t, MODE_REC, t - s0, dummy);
         }
-        for (int t = tb - 1; t >= lo; --t) bwd_step(A, R, m, r, t, t - s0);
+        for (int t = tb - 1; t >= lo; --t) bwd_step<T, STORED>(A, R, m, r, t, t - s0);
         __syncthreads();
         // the weight-gradient fold reads slots lo - s0 .. tb - s0 - 1
         {
