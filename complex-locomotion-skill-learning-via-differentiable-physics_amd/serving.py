@@ -75,6 +75,10 @@ class MSPool:
                  lds_policy=0, device="cuda"):
         self.name, self.C = name, slots
         self.Nc, self.Sc, self.Ic, self.Oc, self.Tc = caps
+        if variant in (4, 5):
+            # stored rows: the forward keeps every step's adjoint rows, so the
+            # scratch (and the checkpoint interval) spans the horizon cap
+            ckpt_every = max(ckpt_every, self.Tc)
         self.phys, self.lay = physics, layout
         self.K, self.W, self.H = ckpt_every, loss_window, n_hidden
         self.act = nat.ACT_SIN if act == "sin" else nat.ACT_TANH
@@ -579,14 +583,17 @@ def make_requests(rate, duration, seed, layout, frac_mpm=0.3, grad_frac=0.5, n_h
 
 
 def default_executors(physics, layout, slots_small=256, slots_large=128, mpm_active=8,
-                      ckpt_every=50, loss_window=100, dtype=torch.float32, device="cuda"):
+                      ckpt_every=50, loss_window=100, dtype=torch.float32, device="cuda",
+                      stored=True):
     """Two mass-spring pools (wave-per-robot kernel for small robots, block
     kernel up to 800 vertices / 7000 springs / 64 actuators / 1000 steps) and
-    the MPM runner."""
+    the MPM runner.  With `stored` the pools' forwards keep every step's
+    adjoint rows (variants 4 and 5), else checkpoints every ckpt_every steps
+    are recomputed (variants 3 and 1)."""
     small = MSPool("wave", slots_small, (WAVE_MAX_VERT, 512, WAVE_MAX_IN, WAVE_MAX_OUT, 1000),
-                   physics, layout, ckpt_every, loss_window, dtype=dtype, variant=3, block=64,
-                   device=device)
+                   physics, layout, ckpt_every, loss_window, dtype=dtype,
+                   variant=4 if stored else 3, block=64, device=device)
     large = MSPool("block", slots_large, (800, 7000, n_inputs(layout, 800), 64, 1000), physics,
-                   layout, ckpt_every, loss_window, dtype=dtype, variant=1, block=256,
-                   device=device)
+                   layout, ckpt_every, loss_window, dtype=dtype, variant=5 if stored else 1,
+                   block=256, device=device)
     return [small, large, MPMRunner(mpm_active, dtype=dtype, device=device)]
