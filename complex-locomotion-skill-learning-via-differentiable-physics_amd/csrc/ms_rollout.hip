@@ -841,7 +841,7 @@ __device__ WShared<T> wave_stage(const RArgs<T> &A, const Rob<T> &R0, unsigned c
 }
 
 template <typename T>
-__device__ __forceinline__ void wave_point_at_shared(Rob<T> &R, const WShared<T> &sh, bool bwd) {
+__device__ __forceinline__ void wave_point_at_shared(Rob<T> &R, const WShared<T> &sh) {
     const int S = R.S, N = R.N, O = R.O;
     const int nv = 2 * S;
     R.L0 = sh.topo_r; R.k = sh.topo_r + S; R.coef = sh.topo_r + 2 * S;
@@ -1180,7 +1180,7 @@ __global__ void __launch_bounds__(256) k_wave_fwd(RArgs<T> A, int n_robots) {
         if (lane == 0 && A.loss) A.loss[r] = T(NAN);
         return;
     }
-    wave_point_at_shared(R, sh, false);
+    wave_point_at_shared(R, sh);
     for (int q = lane; q < (A.max_i > 192 ? A.max_i : 192) + 4; q += 64) m.inb[q] = T(0);
     const int N = R.N;
     const int t0 = A.robots[r].t_fwd;
@@ -1668,7 +1668,7 @@ __global__ void __launch_bounds__(256) k_wave_bwd(RArgs<T> A, int n_robots) {
         if (lane == 0) R.Wg1[0] = T(NAN);   // robots of one workgroup must share a design
         return;
     }
-    wave_point_at_shared(R, sh, true);
+    wave_point_at_shared(R, sh);
     wave_bwd_robot<T, IMAX, OMAX>(A, R, m, sh, r, lane);
 }
 
@@ -1732,7 +1732,7 @@ __global__ void __launch_bounds__(512) k_wave_bwd_pipe(RArgs<T> A, int n_robots)
         active = false;
     }
     Rob<T> R = make_rob(A, active ? r : r0);
-    wave_point_at_shared(R, sh, true);
+    wave_point_at_shared(R, sh);
     T *const scr0 = R.sx, *const scr1 = R.scr2;   // scratch buffers 0 and 1
     T *region = reinterpret_cast<T *>(smem_raw) + (size_t)rl * (Rb + Rf);
     WSm<T> m = role == 0 ? wave_carve_at(A, region, true) : wave_carve_at(A, region + Rb, false);
