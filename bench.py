@@ -75,6 +75,8 @@ def main():
                     help="0: global-atomic commits (staged run sums, LDS tile fallback), "
                          "1: tiles stored and gathered by the grid pass")
     ap.add_argument("--mpm-tile", type=int, default=2048, help="LDS scatter tile, nodes")
+    ap.add_argument("--mpm-store-states", type=int, default=1,
+                    help="MPM forward keeps every step's particle state (adjoint without replay)")
     ap.add_argument("--skip-uneven", action="store_true")
     ap.add_argument("--uneven-robots", type=int, default=512)
     ap.add_argument("--uneven-iters", type=int, default=3)
@@ -196,7 +198,7 @@ def main():
         pos, ids, n_act = dm.quadruped3d_particles(cells_per_cube=6, n_grid=64)
         mb = dm.MPMBatch(dm.quadruped3d_params(64), pos, ids, n_act, Bm, Tm, K=Km,
                          window=Tm // 2, dtype=torch.float32, scatter=args.mpm_scatter,
-                         tile_nodes=args.mpm_tile)
+                         tile_nodes=args.mpm_tile, store_states=bool(args.mpm_store_states))
         rngm = np.random.default_rng(7)
         mb.set_targets(np.stack([rngm.uniform(-0.08, 0.08, Bm), np.zeros(Bm),
                                  rngm.uniform(-0.08, 0.08, Bm)], 1), rngm.uniform(0.1, 0.25, Bm))
@@ -223,12 +225,18 @@ def main():
         ms_rollout = float(np.mean(mf) + np.mean(mbk))
         pps = Bm * P * Tm / (ms_rollout * 1e-3)
         gp = active / P
-        bpp = 684 + 224 * gp + 96 / Km          # algorithmic bytes per particle-step, fwd+bwd
+        # algorithmic bytes per particle-step, fwd + bwd: forward 196 + 56 G/P,
+        # adjoint 292 + 112 G/P, and per recomputed segment the forward again
+        # plus the checkpoint (96 / K); stored states skip the recompute of
+        # every segment whose grids were all saved
+        f_rec = 1.0 - (sum(mb.replay) / mb.nck if mb.store_states else 0.0)
+        bpp = 488 + 168 * gp + f_rec * (196 + 56 * gp + 96 / Km)
         mpm = {"value": pps, "unit": "particle-steps/s", "robots": Bm, "particles": P,
                "grid": 64, "horizon": Tm, "ckpt_every": Km, "dtype": "f32",
                "scatter": ["staged run sums + per-node gather, global atomics",
                            "tile store + gather"][args.mpm_scatter],
                "tile_nodes": args.mpm_tile, "scatter_windows": len(mb.windows),
+               "store_states": mb.store_states, "replayed_segments": int(sum(mb.replay)),
                "resort_every": Km,
                "ms_forward": float(np.mean(mf)), "ms_backward": float(np.mean(mbk)),
                "active_nodes_per_particle": gp,

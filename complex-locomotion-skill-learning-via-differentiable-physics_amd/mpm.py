@@ -287,11 +287,16 @@ def scatter_windows(positions, n_grid, box_limit, dim=3, pb=MPM_PB):
 
 
 class _Slots:
-    """S state slots: x, v [S,B,D,P]; C, F [S,B,D*D,P]."""
+    """S state slots: x, v [S,B,D,P]; C, F [S,B,D*D,P].  With ``parent`` the
+    slots are a view of the parent's slots [base, base + S)."""
 
-    def __init__(self, S, B, D, P, dtype, device):
-        z = lambda *s: torch.zeros(*s, dtype=dtype, device=device)
-        self.x, self.v, self.C, self.F = z(S, B, D, P), z(S, B, D, P), z(S, B, D * D, P), z(S, B, D * D, P)
+    def __init__(self, S, B, D, P, dtype, device, parent=None, base=0):
+        if parent is None:
+            z = lambda *s: torch.zeros(*s, dtype=dtype, device=device)
+            self.x, self.v = z(S, B, D, P), z(S, B, D, P)
+            self.C, self.F = z(S, B, D * D, P), z(S, B, D * D, P)
+        else:
+            self.x, self.v, self.C, self.F = (f[base:base + S] for f in parent.fields())
         self.c = MPMSlotsC(nat.ptr(self.x), nat.ptr(self.v), nat.ptr(self.C), nat.ptr(self.F), S, 0)
 
     def fields(self):
@@ -327,13 +332,18 @@ class MPMBatch:
     rest shape's blocks), so the backward recomputes a segment from its
     checkpoint by restoring the saved grid velocities and running only the
     update of F and G2P; a segment where some step did not fit is recomputed
-    in full."""
+    in full.  With ``store_states`` (and ``save_horizon``) the forward keeps
+    every step's particle state (T + T/K slots of x, v, C, F: one view of
+    K + 1 slots per segment), and the backward of a segment whose every
+    step's grid was saved runs the adjoint on those states with no
+    recompute at all."""
 
     def __init__(self, params, positions, act_id, n_act, B, T, K=100, window=None,
                  dtype=torch.float32, n_sin=10, run_period=100, offset_scale=0.2,
                  max_speed=0.08, activation="tanh", n_hidden=64, device="cuda",
                  sort_particles=True, tile_nodes=None, scatter=SCATTER_ATOMIC, save_grid=True,
-                 unroll=3, box_limit=512, resort=True, save_horizon=True, horizon_blocks=None):
+                 unroll=3, box_limit=512, resort=True, save_horizon=True, horizon_blocks=None,
+                 store_states=False):
         if params.dim != 3:
             raise ValueError("MPMBatch drives 3D robots (the controller's sensors are 3D)")
         if params.n_grid % 4:
@@ -414,9 +424,17 @@ class MPMBatch:
             self.perms = torch.empty(self.nck, B, P, dtype=torch.int64, device=dev)
             self.orders[0].copy_(torch.arange(P, device=dev)[None].expand(B, -1))
         self.ring = _Slots(2, B, D, P, dtype, dev)
-        self.ck = _Slots(self.nck, B, D, P, dtype, dev)
-        self.seg = _Slots(K + 1, B, D, P, dtype, dev)
         self.adj = _Slots(2, B, D, P, dtype, dev)
+        self.store_states = bool(store_states)
+        if self.store_states:
+            # segment s = steps [t0, t0 + n) owns slots [t0 + s, t0 + s + n + 1)
+            self.states = _Slots(T + self.nck, B, D, P, dtype, dev)
+            self.views = [_Slots(n + 1, B, D, P, dtype, dev, parent=self.states, base=t0 + s)
+                          for s, (t0, n) in enumerate(self._segments())]
+            self.ck = self.seg = None
+        else:
+            self.ck = _Slots(self.nck, B, D, P, dtype, dev)
+            self.seg = _Slots(K + 1, B, D, P, dtype, dev)
         P_ = lambda t: nat.ptr(t)
         self.desc = MPMBatchC(
             dtype=self.dc, B=B, P=P, n_act=n_act, n_groups=self.G, n_sin=n_sin,
@@ -528,9 +546,16 @@ class MPMBatch:
             self.perms[s].copy_(pi)
             torch.gather(self.orders[s - 1], 1, pi, out=self.orders[s])
             self._set_order(s)
-        self.ck.copy_slot(s, 0, r)
         flags = MPM_CONTROL | (MPM_SENSE_FIRST if s == 0 else 0) | \
             (MPM_SAVE_HORIZON if self.save_horizon else 0)
+        if self.store_states:
+            v = self.views[s]
+            v.copy_slot(0, 0, r)
+            nat.call("dl_mpm_forward", ctypes.byref(self.desc), ctypes.byref(v.c), t0, n, flags,
+                     nat.stream())
+            r.copy_slot(0, n, v)
+            return
+        self.ck.copy_slot(s, 0, r)
         nat.call("dl_mpm_forward", ctypes.byref(self.desc), ctypes.byref(r.c), t0, n, flags,
                  nat.stream())
         if n % 2:
@@ -568,15 +593,24 @@ class MPMBatch:
         t0, n = self._segments()[s]
         if self.resort:
             self._set_order(s)
-        self.seg.copy_slot(0, s, self.ck)
-        if self.replay[s]:
-            self.desc.restore_from = 1
-            nat.call("dl_mpm_forward", ctypes.byref(self.desc), ctypes.byref(self.seg.c), t0, n,
-                     MPM_REPLAY, nat.stream())
+        if self.store_states:
+            seg = self.views[s]
+            if self.replay[s]:
+                self.desc.restore_from = 1      # states stored, grids saved: adjoint only
+            else:
+                nat.call("dl_mpm_forward", ctypes.byref(self.desc), ctypes.byref(seg.c), t0, n,
+                         MPM_SAVE_GRID if self.save_grid else 0, nat.stream())
         else:
-            nat.call("dl_mpm_forward", ctypes.byref(self.desc), ctypes.byref(self.seg.c), t0, n,
-                     MPM_SAVE_GRID if self.save_grid else 0, nat.stream())
-        nat.call("dl_mpm_backward", ctypes.byref(self.desc), ctypes.byref(self.seg.c),
+            seg = self.seg
+            seg.copy_slot(0, s, self.ck)
+            if self.replay[s]:
+                self.desc.restore_from = 1
+                nat.call("dl_mpm_forward", ctypes.byref(self.desc), ctypes.byref(seg.c), t0, n,
+                         MPM_REPLAY, nat.stream())
+            else:
+                nat.call("dl_mpm_forward", ctypes.byref(self.desc), ctypes.byref(seg.c), t0, n,
+                         MPM_SAVE_GRID if self.save_grid else 0, nat.stream())
+        nat.call("dl_mpm_backward", ctypes.byref(self.desc), ctypes.byref(seg.c),
                  ctypes.byref(self.adj.c), t0, n, nat.stream())
         self.desc.restore_from = 0
         if self.resort and s > 0:
@@ -604,7 +638,9 @@ class MPMBatch:
                   if isinstance(t, torch.Tensor))
         own += sum(t.numel() * t.element_size() for t in self.W)
         return own + sum(f.numel() * f.element_size()
-                         for sl in (self.ring, self.ck, self.seg, self.adj) for f in sl.fields())
+                         for sl in (self.ring, self.ck, self.seg, self.adj,
+                                    getattr(self, "states", None))
+                         if sl is not None for f in sl.fields())
 
     def check(self):
         e = int(self.err.item())

@@ -220,6 +220,47 @@ def test_mpm_batched_replay_matches_full_recompute(hblocks):
         assert rel_err(a.double().cpu(), b.double().cpu()) < 1e-10
 
 
+@pytest.mark.parametrize("hblocks,shuffled", [(None, False), (1, False), (None, True)])
+def test_mpm_batched_stored_states_match_replay(hblocks, shuffled):
+    """Every step's particle state kept by the forward, the adjoint run on
+    those states with no recompute, gives the loss and gradients of the
+    replayed segments and of the oracle.  With a one-block horizon save every
+    segment still falls back to the full recompute from its first stored
+    state; the shuffled case re-sorts particles at every checkpoint."""
+    _say(f"mpm batched stored states hblocks={hblocks} shuffled={shuffled}")
+    if shuffled:
+        params, op, pos, ids = _batched_case(offsets=((5.0, 3.2, 5.0), (8.0, 3.2, 5.0)),
+                                             perm_seed=5)
+        T, K, W, sort = 13, 4, 6, False
+    else:
+        params, op, pos, ids = _batched_case()
+        T, K, W, sort = 24, 7, 12, True
+    runs = []
+    for stored in (False, True):
+        batch, loss, grads, lo, gref = _run_batched(params, op, pos, ids, 2, T, K, W,
+                                                    torch.float64, sort=sort,
+                                                    horizon_blocks=hblocks, store_states=stored)
+        assert abs(loss - lo) / abs(lo) < 1e-10
+        for a, b in zip(grads, gref):
+            assert rel_err(a.double().cpu(), b) < 1e-7
+        assert batch.replay == [hblocks is None] * batch.nck
+        if stored:
+            # the last slot of each segment is the next segment's start state
+            # (before that segment's re-sort)
+            for s in range(batch.nck - 1):
+                nxt = batch.views[s + 1].x[0]
+                if batch.resort:
+                    pi = batch.perms[s + 1]
+                    cur = torch.gather(batch.views[s].x[-1], 2, pi[:, None, :].expand_as(nxt))
+                else:
+                    cur = batch.views[s].x[-1]
+                assert torch.equal(cur, nxt)
+        runs.append((loss, grads))
+    assert abs(runs[0][0] - runs[1][0]) / abs(runs[0][0]) < 1e-12
+    for a, b in zip(runs[0][1], runs[1][1]):
+        assert rel_err(a.double().cpu(), b.double().cpu()) < 1e-10
+
+
 def test_mpm_batched_resort_matches_fixed_order():
     """Shuffled particles re-sorted by block at every checkpoint (state,
     actuator ids and groups permuted; the adjoint permuted back) give the
