@@ -801,15 +801,15 @@ __global__ __launch_bounds__(PB) void k_grid_b(MP<T> p, T *grid, T *gvo, Blk bk,
 // reads nothing else); otherwise also the totals, the active list and
 // count, which the adjoint kernels consume and clear.
 template <int D, typename T>
-__global__ __launch_bounds__(PB) void k_grid_restore(MP<T> p, T *grid, T *gvo, Blk bk,
-                                                     GridSave<T> sv, int vo_only) {
+__device__ __forceinline__ void grid_restore(const MP<T> &p, T *grid, T *gvo, const Blk &bk,
+                                             const GridSave<T> &sv, int vo_only, int bx, int nbx) {
     const int k = blockIdx.y, lane = threadIdx.x & 63;
-    const int nw = gridDim.x * NWAVE;
+    const int nw = nbx * NWAVE;
     const int cnt = sv.count[k];
     const size_t G = gsize<D>(p.n);
     T *gk = grid + (size_t)k * G * 4, *ok = gvo + (size_t)k * G * 4;
-    if (!vo_only && blockIdx.x == 0 && threadIdx.x == 0) bk.count[k] = cnt;
-    for (int e = blockIdx.x * NWAVE + (threadIdx.x >> 6); e < cnt; e += nw) {
+    if (!vo_only && bx == 0 && threadIdx.x == 0) bk.count[k] = cnt;
+    for (int e = bx * NWAVE + (threadIdx.x >> 6); e < cnt; e += nw) {
         const int bid = sv.list[(size_t)k * sv.cap + e];
         bool bnd;
         int c[D];
@@ -825,6 +825,12 @@ __global__ __launch_bounds__(PB) void k_grid_restore(MP<T> p, T *grid, T *gvo, B
             if (lane == 0) bk.list[(size_t)k * bk.nblk + e] = bid;
         }
     }
+}
+
+template <int D, typename T>
+__global__ __launch_bounds__(PB) void k_grid_restore(MP<T> p, T *grid, T *gvo, Blk bk,
+                                                     GridSave<T> sv, int vo_only) {
+    grid_restore<D, T>(p, grid, gvo, bk, sv, vo_only, blockIdx.x, gridDim.x);
 }
 
 // F of the next state for every particle, (I + dt C) F: the part of P2G that
@@ -1009,23 +1015,24 @@ __global__ __launch_bounds__(64) void k_ctrl(Ctrl<T> c, int t, const T *sens, T 
     }
 }
 
-// Adjoint of k_ctrl (with mlp) at step t: consumes (and zeroes) act_b,
-// writes in_b row t and the pre-activation adjoints of the step (zh: hidden
-// [B][H], zo: output [B][O]); k_wgrad_time turns those rows into the weight
-// gradient.
+// Adjoint of k_ctrl (with mlp) at step t for robot k, by the nt threads of
+// a workgroup (each entry computed by one thread, in the same order for any
+// nt): consumes (and zeroes) act_b, writes in_b row t and the pre-activation
+// adjoints of the step (zh: hidden [B][H], zo: output [B][O]); k_wgrad_time
+// turns those rows into the weight gradient.  LDS: (I + 3H + O) values.
 template <typename T>
-__global__ __launch_bounds__(64) void k_ctrl_adj(Ctrl<T> c, const T *inp, T *act_b, T *in_b,
-                                                 T *zh, T *zo) {
+__device__ __forceinline__ void ctrl_adj(const Ctrl<T> &c, int k, int nt, const T *inp, T *act_b,
+                                         T *in_b, T *zh, T *zo) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *si = reinterpret_cast<T *>(smem_raw);
     T *shd = si + c.I;
     T *sd1 = shd + c.H;
     T *sz1b = sd1 + c.H;
     T *sab = sz1b + c.H;
-    const int k = blockIdx.x, tid = threadIdx.x;
-    for (int i = tid; i < c.I; i += 64) si[i] = inp[(size_t)k * c.I + i];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < c.I; i += nt) si[i] = inp[(size_t)k * c.I + i];
     __syncthreads();
-    for (int h = tid; h < c.H; h += 64) {
+    for (int h = tid; h < c.H; h += nt) {
         T z = c.b1[h];
         const T *w = c.W1 + (size_t)h * c.I;
         for (int i = 0; i < c.I; ++i) z += w[i] * si[i];
@@ -1035,7 +1042,7 @@ __global__ __launch_bounds__(64) void k_ctrl_adj(Ctrl<T> c, const T *inp, T *act
         sd1[h] = dy;
     }
     __syncthreads();
-    for (int o = tid; o < c.O; o += 64) {
+    for (int o = tid; o < c.O; o += nt) {
         T z = c.b2[o];
         const T *w = c.W2 + (size_t)o * c.H;
         for (int h = 0; h < c.H; ++h) z += w[h] * shd[h];
@@ -1046,19 +1053,26 @@ __global__ __launch_bounds__(64) void k_ctrl_adj(Ctrl<T> c, const T *inp, T *act
         *ab = T(0);
     }
     __syncthreads();
-    for (int h = tid; h < c.H; h += 64) {
+    for (int h = tid; h < c.H; h += nt) {
         T hb = T(0);
         for (int o = 0; o < c.O; ++o) hb += c.W2[(size_t)o * c.H + h] * sab[o];
         sz1b[h] = hb * sd1[h];
     }
     __syncthreads();
-    for (int i = tid; i < c.I; i += 64) {
+    for (int i = tid; i < c.I; i += nt) {
         T ib = T(0);
         for (int h = 0; h < c.H; ++h) ib += c.W1[(size_t)h * c.I + i] * sz1b[h];
         in_b[(size_t)k * c.I + i] = ib;
     }
-    for (int h = tid; h < c.H; h += 64) zh[(size_t)k * c.H + h] = sz1b[h];
-    for (int o = tid; o < c.O; o += 64) zo[(size_t)k * c.O + o] = sab[o];
+    for (int h = tid; h < c.H; h += nt) zh[(size_t)k * c.H + h] = sz1b[h];
+    for (int o = tid; o < c.O; o += nt) zo[(size_t)k * c.O + o] = sab[o];
+}
+
+// Block per robot: ctrl_adj on 64 threads.
+template <typename T>
+__global__ __launch_bounds__(64) void k_ctrl_adj(Ctrl<T> c, const T *inp, T *act_b, T *in_b,
+                                                 T *zh, T *zo) {
+    ctrl_adj<T>(c, blockIdx.x, 64, inp, act_b, in_b, zh, zo);
 }
 
 // Weight gradient of n controller steps, added once to the robot's slice
@@ -1091,6 +1105,21 @@ __global__ __launch_bounds__(256) void k_wgrad_time(int B, int I, int H, int O, 
         for (int j = n - 1; j >= 0; --j) acc += zo[((size_t)j * B + k) * O + o];
     }
     wg[(size_t)k * nw + e] += acc;
+}
+
+// One launch for two independent pieces of the backward: the controller
+// adjoint of step t (workgroup nr of each robot, after P2G's adjoint of step
+// t has written act_b) and the grid restore of step t - 1 (workgroups
+// 0 .. nr-1), which the adjoint of step t - 1 reads next.
+template <int D, typename T>
+__global__ __launch_bounds__(PB) void k_restore_ctrl_adj(MP<T> p, T *grid, T *gvo, Blk bk,
+                                                         GridSave<T> sv, int nr, Ctrl<T> c,
+                                                         const T *inp, T *act_b, T *in_b, T *zh,
+                                                         T *zo) {
+    if ((int)blockIdx.x < nr)
+        grid_restore<D, T>(p, grid, gvo, bk, sv, 0, blockIdx.x, nr);
+    else
+        ctrl_adj<T>(c, blockIdx.y, PB, inp, act_b, in_b, zh, zo);
 }
 
 // ---------------------------------------------------------------- adjoints
@@ -1582,16 +1611,21 @@ int backward_impl(const dl_mpm_batch *b, const dl_mpm_slots *seg, const dl_mpm_s
                                                            tile_bytes)));
     DL_UNROLL_DISPATCH(U_, DL_CHECK_HIP(ensure_dynamic_lds((const void *)k_g2p_adj_b<D, T, U_>,
                                                            tile_bytes)));
+    const bool restore = b->restore_from == 1 || b->grid_save;
+    auto saved = [&](int jj) {
+        return b->restore_from == 1 ? hsave_at<T>(b, t0 + jj) : save_at<D, T>(b, jj);
+    };
     int cur = 0;
+    bool restored = false;  // step j's grid restored by the previous launch
     for (int j = n - 1; j >= 0; --j) {
         const int t = t0 + j, nxt = 1 - cur;
         const Slot<D, T> a = slot<D, T>(seg, j, B, P);
         const Slot<D, T> A1 = slot<D, T>(adj, cur, B, P), A0 = slot<D, T>(adj, nxt, B, P);
         PView<D, T> pv{a.x, a.v, a.C, a.F, P};
-        if (b->restore_from == 1 || b->grid_save) {
-            hipLaunchKernelGGL((k_grid_restore<D, T>), gg, dim3(PB), 0, s, mp, (T *)b->grid,
-                               (T *)b->grid_vo, bk,
-                               b->restore_from == 1 ? hsave_at<T>(b, t) : save_at<D, T>(b, j), 0);
+        if (restore) {
+            if (!restored)
+                hipLaunchKernelGGL((k_grid_restore<D, T>), gg, dim3(PB), 0, s, mp, (T *)b->grid,
+                                   (T *)b->grid_vo, bk, saved(j), 0);
         } else {
             DL_UNROLL_DISPATCH(U_, hipLaunchKernelGGL((k_p2g_b<D, T, false, U_>), pw, dim3(PB),
                                                       tile_bytes, s, mp, pv,
@@ -1623,12 +1657,23 @@ int backward_impl(const dl_mpm_batch *b, const dl_mpm_slots *seg, const dl_mpm_s
         hipLaunchKernelGGL((k_p2g_adj_b<D, T>), pg, dim3(PB), O * sizeof(T), s, mp, pv,
                            act + (size_t)t * B * O, b->act_id, O, (const T *)b->grid_grad, A1.F,
                            A0.x, A0.v, A0.C, A0.F, (T *)b->act_grad, b->blk_count);
-        if (b->in_grad)
+        restored = false;
+        if (b->in_grad && restore && j > 0) {
+            hipLaunchKernelGGL((k_restore_ctrl_adj<D, T>), dim3(gg.x + 1, B), dim3(PB),
+                               (I + 3 * H + O) * sizeof(T), s, mp, (T *)b->grid, (T *)b->grid_vo,
+                               bk, saved(j - 1), (int)gg.x, ctrl,
+                               (const T *)b->inp + (size_t)t * B * I, (T *)b->act_grad,
+                               (T *)b->in_grad + (size_t)t * B * I,
+                               (T *)b->hid_grad + (size_t)t * B * H,
+                               (T *)b->out_grad + (size_t)t * B * O);
+            restored = true;
+        } else if (b->in_grad) {
             hipLaunchKernelGGL(k_ctrl_adj<T>, dim3(B), dim3(64), (I + 3 * H + O) * sizeof(T), s,
                                ctrl, (const T *)b->inp + (size_t)t * B * I, (T *)b->act_grad,
                                (T *)b->in_grad + (size_t)t * B * I,
                                (T *)b->hid_grad + (size_t)t * B * H,
                                (T *)b->out_grad + (size_t)t * B * O);
+        }
         cur = nxt;
     }
     if (b->in_grad) {
