@@ -67,6 +67,8 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-mpm", action="store_true")
+    ap.add_argument("--skip-ckpt", action="store_true",
+                    help="skip the checkpointed (segment recompute) comparison runs")
     ap.add_argument("--mpm-robots", type=int, default=8)
     ap.add_argument("--mpm-horizon", type=int, default=2000)
     ap.add_argument("--mpm-ckpt", type=int, default=100)
@@ -171,6 +173,42 @@ def main():
     loss = float(batch.loss.double().mean().item())
     gnorm = float(batch.grad_out.double().norm().item())
 
+    # ---- the checkpointed form of the same rollout (variant 3: state every
+    # K steps, each segment recomputed in the adjoint), for comparison
+    ckpt = None
+    if stored and not args.skip_ckpt:
+        g_stored = batch.grad_out.double().clone()
+        sc_stored = batch.scratch.numel() * batch.scratch.element_size()
+        cbat = MassSpringBatch([design], [0] * R, T, _physics(qc), lay, tg, [x0] * R, act="tanh",
+                               ckpt_every=K, dtype=torch.float32, block=args.block,
+                               loss_window=W, variant=3)
+        cbat.set_weights(0, *w)
+        cbat.reset()
+        cbat.forward()
+        cbat.backward(loss_grad=lg)
+        torch.cuda.synchronize()
+        cf, cbk = [], []
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            cbat.reset()
+            ev[0].record()
+            cbat.forward()
+            ev[1].record()
+            cbat.backward(loss_grad=lg)
+            ev[3].record()
+            torch.cuda.synchronize()
+            cf.append(ev[0].elapsed_time(ev[1]))
+            cbk.append(ev[1].elapsed_time(ev[3]))
+        cwall = time.perf_counter() - t1
+        gc = cbat.grad_out.double()
+        ckpt = {"variant": 3, "ckpt_every": cbat.K, "value": robot_steps * args.steps / cwall,
+                "ms_forward": float(np.mean(cf)), "ms_backward": float(np.mean(cbk)),
+                "scratch_gb": cbat.scratch.numel() * cbat.scratch.element_size() / 1e9,
+                "scratch_gb_stored": sc_stored / 1e9,
+                "grad_rel_diff_vs_stored": float((gc - g_stored).norm() / g_stored.norm())}
+        del cbat
+        torch.cuda.empty_cache()
+
     # ---- plain form: per-step kernels, one robot per launch
     plain = None
     if not args.skip_plain and args.plain_robots > 0:
@@ -196,31 +234,37 @@ def main():
         from diffloco import mpm as dm
         Bm, Tm, Km = args.mpm_robots, args.mpm_horizon, args.mpm_ckpt
         pos, ids, n_act = dm.quadruped3d_particles(cells_per_cube=6, n_grid=64)
-        mb = dm.MPMBatch(dm.quadruped3d_params(64), pos, ids, n_act, Bm, Tm, K=Km,
-                         window=Tm // 2, dtype=torch.float32, scatter=args.mpm_scatter,
-                         tile_nodes=args.mpm_tile, store_states=bool(args.mpm_store_states))
-        rngm = np.random.default_rng(7)
-        mb.set_targets(np.stack([rngm.uniform(-0.08, 0.08, Bm), np.zeros(Bm),
-                                 rngm.uniform(-0.08, 0.08, Bm)], 1), rngm.uniform(0.1, 0.25, Bm))
-        mw = init_weights(mb.I, 64, n_act, seed=8)
-        mb.set_weights(*mw)
-        mb.forward()
-        mb.backward()
-        torch.cuda.synchronize()
-        mb.check()
-        active = mb.active_nodes()
-        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
-        mf, mbk = [], []
-        for _ in range(args.mpm_steps):
-            e0.record()
+
+        def mpm_run(store):
+            mb = dm.MPMBatch(dm.quadruped3d_params(64), pos, ids, n_act, Bm, Tm, K=Km,
+                             window=Tm // 2, dtype=torch.float32, scatter=args.mpm_scatter,
+                             tile_nodes=args.mpm_tile, store_states=store)
+            rngm = np.random.default_rng(7)
+            mb.set_targets(np.stack([rngm.uniform(-0.08, 0.08, Bm), np.zeros(Bm),
+                                     rngm.uniform(-0.08, 0.08, Bm)], 1),
+                           rngm.uniform(0.1, 0.25, Bm))
+            mw = init_weights(mb.I, 64, n_act, seed=8)
+            mb.set_weights(*mw)
             mb.forward()
-            e1.record()
             mb.backward()
-            e2.record()
             torch.cuda.synchronize()
             mb.check()
-            mf.append(e0.elapsed_time(e1))
-            mbk.append(e1.elapsed_time(e2))
+            e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+            mf, mbk = [], []
+            for _ in range(args.mpm_steps):
+                e0.record()
+                mb.forward()
+                e1.record()
+                mb.backward()
+                e2.record()
+                torch.cuda.synchronize()
+                mb.check()
+                mf.append(e0.elapsed_time(e1))
+                mbk.append(e1.elapsed_time(e2))
+            return mb, mf, mbk
+
+        mb, mf, mbk = mpm_run(bool(args.mpm_store_states))
+        active = mb.active_nodes()
         P = len(pos)
         ms_rollout = float(np.mean(mf) + np.mean(mbk))
         pps = Bm * P * Tm / (ms_rollout * 1e-3)
@@ -243,7 +287,20 @@ def main():
                "roofline": {"bound": "hbm", "achieved": bpp * pps / 1e9, "peak": HBM_PEAK_GBS,
                             "unit": "GB/s", "frac": bpp * pps / 1e9 / HBM_PEAK_GBS,
                             "algorithmic_bytes_per_particle_step": bpp, "traffic": None},
-               "mean_loss": float(mb.loss.item())}
+               "mean_loss": float(mb.loss.item()), "device_gb": mb.nbytes() / 1e9}
+        if mb.store_states and not args.skip_ckpt:
+            # the checkpointed form: particle state every K steps, each
+            # segment replayed from its checkpoint in the backward
+            del mb
+            torch.cuda.empty_cache()
+            mc, cf, cb = mpm_run(False)
+            mpm["checkpointed"] = {
+                "value": Bm * P * Tm / ((np.mean(cf) + np.mean(cb)) * 1e-3),
+                "ckpt_every": Km, "ms_forward": float(np.mean(cf)),
+                "ms_backward": float(np.mean(cb)), "replayed_segments": int(sum(mc.replay)),
+                "mean_loss": float(mc.loss.item()), "device_gb": mc.nbytes() / 1e9}
+            del mc
+            torch.cuda.empty_cache()
 
     # ---- uneven batch: 512 seeded random voxel robots, size classes, HIP graphs
     uneven = None
@@ -380,6 +437,7 @@ def main():
         "uneven": uneven,
         "serve": serve,
         "check": {"mean_loss": loss, "grad_norm": gnorm},
+        "checkpointed": ckpt,
     }
     print(json.dumps(out))
 
