@@ -58,17 +58,19 @@ def main():
     ap.add_argument("--horizon", type=int, default=1000)
     ap.add_argument("--ckpt", type=int, default=50)
     ap.add_argument("--block", type=int, default=256)
-    ap.add_argument("--variant", type=int, default=4,
-                    help="mass-spring kernel variant (0 auto, 3 pipelined segment recompute, "
-                         "4 every step's rows kept by the forward)")
+    ap.add_argument("--variant", type=int, default=3,
+                    help="mass-spring kernel variant (0 auto, 3 checkpoints every --ckpt steps "
+                         "and pipelined segment recompute, 4 every step's rows kept by the "
+                         "forward)")
     ap.add_argument("--plain-robots", type=int, default=2)
     ap.add_argument("--cpu-robots", type=int, default=96)
     ap.add_argument("--cpu-horizon", type=int, default=1000)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-mpm", action="store_true")
-    ap.add_argument("--skip-ckpt", action="store_true",
-                    help="skip the checkpointed (segment recompute) comparison runs")
+    ap.add_argument("--skip-alt", action="store_true",
+                    help="skip the comparison runs of the other memory form (stored rows / "
+                         "states against checkpoints and segment recompute)")
     ap.add_argument("--mpm-robots", type=int, default=8)
     ap.add_argument("--mpm-horizon", type=int, default=2000)
     ap.add_argument("--mpm-ckpt", type=int, default=100)
@@ -77,11 +79,15 @@ def main():
                     help="0: global-atomic commits (staged run sums, LDS tile fallback), "
                          "1: tiles stored and gathered by the grid pass")
     ap.add_argument("--mpm-tile", type=int, default=2048, help="LDS scatter tile, nodes")
-    ap.add_argument("--mpm-store-states", type=int, default=1,
-                    help="MPM forward keeps every step's particle state (adjoint without replay)")
+    ap.add_argument("--mpm-store-states", type=int, default=0,
+                    help="1: the MPM forward keeps every step's particle state (adjoint without "
+                         "replay); 0: checkpoints every --mpm-ckpt steps")
     ap.add_argument("--skip-uneven", action="store_true")
     ap.add_argument("--uneven-robots", type=int, default=512)
     ap.add_argument("--uneven-iters", type=int, default=3)
+    ap.add_argument("--uneven-stored", type=int, default=0,
+                    help="1: uneven forwards keep every step's adjoint rows; 0: checkpoints "
+                         "every 50 steps")
     ap.add_argument("--skip-serve", action="store_true")
     ap.add_argument("--serve-rate", type=float, default=25.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-duration", type=float, default=2.0, help="seconds of arrivals")
@@ -173,15 +179,17 @@ def main():
     loss = float(batch.loss.double().mean().item())
     gnorm = float(batch.grad_out.double().norm().item())
 
-    # ---- the checkpointed form of the same rollout (variant 3: state every
-    # K steps, each segment recomputed in the adjoint), for comparison
-    ckpt = None
-    if stored and not args.skip_ckpt:
-        g_stored = batch.grad_out.double().clone()
-        sc_stored = batch.scratch.numel() * batch.scratch.element_size()
+    # ---- the same rollout in the other memory form: checkpoints every K steps
+    # with each segment recomputed in the adjoint (variant 3) against every
+    # step's rows kept by the forward (variant 4)
+    alt = None
+    alt_variant = {3: 4, 4: 3}.get(batch.variant)
+    if alt_variant is not None and not args.skip_alt:
+        g_main = batch.grad_out.double().clone()
+        sc_main = batch.scratch.numel() * batch.scratch.element_size()
         cbat = MassSpringBatch([design], [0] * R, T, _physics(qc), lay, tg, [x0] * R, act="tanh",
                                ckpt_every=K, dtype=torch.float32, block=args.block,
-                               loss_window=W, variant=3)
+                               loss_window=W, variant=alt_variant)
         cbat.set_weights(0, *w)
         cbat.reset()
         cbat.forward()
@@ -201,11 +209,12 @@ def main():
             cbk.append(ev[1].elapsed_time(ev[3]))
         cwall = time.perf_counter() - t1
         gc = cbat.grad_out.double()
-        ckpt = {"variant": 3, "ckpt_every": cbat.K, "value": robot_steps * args.steps / cwall,
-                "ms_forward": float(np.mean(cf)), "ms_backward": float(np.mean(cbk)),
-                "scratch_gb": cbat.scratch.numel() * cbat.scratch.element_size() / 1e9,
-                "scratch_gb_stored": sc_stored / 1e9,
-                "grad_rel_diff_vs_stored": float((gc - g_stored).norm() / g_stored.norm())}
+        alt = {"variant": alt_variant, "ckpt_every": cbat.K,
+               "value": robot_steps * args.steps / cwall,
+               "ms_forward": float(np.mean(cf)), "ms_backward": float(np.mean(cbk)),
+               "scratch_gb": cbat.scratch.numel() * cbat.scratch.element_size() / 1e9,
+               "scratch_gb_main": sc_main / 1e9,
+               "grad_rel_diff_vs_main": float((gc - g_main).norm() / g_main.norm())}
         del cbat
         torch.cuda.empty_cache()
 
@@ -288,16 +297,18 @@ def main():
                             "unit": "GB/s", "frac": bpp * pps / 1e9 / HBM_PEAK_GBS,
                             "algorithmic_bytes_per_particle_step": bpp, "traffic": None},
                "mean_loss": float(mb.loss.item()), "device_gb": mb.nbytes() / 1e9}
-        if mb.store_states and not args.skip_ckpt:
-            # the checkpointed form: particle state every K steps, each
-            # segment replayed from its checkpoint in the backward
+        if not args.skip_alt:
+            # the other memory form: every step's particle state stored (the
+            # adjoint without replay) against checkpoints every K steps
+            main_store = mb.store_states
             del mb
             torch.cuda.empty_cache()
-            mc, cf, cb = mpm_run(False)
-            mpm["checkpointed"] = {
+            mc, cf, cb = mpm_run(not main_store)
+            mpm["checkpointed" if main_store else "stored_states"] = {
                 "value": Bm * P * Tm / ((np.mean(cf) + np.mean(cb)) * 1e-3),
-                "ckpt_every": Km, "ms_forward": float(np.mean(cf)),
-                "ms_backward": float(np.mean(cb)), "replayed_segments": int(sum(mc.replay)),
+                "store_states": mc.store_states, "ckpt_every": Km,
+                "ms_forward": float(np.mean(cf)), "ms_backward": float(np.mean(cb)),
+                "replayed_segments": int(sum(mc.replay)),
                 "mean_loss": float(mc.loss.item()), "device_gb": mc.nbytes() / 1e9}
             del mc
             torch.cuda.empty_cache()
@@ -319,50 +330,63 @@ def main():
         # springs (dense voxel bodies), also in the FP64 oracle
         phys_u = _physics(qc)
         phys_u.dashpot = UNEVEN_DASHPOT
-        ub = UnevenBatch(designs_u, hz, phys_u, lay, tgu, ckpt_every=50, loss_window=100)
-        for r, d in enumerate(designs_u):
-            ub.set_weights(r, *init_weights(n_inputs(lay, d.n_vert), 64, d.n_act, seed=1000 + r))
-        t_tune = time.perf_counter()
-        ub.tune()
-        t_tune = time.perf_counter() - t_tune
-        ub.prepare(np.full(Ru, 1.0 / Ru))
-        ub.run_iteration()
-        torch.cuda.synchronize()
-        t3 = time.perf_counter()
-        ub.run_iteration()
-        torch.cuda.synchronize()
-        eager_ms = (time.perf_counter() - t3) * 1e3
-        ub.capture()
-        ub.replay()
-        torch.cuda.synchronize()
-        t3 = time.perf_counter()
-        for _ in range(args.uneven_iters):
+
+        def uneven_run(stored):
+            ub = UnevenBatch(designs_u, hz, phys_u, lay, tgu, ckpt_every=50, loss_window=100,
+                             stored=stored)
+            for r, d in enumerate(designs_u):
+                ub.set_weights(r, *init_weights(n_inputs(lay, d.n_vert), 64, d.n_act, seed=1000 + r))
+            t_tune = time.perf_counter()
+            ub.tune()
+            t_tune = time.perf_counter() - t_tune
+            ub.prepare(np.full(Ru, 1.0 / Ru))
+            ub.run_iteration()
+            torch.cuda.synchronize()
+            t3 = time.perf_counter()
+            ub.run_iteration()
+            torch.cuda.synchronize()
+            eager_ms = (time.perf_counter() - t3) * 1e3
+            ub.capture()
             ub.replay()
-        torch.cuda.synchronize()
-        graph_ms = (time.perf_counter() - t3) * 1e3 / args.uneven_iters
-        rs = ub.robot_steps
-        ubytes = 0.0
-        for d, T_r in zip(designs_u, hz):
-            I_r = n_inputs(lay, d.n_vert)
-            f_r, b_r = algorithmic_bytes(d.n_vert, d.n_spring, I_r, 64, d.n_act,
-                                         T_r if ub.stored else 50, 1, stored=ub.stored)
-            ubytes += (f_r + b_r) * T_r
-        uval = rs / (graph_ms * 1e-3)
-        nv = np.array([d.n_vert for d in designs_u])
-        uneven = {"value": uval, "unit": "robot-steps/s", "robots": Ru, "robot_steps": rs,
-                  "n_vert": [int(nv.min()), float(nv.mean()), int(nv.max())],
-                  "horizon": [int(min(hz)), int(max(hz))], "dtype": "f32",
-                  "ms_per_iteration_graph": graph_ms, "ms_per_iteration_eager": eager_ms,
-                  "autotune_s": t_tune,
-                  "classes": [{"name": bk["name"], "robots": len(bk["ids"]), "launch": bk["config"]}
-                              for bk in ub.buckets],
-                  "roofline": {"bound": "hbm", "achieved": ubytes / (graph_ms * 1e-3) / 1e9,
-                               "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                               "frac": ubytes / (graph_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                               "algorithmic_bytes_per_iteration": ubytes, "traffic": None},
-                  "dashpot_impulse": UNEVEN_DASHPOT,
-                  "nonfinite_robots": int((~torch.isfinite(ub.losses())).sum().item()),
-                  "mean_loss": float(ub.losses().double().mean().item())}
+            torch.cuda.synchronize()
+            t3 = time.perf_counter()
+            for _ in range(args.uneven_iters):
+                ub.replay()
+            torch.cuda.synchronize()
+            graph_ms = (time.perf_counter() - t3) * 1e3 / args.uneven_iters
+            rs = ub.robot_steps
+            ubytes = 0.0
+            for d, T_r in zip(designs_u, hz):
+                I_r = n_inputs(lay, d.n_vert)
+                f_r, b_r = algorithmic_bytes(d.n_vert, d.n_spring, I_r, 64, d.n_act,
+                                             T_r if ub.stored else 50, 1, stored=ub.stored)
+                ubytes += (f_r + b_r) * T_r
+            uval = rs / (graph_ms * 1e-3)
+            nv = np.array([d.n_vert for d in designs_u])
+            out = {"value": uval, "unit": "robot-steps/s", "robots": Ru, "robot_steps": rs,
+                      "n_vert": [int(nv.min()), float(nv.mean()), int(nv.max())],
+                      "horizon": [int(min(hz)), int(max(hz))], "dtype": "f32",
+                      "ms_per_iteration_graph": graph_ms, "ms_per_iteration_eager": eager_ms,
+                      "autotune_s": t_tune,
+                      "classes": [{"name": bk["name"], "robots": len(bk["ids"]), "launch": bk["config"]}
+                                  for bk in ub.buckets],
+                      "roofline": {"bound": "hbm", "achieved": ubytes / (graph_ms * 1e-3) / 1e9,
+                                   "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                   "frac": ubytes / (graph_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                   "algorithmic_bytes_per_iteration": ubytes, "traffic": None},
+                      "dashpot_impulse": UNEVEN_DASHPOT,
+                      "nonfinite_robots": int((~torch.isfinite(ub.losses())).sum().item()),
+                      "mean_loss": float(ub.losses().double().mean().item()), "stored": ub.stored}
+            del ub
+            torch.cuda.empty_cache()
+            return out
+
+        uneven = uneven_run(bool(args.uneven_stored))
+        if not args.skip_alt:
+            ua = uneven_run(not uneven["stored"])
+            uneven["checkpointed" if uneven["stored"] else "stored_rows"] = {
+                k: ua[k] for k in ("value", "stored", "ms_per_iteration_graph", "autotune_s",
+                                   "classes", "nonfinite_robots", "mean_loss")}
 
     # ---- request loop: Poisson arrivals, 70 % random mass-spring robots, 30 % MPM bodies
     serve = None
@@ -416,8 +440,8 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded targets, N(0,1/fan_in) weights)",
-        "config": {"workload": "quadruped mass-spring rollout fwd+adjoint, 1024 tasks x 1000 steps, "
-                   "MLP 179-64-4 tanh, "
+        "config": {"workload": f"quadruped mass-spring rollout fwd+adjoint, {R} tasks x {T} steps, "
+                   f"MLP {I}-{H}-{O} tanh, "
                    + ("every step's adjoint rows kept by the forward (no recompute)" if stored
                       else f"checkpoints every {batch.K} steps, segments recomputed"),
                    "robots": R, "horizon": T, "ckpt_every": batch.K, "variant": batch.variant,
@@ -437,7 +461,7 @@ def main():
         "uneven": uneven,
         "serve": serve,
         "check": {"mean_loss": loss, "grad_norm": gnorm},
-        "checkpointed": ckpt,
+        "stored_rows" if alt_variant == 4 else "checkpointed": alt,
     }
     print(json.dumps(out))
 

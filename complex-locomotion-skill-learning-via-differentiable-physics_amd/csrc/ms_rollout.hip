@@ -1573,8 +1573,8 @@ __device__ __forceinline__ void wave_bwd_begin(const RArgs<T> &A, const Rob<T> &
 }
 
 // Recompute steps [s0, t1) of the segment starting at checkpoint s0 / K into
-// the scratch rows of R (slot t - s0).
-template <typename T, int OMAX>
+// the scratch rows of R (slot t - s0); ROWS: with the per-vertex impulse sums.
+template <typename T, int OMAX, bool ROWS = false>
 __device__ __forceinline__ void wave_recompute(const RArgs<T> &A, const Rob<T> &R, WSm<T> &m,
                                                const WRegs<T, 1, OMAX> &w, const LaneTopo<T> &tp,
                                                const T *sin_tab, int lane, int s0, int t1) {
@@ -1585,8 +1585,8 @@ __device__ __forceinline__ void wave_recompute(const RArgs<T> &A, const Rob<T> &
     wave_sync();
     T dummy = T(0);
     for (int t = s0; t < t1; ++t)
-        wave_fwd_step<T, OMAX, BSK, BDK>(A, R, m, w, tp, sin_tab, lane, t, MODE_REC,
-                                                   t - s0, dummy);
+        wave_fwd_step<T, OMAX, BSK, BDK, ROWS>(A, R, m, w, tp, sin_tab, lane, t, MODE_REC,
+                                               t - s0, dummy);
     // the scratch stores complete before any lane reads them back
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
 }
@@ -1694,6 +1694,8 @@ __device__ __forceinline__ int bwd_segments(int tb, int budget, int K) {
 // One workgroup barrier per segment; every wavefront runs the workgroup's
 // largest segment count, idle once its robot is done (or absent), so the
 // barrier counts always agree.
+// The recompute keeps each step's per-vertex impulse sums with the rows, so
+// the adjoint wavefront runs no spring pass of its own.
 // STORED: the forward wrote every step's rows (scratch of K >= horizon rows,
 // slot = step), so nothing is recomputed: the adjoint wavefront reverses the
 // horizon in chunks of kRowsChunk steps and the second wavefront folds the
@@ -1745,7 +1747,8 @@ __global__ void __launch_bounds__(512) k_wave_bwd_pipe(RArgs<T> A, int n_robots)
         wave_load_weights(R, A.H, lane, w);
         lane_topo_load(R, lane, tp);
         if (role == 0) wave_bwd_begin(A, R, m, r, lane);
-        else if (!STORED) wave_recompute(A, R, m, w, tp, sh.sin_tab, lane, (tb - 1) / K * K, tb);
+        else if (!STORED)
+            wave_recompute<T, OMAX, true>(A, R, m, w, tp, sh.sin_tab, lane, (tb - 1) / K * K, tb);
     }
     __syncthreads();
     int buf = 0;
@@ -1766,7 +1769,7 @@ __global__ void __launch_bounds__(512) k_wave_bwd_pipe(RArgs<T> A, int n_robots)
         } else if (role == 0) {
             if (go) {
                 carve_scratch(R, buf ? scr1 : scr0, K, A.H);
-                wave_bwd_steps<T, IMAX, OMAX>(A, R, m, w, tp, r, lane, s0, lo, tb);
+                wave_bwd_steps<T, IMAX, OMAX, true>(A, R, m, w, tp, r, lane, s0, lo, tb);
             }
         } else {
             carve_scratch(R, buf ? scr0 : scr1, K, A.H);
@@ -1777,7 +1780,7 @@ __global__ void __launch_bounds__(512) k_wave_bwd_pipe(RArgs<T> A, int n_robots)
                 wave_sync();
             }
             if (go && lo == s0 && s0 > 0 && rest > 0)
-                wave_recompute(A, R, m, w, tp, sh.sin_tab, lane, s0 - K, s0);
+                wave_recompute<T, OMAX, true>(A, R, m, w, tp, sh.sin_tab, lane, s0 - K, s0);
         }
         fs0 = s0;
         flo = lo;
