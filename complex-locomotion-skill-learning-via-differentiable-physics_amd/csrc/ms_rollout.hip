@@ -629,7 +629,8 @@ __device__ void seg_wgrad(const RArgs<T> &A, const Rob<T> &R, int n) {
 }
 
 // STORED: the forward kept every step's rows (K >= horizon): no segment
-// recompute.
+// recompute.  Otherwise each segment is recomputed from its checkpoint with
+// the same rows, per-vertex impulse sums included.
 template <typename T, int LB, bool STORED>
 __global__ void __launch_bounds__(LB) k_rollout_bwd(RArgs<T> A) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -688,9 +689,10 @@ __global__ void __launch_bounds__(LB) k_rollout_bwd(RArgs<T> A) {
                 m.vs[i] = ck[3 * N + i];
             }
             __syncthreads();
-            for (int t = s0; t < tb; ++t) fwd_step(A, R, m, t, MODE_REC, t - s0, dummy);
+            // the recompute keeps the impulse sums too: no spring pass below
+            for (int t = s0; t < tb; ++t) fwd_step<T, true>(A, R, m, t, MODE_REC, t - s0, dummy);
         }
-        for (int t = tb - 1; t >= lo; --t) bwd_step<T, STORED>(A, R, m, r, t, t - s0);
+        for (int t = tb - 1; t >= lo; --t) bwd_step<T, true>(A, R, m, r, t, t - s0);
         __syncthreads();
         // the weight-gradient fold reads slots lo - s0 .. tb - s0 - 1
         {
