@@ -1807,16 +1807,37 @@ __global__ void __launch_bounds__(512) k_wave_bwd_pipe(RArgs<T> A, int n_robots)
 }
 
 // Fixed-order sum of per-robot gradient partials into per-design gradients.
+// A workgroup (256 threads) takes kWrEntries entries at a time: thread (g, j)
+// sums robots g, g + kWrPhases, g + 2 kWrPhases, ... of the design's list for
+// entry j, and the kWrPhases partials are added in a fixed tree, so the
+// result does not depend on the launch shape.
+constexpr int kWrPhases = 8, kWrEntries = 32;
 template <typename T>
-__global__ void k_wgrad_reduce(const int32_t *lptr, const int32_t *list, const int64_t *roff,
-                               const int64_t *doff, const int64_t *dsize, const T *wg, T *out) {
+__global__ void __launch_bounds__(256)
+    k_wgrad_reduce(const int32_t *lptr, const int32_t *list, const int64_t *roff,
+                   const int64_t *doff, const int64_t *dsize, const T *wg, T *out) {
+    static_assert(kWrPhases * kWrEntries == 256, "one workgroup of 256 threads");
+    __shared__ T part[kWrPhases][kWrEntries];
     const int d = blockIdx.y;
     const int64_t n = dsize[d];
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
-         e += (int64_t)gridDim.x * blockDim.x) {
+    const int j = threadIdx.x % kWrEntries, g = threadIdx.x / kWrEntries;
+    const int q0 = lptr[d], q1 = lptr[d + 1];
+    for (int64_t e0 = (int64_t)blockIdx.x * kWrEntries; e0 < n;
+         e0 += (int64_t)gridDim.x * kWrEntries) {
+        const int64_t e = e0 + j;
         T s = T(0);
-        for (int q = lptr[d]; q < lptr[d + 1]; ++q) s += wg[roff[list[q]] + e];
-        out[doff[d] + e] = s;
+        if (e < n) {
+#pragma unroll 4
+            for (int q = q0 + g; q < q1; q += kWrPhases) s += wg[roff[list[q]] + e];
+        }
+        part[g][j] = s;
+        __syncthreads();
+        if (g == 0 && e < n) {
+            const T a = (part[0][j] + part[1][j]) + (part[2][j] + part[3][j]);
+            const T b = (part[4][j] + part[5][j]) + (part[6][j] + part[7][j]);
+            out[doff[d] + e] = a + b;
+        }
+        __syncthreads();
     }
 }
 
@@ -2025,12 +2046,15 @@ int32_t dl_wgrad_reduce(int32_t dtype, int32_t n_robots, const int32_t *lptr, co
     DL_CHECK_ARG(n_robots > 0 && n_designs > 0 && n_designs <= 65535 && lptr && list && roff &&
                      doff && dsize && wg && out,
                  "dl_wgrad_reduce: bad arguments");
+    // one design: enough workgroups for its entries in one pass; many
+    // designs: fewer per design
+    const int gx = n_designs == 1 ? 512 : std::max(16, 8192 / n_designs);
     if (dtype == DL_F32)
-        hipLaunchKernelGGL(k_wgrad_reduce<float>, dim3(64, n_designs), dim3(256), 0,
+        hipLaunchKernelGGL(k_wgrad_reduce<float>, dim3(gx, n_designs), dim3(256), 0,
                            (hipStream_t)stream, lptr, list, roff, doff, dsize, (const float *)wg,
                            (float *)out);
     else if (dtype == DL_F64)
-        hipLaunchKernelGGL(k_wgrad_reduce<double>, dim3(64, n_designs), dim3(256), 0,
+        hipLaunchKernelGGL(k_wgrad_reduce<double>, dim3(gx, n_designs), dim3(256), 0,
                            (hipStream_t)stream, lptr, list, roff, doff, dsize, (const double *)wg,
                            (double *)out);
     else
