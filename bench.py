@@ -49,6 +49,15 @@ def algorithmic_bytes(N, S, I, H, O, K, R_d, dtype_bytes=4, stored=False):
     return fwd, bwd
 
 
+def mpm_bytes(g_per_p, K, f_rec=1.0):
+    """MLS-MPM bytes per particle-step, forward + adjoint (state round-trip
+    model): forward 196 + 56 G/P (G = active grid nodes), adjoint
+    292 + 112 G/P, and for the fraction f_rec of segments recomputed in the
+    backward the forward again plus the checkpoint (96 / K); stored states
+    skip the recompute (f_rec = 0)."""
+    return 488 + 168 * g_per_p + f_rec * (196 + 56 * g_per_p + 96 / K)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -278,12 +287,8 @@ def main():
         ms_rollout = float(np.mean(mf) + np.mean(mbk))
         pps = Bm * P * Tm / (ms_rollout * 1e-3)
         gp = active / P
-        # algorithmic bytes per particle-step, fwd + bwd: forward 196 + 56 G/P,
-        # adjoint 292 + 112 G/P, and per recomputed segment the forward again
-        # plus the checkpoint (96 / K); stored states skip the recompute of
-        # every segment whose grids were all saved
         f_rec = 1.0 - (sum(mb.replay) / mb.nck if mb.store_states else 0.0)
-        bpp = 488 + 168 * gp + f_rec * (196 + 56 * gp + 96 / Km)
+        bpp = mpm_bytes(gp, Km, f_rec)
         mpm = {"value": pps, "unit": "particle-steps/s", "robots": Bm, "particles": P,
                "grid": 64, "horizon": Tm, "ckpt_every": Km, "dtype": "f32",
                "scatter": ["staged run sums + per-node gather, global atomics",
