@@ -833,12 +833,11 @@ __global__ __launch_bounds__(PB) void k_grid_restore(MP<T> p, T *grid, T *gvo, B
     grid_restore<D, T>(p, grid, gvo, bk, sv, vo_only, blockIdx.x, gridDim.x);
 }
 
-// F of the next state for every particle, (I + dt C) F: the part of P2G that
-// a replayed recompute (saved grid velocities + G2P) still needs.
+// F of the next state of particle q of robot k, (I + dt C) F: the part of
+// P2G that a replayed recompute (saved grid velocities + G2P) still needs.
 template <int D, typename T>
-__global__ __launch_bounds__(PB) void k_fupdate(MP<T> p, int P, const T *C, const T *F, T *F_out) {
-    const int k = blockIdx.y, q = blockIdx.x * PB + threadIdx.x;
-    if (q >= P) return;
+__device__ __forceinline__ void f_update(const MP<T> &p, int P, const T *C, const T *F, T *F_out,
+                                         int k, int q) {
     const size_t bm = (size_t)k * D * D * P;
     Mat<D, T> Cm, Fm;
 #pragma unroll
@@ -851,6 +850,7 @@ __global__ __launch_bounds__(PB) void k_fupdate(MP<T> p, int P, const T *C, cons
     store_mat<D, T>(F_out, k, q, P, f_next<D, T>(Cm, Fm, p), false);
 }
 
+
 // ---------------------------------------------------------------- G2P
 template <typename T> struct SenseOut {
     T *sens;               // row t+1: [B][3*Gs + 6]
@@ -858,10 +858,13 @@ template <typename T> struct SenseOut {
     int Gs;
 };
 
-template <int D, typename T, bool SENSE>
+// FUPD: also the F update of each particle (f_update: C_in, F_in -> F_out),
+// for the replayed recompute, which has no P2G to do it.
+template <int D, typename T, bool SENSE, bool FUPD = false>
 __global__ __launch_bounds__(PB) void k_g2p_b(MP<T> p, int P, const T *x, const T *gvo, T *x_out,
                                               T *v_out, T *C_out, int32_t *count,
-                                              SenseOut<T> so) {
+                                              SenseOut<T> so, const T *C_in = nullptr,
+                                              const T *F_in = nullptr, T *F_out = nullptr) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *acc = reinterpret_cast<T *>(smem_raw);
     const int k = blockIdx.y, q = blockIdx.x * PB + threadIdx.x;
@@ -872,6 +875,8 @@ __global__ __launch_bounds__(PB) void k_g2p_b(MP<T> p, int P, const T *x, const 
     // the grid pass that read the active list has finished: empty it
     if (blockIdx.x == 0 && threadIdx.x == 0) count[k] = 0;
     const bool valid = q < P;
+    if constexpr (FUPD)
+        if (valid) f_update<D, T>(p, P, C_in, F_in, F_out, k, q);
     T nx[D], nv[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) nx[d] = nv[d] = T(0);
@@ -1552,10 +1557,10 @@ int forward_impl(const dl_mpm_batch *b, const dl_mpm_slots *st, int t0, int n, i
         if (flags & DL_MPM_REPLAY) {
             hipLaunchKernelGGL((k_grid_restore<D, T>), gg, dim3(PB), 0, s, mp, (T *)b->grid,
                                (T *)b->grid_vo, bk, hsave_at<T>(b, t), 1);
-            hipLaunchKernelGGL((k_fupdate<D, T>), pg, dim3(PB), 0, s, mp, P, a.C, a.F, c.F);
-            hipLaunchKernelGGL((k_g2p_b<D, T, false>), pg, dim3(PB), 0, s, mp, P, a.x,
+            hipLaunchKernelGGL((k_g2p_b<D, T, false, true>), pg, dim3(PB), 0, s, mp, P, a.x,
                                (const T *)b->grid_vo, c.x, c.v, c.C, b->blk_count,
-                               SenseOut<T>{nullptr, nullptr, 0});
+                               SenseOut<T>{nullptr, nullptr, 0}, (const T *)a.C,
+                               (const T *)a.F, c.F);
             continue;
         }
         if (control)
