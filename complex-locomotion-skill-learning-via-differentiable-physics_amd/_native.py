@@ -97,6 +97,7 @@ _SIGS = {
     "dl_nn_input_grad": ([_I, POINTER(SensorLayoutC), _I, _I, _P, _P, _P, _P, _P], c_int32),
     "dl_mlp_forward": ([_I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P], c_int32),
     "dl_mlp_backward": ([_I, _I, _I, _I, _I, _I] + [_P] * 13, c_int32),
+    "dl_mlp_backward_static": ([_I, _I, _I, _I, _I, _I] + [_P] * 12, c_int32),
     "dl_ms_advance": ([_I, POINTER(MSTopology), POINTER(MSPhysics), _I, _P, _P, _P, _P, _P, _P],
                       c_int32),
     "dl_ms_advance_grad": ([_I, POINTER(MSTopology), POINTER(MSPhysics), _I] + [_P] * 9, c_int32),

@@ -669,6 +669,64 @@ static int check_topo(const dl_ms_topology *t) {
     return 0;
 }
 
+namespace dl {
+
+// Launch the MLP adjoint with workspace `work` (B*(2H+O) reals): stage 1
+// writes every workspace entry it later reads, so the workspace needs no
+// clearing between calls.
+static int32_t mlp_backward_launch(int32_t dtype, int32_t act, int32_t B, int32_t I, int32_t H,
+                                   int32_t O, const void *in, const void *W1, const void *b1,
+                                   const void *W2, const void *b2, const void *og, void *ing,
+                                   void *W1g, void *b1g, void *W2g, void *b2g, void *work,
+                                   void *stream) {
+    DL_CHECK_ARG(B > 0 && I > 0 && H > 0 && O > 0 && in && W1 && b1 && W2 && b2 && og && W1g &&
+                     b1g && W2g && b2g && work,
+                 "dl_mlp_backward: bad arguments");
+    DL_CHECK_ARG(act == DL_ACT_SIN || act == DL_ACT_TANH, "dl_mlp_backward: bad activation");
+    DL_DISPATCH(dtype, {
+        const size_t sh = sizeof(real) * (I + 3 * H + O);
+        DL_CHECK_ARG(sh <= 65536, "dl_mlp_backward: input too wide for LDS (%d)", I);
+        hipLaunchKernelGGL(k_mlp_bwd1<real>, dim3(B), dim3(256), sh, S_(stream), act, I, H, O,
+                           (const real *)in, (const real *)W1, (const real *)b1,
+                           (const real *)W2, (const real *)b2, (const real *)og, (real *)ing,
+                           (real *)work);
+        const long n = (long)H * I + (long)O * H + H + O;
+        hipLaunchKernelGGL(k_mlp_bwd2<real>, dim3(nblk(n, 256)), dim3(256), 0, S_(stream), B, I,
+                           H, O, (const real *)in, (const real *)work, (real *)W1g, (real *)b1g,
+                           (real *)W2g, (real *)b2g);
+    });
+    DL_CHECK_LAUNCH();
+    return 0;
+}
+
+// Workspace of dl_mlp_backward_static: a file-scope device array, one copy per
+// GPU, held until the process ends.  Host code never passes `scratch` itself
+// to a kernel (in host code the name is a stand-in, not a device address):
+// it looks the array's address up once per device with hipGetSymbolAddress,
+// with that device current, and passes that address.
+static __device__ __align__(256) unsigned char scratch[DL_MLP_STATIC_WORK_BYTES];
+
+constexpr int kMaxDevices = 64;
+static void *g_scratch_addr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static int32_t static_work_ptr(void **out) {
+    int dev = -1;
+    DL_CHECK_HIP(hipGetDevice(&dev));
+    DL_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "static workspace: device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_addr[dev] == nullptr) {
+        void *p = nullptr;
+        DL_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        DL_CHECK_ARG(p != nullptr, "static workspace: no device address");
+        g_scratch_addr[dev] = p;
+    }
+    *out = g_scratch_addr[dev];
+    return 0;
+}
+
+}  // namespace dl
+
 extern "C" {
 
 int32_t dl_ms_center(int32_t dtype, int32_t B, int32_t N, const void *x_t, const void *v_t,
@@ -795,24 +853,27 @@ int32_t dl_mlp_backward(int32_t dtype, int32_t act, int32_t B, int32_t I, int32_
                            const void *in, const void *W1, const void *b1, const void *W2,
                            const void *b2, const void *og, void *ing, void *W1g, void *b1g,
                            void *W2g, void *b2g, void *work, void *stream) {
-    DL_CHECK_ARG(B > 0 && I > 0 && H > 0 && O > 0 && in && W1 && b1 && W2 && b2 && og && W1g &&
-                     b1g && W2g && b2g && work,
-                 "dl_mlp_backward: bad arguments");
-    DL_CHECK_ARG(act == DL_ACT_SIN || act == DL_ACT_TANH, "dl_mlp_backward: bad activation");
-    DL_DISPATCH(dtype, {
-        const size_t sh = sizeof(real) * (I + 3 * H + O);
-        DL_CHECK_ARG(sh <= 65536, "dl_mlp_backward: input too wide for LDS (%d)", I);
-        hipLaunchKernelGGL(k_mlp_bwd1<real>, dim3(B), dim3(256), sh, S_(stream), act, I, H, O,
-                           (const real *)in, (const real *)W1, (const real *)b1,
-                           (const real *)W2, (const real *)b2, (const real *)og, (real *)ing,
-                           (real *)work);
-        const long n = (long)H * I + (long)O * H + H + O;
-        hipLaunchKernelGGL(k_mlp_bwd2<real>, dim3(nblk(n, 256)), dim3(256), 0, S_(stream), B, I,
-                           H, O, (const real *)in, (const real *)work, (real *)W1g, (real *)b1g,
-                           (real *)W2g, (real *)b2g);
-    });
-    DL_CHECK_LAUNCH();
-    return 0;
+    DL_CHECK_ARG(work != nullptr, "dl_mlp_backward: bad arguments (work is NULL)");
+    return dl::mlp_backward_launch(dtype, act, B, I, H, O, in, W1, b1, W2, b2, og, ing, W1g, b1g,
+                                   W2g, b2g, work, stream);
+}
+
+int32_t dl_mlp_backward_static(int32_t dtype, int32_t act, int32_t B, int32_t I, int32_t H,
+                               int32_t O, const void *in, const void *W1, const void *b1,
+                               const void *W2, const void *b2, const void *og, void *ing,
+                               void *W1g, void *b1g, void *W2g, void *b2g, void *stream) {
+    DL_CHECK_ARG(dtype == DL_F32 || dtype == DL_F64, "dl_mlp_backward_static: unknown dtype %d",
+                 (int)dtype);
+    DL_CHECK_ARG(B > 0 && H > 0 && O > 0, "dl_mlp_backward_static: bad arguments");
+    const size_t need = (size_t)B * (2 * (size_t)H + (size_t)O) * (dtype == DL_F32 ? 4 : 8);
+    DL_CHECK_ARG(need <= (size_t)DL_MLP_STATIC_WORK_BYTES,
+                 "dl_mlp_backward_static: workspace of %zu bytes exceeds the static limit of %d "
+                 "bytes (B*(2H+O) reals); use dl_mlp_backward with a caller workspace",
+                 need, (int)DL_MLP_STATIC_WORK_BYTES);
+    void *work = nullptr;
+    if (int e = dl::static_work_ptr(&work)) return e;
+    return dl::mlp_backward_launch(dtype, act, B, I, H, O, in, W1, b1, W2, b2, og, ing, W1g, b1g,
+                                   W2g, b2g, work, stream);
 }
 
 int32_t dl_ms_advance(int32_t dtype, const dl_ms_topology *topo, const dl_ms_physics *phys,

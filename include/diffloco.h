@@ -132,6 +132,23 @@ int32_t dl_mlp_backward(int32_t dtype, int32_t act, int32_t B, int32_t I, int32_
                         const void *b2, const void *out_grad, void *in_grad, void *W1_grad,
                         void *b1_grad, void *W2_grad, void *b2_grad, void *work, void *stream);
 
+/* dl_mlp_backward without the caller workspace: allocates nothing (no
+ * hipMalloc, hipFree or allocating library call) and uses a library-owned
+ * device array of DL_MLP_STATIC_WORK_BYTES bytes per GPU, held until the
+ * process ends.  No counterpart in the reference: it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Requires B*(2H+O)*sizeof(real) <=
+ * DL_MLP_STATIC_WORK_BYTES, else DL_E_ARG.  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (different streams or host
+ * threads) race and must be serialised by the caller.  Results equal
+ * dl_mlp_backward's (same kernels). */
+#define DL_MLP_STATIC_WORK_BYTES (4 << 20)
+int32_t dl_mlp_backward_static(int32_t dtype, int32_t act, int32_t B, int32_t I, int32_t H,
+                               int32_t O, const void *in, const void *W1, const void *b1,
+                               const void *W2, const void *b2, const void *out_grad,
+                               void *in_grad, void *W1_grad, void *b1_grad, void *W2_grad,
+                               void *b2_grad, void *stream);
+
 /* advance(t) = apply_spring_force(t) + advance_toi(t+1)
  * (solver_mass_spring.py:86-148, :200-202), fused: reads x[t], v[t], act[t]
  * ([B][O]), writes x[t+1], v[t+1].  Spring impulses are summed per vertex in
