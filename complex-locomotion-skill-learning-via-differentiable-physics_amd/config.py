@@ -17,19 +17,24 @@ windowed COM loss.
 import copy
 import json
 import math
+import os
 
 from .robot_design import build_quadruped, build_reference
 
 
 class ConfigSim:
-    def __init__(self, config):
+    def __init__(self, config, base_dir=None):
         self._config = copy.deepcopy(config)
+        self._base_dir = base_dir
         self._add_adaptive_configs()
 
     @classmethod
     def from_file(cls, file_name):
+        """Read an experiment file; robot 103's skeleton graph is read from
+        ``skeleton.json`` beside it (the reference reads ``cfg3d/skeleton.json``,
+        where its experiment files live) unless ``robot.skeleton_file`` names one."""
         with open(file_name) as fh:
-            return cls(json.load(fh))
+            return cls(json.load(fh), base_dir=os.path.dirname(os.path.abspath(file_name)))
 
     def get_config(self):
         return self._config
@@ -47,7 +52,12 @@ class ConfigSim:
         if rob.get("design") == "quadruped":
             design = build_quadruped(rob.get("spring_stiffness", 3e4), rob.get("spring_actuation", 0.1))
         else:
-            design = build_reference(rob["robot_id"], rob["spring_stiffness"], rob["spring_actuation"])
+            skeleton = None
+            if rob["robot_id"] == 103:
+                skeleton = rob.get("skeleton_file") or os.path.join(self._base_dir or ".",
+                                                                   "skeleton.json")
+            design = build_reference(rob["robot_id"], rob["spring_stiffness"],
+                                     rob["spring_actuation"], skeleton=skeleton)
         rob["design_obj"] = design
         rob["objects"] = [tuple(p) for p in design.objects.tolist()]
         rob["springs"] = [[int(a), int(b), float(L), float(k), float(s)] for a, b, L, k, s in zip(

@@ -98,7 +98,51 @@ class VoxelRobot3D:
               [(0, 1, 0), (2, 1, 0), (1, 1, 2), (0, 1, 1), (1, 1, 1), (2, 1, 1)]),
     }
 
-    def build_reference(self, robot_id):
+    # Robot 103 (robot_design.py:458-505): a skeleton graph read from a JSON
+    # file ({"nodes": {id: [x, y, z]}, "links": [[a, b], ...]}, the
+    # reference's cfg3d/skeleton.json).  Motor springs are added after the
+    # links, in the order the reference's motor set iterates; then the extra
+    # passive braces.  Faces are the link graph's triangles a < b < c.
+    SKELETON_MOTORS = ((36, 24), (10, 11), (29, 26), (14, 26), (9, 21), (25, 24), (36, 9),
+                       (21, 24), (10, 9), (29, 11), (25, 26), (14, 11))
+    SKELETON_BRACES = ((14, 29), (29, 36), (36, 21), (21, 14), (11, 5), (26, 5), (24, 35),
+                       (9, 20), (26, 35), (11, 20), (9, 8), (24, 8), (11, 31), (26, 16),
+                       (24, 23), (9, 38))
+
+    def skeleton(self, data):
+        for pos in data["nodes"].values():
+            self.objects.append((pos[0] * 0.05, pos[1] * 0.05 + 0.212, pos[2] * 0.05 * 0.75))
+        motors = set(self.SKELETON_MOTORS)
+        motor = lambda a, b: (a, b) in motors or (b, a) in motors
+        linked = set()
+        for a, b in data["links"]:
+            linked.add((a, b))
+            linked.add((b, a))
+            self.spring(a, b, self.spring_stiffness, self.spring_actuation if motor(a, b) else 0.0)
+        for a, b in self.SKELETON_MOTORS + self.SKELETON_BRACES:
+            self.spring(a, b, self.spring_stiffness, self.spring_actuation if motor(a, b) else 0.0)
+        n = len(self.objects)
+        adj = [sorted(b for b in range(n) if (a, b) in linked) for a in range(n)]
+        for a in range(n):
+            for b in adj[a]:
+                if b <= a:
+                    continue
+                for c in adj[b]:
+                    if c > b and (a, c) in linked:
+                        self.faces.append((a, b, c))
+        return self.objects, self.springs, self.faces
+
+    def build_reference(self, robot_id, skeleton=None):
+        """Reference robots 100-104.  Robot 103 needs ``skeleton``: the
+        skeleton graph as a dict or the path of its JSON file."""
+        if robot_id == 103:
+            if skeleton is None:
+                raise ValueError("robot 103 is built from a skeleton file (cfg3d/skeleton.json)")
+            if not isinstance(skeleton, dict):
+                import json
+                with open(skeleton) as fh:
+                    skeleton = json.load(fh)
+            return self.skeleton(skeleton)
         if robot_id == 104:
             for kk in range(6):
                 self.cube(0, 0, kk, actuation=self.spring_actuation, template=LYING)
@@ -197,10 +241,11 @@ def build_quadruped(stiffness=3e4, actuation=0.1):
     return build_voxels(quadruped_cubes(), stiffness, actuation, n_act=4)
 
 
-def build_reference(robot_id, stiffness=1e4, actuation=0.25):
-    """Reference robot with one MLP output per spring (diffphy_trainer.py:178-185)."""
+def build_reference(robot_id, stiffness=1e4, actuation=0.25, skeleton=None):
+    """Reference robot with one MLP output per spring (diffphy_trainer.py:178-185);
+    robot 103 takes its skeleton graph (dict or JSON path)."""
     b = VoxelRobot3D(spring_stiffness=stiffness, spring_actuation=actuation)
-    objects, springs, faces = b.build_reference(robot_id)
+    objects, springs, faces = b.build_reference(robot_id, skeleton=skeleton)
     return make_design(objects, springs, faces=faces)
 
 

@@ -5,8 +5,10 @@ dedupe (first writer wins) and its FP64 ``**0.5`` rest lengths, so that vertex
 numbering, spring order and rest lengths are bit-exact with the reference.
 
 Pinned by: ``tests/golden/robot_design_data_{2,3}.json`` (the reference's own
-dumped builder outputs for the 2D builder, whose insertion/dedupe rules are the
-same as the 3D builder's), spring/vertex counts of robot 100
+dumped builder outputs for the 2D builder), ``tests/golden/builder3d_fixtures.json``
+(outputs of the reference's own 3D builder for robots 100-104, the quadruped
+cube list and three random voxel robots, written by
+``tests/golden/make_builder_fixtures.py``), spring/vertex counts of robot 100
 (``robot_design.py:429-443``), and lattice invariants (rest lengths in
 {0.05, 0.05*sqrt2, 0.05*sqrt3}, no duplicate pairs, contiguous ids).
 """
@@ -160,7 +162,7 @@ class MassSpring3D:
             self.add_mesh_spring(p, q, s, act, group)
         self._faces(a, b, c, d, e, f, g, h)
 
-    # robot 100..104 (:429-570, robotD needs cfg3d/skeleton.json and is not restated)
+    # robots 100..104 (:429-570); robotD reads the skeleton graph (cfg3d/skeleton.json)
     def robotA(self):
         for ijk in [(0, 0, 0), (2, 0, 2), (0, 0, 2), (2, 0, 0)]:
             self.add_mesh_square(*ijk, actuation=self.spring_actuation)
@@ -184,8 +186,39 @@ class MassSpring3D:
         for kk in range(6):
             self.add_mesh_lying(0, 0, kk, actuation=self.spring_actuation)
 
-    def build(self, robot_id):                     # :572-587
-        {100: self.robotA, 101: self.robotB, 102: self.robotC, 104: self.robotE}[robot_id]()
+    def robotD(self, skeleton):                    # :458-505, skeleton = parsed JSON
+        for v in skeleton["nodes"]:
+            pos = list(skeleton["nodes"][v])
+            pos[0] = pos[0] * 0.05
+            pos[1] = pos[1] * 0.05 + 0.212
+            pos[2] = pos[2] * 0.05 * 0.75
+            self.objects.append(pos)
+        motors = {(9, 21), (36, 24), (29, 26), (14, 11), (14, 26), (10, 9), (29, 11), (25, 24),
+                  (25, 26), (36, 9), (21, 24), (10, 11)}
+        s = set()
+        for e in skeleton["links"]:
+            a, b = e
+            s.add((a, b))
+            s.add((b, a))
+            act = self.spring_actuation if (a, b) in motors or (b, a) in motors else 0.
+            self.add_mesh_spring(a, b, self.spring_stiffness, act)
+        for a, b in list(motors) + [(14, 29), (29, 36), (36, 21), (21, 14), (11, 5), (26, 5),
+                                    (24, 35), (9, 20), (26, 35), (11, 20), (9, 8), (24, 8),
+                                    (11, 31), (26, 16), (24, 23), (9, 38)]:
+            act = self.spring_actuation if (a, b) in motors or (b, a) in motors else 0.
+            self.add_mesh_spring(a, b, self.spring_stiffness, act)
+        n = len(self.objects)
+        for a in range(n):
+            for b in range(n):
+                for c in range(n):
+                    if a < b < c and (a, b) in s and (b, c) in s and (a, c) in s:
+                        self.faces.append((a, b, c))
+
+    def build(self, robot_id, skeleton=None):     # :572-587
+        if robot_id == 103:
+            self.robotD(skeleton)
+        else:
+            {100: self.robotA, 101: self.robotB, 102: self.robotC, 104: self.robotE}[robot_id]()
         return self.objects, self.springs, self.faces
 
 

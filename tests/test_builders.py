@@ -109,3 +109,90 @@ def test_config_derivation_matches_reference_formulae():
     assert abs(cfg["process"]["spring_omega"] - 2 * math.pi / 0.004 / 100) < 1e-12
     q = dl.quadruped_config(batch_size=8).get_config()
     assert q["nn"]["n_input_states"] == 179 and q["robot"]["n_actuators"] == 4
+
+
+def _fixtures():
+    return ob.load_json(GOLD + "builder3d_fixtures.json")
+
+
+def _assert_same(case, objects, springs, faces):
+    assert [list(p) for p in objects] == case["objects"]                  # bit for bit
+    assert [[int(s[0]), int(s[1]), float(s[2]), float(s[3]), float(s[4])] for s in springs] \
+        == case["springs"]
+    assert [list(f) for f in faces] == case["faces"]
+
+
+FIXTURE_CASES = ["robot_100", "robot_101", "robot_102", "robot_103", "robot_104", "quadruped",
+                 "random_12", "random_60", "random_230"]
+
+
+@pytest.mark.parametrize("name", FIXTURE_CASES)
+def test_3d_builders_match_reference_builder_output(name):
+    """Oracle and product 3D builders against the reference builder's own
+    output (vertex numbering, 28-spring template order, dedupe, FP64 rest
+    lengths, faces), bit for bit."""
+    case = _fixtures()[name]
+    skel = ob.load_json(GOLD + "skeleton.json")
+    if case["cubes"] is None:
+        rid = int(name.split("_")[1])
+        o = ob.MassSpring3D(case["stiffness"], case["actuation"]).build(rid, skeleton=skel)
+        _assert_same(case, *o)
+        d = dl.build_reference(rid, case["stiffness"], case["actuation"],
+                               skeleton=GOLD + "skeleton.json" if rid == 103 else None)
+        springs = [[a, b, L, k, c] for a, b, L, k, c in zip(d.spring_a, d.spring_b, d.L0,
+                                                           d.stiffness, d.coeff)]
+        _assert_same(case, d.objects.tolist(), springs, d.faces)
+        assert d.act_index.tolist() == list(range(d.n_spring))
+    else:
+        cubes = [(tuple(c), act, g) for c, act, g in case["cubes"]]
+        b = ob.MassSpring3D(case["stiffness"], case["actuation"])
+        for ijk, act, g in cubes:
+            b.add_mesh_square(*ijk, actuation=case["actuation"] if act else 0.0, group=g)
+        _assert_same(case, b.objects, b.springs, b.faces)
+        d = dl.build_voxels(cubes, case["stiffness"], case["actuation"])
+        springs = [[a, b_, L, k, c] for a, b_, L, k, c in zip(d.spring_a, d.spring_b, d.L0,
+                                                            d.stiffness, d.coeff)]
+        _assert_same(case, d.objects.tolist(), springs, d.faces)
+        # the group of each spring is its first writer's cube group
+        assert d.act_index.tolist() == b.groups
+
+
+def test_fixture_template_swap_is_detected():
+    """A plausible builder mistake -- two entries of the 28-spring template
+    swapped -- changes the spring order the fixture pins."""
+    case = _fixtures()["quadruped"]
+    b = ob.MassSpring3D(case["stiffness"], case["actuation"])
+    orig = ob.MassSpring3D.add_mesh_square
+
+    def swapped(self, i, j, k, actuation=0.0, group=0):
+        a, bb, c, d, e, f, g, h = self._cube_points(i, j, k)
+        s, A = self.spring_stiffness, actuation
+        order = [(a, bb, A), (c, d, A), (e, f, A), (g, h, A),
+                 (a, c, 0), (bb, d, 0), (f, h, 0), (e, g, 0),      # first two x-edges swapped
+                 (bb, f, 0), (d, h, 0), (a, e, 0), (c, g, 0),
+                 (bb, g, 0), (d, e, 0), (f, c, 0), (h, a, 0),
+                 (e, c, 0), (a, g, 0), (h, bb, 0), (d, f, 0),
+                 (e, bb, A), (a, f, A), (h, c, A), (d, g, A),
+                 (f, g, A), (e, h, A), (a, d, A), (c, bb, A)]
+        for p, q, act in order:
+            self.add_mesh_spring(p, q, s, act, group)
+        self._faces(a, bb, c, d, e, f, g, h)
+
+    for (ijk, act, g) in [(tuple(c), a, g) for c, a, g in case["cubes"]]:
+        swapped(b, *ijk, actuation=case["actuation"] if act else 0.0, group=g)
+    got = [[int(s[0]), int(s[1]), float(s[2]), float(s[3]), float(s[4])] for s in b.springs]
+    assert got != case["springs"]
+    assert orig is ob.MassSpring3D.add_mesh_square
+
+
+def test_robot_103_from_config_file(tmp_path):
+    """sim_spider.json selects robot 103; the loader reads skeleton.json next
+    to the experiment file (the reference reads cfg3d/skeleton.json)."""
+    import shutil
+    for f in ("sim_spider.json", "skeleton.json"):
+        shutil.copy(GOLD + f, tmp_path / f)
+    cfg = dl.ConfigSim.from_file(str(tmp_path / "sim_spider.json")).get_config()
+    case = _fixtures()["robot_103"]
+    assert cfg["robot"]["n_objects"] == len(case["objects"]) == 39
+    assert cfg["robot"]["n_springs"] == len(case["springs"]) == 175
+    assert [list(p) for p in cfg["robot"]["objects"]] == case["objects"]
