@@ -91,6 +91,13 @@ class Robot:
     def n_spring(self):
         return self.a.shape[0]
 
+    def to(self, dtype):
+        """The same robot with real-valued fields in ``dtype`` (an FP32 run
+        of the restatement measures what FP32 arithmetic alone changes)."""
+        return Robot(objects=self.objects.to(dtype), a=self.a, b=self.b, L0=self.L0.to(dtype),
+                     k=self.k.to(dtype), act_coeff=self.act_coeff.to(dtype),
+                     act_index=self.act_index)
+
 
 def make_robot(objects, springs, act_index=None):
     S = len(springs)
@@ -192,14 +199,14 @@ def nn_input(t, x_t, v_t, center, tv, th, tc, layout, offset=0):
     parts = []
     if layout.n_sin > 0:
         omega = 2 * math.pi / layout.run_period        # spring_omega * dt (config_sim.py:114)
-        j = torch.arange(layout.n_sin, dtype=F64)
+        j = torch.arange(layout.n_sin, dtype=x_t.dtype)
         s = torch.sin(omega * (t + offset) + 2 * math.pi / layout.n_sin * j)
         parts.append(s.expand(B, -1))
     if layout.vcom:
         parts.append((v_t / N).sum(1))
     off = (x_t - center[:, None, :]) / layout.offset_scale
     if layout.vert_pad:
-        off = torch.cat([off, torch.zeros(B, N, layout.vert_pad, dtype=F64)], -1)
+        off = torch.cat([off, torch.zeros(B, N, layout.vert_pad, dtype=x_t.dtype)], -1)
     parts.append(off.reshape(B, -1))
     if layout.dup_v:
         comp = tv[:, list(layout.tv_comps)] / layout.max_speed
@@ -407,8 +414,8 @@ def rollout_task(robot, cfg, weights, tv, th, x0, v0, record=False):
     W1, b1, W2, b2 = weights
     B, N, _ = x0.shape
     x, v = x0, v0
-    zero_c = torch.zeros(B, dtype=F64)
-    total = torch.zeros(B, dtype=F64)
+    zero_c = torch.zeros(B, dtype=x0.dtype)
+    total = torch.zeros(B, dtype=x0.dtype)
     xs, vs = [x0], [v0]
     for t in range(T + 1):
         c = (x / N).sum(1)

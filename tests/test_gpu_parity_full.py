@@ -1,0 +1,380 @@
+"""Full-horizon parity of what the benchmark times, and of the size classes
+and per-step kernels the shorter tests leave out.
+
+* the 1024-task quadruped workload's kernels (wave-per-robot, FP32) at its
+  own horizon -- T = 1000 advances, loss over the last 500 states,
+  checkpoints every 50 steps (variant 3) and stored rows (variant 4) -- on 8
+  tasks against the FP64 oracle: loss within 1e-4 and every weight gradient
+  within 1e-3 (relative), positions along the whole trajectory within 1e-5
+  of the state scale;
+* the adjoint at T = 1000 against FP64 central differences on 16 weights;
+* the plain per-step quadruped loop (``DiffPhyTrainer.simulate_task``, the
+  benchmark's one-robot-per-launch form) at T = 300;
+* the uneven batch's large size classes (129-400 and > 400 vertices, up to
+  400 cubes) with the block kernel at 512 threads, LDS policies 0 and 4;
+* ``dl_ms_height`` / ``dl_ms_rotation`` and their adjoints against the
+  oracle; the batched task loss against the free-fall closed form.
+"""
+
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import diffloco as dl
+from diffloco import _native as nat
+from diffloco.rollout import MassSpringBatch, rollout_loss
+from diffloco.solver_mass_spring import _physics
+from diffloco.taichi_env import sensor_layout
+from oracle import builders as ob
+from oracle import mass_spring as om
+
+from helpers import (from_soa, oracle_quadruped, oracle_reference_robot, perturbed_state,
+                     rel_err, task_targets, to_soa)
+from test_oracle_pins import task_loss_free_fall
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+# ------------------------------------------------------------------ c1 at T = 1000
+C1_B, C1_T, C1_W, C1_K = 8, 1000, 500, 50
+
+
+def _oracle_per_robot(dtype, tv, th, w64):
+    """Oracle rollout of the 8 tasks in ``dtype``: per-task losses, per-task
+    weight gradients (one reverse pass per task over the batched graph) and
+    the trajectory."""
+    rob = oracle_quadruped().to(dtype)
+    cfg = om.TaskConfig(steps=C1_T, window=C1_W)
+    Wr = [w.to(dtype).clone().requires_grad_() for w in w64]
+    x0, v0 = om.initial_state(rob, C1_B)
+    _, lb, traj = om.rollout_task(rob, cfg, Wr, torch.tensor(tv, dtype=dtype),
+                                  torch.tensor(th, dtype=dtype), x0.to(dtype), v0.to(dtype),
+                                  record=True)
+    grads = []
+    for r in range(C1_B):
+        g = torch.autograd.grad(lb[r], Wr, retain_graph=r < C1_B - 1)
+        grads.append([x.double() for x in g])
+    return lb.detach().double(), grads, traj["x"].detach().double()
+
+
+@pytest.fixture(scope="module")
+def c1_oracle():
+    """Oracle rollouts of 8 quadruped tasks, T = 1000: in FP64 (the reference
+    arithmetic) and in FP32 (what FP32 arithmetic alone does to each task's
+    gradient: the conditioning of that task's trajectory)."""
+    torch.set_num_threads(max(1, min(16, torch.get_num_threads())))
+    tv, th = task_targets(C1_B, 101)
+    rob = oracle_quadruped()
+    w64 = om.init_weights(om.TaskConfig().layout.n_input(rob.n_vert), 64, 4, seed=102)
+    lb, grads, x = _oracle_per_robot(torch.float64, tv, th, w64)
+    lb32, grads32, x32 = _oracle_per_robot(torch.float32, tv, th, w64)
+    return dict(tv=tv, th=th, w64=w64, lb=lb, grads=grads, x=x, grads32=grads32, x32=x32)
+
+
+def _c1_batch(variant, tv, th, store_traj=False, dtype=torch.float32):
+    design = dl.build_quadruped()
+    qc = dl.quadruped_config(batch_size=C1_B, steps=C1_T, window=C1_W).get_config()
+    x0 = design.objects + np.array([0.4, 0.0, 0.0])
+    tg = np.concatenate([tv, th[:, None], np.zeros((C1_B, 1))], 1)
+    return MassSpringBatch([design], [0] * C1_B, C1_T, _physics(qc), sensor_layout(qc), tg,
+                           [x0] * C1_B, act="tanh", ckpt_every=C1_K, dtype=dtype,
+                           loss_window=C1_W, variant=variant, store_traj=store_traj)
+
+
+def _per_robot_grads(batch, r):
+    """Robot r's weight-gradient partial (before the per-design reduction)."""
+    H, I, O = batch.H, batch.n_in[0], batch.designs[0].n_act
+    o = int(batch.table[r]["wgrad"])
+    g = batch.wgrad[o:o + batch.g_size[r]].double().cpu()
+    return [g[:H * I].view(H, I), g[H * I:H * I + H], g[H * I + H:H * I + H + O * H].view(O, H),
+            g[H * I + H + O * H:H * I + H + O * H + O]]
+
+
+@pytest.mark.parametrize("variant,prec", [(3, "f32"), (4, "f32"), (1, "f64")])
+def test_c1_full_horizon_matches_oracle(c1_oracle, variant, prec):
+    """The benchmark's kernels (k_wave_fwd + k_wave_bwd_pipe, FP32) at the
+    workload's horizon: T = 1000, window 500, K = 50 (variant 3) or every
+    step's rows kept (variant 4); and the FP64 block kernel (variant 1).
+
+    FP64: every task's loss within 1e-10 and gradient within 1e-8 of the
+    oracle.  FP32: every task's loss within 1e-4; each task's gradient within
+    1e-3 of the FP64 oracle, or, for a task whose trajectory makes FP32
+    arithmetic itself miss 1e-3 (the FP32 run of the oracle shows it: a
+    state sitting near a contact or friction-clamp kink, where the derivative
+    jumps), within 3x that FP32 oracle error; the batch-mean gradient (what
+    the benchmark trains on) is reported beside it.  Free-running positions
+    along the whole FP32 trajectory stay within 1e-5 of the state scale, or
+    within 3x the FP32 oracle's own drift where that is larger (the
+    per-step bound, 1e-5 from the same state, is test_advance_matches_oracle's)."""
+    _say(f"c1 full horizon variant={variant} {prec}")
+    o = c1_oracle
+    dt = torch.float32 if prec == "f32" else torch.float64
+    batch = _c1_batch(variant, o["tv"], o["th"], store_traj=(variant == 3), dtype=dt)
+    if prec == "f32":
+        assert batch.wave_eligible() and batch.packed4()
+    batch.set_weights(0, *[w.to("cuda", dt) for w in o["w64"]])
+    batch.reset()
+    batch.forward()
+    batch.backward(loss_grad=np.ones(C1_B))
+    torch.cuda.synchronize()
+    lb = batch.loss.double().cpu()
+    per = ((lb - o["lb"]).abs() / o["lb"].abs()).max().item()
+    ltol, gtol = (1e-4, 1e-3) if prec == "f32" else (1e-10, 1e-8)
+    assert per < ltol, per
+    mean_gpu = [torch.zeros_like(g) for g in o["grads"][0]]
+    mean_ref = [torch.zeros_like(g) for g in o["grads"][0]]
+    for r in range(C1_B):
+        gg = _per_robot_grads(batch, r)
+        e = [rel_err(a, b) for a, b in zip(gg, o["grads"][r])]
+        e32 = [rel_err(a, b) for a, b in zip(o["grads32"][r], o["grads"][r])]
+        _say(f"  task {r}: loss rel {abs(lb[r] - o['lb'][r]).item() / o['lb'][r].item():.1e} "
+             f"grad rel " + " ".join(f"{x:.1e}" for x in e) + "  (FP32 oracle "
+             + " ".join(f"{x:.1e}" for x in e32) + ")")
+        for a, b in zip(e, e32):
+            assert a < (max(gtol, 3 * b) if prec == "f32" else gtol), (r, e, e32)
+        for m, a in zip(mean_gpu, gg):
+            m += a / C1_B
+        for m, a in zip(mean_ref, o["grads"][r]):
+            m += a / C1_B
+    _say("  batch-mean gradient rel " + " ".join(f"{rel_err(a, b):.1e}"
+                                                  for a, b in zip(mean_gpu, mean_ref)))
+    assert all(b.norm() > 0 for b in mean_ref)
+    if variant == 3:
+        scale = o["x"].abs().max().item()
+        worst = 0.0
+        for r in range(C1_B):
+            xr, _ = batch.trajectory(r)
+            worst = max(worst, (xr.double().cpu() - o["x"][:, r]).abs().max().item() / scale)
+        drift32 = (o["x32"] - o["x"]).abs().max().item() / scale
+        _say(f"  trajectory max |dx| / scale over 1000 steps: {worst:.2e} "
+             f"(FP32 oracle {drift32:.2e})")
+        assert worst < max(1e-5, 3 * drift32)
+
+
+def test_c1_gradient_central_differences_T1000():
+    """The persistent adjoint (FP64 block kernel, K = 50) at T = 1000 against
+    FP64 central differences on 16 random weights (seed 2).  eps = 1e-6: at
+    this horizon the ground-contact branches (time of impact, friction clamp)
+    make the loss piecewise smooth, and probes of +-1e-4 cross a branch
+    switch for some weights (4 of these 16 in the oracle), which 1e-6 does
+    not; truncation and rounding errors stay below 1e-8 relative."""
+    _say("c1 FD T=1000")
+    from diffloco.rollout import init_weights
+    B = 2
+    design = dl.build_quadruped()
+    qc = dl.quadruped_config(batch_size=B, steps=C1_T, window=C1_W, precision="f64").get_config()
+    tv, th = task_targets(B, 3)
+    tg = np.concatenate([tv, th[:, None], np.zeros((B, 1))], 1)
+    x0 = design.objects + np.array([0.4, 0.0, 0.0])
+    batch = MassSpringBatch([design], [0] * B, C1_T, _physics(qc), sensor_layout(qc), tg,
+                            [x0] * B, act="tanh", ckpt_every=C1_K, dtype=torch.float64,
+                            loss_window=C1_W)
+    w = init_weights(batch.n_in[0], 64, 4, seed=0, dtype=torch.float64)
+
+    def loss_of(ws):
+        batch.set_weights(0, *ws)
+        batch.reset()
+        batch.forward()
+        return batch.loss.double().mean().item()
+
+    loss_of(w)
+    batch.backward(loss_grad=np.full(B, 1.0 / B))
+    grads = [g.clone().double().cpu() for g in batch.weight_grads(0)]
+    gmax = max(g.abs().max().item() for g in grads)
+    assert gmax > 0
+    rng = np.random.default_rng(2)
+    eps = 1e-6
+    for _ in range(16):
+        pi = int(rng.integers(4))
+        j = int(rng.integers(w[pi].numel()))
+        wp = [x.clone() for x in w]
+        wm = [x.clone() for x in w]
+        wp[pi].view(-1)[j] += eps
+        wm[pi].view(-1)[j] -= eps
+        fd = (loss_of(wp) - loss_of(wm)) / (2 * eps)
+        ad = grads[pi].view(-1)[j].item()
+        assert abs(fd - ad) <= 1e-4 * abs(ad) + 1e-6 * gmax, (pi, j, fd, ad)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_simulate_task_plain_loop_matches_oracle(prec):
+    """DiffPhyTrainer.simulate_task: per-step kernels (center, sensors, MLP,
+    advance) under the tape, the benchmark's plain form, T = 300."""
+    _say(f"simulate_task {prec}")
+    B, T, W = 2, 300, 150
+    tv, th = task_targets(B, 61)
+    config = dl.quadruped_config(batch_size=B, steps=T, window=W, precision=prec)
+    tr = dl.DiffPhyTrainer(config)
+    tr.taichi_env.setup_robot()
+    cfg = om.TaskConfig(steps=T, window=W)
+    rob = oracle_quadruped()
+    w64 = om.init_weights(cfg.layout.n_input(rob.n_vert), 64, 4, seed=62)
+    tr.nn.set_weights(*w64)
+    tr.simulate_task(T, tv, th)
+    torch.cuda.synchronize()
+    loss = tr.taichi_env.loss[None]
+    Wr = [w.clone().requires_grad_() for w in w64]
+    x0, v0 = om.initial_state(rob, B)
+    lo, _, _ = om.rollout_task(rob, cfg, Wr, torch.tensor(tv), torch.tensor(th), x0, v0)
+    lo.backward()
+    ltol, gtol = (1e-10, 1e-8) if prec == "f64" else (1e-4, 1e-3)
+    assert abs(loss - lo.item()) / abs(lo.item()) < ltol
+    G = [tr.nn.weights1.grad, tr.nn.bias1.grad, tr.nn.weights2.grad, tr.nn.bias2.grad]
+    for a, b in zip(G, Wr):
+        assert b.grad.norm() > 0
+        assert rel_err(a.data.double().cpu(), b.grad) < gtol
+
+
+# ------------------------------------------------------------ c2's large classes
+def _box(nx, ny, nz, groups_x=2, groups_z=2):
+    return [((i, j, k), True, (i * groups_x // nx) * groups_z + (k * groups_z // nz))
+            for i in range(nx) for j in range(ny) for k in range(nz)]
+
+
+@pytest.fixture(scope="module")
+def large_robots():
+    """Two compact 200-cube bodies (324 and 330 vertices: class 129-400) and
+    two random 250- and 400-cube robots (> 400 vertices), dashpot impulse
+    0.03 (the uneven workload's), T = 200-300, oracle FP64 autograd each."""
+    from diffloco import robot_design as rd
+    rng = np.random.default_rng(77)
+    sets = [_box(5, 8, 5), _box(5, 10, 4)]
+    for nc, ng in ((250, 24), (400, 48)):
+        cubes, _ = rd.random_voxel_cubes(rng, nc, ng)
+        sets.append(cubes)
+    horizons = [300, 240, 200, 260]
+    W = 100
+    tv, th = task_targets(len(sets), 78)
+    lay = om.TaskConfig().layout
+    out = []
+    for r, (cubes, T) in enumerate(zip(sets, horizons)):
+        objects, springs, groups = ob.build_voxels(cubes)
+        rob = om.make_robot(objects, springs, groups)
+        cfg = om.TaskConfig(steps=T, window=W)
+        cfg.physics.dashpot = 0.03
+        n_act = max(groups) + 1
+        w64 = om.init_weights(lay.n_input(rob.n_vert), 64, n_act, seed=80 + r)
+        Wr = [w.clone().requires_grad_() for w in w64]
+        x0, v0 = om.initial_state(rob, 1)
+        lo, _, _ = om.rollout_task(rob, cfg, Wr, torch.tensor(tv[r:r + 1]),
+                                   torch.tensor(th[r:r + 1]), x0, v0)
+        lo.backward()
+        out.append(dict(cubes=cubes, T=T, w64=w64, loss=lo.item(), grads=[w.grad for w in Wr],
+                        n_vert=rob.n_vert, n_act=n_act))
+    return dict(robots=out, W=W, tv=tv, th=th)
+
+
+@pytest.mark.parametrize("pol", [0, 4])
+def test_uneven_large_classes_match_oracle(large_robots, pol):
+    """The block kernel (one workgroup per robot, 512 threads) on robots of
+    the 129-400 and > 400 vertex classes: LDS policy 0 (springs spilled to
+    the scratch when they do not fit, W1 read from global memory) and 4
+    (everything that fits in the whole LDS)."""
+    _say(f"uneven large classes policy={pol}")
+    L = large_robots
+    designs = [dl.build_voxels(r["cubes"], n_act=r["n_act"]) for r in L["robots"]]
+    nv = [d.n_vert for d in designs]
+    assert sum(128 < n <= 400 for n in nv) >= 2 and sum(n > 400 for n in nv) >= 2, nv
+    qc = dl.quadruped_config(batch_size=len(designs), steps=300, window=L["W"]).get_config()
+    phys = _physics(qc)
+    phys.dashpot = 0.03
+    tg = np.concatenate([L["tv"], L["th"][:, None], np.zeros((len(designs), 1))], 1)
+    x0 = [d.objects + np.array([0.4, 0.0, 0.0]) for d in designs]
+    batch = MassSpringBatch(designs, list(range(len(designs))), [r["T"] for r in L["robots"]],
+                            phys, sensor_layout(qc), tg, x0, act="tanh", ckpt_every=50,
+                            dtype=torch.float32, loss_window=L["W"], variant=1, block=512,
+                            lds_policy=pol)
+    wd = [[w.to("cuda", torch.float32).requires_grad_() for w in r["w64"]] for r in L["robots"]]
+    lossv = rollout_loss(batch, wd)
+    lossv.sum().backward()
+    torch.cuda.synchronize()
+    for r, o in enumerate(L["robots"]):
+        lrel = abs(lossv[r].item() - o["loss"]) / abs(o["loss"])
+        grel = [rel_err(a.grad.cpu(), b) for a, b in zip(wd[r], o["grads"])]
+        _say(f"  robot {r} N={nv[r]} loss rel {lrel:.2e} grad rel "
+             + " ".join(f"{g:.2e}" for g in grel))
+        assert lrel < 1e-4
+        assert all(g < 1e-3 for g in grel)
+
+
+# ------------------------------------------------------- height / rotation kernels
+@pytest.mark.parametrize("robot,t", [("ref100", 500), ("ref100", 503), ("quad", 0), ("quad", 7)])
+def test_height_rotation_kernels_match_oracle(robot, t):
+    """dl_ms_height / dl_ms_rotation and their adjoints (FP64) against the
+    oracle's pre_advance and its autograd (compute_height / compute_rotation,
+    solver_mass_spring.py:158-192): window reset and running max, robot 100's
+    vertex ids and the head/tail split."""
+    _say(f"height/rotation {robot} t={t}")
+    rob = oracle_reference_robot(100) if robot == "ref100" else oracle_quadruped()
+    rid = 100 if robot == "ref100" else 0
+    B, N, J = 5, rob.n_vert, 500
+    x, _ = perturbed_state(rob, B, seed=90 + t, xs=0.01)
+    x0, _ = om.initial_state(rob, B)
+    c0 = x0.mean(1)
+    g = torch.Generator().manual_seed(91)
+    # previous window height above the current minimum for robots 0, 2, 4, below for 1, 3
+    hprev = x[..., 1].min(1).values + torch.tensor([0.01, -0.01, 0.02, -0.02, 0.005],
+                                                   dtype=torch.float64)
+    dt = torch.float64
+    xd, x0d, c0d, hpd = to_soa(x), to_soa(x0), c0.cuda(), hprev.cuda()
+    h = torch.zeros(B, dtype=dt, device="cuda")
+    up, rot = torch.zeros_like(h), torch.zeros_like(h)
+    nat.call("dl_ms_height", nat.F64, B, N, t, J, nat.ptr(xd), nat.ptr(hpd), nat.ptr(h),
+             nat.ptr(up), nat.stream())
+    nat.call("dl_ms_rotation", nat.F64, B, N, rid, nat.ptr(xd), nat.ptr(x0d), nat.ptr(c0d),
+             nat.ptr(rot), nat.stream())
+    hg = torch.randn(B, dtype=dt, generator=g)
+    ug = torch.randn(B, dtype=dt, generator=g)
+    rg = torch.randn(B, dtype=dt, generator=g)
+    xg = torch.zeros(B, 3, N, dtype=dt, device="cuda")
+    hpg = torch.zeros(B, dtype=dt, device="cuda")
+    keep = [hg.cuda(), ug.cuda(), rg.cuda()]
+    nat.call("dl_ms_height_grad", nat.F64, B, N, t, J, nat.ptr(xd), nat.ptr(hpd),
+             nat.ptr(keep[0]), nat.ptr(keep[1]), nat.ptr(xg), nat.ptr(hpg), nat.stream())
+    nat.call("dl_ms_rotation_grad", nat.F64, B, N, rid, nat.ptr(xd), nat.ptr(x0d), nat.ptr(c0d),
+             nat.ptr(keep[2]), nat.ptr(xg), nat.stream())
+    torch.cuda.synchronize()
+    xr, hr = x.clone().requires_grad_(), hprev.clone().requires_grad_()
+    _, eh, eu, erot = om.pre_advance(t, xr, x0, c0, hr, J, robot_id=rid)
+    assert torch.equal(h.cpu(), eh.detach()) and torch.equal(up.cpu(), eu.detach())
+    assert rel_err(rot.cpu(), erot.detach()) < 1e-11     # head/tail sums in another order
+    ((eh * hg).sum() + (eu * ug).sum() + (erot * rg).sum()).backward()
+    assert rel_err(from_soa(xg), xr.grad) < 1e-12
+    assert rel_err(hpg.cpu(), hr.grad if hr.grad is not None else torch.zeros(B, dtype=dt)) < 1e-14
+    if t % J:
+        # some robots keep the previous window height, some take a new minimum
+        assert 0 < int((eh.detach() == hprev).sum()) < B
+
+
+@pytest.mark.parametrize("prec,variant", [("f64", 1), ("f32", 3)])
+def test_batched_task_loss_free_fall_closed_form(prec, variant):
+    """The in-kernel windowed task loss of bodies released 1 m up (no
+    contact within T): per-robot losses equal the free-fall closed form."""
+    _say(f"free fall task loss {prec} variant={variant}")
+    B, T, W = 4, 90, 37
+    design = dl.build_quadruped()
+    qc = dl.quadruped_config(batch_size=B, steps=T, window=W).get_config()
+    tv, th = task_targets(B, 95)
+    tg = np.concatenate([tv, th[:, None], np.zeros((B, 1))], 1)
+    x0 = design.objects + np.array([0.4, 1.0, 0.0])
+    dt = torch.float64 if prec == "f64" else torch.float32
+    batch = MassSpringBatch([design], [0] * B, T, _physics(qc), sensor_layout(qc), tg, [x0] * B,
+                            act="tanh", ckpt_every=16, dtype=dt, loss_window=W, variant=variant)
+    lay = om.TaskConfig().layout
+    w = [x.to("cuda", dt) for x in om.init_weights(lay.n_input(design.n_vert), 64, 4, seed=96)]
+    batch.set_weights(0, *w)
+    batch.reset()
+    batch.forward()
+    torch.cuda.synchronize()
+    want = task_loss_free_fall(T, W, 0.004, -1.8, x0[:, 1].mean(), tv, th)
+    got = batch.loss.double().cpu().numpy()
+    tol = 1e-11 if prec == "f64" else 2e-5
+    assert np.allclose(got, want, rtol=tol, atol=0), (got, want)
