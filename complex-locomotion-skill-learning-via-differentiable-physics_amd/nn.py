@@ -9,10 +9,25 @@ with bias correction by ``iter + 1``; the reference's betas are 0.9/0.9).
 import math
 import pickle
 
+import numpy as np
 import torch
 
 from . import _native as nat
 from .fields import Field, record
+
+
+class _ArrayUnpickler(pickle.Unpickler):
+    """Unpickler for lists of numpy arrays: only the globals numpy's array
+    pickles use (numpy 1.x and 2.x module paths) resolve."""
+
+    _ALLOWED = {("numpy.core.multiarray", "_reconstruct"), ("numpy._core.multiarray", "_reconstruct"),
+                ("numpy.core.multiarray", "scalar"), ("numpy._core.multiarray", "scalar"),
+                ("numpy", "ndarray"), ("numpy", "dtype")}
+
+    def find_class(self, module, name):
+        if (module, name) in self._ALLOWED:
+            return super().find_class(module, name)
+        raise pickle.UnpicklingError(f"weights file refers to {module}.{name}: refused")
 
 
 class Model:
@@ -103,11 +118,18 @@ class Model:
             pickle.dump([w.to_numpy() for w in self.weights], fh)
 
     def load_weights(self, name="save.pkl"):
-        """Load a weights file written by ``dump_weights`` (nn.py:158-161).
-        Only load files this program wrote: pickle executes code."""
+        """Load a ``[W1, W2, b1, b2]`` weights pickle (nn.py:158-161), written
+        by ``dump_weights`` or by the reference.  The file is read with a
+        restricted unpickler that resolves only numpy's array-reconstruction
+        globals, so a file naming any other callable is refused instead of run."""
         with open(name, "rb") as fh:
-            vals = pickle.load(fh)
+            vals = _ArrayUnpickler(fh).load()
+        if not isinstance(vals, (list, tuple)) or len(vals) != len(self.weights):
+            raise ValueError(f"{name}: expected a list of {len(self.weights)} arrays")
         for w, val in zip(self.weights, vals):
+            if not isinstance(val, np.ndarray) or tuple(val.shape) != tuple(w.view.shape):
+                raise ValueError(f"{name}: array of shape {getattr(val, 'shape', None)} "
+                                 f"for a field of shape {tuple(w.view.shape)}")
             w.from_numpy(val)
 
     def gradient_update(self, iter=0):
