@@ -77,7 +77,9 @@ class Rollout(Structure):
                                         "spring_L0", "spring_k", "spring_coeff", "vert_ptr",
                                         "vert_idx", "act_ptr", "act_idx", "weights", "wgrad",
                                         "state", "adjoint", "ckpt", "scratch", "targets", "loss",
-                                        "loss_grad", "traj", "traj_grad")]
+                                        "loss_grad", "traj", "traj_grad", "err")]
+
+ROLLOUT_ERR_HORIZON, ROLLOUT_ERR_PACKING = 1, 2
 
 
 _P = c_void_p

@@ -9,11 +9,22 @@ of launch parameters (block size, LDS placement policy, kernel variant, MPM
 tile capacity ...).
 """
 
+import hashlib
 import json
 import os
 from pathlib import Path
 
 _DEFAULT = Path(__file__).resolve().parent / "_lib" / "autotune.json"
+_LIB = Path(__file__).resolve().parent / "_lib" / "libdiffloco.so"
+
+
+def library_tag(path=_LIB):
+    """Short content hash of the built library: part of every tuning key, so
+    a rebuilt library never reuses choices timed on another build."""
+    try:
+        return hashlib.sha1(Path(path).read_bytes()).hexdigest()[:12]
+    except OSError:
+        return "none"
 
 
 class Autotuner:
@@ -31,16 +42,26 @@ class Autotuner:
     def key(**parts):
         return "|".join(f"{k}={parts[k]}" for k in sorted(parts))
 
-    def pick(self, key, candidates, measure, repeat=2):
+    def pick(self, key, candidates, measure, repeat=2, verify=True):
         """Return the candidate with the smallest ``measure(candidate)``
         (milliseconds, best of ``repeat``); cached per key.  Candidates whose
         measurement raises are skipped; with tuning disabled (or one
-        candidate) the first candidate is returned untimed."""
+        candidate) the first candidate is returned untimed.  With ``verify`` a
+        cached choice is run once first; if that raises (a stale entry: the
+        workload or the library changed under the same key) the entry is
+        dropped and the key re-tuned."""
         if key in self.cache:
             want = self.cache[key]["choice"]
             for c in candidates:
                 if c == want:
-                    return c
+                    if not verify:
+                        return c
+                    try:
+                        measure(c)
+                        return c
+                    except Exception:
+                        del self.cache[key]
+                        break
         if not self.enabled or len(candidates) == 1:
             return candidates[0]
         timings = []

@@ -32,6 +32,7 @@ struct RArgs {
     const T *loss_grad;
     T *traj;
     const T *traj_grad;
+    int32_t *err;      // optional: bits of contract violations (DL_ROLLOUT_ERR_*)
     Phys<T> P;
     Layout L;
     int H, K, max_steps, act, loss_kind, window, store_traj;
@@ -394,6 +395,7 @@ __global__ void __launch_bounds__(LB) k_rollout_fwd(RArgs<T> A) {
     if (ROWS && (R.Th > A.K || R.sx == nullptr)) {
         // the stored rows need K >= horizon rows of scratch
         if (tid == 0 && A.loss) A.loss[r] = T(NAN);
+        if (tid == 0 && A.err) atomicOr(A.err, DL_ROLLOUT_ERR_HORIZON);
         return;
     }
     for (int i = tid; i < 3 * N; i += BT) {
@@ -643,6 +645,7 @@ __global__ void __launch_bounds__(LB) k_rollout_bwd(RArgs<T> A) {
     if (tb <= 0) return;
     if (STORED && R.Th > K) {
         if (tid == 0) R.Wg1[0] = T(NAN);
+        if (tid == 0 && A.err) atomicOr(A.err, DL_ROLLOUT_ERR_HORIZON);
         return;
     }
     for (int i = tid; i < 3 * N; i += BT) {
@@ -1180,6 +1183,7 @@ __global__ void __launch_bounds__(256) k_wave_fwd(RArgs<T> A, int n_robots) {
     if (A.robots[r].spring0 != A.robots[r0].spring0 || A.robots[r].weights != A.robots[r0].weights) {
         // robots of one workgroup must share a design (the host packs them so)
         if (lane == 0 && A.loss) A.loss[r] = T(NAN);
+        if (lane == 0 && A.err) atomicOr(A.err, DL_ROLLOUT_ERR_PACKING);
         return;
     }
     wave_point_at_shared(R, sh);
@@ -1192,6 +1196,7 @@ __global__ void __launch_bounds__(256) k_wave_fwd(RArgs<T> A, int n_robots) {
     if (ROWS && (R.Th > A.K || R.sx == nullptr)) {
         // the stored rows need K >= horizon rows of scratch
         if (lane == 0 && A.loss) A.loss[r] = T(NAN);
+        if (lane == 0 && A.err) atomicOr(A.err, DL_ROLLOUT_ERR_HORIZON);
         return;
     }
     WRegs<T, 1, OMAX> w;
@@ -1668,6 +1673,7 @@ __global__ void __launch_bounds__(256) k_wave_bwd(RArgs<T> A, int n_robots) {
     WSm<T> m = wave_carve(A, reinterpret_cast<T *>(smem_raw), wid, true);
     if (A.robots[r].spring0 != A.robots[r0].spring0 || A.robots[r].weights != A.robots[r0].weights) {
         if (lane == 0) R.Wg1[0] = T(NAN);   // robots of one workgroup must share a design
+        if (lane == 0 && A.err) atomicOr(A.err, DL_ROLLOUT_ERR_PACKING);
         return;
     }
     wave_point_at_shared(R, sh);
@@ -1733,6 +1739,9 @@ __global__ void __launch_bounds__(512) k_wave_bwd_pipe(RArgs<T> A, int n_robots)
         // robots of one workgroup must share a design (and stored rows need
         // K >= horizon rows of scratch)
         if (role == 0 && lane == 0) make_rob(A, r).Wg1[0] = T(NAN);
+        if (role == 0 && lane == 0 && A.err)
+            atomicOr(A.err, STORED && A.robots[r].horizon > A.K ? DL_ROLLOUT_ERR_HORIZON
+                                                                : DL_ROLLOUT_ERR_PACKING);
         active = false;
     }
     Rob<T> R = make_rob(A, active ? r : r0);
@@ -1858,6 +1867,7 @@ static RArgs<T> make_rargs(const dl_rollout *r) {
     A.targets = (const T *)r->targets;
     A.loss = (T *)r->loss; A.loss_grad = (const T *)r->loss_grad;
     A.traj = (T *)r->traj; A.traj_grad = (const T *)r->traj_grad;
+    A.err = r->err;
     A.P = make_phys<T>(r->phys);
     A.L = make_layout(r->lay);
     A.H = r->n_hidden; A.K = r->ckpt_every; A.max_steps = r->max_steps; A.act = r->act;

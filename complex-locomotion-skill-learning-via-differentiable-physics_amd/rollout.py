@@ -166,6 +166,7 @@ class MassSpringBatch:
         self.traj_grad = None
         self.loss = z(R)
         self.loss_grad = z(R)
+        self.err = torch.zeros(1, dtype=torch.int32, device=self.device)
         self.x0 = x0
         self.max_n = max(d.n_vert for d in designs)
         self.max_s = max(d.n_spring for d in designs)
@@ -205,6 +206,7 @@ class MassSpringBatch:
         self.loss.zero_()
         self.adjoint.zero_()
         self.wgrad.zero_()
+        self.err.zero_()
         if self.traj is not None:
             self.traj.zero_()
 
@@ -264,7 +266,27 @@ class MassSpringBatch:
         D.loss, D.loss_grad = P(self.loss), P(self.loss_grad)
         D.traj = P(self.traj) if self.traj is not None else None
         D.traj_grad = P(self.traj_grad) if self.traj_grad is not None else None
+        D.err = P(self.err)
         return D
+
+    def _check_contract(self):
+        """Host-side checks of the launch contract (the kernels would mark the
+        offending robots with NaN and an error bit instead)."""
+        if self.variant in (4, 5) and int(self.horizons.max()) > self.K:
+            raise ValueError(f"stored rows (variant {self.variant}) need ckpt_every >= every "
+                             f"horizon ({self.K} < {int(self.horizons.max())})")
+
+    def check(self):
+        """Raise if a launch reported a contract violation (reads the device
+        error word: synchronises)."""
+        e = int(self.err.item())
+        if e:
+            why = []
+            if e & nat.ROLLOUT_ERR_HORIZON:
+                why.append("a robot's horizon exceeds the stored-rows scratch (ckpt_every)")
+            if e & nat.ROLLOUT_ERR_PACKING:
+                why.append("robots packed into one workgroup do not share a design")
+            raise nat.NativeError("rollout: " + "; ".join(why or [f"error bits {e:#x}"]))
 
     def wave_eligible(self):
         """The wave-per-robot kernel covers FP32, hidden 64, inputs <= 192, outputs <= 8."""
@@ -279,6 +301,7 @@ class MassSpringBatch:
 
     def forward(self, max_steps=0):
         """Advance every robot from its clock by up to max_steps (0: to the end)."""
+        self._check_contract()
         D = self._desc(max_steps)
         nat.call("dl_ms_rollout_forward", ctypes.byref(D), nat.stream())
         step = self.horizons - self.table["t_fwd"] if max_steps <= 0 else \
@@ -293,6 +316,7 @@ class MassSpringBatch:
             raise RuntimeError("backward needs every robot's forward rollout to be complete")
         if loss_grad is not None:
             self.loss_grad.copy_(torch.as_tensor(loss_grad, dtype=self.dtype).reshape(-1))
+        self._check_contract()
         D = self._desc(max_steps)
         nat.call("dl_ms_rollout_backward", ctypes.byref(D), nat.stream())
         step = self.table["t_bwd"] if max_steps <= 0 else np.minimum(max_steps, self.table["t_bwd"])
@@ -308,6 +332,7 @@ class MassSpringBatch:
         (the forward reads only t_fwd, the adjoint only t_bwd) and the loss
         adjoint stay on the device, so ``run_iteration`` issues device work
         only and can be captured in a HIP graph."""
+        self._check_contract()
         self.reset()
         tab = self.table.copy()
         tab["t_fwd"] = 0

@@ -54,6 +54,7 @@ class Request:
     loss: float = None
     grads: list = None
     nbytes: int = 0
+    rejected: str = None       # why the request was turned away (no executor fits it)
 
     @property
     def latency(self):
@@ -119,10 +120,8 @@ class MSPool:
 
     # ------------------------------------------------------------ admission
     def fits(self, req):
-        d = req.design
-        return (req.kind == "ms" and d.n_vert <= self.Nc and d.n_spring <= self.Sc
-                and n_inputs(self.lay, d.n_vert) <= self.Ic and d.n_act <= self.Oc
-                and req.horizon <= self.Tc)
+        return req.kind == "ms" and fits_caps(req.design, req.horizon, self.lay,
+                                              (self.Nc, self.Sc, self.Ic, self.Oc, self.Tc))
 
     def bytes_for(self, req):
         """Device bytes this request actually uses (admission accounting)."""
@@ -424,12 +423,13 @@ class Server:
         self.max_active = max_active
         self.poll_s = poll_s
         self.ticks = 0
+        self.rejected = []
 
     def _executor(self, req):
         for e in self.ex:
             if e.fits(req):
                 return e
-        raise ValueError(f"request {req.rid}: no executor takes it")
+        return None
 
     def run(self, requests):
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
@@ -446,10 +446,17 @@ class Server:
             t = now()
             while pending and pending[0].arrival <= t:
                 queue.append(pending.popleft())
-            # admit in arrival order while the head fits (no overtaking)
+            # admit in arrival order while the head fits (no overtaking); a
+            # request no executor can take is turned away, the others go on
             while queue:
                 r = queue[0]
                 e = self._executor(r)
+                if e is None:
+                    queue.popleft()
+                    r.rejected = "no executor takes this robot (sizes beyond every pool's caps)"
+                    r.done = now()
+                    self.rejected.append(r)
+                    continue
                 need = e.bytes_for(r)
                 if need > self.budget:
                     raise ValueError(f"request {r.rid} needs {need} B, over the whole budget")
@@ -582,18 +589,35 @@ def make_requests(rate, duration, seed, layout, frac_mpm=0.3, grad_frac=0.5, n_h
     return reqs
 
 
+def pool_caps(layout):
+    """Caps (vertices, springs, inputs, outputs, horizon) of the two
+    mass-spring pools.  The wave pool's are the wave kernel's limits.  The
+    request generator draws 8-400 connected cubes, 4-64 actuator groups and
+    horizons up to 1000; its 400-cube robots reach ~820 vertices and ~6700
+    springs, so the block pool takes 1024 vertices and 8192 springs."""
+    return {"wave": (WAVE_MAX_VERT, 512, WAVE_MAX_IN, WAVE_MAX_OUT, 1000),
+            "block": (1024, 8192, n_inputs(layout, 1024), 64, 1000)}
+
+
+def fits_caps(design, horizon, layout, caps):
+    N, S, I, O, T = caps
+    return (design.n_vert <= N and design.n_spring <= S and n_inputs(layout, design.n_vert) <= I
+            and design.n_act <= O and horizon <= T)
+
+
 def default_executors(physics, layout, slots_small=256, slots_large=128, mpm_active=8,
                       ckpt_every=50, loss_window=100, dtype=torch.float32, device="cuda",
                       stored=True):
     """Two mass-spring pools (wave-per-robot kernel for small robots, block
-    kernel up to 800 vertices / 7000 springs / 64 actuators / 1000 steps) and
-    the MPM runner.  With `stored` the pools' forwards keep every step's
+    kernel up to 1024 vertices / 8192 springs / 64 actuators / 1000 steps)
+    and the MPM runner.  With `stored` the pools' forwards keep every step's
     adjoint rows (variants 4 and 5), else checkpoints every ckpt_every steps
     are recomputed (variants 3 and 1)."""
-    small = MSPool("wave", slots_small, (WAVE_MAX_VERT, 512, WAVE_MAX_IN, WAVE_MAX_OUT, 1000),
+    caps = pool_caps(layout)
+    small = MSPool("wave", slots_small, caps["wave"],
                    physics, layout, ckpt_every, loss_window, dtype=dtype,
                    variant=4 if stored else 3, block=64, device=device)
-    large = MSPool("block", slots_large, (800, 7000, n_inputs(layout, 800), 64, 1000), physics,
+    large = MSPool("block", slots_large, caps["block"], physics,
                    layout, ckpt_every, loss_window, dtype=dtype, variant=5 if stored else 1,
                    block=256, device=device)
     return [small, large, MPMRunner(mpm_active, dtype=dtype, device=device)]

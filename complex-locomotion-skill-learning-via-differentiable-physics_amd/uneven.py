@@ -18,7 +18,7 @@ grows with the robot).
 import numpy as np
 import torch
 
-from .autotune import Autotuner
+from .autotune import Autotuner, library_tag
 from .rollout import GraphedIteration, MassSpringBatch
 
 WAVE_MAX_IN, WAVE_MAX_OUT, WAVE_MAX_VERT = 192, 8, 64
@@ -124,7 +124,9 @@ class UnevenBatch:
             nv = np.array([b.designs[int(d)].n_vert for d in b.robot_design])
             key = Autotuner.key(kernel="ms_rollout", cls=bk["name"], dtype=str(self.dtype),
                                 robots=len(bk["ids"]), mean_n=int(nv.mean()),
-                                mean_T=int(b.horizons.mean()), stored=int(self.stored))
+                                mean_T=int(b.horizons.mean()), stored=int(self.stored),
+                                max_n=b.max_n, max_s=b.max_s, max_i=b.max_i, max_o=b.max_o,
+                                lib=library_tag())
 
             def measure(cfg, bk=bk):
                 self._apply(bk, cfg)

@@ -274,7 +274,15 @@ typedef struct dl_rollout {
     const void *loss_grad;   /* [n_robots] upstream dL/dloss_r for the backward */
     void *traj;              /* forward trajectory rows */
     const void *traj_grad;   /* backward: external dL/dx_t, dL/dc_t rows (NULL = none) */
+    int32_t *err;            /* optional device word (NULL = none): the launches OR in
+                                DL_ROLLOUT_ERR_* bits when a robot breaks the table's
+                                contract (that robot's loss or gradient is also set to NaN
+                                and it is skipped); zero it before the launches */
 } dl_rollout;
+
+enum { DL_ROLLOUT_ERR_HORIZON = 1,   /* stored rows (variant 4/5): horizon > ckpt_every */
+       DL_ROLLOUT_ERR_PACKING = 2 }; /* wave kernels: robots of one workgroup of 4 do not
+                                        share a design */
 
 /* Advance every robot of the table from its t_fwd by up to max_steps steps
  * (one persistent launch), writing checkpoints every K steps. */

@@ -378,3 +378,52 @@ def test_batched_task_loss_free_fall_closed_form(prec, variant):
     got = batch.loss.double().cpu().numpy()
     tol = 1e-11 if prec == "f64" else 2e-5
     assert np.allclose(got, want, rtol=tol, atol=0), (got, want)
+
+
+def test_rollout_contract_violations_set_the_error_word():
+    """A launch that breaks the offset table's contract reports it in the
+    descriptor's error word (and NaN for the robots concerned) instead of
+    computing garbage: stored rows with a horizon beyond ckpt_every, and a
+    4-robot wave workgroup mixing designs.  MassSpringBatch refuses the
+    first on the host before launching."""
+    _say("rollout contract violations")
+    B, T, W = 4, 60, 20
+    tv, th = task_targets(B, 97)
+    tg = np.concatenate([tv, th[:, None], np.zeros((B, 1))], 1)
+    design = dl.build_quadruped()
+    qc = dl.quadruped_config(batch_size=B, steps=T, window=W).get_config()
+    lay = om.TaskConfig().layout
+    w = [x.to("cuda", torch.float32) for x in om.init_weights(lay.n_input(design.n_vert), 64, 4,
+                                                               seed=98)]
+    x0 = design.objects + np.array([0.4, 0.0, 0.0])
+    b = MassSpringBatch([design], [0] * B, T, _physics(qc), sensor_layout(qc), tg, [x0] * B,
+                        act="tanh", ckpt_every=16, dtype=torch.float32, loss_window=W, variant=4)
+    b.set_weights(0, *w)
+    b.reset()
+    D = b._desc()
+    D.ckpt_every = 16                          # scratch holds T rows; claim only 16
+    nat.call("dl_ms_rollout_forward", ctypes.byref(D), nat.stream())
+    torch.cuda.synchronize()
+    assert int(b.err.item()) == nat.ROLLOUT_ERR_HORIZON
+    assert torch.isnan(b.loss).all()
+    with pytest.raises(nat.NativeError, match="horizon"):
+        b.check()
+    b.K = 16
+    with pytest.raises(ValueError, match="stored rows"):
+        b.forward()
+    # two designs inside one 4-robot workgroup
+    d2 = dl.build_voxels(dl.quadruped_cubes()[::-1], n_act=4)
+    assert d2.n_vert == design.n_vert
+    m = MassSpringBatch([design, d2], [0, 1, 0, 0], T, _physics(qc), sensor_layout(qc), tg,
+                        [x0] * B, act="tanh", ckpt_every=16, dtype=torch.float32, loss_window=W,
+                        variant=3)
+    assert not m.packed4()
+    m.set_weights(0, *w)
+    m.set_weights(1, *w)
+    m.reset()
+    D = m._desc()
+    D.block = 256                              # force 4 robots per workgroup
+    nat.call("dl_ms_rollout_forward", ctypes.byref(D), nat.stream())
+    torch.cuda.synchronize()
+    assert int(m.err.item()) == nat.ROLLOUT_ERR_PACKING
+    assert torch.isnan(m.loss[1]) and torch.isfinite(m.loss[[0, 2, 3]]).all()

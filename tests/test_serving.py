@@ -106,6 +106,41 @@ def test_request_over_the_whole_budget_is_rejected():
         srv.run(_reqs([("ms", 500, False, 0.0)]))
 
 
+def test_unfit_request_is_rejected_and_the_rest_served():
+    """A request no executor takes is turned away with a reason; the loop
+    goes on serving the others."""
+    ck = Clock()
+    log = []
+    srv = sv.Server([FakeExec("ms", 4, log)], budget_bytes=10000, chunk=50, clock=ck,
+                    sleep=ck.sleep, events=ck)
+    done = srv.run(_reqs([("ms", 100, False, 0.0), ("mpm", 100, False, 0.1),
+                          ("ms", 100, True, 0.2)]))
+    assert sorted(r.rid for r in done) == [0, 2]
+    assert [r.rid for r in srv.rejected] == [1] and "no executor" in srv.rejected[0].rejected
+
+
+def test_every_generated_robot_fits_a_pool():
+    """The mass-spring robots of the request generator (8-400 cubes
+    log-uniform, 4-64 groups, horizons 100-1000) all fit one of the two
+    pools' caps: 2000 draws from the generator's distribution."""
+    import diffloco as dl
+    from diffloco.robot_design import build_voxels, random_voxel_cubes
+    from diffloco.taichi_env import sensor_layout
+    lay = sensor_layout(dl.quadruped_config(batch_size=1).get_config())
+    caps = sv.pool_caps(lay)
+    rng = np.random.default_rng(4)
+    biggest = 0
+    for _ in range(2000):
+        nc = int(round(math.exp(rng.uniform(math.log(8), math.log(400)))))
+        cubes, g = random_voxel_cubes(rng, nc, int(rng.integers(4, 65)))
+        d = build_voxels(cubes, n_act=g)
+        T = int(rng.integers(100, 1001))
+        assert sv.fits_caps(d, T, lay, caps["wave"]) or sv.fits_caps(d, T, lay, caps["block"]), \
+            (nc, d.n_vert, d.n_spring, d.n_act)
+        biggest = max(biggest, d.n_vert)
+    assert biggest > 800          # the old 800-vertex cap would have refused some
+
+
 @pytest.mark.gpu
 def test_results_same_alone_and_in_the_mix():
     import diffloco as dl

@@ -58,8 +58,35 @@ def test_autotuner_picks_fastest_and_caches(tmp_path):
     assert t.pick("k", cands, measure) == {"block": 128}
     n = len(calls)
     t2 = Autotuner(tmp_path / "tune.json")
-    assert t2.pick("k", cands, measure) == {"block": 128} and len(calls) == n   # cached
+    # cached: one verification run of the cached choice, no re-tune
+    assert t2.pick("k", cands, measure) == {"block": 128} and calls[n:] == [128]
     assert "error" in str(t2.cache["k"]["timings"])
+    assert t2.pick("k", cands, measure, verify=False) == {"block": 128} and len(calls) == n + 1
+
+
+def test_autotuner_drops_a_stale_entry(tmp_path):
+    """A cached choice that no longer runs (the workload or library changed
+    under the same key) is dropped and the key re-tuned."""
+    t = Autotuner(tmp_path / "tune.json")
+    cands = [{"block": 64}, {"block": 128}]
+    t.pick("k", cands, lambda c: {64: 2.0, 128: 1.0}[c["block"]])
+    assert t.cache["k"]["choice"] == {"block": 128}
+
+    def measure(c):
+        if c["block"] == 128:
+            raise RuntimeError("LDS limit exceeded")
+        return 5.0
+
+    assert Autotuner(tmp_path / "tune.json").pick("k", cands, measure) == {"block": 64}
+    assert Autotuner(tmp_path / "tune.json").cache["k"]["choice"] == {"block": 64}
+
+
+def test_tuning_key_names_sizes_and_library():
+    from diffloco.autotune import library_tag
+    tag = library_tag()
+    assert len(tag) == 12 and tag != "none"
+    k1 = Autotuner.key(cls="n<=400", max_n=400, lib=tag)
+    assert k1 != Autotuner.key(cls="n<=400", max_n=399, lib=tag)
 
 
 @pytest.mark.gpu
