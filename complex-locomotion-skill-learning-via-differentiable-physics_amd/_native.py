@@ -107,6 +107,14 @@ _SIGS = {
     "dl_loss_velocity_grad": ([_I] * 7 + [_P, _P, c_double, _P, _P], c_int32),
     "dl_loss_rotation": ([_I] * 8 + [_P] * 7, c_int32),
     "dl_loss_rotation_grad": ([_I] * 8 + [_P, _P, _P, c_double, _P, _P, _P], c_int32),
+    "dl_loss_height": ([_I] * 6 + [_P] * 7, c_int32),
+    "dl_loss_height_grad": ([_I] * 6 + [_P, _P, c_double, _P, _P], c_int32),
+    "dl_loss_pose": ([_I] * 7 + [_P] * 6, c_int32),
+    "dl_loss_pose_grad": ([_I] * 7 + [_P, _P, c_double, _P, _P], c_int32),
+    "dl_loss_actuation": ([_I] * 6 + [_P] * 7, c_int32),
+    "dl_loss_actuation_grad": ([_I] * 6 + [_P, _P, _P, c_double, _P, _P], c_int32),
+    "dl_loss_crawl": ([_I] * 5 + [_P] * 6, c_int32),
+    "dl_loss_crawl_grad": ([_I] * 5 + [_P, _P, c_double, _P, _P], c_int32),
     "dl_loss_task": ([_I] * 4 + [_P] * 7, c_int32),
     "dl_loss_task_grad": ([_I] * 4 + [_P] * 4 + [c_double] + [_P] * 3, c_int32),
     "dl_weights_size": ([_I, _I, _I], c_int64),

@@ -638,6 +638,157 @@ __global__ void k_loss_task_grad(int B, int Tn, int W, const T *c, const T *vc, 
     cg[o + 1] += sc * (c[o + 1] - th[k]);
 }
 
+
+// ------------------------------------------------ height / pose / actuation / crawl
+// Rows t >= Tw of a field the rollout did not write read as zero (as in the
+// other losses).  Per-robot partial sums go to `part` [B]; k_loss_sum adds
+// them in a fixed order.
+
+// compute_loss_height (taichi_env.py:213-222): t in [1, steps] with
+// t % jp == jp - 1 and th > 0.1 + 1e-6.  hmax[k]: the largest contributing
+// height of robot k (-inf when none; current_max_height).
+template <typename T>
+__global__ void k_loss_height(int B, int Tw, int steps, int jp, const T *h, const T *th, T *part,
+                              T *hmax) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= B) return;
+    const T sc = T(100) / T(B) / T(steps / jp);
+    T s = T(0), mx = -INFINITY;
+    for (int t = jp - 1; t <= steps; t += jp) {
+        if (t < 1) continue;
+        const T tg = th[(size_t)t * B + k];
+        if (!(tg > T(0.1) + T(1e-6))) continue;
+        const T hv = t < Tw ? h[(size_t)t * B + k] : T(0);
+        s += (hv - tg) * (hv - tg) * sc;
+        mx = fmax(mx, hv);
+    }
+    part[k] = s;
+    if (hmax) hmax[k] = mx;
+}
+
+template <typename T>
+__global__ void k_loss_height_grad(int B, int Tw, int steps, int jp, const T *h, const T *th, T g,
+                                   T *hg) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= B) return;
+    const T sc = g * T(200) / T(B) / T(steps / jp);
+    for (int t = jp - 1; t <= steps && t < Tw; t += jp) {
+        if (t < 1) continue;
+        const T tg = th[(size_t)t * B + k];
+        if (tg > T(0.1) + T(1e-6)) hg[(size_t)t * B + k] += (h[(size_t)t * B + k] - tg) * sc;
+    }
+}
+
+// compute_loss_pose (taichi_env.py:224-233): t in [1, steps], t % jp == 0,
+// sum_i |x_i - obj_i|^2 / B / (steps / jp).  x [T][B][3][N], obj [N][3];
+// block per robot.
+template <typename T>
+__global__ void k_loss_pose(int B, int N, int Tw, int steps, int jp, const T *x, const T *obj,
+                            T *part) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *red = reinterpret_cast<T *>(smem_raw);
+    const int k = blockIdx.x;
+    T s[1] = {T(0)};
+    for (int t = jp; t <= steps; t += jp) {
+        for (int i = threadIdx.x; i < N; i += blockDim.x)
+            for (int d = 0; d < 3; ++d) {
+                const T xv = t < Tw ? x[((size_t)t * B + k) * 3 * N + d * N + i] : T(0);
+                const T e = xv - obj[i * 3 + d];
+                s[0] += e * e;
+            }
+    }
+    block_sum<T, 1>(s, red);
+    if (threadIdx.x == 0) part[k] = s[0] / T(B) / T(steps / jp);
+}
+
+template <typename T>
+__global__ void k_loss_pose_grad(int B, int N, int Tw, int steps, int jp, const T *x,
+                                 const T *obj, T g, T *xg) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= B * N) return;
+    const int k = id / N, i = id - k * N;
+    const T sc = g * T(2) / T(B) / T(steps / jp);
+    for (int t = jp; t <= steps && t < Tw; t += jp)
+        for (int d = 0; d < 3; ++d) {
+            const size_t o = ((size_t)t * B + k) * 3 * N + d * N + i;
+            xg[o] += (x[o] - obj[i * 3 + d]) * sc;
+        }
+}
+
+// compute_loss_actuation (taichi_env.py:276-285): t in [0, steps) with
+// th < 0.1 + 1e-4, max(|a| - sqrt(|tv_x| / 0.08), 0) / O / B / steps * 10 per
+// output (the reference's outputs are its springs).  act [T][B][O],
+// tv [T][B][3].  The adjoint of |a| is sign(a), of max(u, 0) the step u > 0.
+template <typename T>
+__global__ void k_loss_actuation(int B, int O, int Tw, int steps, const T *a, const T *tv,
+                                 const T *th, T *part) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= B) return;
+    T s = T(0);
+    for (int t = 0; t < steps; ++t) {
+        if (!(th[(size_t)t * B + k] < T(0.1) + T(1e-4))) continue;
+        const T lim = M<T>::sqrt(fabs(tv[((size_t)t * B + k) * 3]) / T(0.08));
+        for (int o = 0; o < O; ++o) {
+            const T av = t < Tw ? a[((size_t)t * B + k) * O + o] : T(0);
+            s += fmax(fabs(av) - lim, T(0));
+        }
+    }
+    part[k] = s / T(O) / T(B) / T(steps) * T(10);
+}
+
+template <typename T>
+__global__ void k_loss_actuation_grad(int B, int O, int Tw, int steps, const T *a, const T *tv,
+                                      const T *th, T g, T *ag) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= B * O) return;
+    const int k = id / O, o = id - k * O;
+    const T sc = g / T(O) / T(B) / T(steps) * T(10);
+    for (int t = 0; t < steps && t < Tw; ++t) {
+        if (!(th[(size_t)t * B + k] < T(0.1) + T(1e-4))) continue;
+        const T lim = M<T>::sqrt(fabs(tv[((size_t)t * B + k) * 3]) / T(0.08));
+        const size_t q = ((size_t)t * B + k) * O + o;
+        const T av = a[q];
+        if (fabs(av) - lim > T(0)) ag[q] += (av > T(0) ? sc : (av < T(0) ? -sc : T(0)));
+    }
+}
+
+// compute_loss_crawl (taichi_env.py:287-293): t in [1, steps] with
+// tc > 1 - 1e-4, max(upper - 0.1, 0)^2 / B / steps * 5.
+template <typename T>
+__global__ void k_loss_crawl(int B, int Tw, int steps, const T *up, const T *tc, T *part) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= B) return;
+    T s = T(0);
+    for (int t = 1; t <= steps; ++t) {
+        if (!(tc[(size_t)t * B + k] > T(1) - T(1e-4))) continue;
+        const T u = fmax((t < Tw ? up[(size_t)t * B + k] : T(0)) - T(0.1), T(0));
+        s += u * u;
+    }
+    part[k] = s / T(B) / T(steps) * T(5);
+}
+
+template <typename T>
+__global__ void k_loss_crawl_grad(int B, int Tw, int steps, const T *up, const T *tc, T g,
+                                  T *ug) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= B) return;
+    const T sc = g * T(10) / T(B) / T(steps);
+    for (int t = 1; t <= steps && t < Tw; ++t) {
+        if (!(tc[(size_t)t * B + k] > T(1) - T(1e-4))) continue;
+        const T u = up[(size_t)t * B + k] - T(0.1);
+        if (u > T(0)) ug[(size_t)t * B + k] += u * sc;
+    }
+}
+
+// Largest entry of hmax [B] (fixed order) into out[0] when it exceeds it.
+template <typename T>
+__global__ void k_max_update(int B, const T *hmax, T *out) {
+    if (threadIdx.x != 0) return;
+    T m = -INFINITY;
+    for (int k = 0; k < B; ++k) m = fmax(m, hmax[k]);
+    if (m > out[0]) out[0] = m;
+}
+
 }  // namespace dl
 
 // ============================================================== C ABI
@@ -996,6 +1147,126 @@ int32_t dl_loss_task_grad(int32_t dtype, int32_t B, int32_t Tn, int32_t W, const
                                           dim3(256), 0, S_(stream), B, Tn, W, (const real *)c,
                                           (const real *)vc, (const real *)tv, (const real *)th,
                                           (real)g, (real *)cg, (real *)vcg));
+    DL_CHECK_LAUNCH();
+    return 0;
+}
+
+int32_t dl_loss_height(int32_t dtype, int32_t B, int32_t T, int32_t Tw, int32_t steps, int32_t jp,
+                       const void *h, const void *th, void *loss, void *lb, void *max_height,
+                       void *work, void *stream) {
+    DL_CHECK_ARG(B > 0 && T > 0 && Tw <= T && steps > 0 && jp > 0 && steps / jp > 0 &&
+                     steps < T && h && th && loss && work,
+                 "dl_loss_height: bad arguments");
+    DL_DISPATCH(dtype, {
+        real *part = (real *)work, *hm = part + B;
+        hipLaunchKernelGGL(k_loss_height<real>, dim3(nblk(B, 64)), dim3(64), 0, S_(stream), B, Tw,
+                           steps, jp, (const real *)h, (const real *)th, part, hm);
+        hipLaunchKernelGGL(k_loss_sum<real>, dim3(1), dim3(256), sizeof(real) * 8, S_(stream), B,
+                           (const real *)part, (real *)loss, (real *)lb);
+        if (max_height)
+            hipLaunchKernelGGL(k_max_update<real>, dim3(1), dim3(64), 0, S_(stream), B,
+                               (const real *)hm, (real *)max_height);
+    });
+    DL_CHECK_LAUNCH();
+    return 0;
+}
+
+int32_t dl_loss_height_grad(int32_t dtype, int32_t B, int32_t T, int32_t Tw, int32_t steps,
+                            int32_t jp, const void *h, const void *th, double g, void *hg,
+                            void *stream) {
+    DL_CHECK_ARG(B > 0 && T > 0 && Tw <= T && steps > 0 && jp > 0 && steps / jp > 0 &&
+                     steps < T && h && th && hg,
+                 "dl_loss_height_grad: bad arguments");
+    DL_DISPATCH(dtype, hipLaunchKernelGGL(k_loss_height_grad<real>, dim3(nblk(B, 64)), dim3(64), 0,
+                                          S_(stream), B, Tw, steps, jp, (const real *)h,
+                                          (const real *)th, (real)g, (real *)hg));
+    DL_CHECK_LAUNCH();
+    return 0;
+}
+
+int32_t dl_loss_pose(int32_t dtype, int32_t B, int32_t N, int32_t T, int32_t Tw, int32_t steps,
+                     int32_t jp, const void *x, const void *obj, void *loss, void *lb, void *work,
+                     void *stream) {
+    DL_CHECK_ARG(B > 0 && N > 0 && T > 0 && Tw <= T && steps > 0 && jp > 0 && steps / jp > 0 &&
+                     x && obj && loss && work,
+                 "dl_loss_pose: bad arguments");
+    DL_DISPATCH(dtype, {
+        hipLaunchKernelGGL(k_loss_pose<real>, dim3(B), dim3(64), sizeof(real) * 8, S_(stream), B,
+                           N, Tw, steps, jp, (const real *)x, (const real *)obj, (real *)work);
+        hipLaunchKernelGGL(k_loss_sum<real>, dim3(1), dim3(256), sizeof(real) * 8, S_(stream), B,
+                           (const real *)work, (real *)loss, (real *)lb);
+    });
+    DL_CHECK_LAUNCH();
+    return 0;
+}
+
+int32_t dl_loss_pose_grad(int32_t dtype, int32_t B, int32_t N, int32_t T, int32_t Tw,
+                          int32_t steps, int32_t jp, const void *x, const void *obj, double g,
+                          void *xg, void *stream) {
+    DL_CHECK_ARG(B > 0 && N > 0 && T > 0 && Tw <= T && steps > 0 && jp > 0 && steps / jp > 0 &&
+                     x && obj && xg,
+                 "dl_loss_pose_grad: bad arguments");
+    DL_DISPATCH(dtype, hipLaunchKernelGGL(k_loss_pose_grad<real>, dim3(nblk((long)B * N, 256)),
+                                          dim3(256), 0, S_(stream), B, N, Tw, steps, jp,
+                                          (const real *)x, (const real *)obj, (real)g,
+                                          (real *)xg));
+    DL_CHECK_LAUNCH();
+    return 0;
+}
+
+int32_t dl_loss_actuation(int32_t dtype, int32_t B, int32_t O, int32_t T, int32_t Tw,
+                          int32_t steps, const void *a, const void *tv, const void *th,
+                          void *loss, void *lb, void *work, void *stream) {
+    DL_CHECK_ARG(B > 0 && O > 0 && T > 0 && Tw <= T && steps > 0 && steps <= T && a && tv && th &&
+                     loss && work,
+                 "dl_loss_actuation: bad arguments");
+    DL_DISPATCH(dtype, {
+        hipLaunchKernelGGL(k_loss_actuation<real>, dim3(nblk(B, 64)), dim3(64), 0, S_(stream), B,
+                           O, Tw, steps, (const real *)a, (const real *)tv, (const real *)th,
+                           (real *)work);
+        hipLaunchKernelGGL(k_loss_sum<real>, dim3(1), dim3(256), sizeof(real) * 8, S_(stream), B,
+                           (const real *)work, (real *)loss, (real *)lb);
+    });
+    DL_CHECK_LAUNCH();
+    return 0;
+}
+
+int32_t dl_loss_actuation_grad(int32_t dtype, int32_t B, int32_t O, int32_t T, int32_t Tw,
+                               int32_t steps, const void *a, const void *tv, const void *th,
+                               double g, void *ag, void *stream) {
+    DL_CHECK_ARG(B > 0 && O > 0 && T > 0 && Tw <= T && steps > 0 && steps <= T && a && tv && th &&
+                     ag,
+                 "dl_loss_actuation_grad: bad arguments");
+    DL_DISPATCH(dtype, hipLaunchKernelGGL(k_loss_actuation_grad<real>,
+                                          dim3(nblk((long)B * O, 256)), dim3(256), 0, S_(stream),
+                                          B, O, Tw, steps, (const real *)a, (const real *)tv,
+                                          (const real *)th, (real)g, (real *)ag));
+    DL_CHECK_LAUNCH();
+    return 0;
+}
+
+int32_t dl_loss_crawl(int32_t dtype, int32_t B, int32_t T, int32_t Tw, int32_t steps,
+                      const void *up, const void *tc, void *loss, void *lb, void *work,
+                      void *stream) {
+    DL_CHECK_ARG(B > 0 && T > 0 && Tw <= T && steps > 0 && steps < T && up && tc && loss && work,
+                 "dl_loss_crawl: bad arguments");
+    DL_DISPATCH(dtype, {
+        hipLaunchKernelGGL(k_loss_crawl<real>, dim3(nblk(B, 64)), dim3(64), 0, S_(stream), B, Tw,
+                           steps, (const real *)up, (const real *)tc, (real *)work);
+        hipLaunchKernelGGL(k_loss_sum<real>, dim3(1), dim3(256), sizeof(real) * 8, S_(stream), B,
+                           (const real *)work, (real *)loss, (real *)lb);
+    });
+    DL_CHECK_LAUNCH();
+    return 0;
+}
+
+int32_t dl_loss_crawl_grad(int32_t dtype, int32_t B, int32_t T, int32_t Tw, int32_t steps,
+                           const void *up, const void *tc, double g, void *ug, void *stream) {
+    DL_CHECK_ARG(B > 0 && T > 0 && Tw <= T && steps > 0 && steps < T && up && tc && ug,
+                 "dl_loss_crawl_grad: bad arguments");
+    DL_DISPATCH(dtype, hipLaunchKernelGGL(k_loss_crawl_grad<real>, dim3(nblk(B, 64)), dim3(64), 0,
+                                          S_(stream), B, Tw, steps, (const real *)up,
+                                          (const real *)tc, (real)g, (real *)ug));
     DL_CHECK_LAUNCH();
     return 0;
 }

@@ -87,6 +87,7 @@ class TaichiEnv:
         self.x, self.v, self.center, self.height = s.x, s.v, s.center, s.height
         self.upper_height, self.rotation, self.actuation = s.upper_height, s.rotation, s.actuation
         self._work = torch.zeros(B, dtype=self.dtype, device=self.device)
+        self._work2 = torch.zeros(2 * B, dtype=self.dtype, device=self.device)
         self._gen = torch.Generator(device=self.device)
 
     def n_objects_(self):
@@ -223,6 +224,76 @@ class TaichiEnv:
                      nat.ptr(s.vcom.grad.data), nat.stream())
         record(adj)
 
+    def compute_loss_height(self, steps, written):
+        """taichi_env.py:213-222; also raises current_max_height."""
+        B, s = self.batch_size, self.solver
+        Tn = s.height.data.shape[0]
+        nat.call("dl_loss_height", self._dc, B, Tn, written, steps, self.jump_period,
+                 nat.ptr(s.height.data), nat.ptr(self.target_h.data),
+                 nat.ptr(self.loss_dict["loss_height"].data),
+                 nat.ptr(self.loss_dict_batch["loss_height"].data),
+                 nat.ptr(self.current_max_height.data), nat.ptr(self._work2), nat.stream())
+
+        def adj():
+            g = self.loss_dict["loss_height"].grad[None]
+            nat.call("dl_loss_height_grad", self._dc, B, Tn, written, steps, self.jump_period,
+                     nat.ptr(s.height.data), nat.ptr(self.target_h.data), float(g),
+                     nat.ptr(s.height.grad.data), nat.stream())
+        record(adj)
+
+    def compute_loss_pose(self, steps, written):
+        """taichi_env.py:224-233 (x rows up to ``written`` hold states)."""
+        B, N, s = self.batch_size, self.n_objects_(), self.solver
+        Tn = s.x.data.shape[0]
+        Tx = min(written + 1, Tn)
+        nat.call("dl_loss_pose", self._dc, B, N, Tn, Tx, steps, self.jump_period,
+                 nat.ptr(s.x.data), nat.ptr(self.initial_objects.data),
+                 nat.ptr(self.loss_dict["loss_pose"].data),
+                 nat.ptr(self.loss_dict_batch["loss_pose"].data), nat.ptr(self._work), nat.stream())
+
+        def adj():
+            g = self.loss_dict["loss_pose"].grad[None]
+            nat.call("dl_loss_pose_grad", self._dc, B, N, Tn, Tx, steps, self.jump_period,
+                     nat.ptr(s.x.data), nat.ptr(self.initial_objects.data), float(g),
+                     nat.ptr(s.x.grad.data), nat.stream())
+        record(adj)
+
+    def compute_loss_actuation(self, steps, written):
+        """taichi_env.py:276-285 over the network outputs (the reference's
+        robots have one output per spring)."""
+        B, s = self.batch_size, self.solver
+        O = s.actuation.data.shape[-1]
+        Tn = s.actuation.data.shape[0]
+        nat.call("dl_loss_actuation", self._dc, B, O, Tn, written, steps,
+                 nat.ptr(s.actuation.data), nat.ptr(self.target_v.data),
+                 nat.ptr(self.target_h.data), nat.ptr(self.loss_dict["loss_actuation"].data),
+                 nat.ptr(self.loss_dict_batch["loss_actuation"].data), nat.ptr(self._work),
+                 nat.stream())
+
+        def adj():
+            g = self.loss_dict["loss_actuation"].grad[None]
+            nat.call("dl_loss_actuation_grad", self._dc, B, O, Tn, written, steps,
+                     nat.ptr(s.actuation.data), nat.ptr(self.target_v.data),
+                     nat.ptr(self.target_h.data), float(g), nat.ptr(s.actuation.grad.data),
+                     nat.stream())
+        record(adj)
+
+    def compute_loss_crawl(self, steps, written):
+        """taichi_env.py:287-293."""
+        B, s = self.batch_size, self.solver
+        Tn = s.upper_height.data.shape[0]
+        nat.call("dl_loss_crawl", self._dc, B, Tn, written, steps, nat.ptr(s.upper_height.data),
+                 nat.ptr(self.target_c.data), nat.ptr(self.loss_dict["loss_crawl"].data),
+                 nat.ptr(self.loss_dict_batch["loss_crawl"].data), nat.ptr(self._work),
+                 nat.stream())
+
+        def adj():
+            g = self.loss_dict["loss_crawl"].grad[None]
+            nat.call("dl_loss_crawl_grad", self._dc, B, Tn, written, steps,
+                     nat.ptr(s.upper_height.data), nat.ptr(self.target_c.data), float(g),
+                     nat.ptr(s.upper_height.grad.data), nat.stream())
+        record(adj)
+
     def compute_loss_final(self, l):
         """loss += l (taichi_env.py:295-297)."""
         self.loss.data += l.data
@@ -232,20 +303,27 @@ class TaichiEnv:
         record(adj)
 
     def get_loss(self, steps, loss_enable, written=None):
-        """taichi_env.py:304-331 for the velocity/rotation/task losses.
+        """taichi_env.py:304-331: the velocity, height, pose, actuation,
+        rotation and crawl losses (in the reference's order) and the
+        quadruped task loss.
         ``written``: number of leading center/x rows the rollout wrote
         (rows at and after it read as zero, as the unwritten field entries do)."""
         if written is None:
             written = steps
         if "velocity" in loss_enable:
             self.compute_loss_velocity(steps, written)
+        if "height" in loss_enable:
+            self.compute_loss_height(steps, written)
+        if "pose" in loss_enable:
+            self.compute_loss_pose(steps, written)
+        if "actuation" in loss_enable:
+            self.compute_loss_actuation(steps, written)
         if "rotation" in loss_enable:
             self.compute_loss_rotation(steps, written)
+        if "crawl" in loss_enable:
+            self.compute_loss_crawl(steps, written)
         if "task" in loss_enable:
             self.compute_loss_task(steps)
-        for name in ("height", "pose", "actuation", "crawl"):
-            if name in loss_enable:
-                raise NotImplementedError(f"loss '{name}' is not part of this path")
         for l in self.losses:
             self.compute_loss_final(l)
         self.loss_batch.data.zero_()

@@ -185,6 +185,52 @@ int32_t dl_loss_rotation_grad(int32_t dtype, int32_t B, int32_t N, int32_t T, in
                               const void *x, const void *center, const void *th,
                               double loss_grad, void *x_grad, void *center_grad, void *stream);
 
+/* compute_loss_height (taichi_env.py:213-222) over height [T][B], target_h
+ * [T][B] (rows t <= steps < T read; height rows >= T_written read as zero):
+ * sum over t in [1, steps] with t % jump_period == jump_period - 1 and
+ * target_h > 0.1 + 1e-6 of (height - target_h)^2 / B / (steps / jump_period)
+ * * 100.  loss_batch [B] = per-robot sum * B (may be NULL); max_height [1]
+ * (may be NULL) is raised to the largest contributing height
+ * (current_max_height).  work: B*2 reals. */
+int32_t dl_loss_height(int32_t dtype, int32_t B, int32_t T, int32_t T_written, int32_t steps,
+                       int32_t jump_period, const void *height, const void *target_h, void *loss,
+                       void *loss_batch, void *max_height, void *work, void *stream);
+int32_t dl_loss_height_grad(int32_t dtype, int32_t B, int32_t T, int32_t T_written,
+                            int32_t steps, int32_t jump_period, const void *height,
+                            const void *target_h, double loss_grad, void *height_grad,
+                            void *stream);
+/* compute_loss_pose (taichi_env.py:224-233): x [T][B][3][N], initial objects
+ * [N][3]; sum over t in [1, steps], t % jump_period == 0, of |x - obj|^2 /
+ * B / (steps / jump_period).  work: B reals. */
+int32_t dl_loss_pose(int32_t dtype, int32_t B, int32_t N, int32_t T, int32_t T_written,
+                     int32_t steps, int32_t jump_period, const void *x, const void *objects,
+                     void *loss, void *loss_batch, void *work, void *stream);
+int32_t dl_loss_pose_grad(int32_t dtype, int32_t B, int32_t N, int32_t T, int32_t T_written,
+                          int32_t steps, int32_t jump_period, const void *x, const void *objects,
+                          double loss_grad, void *x_grad, void *stream);
+/* compute_loss_actuation (taichi_env.py:276-285): actuation [T][B][O],
+ * target_v [T][B][3], target_h [T][B]; sum over t in [0, steps) with
+ * target_h < 0.1 + 1e-4 of max(|a| - sqrt(|tv_x| / 0.08), 0) / O / B / steps
+ * * 10 (O = the reference's spring count for its robots).  Adjoint: sign(a)
+ * where |a| exceeds the allowance.  work: B reals. */
+int32_t dl_loss_actuation(int32_t dtype, int32_t B, int32_t O, int32_t T, int32_t T_written,
+                          int32_t steps, const void *act, const void *target_v,
+                          const void *target_h, void *loss, void *loss_batch, void *work,
+                          void *stream);
+int32_t dl_loss_actuation_grad(int32_t dtype, int32_t B, int32_t O, int32_t T, int32_t T_written,
+                               int32_t steps, const void *act, const void *target_v,
+                               const void *target_h, double loss_grad, void *act_grad,
+                               void *stream);
+/* compute_loss_crawl (taichi_env.py:287-293): upper_height, target_c [T][B];
+ * sum over t in [1, steps] with target_c > 1 - 1e-4 of max(upper - 0.1, 0)^2
+ * / B / steps * 5.  work: B reals. */
+int32_t dl_loss_crawl(int32_t dtype, int32_t B, int32_t T, int32_t T_written, int32_t steps,
+                      const void *upper_height, const void *target_c, void *loss,
+                      void *loss_batch, void *work, void *stream);
+int32_t dl_loss_crawl_grad(int32_t dtype, int32_t B, int32_t T, int32_t T_written, int32_t steps,
+                           const void *upper_height, const void *target_c, double loss_grad,
+                           void *upper_grad, void *stream);
+
 /* Windowed COM loss of the quadruped task over stored center/vcom [T+1][B][3]
  * with per-robot targets tv [B][3], th [B]:
  *   loss = mean_k mean_{t in (T-W, T]} |vc_xz - tv_xz|^2 + (c_y - th)^2.

@@ -17,6 +17,8 @@ Reference functions restated here (file:line in the reference repo):
 * ``mlp_forward``      -- ``Model.nn1/nn2/forward`` (``multitask/nn.py:130-150``)
 * ``loss_velocity``    -- ``compute_loss_velocity`` (``taichi_env.py:185-211``)
 * ``loss_rotation``    -- ``compute_loss_rotation`` (``taichi_env.py:235-252``)
+* ``loss_height`` / ``loss_pose`` / ``loss_actuation`` / ``loss_crawl`` --
+  ``compute_loss_height/pose/actuation/crawl`` (``taichi_env.py:213-233, 276-293``)
 * ``train_targets``    -- ``initialize_train`` (``taichi_env.py:385-447``) with an
   injected random pool instead of ``ti.random``
 * ``rollout_ref``      -- ``DiffPhyTrainer.simulate(train=True)``
@@ -301,6 +303,68 @@ def loss_rotation(x, center, th, steps, run_period, turn_period):
     return total.sum(), total * B
 
 
+def loss_height(height, th, steps, jump_period):
+    """compute_loss_height (taichi_env.py:213-222): at t in [1, steps] with
+    t % jump == jump - 1 and target_h > 0.1 + 1e-6, (height - target_h)^2 /
+    B / (steps // jump) * 100.  height [T',B] (unwritten rows read zero),
+    th [T,B].  Returns (loss, loss_batch [B], max contributing height)."""
+    B = th.shape[1]
+    total = torch.zeros(B, dtype=th.dtype)
+    hmax = None
+    # target rows past the given ones (t = steps in training) read zero, as
+    # the fields' never-written rows do
+    for t in range(1, steps + 1):
+        if t % jump_period != jump_period - 1:
+            continue
+        h, tg = _at(height, t), _at(th, t)
+        on = tg > 0.1 + 1e-6
+        term = (h - tg) ** 2 / B / (steps // jump_period) * 100
+        total = total + torch.where(on, term, torch.zeros_like(term))
+        if on.any():
+            m = h[on].max().item()
+            hmax = m if hmax is None else max(hmax, m)
+    return total.sum(), total * B, hmax
+
+
+def loss_pose(x, objects, steps, jump_period):
+    """compute_loss_pose (taichi_env.py:224-233): at t in [1, steps] with
+    t % jump == 0, sum_i |x_i - initial_object_i|^2 / B / (steps // jump).
+    x [T',B,N,3] (unwritten rows read zero), objects [N,3]."""
+    B = x.shape[1]
+    total = torch.zeros(B, dtype=x.dtype)
+    for t in range(1, steps + 1):
+        if t % jump_period == 0:
+            d = _at(x, t) - objects[None]
+            total = total + (d * d).sum(-1).sum(-1) / B / (steps // jump_period)
+    return total.sum(), total * B
+
+
+def loss_actuation(act, tv, th, steps):
+    """compute_loss_actuation (taichi_env.py:276-285): at t in [0, steps)
+    with target_h < 0.1 + 1e-4, per actuation output max(|a| -
+    sqrt(|tv_x| / 0.08), 0) / n_out / B / steps * 10.  act [T',B,O]."""
+    B, O = act.shape[1], act.shape[2]
+    total = torch.zeros(B, dtype=act.dtype)
+    for t in range(steps):
+        a = _at(act, t)
+        lim = torch.sqrt(tv[t, :, 0].abs() / 0.08)
+        term = torch.clamp(a.abs() - lim[:, None], min=0.0).sum(-1) / O / B / steps * 10
+        total = total + torch.where(th[t] < 0.1 + 1e-4, term, torch.zeros_like(term))
+    return total.sum(), total * B
+
+
+def loss_crawl(upper, tc, steps):
+    """compute_loss_crawl (taichi_env.py:287-293): at t in [1, steps] with
+    target_c > 1 - 1e-4, max(upper_height - 0.1, 0)^2 / B / steps * 5."""
+    B = tc.shape[1]
+    total = torch.zeros(B, dtype=tc.dtype)
+    for t in range(1, steps + 1):
+        u = torch.clamp(_at(upper, t) - 0.1, min=0.0)
+        term = u * u / B / steps * 5
+        total = total + torch.where(_at(tc, t) > 1 - 1e-4, term, torch.zeros_like(term))
+    return total.sum(), total * B
+
+
 def loss_task_terms(center, vcom, tv, th):
     """Per-step term of the quadruped workload's loss:
     |v_com,xz - v_tgt,xz|^2 + (c_y - h_tgt)^2, per robot."""
@@ -332,27 +396,50 @@ def initial_state(robot, batch, shift=(0.4, 0.0, 0.0)):
     return x0.expand(batch, -1, -1).clone(), torch.zeros(batch, robot.n_vert, 3, dtype=F64)
 
 
-def rollout_ref(robot, cfg, weights, tv, th, tc, x0, v0, record=False):
-    """simulate(train=True) (diffphy_trainer.py:231-256) with loss_enable =
-    {velocity, rotation} (sim_quad.json:7).  Returns (loss, loss_batch, traj)."""
+def rollout_ref(robot, cfg, weights, tv, th, tc, x0, v0, record=False,
+                losses=("velocity", "rotation")):
+    """simulate(train=True) (diffphy_trainer.py:231-256); the default
+    ``losses`` are sim_quad's loss_enable = {velocity, rotation}
+    (sim_quad.json:7), any of height, pose, actuation and crawl may be added
+    (taichi_env.py:304-331).  pre_advance fills center / height / upper rows
+    0..T-2 only; rows T-1.. stay zero and the losses read them.  Returns
+    (loss, loss_batch, traj)."""
     T = cfg.steps
     W1, b1, W2, b2 = weights
     B, N, _ = x0.shape
-    xs, vs, cs, acts = [x0], [v0], [], []
+    xs, vs, cs, acts, hs, us = [x0], [v0], [], [], [], []
     x, v = x0, v0
+    height = None
     for t in range(T - 1):
-        c = (x / N).sum(1)
+        c, height, upper, _ = pre_advance(t, x, x0, (x0 / N).sum(1), height, cfg.jump_period)
         inp = nn_input(t, x, v, c, tv[t], th[t], tc[t], cfg.layout)
         _, a = mlp_forward(inp, W1, b1, W2, b2, cfg.act)
         x, v = step(x, v, a, robot, cfg.physics)
-        xs.append(x); vs.append(v); cs.append(c); acts.append(a)
+        xs.append(x); vs.append(v); cs.append(c); acts.append(a); hs.append(height); us.append(upper)
     # center[T-1] is never written (the loop stops at T-2) and reads as zero
     center = torch.stack(cs + [torch.zeros_like(cs[0])])         # [T,B,3]
     X = torch.stack(xs)                                           # [T,B,N,3]
-    lv, lvb = loss_velocity(center, tv, T, cfg.run_period, cfg.turn_period)
-    lr, lrb = loss_rotation(X, center, th, T, cfg.run_period, cfg.turn_period)
-    traj = dict(x=X, v=torch.stack(vs), center=center, act=torch.stack(acts)) if record else None
-    return lv + lr, lvb + lrb, traj
+    loss = torch.zeros((), dtype=x0.dtype)
+    lb = torch.zeros(B, dtype=x0.dtype)
+    parts = []
+    if "velocity" in losses:
+        parts.append(loss_velocity(center, tv, T, cfg.run_period, cfg.turn_period))
+    if "height" in losses:
+        parts.append(loss_height(torch.stack(hs), th, T, cfg.jump_period)[:2])
+    if "pose" in losses:
+        parts.append(loss_pose(X, x0[0], T, cfg.jump_period))
+    if "actuation" in losses:
+        parts.append(loss_actuation(torch.stack(acts), tv, th, T))
+    if "rotation" in losses:
+        parts.append(loss_rotation(X, center, th, T, cfg.run_period, cfg.turn_period))
+    if "crawl" in losses:
+        parts.append(loss_crawl(torch.stack(us), tc, T))
+    for l, b in parts:
+        loss = loss + l
+        lb = lb + b
+    traj = dict(x=X, v=torch.stack(vs), center=center, act=torch.stack(acts),
+                height=torch.stack(hs), upper=torch.stack(us)) if record else None
+    return loss, lb, traj
 
 
 def script_targets(steps, B, x0, y0, x1, y1, x2, y2, x3, y3):

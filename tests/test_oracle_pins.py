@@ -168,3 +168,64 @@ def test_mpm_task_loss_against_free_fall():
     want = task_loss_free_fall(T, W, p.dt, p.gravity, pos[:, 1].mean().item(), tv.numpy(),
                                th.numpy())
     assert abs(loss.item() - want.mean()) < 1e-9 * want.mean()
+
+
+def test_height_loss_closed_form():
+    """compute_loss_height: only t % jump == jump - 1 with target_h above
+    0.1 counts; (h - th)^2 / B / (steps // jump) * 100; rows past the
+    written ones read zero."""
+    T, B, J, steps = 13, 2, 5, 12
+    h = torch.zeros(T, B, dtype=F64)
+    h[4] = torch.tensor([0.30, 0.12], dtype=F64)
+    h[9] = torch.tensor([0.25, 0.50], dtype=F64)
+    th = torch.full((T, B), 0.2, dtype=F64)
+    th[9, 1] = 0.05                               # below 0.1: excluded
+    loss, lb, hmax = om.loss_height(h[:11], th, steps, J)
+    per = [(0.30 - 0.2) ** 2 + (0.25 - 0.2) ** 2, (0.12 - 0.2) ** 2]
+    want = [p / B / (steps // J) * 100 for p in per]
+    assert torch.allclose(lb, torch.tensor(want, dtype=F64) * B, rtol=1e-14)
+    assert abs(loss.item() - sum(want)) < 1e-15
+    assert hmax == 0.30
+
+
+def test_pose_loss_closed_form():
+    """compute_loss_pose of a body shifted by delta from its rest pose at
+    every step: N |delta|^2 per jump-period step, / B / (steps // jump)."""
+    N, B, T, J, steps = 5, 3, 11, 4, 10
+    obj = torch.rand(N, 3, dtype=F64, generator=torch.Generator().manual_seed(1))
+    delta = torch.tensor([0.01, -0.02, 0.005], dtype=F64)
+    x = (obj + delta).expand(T, B, N, 3).clone()
+    loss, lb = om.loss_pose(x, obj, steps, J)
+    n_t = len([t for t in range(1, steps + 1) if t % J == 0])           # t = 4, 8
+    want = N * float((delta ** 2).sum()) * n_t / (steps // J)
+    assert abs(loss.item() - want) < 1e-15 and torch.allclose(lb, torch.full((B,), want, dtype=F64))
+    assert om.loss_pose(obj.expand(T, B, N, 3).clone(), obj, steps, J)[0].item() == 0.0
+
+
+def test_actuation_loss_closed_form():
+    """compute_loss_actuation: a constant actuation a above the allowance
+    sqrt(|tv_x| / 0.08) costs 10 (|a| - allowance), summed over the batch's
+    mean; steps with target_h above 0.1 are free."""
+    T, B, O, steps = 8, 2, 3, 8
+    a, v = 0.7, 0.02
+    act = torch.full((T, B, O), -a, dtype=F64)
+    tv = torch.zeros(T, B, 3, dtype=F64)
+    tv[..., 0] = v
+    th = torch.zeros(T, B, dtype=F64)
+    loss, _ = om.loss_actuation(act, tv, th, steps)
+    assert abs(loss.item() - 10 * (a - math.sqrt(v / 0.08))) < 1e-14
+    th[:4] = 0.2                                   # half the steps exempt
+    assert abs(om.loss_actuation(act, tv, th, steps)[0].item()
+               - 5 * (a - math.sqrt(v / 0.08))) < 1e-14
+
+
+def test_crawl_loss_closed_form():
+    T, B, steps = 9, 2, 8
+    up = torch.full((T, B), 0.3, dtype=F64)
+    tc = torch.ones(T, B, dtype=F64)
+    tc[:, 1] = 0.0
+    loss, lb = om.loss_crawl(up, tc, steps)
+    assert abs(loss.item() - (0.3 - 0.1) ** 2 / B * 5) < 1e-15
+    assert lb[1].item() == 0.0
+    up[:] = 0.05                                   # below 0.1: free
+    assert om.loss_crawl(up, tc, steps)[0].item() == 0.0
