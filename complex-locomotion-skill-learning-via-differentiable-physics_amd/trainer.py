@@ -29,6 +29,11 @@ class DiffPhyTrainer:
         self.max_speed = cfg["process"]["max_speed"]
         self.control_length = cfg["robot"].get("control_length", 1)
         self.loss_enable = set(env.task)
+        # restarts from the previous rollout's final state after change_iter
+        # iterations (diffphy_trainer.py:186-193, 270-281)
+        self.change_iter = 5000
+        self.reset_step = 2
+        self.max_reset_step = cfg["nn"].get("max_reset_step", 4)
         self.iter = 0
         self.losses_list = []
         self.total_norm_sqr = 0.0
@@ -169,10 +174,31 @@ class DiffPhyTrainer:
             paths.append(fn)
         return paths
 
-    def run_step(self):
-        """diffphy_trainer.py:277-302 without logging: one rollout, TNS, Adam."""
+    def rounded_train(self, steps, iter, reset_step):
+        """diffphy_trainer.py:270-275: every robot starts the next rollout from
+        the previous rollout's final state, except robots start, start + step,
+        ... (start = iter % reset_step) whose positions go back to the
+        initial pose (their velocities stay copied, as in the reference).
+        The final state of a training rollout is row ``steps - 1``: the
+        reference's copy_robot(steps) reads row ``steps``, one past its
+        fields and never written by training."""
+        env = self.taichi_env
+        env.copy_robot(steps - 1)
+        start = iter % reset_step
+        times = (env.batch_size + reset_step - start - 1) // reset_step
+        env.reset_robot(start, reset_step, times)
+
+    def run_step(self, pool=None):
+        """diffphy_trainer.py:277-302 without logging: after change_iter
+        iterations the restarts of rounded_train (the reset stride growing by
+        one every 500 iterations up to max_reset_step), then one rollout, TNS,
+        Adam.  ``pool`` optionally fixes the target draw."""
+        if self.iter > self.change_iter:
+            if self.iter % 500 == 0 and self.reset_step < self.max_reset_step:
+                self.reset_step += 1
+            self.rounded_train(self.taichi_env.max_steps, self.iter, reset_step=self.reset_step)
         self.simulate(self.taichi_env.max_steps, iter=self.iter, max_height=self.max_height,
-                      max_speed=self.max_speed, loss_enable=self.loss_enable)
+                      max_speed=self.max_speed, loss_enable=self.loss_enable, pool=pool)
         self.total_norm_sqr = self.nn.get_TNS()
         self.nn.gradient_update(self.iter)
         self.losses_list.append(self.taichi_env.loss[None])

@@ -397,13 +397,16 @@ def initial_state(robot, batch, shift=(0.4, 0.0, 0.0)):
 
 
 def rollout_ref(robot, cfg, weights, tv, th, tc, x0, v0, record=False,
-                losses=("velocity", "rotation")):
+                losses=("velocity", "rotation"), control_length=1, objects=None):
     """simulate(train=True) (diffphy_trainer.py:231-256); the default
     ``losses`` are sim_quad's loss_enable = {velocity, rotation}
     (sim_quad.json:7), any of height, pose, actuation and crawl may be added
     (taichi_env.py:304-331).  pre_advance fills center / height / upper rows
-    0..T-2 only; rows T-1.. stay zero and the losses read them.  Returns
-    (loss, loss_batch, traj)."""
+    0..T-2 only; rows T-1.. stay zero and the losses read them.  With
+    control_length > 1 the network runs at t % control_length == 0 and its
+    output is held in between (diff_copy, diffphy_trainer.py:215-218,
+    249-254).  ``objects``: the pose loss's rest positions (default x0[0]).
+    Returns (loss, loss_batch, traj)."""
     T = cfg.steps
     W1, b1, W2, b2 = weights
     B, N, _ = x0.shape
@@ -413,7 +416,8 @@ def rollout_ref(robot, cfg, weights, tv, th, tc, x0, v0, record=False,
     for t in range(T - 1):
         c, height, upper, _ = pre_advance(t, x, x0, (x0 / N).sum(1), height, cfg.jump_period)
         inp = nn_input(t, x, v, c, tv[t], th[t], tc[t], cfg.layout)
-        _, a = mlp_forward(inp, W1, b1, W2, b2, cfg.act)
+        if t % control_length == 0:
+            _, a = mlp_forward(inp, W1, b1, W2, b2, cfg.act)
         x, v = step(x, v, a, robot, cfg.physics)
         xs.append(x); vs.append(v); cs.append(c); acts.append(a); hs.append(height); us.append(upper)
     # center[T-1] is never written (the loop stops at T-2) and reads as zero
@@ -427,7 +431,7 @@ def rollout_ref(robot, cfg, weights, tv, th, tc, x0, v0, record=False,
     if "height" in losses:
         parts.append(loss_height(torch.stack(hs), th, T, cfg.jump_period)[:2])
     if "pose" in losses:
-        parts.append(loss_pose(X, x0[0], T, cfg.jump_period))
+        parts.append(loss_pose(X, x0[0] if objects is None else objects, T, cfg.jump_period))
     if "actuation" in losses:
         parts.append(loss_actuation(torch.stack(acts), tv, th, T))
     if "rotation" in losses:
