@@ -47,8 +47,11 @@ class ConfigSim:
         c = self._config
         reference = c.get("preset", "reference") == "reference"
         rob, sim, proc, nn = c["robot"], c["simulator"], c["process"], c["nn"]
+        if rob.get("simulator", "mass_spring") == "mpm":
+            self._add_mpm_configs()
+            return
         if rob.get("simulator", "mass_spring") != "mass_spring" or rob.get("dim", 3) != 3:
-            raise NotImplementedError("only 3D mass-spring experiment files are supported here")
+            raise NotImplementedError("mass-spring experiment files are 3D here")
         if rob.get("design") == "quadruped":
             design = build_quadruped(rob.get("spring_stiffness", 3e4), rob.get("spring_actuation", 0.1))
         else:
@@ -87,6 +90,49 @@ class ConfigSim:
                                 nn["duplicate_h"] + nn["duplicate_c"])
         nn["adam_a"] = nn["learning_rate"]
         c.setdefault("precision", "f64" if reference else "f32")
+
+
+    def _add_mpm_configs(self):
+        """config_sim.py:80-82, 104-129 for the MPM solver: the particle
+        design (RobotDesignMPM, robot_design.py:38-110) from
+        ``robot.design_file`` (relative to the experiment file) or an inline
+        ``robot.design``; objects = particle positions, springs = their
+        actuator ids, n_springs = their count (the reference overwrites it
+        with the particle count), dt 1e-3, gravity -10, dashpot 0.2 (2D),
+        dx = 1 / n_grid, and the sensor width of taichi_env.py:149-183."""
+        from .mpm import particles_2d
+        c = self._config
+        rob, sim, proc, nn = c["robot"], c["simulator"], c["process"], c["nn"]
+        design = rob.get("design")
+        if design is None:
+            path = rob["design_file"]
+            if not os.path.isabs(path) and self._base_dir is not None:
+                cand = os.path.join(self._base_dir, os.path.basename(path))
+                path = cand if os.path.exists(cand) else path
+            with open(path) as fh:
+                design = json.load(fh)
+        dim = rob.setdefault("dim", 2)
+        if dim != 2:
+            raise NotImplementedError("MPM experiment files describe 2D particle designs")
+        pos, ids, n_act = particles_2d(design)
+        rob["objects"] = [tuple(p) for p in pos.tolist()]
+        rob["springs"] = [int(i) for i in ids]
+        rob["faces"] = []
+        rob["n_objects"] = len(pos)
+        rob["n_springs"] = len(ids)
+        rob["n_actuators"] = int(n_act)
+        proc["dt"] = 0.001
+        proc["spring_omega"] = 2 * math.pi / proc["dt"] / proc["run_period"]
+        sim["gravity"] = -10.0
+        sim["dashpot_damping"] = 0.2
+        sim["n_particles"] = rob["n_objects"]
+        sim["mpm"]["dx"] = 1 / sim["mpm"]["n_grid"]
+        sim["mpm"]["inv_dx"] = 1 / sim["mpm"]["dx"]
+        nn["n_input_states"] = (nn["n_sin_waves"] + dim * 2 * rob["n_objects"]
+                                + nn["duplicate_v"] * (dim - 1) + nn["duplicate_h"]
+                                + nn["duplicate_c"])
+        nn["adam_a"] = nn["learning_rate"]
+        c.setdefault("precision", "f64")
 
 
 def quadruped_config(batch_size=32, steps=1000, window=500, precision="f32"):

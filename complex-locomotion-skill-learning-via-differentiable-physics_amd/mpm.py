@@ -17,6 +17,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
+from .fields import record
 
 
 class MPMParamsC(Structure):
@@ -58,7 +59,8 @@ def particles_2d(design):
         for i, j in itertools.product(range(shape[0]), range(shape[1])):
             sx, sy = anchor[0] + i, anchor[1] + j
             for a, b in itertools.product(range(n_num), range(n_num)):
-                pos.append((off["x"] + sx * size + (a + 0.5) * dx, off["y"] + sy * size + (b + 0.5) * dx))
+                pos.append((off["x"] + (sx * size + (a + 0.5) * dx),
+                            off["y"] + (sy * size + (b + 0.5) * dx)))
                 ids.append(act)
     return np.array(pos), np.array(ids, np.int32), p["actuator_num"]
 
@@ -154,72 +156,229 @@ class MPMStep:
 
 
 class SolverMPM:
-    """Drop-in ``SolverMPM`` (``multitask/solver_mpm.py:5-196``) driven by a
-    particle design.  Fields: x, v [T+1,B,D,P] (SoA), C, F [T+1,B,D*D,P],
-    center [T+1,B,3], actuation [T+1,B,n_act]; grid fields per robot."""
+    """Drop-in ``SolverMPM`` (``multitask/solver_mpm.py:5-196``), built from a
+    configuration like the reference's (``ConfigSim`` with ``robot.simulator
+    = "mpm"``; ``config_sim.py:80-129`` derives the keys read here).
 
-    def __init__(self, params, positions, act_id, n_act, batch_size, max_steps,
-                 dtype=torch.float64, device="cuda"):
-        self.params = params
-        self.dim, self.batch_size, self.max_steps = params.dim, batch_size, max_steps
-        self.n_particles = len(positions)
-        self.n_act = n_act
-        self.dtype, self.device = dtype, torch.device(device)
-        self.positions = np.asarray(positions, np.float64)
-        D, P, B, T = self.dim, self.n_particles, batch_size, max_steps + 1
-        z = lambda *s: torch.zeros(*s, dtype=dtype, device=self.device)
-        self.x, self.v = z(T, B, D, P), z(T, B, D, P)
-        self.C, self.F = z(T, B, D * D, P), z(T, B, D * D, P)
-        self.center = z(T, B, 3)
-        self.actuation = z(T, B, n_act)
-        self.x_grad, self.v_grad = torch.zeros_like(self.x), torch.zeros_like(self.v)
-        self.C_grad, self.F_grad = torch.zeros_like(self.C), torch.zeros_like(self.F)
-        self.actuation_grad = torch.zeros_like(self.actuation)
-        self.step_ = MPMStep(params, B, P, n_act, act_id, dtype, device)
+    Fields (user layout as the reference's, storage structure-of-arrays):
+    ``x, v`` [T+1, B, P, D] (``Field``, storage [T+1][B][D][P]); ``C, F``
+    storage [T+1][B][D*D][P] (``C_view`` / ``F_view`` give [T+1, B, P, D, D]);
+    ``center`` [T+1, B, D]; ``actuation`` [T+1, B, n_springs] (the reference
+    sizes it by n_springs, which its config sets to the particle count);
+    ``actuator_id``, ``particle_type`` [B, P]; per-robot grids
+    ``grid_v_in`` [B, D, n^D], ``grid_m_in`` [B, n^D], ``grid_v_out``.
 
+    ``p2g(f)``, ``grid_op()``, ``g2p(f)`` launch the HIP kernels and, inside a
+    ``Tape``, record their adjoints like Taichi's kernel autodiff (reading the
+    grid as it is when the adjoint replays).  ``advance_core(t)`` is the
+    reference's ``ti.ad.grad_replaced`` step: clear grid, P2G, grid update,
+    G2P, with ``advance_core_grad(t)`` recorded in their place -- it rebuilds
+    the step's grid (clear, P2G, grid update) and runs the G2P, grid and P2G
+    adjoints, accumulating into the gradients of state t and actuation t.
+    Only solid particles (``particle_type`` 1, the reference's fill) are
+    supported."""
+
+    def __init__(self, config, device="cuda"):
+        from .fields import Field
+        cfg = config.get_config()
+        self.config = config
+        self.simulator = "mpm"
+        proc, sim, rob = cfg["process"], cfg["simulator"], cfg["robot"]
+        self.max_steps = proc["max_steps"]
+        self.dt = proc["dt"]
+        self.ground_height = sim["ground_height"]
+        self.gravity = sim["gravity"]
+        self.drag_damping = sim["drag_damping"]
+        self.dashpot_damping = sim["dashpot_damping"]
+        self.batch_size = cfg["nn"]["batch_size"]
+        self.n_objects = rob["n_objects"]
+        self.n_springs = rob["n_springs"]
+        self.springs = rob["springs"]
+        self.dim = rob["dim"]
+        self.n_particles = sim["n_particles"]
+        self.n_grid = sim["mpm"]["n_grid"]
+        self.dx = sim["mpm"]["dx"]
+        self.inv_dx = sim["mpm"]["inv_dx"]
+        self.act_strength = sim["act_strength"]
+        self.p_vol = sim["p_vol"]
+        self.E = sim["E"]
+        self.mu = sim["mu"]
+        self.la = sim["la"]
+        self.bound = sim["bound"]
+        self.height = None
+        self.rotation = None
+        self.dtype = torch.float64 if cfg.get("precision", "f64") == "f64" else torch.float32
+        self.device = torch.device(device)
+        # solid particles have mass 1 (solver_mpm.py:104-107); the active
+        # stress acts along y in 2D (:105), along x for the 3D bodies
+        self.params = MPMParamsC(dim=self.dim, n_grid=self.n_grid, bound=self.bound,
+                                 act_axis=sim.get("act_axis", 1 if self.dim == 2 else 0),
+                                 dt=self.dt, gravity=self.gravity, p_vol=self.p_vol, p_mass=1.0,
+                                 mu=self.mu, la=self.la, act_strength=self.act_strength)
+        D, P, B, T = self.dim, self.n_particles, self.batch_size, self.max_steps + 1
+        f = lambda shape, soa=False, name="": Field(shape, self.dtype, self.device, soa=soa,
+                                                    name=name)
+        self.x = f((T, B, P, D), soa=True, name="x")
+        self.v = f((T, B, P, D), soa=True, name="v")
+        self.C = f((T, B, D * D, P), name="C")
+        self.F = f((T, B, D * D, P), name="F")
+        self.center = f((T, B, D), name="center")
+        self.actuation = f((T, B, self.n_springs), name="actuation")
+        self.actuator_id = torch.zeros(B, P, dtype=torch.int32, device=self.device)
+        self.particle_type = torch.zeros(B, P, dtype=torch.int32, device=self.device)
+        G = self.n_grid ** D
+        z = lambda *sh: torch.zeros(*sh, dtype=self.dtype, device=self.device)
+        self.grid_v_in, self.grid_m_in, self.grid_v_out = z(B, D, G), z(B, G), z(B, D, G)
+        self._gvo_b, self._gv_b, self._gm_b = z(B, D, G), z(B, D, G), z(B, G)
+        self._err = torch.zeros(1, dtype=torch.int32, device=self.device)
+        self._dc = nat.dtype_code(self.dtype)
+
+    @property
+    def C_view(self):
+        T, B, _, P = self.C.data.shape
+        return self.C.data.view(T, B, self.dim, self.dim, P).permute(0, 1, 4, 2, 3)
+
+    @property
+    def F_view(self):
+        T, B, _, P = self.F.data.shape
+        return self.F.data.view(T, B, self.dim, self.dim, P).permute(0, 1, 4, 2, 3)
+
+    # ------------------------------------------------------------ set-up
     def initialize_robot(self):
-        """x[0] = design positions, actuator ids set (solver_mpm.py:54-58)."""
-        self.x[0] = _soa(self.positions, self.batch_size, self.dtype, self.device)
+        """solver_mpm.py:54-58: actuator ids from the design (the config's
+        ``springs``), every particle solid."""
+        ids = torch.as_tensor(np.asarray(self.springs, np.int32)[: self.n_particles])
+        self.actuator_id.copy_(ids[None].expand(self.batch_size, -1))
+        self.particle_type.fill_(1)
+        if int(self.actuator_id.max()) >= self.n_springs:
+            raise ValueError("actuator id beyond the actuation field's width")
 
     def clear_states(self, steps):
-        """C = 0, F = I, gradients zeroed (solver_mpm.py:60-71)."""
+        """solver_mpm.py:60-71: C = 0, F = I and the x, v, C, F gradients of the
+        first ``steps`` rows zeroed."""
         D = self.dim
-        self.C[:steps].zero_()
-        self.F[:steps].zero_()
+        self.C.data[:steps].zero_()
+        self.F.data[:steps].zero_()
         for i in range(D):
-            self.F[:steps, :, i * D + i] = 1.0
-        for g in (self.x_grad, self.v_grad, self.C_grad, self.F_grad, self.actuation_grad):
-            g.zero_()
+            self.F.data[:steps, :, i * D + i] = 1.0
+        for fld in (self.x, self.v, self.C, self.F):
+            fld.grad.data[:steps].zero_()
+
+    def clear_grid(self):
+        """solver_mpm.py:73-81 (graph-capture-safe fill kernels)."""
+        for t in (self.grid_v_in, self.grid_m_in, self.grid_v_out, self._gvo_b, self._gv_b,
+                  self._gm_b):
+            nat.call("dl_mpm_fill", self._dc, nat.ptr(t), t.numel(), 0.0, nat.stream())
+
+    def _check_types(self):
+        if int(self.particle_type.min()) != 1:
+            raise NotImplementedError("fluid particles (particle_type 0) are not supported")
+
+    # ------------------------------------------------------------ kernels
+    def _p2g(self, f, F_out):
+        P = nat.ptr
+        nat.call("dl_mpm_p2g", self._dc, ctypes.byref(self.params), self.batch_size,
+                 self.n_particles, P(self.x.at(f)), P(self.v.at(f)), P(self.C.at(f)),
+                 P(self.F.at(f)), P(self.actuation.at(f)), P(self.actuator_id), self.n_springs,
+                 P(F_out), P(self.grid_v_in), P(self.grid_m_in), P(self._err), nat.stream())
+
+    def _grid_op(self):
+        nat.call("dl_mpm_grid", self._dc, ctypes.byref(self.params), self.batch_size,
+                 nat.ptr(self.grid_v_in), nat.ptr(self.grid_m_in), nat.ptr(self.grid_v_out),
+                 nat.stream())
+
+    def _g2p(self, f):
+        P = nat.ptr
+        nat.call("dl_mpm_g2p", self._dc, ctypes.byref(self.params), self.batch_size,
+                 self.n_particles, P(self.x.at(f)), P(self.grid_v_out), P(self.x.at(f + 1)),
+                 P(self.v.at(f + 1)), P(self.C.at(f + 1)), nat.stream())
+
+    def _g2p_adj(self, f):
+        """x.grad[f] += G2P adjoint; grid_v_out adjoint accumulated."""
+        P = nat.ptr
+        xb = torch.zeros_like(self.x.at(f))
+        nat.call("dl_mpm_g2p_adj", self._dc, ctypes.byref(self.params), self.batch_size,
+                 self.n_particles, P(self.x.at(f)), P(self.grid_v_out), P(self.x.grad.at(f + 1)),
+                 P(self.v.grad.at(f + 1)), P(self.C.grad.at(f + 1)), P(xb), P(self._gvo_b),
+                 nat.stream())
+        self.x.grad.at(f).add_(xb)
+
+    def _grid_adj(self):
+        nat.call("dl_mpm_grid_adj", self._dc, ctypes.byref(self.params), self.batch_size,
+                 nat.ptr(self.grid_v_in), nat.ptr(self.grid_m_in), nat.ptr(self._gvo_b),
+                 nat.ptr(self._gv_b), nat.ptr(self._gm_b), nat.stream())
+
+    def _p2g_adj(self, f):
+        """x/v/C/F.grad[f] += P2G adjoint (from the grid adjoints and
+        F.grad[f+1]); actuation.grad[f] accumulated."""
+        P = nat.ptr
+        xb = torch.zeros_like(self.x.at(f))
+        vb, Cb, Fb = torch.empty_like(xb), torch.empty_like(self.C.at(f)), torch.empty_like(self.F.at(f))
+        nat.call("dl_mpm_p2g_adj", self._dc, ctypes.byref(self.params), self.batch_size,
+                 self.n_particles, P(self.x.at(f)), P(self.v.at(f)), P(self.C.at(f)),
+                 P(self.F.at(f)), P(self.actuation.at(f)), P(self.actuator_id), self.n_springs,
+                 P(self._gv_b), P(self._gm_b), P(self.F.grad.at(f + 1)), P(xb), P(vb), P(Cb),
+                 P(Fb), P(self.actuation.grad.at(f)), nat.stream())
+        self.x.grad.at(f).add_(xb)
+        self.v.grad.at(f).add_(vb)
+        self.C.grad.at(f).add_(Cb)
+        self.F.grad.at(f).add_(Fb)
+
+    def p2g(self, f):
+        """solver_mpm.py:83-124: writes F[f+1] and the (cleared) grid."""
+        self._check_types()
+        self._p2g(f, self.F.at(f + 1))
+        record(lambda: self._p2g_adj(f))
+
+    def grid_op(self):
+        """solver_mpm.py:126-144."""
+        self._grid_op()
+        record(self._grid_adj)
+
+    def g2p(self, f):
+        """solver_mpm.py:146-165: writes x, v, C of row f+1."""
+        self._g2p(f)
+        record(lambda: self._g2p_adj(f))
 
     def compute_center(self, t):
-        """solver_mpm.py:167-173 (3D center padded for 2D robots)."""
-        if self.dim == 3:
-            nat.call("dl_ms_center", self.step_.dc, self.batch_size, self.n_particles,
-                     nat.ptr(self.x[t]), None, nat.ptr(self.center[t]), None, nat.stream())
-        else:
-            self.center[t, :, :2] = self.x[t].mean(-1)
+        """solver_mpm.py:167-173: mean particle position per robot."""
+        with torch.no_grad():
+            self.center.data[t].copy_(self.x.at(t).mean(-1))
+
+        def adj():
+            self.x.grad.at(t).add_(self.center.grad.at(t)[..., None] / self.n_particles)
+        record(adj)
+
+    def advance_core(self, t):
+        """solver_mpm.py:175-180 (grad_replaced): the adjoint recorded for
+        the step is advance_core_grad(t)."""
+        self._check_types()
+        self.clear_grid()
+        self._p2g(t, self.F.at(t + 1))
+        self._grid_op()
+        self._g2p(t)
+        record(lambda: self.advance_core_grad(t))
+
+    def advance_core_grad(self, t):
+        """solver_mpm.py:182-190: rebuild step t's grid, then the G2P, grid
+        and P2G adjoints."""
+        self.clear_grid()
+        scratch = torch.empty_like(self.F.at(t + 1))
+        self._p2g(t, scratch)                   # F[t+1] is already in place
+        self._grid_op()
+        self._g2p_adj(t)
+        self._grid_adj()
+        self._p2g_adj(t)
 
     def pre_advance(self, s):
         self.compute_center(s)
 
     def advance(self, s):
-        """advance_core(s) (solver_mpm.py:175-180)."""
-        self.step_.forward(self.x[s], self.v[s], self.C[s], self.F[s], self.actuation[s],
-                           self.x[s + 1], self.v[s + 1], self.C[s + 1], self.F[s + 1])
+        self.advance_core(s)
 
-    def advance_grad(self, s):
-        """advance_core_grad(s) (solver_mpm.py:182-190): recompute the grid of
-        step s, then the G2P, grid and P2G adjoints; accumulates into the
-        gradient fields of step s from those of step s+1."""
-        xb = torch.zeros_like(self.x_grad[s])
-        vb, Cb, Fb = torch.zeros_like(xb), torch.zeros_like(self.C_grad[s]), torch.zeros_like(self.F_grad[s])
-        self.step_.backward(self.x[s], self.v[s], self.C[s], self.F[s], self.actuation[s],
-                            self.x_grad[s + 1], self.v_grad[s + 1], self.C_grad[s + 1],
-                            self.F_grad[s + 1], xb, vb, Cb, Fb, self.actuation_grad[s])
-        self.x_grad[s] += xb
-        self.v_grad[s] += vb
-        self.C_grad[s] += Cb
-        self.F_grad[s] += Fb
+    def check(self):
+        if int(self._err.item()) != 0:
+            raise nat.NativeError("MPM: a particle left the grid (stencil out of bounds)")
 
 
 class MPMBatchC(Structure):
@@ -649,3 +808,26 @@ class MPMBatch:
         if e:
             raise nat.NativeError(f"MPM: error flags {e:#x} (2: a scatter window wider than "
                                   f"{MPM_PB} particles)")
+
+
+class _MPMRolloutFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, batch, W1, b1, W2, b2):
+        batch.set_weights(W1, b1, W2, b2)
+        loss = batch.forward()
+        ctx.batch = batch
+        return loss.clone()
+
+    @staticmethod
+    def backward(ctx, g):
+        grads = ctx.batch.backward(loss_grad=float(g.reshape(-1)[0].item()))
+        return (None,) + tuple(x.clone() for x in grads)
+
+
+def mpm_rollout_loss(batch, W1, b1, W2, b2):
+    """The batched MPM rollout's task loss (mean over robots, shape [1]) as a
+    differentiable function of the controller weights: the forward runs
+    ``MPMBatch.forward`` (checkpoints every K steps), the backward
+    ``MPMBatch.backward`` (segment recompute + hand-written adjoints).  The
+    per-robot losses are in ``batch.loss_b``."""
+    return _MPMRolloutFn.apply(batch, W1, b1, W2, b2)

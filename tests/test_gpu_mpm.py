@@ -345,3 +345,157 @@ def test_mpm_batched_scatter_paths_and_invariants(case, scatter):
         assert t.abs().max().item() == 0.0
     assert batch.blk_flag.abs().max().item() == 0 and batch.blk_count.abs().max().item() == 0
     assert batch.blk_tile_count.abs().max().item() == 0
+
+
+def _mpm_config(B, T):
+    import json
+    import diffloco as dl
+    with open("tests/golden/sim_quad.json") as fh:
+        raw = json.load(fh)
+    raw["robot"].update(simulator="mpm", dim=2, design_file="robot_mpm_1.json")
+    raw["process"]["max_steps"] = T
+    raw["nn"]["batch_size"] = B
+    raw["precision"] = "f64"
+    return dl.ConfigSim(raw, base_dir="tests/golden")
+
+
+def test_solver_mpm_config_dropin_under_tape():
+    """SolverMPM(config) driven like the reference (pre_advance / advance per
+    step inside a Tape; advance_core records advance_core_grad): loss on the
+    final state and on every step's centre, gradients with respect to the
+    actuation of every step and the initial state against the oracle's
+    autograd (2D, robot_mpm_1, FP64)."""
+    _say("SolverMPM(config) drop-in")
+    from diffloco.fields import Field, Tape, record
+    B, T = 2, 6
+    config = _mpm_config(B, T)
+    sv = dm.SolverMPM(config)
+    assert (sv.n_particles, sv.n_springs, sv.n_grid) == (320, 320, 32)
+    sv.initialize_robot()
+    sv.clear_states(T + 1)
+    g = torch.Generator().manual_seed(61)
+    pos = torch.tensor(np.asarray(config.get_config()["robot"]["objects"])) + torch.tensor([0.2, 0.15])
+    x0 = pos[None].expand(B, -1, -1) + 0.002 * torch.randn(B, 320, 2, generator=g, dtype=torch.float64)
+    v0 = 0.2 * torch.randn(B, 320, 2, generator=g, dtype=torch.float64)
+    act = torch.rand(T, B, 4, generator=g, dtype=torch.float64) * 2 - 1
+    sv.x.view[0] = x0
+    sv.v.view[0] = v0
+    sv.actuation.data[:T, :, :4] = act.cuda()
+    Wx = torch.randn(B, 320, 2, generator=g, dtype=torch.float64)
+    Wc = torch.randn(T, B, 2, generator=g, dtype=torch.float64)
+    loss = Field((), torch.float64, "cuda", name="loss")
+    Wxd, Wcd = Wx.transpose(1, 2).contiguous().cuda(), Wc.cuda()
+    with Tape(loss):
+        for t in range(T):
+            sv.pre_advance(t)
+            sv.advance(t)
+        loss.data.copy_((sv.x.at(T) * Wxd).sum() + (sv.center.data[:T] * Wcd).sum())
+
+        def adj():
+            sv.x.grad.at(T).add_(Wxd)
+            sv.center.grad.data[:T].add_(Wcd)
+        record(adj)
+    torch.cuda.synchronize()
+    sv.check()
+    op = omp.MPMParams()
+    act_id = torch.tensor(config.get_config()["robot"]["springs"])[None].expand(B, -1)
+    xr, vr, ar = x0.clone().requires_grad_(), v0.clone().requires_grad_(), act.clone().requires_grad_()
+    x, v = xr, vr
+    C = torch.zeros(B, 320, 2, 2, dtype=torch.float64)
+    F = torch.eye(2, dtype=torch.float64).expand(B, 320, 2, 2).clone()
+    lo = torch.zeros((), dtype=torch.float64)
+    for t in range(T):
+        lo = lo + (x.mean(1) * Wc[t]).sum()
+        x, v, C, F = omp.step(x, v, C, F, ar[t], act_id, op)
+    lo = lo + (x * Wx).sum()
+    lo.backward()
+    assert abs(loss[None] - lo.item()) < 1e-10 * abs(lo.item())
+    assert rel_err(sv.actuation.grad.data[:T, :, :4].cpu(), ar.grad) < 1e-9
+    assert rel_err(sv.x.grad.view[0].cpu(), xr.grad) < 1e-9
+    assert rel_err(sv.v.grad.view[0].cpu(), vr.grad) < 1e-9
+    assert sv.actuation.grad.data[:, :, 4:].abs().max() == 0
+
+
+def test_solver_mpm_kernel_calls_record_their_adjoints():
+    """Direct clear_grid / p2g / grid_op / g2p calls inside a Tape record
+    each kernel's adjoint (Taichi's kernel autodiff): one step's gradients
+    with respect to x, v, C, F and the actuation against the oracle."""
+    _say("SolverMPM kernel-level adjoints")
+    from diffloco.fields import Field, Tape, record
+    B = 2
+    config = _mpm_config(B, 2)
+    sv = dm.SolverMPM(config)
+    sv.initialize_robot()
+    sv.clear_states(3)
+    params, op, x, v, C, F, act, act_id, n_act = _case(2, B, seed=62)
+    sv.x.view[0] = x
+    sv.v.view[0] = v
+    sv.C.data[0] = _msoa(C, torch.float64)
+    sv.F.data[0] = _msoa(F, torch.float64)
+    sv.actuation.data[0, :, :4] = act.cuda()
+    ids = torch.as_tensor(act_id.numpy(), dtype=torch.int32).cuda()
+    sv.actuator_id.copy_(ids)
+    g = torch.Generator().manual_seed(63)
+    ws = [torch.randn(B, 320, 2, generator=g, dtype=torch.float64),
+          torch.randn(B, 320, 2, generator=g, dtype=torch.float64),
+          torch.randn(B, 320, 2, 2, generator=g, dtype=torch.float64),
+          torch.randn(B, 320, 2, 2, generator=g, dtype=torch.float64)]
+    wd = [_vsoa(ws[0], torch.float64), _vsoa(ws[1], torch.float64), _msoa(ws[2], torch.float64),
+          _msoa(ws[3], torch.float64)]
+    loss = Field((), torch.float64, "cuda", name="loss")
+    with Tape(loss):
+        sv.clear_grid()
+        sv.p2g(0)
+        sv.grid_op()
+        sv.g2p(0)
+        flds = (sv.x, sv.v, sv.C, sv.F)
+        loss.data.copy_(sum((f_.at(1) * w).sum() for f_, w in zip(flds, wd)))
+
+        def adj():
+            for f_, w in zip(flds, wd):
+                f_.grad.at(1).add_(w)
+        record(adj)
+    torch.cuda.synchronize()
+    xr, vr, Cr, Fr, ar = [t.clone().requires_grad_() for t in (x, v, C, F, act)]
+    outs = omp.step(xr, vr, Cr, Fr, ar, act_id, op)
+    sum((o * w).sum() for o, w in zip(outs, ws)).backward()
+    assert rel_err(_vback(sv.x.grad.at(0)), xr.grad) < 1e-9
+    assert rel_err(_vback(sv.v.grad.at(0)), vr.grad) < 1e-9
+    assert rel_err(_mback(sv.C.grad.at(0), 2), Cr.grad) < 1e-9
+    assert rel_err(_mback(sv.F.grad.at(0), 2), Fr.grad) < 1e-9
+    assert rel_err(sv.actuation.grad.data[0, :, :4].cpu(), ar.grad) < 1e-9
+
+
+def test_mpm_rollout_autograd_function():
+    """mpm_rollout_loss: the batched MPM rollout behind torch.autograd
+    (weights in, mean task loss out) gives MPMBatch.backward's gradients and
+    matches the oracle (3D, FP64)."""
+    _say("MPM autograd.Function")
+    from oracle import mass_spring as om
+    n, B, T, K, W = 16, 2, 12, 5, 6
+    params = dm.quadruped3d_params(n_grid=n)
+    params.dt = 2e-4
+    ax = torch.arange(5, dtype=torch.float64)
+    pos = (torch.stack(torch.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3) * 0.5
+           + torch.tensor([5.0, 3.2, 5.0])) / n
+    ids = torch.tensor([(i % 5) - 1 for i in range(len(pos))])
+    mb = dm.MPMBatch(params, pos.numpy(), ids.numpy(), 4, B, T, K=K, window=W, dtype=torch.float64)
+    tv = np.array([[0.05, 0.0, -0.02], [-0.03, 0.0, 0.06]])
+    th = np.array([0.12, 0.2])
+    mb.set_targets(tv, th)
+    w64 = om.init_weights(mb.I, 64, 4, seed=64)
+    wd = [w.cuda().requires_grad_() for w in w64]
+    loss = dm.mpm_rollout_loss(mb, *wd)
+    loss.sum().backward()
+    torch.cuda.synchronize()
+    mb.check()
+    op = omp.MPMParams(dim=3, n_grid=n, dt=params.dt, gravity=params.gravity, p_vol=params.p_vol,
+                       p_mass=params.p_mass, mu=params.mu, la=params.la,
+                       act_strength=params.act_strength, act_axis=0, bound=3)
+    wr = [w.clone().requires_grad_() for w in w64]
+    lo = omp.rollout_task(pos[None].expand(B, -1, -1).clone(), ids[None].expand(B, -1), 4, op, wr,
+                          torch.tensor(tv), torch.tensor(th), T, W)
+    lo.backward()
+    assert abs(loss.item() - lo.item()) < 1e-10 * abs(lo.item())
+    for a, b in zip(wd, wr):
+        assert rel_err(a.grad.cpu(), b.grad) < 1e-7
