@@ -80,6 +80,7 @@ class Rollout(Structure):
                                         "loss_grad", "traj", "traj_grad", "err")]
 
 ROLLOUT_ERR_HORIZON, ROLLOUT_ERR_PACKING = 1, 2
+OPT_MLP_MFMA = 1
 
 
 _P = c_void_p
@@ -99,6 +100,7 @@ _SIGS = {
     "dl_nn_input_grad": ([_I, POINTER(SensorLayoutC), _I, _I, _P, _P, _P, _P, _P], c_int32),
     "dl_mlp_forward": ([_I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P], c_int32),
     "dl_mlp_backward": ([_I, _I, _I, _I, _I, _I] + [_P] * 13, c_int32),
+    "dl_set_option": ([_I, _I], c_int32),
     "dl_mlp_backward_static": ([_I, _I, _I, _I, _I, _I] + [_P] * 12, c_int32),
     "dl_ms_advance": ([_I, POINTER(MSTopology), POINTER(MSPhysics), _I, _P, _P, _P, _P, _P, _P],
                       c_int32),

@@ -352,6 +352,224 @@ __global__ void k_mlp_bwd2(int B, int I, int H, int O, const T *in, const T *wor
     }
 }
 
+
+// ------------------------------------------------------------- MLP on MFMA
+// FP32 batched MLP with the two big products on the matrix cores
+// (v_mfma_f32_16x16x4f32): a workgroup of 4 wavefronts takes a tile of 16
+// robots; wavefront w computes hidden units 16w .. 16w+15 of layer 1
+// (Z1 = X W1^T, K = I in steps of 4), and in the adjoint the input gradient
+// columns (in_b = z1b W1, K = H).  Operands come from LDS: the robots' input
+// rows and W1, both with an odd row pitch.  Layer 2 (O <= 16 outputs) runs on
+// VALU from the LDS hidden tile.
+//
+// MFMA 16x16x4 (wave64) operand layout: lane l holds A[l % 16][l / 16] and
+// B[l / 16][l % 16]; result register j of lane l is D[4 (l / 16) + j][l % 16].
+typedef float v4f __attribute__((ext_vector_type(4)));
+constexpr int MT = 16;            // robots per workgroup tile
+constexpr int MFMA_HMAX = 64, MFMA_IMAX = 256, MFMA_OMAX = 16;
+
+__device__ __forceinline__ v4f mfma16x16x4(float a, float b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// LDS: x tile [MT][IP], W1 [H][IP] (IP = I rounded up to a multiple of 4, plus
+// one: odd pitch), then the hidden tile and friends.
+__device__ __forceinline__ int mlp_pitch(int I) { return ((I + 3) & ~3) + 1; }
+
+__device__ __forceinline__ void mlp_stage(int B, int I, int H, int IP, int r0, const float *in,
+                                          const float *W1, float *xs, float *ws) {
+    const int I4 = (I + 3) & ~3;
+    for (int e = threadIdx.x; e < MT * I4; e += blockDim.x) {
+        const int m = e / I4, i = e - m * I4;
+        xs[m * IP + i] = (r0 + m < B && i < I) ? in[(size_t)(r0 + m) * I + i] : 0.f;
+    }
+    for (int e = threadIdx.x; e < H * I4; e += blockDim.x) {
+        const int n = e / I4, i = e - n * I4;
+        ws[n * IP + i] = i < I ? W1[(size_t)n * I + i] : 0.f;
+    }
+}
+
+// Z1 tile of wavefront w: D[m][n] = sum_k X[m][k] W1[16w + n][k].
+__device__ __forceinline__ v4f mlp_layer1(int I, int IP, const float *xs, const float *ws, int w,
+                                          int lane) {
+    v4f acc = {0.f, 0.f, 0.f, 0.f};
+    const int r = lane & 15, kq = lane >> 4;
+    const float *xa = xs + r * IP + kq;
+    const float *wb = ws + (16 * w + r) * IP + kq;
+    for (int k0 = 0; k0 < I; k0 += 4) acc = mfma16x16x4(xa[k0], wb[k0], acc);
+    return acc;
+}
+
+__global__ void __launch_bounds__(256) k_mlp_fwd_mfma(int B, int act, int I, int H, int O,
+                                                      const float *in, const float *W1,
+                                                      const float *b1, const float *W2,
+                                                      const float *b2, float *hid, float *out) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int IP = mlp_pitch(I);
+    float *xs = reinterpret_cast<float *>(smem_raw);
+    float *ws = xs + MT * IP;
+    float *hs = ws + H * IP;                      // [MT][H + 1]
+    const int r0 = blockIdx.x * MT, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    mlp_stage(B, I, H, IP, r0, in, W1, xs, ws);
+    __syncthreads();
+    if (16 * w < H) {
+        const v4f z = mlp_layer1(I, IP, xs, ws, w, lane);
+        const int n = 16 * w + (lane & 15);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int m = 4 * (lane >> 4) + j;
+            float y, dy;
+            activate<float>(act, z[j] + b1[n], y, dy);
+            hs[m * (H + 1) + n] = y;
+            if (r0 + m < B && hid) hid[(size_t)(r0 + m) * H + n] = y;
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < MT * O; e += blockDim.x) {
+        const int m = e / O, o = e - m * O;
+        float z = b2[o];
+        for (int n = 0; n < H; ++n) z += W2[(size_t)o * H + n] * hs[m * (H + 1) + n];
+        float y, dy;
+        activate<float>(act, z, y, dy);
+        if (r0 + m < B) out[(size_t)(r0 + m) * O + o] = y;
+    }
+}
+
+// Adjoint, stage 1 (tile of 16 robots): recompute z1 (MFMA), h, z2; z2b =
+// og act'(z2), z1b = (z2b W2) act'(z1); in_b += z1b W1 (MFMA, K = H); the
+// rows z1b, h, z2b go to work [B][2H+O] for the weight-gradient stage.
+__global__ void __launch_bounds__(256) k_mlp_bwd1_mfma(int B, int act, int I, int H, int O,
+                                                       const float *in, const float *W1,
+                                                       const float *b1, const float *W2,
+                                                       const float *b2, const float *og,
+                                                       float *ing, float *work) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int IP = mlp_pitch(I), HP = H + 1;
+    float *xs = reinterpret_cast<float *>(smem_raw);
+    float *ws = xs + MT * IP;
+    float *hs = ws + H * IP;                      // h    [MT][HP]
+    float *ds = hs + MT * HP;                     // act' [MT][HP], then z1b
+    float *zb = ds + MT * HP;                     // z2b  [MT][O]
+    const int r0 = blockIdx.x * MT, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int st = 2 * H + O;
+    mlp_stage(B, I, H, IP, r0, in, W1, xs, ws);
+    __syncthreads();
+    if (16 * w < H) {
+        const v4f z = mlp_layer1(I, IP, xs, ws, w, lane);
+        const int n = 16 * w + (lane & 15);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int m = 4 * (lane >> 4) + j;
+            float y, dy;
+            activate<float>(act, z[j] + b1[n], y, dy);
+            hs[m * HP + n] = y;
+            ds[m * HP + n] = dy;
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < MT * O; e += blockDim.x) {
+        const int m = e / O, o = e - m * O;
+        float z = b2[o];
+        for (int n = 0; n < H; ++n) z += W2[(size_t)o * H + n] * hs[m * HP + n];
+        float y, dy;
+        activate<float>(act, z, y, dy);
+        zb[m * O + o] = r0 + m < B ? og[(size_t)(r0 + m) * O + o] * dy : 0.f;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < MT * H; e += blockDim.x) {
+        const int m = e / H, n = e - m * H;
+        float hb = 0.f;
+        for (int o = 0; o < O; ++o) hb += W2[(size_t)o * H + n] * zb[m * O + o];
+        const float z1b = hb * ds[m * HP + n];
+        ds[m * HP + n] = z1b;
+        if (r0 + m < B) {
+            work[(size_t)(r0 + m) * st + n] = z1b;
+            work[(size_t)(r0 + m) * st + H + n] = hs[m * HP + n];
+        }
+    }
+    for (int e = threadIdx.x; e < MT * O; e += blockDim.x) {
+        const int m = e / O, o = e - m * O;
+        if (r0 + m < B) work[(size_t)(r0 + m) * st + 2 * H + o] = zb[m * O + o];
+    }
+    __syncthreads();
+    if (!ing) return;
+    // in_b[m][i] = sum_n z1b[m][n] W1[n][i]: A = z1b (16 x H), B = W1 (H x I)
+    const int r = lane & 15, kq = lane >> 4;
+    for (int nt = w; nt * 16 < I; nt += 4) {
+        v4f acc = {0.f, 0.f, 0.f, 0.f};
+        const int i = nt * 16 + r;
+        for (int k0 = 0; k0 < H; k0 += 4)
+            acc = mfma16x16x4(ds[r * HP + k0 + kq], ws[(k0 + kq) * IP + i], acc);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int m = 4 * kq + j;
+            if (r0 + m < B && i < I) ing[(size_t)(r0 + m) * I + i] += acc[j];
+        }
+    }
+}
+
+// Adjoint, stage 2: W1g += z1b^T X on MFMA, one 16 x 16 tile per
+// workgroup, the batch (K) split over its 4 wavefronts in contiguous
+// quarters and the partial tiles added in wavefront order (a fixed order:
+// deterministic).  Workgroups past the W1 tiles add W2g, b1g, b2g (VALU,
+// batch summed in order).
+__global__ void __launch_bounds__(256) k_mlp_bwd2_mfma(int B, int I, int H, int O, int ntI,
+                                                       const float *in, const float *work,
+                                                       float *W1g, float *b1g, float *W2g,
+                                                       float *b2g) {
+    __shared__ float part[4][256];
+    const int st = 2 * H + O;
+    const int tiles = (H / 16) * ntI;
+    if ((int)blockIdx.x < tiles) {
+        const int mt = blockIdx.x / ntI, nt = blockIdx.x - mt * ntI;
+        const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, kq = lane >> 4;
+        const int q = (((B + 3) >> 2) + 3) & ~3;  // batch rows per wavefront, multiple of 4
+        const int k_lo = w * q, k_hi = min(B, k_lo + q);
+        const int h = mt * 16 + r, i = nt * 16 + r;
+        v4f acc = {0.f, 0.f, 0.f, 0.f};
+        for (int k0 = k_lo; k0 < k_hi; k0 += 4) {
+            const int b = k0 + kq;
+            const bool ok = b < k_hi;
+            const float a = ok ? work[(size_t)b * st + h] : 0.f;
+            const float x = ok && i < I ? in[(size_t)b * I + i] : 0.f;
+            acc = mfma16x16x4(a, x, acc);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) part[w][(4 * kq + j) * 16 + r] = acc[j];
+        __syncthreads();
+        const int m = threadIdx.x >> 4, c = threadIdx.x & 15;
+        const int hh = mt * 16 + m, ii = nt * 16 + c;
+        const float v = ((part[0][threadIdx.x] + part[1][threadIdx.x]) + part[2][threadIdx.x]) +
+                        part[3][threadIdx.x];
+        if (ii < I) W1g[(size_t)hh * I + ii] += v;
+        return;
+    }
+    const int id = (blockIdx.x - tiles) * blockDim.x + threadIdx.x;
+    const int n2 = O * H;
+    if (id < n2) {
+        const int o = id / H, j = id - o * H;
+        float acc = 0.f;
+        for (int k = 0; k < B; ++k) acc += work[(size_t)k * st + 2 * H + o] * work[(size_t)k * st + H + j];
+        W2g[id] += acc;
+    } else if (id < n2 + H) {
+        const int j = id - n2;
+        float acc = 0.f;
+        for (int k = 0; k < B; ++k) acc += work[(size_t)k * st + j];
+        b1g[j] += acc;
+    } else if (id < n2 + H + O) {
+        const int o = id - n2 - H;
+        float acc = 0.f;
+        for (int k = 0; k < B; ++k) acc += work[(size_t)k * st + 2 * H + o];
+        b2g[o] += acc;
+    }
+}
+
+static inline bool mlp_mfma_ok(int I, int H, int O) {
+    return H % 16 == 0 && H <= MFMA_HMAX && I <= MFMA_IMAX && O <= MFMA_OMAX;
+}
+
+static int g_mlp_mfma = 1;        // DL_OPT_MLP_MFMA
+
 // ------------------------------------------------------------- advance
 template <typename T>
 __device__ __forceinline__ void load3(const T *a, int N, int i, T r[3]) {
@@ -834,6 +1052,23 @@ static int32_t mlp_backward_launch(int32_t dtype, int32_t act, int32_t B, int32_
                      b1g && W2g && b2g && work,
                  "dl_mlp_backward: bad arguments");
     DL_CHECK_ARG(act == DL_ACT_SIN || act == DL_ACT_TANH, "dl_mlp_backward: bad activation");
+    if (dtype == DL_F32 && g_mlp_mfma && mlp_mfma_ok(I, H, O)) {
+        const int IP = ((I + 3) & ~3) + 1;
+        const size_t sh = sizeof(float) * ((size_t)(MT + H) * IP + 2 * (size_t)MT * (H + 1) +
+                                           (size_t)MT * O);
+        DL_CHECK_HIP(ensure_dynamic_lds((const void *)k_mlp_bwd1_mfma, sh));
+        hipLaunchKernelGGL(k_mlp_bwd1_mfma, dim3((B + MT - 1) / MT), dim3(256), sh, S_(stream), B,
+                           act, I, H, O, (const float *)in, (const float *)W1, (const float *)b1,
+                           (const float *)W2, (const float *)b2, (const float *)og, (float *)ing,
+                           (float *)work);
+        const int ntI = (I + 15) / 16, tiles = (H / 16) * ntI;
+        const int rest = (O * H + H + O + 255) / 256;
+        hipLaunchKernelGGL(k_mlp_bwd2_mfma, dim3(tiles + rest), dim3(256), 0, S_(stream), B, I, H,
+                           O, ntI, (const float *)in, (const float *)work, (float *)W1g,
+                           (float *)b1g, (float *)W2g, (float *)b2g);
+        DL_CHECK_LAUNCH();
+        return 0;
+    }
     DL_DISPATCH(dtype, {
         const size_t sh = sizeof(real) * (I + 3 * H + O);
         DL_CHECK_ARG(sh <= 65536, "dl_mlp_backward: input too wide for LDS (%d)", I);
@@ -988,6 +1223,16 @@ int32_t dl_mlp_forward(int32_t dtype, int32_t act, int32_t B, int32_t I, int32_t
     DL_CHECK_ARG(B > 0 && I > 0 && H > 0 && O > 0 && in && W1 && b1 && W2 && b2 && out,
                  "dl_mlp_forward: bad arguments");
     DL_CHECK_ARG(act == DL_ACT_SIN || act == DL_ACT_TANH, "dl_mlp_forward: bad activation");
+    if (dtype == DL_F32 && g_mlp_mfma && mlp_mfma_ok(I, H, O)) {
+        const int IP = ((I + 3) & ~3) + 1;
+        const size_t sh = sizeof(float) * ((size_t)(MT + H) * IP + (size_t)MT * (H + 1));
+        DL_CHECK_HIP(ensure_dynamic_lds((const void *)k_mlp_fwd_mfma, sh));
+        hipLaunchKernelGGL(k_mlp_fwd_mfma, dim3((B + MT - 1) / MT), dim3(256), sh, S_(stream), B,
+                           act, I, H, O, (const float *)in, (const float *)W1, (const float *)b1,
+                           (const float *)W2, (const float *)b2, (float *)hid, (float *)out);
+        DL_CHECK_LAUNCH();
+        return 0;
+    }
     DL_DISPATCH(dtype, {
         const size_t sh = sizeof(real) * (I + H);
         DL_CHECK_ARG(sh <= 65536, "dl_mlp_forward: input too wide for LDS (%d)", I);
@@ -997,6 +1242,12 @@ int32_t dl_mlp_forward(int32_t dtype, int32_t act, int32_t B, int32_t I, int32_t
                            (real *)nullptr, (real *)nullptr);
     });
     DL_CHECK_LAUNCH();
+    return 0;
+}
+
+int32_t dl_set_option(int32_t option, int32_t value) {
+    DL_CHECK_ARG(option == DL_OPT_MLP_MFMA, "dl_set_option: unknown option %d", (int)option);
+    g_mlp_mfma = value != 0;
     return 0;
 }
 

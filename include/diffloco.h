@@ -126,11 +126,21 @@ int32_t dl_mlp_forward(int32_t dtype, int32_t act, int32_t B, int32_t I, int32_t
                        const void *b2, void *hidden_act, void *out_act, void *stream);
 /* Adjoint of dl_mlp_forward given out_grad; accumulates into in_grad (may be
  * NULL), W1_grad, b1_grad, W2_grad, b2_grad.  work: caller workspace of
- * B*(2H+O) reals.  Weight gradients are summed over the batch in order. */
+ * B*(2H+O) reals.  Weight gradients are summed over the batch in a fixed
+ * order (deterministic).  FP32 with H a multiple of 16, H <= 64, I <= 256
+ * and O <= 16 runs the products X W1^T, z1b W1 and z1b^T X on the matrix
+ * cores (MFMA 16x16x4 FP32 tiles of 16 robots); other shapes and FP64 run
+ * the VALU kernels. */
 int32_t dl_mlp_backward(int32_t dtype, int32_t act, int32_t B, int32_t I, int32_t H, int32_t O,
                         const void *in, const void *W1, const void *b1, const void *W2,
                         const void *b2, const void *out_grad, void *in_grad, void *W1_grad,
                         void *b1_grad, void *W2_grad, void *b2_grad, void *work, void *stream);
+
+/* Library options (process-wide).  DL_OPT_MLP_MFMA: 1 (default) runs the
+ * eligible FP32 MLP products of dl_mlp_forward / dl_mlp_backward on MFMA, 0
+ * on the VALU kernels (for comparison). */
+enum { DL_OPT_MLP_MFMA = 1 };
+int32_t dl_set_option(int32_t option, int32_t value);
 
 /* dl_mlp_backward without the caller workspace: allocates nothing (no
  * hipMalloc, hipFree or allocating library call) and uses a library-owned

@@ -138,6 +138,59 @@ def test_mlp_forward_backward(prec, act):
         assert rel_err(a.cpu(), b.grad) < gtol
 
 
+def _mlp_run(B, I, H, O, act, seed, mfma):
+    g = torch.Generator().manual_seed(seed)
+    inp = torch.randn(B, I, dtype=torch.float64, generator=g)
+    W = om.init_weights(I, H, O, seed=seed + 1)
+    W[1] = 0.1 * torch.randn(H, dtype=torch.float64, generator=g)
+    W[3] = 0.1 * torch.randn(O, dtype=torch.float64, generator=g)
+    og = torch.randn(B, O, dtype=torch.float64, generator=g)
+    dt = torch.float32
+    Wd = [w.to("cuda", dt).contiguous() for w in W]
+    ind, ogd = inp.to("cuda", dt), og.to("cuda", dt)
+    hid = torch.zeros(B, H, dtype=dt, device="cuda")
+    out = torch.zeros(B, O, dtype=dt, device="cuda")
+    Gs = [torch.zeros_like(w) for w in Wd]
+    ing = torch.full_like(ind, 0.5)                 # the input adjoint accumulates
+    work = torch.zeros(B * (2 * H + O), dtype=dt, device="cuda")
+    code = nat.ACT_SIN if act == "sin" else nat.ACT_TANH
+    nat.call("dl_set_option", nat.OPT_MLP_MFMA, int(mfma))
+    try:
+        nat.call("dl_mlp_forward", nat.F32, code, B, I, H, O, nat.ptr(ind),
+                 *[nat.ptr(w) for w in Wd], nat.ptr(hid), nat.ptr(out), nat.stream())
+        nat.call("dl_mlp_backward", nat.F32, code, B, I, H, O, nat.ptr(ind),
+                 *[nat.ptr(w) for w in Wd], nat.ptr(ogd), nat.ptr(ing),
+                 *[nat.ptr(x) for x in Gs], nat.ptr(work), nat.stream())
+        torch.cuda.synchronize()
+    finally:
+        nat.call("dl_set_option", nat.OPT_MLP_MFMA, 1)
+    Wr = [w.clone().requires_grad_() for w in W]
+    ir = inp.clone().requires_grad_()
+    eh, eo = om.mlp_forward(ir, *Wr, act=act)
+    (eo * og).sum().backward()
+    return (out, hid, ing - 0.5, Gs), (eo.detach(), eh.detach(), ir.grad, [w.grad for w in Wr])
+
+
+@pytest.mark.parametrize("B,I,H,O", [(1024, 179, 64, 4), (37, 179, 64, 4), (1, 179, 64, 4),
+                                     (300, 38, 64, 4), (70, 240, 48, 16)])
+@pytest.mark.parametrize("act", ["tanh", "sin"])
+def test_mlp_mfma_matches_fp64_and_valu(B, I, H, O, act):
+    """The MFMA path of dl_mlp_forward / dl_mlp_backward (FP32, tiles of 16
+    robots; ragged batches, inputs not a multiple of 4, 48 hidden units)
+    against torch FP64 and against the VALU kernels (DL_OPT_MLP_MFMA = 0)."""
+    _say(f"mlp mfma B={B} I={I} H={H} O={O} {act}")
+    got, ref = _mlp_run(B, I, H, O, act, 70, True)
+    valu, _ = _mlp_run(B, I, H, O, act, 70, False)
+    out, hid, ing, G = got
+    eo, eh, eig, eG = ref
+    assert rel_err(out.cpu(), eo) < 2e-6 and rel_err(hid.cpu(), eh) < 2e-6
+    assert rel_err(ing.cpu(), eig) < 2e-5
+    for a, b in zip(G, eG):
+        assert rel_err(a.cpu(), b) < 2e-5
+    for a, b in zip(got[:2] + (got[2],) + tuple(got[3]), valu[:2] + (valu[2],) + tuple(valu[3])):
+        assert rel_err(a.cpu(), b.cpu()) < 2e-5
+
+
 @pytest.mark.parametrize("layout", ["ref", "quad"])
 def test_nn_input_and_grad(layout):
     _say(f"nn_input {layout}")
