@@ -28,6 +28,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "robot-steps/s fwd+bwd batched vs plain; MPM particle-steps/s; p99 latency"
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3          # dense FP32 matrix (MFMA) peak, MI355X
 UNEVEN_DASHPOT = 0.03
 
 
@@ -58,6 +59,50 @@ def mpm_bytes(g_per_p, K, f_rec=1.0):
     return 488 + 168 * g_per_p + f_rec * (196 + 56 * g_per_p + 96 / K)
 
 
+def mlp_ab(B, I, H, O, reps=50):
+    """The batched MLP's forward + adjoint (dl_mlp_forward + dl_mlp_backward,
+    FP32, the [B x I] x [I x H] layer-1 GEMM of the per-step form) on the
+    matrix cores and on the VALU kernels, timed with HIP events on the
+    launching stream.  FLOPs per pair: 6 (I H + H O) B (forward 2, adjoint
+    4: input gradient and weight gradient)."""
+    import torch
+    from diffloco import _native as nat
+    g = torch.Generator(device="cpu").manual_seed(5)
+    mk = lambda *s_: torch.randn(*s_, generator=g).cuda()
+    inp, og = mk(B, I), mk(B, O)
+    W = [mk(H, I) / I ** 0.5, 0.1 * mk(H), mk(O, H) / H ** 0.5, 0.1 * mk(O)]
+    hid, out = torch.zeros(B, H, device="cuda"), torch.zeros(B, O, device="cuda")
+    G = [torch.zeros_like(w) for w in W]
+    ing = torch.zeros_like(inp)
+    work = torch.zeros(B * (2 * H + O), device="cuda")
+    P = nat.ptr
+    flops = 6.0 * (I * H + H * O) * B
+    res = {"shape": [B, I, H, O], "flops_per_fwd_bwd": flops}
+    st = torch.cuda.current_stream()
+    for name, on in (("mfma", 1), ("valu", 0)):
+        nat.call("dl_set_option", nat.OPT_MLP_MFMA, on)
+
+        def pair():
+            nat.call("dl_mlp_forward", nat.F32, nat.ACT_TANH, B, I, H, O, P(inp), *map(P, W),
+                     P(hid), P(out), nat.stream())
+            nat.call("dl_mlp_backward", nat.F32, nat.ACT_TANH, B, I, H, O, P(inp), *map(P, W),
+                     P(og), P(ing), *map(P, G), P(work), nat.stream())
+        for _ in range(5):
+            pair()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            pair()
+        e1.record(st)
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / reps
+        res[name] = {"us_per_fwd_bwd": us, "tflops": flops / (us * 1e-6) / 1e12,
+                     "frac_of_fp32_peak": flops / (us * 1e-6) / 1e12 / FP32_PEAK_TFLOPS}
+    nat.call("dl_set_option", nat.OPT_MLP_MFMA, 1)
+    res["mfma_speedup"] = res["valu"]["us_per_fwd_bwd"] / res["mfma"]["us_per_fwd_bwd"]
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +117,10 @@ def main():
                          "and pipelined segment recompute, 4 every step's rows kept by the "
                          "forward)")
     ap.add_argument("--plain-robots", type=int, default=2)
+    ap.add_argument("--skip-per-step", action="store_true",
+                    help="skip the per-step batched form (reference launch structure at the "
+                         "full batch: per-step kernels over all tasks, MLP GEMMs on MFMA)")
+    ap.add_argument("--skip-mlp", action="store_true", help="skip the MLP MFMA / VALU A/B")
     ap.add_argument("--cpu-robots", type=int, default=96)
     ap.add_argument("--cpu-horizon", type=int, default=1000)
     ap.add_argument("--skip-cpu", action="store_true")
@@ -245,6 +294,34 @@ def main():
         plain = {"value": pv, "unit": "robot-steps/s", "sample": f"{args.plain_robots} robots x {T} steps, "
                  "per-step kernels (center, sensors, MLP, advance) + tape adjoints, B=1 per launch",
                  "ms_per_robot_rollout": pw * 1e3 / args.plain_robots}
+
+    # ---- per-step batched form: the reference's launch structure (one launch
+    # per kernel per step over all tasks, tape replay), the per-step MLP GEMMs
+    # [R x I] x [I x H] on the matrix cores; one rollout, MFMA then VALU
+    per_step = None
+    if not args.skip_per_step:
+        from diffloco import _native as nat
+        qcb = dl.quadruped_config(batch_size=R, steps=T, window=W)
+        trb = dl.DiffPhyTrainer(qcb)
+        trb.taichi_env.setup_robot()
+        trb.nn.set_weights(*[x.float() for x in w])
+        per_step = {"robots": R, "horizon": T,
+                    "form": "per-step kernels (center, sensors, MLP, advance) over all tasks + "
+                            "tape replay of their adjoints"}
+        for name, on in (("mfma", 1), ("valu", 0)):
+            nat.call("dl_set_option", nat.OPT_MLP_MFMA, on)
+            trb.simulate_task(T, tv, th)        # warm-up
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            trb.simulate_task(T, tv, th)
+            torch.cuda.synchronize()
+            pw = time.perf_counter() - t1
+            per_step[name] = {"value": R * T / pw, "unit": "robot-steps/s",
+                              "ms_per_rollout": pw * 1e3}
+        nat.call("dl_set_option", nat.OPT_MLP_MFMA, 1)
+        del trb
+        torch.cuda.empty_cache()
+    mlp = None if args.skip_mlp else mlp_ab(R, I, H, O)
 
     # ---- MLS-MPM: 8 soft quadrupeds, each on its own 64^3 grid
     mpm = None
@@ -461,6 +538,8 @@ def main():
                                  "algorithmic_bytes_per_robot_step": f_b}},
         "plain": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
+        "per_step_batched": per_step,
+        "mlp": mlp,
         "cpu_baseline": cpu,
         "mpm": mpm,
         "uneven": uneven,

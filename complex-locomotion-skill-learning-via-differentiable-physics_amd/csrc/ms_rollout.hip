@@ -645,7 +645,6 @@ __global__ void __launch_bounds__(LB) k_rollout_bwd(RArgs<T> A) {
     if (tb <= 0) return;
     if (STORED && R.Th > K) {
         if (tid == 0) R.Wg1[0] = T(NAN);
-        if (tid == 0 && A.err) atomicOr(A.err, DL_ROLLOUT_ERR_HORIZON);
         return;
     }
     for (int i = tid; i < 3 * N; i += BT) {
@@ -1673,7 +1672,6 @@ __global__ void __launch_bounds__(256) k_wave_bwd(RArgs<T> A, int n_robots) {
     WSm<T> m = wave_carve(A, reinterpret_cast<T *>(smem_raw), wid, true);
     if (A.robots[r].spring0 != A.robots[r0].spring0 || A.robots[r].weights != A.robots[r0].weights) {
         if (lane == 0) R.Wg1[0] = T(NAN);   // robots of one workgroup must share a design
-        if (lane == 0 && A.err) atomicOr(A.err, DL_ROLLOUT_ERR_PACKING);
         return;
     }
     wave_point_at_shared(R, sh);
@@ -1738,10 +1736,8 @@ __global__ void __launch_bounds__(512) k_wave_bwd_pipe(RArgs<T> A, int n_robots)
                    (STORED && A.robots[r].horizon > A.K))) {
         // robots of one workgroup must share a design (and stored rows need
         // K >= horizon rows of scratch)
+        // (the forward launch over the same table reports it in A.err)
         if (role == 0 && lane == 0) make_rob(A, r).Wg1[0] = T(NAN);
-        if (role == 0 && lane == 0 && A.err)
-            atomicOr(A.err, STORED && A.robots[r].horizon > A.K ? DL_ROLLOUT_ERR_HORIZON
-                                                                : DL_ROLLOUT_ERR_PACKING);
         active = false;
     }
     Rob<T> R = make_rob(A, active ? r : r0);

@@ -508,59 +508,89 @@ __global__ void __launch_bounds__(256) k_mlp_bwd1_mfma(int B, int act, int I, in
     }
 }
 
-// Adjoint, stage 2: W1g += z1b^T X on MFMA, one 16 x 16 tile per
-// workgroup, the batch (K) split over its 4 wavefronts in contiguous
-// quarters and the partial tiles added in wavefront order (a fixed order:
-// deterministic).  Workgroups past the W1 tiles add W2g, b1g, b2g (VALU,
-// batch summed in order).
+// Adjoint, stage 2: every weight gradient as an MFMA product over the batch
+// (K = B), one 16 x 16 output tile per workgroup: W1g = z1b^T X (H/16 x
+// ceil(I/16) tiles), W2g = z2b^T h (H/16 tiles, rows o < O), b1g = 1^T z1b
+// (H/16 tiles, row 0) and b2g = 1^T z2b (one tile, row 0).  The batch is
+// split over the 4 wavefronts in contiguous quarters and the partial tiles
+// are added in wavefront order, so the sums are deterministic.  work rows
+// are [z1b (H) | h (H) | z2b (O)].
 __global__ void __launch_bounds__(256) k_mlp_bwd2_mfma(int B, int I, int H, int O, int ntI,
                                                        const float *in, const float *work,
                                                        float *W1g, float *b1g, float *W2g,
                                                        float *b2g) {
     __shared__ float part[4][256];
-    const int st = 2 * H + O;
-    const int tiles = (H / 16) * ntI;
-    if ((int)blockIdx.x < tiles) {
-        const int mt = blockIdx.x / ntI, nt = blockIdx.x - mt * ntI;
-        const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, kq = lane >> 4;
-        const int q = (((B + 3) >> 2) + 3) & ~3;  // batch rows per wavefront, multiple of 4
-        const int k_lo = w * q, k_hi = min(B, k_lo + q);
-        const int h = mt * 16 + r, i = nt * 16 + r;
-        v4f acc = {0.f, 0.f, 0.f, 0.f};
-        for (int k0 = k_lo; k0 < k_hi; k0 += 4) {
-            const int b = k0 + kq;
-            const bool ok = b < k_hi;
-            const float a = ok ? work[(size_t)b * st + h] : 0.f;
-            const float x = ok && i < I ? in[(size_t)b * I + i] : 0.f;
-            acc = mfma16x16x4(a, x, acc);
-        }
-#pragma unroll
-        for (int j = 0; j < 4; ++j) part[w][(4 * kq + j) * 16 + r] = acc[j];
-        __syncthreads();
-        const int m = threadIdx.x >> 4, c = threadIdx.x & 15;
-        const int hh = mt * 16 + m, ii = nt * 16 + c;
-        const float v = ((part[0][threadIdx.x] + part[1][threadIdx.x]) + part[2][threadIdx.x]) +
-                        part[3][threadIdx.x];
-        if (ii < I) W1g[(size_t)hh * I + ii] += v;
-        return;
+    const int st = 2 * H + O, nh = H / 16;
+    int t = blockIdx.x, kind, col0;
+    if (t < nh * ntI) {
+        kind = 0;
+    } else if ((t -= nh * ntI) < nh) {
+        kind = 1;
+    } else if ((t -= nh) < nh) {
+        kind = 2;
+    } else {
+        kind = 3;
+        t = 0;
     }
-    const int id = (blockIdx.x - tiles) * blockDim.x + threadIdx.x;
-    const int n2 = O * H;
-    if (id < n2) {
-        const int o = id / H, j = id - o * H;
-        float acc = 0.f;
-        for (int k = 0; k < B; ++k) acc += work[(size_t)k * st + 2 * H + o] * work[(size_t)k * st + H + j];
-        W2g[id] += acc;
-    } else if (id < n2 + H) {
-        const int j = id - n2;
-        float acc = 0.f;
-        for (int k = 0; k < B; ++k) acc += work[(size_t)k * st + j];
-        b1g[j] += acc;
-    } else if (id < n2 + H + O) {
-        const int o = id - n2 - H;
-        float acc = 0.f;
-        for (int k = 0; k < B; ++k) acc += work[(size_t)k * st + 2 * H + o];
-        b2g[o] += acc;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, kq = lane >> 4;
+    const int q = (((B + 3) >> 2) + 3) & ~3;      // batch rows per wavefront, multiple of 4
+    const int k_lo = w * q, k_hi = min(B, k_lo + q);
+    // A[m][b] and B[b][n] of this tile for lane row / column r
+    int a_off = -1, b_off = -1, b_lim = 0;
+    float a_one = 0.f;
+    if (kind == 0) {                 // A = z1b[b][16 mt + m], B = X[b][16 nt + n]
+        const int mt = t / ntI, nt = t - mt * ntI;
+        a_off = mt * 16 + r;
+        col0 = nt * 16;
+        b_lim = I;
+    } else if (kind == 1) {          // A = z2b[b][m] (m < O), B = h[b][16 t + n]
+        a_off = r < O ? 2 * H + r : -1;
+        col0 = t * 16;
+        b_lim = H;
+    } else if (kind == 2) {          // A = 1 (row 0), B = z1b[b][16 t + n]
+        a_one = r == 0 ? 1.f : 0.f;
+        col0 = t * 16;
+        b_lim = H;
+    } else {                         // A = 1 (row 0), B = z2b[b][n] (n < O)
+        a_one = r == 0 ? 1.f : 0.f;
+        col0 = 0;
+        b_lim = O;
+    }
+    const int n = col0 + r;
+    const bool n_ok = n < b_lim;
+    v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+    for (int k0 = k_lo; k0 < k_hi; k0 += 4) {
+        const int b = k0 + kq;
+        const bool ok = b < k_hi;
+        const float *row = work + (size_t)b * st;
+        float a = a_one, x = 0.f;
+        if (ok && a_off >= 0) a = row[a_off];
+        if (!ok) a = 0.f;
+        if (ok && n_ok) {
+            if (kind == 0) x = in[(size_t)b * I + n];
+            else if (kind == 1) x = row[H + n];
+            else if (kind == 2) x = row[n];
+            else x = row[2 * H + n];
+        }
+        acc = mfma16x16x4(a, x, acc);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) part[w][(4 * kq + j) * 16 + r] = acc[j];
+    __syncthreads();
+    const int m = threadIdx.x >> 4, c = threadIdx.x & 15;
+    const float v = ((part[0][threadIdx.x] + part[1][threadIdx.x]) + part[2][threadIdx.x]) +
+                    part[3][threadIdx.x];
+    const int cc = col0 + c;
+    if (kind == 0) {
+        const int mt = t / ntI;
+        if (cc < I) W1g[(size_t)(mt * 16 + m) * I + cc] += v;
+    } else if (kind == 1) {
+        if (m < O) W2g[(size_t)m * H + cc] += v;
+    } else if (kind == 2) {
+        if (m == 0) b1g[cc] += v;
+    } else if (m == 0 && cc < O) {
+        b2g[cc] += v;
     }
 }
 
@@ -1061,9 +1091,8 @@ static int32_t mlp_backward_launch(int32_t dtype, int32_t act, int32_t B, int32_
                            act, I, H, O, (const float *)in, (const float *)W1, (const float *)b1,
                            (const float *)W2, (const float *)b2, (const float *)og, (float *)ing,
                            (float *)work);
-        const int ntI = (I + 15) / 16, tiles = (H / 16) * ntI;
-        const int rest = (O * H + H + O + 255) / 256;
-        hipLaunchKernelGGL(k_mlp_bwd2_mfma, dim3(tiles + rest), dim3(256), 0, S_(stream), B, I, H,
+        const int ntI = (I + 15) / 16, tiles = (H / 16) * (ntI + 2) + 1;
+        hipLaunchKernelGGL(k_mlp_bwd2_mfma, dim3(tiles), dim3(256), 0, S_(stream), B, I, H,
                            O, ntI, (const float *)in, (const float *)work, (float *)W1g,
                            (float *)b1g, (float *)W2g, (float *)b2g);
         DL_CHECK_LAUNCH();
