@@ -356,6 +356,19 @@ def main():
                 mb.check()
                 mf.append(e0.elapsed_time(e1))
                 mbk.append(e1.elapsed_time(e2))
+            mb.seq_backward_ms = None
+            if mb.overlap_replay:
+                # the same backward with every recompute on the main stream
+                mb.overlap_replay, seq = False, []
+                for _ in range(args.mpm_steps):
+                    mb.forward()
+                    e1.record()
+                    mb.backward()
+                    e2.record()
+                    torch.cuda.synchronize()
+                    seq.append(e1.elapsed_time(e2))
+                mb.overlap_replay = True
+                mb.seq_backward_ms = float(np.mean(seq))
             return mb, mf, mbk
 
         mb, mf, mbk = mpm_run(bool(args.mpm_store_states))
@@ -374,6 +387,8 @@ def main():
                "store_states": mb.store_states, "replayed_segments": int(sum(mb.replay)),
                "resort_every": Km,
                "ms_forward": float(np.mean(mf)), "ms_backward": float(np.mean(mbk)),
+               "overlap_replay": mb.overlap_replay,
+               "ms_backward_one_stream": mb.seq_backward_ms,
                "active_nodes_per_particle": gp,
                "roofline": {"bound": "hbm", "achieved": bpp * pps / 1e9, "peak": HBM_PEAK_GBS,
                             "unit": "GB/s", "frac": bpp * pps / 1e9 / HBM_PEAK_GBS,

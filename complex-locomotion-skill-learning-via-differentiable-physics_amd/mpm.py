@@ -502,7 +502,7 @@ class MPMBatch:
                  max_speed=0.08, activation="tanh", n_hidden=64, device="cuda",
                  sort_particles=True, tile_nodes=None, scatter=SCATTER_ATOMIC, save_grid=True,
                  unroll=3, box_limit=512, resort=True, save_horizon=True, horizon_blocks=None,
-                 store_states=False):
+                 store_states=False, overlap_replay=True):
         if params.dim != 3:
             raise ValueError("MPMBatch drives 3D robots (the controller's sensors are 3D)")
         if params.n_grid % 4:
@@ -594,6 +594,17 @@ class MPMBatch:
         else:
             self.ck = _Slots(self.nck, B, D, P, dtype, dev)
             self.seg = _Slots(K + 1, B, D, P, dtype, dev)
+        # Replayed recomputes (saved grid velocities + G2P) of segment s - 1
+        # run on a side stream while the adjoint of segment s runs: a second
+        # segment buffer, and the replay's own velocity grid and active-block
+        # count (the only grid state the replay writes), so nothing it
+        # touches is shared with the adjoint in flight.
+        self.overlap_replay = bool(overlap_replay and save_horizon and not self.store_states)
+        if self.overlap_replay:
+            self.seg2 = _Slots(K + 1, B, D, P, dtype, dev)
+            self.grid_vo2 = z(B, ng, 4)
+            self.blk_count2 = zi(B)
+            self.replay_stream = torch.cuda.Stream(device=dev)
         P_ = lambda t: nat.ptr(t)
         self.desc = MPMBatchC(
             dtype=self.dc, B=B, P=P, n_act=n_act, n_groups=self.G, n_sin=n_sin,
@@ -787,9 +798,74 @@ class MPMBatch:
 
     def backward(self, loss_grad=1.0):
         self.start_backward(loss_grad)
-        for s in reversed(range(self.nck)):
-            self.backward_segment(s)
+        if self.overlap_replay and any(self.replay):
+            self._backward_overlapped()
+        else:
+            for s in reversed(range(self.nck)):
+                self.backward_segment(s)
         return self.finish_backward()
+
+    def _adjoint(self, s, seg):
+        """Adjoint of segment s over the recomputed states in ``seg``."""
+        t0, n = self._segments()[s]
+        self.desc.restore_from = 1 if self.replay[s] else 0
+        nat.call("dl_mpm_backward", ctypes.byref(self.desc), ctypes.byref(seg.c),
+                 ctypes.byref(self.adj.c), t0, n, nat.stream())
+        self.desc.restore_from = 0
+        if self.resort and s > 0:
+            pi = self.perms[s]
+            for f in self.adj.fields():
+                f[0].copy_(torch.empty_like(f[0]).scatter_(2, pi[:, None, :].expand_as(f[0]), f[0]))
+
+    def _backward_overlapped(self):
+        """Segments last to first; the replayed recompute of segment s - 1 is
+        issued on the side stream before the adjoint of segment s, into the
+        other segment buffer once the adjoint that last read that buffer
+        (segment s + 1) has finished.  Segments that could not be replayed
+        are recomputed in full on the main stream (they use the main grid)."""
+        main, rs = torch.cuda.current_stream(), self.replay_stream
+        bufs = [self.seg, self.seg2]
+        d2 = MPMBatchC.from_buffer_copy(self.desc)
+        d2.grid_vo, d2.blk_count, d2.restore_from = nat.ptr(self.grid_vo2), nat.ptr(self.blk_count2), 0
+        done_adj, done_rec = {}, {}
+        rs.wait_stream(main)
+
+        def replay(s):
+            t0, n = self._segments()[s]
+            with torch.cuda.stream(rs):
+                if s + 2 in done_adj:
+                    rs.wait_event(done_adj[s + 2])
+                buf = bufs[s % 2]
+                buf.copy_slot(0, s, self.ck)
+                nat.call("dl_mpm_forward", ctypes.byref(d2), ctypes.byref(buf.c), t0, n,
+                         MPM_REPLAY, nat.stream())
+                ev = torch.cuda.Event()
+                ev.record(rs)
+                done_rec[s] = ev
+
+        last = self.nck - 1
+        if self.replay[last]:
+            replay(last)
+        for s in range(last, -1, -1):
+            if s > 0 and self.replay[s - 1]:
+                replay(s - 1)
+            if self.resort:
+                self._set_order(s)
+            buf = bufs[s % 2]
+            if self.replay[s]:
+                main.wait_event(done_rec[s])
+            else:
+                t0, n = self._segments()[s]
+                if s + 2 in done_adj:
+                    main.wait_event(done_adj[s + 2])
+                buf.copy_slot(0, s, self.ck)
+                nat.call("dl_mpm_forward", ctypes.byref(self.desc), ctypes.byref(buf.c), t0, n,
+                         MPM_SAVE_GRID if self.save_grid else 0, nat.stream())
+            self._adjoint(s, buf)
+            ev = torch.cuda.Event()
+            ev.record(main)
+            done_adj[s] = ev
+        main.wait_stream(rs)
 
     def nbytes(self):
         """Device bytes held by this rollout's buffers."""
@@ -798,6 +874,7 @@ class MPMBatch:
         own += sum(t.numel() * t.element_size() for t in self.W)
         return own + sum(f.numel() * f.element_size()
                          for sl in (self.ring, self.ck, self.seg, self.adj,
+                                    getattr(self, "seg2", None),
                                     getattr(self, "states", None))
                          if sl is not None for f in sl.fields())
 

@@ -220,6 +220,46 @@ def test_mpm_batched_replay_matches_full_recompute(hblocks):
         assert rel_err(a.double().cpu(), b.double().cpu()) < 1e-10
 
 
+@pytest.mark.parametrize("shuffled", [False, True])
+def test_mpm_batched_overlapped_backward_matches_sequential(shuffled):
+    """The backward that replays segment s - 1 on a side stream (its own
+    segment buffer and velocity grid) while the adjoint of segment s runs
+    gives the gradients of the one-stream segment loop and of the oracle,
+    also with a segment forced to the full recompute between replayed ones
+    (the full recompute runs on the main stream and main grid)."""
+    _say(f"mpm batched overlapped backward shuffled={shuffled}")
+    if shuffled:
+        params, op, pos, ids = _batched_case(offsets=((5.0, 3.2, 5.0), (8.0, 3.2, 5.0)),
+                                             perm_seed=5)
+        T, K, W, sort = 13, 4, 6, False
+    else:
+        params, op, pos, ids = _batched_case()
+        T, K, W, sort = 24, 7, 12, True
+    batch, loss, grads, lo, gref = _run_batched(params, op, pos, ids, 2, T, K, W, torch.float64,
+                                                sort=sort)
+    assert batch.overlap_replay and batch.nck >= 3
+    assert batch.replay == [True] * batch.nck
+    for a, b in zip(grads, gref):
+        assert rel_err(a.double().cpu(), b) < 1e-7
+    runs = []
+    for overlapped in (True, False):
+        batch.forward()
+        batch.start_backward()
+        batch.replay[1] = False
+        if overlapped:
+            batch._backward_overlapped()
+        else:
+            for s in reversed(range(batch.nck)):
+                batch.backward_segment(s)
+        g = [x.clone() for x in batch.finish_backward()]
+        torch.cuda.synchronize()
+        batch.check()
+        runs.append(g)
+    for a, b, c in zip(runs[0], runs[1], gref):
+        assert rel_err(a.cpu(), b.cpu()) < 1e-12
+        assert rel_err(a.cpu(), c) < 1e-7
+
+
 @pytest.mark.parametrize("hblocks,shuffled", [(None, False), (1, False), (None, True)])
 def test_mpm_batched_stored_states_match_replay(hblocks, shuffled):
     """Every step's particle state kept by the forward, the adjoint run on
