@@ -31,6 +31,28 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3          # dense FP32 matrix (MFMA) peak, MI355X
 UNEVEN_DASHPOT = 0.03
 
+# HBM bytes moved per launch, measured with rocprofv3 counters on the default
+# workloads (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, each from its own
+# run, scaled by what the same counters read for a 256 MiB device copy:
+# FETCH_SIZE x 2.00, WRITE_SIZE x 1.00).  The commands and tables are in
+# profiles/ms_rollout_quadruped1024_kernel_stats.md and
+# profiles/mpm_quadruped64_8robots_kernel_stats.md.  Reported only when the
+# run's configuration is the measured one.
+FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
+MEASURED_KIB = {
+    # (kernel, robots, horizon, checkpoint interval): (FETCH_SIZE, WRITE_SIZE)
+    ("k_wave_bwd_pipe", 1024, 1000, 50): (3050822.0, 4575680.0),
+    ("k_wave_fwd", 1024, 1000, 50): (1250.6, 27251.4),
+}
+# MLS-MPM, 8 robots x 27 648 particles on 64^3 grids, checkpoints every 100
+# steps, replayed segments: FETCH_SIZE and WRITE_SIZE (KiB) of one step of
+# every forward, replay and adjoint kernel, summed
+MPM_STEP_KIB = {(8, 100): (63832.0, 98505.0)}
+
+
+def measured_bytes(fetch_kib, write_kib):
+    return (fetch_kib * FETCH_SCALE + write_kib * WRITE_SCALE) * 1024.0
+
 
 def algorithmic_bytes(N, S, I, H, O, K, R_d, dtype_bytes=4, stored=False):
     """Bytes per robot-step by kernel (state round-trip model): forward reads
@@ -50,21 +72,28 @@ def algorithmic_bytes(N, S, I, H, O, K, R_d, dtype_bytes=4, stored=False):
     return fwd, bwd
 
 
-def mpm_bytes(g_per_p, K, f_rec=1.0):
+def mpm_bytes(g_per_p, K, f_full=1.0, f_replay=0.0, save_horizon=False):
     """MLS-MPM bytes per particle-step, forward + adjoint (state round-trip
-    model): forward 196 + 56 G/P (G = active grid nodes), adjoint
-    292 + 112 G/P, and for the fraction f_rec of segments recomputed in the
-    backward the forward again plus the checkpoint (96 / K); stored states
-    skip the recompute (f_rec = 0)."""
-    return 488 + 168 * g_per_p + f_rec * (196 + 56 * g_per_p + 96 / K)
+    model; G = active grid nodes, 16 B per node record): forward
+    196 + 56 G/P, plus 32 G/P when it saves each step's grid (momentum/mass
+    and velocity records) for the backward; adjoint 292 + 112 G/P.  Segments
+    recomputed in the backward are charged for the work they do: the
+    fraction f_full in full (the forward again plus the checkpoint read,
+    96 / K), the fraction f_replay as the replay (saved velocities read and
+    restored, 32 G/P; G2P with the F update: x, C, F read, x, v, C, F
+    written, velocities gathered: 180 + 16 G/P; the checkpoint read)."""
+    fwd = 196 + 56 * g_per_p + (32 * g_per_p if save_horizon else 0.0)
+    return (fwd + 292 + 112 * g_per_p + f_full * (196 + 56 * g_per_p + 96 / K)
+            + f_replay * (180 + 48 * g_per_p + 96 / K))
 
 
 def mlp_ab(B, I, H, O, reps=50):
     """The batched MLP's forward + adjoint (dl_mlp_forward + dl_mlp_backward,
-    FP32, the [B x I] x [I x H] layer-1 GEMM of the per-step form) on the
-    matrix cores and on the VALU kernels, timed with HIP events on the
-    launching stream.  FLOPs per pair: 6 (I H + H O) B (forward 2, adjoint
-    4: input gradient and weight gradient)."""
+    FP32, the [B x I] x [I x H] layer-1 GEMM of the per-step form) in the
+    library's default mix (MFMA forward and weight gradients, VALU input
+    adjoint), every stage on the matrix cores, and on the VALU kernels, timed
+    with HIP events on the launching stream.  FLOPs per pair: 6 (I H + H O) B
+    (forward 2, adjoint 4: input gradient and weight gradient)."""
     import torch
     from diffloco import _native as nat
     g = torch.Generator(device="cpu").manual_seed(5)
@@ -79,7 +108,7 @@ def mlp_ab(B, I, H, O, reps=50):
     flops = 6.0 * (I * H + H * O) * B
     res = {"shape": [B, I, H, O], "flops_per_fwd_bwd": flops}
     st = torch.cuda.current_stream()
-    for name, on in (("mfma", 1), ("valu", 0)):
+    for name, on in (("mfma", 1), ("mfma_all_stages", 2), ("valu", 0)):
         nat.call("dl_set_option", nat.OPT_MLP_MFMA, on)
 
         def pair():
@@ -330,10 +359,11 @@ def main():
         Bm, Tm, Km = args.mpm_robots, args.mpm_horizon, args.mpm_ckpt
         pos, ids, n_act = dm.quadruped3d_particles(cells_per_cube=6, n_grid=64)
 
-        def mpm_run(store):
+        def mpm_run(store, save_horizon=True):
             mb = dm.MPMBatch(dm.quadruped3d_params(64), pos, ids, n_act, Bm, Tm, K=Km,
                              window=Tm // 2, dtype=torch.float32, scatter=args.mpm_scatter,
-                             tile_nodes=args.mpm_tile, store_states=store)
+                             tile_nodes=args.mpm_tile, store_states=store,
+                             save_horizon=save_horizon)
             rngm = np.random.default_rng(7)
             mb.set_targets(np.stack([rngm.uniform(-0.08, 0.08, Bm), np.zeros(Bm),
                                      rngm.uniform(-0.08, 0.08, Bm)], 1),
@@ -377,8 +407,8 @@ def main():
         ms_rollout = float(np.mean(mf) + np.mean(mbk))
         pps = Bm * P * Tm / (ms_rollout * 1e-3)
         gp = active / P
-        f_rec = 1.0 - (sum(mb.replay) / mb.nck if mb.store_states else 0.0)
-        bpp = mpm_bytes(gp, Km, f_rec)
+        f_rep = sum(mb.replay) / mb.nck
+        bpp = mpm_bytes(gp, Km, 1.0 - f_rep, 0.0 if mb.store_states else f_rep, mb.save_horizon)
         mpm = {"value": pps, "unit": "particle-steps/s", "robots": Bm, "particles": P,
                "grid": 64, "horizon": Tm, "ckpt_every": Km, "dtype": "f32",
                "scatter": ["staged run sums + per-node gather, global atomics",
@@ -392,7 +422,12 @@ def main():
                "active_nodes_per_particle": gp,
                "roofline": {"bound": "hbm", "achieved": bpp * pps / 1e9, "peak": HBM_PEAK_GBS,
                             "unit": "GB/s", "frac": bpp * pps / 1e9 / HBM_PEAK_GBS,
-                            "algorithmic_bytes_per_particle_step": bpp, "traffic": None},
+                            "algorithmic_bytes_per_particle_step": bpp,
+                            "traffic": (measured_bytes(*MPM_STEP_KIB[(Bm, Km)]) / (Bm * P)
+                                        if (Bm, Km) in MPM_STEP_KIB and P == 27648
+                                        and not mb.store_states and mb.overlap_replay
+                                        else None),
+                            "traffic_unit": "bytes per particle-step, all MPM kernels"},
                "mean_loss": float(mb.loss.item()), "device_gb": mb.nbytes() / 1e9}
         if not args.skip_alt:
             # the other memory form: every step's particle state stored (the
@@ -408,6 +443,19 @@ def main():
                 "replayed_segments": int(sum(mc.replay)),
                 "mean_loss": float(mc.loss.item()), "device_gb": mc.nbytes() / 1e9}
             del mc
+            torch.cuda.empty_cache()
+            # checkpoints only (no per-step grid save): every segment
+            # recomputed in full in the backward, memory O(T / K)
+            mr, rf, rb = mpm_run(False, save_horizon=False)
+            rpps = Bm * P * Tm / ((np.mean(rf) + np.mean(rb)) * 1e-3)
+            rbpp = mpm_bytes(gp, Km, 1.0, 0.0, False)
+            mpm["recompute"] = {
+                "value": rpps, "store_states": False, "save_horizon": False, "ckpt_every": Km,
+                "ms_forward": float(np.mean(rf)), "ms_backward": float(np.mean(rb)),
+                "algorithmic_bytes_per_particle_step": rbpp,
+                "frac": rbpp * rpps / 1e9 / HBM_PEAK_GBS,
+                "mean_loss": float(mr.loss.item()), "device_gb": mr.nbytes() / 1e9}
+            del mr
             torch.cuda.empty_cache()
 
     # ---- uneven batch: 512 seeded random voxel robots, size classes, HIP graphs
@@ -532,6 +580,10 @@ def main():
                "kind": "port", "sample": f"FP64 torch-CPU oracle, {Bc} robots x {Tc} steps, "
                "forward + autograd backward (no checkpointing)"}
 
+    key = (R, T, batch.K) if batch.variant == 3 and not stored else None
+    mk = {k[0]: v for k, v in MEASURED_KIB.items() if key and k[1:] == key}
+    traffic_bwd = measured_bytes(*mk[kname_bwd]) if kname_bwd in mk else None
+    traffic_fwd = measured_bytes(*mk[kname + "_fwd"]) if kname + "_fwd" in mk else None
     out = {
         "metric": METRIC, "value": value, "unit": "robot-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
@@ -545,11 +597,12 @@ def main():
                    "n_vert": N, "n_spring": S, "mlp": [I, H, O], "block": args.block,
                    "form": "batched persistent (one launch forward, one launch adjoint)"},
         "roofline": {"bound": "hbm", "achieved": achieved_bwd, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved_bwd / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved_bwd / HBM_PEAK_GBS, "traffic": traffic_bwd,
+                     "algorithmic_bytes_per_launch": b_b * robot_steps,
                      "kernel": kname_bwd, "algorithmic_bytes_per_robot_step": b_b,
                      "avg_launch_ms": bwd_avg_ms,
                      "forward": {"kernel": kname + "_fwd", "achieved": achieved_fwd,
-                                 "avg_launch_ms": fwd_avg_ms,
+                                 "avg_launch_ms": fwd_avg_ms, "traffic": traffic_fwd,
                                  "algorithmic_bytes_per_robot_step": f_b}},
         "plain": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,

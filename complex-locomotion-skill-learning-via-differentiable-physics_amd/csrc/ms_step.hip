@@ -358,9 +358,11 @@ __global__ void k_mlp_bwd2(int B, int I, int H, int O, const T *in, const T *wor
 // (v_mfma_f32_16x16x4f32): a workgroup of 4 wavefronts takes a tile of 16
 // robots; wavefront w computes hidden units 16w .. 16w+15 of layer 1
 // (Z1 = X W1^T, K = I in steps of 4), and in the adjoint the input gradient
-// columns (in_b = z1b W1, K = H).  Operands come from LDS: the robots' input
-// rows and W1, both with an odd row pitch.  Layer 2 (O <= 16 outputs) runs on
-// VALU from the LDS hidden tile.
+// columns (in_b = z1b W1, K = H).  The robots' input rows are staged in LDS
+// (odd row pitch); W1 operands are read straight from global memory (it is
+// small and stays in L2: staging all of it per workgroup cost more than the
+// products).  Layer 2 (O <= 16 outputs) runs on VALU from the LDS hidden
+// tile, four lanes per output.
 //
 // MFMA 16x16x4 (wave64) operand layout: lane l holds A[l % 16][l / 16] and
 // B[l / 16][l % 16]; result register j of lane l is D[4 (l / 16) + j][l % 16].
@@ -372,32 +374,66 @@ __device__ __forceinline__ v4f mfma16x16x4(float a, float b, v4f c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-// LDS: x tile [MT][IP], W1 [H][IP] (IP = I rounded up to a multiple of 4, plus
-// one: odd pitch), then the hidden tile and friends.
+// LDS: x tile [MT][IP] (IP = I rounded up to a multiple of 4, plus one: odd
+// pitch; the columns past I are zero), then the hidden tile and friends.
 __device__ __forceinline__ int mlp_pitch(int I) { return ((I + 3) & ~3) + 1; }
 
-__device__ __forceinline__ void mlp_stage(int B, int I, int H, int IP, int r0, const float *in,
-                                          const float *W1, float *xs, float *ws) {
-    const int I4 = (I + 3) & ~3;
-    for (int e = threadIdx.x; e < MT * I4; e += blockDim.x) {
-        const int m = e / I4, i = e - m * I4;
-        xs[m * IP + i] = (r0 + m < B && i < I) ? in[(size_t)(r0 + m) * I + i] : 0.f;
-    }
-    for (int e = threadIdx.x; e < H * I4; e += blockDim.x) {
-        const int n = e / I4, i = e - n * I4;
-        ws[n * IP + i] = i < I ? W1[(size_t)n * I + i] : 0.f;
-    }
+// Wavefront w stages rows w, w + 4, .. of the tile (blockDim 256), all its
+// loads issued before the first LDS store.
+__device__ __forceinline__ void mlp_stage(int B, int I, int IP, int r0, const float *in,
+                                          float *xs) {
+    constexpr int RW = MT / 4, CW = MFMA_IMAX / 64;
+    const int I4 = (I + 3) & ~3, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float v[RW][CW];
+#pragma unroll
+    for (int a = 0; a < RW; ++a)
+#pragma unroll
+        for (int c = 0; c < CW; ++c) {
+            const int m = w + 4 * a, i = lane + 64 * c;
+            v[a][c] = (r0 + m < B && i < I) ? in[(size_t)(r0 + m) * I + i] : 0.f;
+        }
+#pragma unroll
+    for (int a = 0; a < RW; ++a)
+#pragma unroll
+        for (int c = 0; c < CW; ++c) {
+            const int i = lane + 64 * c;
+            if (i < I4) xs[(w + 4 * a) * IP + i] = v[a][c];
+        }
 }
 
-// Z1 tile of wavefront w: D[m][n] = sum_k X[m][k] W1[16w + n][k].
-__device__ __forceinline__ v4f mlp_layer1(int I, int IP, const float *xs, const float *ws, int w,
+// Z1 tile of wavefront w: D[m][n] = sum_k X[m][k] W1[16w + n][k], W1 rows
+// from global memory (16 w + 15 < H).  Every W1 operand of the tile is
+// loaded before the first product (one round trip to L2 instead of one per
+// few k steps); the k-step guards are uniform across the wavefront.
+__device__ __forceinline__ v4f mlp_layer1(int I, int IP, const float *xs, const float *W1, int w,
                                           int lane) {
+    constexpr int KS = MFMA_IMAX / 4;
     v4f acc = {0.f, 0.f, 0.f, 0.f};
     const int r = lane & 15, kq = lane >> 4;
     const float *xa = xs + r * IP + kq;
-    const float *wb = ws + (16 * w + r) * IP + kq;
-    for (int k0 = 0; k0 < I; k0 += 4) acc = mfma16x16x4(xa[k0], wb[k0], acc);
+    const float *wb = W1 + (size_t)(16 * w + r) * I + kq;
+    float wv[KS];
+#pragma unroll
+    for (int k = 0; k < KS; ++k) wv[k] = 4 * k + kq < I ? wb[4 * k] : 0.f;
+#pragma unroll
+    for (int k = 0; k < KS; ++k)
+        if (4 * k < I) acc = mfma16x16x4(xa[4 * k], wv[k], acc);
     return acc;
+}
+
+// Layer 2 of the tile from the LDS hidden rows hs [MT][HP]: four lanes per
+// (robot, output) pair, each summing a quarter of the hidden units, combined
+// by lane shuffles; returns z2 (valid in every lane of the four) for the pair
+// e = (thread index) / 4 + 64 round.
+__device__ __forceinline__ float mlp_layer2_part(int H, int HP, int O, const float *W2,
+                                                 const float *b2, const float *hs, int e) {
+    const int m = e / O, o = e - m * O, q = threadIdx.x & 3, hq = H >> 2;
+    float z = 0.f;
+    if (m < MT)
+        for (int n = q * hq; n < (q + 1) * hq; ++n) z += W2[(size_t)o * H + n] * hs[m * HP + n];
+    z += __shfl_xor(z, 1);
+    z += __shfl_xor(z, 2);
+    return z + (m < MT ? b2[o] : 0.f);
 }
 
 __global__ void __launch_bounds__(256) k_mlp_fwd_mfma(int B, int act, int I, int H, int O,
@@ -407,13 +443,12 @@ __global__ void __launch_bounds__(256) k_mlp_fwd_mfma(int B, int act, int I, int
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int IP = mlp_pitch(I);
     float *xs = reinterpret_cast<float *>(smem_raw);
-    float *ws = xs + MT * IP;
-    float *hs = ws + H * IP;                      // [MT][H + 1]
+    float *hs = xs + MT * IP;                     // [MT][H + 1]
     const int r0 = blockIdx.x * MT, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    mlp_stage(B, I, H, IP, r0, in, W1, xs, ws);
+    mlp_stage(B, I, IP, r0, in, xs);
     __syncthreads();
     if (16 * w < H) {
-        const v4f z = mlp_layer1(I, IP, xs, ws, w, lane);
+        const v4f z = mlp_layer1(I, IP, xs, W1, w, lane);
         const int n = 16 * w + (lane & 15);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -425,13 +460,13 @@ __global__ void __launch_bounds__(256) k_mlp_fwd_mfma(int B, int act, int I, int
         }
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < MT * O; e += blockDim.x) {
+    for (int e0 = 0; e0 < MT * O; e0 += blockDim.x >> 2) {
+        const int e = e0 + (threadIdx.x >> 2);
+        const float z = mlp_layer2_part(H, H + 1, O, W2, b2, hs, e);
         const int m = e / O, o = e - m * O;
-        float z = b2[o];
-        for (int n = 0; n < H; ++n) z += W2[(size_t)o * H + n] * hs[m * (H + 1) + n];
         float y, dy;
         activate<float>(act, z, y, dy);
-        if (r0 + m < B) out[(size_t)(r0 + m) * O + o] = y;
+        if ((threadIdx.x & 3) == 0 && m < MT && r0 + m < B) out[(size_t)(r0 + m) * O + o] = y;
     }
 }
 
@@ -446,16 +481,15 @@ __global__ void __launch_bounds__(256) k_mlp_bwd1_mfma(int B, int act, int I, in
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int IP = mlp_pitch(I), HP = H + 1;
     float *xs = reinterpret_cast<float *>(smem_raw);
-    float *ws = xs + MT * IP;
-    float *hs = ws + H * IP;                      // h    [MT][HP]
+    float *hs = xs + MT * IP;                     // h    [MT][HP]
     float *ds = hs + MT * HP;                     // act' [MT][HP], then z1b
     float *zb = ds + MT * HP;                     // z2b  [MT][O]
     const int r0 = blockIdx.x * MT, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int st = 2 * H + O;
-    mlp_stage(B, I, H, IP, r0, in, W1, xs, ws);
+    mlp_stage(B, I, IP, r0, in, xs);
     __syncthreads();
     if (16 * w < H) {
-        const v4f z = mlp_layer1(I, IP, xs, ws, w, lane);
+        const v4f z = mlp_layer1(I, IP, xs, W1, w, lane);
         const int n = 16 * w + (lane & 15);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -467,24 +501,43 @@ __global__ void __launch_bounds__(256) k_mlp_bwd1_mfma(int B, int act, int I, in
         }
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < MT * O; e += blockDim.x) {
+    for (int e0 = 0; e0 < MT * O; e0 += blockDim.x >> 2) {
+        const int e = e0 + (threadIdx.x >> 2);
+        const float z = mlp_layer2_part(H, HP, O, W2, b2, hs, e);
         const int m = e / O, o = e - m * O;
-        float z = b2[o];
-        for (int n = 0; n < H; ++n) z += W2[(size_t)o * H + n] * hs[m * HP + n];
         float y, dy;
         activate<float>(act, z, y, dy);
-        zb[m * O + o] = r0 + m < B ? og[(size_t)(r0 + m) * O + o] * dy : 0.f;
+        if ((threadIdx.x & 3) == 0 && m < MT)
+            zb[m * O + o] = r0 + m < B ? og[(size_t)(r0 + m) * O + o] * dy : 0.f;
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < MT * H; e += blockDim.x) {
-        const int m = e / H, n = e - m * H;
-        float hb = 0.f;
-        for (int o = 0; o < O; ++o) hb += W2[(size_t)o * H + n] * zb[m * O + o];
-        const float z1b = hb * ds[m * HP + n];
-        ds[m * HP + n] = z1b;
-        if (r0 + m < B) {
-            work[(size_t)(r0 + m) * st + n] = z1b;
-            work[(size_t)(r0 + m) * st + H + n] = hs[m * HP + n];
+    {
+        // (robot, hidden) pairs e = threadIdx.x + 256 c; W2 operands first
+        constexpr int NC = MT * MFMA_HMAX / 256;
+        float w2v[NC][MFMA_OMAX];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int e = threadIdx.x + 256 * c, n = e % H;
+#pragma unroll
+            for (int o = 0; o < MFMA_OMAX; ++o)
+                w2v[c][o] = o < O && e < MT * H ? W2[(size_t)o * H + n] : 0.f;
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int e = threadIdx.x + 256 * c;
+            if (e < MT * H) {
+                const int m = e / H, n = e - m * H;
+                float hb = 0.f;
+#pragma unroll
+                for (int o = 0; o < MFMA_OMAX; ++o)
+                    if (o < O) hb += w2v[c][o] * zb[m * O + o];
+                const float z1b = hb * ds[m * HP + n];
+                ds[m * HP + n] = z1b;
+                if (r0 + m < B) {
+                    work[(size_t)(r0 + m) * st + n] = z1b;
+                    work[(size_t)(r0 + m) * st + H + n] = hs[m * HP + n];
+                }
+            }
         }
     }
     for (int e = threadIdx.x; e < MT * O; e += blockDim.x) {
@@ -494,16 +547,37 @@ __global__ void __launch_bounds__(256) k_mlp_bwd1_mfma(int B, int act, int I, in
     __syncthreads();
     if (!ing) return;
     // in_b[m][i] = sum_n z1b[m][n] W1[n][i]: A = z1b (16 x H), B = W1 (H x I)
+    // column tiles nt = w, w + 4, .. (at most MFMA_IMAX / 64 per wavefront);
+    // the W1 operands and the in_b values to add to of every tile are loaded
+    // before the first product
     const int r = lane & 15, kq = lane >> 4;
-    for (int nt = w; nt * 16 < I; nt += 4) {
-        v4f acc = {0.f, 0.f, 0.f, 0.f};
-        const int i = nt * 16 + r;
-        for (int k0 = 0; k0 < H; k0 += 4)
-            acc = mfma16x16x4(ds[r * HP + k0 + kq], ws[(k0 + kq) * IP + i], acc);
+    constexpr int KS = MFMA_HMAX / 4, NT = MFMA_IMAX / 64;
+    float wv[NT][KS], gv[NT][4];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int i = (w + 4 * t) * 16 + r;
+        const bool iok = i < I;
+        const float *wc = W1 + (size_t)kq * I + (iok ? i : 0);
+#pragma unroll
+        for (int k = 0; k < KS; ++k) wv[t][k] = 4 * k < H && iok ? wc[(size_t)4 * k * I] : 0.f;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int m = 4 * kq + j;
-            if (r0 + m < B && i < I) ing[(size_t)(r0 + m) * I + i] += acc[j];
+            gv[t][j] = r0 + m < B && iok ? ing[(size_t)(r0 + m) * I + i] : 0.f;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int i = (w + 4 * t) * 16 + r;
+        if ((w + 4 * t) * 16 >= I) break;
+        v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k = 0; k < KS; ++k)
+            if (4 * k < H) acc = mfma16x16x4(ds[r * HP + 4 * k + kq], wv[t][k], acc);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int m = 4 * kq + j;
+            if (r0 + m < B && i < I) ing[(size_t)(r0 + m) * I + i] = gv[t][j] + acc[j];
         }
     }
 }
@@ -598,7 +672,11 @@ static inline bool mlp_mfma_ok(int I, int H, int O) {
     return H % 16 == 0 && H <= MFMA_HMAX && I <= MFMA_IMAX && O <= MFMA_OMAX;
 }
 
-static int g_mlp_mfma = 1;        // DL_OPT_MLP_MFMA
+// DL_OPT_MLP_MFMA: 0 VALU kernels only; 1 the faster kernel of each stage by
+// measurement at 1024 x 179 x 64 x 4 (MFMA forward and weight-gradient stage,
+// VALU input-adjoint stage: one workgroup per robot keeps more of the chip
+// busy than the 16-robot MFMA tiles there); 2 every stage on MFMA.
+static int g_mlp_mfma = 1;
 
 // ------------------------------------------------------------- advance
 template <typename T>
@@ -1083,14 +1161,22 @@ static int32_t mlp_backward_launch(int32_t dtype, int32_t act, int32_t B, int32_
                  "dl_mlp_backward: bad arguments");
     DL_CHECK_ARG(act == DL_ACT_SIN || act == DL_ACT_TANH, "dl_mlp_backward: bad activation");
     if (dtype == DL_F32 && g_mlp_mfma && mlp_mfma_ok(I, H, O)) {
-        const int IP = ((I + 3) & ~3) + 1;
-        const size_t sh = sizeof(float) * ((size_t)(MT + H) * IP + 2 * (size_t)MT * (H + 1) +
-                                           (size_t)MT * O);
-        DL_CHECK_HIP(ensure_dynamic_lds((const void *)k_mlp_bwd1_mfma, sh));
-        hipLaunchKernelGGL(k_mlp_bwd1_mfma, dim3((B + MT - 1) / MT), dim3(256), sh, S_(stream), B,
-                           act, I, H, O, (const float *)in, (const float *)W1, (const float *)b1,
-                           (const float *)W2, (const float *)b2, (const float *)og, (float *)ing,
-                           (float *)work);
+        if (g_mlp_mfma == 2) {
+            const int IP = ((I + 3) & ~3) + 1;
+            const size_t sh = sizeof(float) * ((size_t)MT * IP + 2 * (size_t)MT * (H + 1) +
+                                               (size_t)MT * O);
+            DL_CHECK_HIP(ensure_dynamic_lds((const void *)k_mlp_bwd1_mfma, sh));
+            hipLaunchKernelGGL(k_mlp_bwd1_mfma, dim3((B + MT - 1) / MT), dim3(256), sh,
+                               S_(stream), B, act, I, H, O, (const float *)in, (const float *)W1,
+                               (const float *)b1, (const float *)W2, (const float *)b2,
+                               (const float *)og, (float *)ing, (float *)work);
+        } else {
+            const size_t sh = sizeof(float) * (I + 3 * H + O);
+            hipLaunchKernelGGL(k_mlp_bwd1<float>, dim3(B), dim3(256), sh, S_(stream), act, I, H,
+                               O, (const float *)in, (const float *)W1, (const float *)b1,
+                               (const float *)W2, (const float *)b2, (const float *)og,
+                               (float *)ing, (float *)work);
+        }
         const int ntI = (I + 15) / 16, tiles = (H / 16) * (ntI + 2) + 1;
         hipLaunchKernelGGL(k_mlp_bwd2_mfma, dim3(tiles), dim3(256), 0, S_(stream), B, I, H,
                            O, ntI, (const float *)in, (const float *)work, (float *)W1g,
@@ -1254,7 +1340,7 @@ int32_t dl_mlp_forward(int32_t dtype, int32_t act, int32_t B, int32_t I, int32_t
     DL_CHECK_ARG(act == DL_ACT_SIN || act == DL_ACT_TANH, "dl_mlp_forward: bad activation");
     if (dtype == DL_F32 && g_mlp_mfma && mlp_mfma_ok(I, H, O)) {
         const int IP = ((I + 3) & ~3) + 1;
-        const size_t sh = sizeof(float) * ((size_t)(MT + H) * IP + (size_t)MT * (H + 1));
+        const size_t sh = sizeof(float) * ((size_t)MT * IP + (size_t)MT * (H + 1));
         DL_CHECK_HIP(ensure_dynamic_lds((const void *)k_mlp_fwd_mfma, sh));
         hipLaunchKernelGGL(k_mlp_fwd_mfma, dim3((B + MT - 1) / MT), dim3(256), sh, S_(stream), B,
                            act, I, H, O, (const float *)in, (const float *)W1, (const float *)b1,
@@ -1276,7 +1362,8 @@ int32_t dl_mlp_forward(int32_t dtype, int32_t act, int32_t B, int32_t I, int32_t
 
 int32_t dl_set_option(int32_t option, int32_t value) {
     DL_CHECK_ARG(option == DL_OPT_MLP_MFMA, "dl_set_option: unknown option %d", (int)option);
-    g_mlp_mfma = value != 0;
+    DL_CHECK_ARG(value >= 0 && value <= 2, "dl_set_option: DL_OPT_MLP_MFMA takes 0, 1 or 2");
+    g_mlp_mfma = value;
     return 0;
 }
 

@@ -136,9 +136,10 @@ int32_t dl_mlp_backward(int32_t dtype, int32_t act, int32_t B, int32_t I, int32_
                         const void *b2, const void *out_grad, void *in_grad, void *W1_grad,
                         void *b1_grad, void *W2_grad, void *b2_grad, void *work, void *stream);
 
-/* Library options (process-wide).  DL_OPT_MLP_MFMA: 1 (default) runs the
- * eligible FP32 MLP products of dl_mlp_forward / dl_mlp_backward on MFMA, 0
- * on the VALU kernels (for comparison). */
+/* Library options (process-wide).  DL_OPT_MLP_MFMA for the eligible FP32
+ * calls of dl_mlp_forward / dl_mlp_backward: 1 (default) the faster kernel
+ * of each stage (forward and weight gradients on MFMA, the input adjoint on
+ * VALU), 2 every stage on MFMA, 0 the VALU kernels only (for comparison). */
 enum { DL_OPT_MLP_MFMA = 1 };
 int32_t dl_set_option(int32_t option, int32_t value);
 

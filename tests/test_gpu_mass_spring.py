@@ -174,13 +174,16 @@ def _mlp_run(B, I, H, O, act, seed, mfma):
 @pytest.mark.parametrize("B,I,H,O", [(1024, 179, 64, 4), (37, 179, 64, 4), (1, 179, 64, 4),
                                      (300, 38, 64, 4), (70, 240, 48, 16)])
 @pytest.mark.parametrize("act", ["tanh", "sin"])
-def test_mlp_mfma_matches_fp64_and_valu(B, I, H, O, act):
-    """The MFMA path of dl_mlp_forward / dl_mlp_backward (FP32, tiles of 16
-    robots; ragged batches, inputs not a multiple of 4, 48 hidden units)
-    against torch FP64 and against the VALU kernels (DL_OPT_MLP_MFMA = 0)."""
-    _say(f"mlp mfma B={B} I={I} H={H} O={O} {act}")
-    got, ref = _mlp_run(B, I, H, O, act, 70, True)
-    valu, _ = _mlp_run(B, I, H, O, act, 70, False)
+@pytest.mark.parametrize("mode", [1, 2])
+def test_mlp_mfma_matches_fp64_and_valu(B, I, H, O, act, mode):
+    """The MFMA paths of dl_mlp_forward / dl_mlp_backward (FP32, tiles of 16
+    robots; ragged batches, inputs not a multiple of 4, 48 hidden units):
+    DL_OPT_MLP_MFMA = 1 (MFMA forward and weight gradients, VALU input
+    adjoint) and 2 (every stage on MFMA), against torch FP64 and against the
+    VALU kernels (DL_OPT_MLP_MFMA = 0)."""
+    _say(f"mlp mfma B={B} I={I} H={H} O={O} {act} mode={mode}")
+    got, ref = _mlp_run(B, I, H, O, act, 70, mode)
+    valu, _ = _mlp_run(B, I, H, O, act, 70, 0)
     out, hid, ing, G = got
     eo, eh, eig, eG = ref
     assert rel_err(out.cpu(), eo) < 2e-6 and rel_err(hid.cpu(), eh) < 2e-6
