@@ -12,8 +12,10 @@ Printed as ONE JSON line (rank 0):
   plain            one-robot-per-launch form (reference launch structure,
                    per-step kernels + tape replay) on a sample of robots
   roofline         HBM roofline of the dominant kernel (adjoint rollout)
-  cpu_baseline     FP64 CPU restatement (oracle), forward+backward, on a
-                   bounded sample of the same workload
+  cpu_baseline     C/OpenMP restatement (oracle/cpu_rollout.c), FP32, 32
+                   tasks x 1000 steps with the same checkpointing and one
+                   Adam step, all cores and one thread; the FP64 torch
+                   oracle's figure beside it
 """
 
 import argparse
@@ -150,8 +152,10 @@ def main():
                     help="skip the per-step batched form (reference launch structure at the "
                          "full batch: per-step kernels over all tasks, MLP GEMMs on MFMA)")
     ap.add_argument("--skip-mlp", action="store_true", help="skip the MLP MFMA / VALU A/B")
-    ap.add_argument("--cpu-robots", type=int, default=96)
-    ap.add_argument("--cpu-horizon", type=int, default=1000)
+    ap.add_argument("--cpu-robots", type=int, default=32,
+                    help="robots of the FP64 torch oracle's CPU figure")
+    ap.add_argument("--cpu-tasks", type=int, default=32,
+                    help="tasks of the C/OpenMP FP32 CPU baseline")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-mpm", action="store_true")
@@ -561,24 +565,55 @@ def main():
         if not args.skip_serial:
             serve["one_at_a_time"] = serve_run(1)
 
-    # ---- CPU baseline: FP64 oracle forward+backward on a bounded sample
+    # ---- CPU baseline: the C/OpenMP restatement (FP32) and the FP64 torch oracle
     cpu = None
     if not args.skip_cpu:
         from oracle import builders as ob
         from oracle import mass_spring as om
-        Bc, Tc = args.cpu_robots, args.cpu_horizon
+        from oracle.cpu_rollout import CpuRollout
         objects, springs, groups = ob.build_quadruped()
         rob = om.make_robot(objects, springs, groups)
+        # the C/OpenMP restatement in FP32 on the small training configuration:
+        # 32 tasks x 1000 steps, checkpoints every 50 steps recomputed in the
+        # backward, one Adam step per iteration; median of 5 after a warm-up
+        Bc, Tc = args.cpu_tasks, T
         cfg = om.TaskConfig(steps=Tc, window=min(W, Tc))
+        x0c = np.repeat(om.initial_state(rob, 1)[0].numpy(), Bc, 0)
+        threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+
+        def c_iters(nthreads, iters=5):
+            cr = CpuRollout(rob, cfg, K=K, dtype="f32")
+            cr.set_weights(*[x.float().cpu().numpy() for x in w])
+            times = []
+            for it in range(iters + 1):
+                t2 = time.perf_counter()
+                _, _, gcpu = cr.run(x0c, tv[:Bc], th[:Bc], threads=nthreads)
+                cr.adam(gcpu)
+                if it:
+                    times.append(time.perf_counter() - t2)
+            return Bc * Tc / float(np.median(times))
+
+        v_all, v_one = c_iters(threads), c_iters(1)
+        # the FP64 torch restatement (autograd, no checkpointing) beside it
+        Bt = args.cpu_robots
+        cfg64 = om.TaskConfig(steps=Tc, window=min(W, Tc))
         w64 = [x.double().cpu().clone().requires_grad_() for x in w]
-        x0o, v0o = om.initial_state(rob, Bc)
+        x0o, v0o = om.initial_state(rob, Bt)
         t2 = time.perf_counter()
-        lo, _, _ = om.rollout_task(rob, cfg, w64, torch.tensor(tv[:Bc]), torch.tensor(th[:Bc]), x0o, v0o)
+        lo, _, _ = om.rollout_task(rob, cfg64, w64, torch.tensor(tv[:Bt]), torch.tensor(th[:Bt]),
+                                   x0o, v0o)
         lo.backward()
         cw = time.perf_counter() - t2
-        cpu = {"value": Bc * Tc / cw, "unit": "robot-steps/s", "cores": torch.get_num_threads(),
-               "kind": "port", "sample": f"FP64 torch-CPU oracle, {Bc} robots x {Tc} steps, "
-               "forward + autograd backward (no checkpointing)"}
+        cpu = {"value": v_all, "unit": "robot-steps/s", "cores": threads, "kind": "port",
+               "sample": f"C/OpenMP restatement (oracle/cpu_rollout.c), FP32, {Bc} tasks x {Tc} "
+                         f"steps, checkpoints every {K} steps recomputed in the backward, one "
+                         f"Adam step per iteration, median of 5 iterations after 1 warm-up",
+               "one_thread": v_one,
+               "fp64_torch_oracle": {"value": Bt * Tc / cw, "unit": "robot-steps/s",
+                                     "cores": torch.get_num_threads(),
+                                     "sample": f"FP64 torch-CPU oracle, {Bt} robots x {Tc} "
+                                               "steps, forward + autograd backward (no "
+                                               "checkpointing)"}}
 
     key = (R, T, batch.K) if batch.variant == 3 and not stored else None
     mk = {k[0]: v for k, v in MEASURED_KIB.items() if key and k[1:] == key}

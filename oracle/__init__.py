@@ -16,6 +16,12 @@ everywhere: ``multitask/utils.py:12-15``) in FP64 on the CPU:
   FP64 CPU tensors so that ``torch.autograd`` over the restated forward stands
   in for Taichi's generated adjoints (Taichi is not installed here).
 * ``mpm``         -- MLS-MPM step, 2D and 3D (``multitask/solver_mpm.py``).
+* ``cpu_rollout`` -- ctypes binding of ``cpu_rollout.c``: the quadruped
+  training rollout restated in C with OpenMP (forward, hand-written adjoint,
+  checkpoints with segment recompute, Adam), built in FP32 and FP64 by
+  ``build_cpu.py``.  The FP32 build is the benchmark's CPU baseline; the
+  FP64 build is pinned against ``mass_spring`` autograd
+  (``tests/test_cpu_rollout.py``).
 
 Pinning (what fixes each function independently of its own formula) is listed
 in each module's header and exercised by ``tests/test_oracle_*.py``.
