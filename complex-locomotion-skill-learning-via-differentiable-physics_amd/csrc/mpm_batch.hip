@@ -681,7 +681,8 @@ __global__ __launch_bounds__(PB) void k_p2g_b(MP<T> p, PView<D, T> pv, const T *
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *tile = reinterpret_cast<T *>(smem_raw);     // [tb.tn][4]
     __shared__ int red[2 * D * NWAVE];
-    const Window wn = window_of(win, pv.P);
+    if (idle_at(p, blockIdx.y, p.t)) return;
+    const Window wn = window_of(win, n_part(p, blockIdx.y, pv.P));
     if (threadIdx.x == 0 && wn.q1 - wn.q0 > PB) atomicOr(err, 2);
     const int k = blockIdx.y, q = wn.q0 + (int)threadIdx.x, P = pv.P;
     bool valid = q >= 0 && q < wn.q1;
@@ -747,6 +748,7 @@ template <int D, typename T>
 __global__ __launch_bounds__(PB) void k_grid_b(MP<T> p, T *grid, T *gvo, Blk bk, int clear,
                                                TileBuf<T> tb, GridSave<T> sv) {
     const int k = blockIdx.y, lane = threadIdx.x & 63;
+    if (idle_at(p, k, p.t)) return;
     const TileBuf<T> tbk = tb.robot(k, bk.nblk);
     const int nw = gridDim.x * NWAVE;
     const int cnt = bk.count[k];
@@ -830,6 +832,7 @@ __device__ __forceinline__ void grid_restore(const MP<T> &p, T *grid, T *gvo, co
 template <int D, typename T>
 __global__ __launch_bounds__(PB) void k_grid_restore(MP<T> p, T *grid, T *gvo, Blk bk,
                                                      GridSave<T> sv, int vo_only) {
+    if (idle_at(p, blockIdx.y, p.t)) return;
     grid_restore<D, T>(p, grid, gvo, bk, sv, vo_only, blockIdx.x, gridDim.x);
 }
 
@@ -868,13 +871,14 @@ __global__ __launch_bounds__(PB) void k_g2p_b(MP<T> p, int P, const T *x, const 
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *acc = reinterpret_cast<T *>(smem_raw);
     const int k = blockIdx.y, q = blockIdx.x * PB + threadIdx.x;
+    if (idle_at(p, k, p.t)) return;
     if (SENSE) {
         for (int i = threadIdx.x; i < 3 * so.Gs + 6; i += PB) acc[i] = T(0);
         __syncthreads();
     }
     // the grid pass that read the active list has finished: empty it
     if (blockIdx.x == 0 && threadIdx.x == 0) count[k] = 0;
-    const bool valid = q < P;
+    const bool valid = q < n_part(p, k, P);
     if constexpr (FUPD)
         if (valid) f_update<D, T>(p, P, C_in, F_in, F_out, k, q);
     T nx[D], nv[D];
@@ -935,13 +939,14 @@ __global__ __launch_bounds__(PB) void k_g2p_b(MP<T> p, int P, const T *x, const 
 
 // Sensor sums of a stored state (the first step of a rollout).
 template <typename T>
-__global__ __launch_bounds__(PB) void k_sense(int P, const T *x, const T *v, SenseOut<T> so) {
+__global__ __launch_bounds__(PB) void k_sense(int P, const int32_t *np, const T *x, const T *v,
+                                              SenseOut<T> so) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *acc = reinterpret_cast<T *>(smem_raw);
     const int k = blockIdx.y, q = blockIdx.x * PB + threadIdx.x;
     for (int i = threadIdx.x; i < 3 * so.Gs + 6; i += PB) acc[i] = T(0);
     __syncthreads();
-    const bool valid = q < P;
+    const bool valid = q < (np ? min(np[k], P) : P);
     T nx[3] = {T(0), T(0), T(0)}, nv[3] = {T(0), T(0), T(0)};
     int g = 0;
     if (valid) {
@@ -961,6 +966,14 @@ template <typename T> struct Ctrl {
     int P, Gs, n_sin, run_period, I, H, O, act;
     T scale, max_speed;
     const T *inv_count, *targets, *W1, *b1, *W2, *b2;
+    const int32_t *np, *hz;   // uneven batches (see MP); nullptr: even
+    int per_robot_weights;    // robot k's W1, b1, W2, b2 at k (H I), k H, k (O H), k O
+    int t;                    // the step (adjoint launches)
+    // robot k's weights
+    __device__ __forceinline__ const T *w1(int k) const { return W1 + (per_robot_weights ? (size_t)k * H * I : 0); }
+    __device__ __forceinline__ const T *bb1(int k) const { return b1 + (per_robot_weights ? (size_t)k * H : 0); }
+    __device__ __forceinline__ const T *w2(int k) const { return W2 + (per_robot_weights ? (size_t)k * O * H : 0); }
+    __device__ __forceinline__ const T *bb2(int k) const { return b2 + (per_robot_weights ? (size_t)k * O : 0); }
 };
 
 // Block per robot: COM / COM velocity of step t from the sensor sums, and
@@ -974,13 +987,17 @@ __global__ __launch_bounds__(64) void k_ctrl(Ctrl<T> c, int t, const T *sens, T 
     T *shd = si + c.I;
     const int k = blockIdx.x, tid = threadIdx.x;
     const int NS = 3 * c.Gs + 6;
+    // robot k: state t exists for t <= hz[k]; the controller runs for t < hz[k]
+    if (c.hz && t > c.hz[k]) return;
+    if (c.hz && t == c.hz[k]) mlp = 0;
     const T *sk = sens + (size_t)k * NS;
-    const T invP = T(1) / T(c.P);
+    const T invP = T(1) / T(c.np ? min(c.np[k], c.P) : c.P);
     if (tid < 3) {
         center[k * 3 + tid] = sk[3 * c.Gs + tid] * invP;
         vcom[k * 3 + tid] = sk[3 * c.Gs + 3 + tid] * invP;
     }
     if (!mlp) return;
+    const T *W1 = c.w1(k), *b1 = c.bb1(k), *W2 = c.w2(k), *b2 = c.bb2(k);
     const T *tg = c.targets + (size_t)k * 5;
     for (int j = tid; j < c.I; j += 64) {
         T val;
@@ -1001,8 +1018,8 @@ __global__ __launch_bounds__(64) void k_ctrl(Ctrl<T> c, int t, const T *sens, T 
     }
     __syncthreads();
     for (int h = tid; h < c.H; h += 64) {
-        T z = c.b1[h];
-        const T *w = c.W1 + (size_t)h * c.I;
+        T z = b1[h];
+        const T *w = W1 + (size_t)h * c.I;
         for (int i = 0; i < c.I; ++i) z += w[i] * si[i];
         T y, dy;
         activate<T>(c.act, z, y, dy);
@@ -1011,8 +1028,8 @@ __global__ __launch_bounds__(64) void k_ctrl(Ctrl<T> c, int t, const T *sens, T 
     }
     __syncthreads();
     for (int o = tid; o < c.O; o += 64) {
-        T z = c.b2[o];
-        const T *w = c.W2 + (size_t)o * c.H;
+        T z = b2[o];
+        const T *w = W2 + (size_t)o * c.H;
         for (int h = 0; h < c.H; ++h) z += w[h] * shd[h];
         T y, dy;
         activate<T>(c.act, z, y, dy);
@@ -1035,11 +1052,12 @@ __device__ __forceinline__ void ctrl_adj(const Ctrl<T> &c, int k, int nt, const 
     T *sz1b = sd1 + c.H;
     T *sab = sz1b + c.H;
     const int tid = threadIdx.x;
+    const T *W1 = c.w1(k), *b1 = c.bb1(k), *W2 = c.w2(k), *b2 = c.bb2(k);
     for (int i = tid; i < c.I; i += nt) si[i] = inp[(size_t)k * c.I + i];
     __syncthreads();
     for (int h = tid; h < c.H; h += nt) {
-        T z = c.b1[h];
-        const T *w = c.W1 + (size_t)h * c.I;
+        T z = b1[h];
+        const T *w = W1 + (size_t)h * c.I;
         for (int i = 0; i < c.I; ++i) z += w[i] * si[i];
         T y, dy;
         activate<T>(c.act, z, y, dy);
@@ -1048,8 +1066,8 @@ __device__ __forceinline__ void ctrl_adj(const Ctrl<T> &c, int k, int nt, const 
     }
     __syncthreads();
     for (int o = tid; o < c.O; o += nt) {
-        T z = c.b2[o];
-        const T *w = c.W2 + (size_t)o * c.H;
+        T z = b2[o];
+        const T *w = W2 + (size_t)o * c.H;
         for (int h = 0; h < c.H; ++h) z += w[h] * shd[h];
         T y, dy;
         activate<T>(c.act, z, y, dy);
@@ -1060,13 +1078,13 @@ __device__ __forceinline__ void ctrl_adj(const Ctrl<T> &c, int k, int nt, const 
     __syncthreads();
     for (int h = tid; h < c.H; h += nt) {
         T hb = T(0);
-        for (int o = 0; o < c.O; ++o) hb += c.W2[(size_t)o * c.H + h] * sab[o];
+        for (int o = 0; o < c.O; ++o) hb += W2[(size_t)o * c.H + h] * sab[o];
         sz1b[h] = hb * sd1[h];
     }
     __syncthreads();
     for (int i = tid; i < c.I; i += nt) {
         T ib = T(0);
-        for (int h = 0; h < c.H; ++h) ib += c.W1[(size_t)h * c.I + i] * sz1b[h];
+        for (int h = 0; h < c.H; ++h) ib += W1[(size_t)h * c.I + i] * sz1b[h];
         in_b[(size_t)k * c.I + i] = ib;
     }
     for (int h = tid; h < c.H; h += nt) zh[(size_t)k * c.H + h] = sz1b[h];
@@ -1077,6 +1095,7 @@ __device__ __forceinline__ void ctrl_adj(const Ctrl<T> &c, int k, int nt, const 
 template <typename T>
 __global__ __launch_bounds__(64) void k_ctrl_adj(Ctrl<T> c, const T *inp, T *act_b, T *in_b,
                                                  T *zh, T *zo) {
+    if (c.hz && c.t >= c.hz[blockIdx.x]) return;
     ctrl_adj<T>(c, blockIdx.x, 64, inp, act_b, in_b, zh, zo);
 }
 
@@ -1089,10 +1108,12 @@ __global__ __launch_bounds__(64) void k_ctrl_adj(Ctrl<T> c, const T *inp, T *act
 template <typename T>
 __global__ __launch_bounds__(256) void k_wgrad_time(int B, int I, int H, int O, int n,
                                                     const T *inp, const T *hid, const T *zh,
-                                                    const T *zo, T *wg) {
+                                                    const T *zo, T *wg, const int32_t *hz,
+                                                    int t0) {
     const int k = blockIdx.y, e = blockIdx.x * 256 + threadIdx.x;
     const int nW1 = H * I, nb1 = nW1 + H, nW2 = nb1 + O * H, nw = nW2 + O;
     if (e >= nw) return;
+    if (hz) n = max(0, min(n, hz[k] - t0));   // robot k's steps of the segment
     T acc = T(0);
     if (e < nW1) {
         const int h = e / I, i = e - h * I;
@@ -1121,10 +1142,12 @@ __global__ __launch_bounds__(PB) void k_restore_ctrl_adj(MP<T> p, T *grid, T *gv
                                                          GridSave<T> sv, int nr, Ctrl<T> c,
                                                          const T *inp, T *act_b, T *in_b, T *zh,
                                                          T *zo) {
-    if ((int)blockIdx.x < nr)
-        grid_restore<D, T>(p, grid, gvo, bk, sv, 0, blockIdx.x, nr);
-    else
+    if ((int)blockIdx.x < nr) {
+        if (!idle_at(p, blockIdx.y, p.t - 1))
+            grid_restore<D, T>(p, grid, gvo, bk, sv, 0, blockIdx.x, nr);
+    } else if (!idle_at(p, blockIdx.y, p.t)) {
         ctrl_adj<T>(c, blockIdx.y, PB, inp, act_b, in_b, zh, zo);
+    }
 }
 
 // ---------------------------------------------------------------- adjoints
@@ -1147,7 +1170,8 @@ __global__ __launch_bounds__(PB) void k_g2p_adj_b(MP<T> p, int P, const T *x, co
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *tile = reinterpret_cast<T *>(smem_raw);     // [tb.tn][4]
     __shared__ int red[2 * D * NWAVE];
-    const Window wn = window_of(win, P);
+    if (idle_at(p, blockIdx.y, p.t)) return;
+    const Window wn = window_of(win, n_part(p, blockIdx.y, P));
     const int k = blockIdx.y, q = wn.q0 + (int)threadIdx.x;
     const bool valid = q >= 0 && q < wn.q1;
     const size_t bv = (size_t)k * D * P, bm = (size_t)k * D * D * P;
@@ -1168,7 +1192,7 @@ __global__ __launch_bounds__(PB) void k_g2p_adj_b(MP<T> p, int P, const T *x, co
             if (sa.in_b) {
                 const T *ib = sa.in_b + (size_t)k * sa.I;
                 const int g = sa.group[(size_t)k * P + q];
-                const T invP = T(1) / T(P);
+                const T invP = T(1) / T(n_part(p, k, P));
                 const T icg = sa.inv_count[(size_t)k * sa.Gs + g];
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
@@ -1267,6 +1291,7 @@ template <int D, typename T>
 __global__ __launch_bounds__(PB) void k_grid_adj_b(MP<T> p, T *grid, T *gvo_b, T *grid_b, Blk bk,
                                                    TileBuf<T> tb) {
     const int k = blockIdx.y, lane = threadIdx.x & 63;
+    if (idle_at(p, k, p.t)) return;
     const TileBuf<T> tbk = tb.robot(k, bk.nblk);
     const int nw = gridDim.x * NWAVE;
     const int cnt = bk.count[k];
@@ -1310,11 +1335,12 @@ __global__ __launch_bounds__(PB) void k_p2g_adj_b(MP<T> p, PView<D, T> pv, const
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *sab = reinterpret_cast<T *>(smem_raw);     // [n_act]
     const int k = blockIdx.y, q = blockIdx.x * PB + threadIdx.x, P = pv.P;
+    if (idle_at(p, k, p.t)) return;
     for (int i = threadIdx.x; i < n_act; i += PB) sab[i] = T(0);
     __syncthreads();
     // the grid adjoint pass that read the active list has finished
     if (blockIdx.x == 0 && threadIdx.x == 0) count[k] = 0;
-    const bool valid = q < P;
+    const bool valid = q < n_part(p, k, P);
     int id = -1;
     T abar[1] = {T(0)};
     if (valid) {
@@ -1440,6 +1466,12 @@ int check_batch(const dl_mpm_batch *b) {
     DL_CHECK_ARG(b->scatter == DL_MPM_SCATTER_ATOMIC || b->scatter == DL_MPM_SCATTER_STORE,
                  "MPM batch: bad scatter mode %d", b->scatter);
     DL_CHECK_ARG(b->B > 0 && b->P > 0 && b->n_act > 0 && b->horizon > 0, "MPM batch: bad sizes");
+    DL_CHECK_ARG(!(b->n_part || b->horizons || b->per_robot_weights) ||
+                     (!b->windows && !b->hgrid && b->restore_from == 0),
+                 "MPM batch: uneven batches (n_part, horizons, per-robot weights) run with plain "
+                 "windows and without the horizon save");
+    DL_CHECK_ARG(b->per_robot_weights == 0 || b->per_robot_weights == 1,
+                 "MPM batch: per_robot_weights must be 0 or 1");
     DL_CHECK_ARG(!b->windows || (b->n_windows >= (b->P + PB - 1) / PB && b->n_windows <= b->P),
                  "MPM batch: %d windows cannot partition %d particles in windows of at most %d",
                  b->n_windows, b->P, PB);
@@ -1482,6 +1514,10 @@ template <typename T> Ctrl<T> make_ctrl(const dl_mpm_batch *b) {
     c.inv_count = (const T *)b->inv_count; c.targets = (const T *)b->targets;
     c.W1 = (const T *)b->W1; c.b1 = (const T *)b->b1; c.W2 = (const T *)b->W2;
     c.b2 = (const T *)b->b2;
+    c.np = b->n_part;
+    c.hz = b->horizons;
+    c.per_robot_weights = b->per_robot_weights;
+    c.t = 0;
     return c;
 }
 
@@ -1533,7 +1569,9 @@ int forward_impl(const dl_mpm_batch *b, const dl_mpm_slots *st, int t0, int n, i
                  hipStream_t s) {
     const int B = b->B, P = b->P, O = b->n_act, NS = 3 * b->n_groups + 6;
     const bool control = flags & DL_MPM_CONTROL;
-    const MP<T> mp = make_mp<T>(b->params);
+    MP<T> mp = make_mp<T>(b->params);
+    mp.np = b->n_part;
+    mp.hz = b->horizons;
     const Blk bk = make_blk<D>(b);
     const dim3 pg((P + PB - 1) / PB, B), gg(grid_wgs(bk), B), pw(n_windows(b), B);
     const int I = n_in(b), H = b->n_hidden;
@@ -1546,12 +1584,13 @@ int forward_impl(const dl_mpm_batch *b, const dl_mpm_slots *st, int t0, int n, i
     if (control && (flags & DL_MPM_SENSE_FIRST)) {
         const Slot<D, T> s0 = slot<D, T>(st, 0, B, P);
         if constexpr (D == 3)
-            hipLaunchKernelGGL(k_sense<T>, pg, dim3(PB), NS * sizeof(T), s, P, s0.x, s0.v,
+            hipLaunchKernelGGL(k_sense<T>, pg, dim3(PB), NS * sizeof(T), s, P, b->n_part, s0.x, s0.v,
                                SenseOut<T>{(T *)b->sens + (size_t)t0 * B * NS, b->group,
                                            b->n_groups});
     }
     for (int j = 0; j < n; ++j) {
         const int t = t0 + j;
+        mp.t = t;
         const Slot<D, T> a = slot<D, T>(st, j % st->n_slots, B, P);
         const Slot<D, T> c = slot<D, T>(st, (j + 1) % st->n_slots, B, P);
         if (flags & DL_MPM_REPLAY) {
@@ -1603,11 +1642,13 @@ template <int D, typename T>
 int backward_impl(const dl_mpm_batch *b, const dl_mpm_slots *seg, const dl_mpm_slots *adj, int t0,
                   int n, hipStream_t s) {
     const int B = b->B, P = b->P, O = b->n_act;
-    const MP<T> mp = make_mp<T>(b->params);
+    MP<T> mp = make_mp<T>(b->params);
+    mp.np = b->n_part;
+    mp.hz = b->horizons;
     const Blk bk = make_blk<D>(b);
     const dim3 pg((P + PB - 1) / PB, B), gg(grid_wgs(bk), B), pw(n_windows(b), B);
     const int I = n_in(b), H = b->n_hidden;
-    const Ctrl<T> ctrl = make_ctrl<T>(b);
+    Ctrl<T> ctrl = make_ctrl<T>(b);
     const T *act = (const T *)b->act_out;
     const TileBuf<T> tb = make_tiles<T>(b);
     const size_t tile_bytes = (size_t)tb.tn * 4 * sizeof(T);
@@ -1624,6 +1665,8 @@ int backward_impl(const dl_mpm_batch *b, const dl_mpm_slots *seg, const dl_mpm_s
     bool restored = false;  // step j's grid restored by the previous launch
     for (int j = n - 1; j >= 0; --j) {
         const int t = t0 + j, nxt = 1 - cur;
+        mp.t = t;
+        ctrl.t = t;
         const Slot<D, T> a = slot<D, T>(seg, j, B, P);
         const Slot<D, T> A1 = slot<D, T>(adj, cur, B, P), A0 = slot<D, T>(adj, nxt, B, P);
         PView<D, T> pv{a.x, a.v, a.C, a.F, P};
@@ -1687,7 +1730,8 @@ int backward_impl(const dl_mpm_batch *b, const dl_mpm_slots *seg, const dl_mpm_s
                            O, n, (const T *)b->inp + (size_t)t0 * B * I,
                            (const T *)b->hid + (size_t)t0 * B * H,
                            (const T *)b->hid_grad + (size_t)t0 * B * H,
-                           (const T *)b->out_grad + (size_t)t0 * B * O, (T *)b->wgrad);
+                           (const T *)b->out_grad + (size_t)t0 * B * O, (T *)b->wgrad,
+                           b->horizons, t0);
     }
     if (cur != 0) {
         const Slot<D, T> from = slot<D, T>(adj, 1, B, P), to = slot<D, T>(adj, 0, B, P);

@@ -11,7 +11,22 @@ namespace dl {
 template <typename T> struct MP {
     int n, bound, act_axis;
     T dx, inv_dx, dt, g_dt, p_vol, p_mass, mu, la, act_strength, kappa;
+    // uneven batches: particles [0, np[k]) of robot k are its own (the rest of
+    // its P slots are padding, never read); robot k advances steps t < hz[k]
+    // only (t = the step a launch computes).  nullptr: every robot has P
+    // particles and the whole horizon.
+    const int32_t *np = nullptr, *hz = nullptr;
+    int t = 0;
 };
+
+// Robot k does not take step t (past its horizon).
+template <typename T> __host__ __device__ __forceinline__ bool idle_at(const MP<T> &p, int k, int t) {
+    return p.hz && t >= p.hz[k];
+}
+// Particles of robot k (P when the batch is even).
+template <typename T> __host__ __device__ __forceinline__ int n_part(const MP<T> &p, int k, int P) {
+    return p.np ? min(p.np[k], P) : P;
+}
 
 template <typename T>
 static MP<T> make_mp(const dl_mpm_params &c) {

@@ -397,7 +397,9 @@ class MPMBatchC(Structure):
                 [(n, ctypes.c_void_p) for n in ("hid_grad", "out_grad", "windows")] +
                 [("n_windows", c_int32), ("pad_", c_int32)] +
                 [(n, ctypes.c_void_p) for n in ("hgrid", "hvo", "hlist", "hcount", "hover")] +
-                [("hcap", c_int32), ("restore_from", c_int32)])
+                [("hcap", c_int32), ("restore_from", c_int32)] +
+                [(n, ctypes.c_void_p) for n in ("n_part", "horizons")] +
+                [("per_robot_weights", c_int32), ("pad2_", c_int32)])
 
 
 class MPMSlotsC(Structure):
@@ -495,26 +497,67 @@ class MPMBatch:
     every step's particle state (T + T/K slots of x, v, C, F: one view of
     K + 1 slots per segment), and the backward of a segment whose every
     step's grid was saved runs the adjoint on those states with no
-    recompute at all."""
+    recompute at all.
+
+    Uneven batches: ``positions`` and ``act_id`` given as lists of B arrays
+    (one body per robot, any particle counts) pad every robot to the largest
+    count (padding is never read); ``horizons`` (B values, at most T) stop
+    each robot at its own step; ``per_robot_weights`` gives every robot its
+    own controller (``set_weights`` then takes [B, ...] arrays, ``grads`` are
+    [B, ...]).  Each robot's loss is the windowed COM loss over its own last
+    ``min(window, horizon)`` states (``loss_b``; ``loss`` is their mean).
+    Uneven batches run with plain 256-particle windows and without the
+    horizon grid save."""
 
     def __init__(self, params, positions, act_id, n_act, B, T, K=100, window=None,
                  dtype=torch.float32, n_sin=10, run_period=100, offset_scale=0.2,
                  max_speed=0.08, activation="tanh", n_hidden=64, device="cuda",
                  sort_particles=True, tile_nodes=None, scatter=SCATTER_ATOMIC, save_grid=True,
                  unroll=3, box_limit=512, resort=True, save_horizon=True, horizon_blocks=None,
-                 store_states=False, overlap_replay=True):
+                 store_states=False, overlap_replay=True, horizons=None, per_robot_weights=False):
         if params.dim != 3:
             raise ValueError("MPMBatch drives 3D robots (the controller's sensors are 3D)")
         if params.n_grid % 4:
             raise ValueError("MPMBatch: n_grid must be a multiple of 4")
         self.p, self.B, self.T, self.K = params, B, T, K
         self.D = D = 3
-        pos = np.asarray(positions, np.float64)
-        ids = np.asarray(act_id, np.int32)
-        self.order = block_order(pos, params.n_grid) if sort_particles else np.arange(len(pos))
-        pos, ids = pos[self.order], ids[self.order]
+        self.uneven = isinstance(positions, (list, tuple))
+        self.per_robot_weights = bool(per_robot_weights)
+        self.horizons = None if horizons is None else np.asarray(horizons, np.int64)
+        if self.uneven or self.horizons is not None or self.per_robot_weights:
+            if store_states or save_horizon:
+                raise ValueError("MPMBatch: uneven batches run without the horizon grid save "
+                                 "and stored states (save_horizon=False, store_states=False)")
+            box_limit = None
+        if self.horizons is not None:
+            if self.horizons.shape != (B,) or self.horizons.min() < 1 or self.horizons.max() > T:
+                raise ValueError(f"MPMBatch: horizons must be {B} values in [1, {T}]")
+        if self.uneven:
+            if len(positions) != B or len(act_id) != B:
+                raise ValueError(f"MPMBatch: {B} bodies expected")
+            bodies = []
+            for pk, ik in zip(positions, act_id):
+                pk, ik = np.asarray(pk, np.float64), np.asarray(ik, np.int32)
+                ok = block_order(pk, params.n_grid) if sort_particles else np.arange(len(pk))
+                bodies.append((pk[ok], ik[ok], ok))
+            self.n_parts = np.array([len(b[0]) for b in bodies], np.int64)
+            P = int(self.n_parts.max())
+            # padding: the robot's first particle's position, no actuator
+            pos_b = np.stack([np.concatenate([b[0], np.repeat(b[0][:1], P - len(b[0]), 0)])
+                              for b in bodies])
+            ids_b = np.stack([np.concatenate([b[1], np.full(P - len(b[1]), -1, np.int32)])
+                              for b in bodies])
+            self.order = [b[2] for b in bodies]
+            pos, ids = bodies[0][0], bodies[0][1]
+        else:
+            pos = np.asarray(positions, np.float64)
+            ids = np.asarray(act_id, np.int32)
+            self.order = block_order(pos, params.n_grid) if sort_particles else np.arange(len(pos))
+            pos, ids = pos[self.order], ids[self.order]
+            P = len(pos)
+            self.n_parts = None
         self.positions = pos
-        self.P = P = len(pos)
+        self.P = P
         self.n_act = n_act
         self.window = window if window is not None else T // 2
         self.dtype, self.device = dtype, torch.device(device)
@@ -525,16 +568,25 @@ class MPMBatch:
         self.G = n_act + 1                                  # actuator groups + passive body
         self.I = n_sin + 3 + 3 * self.G + 4
         self.H, self.O = n_hidden, n_act
-        grp = np.where(ids >= 0, ids, n_act).astype(np.int32)
-        counts = np.bincount(grp, minlength=self.G).astype(np.float64)
         dev = self.device
         z = lambda *s: torch.zeros(*s, dtype=dtype, device=dev)
         zi = lambda *s: torch.zeros(*s, dtype=torch.int32, device=dev)
         bt = lambda a, dt_: torch.as_tensor(a, dtype=dt_)[None].expand(B, *np.shape(a)).contiguous().to(dev)
-        self.act_id = bt(ids, torch.int32)
-        self.group = bt(grp, torch.int32)
+        inv = lambda c: np.where(c > 0, 1.0 / np.maximum(c, 1), 0.0)
+        if self.uneven:
+            grp_b = np.where(ids_b >= 0, ids_b, n_act).astype(np.int32)
+            counts = np.stack([np.bincount(g[:n], minlength=self.G)
+                               for g, n in zip(grp_b, self.n_parts)]).astype(np.float64)
+            self.act_id = torch.as_tensor(ids_b, dtype=torch.int32).to(dev)
+            self.group = torch.as_tensor(grp_b, dtype=torch.int32).to(dev)
+            self.inv_count = torch.as_tensor(inv(counts), dtype=dtype).to(dev)
+        else:
+            grp = np.where(ids >= 0, ids, n_act).astype(np.int32)
+            counts = np.bincount(grp, minlength=self.G).astype(np.float64)
+            self.act_id = bt(ids, torch.int32)
+            self.group = bt(grp, torch.int32)
+            self.inv_count = bt(inv(counts), dtype)
         self.resort = resort
-        self.inv_count = bt(np.where(counts > 0, 1.0 / np.maximum(counts, 1), 0.0), dtype)
         ng = params.n_grid ** 3
         nblk = (params.n_grid // 4) ** 3
         self.grid, self.grid_vo, self.grid_vo_grad, self.grid_grad = (z(B, ng, 4) for _ in range(4))
@@ -570,10 +622,19 @@ class MPMBatch:
         self.nw = H * I + H + O * H + O
         self.wgrad = z(B, self.nw)
         self.hid_grad, self.out_grad = emp(T, B * H), emp(T, B * O)
-        self.W = [z(H, I), z(H), z(O, H), z(O)]
+        self.W = ([z(B, H, I), z(B, H), z(B, O, H), z(B, O)] if self.per_robot_weights
+                  else [z(H, I), z(H), z(O, H), z(O)])
         self.targets, self.tv, self.th = z(B, 5), z(B, 3), z(B)
         self.loss, self.loss_b = z(1), z(B)
-        self.x0 = torch.as_tensor(np.ascontiguousarray(pos.T), dtype=dtype).to(dev)   # [3,P]
+        if self.uneven:                                                             # [B,3,P]
+            self.x0 = torch.as_tensor(np.ascontiguousarray(pos_b.transpose(0, 2, 1)),
+                                      dtype=dtype).to(dev)
+        else:                                                                       # [3,P]
+            self.x0 = torch.as_tensor(np.ascontiguousarray(pos.T), dtype=dtype).to(dev)
+        self.n_part_t = (torch.as_tensor(self.n_parts, dtype=torch.int32).to(dev)
+                         if self.uneven else None)
+        self.horizons_t = (torch.as_tensor(self.horizons, dtype=torch.int32).to(dev)
+                           if self.horizons is not None else None)
         self.nck = (T + K - 1) // K
         if resort:
             # segment s runs in order orders[s] (indices into the initial
@@ -628,7 +689,10 @@ class MPMBatch:
             count_save=P_(self.count_save) if save_grid else None,
             save_steps=K if save_grid else 0, scatter_unroll=unroll,
             hid_grad=P_(self.hid_grad), out_grad=P_(self.out_grad),
-            windows=P_(self.windows) if win is not None else None, n_windows=wpr)
+            windows=P_(self.windows) if win is not None else None, n_windows=wpr,
+            n_part=P_(self.n_part_t) if self.uneven else None,
+            horizons=P_(self.horizons_t) if self.horizons is not None else None,
+            per_robot_weights=int(self.per_robot_weights))
         self.save_horizon = save_horizon
         if save_horizon:
             if horizon_blocks is None:
@@ -662,8 +726,18 @@ class MPMBatch:
         self.th.copy_(self.targets[:, 3])
 
     def set_weights(self, W1, b1, W2, b2):
+        """Controller weights W1 [H, I], b1 [H], W2 [O, H], b2 [O]; with
+        per-robot weights each with a leading robot axis (or without it: the
+        same weights for every robot)."""
         for d, s in zip(self.W, (W1, b1, W2, b2)):
-            d.copy_(torch.as_tensor(s, dtype=self.dtype))
+            d.copy_(torch.as_tensor(s, dtype=self.dtype, device=d.device).expand_as(d))
+
+    def set_robot_weights(self, k, W1, b1, W2, b2):
+        """Robot k's controller (per-robot weights)."""
+        if not self.per_robot_weights:
+            raise ValueError("MPMBatch: set_robot_weights needs per_robot_weights=True")
+        for d, s in zip(self.W, (W1, b1, W2, b2)):
+            d[k].copy_(torch.as_tensor(s, dtype=self.dtype, device=d.device))
 
     def active_nodes(self):
         """Grid nodes inside some particle's stencil in the initial state (per robot)."""
@@ -686,7 +760,11 @@ class MPMBatch:
         nb = n // 4
         blk = ((base[:, 0] // 4) * nb + base[:, 1] // 4) * nb + base[:, 2] // 4
         cell = ((base[:, 0] % 4) * 4 + base[:, 1] % 4) * 4 + base[:, 2] % 4
-        return blk * 64 + cell
+        key = blk * 64 + cell
+        if self.uneven:       # padding slots stay behind each robot's particles
+            pad = torch.arange(self.P, device=x.device)[None] >= self.n_part_t[:, None]
+            key = torch.where(pad, torch.full_like(key, 1 << 62), key)
+        return key
 
     def _set_order(self, s):
         self.act_id.copy_(torch.gather(self.act_id0, 1, self.orders[s]))
@@ -696,7 +774,7 @@ class MPMBatch:
         r = self.ring
         if self.resort:
             self._set_order(0)
-        r.x[0].copy_(self.x0[None].expand(self.B, -1, -1))
+        r.x[0].copy_(self.x0 if self.uneven else self.x0[None].expand(self.B, -1, -1))
         r.v[0].zero_()
         r.C[0].zero_()
         r.F[0].zero_()
@@ -731,7 +809,25 @@ class MPMBatch:
         if n % 2:
             r.copy_slot(0, 1)
 
+    def _loss_mask(self):
+        """[T+1, B] mask of each robot's loss states (t in (T_k - W_k, T_k],
+        W_k = min(window, T_k)) and the windows W_k."""
+        hz = self.horizons_t.to(torch.int64)
+        wk = torch.clamp(hz, max=self.window)
+        t = torch.arange(self.T + 1, device=hz.device)[:, None]
+        return ((t > hz - wk) & (t <= hz)).to(self.dtype), wk.to(self.dtype)
+
     def finish_forward(self):
+        if self.horizons is not None:
+            # per-robot horizons: each robot's windowed loss over its own
+            # last states (the terms of dl_loss_task; torch on the device)
+            m, wk = self._loss_mask()
+            c, vc = self.center, self.vcom
+            term = ((vc[..., 0] - self.tv[:, 0]) ** 2 + (vc[..., 2] - self.tv[:, 2]) ** 2
+                    + (c[..., 1] - self.th) ** 2)
+            self.loss_b.copy_((term * m).sum(0) / wk)
+            self.loss.copy_(self.loss_b.mean().reshape(1))
+            return self.loss
         nat.call("dl_loss_task", self.dc, self.B, self.T, self.window, nat.ptr(self.center),
                  nat.ptr(self.vcom), nat.ptr(self.tv), nat.ptr(self.th), nat.ptr(self.loss),
                  nat.ptr(self.loss_b), nat.stream())
@@ -744,6 +840,9 @@ class MPMBatch:
         return self.finish_forward()
 
     def start_backward(self, loss_grad=1.0):
+        """Seeds the adjoint with d(loss_grad * loss)/d(COM terms); with
+        per-robot horizons ``loss_grad`` may be B values, robot k's loss_b[k]
+        then weighted by loss_grad[k] (not divided by B)."""
         if self.save_horizon:
             # segments whose every step's grid was saved are replayed
             over = self.hover.cpu().numpy()
@@ -754,6 +853,17 @@ class MPMBatch:
         self.wgrad.zero_()
         for f in self.adj.fields():
             f[0].zero_()
+            if self.horizons is not None:
+                f[1].zero_()     # robots past their horizon keep a zero adjoint
+        if self.horizons is not None:
+            m, wk = self._loss_mask()
+            g = torch.as_tensor(loss_grad, dtype=self.dtype, device=m.device)
+            g = g / self.B if g.dim() == 0 else g
+            s = m * (2.0 * g / wk)
+            self.cg[..., 1].copy_(s * (self.center[..., 1] - self.th))
+            self.vg[..., 0].copy_(s * (self.vcom[..., 0] - self.tv[:, 0]))
+            self.vg[..., 2].copy_(s * (self.vcom[..., 2] - self.tv[:, 2]))
+            return
         nat.call("dl_loss_task_grad", self.dc, self.B, self.T, self.window, nat.ptr(self.center),
                  nat.ptr(self.vcom), nat.ptr(self.tv), nat.ptr(self.th), float(loss_grad),
                  nat.ptr(self.cg), nat.ptr(self.vg), nat.stream())
@@ -790,8 +900,13 @@ class MPMBatch:
                 f[0].copy_(torch.empty_like(f[0]).scatter_(2, pi[:, None, :].expand_as(f[0]), f[0]))
 
     def finish_backward(self):
+        H, I, O, B = self.H, self.I, self.O, self.B
+        if self.per_robot_weights:
+            parts = torch.split(self.wgrad, [H * I, H, O * H, O], dim=1)
+            self.grads = [parts[0].reshape(B, H, I), parts[1], parts[2].reshape(B, O, H),
+                          parts[3]]
+            return self.grads
         g = self.wgrad.sum(0)
-        H, I, O = self.H, self.I, self.O
         parts = torch.split(g, [H * I, H, O * H, O])
         self.grads = [parts[0].view(H, I), parts[1], parts[2].view(O, H), parts[3]]
         return self.grads

@@ -514,6 +514,18 @@ typedef struct dl_mpm_batch {
     void *hgrid, *hvo;
     int32_t *hlist, *hcount, *hover;
     int32_t hcap, restore_from;
+    /* Uneven batches (optional; NULL / 0: every robot has P particles, the
+     * whole horizon and the shared weights).  n_part [B]: robot k's particles
+     * are slots [0, n_part[k]) of its P (the rest are padding, never read;
+     * sensor and COM sums divide by n_part[k]).  horizons [B]: robot k
+     * advances steps t < horizons[k] only (its state t = horizons[k] is its
+     * last; the controller, P2G, grid and G2P kernels and their adjoints
+     * skip the robot at later steps; the caller seeds its loss gradient at
+     * its own steps).  per_robot_weights = 1: W1, b1, W2, b2 hold B blocks
+     * (robot k's at k*H*I, k*H, k*O*H, k*O reals).  Uneven batches run with
+     * plain windows (windows = NULL) and without the horizon save. */
+    const int32_t *n_part, *horizons;
+    int32_t per_robot_weights, pad2_;
 } dl_mpm_batch;
 
 /* Ring of particle-state slots: slot s of x/v at x + s*B*D*P reals, of C/F

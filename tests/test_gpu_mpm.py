@@ -260,6 +260,58 @@ def test_mpm_batched_overlapped_backward_matches_sequential(shuffled):
         assert rel_err(a.cpu(), c) < 1e-7
 
 
+def _blob(side, origin, n=16):
+    ax = torch.arange(side, dtype=torch.float64)
+    g = torch.stack(torch.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3)
+    pos = (g * 0.5 + torch.tensor(origin)) / n
+    ids = torch.tensor([(i % 5) - 1 for i in range(len(pos))], dtype=torch.int32)
+    return pos.numpy(), ids.numpy()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_mpm_uneven_batch_matches_robots_alone(prec):
+    """One batch of three different bodies (125, 343 and 64 particles:
+    padded to 343), each with its own controller and horizon (24, 13 and 20
+    steps of a 24-step batch; checkpoints every 7 steps, so two robots stop
+    inside a segment), against each robot run alone in an even one-robot
+    batch: per-robot loss and every weight gradient."""
+    _say(f"mpm uneven batch {prec}")
+    dt = DT[prec]
+    params = dm.quadruped3d_params(n_grid=16)
+    params.dt = 2e-4
+    bodies = [_blob(5, (5.0, 3.2, 5.0)), _blob(7, (4.0, 3.2, 4.0)), _blob(4, (6.0, 3.2, 6.0))]
+    hz, T, K, W = [24, 13, 20], 24, 7, 12
+    rng = np.random.default_rng(11)
+    tv = np.stack([rng.uniform(-0.08, 0.08, 3), np.zeros(3), rng.uniform(-0.08, 0.08, 3)], 1)
+    th = rng.uniform(0.1, 0.25, 3)
+    ub = dm.MPMBatch(params, [b[0] for b in bodies], [b[1] for b in bodies], 4, 3, T, K=K,
+                     window=W, dtype=dt, save_horizon=False, horizons=hz,
+                     per_robot_weights=True)
+    Ws = [_weights(ub.I, seed=20 + k) for k in range(3)]
+    ub.set_targets(tv, th)
+    for k in range(3):
+        ub.set_robot_weights(k, *[w.cuda() for w in Ws[k]])
+    ub.forward()
+    grads = ub.backward(loss_grad=torch.ones(3, dtype=torch.float64))
+    torch.cuda.synchronize()
+    ub.check()
+    tl, tg = (1e-11, 1e-9) if prec == "f64" else (1e-5, 1e-4)
+    for k in range(3):
+        one = dm.MPMBatch(params, bodies[k][0], bodies[k][1], 4, 1, hz[k], K=K,
+                          window=min(W, hz[k]), dtype=dt, save_horizon=False)
+        one.set_targets(tv[k:k + 1], th[k:k + 1])
+        one.set_weights(*[w.cuda() for w in Ws[k]])
+        l1 = one.forward().item()
+        g1 = one.backward()
+        torch.cuda.synchronize()
+        one.check()
+        lk = ub.loss_b[k].item()
+        assert abs(lk - l1) / abs(l1) < tl, (k, lk, l1)
+        for a, b in zip(grads, g1):
+            assert b.abs().max() > 0
+            assert rel_err(a[k].double().cpu(), b.double().cpu()) < tg, k
+
+
 @pytest.mark.parametrize("hblocks,shuffled", [(None, False), (1, False), (None, True)])
 def test_mpm_batched_stored_states_match_replay(hblocks, shuffled):
     """Every step's particle state kept by the forward, the adjoint run on
