@@ -180,8 +180,12 @@ def main():
                     help="1: uneven forwards keep every step's adjoint rows; 0: checkpoints "
                          "every 50 steps")
     ap.add_argument("--skip-serve", action="store_true")
-    ap.add_argument("--serve-rate", type=float, default=25.0, help="Poisson arrivals per second")
-    ap.add_argument("--serve-duration", type=float, default=2.0, help="seconds of arrivals")
+    ap.add_argument("--serve-rates", default="50,200,1000,2000",
+                    help="Poisson arrival rates (per second) of the serving sweep")
+    ap.add_argument("--serve-requests", type=int, default=120,
+                    help="requests per sweep point (the same robots at every rate)")
+    ap.add_argument("--serve-p99-bound", type=float, default=2.0,
+                    help="p99 latency bound (s) the sweep reports the highest rate for")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--skip-serial", action="store_true")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
@@ -544,8 +548,8 @@ def main():
         phys_s = _physics(qc)
         phys_s.dashpot = UNEVEN_DASHPOT
 
-        def serve_run(max_active):
-            reqs = sv.make_requests(args.serve_rate, args.serve_duration, seed=4, layout=lay)
+        def serve_run(rate, max_active):
+            reqs = sv.make_requests(rate, None, seed=4, layout=lay, count=args.serve_requests)
             ex = sv.default_executors(phys_s, lay)
             srv = sv.Server(ex, budget_bytes=int(args.serve_budget_gb * 2 ** 30), chunk=50,
                             max_active=max_active)
@@ -553,17 +557,33 @@ def main():
             done = srv.run(reqs)
             wall_s = time.perf_counter() - t4
             st = sv.latency_stats(done, wall_s)
-            st.update({"wall_s": wall_s, "ticks": srv.ticks,
+            st.update({"rate_per_s": rate, "arrival_window_s": max(r.arrival for r in reqs),
+                       "wall_s": wall_s, "ticks": srv.ticks,
+                       "rejected": len(srv.rejected),
                        "mpm_requests": sum(r.kind == "mpm" for r in done),
                        "grad_requests": sum(r.grad for r in done),
                        "robot_steps_per_s": st["robot_steps"] / wall_s})
+            del ex, srv
+            torch.cuda.empty_cache()
             return st
 
-        serve = {"arrival_rate_per_s": args.serve_rate, "arrival_window_s": args.serve_duration,
+        rates = [float(r) for r in args.serve_rates.split(",")]
+        sweep = [serve_run(r, None) for r in rates]
+        ok = [p["rate_per_s"] for p in sweep if p["p99_s"] <= args.serve_p99_bound]
+        serve = {"requests_per_point": args.serve_requests,
                  "hbm_budget_gb": args.serve_budget_gb, "tick_steps": 50,
-                 "continuous": serve_run(None)}
+                 "mix": "70 % random voxel mass-spring robots (horizons 100-1000), 30 % MPM "
+                        "bodies of 8k-60k particles on 32^3 / 64^3 grids (horizons 200-2000), "
+                        "half of each with gradients",
+                 "sweep": sweep, "p99_bound_s": args.serve_p99_bound,
+                 "max_rate_within_p99_bound": max(ok) if ok else None,
+                 "sustained_req_s": max(p["throughput_req_s"] for p in sweep)}
         if not args.skip_serial:
-            serve["one_at_a_time"] = serve_run(1)
+            # one request at a time at the highest rate (both forms work-bound)
+            one = serve_run(rates[-1], 1)
+            serve["one_at_a_time"] = one
+            serve["throughput_vs_one_at_a_time"] = (sweep[-1]["throughput_req_s"]
+                                                    / one["throughput_req_s"])
 
     # ---- CPU baseline: the C/OpenMP restatement (FP32) and the FP64 torch oracle
     cpu = None

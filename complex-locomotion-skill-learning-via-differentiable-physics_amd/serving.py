@@ -17,9 +17,10 @@ per tick:
   request's slot is fixed for its lifetime, so its result does not depend
   on what else is in the batch.  Robots that fit the wave-per-robot kernel
   and the rest use two pools.
-* MPM requests each own an ``MPMBatch`` (one robot, its own grid) and
-  advance one checkpoint segment per tick on a side stream, overlapping the
-  mass-spring launches.
+* MPM requests are staged per grid size and start together as one uneven
+  ``MPMBatch`` (every robot its own grid, body, controller and horizon),
+  which advances one checkpoint segment per tick on its own stream,
+  overlapping the mass-spring launches.
 
 The reference has no serving path; the closest is the batch of tasks that
 ``DiffPhyTrainer.simulate`` advances together (``diffphy_trainer.py:219-268``).
@@ -296,100 +297,128 @@ class MSPool:
         return req
 
 
-class MPMRunner:
-    """MPM requests, each its own one-robot MPMBatch on its own stream,
-    advanced one checkpoint segment per tick (jobs overlap on the device)."""
+class MPMPool:
+    """MPM requests on one grid size, packed into shared uneven MPMBatch
+    launches.  Admitted requests are staged; at a tick, when fewer than
+    ``max_cohorts`` batches are running, the staged requests (up to
+    ``max_robots``, in admission order) start together as one batch: every
+    robot on its own grid with its own body (padded to the largest),
+    controller, target and horizon.  A tick advances every running batch by
+    one checkpoint segment (forward, then the adjoint when some request of
+    the batch wants gradients), each batch on its own stream; forward-only
+    requests complete when their batch's forward does."""
 
-    def __init__(self, max_active=8, ckpt_every=25, dtype=torch.float32, device="cuda"):
-        self.max_active, self.K = max_active, ckpt_every
+    def __init__(self, n_grid, max_robots=8, max_cohorts=2, ckpt_every=25, window=100,
+                 max_particles=60000, dtype=torch.float32, device="cuda"):
+        self.n_grid, self.max_robots, self.max_cohorts = n_grid, max_robots, max_cohorts
+        self.K, self.window, self.max_particles = ckpt_every, window, max_particles
         self.dtype, self.device = dtype, device
-        self.jobs = []
-        self.stream = torch.cuda.Stream(device=device)          # joins the job streams
-        self.free_streams = [torch.cuda.Stream(device=device) for _ in range(max_active)]
+        self.staged, self.cohorts = [], []
+        self.stream = torch.cuda.Stream(device=device)          # joins the batch streams
+        self.free_streams = [torch.cuda.Stream(device=device) for _ in range(max_cohorts)]
 
     def fits(self, req):
-        return req.kind == "mpm"
+        return (req.kind == "mpm" and req.mpm["params"].n_grid == self.n_grid
+                and len(req.mpm["positions"]) <= self.max_particles)
 
     def has_slot(self):
-        return len(self.jobs) < self.max_active
+        return (len(self.staged) + sum(len(c["reqs"]) for c in self.cohorts)
+                < self.max_robots * self.max_cohorts)
 
     def busy(self):
-        return any(j["phase"] in ("fwd", "bwd") for j in self.jobs)
+        return bool(self.staged) or any(c["phase"] in ("fwd", "bwd") for c in self.cohorts)
 
     @staticmethod
-    def estimate_bytes(req, ckpt_every=25, es=4):
-        """Device bytes of a one-robot MPMBatch: state slots (ring, checkpoints,
-        segment, adjoint) of 24 reals per particle, four 16-byte grid records
-        per node, block tables, per-checkpoint particle orders."""
-        m = req.mpm
-        P, n = len(m["positions"]), m["params"].n_grid
-        nck = (req.horizon + ckpt_every - 1) // ckpt_every
+    def estimate_bytes(P, n, horizon, ckpt_every=25, es=4):
+        """Device bytes of one robot of an MPMBatch with P particle slots on an
+        n^3 grid: state slots (ring, checkpoints, segment, adjoint) of 24 reals
+        per particle, four 16-byte grid records per node, the segment's saved
+        grids, block tables, per-checkpoint particle orders."""
+        nck = (horizon + ckpt_every - 1) // ckpt_every
         slots = 2 + nck + ckpt_every + 1 + 2
         nblk = (n // 4) ** 3
-        saved = ckpt_every * nblk * 64 * 4 * 2           # per-step grids of a segment
-        reorder = 2 * 8 * nck * P                        # per-checkpoint particle orders
+        saved = ckpt_every * nblk * 64 * 4 * 2
+        reorder = 2 * 8 * nck * P
         return (es * (slots * 24 * P + 4 * 4 * n ** 3 + saved) + 4 * (3 + ckpt_every) * nblk
                 + reorder)
 
     def bytes_for(self, req):
-        return self.estimate_bytes(req, self.K, torch.finfo(self.dtype).bits // 8)
+        # a robot's slots are padded to the largest body of its batch
+        return self.estimate_bytes(self.max_particles, self.n_grid, req.horizon, self.K,
+                                   torch.finfo(self.dtype).bits // 8)
 
     def admit(self, req):
+        self.staged.append(req)
+
+    def active(self):
+        return self.staged + [r for c in self.cohorts for r in c["reqs"]]
+
+    def _start(self):
         from .mpm import MPMBatch
-        m = req.mpm
+        reqs, self.staged = self.staged[:self.max_robots], self.staged[self.max_robots:]
+        m0 = reqs[0].mpm
+        B, T = len(reqs), max(r.horizon for r in reqs)
         st = self.free_streams.pop()
         st.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(st):
-            b = MPMBatch(m["params"], m["positions"], m["act_id"], m["n_act"], 1, req.horizon,
-                         save_horizon=False,
-                         K=self.K, window=min(100, req.horizon), dtype=self.dtype,
-                         device=self.device)
-            b.set_targets(np.asarray(req.target)[None, :3], np.asarray(req.target)[3:4])
-            b.set_weights(*req.weights)
+            b = MPMBatch(m0["params"], [r.mpm["positions"] for r in reqs],
+                         [r.mpm["act_id"] for r in reqs], m0["n_act"], B, T, K=self.K,
+                         window=self.window, dtype=self.dtype, device=self.device,
+                         save_horizon=False, horizons=[r.horizon for r in reqs],
+                         per_robot_weights=True)
+            tg = np.stack([np.asarray(r.target, np.float64) for r in reqs])
+            b.set_targets(tg[:, :3], tg[:, 3])
+            for k, r in enumerate(reqs):
+                b.set_robot_weights(k, *r.weights)
             b.start_forward()
-        self.jobs.append({"req": req, "b": b, "phase": "fwd", "seg": 0, "stream": st})
-
-    def active(self):
-        return self.jobs
+        self.cohorts.append({"reqs": list(reqs), "all": list(reqs), "b": b, "phase": "fwd",
+                             "seg": 0, "stream": st})
 
     def tick(self, chunk=None):
         finished = []
         cur = torch.cuda.current_stream()
-        for j in self.jobs:
-            if j["phase"] not in ("fwd", "bwd"):
+        while self.staged and self.free_streams:
+            self._start()
+        for c in self.cohorts:
+            if c["phase"] not in ("fwd", "bwd"):
                 continue
-            st, b = j["stream"], j["b"]
+            st, b = c["stream"], c["b"]
             st.wait_stream(cur)
             with torch.cuda.stream(st):
-                if j["phase"] == "fwd":
-                    b.forward_segment(j["seg"])
-                    j["seg"] += 1
-                    if j["seg"] == b.nck:
+                if c["phase"] == "fwd":
+                    b.forward_segment(c["seg"])
+                    c["seg"] += 1
+                    if c["seg"] == b.nck:
                         b.finish_forward()
-                        if j["req"].grad:
-                            b.start_backward()
-                            j["phase"], j["seg"] = "bwd", b.nck - 1
+                        finished += [(c, r) for r in c["all"] if not r.grad]
+                        want = [float(r.grad) for r in c["all"]]
+                        if any(want):
+                            b.start_backward(loss_grad=torch.tensor(want, dtype=torch.float64))
+                            c["phase"], c["seg"] = "bwd", b.nck - 1
                         else:
-                            j["phase"] = "done"
-                            finished.append(j)
+                            c["phase"] = "done"
                 else:
-                    b.backward_segment(j["seg"])
-                    j["seg"] -= 1
-                    if j["seg"] < 0:
+                    b.backward_segment(c["seg"])
+                    c["seg"] -= 1
+                    if c["seg"] < 0:
                         b.finish_backward()
-                        j["phase"] = "done"
-                        finished.append(j)
+                        c["phase"] = "done"
+                        finished += [(c, r) for r in c["all"] if r.grad]
             self.stream.wait_stream(st)
         return finished
 
-    def collect(self, j):
-        req, b = j["req"], j["b"]
+    def collect(self, item):
+        c, req = item
+        b = c["b"]
         b.check()
-        req.loss = float(b.loss.item())
+        k = c["all"].index(req)
+        req.loss = float(b.loss_b[k].item())
         if req.grad:
-            req.grads = [g.clone() for g in b.grads]
-        self.jobs.remove(j)
-        self.free_streams.append(j["stream"])
+            req.grads = [g[k].clone() for g in b.grads]
+        c["reqs"].remove(req)
+        if not c["reqs"]:
+            self.cohorts.remove(c)
+            self.free_streams.append(c["stream"])
         return req
 
 
@@ -406,7 +435,7 @@ class CudaEvents:
 
 
 class Server:
-    """Continuous-batching loop over executors (MSPool, MPMRunner) with an
+    """Continuous-batching loop over executors (MSPool, MPMPool) with an
     HBM admission budget.  Each executor ticks at its own pace: a tick is
     launched as soon as the executor's previous one has completed on the
     device, so mass-spring slots are not held back by MPM segments.
@@ -513,7 +542,7 @@ def mpm_particle_robot(rng, n_grid, n_particles, n_act=4, ppc_edge=2):
     actuator groups = nearest of n_act seed cubes."""
     c = 3 if n_grid <= 32 else 4
     per_cube = c ** 3 * ppc_edge ** 3
-    n_cubes = max(1, int(round(n_particles / per_cube)))
+    n_cubes = max(1, n_particles // per_cube)        # at most n_particles
     box = (n_grid - 8) // c                      # cubes per axis, 4 cells margin
     cells = [(0, 0, 0)]
     have = {(0, 0, 0)}
@@ -539,7 +568,10 @@ def mpm_particle_robot(rng, n_grid, n_particles, n_act=4, ppc_edge=2):
     sub = (np.arange(ppc_edge) + 0.5) / ppc_edge
     local = np.stack(np.meshgrid(*[np.arange(c)] * 3, indexing="ij"), -1).reshape(-1, 3)
     subs = np.stack(np.meshgrid(sub, sub, sub, indexing="ij"), -1).reshape(-1, 3)
-    pos = ((cells[:, None, None, :] * c + local[None, :, None, :] + subs[None, None] + 4.0) * dx)
+    # 4 cells from the side walls; the bottom cubes start on the floor (the
+    # sticky boundary's 3 cells), so the body is in contact from step 0
+    lift = np.array([4.0, 3.0, 4.0])
+    pos = ((cells[:, None, None, :] * c + local[None, :, None, :] + subs[None, None] + lift) * dx)
     pos = pos.reshape(-1, 3)
     ids = np.repeat(grp, c ** 3 * ppc_edge ** 3).astype(np.int32)
     return pos, ids, n_act
@@ -547,19 +579,21 @@ def mpm_particle_robot(rng, n_grid, n_particles, n_act=4, ppc_edge=2):
 
 def make_requests(rate, duration, seed, layout, frac_mpm=0.3, grad_frac=0.5, n_hidden=64,
                   ms_horizon=(100, 1000), mpm_horizon=(200, 2000), mpm_particles=(8000, 60000),
-                  device="cuda"):
-    """Poisson arrivals at `rate` per second over `duration` seconds: a
-    fraction frac_mpm of MPM bodies (config-3 physics, 32^3 or 64^3 grids),
-    the rest random voxel mass-spring robots (the 512-robot generator's
-    distribution), half of each wanting gradients."""
+                  device="cuda", count=None):
+    """Poisson arrivals at `rate` per second over `duration` seconds (or
+    exactly `count` arrivals): a fraction frac_mpm of MPM bodies (the soft
+    quadruped's physics, 32^3 or 64^3 grids), the rest random voxel
+    mass-spring robots (the 512-robot generator's distribution), half of
+    each wanting gradients.  The robots drawn do not depend on `rate` (its
+    inter-arrival draws scale one unit-rate sequence)."""
     from .mpm import quadruped3d_params
     from .robot_design import random_voxel_cubes, build_voxels
     from .rollout import init_weights
     rng = np.random.default_rng(seed)
     t, reqs = 0.0, []
     while True:
-        t += rng.exponential(1.0 / rate)
-        if t > duration:
+        t += rng.exponential(1.0) / rate
+        if (count is None and t > duration) or (count is not None and len(reqs) >= count):
             break
         rid = len(reqs)
         grad = bool(rng.random() < grad_frac)
@@ -599,6 +633,11 @@ def pool_caps(layout):
             "block": (1024, 8192, n_inputs(layout, 1024), 64, 1000)}
 
 
+# particle capacity of the MPM pools per grid size (the request generator's
+# ranges: up to 30k particles on 32^3 grids, 60k on 64^3)
+MPM_POOL_PARTICLES = {32: 30000, 64: 60000}
+
+
 def fits_caps(design, horizon, layout, caps):
     N, S, I, O, T = caps
     return (design.n_vert <= N and design.n_spring <= S and n_inputs(layout, design.n_vert) <= I
@@ -610,9 +649,10 @@ def default_executors(physics, layout, slots_small=256, slots_large=128, mpm_act
                       stored=True):
     """Two mass-spring pools (wave-per-robot kernel for small robots, block
     kernel up to 1024 vertices / 8192 springs / 64 actuators / 1000 steps)
-    and the MPM runner.  With `stored` the pools' forwards keep every step's
-    adjoint rows (variants 4 and 5), else checkpoints every ckpt_every steps
-    are recomputed (variants 3 and 1)."""
+    and two MPM pools (32^3 and 64^3 grids, batches of up to mpm_active
+    robots).  With `stored` the mass-spring pools' forwards keep every
+    step's adjoint rows (variants 4 and 5), else checkpoints every
+    ckpt_every steps are recomputed (variants 3 and 1)."""
     caps = pool_caps(layout)
     small = MSPool("wave", slots_small, caps["wave"],
                    physics, layout, ckpt_every, loss_window, dtype=dtype,
@@ -620,4 +660,6 @@ def default_executors(physics, layout, slots_small=256, slots_large=128, mpm_act
     large = MSPool("block", slots_large, caps["block"], physics,
                    layout, ckpt_every, loss_window, dtype=dtype, variant=5 if stored else 1,
                    block=256, device=device)
-    return [small, large, MPMRunner(mpm_active, dtype=dtype, device=device)]
+    return [small, large] + [MPMPool(n, mpm_active, max_particles=cap, window=loss_window,
+                                     dtype=dtype, device=device)
+                             for n, cap in MPM_POOL_PARTICLES.items()]

@@ -141,6 +141,24 @@ def test_every_generated_robot_fits_a_pool():
     assert biggest > 800          # the old 800-vertex cap would have refused some
 
 
+def test_every_generated_mpm_body_fits_a_pool():
+    """The request generator's MPM bodies (8k-60k particles, 30k at most on
+    32^3 grids) never exceed their grid size's pool capacity, and stay off
+    the side walls while resting on the floor."""
+    import diffloco as dl
+    from diffloco.taichi_env import sensor_layout
+    lay = sensor_layout(dl.quadruped_config(batch_size=1).get_config())
+    reqs = sv.make_requests(50, None, seed=4, layout=lay, count=300, device="cpu")
+    mpm = [r for r in reqs if r.kind == "mpm"]
+    assert len(mpm) > 50
+    for r in mpm:
+        n = r.mpm["params"].n_grid
+        pos = r.mpm["positions"]
+        assert len(pos) <= sv.MPM_POOL_PARTICLES[n]
+        cell = pos * n
+        assert cell[:, 1].min() < 3.5 and cell.min() >= 3.0 and cell.max() <= n - 3.0
+
+
 @pytest.mark.gpu
 def test_results_same_alone_and_in_the_mix():
     import diffloco as dl
@@ -172,8 +190,9 @@ def test_results_same_alone_and_in_the_mix():
 
     A = ms_req(0, 6, 4, 130, True)          # small robot (wave pool)
     B = ms_req(1, 60, 10, 170, True)        # large robot (block pool)
-    C = mpm_req(2, 60)
+    C = mpm_req(2, 150)
     D = ms_req(3, 5, 4, 120, False)
+    E = mpm_req(4, 100)                     # shares C's MPM batch in the mix
 
     def serve(reqs):
         ex = sv.default_executors(phys, lay, slots_small=8, slots_large=4, mpm_active=2)
@@ -183,7 +202,7 @@ def test_results_same_alone_and_in_the_mix():
         return {r.rid: (r.loss, [g.clone() for g in r.grads] if r.grads else None) for r in out}
 
     alone = {**serve([A]), **serve([B]), **serve([C])}
-    mix = serve([A, B, C, D])
+    mix = serve([A, B, C, D, E])
     for rid in (0, 1):
         la, ga = alone[rid]
         lm, gm = mix[rid]
@@ -192,12 +211,15 @@ def test_results_same_alone_and_in_the_mix():
             assert torch.equal(a, b), rid
     la, ga = alone[2]
     lm, gm = mix[2]
-    # MPM: FP32 atomic summation order in the scatter kernels differs run to
-    # run; after 60 steps from rest the weight gradients are ~1e-8 and carry
-    # that variation at the 1e-3 level
+    # MPM: alone, C is a one-robot batch; in the mix it shares a batch with E
+    # (its body padded to E's particle count).  FP32 atomic summation order in
+    # the scatter kernels differs run to run; the body touches the sticky
+    # floor from step 0, so its gradients are far from vanishing
+    print("mpm dW1 norm alone", ga[0].norm().item(), flush=True)
+    assert ga[0].norm() > 1e-6
     assert abs(la - lm) <= 1e-4 * abs(la)
     for a, b in zip(ga, gm):
-        assert (a - b).norm() <= 1e-2 * a.norm()
+        assert (a - b).norm() <= 1e-3 * a.norm()
     assert mix[3][0] is not None and mix[3][1] is None
 
     # the slot pool against an independent single-robot batched rollout
