@@ -50,6 +50,7 @@ MEASURED_KIB = {
 # steps, replayed segments: FETCH_SIZE and WRITE_SIZE (KiB) of one step of
 # every forward, replay and adjoint kernel, summed
 MPM_STEP_KIB = {(8, 100): (63832.0, 98505.0)}
+MPM_MEASURED_SCATTER = {"scatter": 0, "box_limit": 512, "tile_nodes": 2048, "unroll": 3}
 
 
 def measured_bytes(fetch_kib, write_kib):
@@ -166,6 +167,9 @@ def main():
     ap.add_argument("--mpm-horizon", type=int, default=2000)
     ap.add_argument("--mpm-ckpt", type=int, default=100)
     ap.add_argument("--mpm-steps", type=int, default=1)
+    ap.add_argument("--mpm-tune", type=int, default=1,
+                    help="1: the autotuner picks the MPM scatter configuration; 0: "
+                         "--mpm-scatter / --mpm-tile with box limit 512, unroll 3")
     ap.add_argument("--mpm-scatter", type=int, default=0,
                     help="0: global-atomic commits (staged run sums, LDS tile fallback), "
                          "1: tiles stored and gathered by the grid pass")
@@ -367,11 +371,25 @@ def main():
         Bm, Tm, Km = args.mpm_robots, args.mpm_horizon, args.mpm_ckpt
         pos, ids, n_act = dm.quadruped3d_particles(cells_per_cube=6, n_grid=64)
 
+        scfg = {"scatter": args.mpm_scatter, "tile_nodes": args.mpm_tile or 2048,
+                "box_limit": 512, "unroll": 3}
+        t_st, tms = 0.0, []
+        if args.mpm_tune:
+            # scatter configuration of this body and batch, by the autotuner
+            from diffloco.autotune import Autotuner
+            t5 = time.perf_counter()
+            tuner = Autotuner()
+            scfg = dm.tune_scatter(dm.quadruped3d_params(64), pos, ids, n_act, Bm,
+                                   dtype=torch.float32, K=Km, tuner=tuner)
+            t_st = time.perf_counter() - t5
+            tms = sorted(t["ms"] for e in tuner.cache.values() if "timings" in e
+                         and any("scatter" in t["candidate"] for t in e["timings"])
+                         for t in e["timings"] if "ms" in t)
+
         def mpm_run(store, save_horizon=True):
             mb = dm.MPMBatch(dm.quadruped3d_params(64), pos, ids, n_act, Bm, Tm, K=Km,
-                             window=Tm // 2, dtype=torch.float32, scatter=args.mpm_scatter,
-                             tile_nodes=args.mpm_tile, store_states=store,
-                             save_horizon=save_horizon)
+                             window=Tm // 2, dtype=torch.float32, store_states=store,
+                             save_horizon=save_horizon, **scfg)
             rngm = np.random.default_rng(7)
             mb.set_targets(np.stack([rngm.uniform(-0.08, 0.08, Bm), np.zeros(Bm),
                                      rngm.uniform(-0.08, 0.08, Bm)], 1),
@@ -420,8 +438,11 @@ def main():
         mpm = {"value": pps, "unit": "particle-steps/s", "robots": Bm, "particles": P,
                "grid": 64, "horizon": Tm, "ckpt_every": Km, "dtype": "f32",
                "scatter": ["staged run sums + per-node gather, global atomics",
-                           "tile store + gather"][args.mpm_scatter],
-               "tile_nodes": args.mpm_tile, "scatter_windows": len(mb.windows),
+                           "tile store + gather"][scfg["scatter"]],
+               "scatter_config": scfg, "scatter_tune_s": t_st,
+               "scatter_tune_ms_best_median_worst": ([tms[0], tms[len(tms) // 2], tms[-1]]
+                                                     if args.mpm_tune and tms else None),
+               "tile_nodes": scfg["tile_nodes"], "scatter_windows": len(mb.windows),
                "store_states": mb.store_states, "replayed_segments": int(sum(mb.replay)),
                "resort_every": Km,
                "ms_forward": float(np.mean(mf)), "ms_backward": float(np.mean(mbk)),
@@ -434,7 +455,7 @@ def main():
                             "traffic": (measured_bytes(*MPM_STEP_KIB[(Bm, Km)]) / (Bm * P)
                                         if (Bm, Km) in MPM_STEP_KIB and P == 27648
                                         and not mb.store_states and mb.overlap_replay
-                                        else None),
+                                        and scfg == MPM_MEASURED_SCATTER else None),
                             "traffic_unit": "bytes per particle-step, all MPM kernels"},
                "mean_loss": float(mb.loss.item()), "device_gb": mb.nbytes() / 1e9}
         if not args.skip_alt:

@@ -1023,3 +1023,62 @@ def mpm_rollout_loss(batch, W1, b1, W2, b2):
     ``MPMBatch.backward`` (segment recompute + hand-written adjoints).  The
     per-robot losses are in ``batch.loss_b``."""
     return _MPMRolloutFn.apply(batch, W1, b1, W2, b2)
+
+
+# ------------------------------------------------------------------ autotuning
+def scatter_candidates(dtype=torch.float32):
+    """Scatter configurations of the batched MPM kernels: commit mode (global
+    atomics or tile store + gather), the box limit of the scatter windows
+    (stencil-box nodes per workgroup), the LDS tile capacity (nodes; the
+    largest that fits 128 KiB of LDS in FP64 is 4096) and the run-sum unroll.
+    The first is the batch's default."""
+    tiles = (2048, 4096) if dtype == torch.float64 else (2048, 4096, 8192)
+    out = [{"scatter": SCATTER_ATOMIC, "box_limit": 512, "tile_nodes": 2048, "unroll": 3}]
+    for sc, box, tn, un in itertools.product((SCATTER_ATOMIC, SCATTER_STORE), (256, 512, 1024),
+                                             tiles, (1, 3)):
+        c = {"scatter": sc, "box_limit": box, "tile_nodes": tn, "unroll": un}
+        if c not in out and tn >= box:
+            out.append(c)
+    return out
+
+
+def tune_scatter(params, positions, act_id, n_act, B, dtype=torch.float32, K=100, steps=None,
+                 tuner=None, candidates=None, device="cuda"):
+    """Pick the scatter configuration of a B-robot batch of this body by
+    timing the forward and adjoint of ``steps`` (default 2 K) steps for every
+    candidate of ``scatter_candidates`` (best of two runs after a warm-up);
+    cached by the autotuner per body size, grid, batch, dtype and library
+    build.  Returns the chosen keyword arguments of ``MPMBatch``."""
+    from .autotune import Autotuner, library_tag
+    tuner = tuner or Autotuner()
+    cands = candidates or scatter_candidates(dtype)
+    T = steps or 2 * K
+    key = Autotuner.key(kernel="mpm_batch_scatter", P=len(positions), n_grid=params.n_grid,
+                        B=B, dtype=str(dtype), K=K, T=T, lib=library_tag())
+    rng = np.random.default_rng(0)
+    tv = np.stack([rng.uniform(-0.08, 0.08, B), np.zeros(B), rng.uniform(-0.08, 0.08, B)], 1)
+    th = rng.uniform(0.1, 0.25, B)
+
+    def measure(c):
+        mb = MPMBatch(params, positions, act_id, n_act, B, T, K=K, dtype=dtype, device=device,
+                      save_horizon=False, **c)
+        mb.set_targets(tv, th)
+        g = torch.Generator().manual_seed(1)
+        mb.set_weights(*[0.1 * torch.randn(w.shape, generator=g, dtype=torch.float64)
+                         for w in mb.W])
+        best = None
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        for it in range(3):
+            e0.record()
+            mb.forward()
+            mb.backward()
+            e1.record()
+            torch.cuda.synchronize()
+            if it:
+                best = min(best or 1e30, e0.elapsed_time(e1))
+        mb.check()
+        del mb
+        torch.cuda.empty_cache()
+        return best
+
+    return tuner.pick(key, cands, measure, repeat=1)

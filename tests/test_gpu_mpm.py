@@ -3,6 +3,7 @@
 FP32 within 1e-5 relative on positions (atomics reorder the P2G sums, the
 only run-to-run variation of this path)."""
 
+import json
 import sys
 
 import numpy as np
@@ -310,6 +311,25 @@ def test_mpm_uneven_batch_matches_robots_alone(prec):
         for a, b in zip(grads, g1):
             assert b.abs().max() > 0
             assert rel_err(a[k].double().cpu(), b.double().cpu()) < tg, k
+
+
+def test_mpm_scatter_tuner_picks_and_caches(tmp_path):
+    """The scatter autotuner times every candidate, returns the fastest, and
+    answers the same key from its cache (after one verifying run)."""
+    from diffloco.autotune import Autotuner
+    _say("mpm scatter tuner")
+    params, _, pos, ids = _batched_case()
+    cands = dm.scatter_candidates(torch.float32)[:4]
+    tuner = Autotuner(path=tmp_path / "tune.json")
+    pick = dm.tune_scatter(params, pos.numpy(), ids.numpy(), 4, 2, dtype=torch.float32, K=5,
+                           tuner=tuner, candidates=cands)
+    assert pick in cands
+    (entry,) = tuner.cache.values()
+    ms = {json.dumps(t["candidate"], sort_keys=True): t["ms"] for t in entry["timings"]}
+    assert len(ms) == len(cands) and ms[json.dumps(pick, sort_keys=True)] == min(ms.values())
+    again = dm.tune_scatter(params, pos.numpy(), ids.numpy(), 4, 2, dtype=torch.float32, K=5,
+                            tuner=Autotuner(path=tmp_path / "tune.json"), candidates=cands)
+    assert again == pick
 
 
 @pytest.mark.parametrize("hblocks,shuffled", [(None, False), (1, False), (None, True)])
