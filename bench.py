@@ -491,7 +491,7 @@ def main():
     uneven = None
     if not args.skip_uneven:
         from diffloco import robot_design as rdm
-        from diffloco.uneven import UnevenBatch, n_inputs
+        from diffloco.uneven import UnevenBatch, n_inputs, tune_schedule
         designs_u, hz = rdm.random_robot_batch(args.uneven_robots, seed=3)
         Ru = len(designs_u)
         rngu = np.random.default_rng(4)
@@ -505,12 +505,22 @@ def main():
         phys_u = _physics(qc)
         phys_u.dashpot = UNEVEN_DASHPOT
 
+        wu = [init_weights(n_inputs(lay, d.n_vert), 64, d.n_act, seed=1000 + r)
+              for r, d in enumerate(designs_u)]
+
+        def load_weights(ub):
+            for r in range(Ru):
+                ub.set_weights(r, *wu[r])
+
         def uneven_run(stored):
-            ub = UnevenBatch(designs_u, hz, phys_u, lay, tgu, ckpt_every=50, loss_window=100,
-                             stored=stored)
-            for r, d in enumerate(designs_u):
-                ub.set_weights(r, *init_weights(n_inputs(lay, d.n_vert), 64, d.n_act, seed=1000 + r))
+            # robot-to-launch schedule (size-class boundaries), then each
+            # class's launch shape, both by the autotuner
             t_tune = time.perf_counter()
+            bounds, sched = tune_schedule(designs_u, hz, phys_u, lay, tgu, set_weights=load_weights,
+                                          ckpt_every=50, loss_window=100, stored=stored)
+            ub = UnevenBatch(designs_u, hz, phys_u, lay, tgu, ckpt_every=50, loss_window=100,
+                             stored=stored, bounds=bounds)
+            load_weights(ub)
             ub.tune()
             t_tune = time.perf_counter() - t_tune
             ub.prepare(np.full(Ru, 1.0 / Ru))
@@ -528,6 +538,8 @@ def main():
                 ub.replay()
             torch.cuda.synchronize()
             graph_ms = (time.perf_counter() - t3) * 1e3 / args.uneven_iters
+            ub.prepare(np.full(Ru, 1.0 / Ru))
+            per_class = ub.class_times()
             rs = ub.robot_steps
             ubytes = 0.0
             for d, T_r in zip(designs_u, hz):
@@ -542,8 +554,12 @@ def main():
                       "horizon": [int(min(hz)), int(max(hz))], "dtype": "f32",
                       "ms_per_iteration_graph": graph_ms, "ms_per_iteration_eager": eager_ms,
                       "autotune_s": t_tune,
-                      "classes": [{"name": bk["name"], "robots": len(bk["ids"]), "launch": bk["config"]}
+                      "classes": [{"name": bk["name"], "robots": len(bk["ids"]), "launch": bk["config"],
+                                   "ms_alone": per_class[bk["name"]]}
                                   for bk in ub.buckets],
+                      "schedule": {"bounds": list(bounds),
+                                   "candidates_ms": [(t["candidate"]["bounds"], t.get("ms"))
+                                                     for t in (sched or [])]},
                       "roofline": {"bound": "hbm", "achieved": ubytes / (graph_ms * 1e-3) / 1e9,
                                    "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                    "frac": ubytes / (graph_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
@@ -560,7 +576,7 @@ def main():
             ua = uneven_run(not uneven["stored"])
             uneven["checkpointed" if uneven["stored"] else "stored_rows"] = {
                 k: ua[k] for k in ("value", "stored", "ms_per_iteration_graph", "autotune_s",
-                                   "classes", "nonfinite_robots", "mean_loss")}
+                                   "classes", "schedule", "nonfinite_robots", "mean_loss")}
 
     # ---- request loop: Poisson arrivals, 70 % random mass-spring robots, 30 % MPM bodies
     serve = None

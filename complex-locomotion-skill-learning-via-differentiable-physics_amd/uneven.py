@@ -103,10 +103,16 @@ class UnevenBatch:
             return 4 if self.stored else 3
         return 5 if self.stored else 1
 
-    def candidates(self, name):
+    def candidates(self, name, quick=False):
+        """Launch shapes of a size class; ``quick``: the two that win on the
+        random-robot batches (block 256 / 512 with the LDS policy 4), for
+        the schedule search."""
         if name == "wave":
             vs = (4,) if self.stored else (3, 2)
-            return [{"variant": v, "block": 64, "lds_policy": 0} for v in vs]
+            return [{"variant": v, "block": 64, "lds_policy": 0} for v in vs][:1 if quick else 2]
+        if quick:
+            return [{"variant": self._variant(name), "block": b, "lds_policy": 4}
+                    for b in (256, 512)]
         return [{"variant": self._variant(name), "block": b, "lds_policy": p}
                 for b in (64, 128, 256, 512) for p in (0, 1, 4)]
 
@@ -115,7 +121,7 @@ class UnevenBatch:
         b.variant, b.block, b.lds_policy = cfg["variant"], cfg["block"], cfg["lds_policy"]
         bk["config"] = cfg
 
-    def tune(self, tuner=None):
+    def tune(self, tuner=None, quick=False):
         """Pick each size class's launch shape by timing a full iteration of
         that class with every candidate (cached by the autotuner)."""
         tuner = tuner or Autotuner()
@@ -139,8 +145,23 @@ class UnevenBatch:
                 torch.cuda.synchronize()
                 return e0.elapsed_time(e1)
 
-            self._apply(bk, tuner.pick(key, self.candidates(bk["name"]), measure))
+            self._apply(bk, tuner.pick(key + f"|quick={int(quick)}",
+                                       self.candidates(bk["name"], quick), measure))
         return [bk["config"] for bk in self.buckets]
+
+    def class_times(self):
+        """Milliseconds of one forward + adjoint iteration of each size class
+        run alone (after prepare), by class name."""
+        out = {}
+        for bk in self.buckets:
+            bk["batch"].run_iteration()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            bk["batch"].run_iteration()
+            e1.record()
+            torch.cuda.synchronize()
+            out[bk["name"]] = e0.elapsed_time(e1)
+        return out
 
     # ------------------------------------------------------------ iterations
     def prepare(self, loss_grad=None):
@@ -181,3 +202,43 @@ class UnevenBatch:
 
     def replay(self):
         self._on_streams([g.replay for g in self.graphs])
+
+
+# Size-class boundaries (vertex counts) the schedule search tries: which
+# robots share a launch, and so which kernel and launch shape each robot
+# runs with.  The first is the default.
+SCHEDULES = ((128, 400), (96, 256, 512), (160, 480), (200,), (128, 256, 512))
+
+
+def tune_schedule(designs, horizons, physics, layout, targets, set_weights=None, tuner=None,
+                  schedules=SCHEDULES, **kw):
+    """Robot-to-launch schedule of an uneven batch: for every class-boundary
+    candidate, build the batch, pick each class's launch shape among the
+    quick candidates, and time a whole iteration (classes concurrent);
+    returns the fastest boundaries (cached by the autotuner per robot set
+    and library build).  ``set_weights(batch)`` loads the controllers."""
+    tuner = tuner or Autotuner()
+    nv = np.array([d.n_vert for d in designs])
+    key = Autotuner.key(kernel="ms_uneven_schedule", robots=len(designs), sum_n=int(nv.sum()),
+                        max_n=int(nv.max()), sum_T=int(np.sum(horizons)),
+                        stored=int(kw.get("stored", True)), lib=library_tag())
+
+    def measure(c):
+        ub = UnevenBatch(designs, horizons, physics, layout, targets, bounds=tuple(c["bounds"]),
+                         **kw)
+        if set_weights is not None:
+            set_weights(ub)
+        ub.tune(tuner, quick=True)
+        ub.prepare()
+        ub.run_iteration()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        ub.run_iteration()
+        e1.record()
+        torch.cuda.synchronize()
+        del ub
+        torch.cuda.empty_cache()
+        return e0.elapsed_time(e1)
+
+    pick = tuner.pick(key, [{"bounds": list(b)} for b in schedules], measure, repeat=1)
+    return tuple(pick["bounds"]), tuner.cache.get(key, {}).get("timings")

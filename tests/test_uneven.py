@@ -135,3 +135,39 @@ def test_uneven_iteration_and_graph_replay_match_single_launch(tmp_path, stored)
         for r in range(R):
             for a, b in zip(ub.weight_grads(r), grads_e[r]):
                 assert torch.equal(a, b), (rep, r)
+
+
+@pytest.mark.gpu
+def test_schedule_search_picks_bounds_and_results_do_not_depend_on_them(tmp_path):
+    """The robot-to-launch schedule search times every class-boundary
+    candidate and returns the fastest; the losses and gradients do not
+    depend on which boundaries (so which kernels and launch shapes) run."""
+    from diffloco.rollout import init_weights
+    from diffloco.uneven import UnevenBatch, n_inputs, tune_schedule
+    print("uneven schedule search", flush=True)
+    designs, horizons, phys, lay, tg = _setup(n=12, cube_range=(2, 60))
+    W = [init_weights(n_inputs(lay, d.n_vert), 64, d.n_act, seed=200 + r)
+         for r, d in enumerate(designs)]
+
+    def load(ub):
+        for r in range(len(designs)):
+            ub.set_weights(r, *W[r])
+
+    cands = ((128, 400), (60,))
+    kw = dict(ckpt_every=16, loss_window=30, stored=False)
+    bounds, timings = tune_schedule(designs, horizons, phys, lay, tg, set_weights=load,
+                                    tuner=Autotuner(tmp_path / "t.json"), schedules=cands, **kw)
+    assert bounds in cands and len([t for t in timings if "ms" in t]) == 2
+    runs = []
+    for b in cands:
+        ub = UnevenBatch(designs, horizons, phys, lay, tg, bounds=b, **kw)
+        load(ub)
+        ub.prepare()
+        ub.run_iteration()
+        torch.cuda.synchronize()
+        runs.append((ub.losses().double().cpu(),
+                     [[g.double().cpu() for g in ub.weight_grads(r)] for r in range(len(designs))]))
+    assert rel_err(runs[0][0], runs[1][0]) < 1e-4
+    for r in range(len(designs)):
+        for a, b in zip(runs[0][1][r], runs[1][1][r]):
+            assert rel_err(a, b) < 1e-3, r
