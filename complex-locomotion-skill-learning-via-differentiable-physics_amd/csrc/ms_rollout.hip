@@ -378,7 +378,7 @@ __device__ void fwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int t, in
     __syncthreads();
 }
 
-// LB: the largest workgroup the instantiation is launched with (256 or 512).
+// LB: the largest workgroup the instantiation is launched with (256, 512, or 1024 in FP32).
 // ROWS: also write every step's adjoint rows (slot = step; K >= horizon).
 template <typename T, int LB, bool ROWS>
 __global__ void __launch_bounds__(LB) k_rollout_fwd(RArgs<T> A) {
@@ -1882,8 +1882,9 @@ static int check_rollout(const dl_rollout *r, bool bwd) {
     DL_CHECK_ARG(r->n_robots > 0 && r->n_robots <= 65535 * 1024, "rollout: bad robot count");
     DL_CHECK_ARG(r->n_hidden > 0 && r->n_hidden <= HMAX, "rollout: hidden width must be 1..%d", HMAX);
     DL_CHECK_ARG(r->ckpt_every > 0, "rollout: checkpoint interval must be positive");
-    DL_CHECK_ARG(r->block >= 64 && r->block <= 512 && r->block % 64 == 0,
-                 "rollout: block must be a multiple of 64 in [64, 512]");
+    DL_CHECK_ARG(r->block >= 64 && r->block % 64 == 0 &&
+                     (r->block <= 512 || (r->block == 1024 && r->dtype == DL_F32)),
+                 "rollout: block must be a multiple of 64 in [64, 512] (or 1024 in FP32)");
     DL_CHECK_ARG(r->max_n > 0 && r->max_s >= 0 && r->max_i > 0 && r->max_o > 0,
                  "rollout: bad size maxima");
     DL_CHECK_ARG(r->lds_policy >= 0 && r->lds_policy <= 4, "rollout: bad LDS policy");
@@ -2000,13 +2001,30 @@ static int launch_rollout(const dl_rollout *r, bool bwd, hipStream_t st) {
     A.topo_in_lds = topo;
     A.w_in_lds = wl;
     const bool big = r->block > 256, rows = variant == 5;
+    const dim3 grid(r->n_robots), block(r->block);
+    if constexpr (std::is_same_v<T, float>) {
+        // 16 wavefronts per robot (FP32 only: the register budget per lane is
+        // 128 at this size)
+        if (r->block > 512) {
+            const void *kh = bwd ? (rows ? (const void *)k_rollout_bwd<T, 1024, true>
+                                         : (const void *)k_rollout_bwd<T, 1024, false>)
+                                 : (rows ? (const void *)k_rollout_fwd<T, 1024, true>
+                                         : (const void *)k_rollout_fwd<T, 1024, false>);
+            DL_CHECK_HIP(ensure_dynamic_lds(kh, sh));
+            if (bwd && rows) hipLaunchKernelGGL((k_rollout_bwd<T, 1024, true>), grid, block, sh, st, A);
+            else if (bwd) hipLaunchKernelGGL((k_rollout_bwd<T, 1024, false>), grid, block, sh, st, A);
+            else if (rows) hipLaunchKernelGGL((k_rollout_fwd<T, 1024, true>), grid, block, sh, st, A);
+            else hipLaunchKernelGGL((k_rollout_fwd<T, 1024, false>), grid, block, sh, st, A);
+            DL_CHECK_LAUNCH();
+            return 0;
+        }
+    }
     const void *kf =
         bwd ? (big ? (rows ? (const void *)k_rollout_bwd<T, 512, true> : (const void *)k_rollout_bwd<T, 512, false>)
                    : (rows ? (const void *)k_rollout_bwd<T, 256, true> : (const void *)k_rollout_bwd<T, 256, false>))
             : (big ? (rows ? (const void *)k_rollout_fwd<T, 512, true> : (const void *)k_rollout_fwd<T, 512, false>)
                    : (rows ? (const void *)k_rollout_fwd<T, 256, true> : (const void *)k_rollout_fwd<T, 256, false>));
     DL_CHECK_HIP(ensure_dynamic_lds(kf, sh));
-    const dim3 grid(r->n_robots), block(r->block);
     if (bwd && big && rows) hipLaunchKernelGGL((k_rollout_bwd<T, 512, true>), grid, block, sh, st, A);
     else if (bwd && big) hipLaunchKernelGGL((k_rollout_bwd<T, 512, false>), grid, block, sh, st, A);
     else if (bwd && rows) hipLaunchKernelGGL((k_rollout_bwd<T, 256, true>), grid, block, sh, st, A);

@@ -104,17 +104,18 @@ class UnevenBatch:
         return 5 if self.stored else 1
 
     def candidates(self, name, quick=False):
-        """Launch shapes of a size class; ``quick``: the two that win on the
-        random-robot batches (block 256 / 512 with the LDS policy 4), for
-        the schedule search."""
+        """Launch shapes of a size class; ``quick``: block sizes 256 and up
+        with the LDS policy 4 (what wins on the random-robot batches), for the
+        schedule search."""
         if name == "wave":
             vs = (4,) if self.stored else (3, 2)
             return [{"variant": v, "block": 64, "lds_policy": 0} for v in vs][:1 if quick else 2]
+        blocks = (256, 512, 1024) if self.dtype == torch.float32 else (256, 512)
         if quick:
             return [{"variant": self._variant(name), "block": b, "lds_policy": 4}
-                    for b in (256, 512)]
+                    for b in blocks]
         return [{"variant": self._variant(name), "block": b, "lds_policy": p}
-                for b in (64, 128, 256, 512) for p in (0, 1, 4)]
+                for b in (64, 128) + blocks for p in (0, 1, 4)]
 
     def _apply(self, bk, cfg):
         b = bk["batch"]

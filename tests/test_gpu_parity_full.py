@@ -272,13 +272,15 @@ def large_robots():
     return dict(robots=out, W=W, tv=tv, th=th)
 
 
-@pytest.mark.parametrize("pol", [0, 4])
-def test_uneven_large_classes_match_oracle(large_robots, pol):
-    """The block kernel (one workgroup per robot, 512 threads) on robots of
-    the 129-400 and > 400 vertex classes: LDS policy 0 (springs spilled to
-    the scratch when they do not fit, W1 read from global memory) and 4
-    (everything that fits in the whole LDS)."""
-    _say(f"uneven large classes policy={pol}")
+@pytest.mark.parametrize("pol,block,variant", [(0, 512, 1), (4, 512, 1), (4, 1024, 1),
+                                               (4, 1024, 5)])
+def test_uneven_large_classes_match_oracle(large_robots, pol, block, variant):
+    """The block kernel (one workgroup per robot, 512 or 1024 threads) on
+    robots of the 129-400 and > 400 vertex classes: LDS policy 0 (springs
+    spilled to the scratch when they do not fit, W1 read from global memory)
+    and 4 (everything that fits in the whole LDS); checkpoint segments
+    recomputed (variant 1) or every step's rows kept (variant 5)."""
+    _say(f"uneven large classes policy={pol} block={block} variant={variant}")
     L = large_robots
     designs = [dl.build_voxels(r["cubes"], n_act=r["n_act"]) for r in L["robots"]]
     nv = [d.n_vert for d in designs]
@@ -290,8 +292,8 @@ def test_uneven_large_classes_match_oracle(large_robots, pol):
     x0 = [d.objects + np.array([0.4, 0.0, 0.0]) for d in designs]
     batch = MassSpringBatch(designs, list(range(len(designs))), [r["T"] for r in L["robots"]],
                             phys, sensor_layout(qc), tg, x0, act="tanh", ckpt_every=50,
-                            dtype=torch.float32, loss_window=L["W"], variant=1, block=512,
-                            lds_policy=pol)
+                            dtype=torch.float32, loss_window=L["W"], variant=variant,
+                            block=block, lds_policy=pol)
     wd = [[w.to("cuda", torch.float32).requires_grad_() for w in r["w64"]] for r in L["robots"]]
     lossv = rollout_loss(batch, wd)
     lossv.sum().backward()
