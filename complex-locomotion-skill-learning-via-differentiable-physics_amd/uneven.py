@@ -110,7 +110,11 @@ class UnevenBatch:
         if name == "wave":
             vs = (4,) if self.stored else (3, 2)
             return [{"variant": v, "block": 64, "lds_policy": 0} for v in vs][:1 if quick else 2]
-        blocks = (256, 512, 1024) if self.dtype == torch.float32 else (256, 512)
+        # 1024-thread blocks (FP32) win for the largest robots alone but lose
+        # in the concurrent iteration: a workgroup that needs a whole CU waits
+        # for the other classes' workgroups to leave one (measured: 246 ->
+        # 360 ms for the 512 random robots), so the tuner does not offer them
+        blocks = (256, 512)
         if quick:
             return [{"variant": self._variant(name), "block": b, "lds_policy": 4}
                     for b in blocks]
