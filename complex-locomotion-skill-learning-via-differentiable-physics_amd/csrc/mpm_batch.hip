@@ -368,6 +368,45 @@ __device__ __forceinline__ T run_sum(T v, const RunMask<T> &m) {
     return v;
 }
 
+// Run sums of NC channels.  The compiler emits each step above as a DPP move
+// plus an FMA (the DPP combiner does not fold into v_fmac, whose accumulator
+// is tied); in FP32 the 3- and 4-channel forms issue each step as one
+// v_fmac_f32 whose first operand is the row-shifted register itself
+// (v[i] += m_n[i] * v[i - n], rows without a source lane read 0, as above):
+// the same single-rounding FMA, so the results are bit for bit those of
+// run_sum.  The channels interleave, so a register is read through DPP at
+// least two instructions after its last write (the two wait states a DPP
+// read needs after a VALU write); the leading s_nop 4 covers the five that
+// follow a write of EXEC.  Every lane must be active.
+template <typename T, int NC>
+__device__ __forceinline__ void run_sums(T (&v)[NC], const RunMask<T> &m) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] = run_sum(v[c], m);
+}
+#define DL_FMAC_ROW(R, M, N) \
+    "v_fmac_f32_dpp %" #R ", %" #R ", %" #M " row_shr:" #N " row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+template <>
+__device__ __forceinline__ void run_sums<float, 4>(float (&v)[4], const RunMask<float> &m) {
+    asm volatile("s_nop 4\n"
+                 DL_FMAC_ROW(0, 4, 1) DL_FMAC_ROW(1, 4, 1) DL_FMAC_ROW(2, 4, 1) DL_FMAC_ROW(3, 4, 1)
+                 DL_FMAC_ROW(0, 5, 2) DL_FMAC_ROW(1, 5, 2) DL_FMAC_ROW(2, 5, 2) DL_FMAC_ROW(3, 5, 2)
+                 DL_FMAC_ROW(0, 6, 4) DL_FMAC_ROW(1, 6, 4) DL_FMAC_ROW(2, 6, 4) DL_FMAC_ROW(3, 6, 4)
+                 DL_FMAC_ROW(0, 7, 8) DL_FMAC_ROW(1, 7, 8) DL_FMAC_ROW(2, 7, 8) DL_FMAC_ROW(3, 7, 8)
+                 : "+v"(v[0]), "+v"(v[1]), "+v"(v[2]), "+v"(v[3])
+                 : "v"(m.m1), "v"(m.m2), "v"(m.m4), "v"(m.m8));
+}
+template <>
+__device__ __forceinline__ void run_sums<float, 3>(float (&v)[3], const RunMask<float> &m) {
+    asm volatile("s_nop 4\n"
+                 DL_FMAC_ROW(0, 3, 1) DL_FMAC_ROW(1, 3, 1) DL_FMAC_ROW(2, 3, 1)
+                 DL_FMAC_ROW(0, 4, 2) DL_FMAC_ROW(1, 4, 2) DL_FMAC_ROW(2, 4, 2)
+                 DL_FMAC_ROW(0, 5, 4) DL_FMAC_ROW(1, 5, 4) DL_FMAC_ROW(2, 5, 4)
+                 DL_FMAC_ROW(0, 6, 8) DL_FMAC_ROW(1, 6, 8) DL_FMAC_ROW(2, 6, 8)
+                 : "+v"(v[0]), "+v"(v[1]), "+v"(v[2])
+                 : "v"(m.m1), "v"(m.m2), "v"(m.m4), "v"(m.m8));
+}
+#undef DL_FMAC_ROW
+
 // Values scattered to stencil node base + off: channel c is
 // W(off) * (b[c] + sum_d L[c][d] * off[d]) for c < NL and W(off) * b[c] for
 // the rest, W(off) the product of the particle's quadratic B-spline weights.
@@ -407,7 +446,8 @@ __device__ __forceinline__ void run_offsets(const Stencil<D, T> &s, const T wz[3
             const T W = Wo * wz[oz];
             T v[NC];
 #pragma unroll
-            for (int c = 0; c < NC; ++c) v[c] = run_sum(W * m[c], rm);
+            for (int c = 0; c < NC; ++c) v[c] = W * m[c];
+            run_sums<T, NC>(v, rm);
             emit(3 * oo + oz, off, v);
 #pragma unroll
             for (int c = 0; c < NL; ++c) m[c] += z.L[c][D - 1];
