@@ -246,6 +246,91 @@ __device__ __forceinline__ T task_term(const T *cv, const T *tg) {
 
 enum { MODE_FWD = 0, MODE_REC = 1 };
 
+// W1^T [I][64] in global memory read as 16-byte rows: FP32, hidden width 64,
+// base 16-byte aligned (dl_weights_size keeps every packed block aligned).
+template <typename T>
+__device__ __forceinline__ bool w1_rows16(const Rob<T> &R, int H) {
+    if constexpr (std::is_same_v<T, float>)
+        return !R.WL && H == HMAX && (reinterpret_cast<uintptr_t>(R.W1T) & 15) == 0;
+    return false;
+}
+
+// Layer-1 pre-activations z = W1 x from global W1^T (the robots whose W1 does
+// not fit in LDS): thread (q = tid % 16, c = tid / 16) accumulates hidden
+// units 4q..4q+3 over rows i = c, c + C, ... (C = BT / 16), one 16-byte load
+// per row, so a wavefront reads 4 consecutive rows (1 KiB) per step and keeps
+// eight steps in flight.  The four row groups of a wavefront are summed by
+// lane shuffles, the wavefronts' partials in LDS (red: BT / 64 rows of 64);
+// threads tid < 64 return z for hidden unit tid (bias included).
+__device__ __forceinline__ float layer1_rows16(const float *W1T, const float *b1, const float *x,
+                                               int I, float *red) {
+    const int tid = threadIdx.x, q = tid & 15, C = blockDim.x >> 4;
+    const float4 *w4 = reinterpret_cast<const float4 *>(W1T) + q;
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+#pragma unroll 8
+    for (int i = tid >> 4; i < I; i += C) {
+        const float4 w = w4[(size_t)i * (HMAX / 4)];
+        const float xi = x[i];
+        a0 += w.x * xi; a1 += w.y * xi; a2 += w.z * xi; a3 += w.w * xi;
+    }
+#pragma unroll
+    for (int off = 16; off < 64; off <<= 1) {
+        a0 += __shfl_xor(a0, off, 64); a1 += __shfl_xor(a1, off, 64);
+        a2 += __shfl_xor(a2, off, 64); a3 += __shfl_xor(a3, off, 64);
+    }
+    if ((tid & 63) < 16) {
+        float *r = red + (tid >> 6) * HMAX + 4 * q;
+        r[0] = a0; r[1] = a1; r[2] = a2; r[3] = a3;
+    }
+    __syncthreads();
+    float z = 0.f;
+    if (tid < HMAX) {
+        z = b1[tid];
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) z += red[w * HMAX + tid];
+    }
+    return z;
+}
+
+// Input adjoint xb = W1^T z1b from global W1^T rows (as layer1_rows16): the
+// 16 lanes of a DPP row take one input row (4 hidden units each), their
+// products are summed by an inclusive row scan (lane 15 holds the total), and
+// the workgroup's 16-lane groups stride over the rows, eight rows' loads in
+// flight per lane.  The trip count is the same for every lane (rows past I
+// load nothing and store nothing), so the lane shifts run with all lanes
+// active.
+__device__ __forceinline__ void input_adj_rows16(const float *W1T, const float *z1b, int I,
+                                                 float *xb) {
+    constexpr int U = 8;
+    const int tid = threadIdx.x, q = tid & 15, g = tid >> 4, C = blockDim.x >> 4;
+    const float4 *w4 = reinterpret_cast<const float4 *>(W1T) + q;
+    const float z0 = z1b[4 * q], z1 = z1b[4 * q + 1], z2 = z1b[4 * q + 2], z3 = z1b[4 * q + 3];
+    const int nit = (I + C - 1) / C;
+    for (int k0 = 0; k0 < nit; k0 += U) {
+        float p[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = g + (k0 + u) * C;
+            float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (i < I) w = w4[(size_t)i * (HMAX / 4)];
+            p[u] = w.x * z0 + w.y * z1 + w.z * z2 + w.w * z3;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            float v = p[u];
+            v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xF, 0xF, true));
+            v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x112, 0xF, 0xF, true));
+            v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x114, 0xF, 0xF, true));
+            v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x118, 0xF, 0xF, true));
+            p[u] = v;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = g + (k0 + u) * C;
+            if (q == 15 && i < I) xb[i] = p[u];
+        }
+    }
+}
+
 // One forward step from the resident state at step t.  In MODE_REC (and in
 // every mode with ROWS) the step inputs and intermediates are written to
 // scratch slot `slot`.
@@ -289,24 +374,31 @@ __device__ void fwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int t, in
     __syncthreads();
     // MLP layer 1: thread (j, chunk c); W1T rows are coalesced across j
     {
-        const int j = tid & (HMAX - 1), c = tid / HMAX, C = BT / HMAX;
-        const int Ic = (I + C - 1) / C;
-        const int i0 = c * Ic, i1 = min(I, i0 + Ic);
-        T acc = T(0);
-        if (j < H) {
-            if (R.WL) {
+        T z = T(0);
+        if (w1_rows16(R, H)) {
+            if constexpr (std::is_same_v<T, float>) z = layer1_rows16(R.W1T, R.b1, m.inb, I, m.red);
+        } else {
+            const int j = tid & (HMAX - 1), c = tid / HMAX, C = BT / HMAX;
+            const int Ic = (I + C - 1) / C;
+            const int i0 = c * Ic, i1 = min(I, i0 + Ic);
+            T acc = T(0);
+            if (j < H) {
+                if (R.WL) {
 #pragma unroll 8
-                for (int i = i0; i < i1; ++i) acc += R.WL[i * (HMAX + 1) + j] * m.inb[i];
-            } else {
+                    for (int i = i0; i < i1; ++i) acc += R.WL[i * (HMAX + 1) + j] * m.inb[i];
+                } else {
 #pragma unroll 8
-                for (int i = i0; i < i1; ++i) acc += R.W1T[(size_t)i * H + j] * m.inb[i];
+                    for (int i = i0; i < i1; ++i) acc += R.W1T[(size_t)i * H + j] * m.inb[i];
+                }
+            }
+            m.red[c * HMAX + j] = acc;
+            __syncthreads();
+            if (tid < H) {
+                z = R.b1[tid];
+                for (int cc = 0; cc < C; ++cc) z += m.red[cc * HMAX + tid];
             }
         }
-        m.red[c * HMAX + j] = acc;
-        __syncthreads();
         if (tid < H) {
-            T z = R.b1[tid];
-            for (int cc = 0; cc < C; ++cc) z += m.red[cc * HMAX + tid];
             T y, dy;
             activate<T>(A.act, z, y, dy);
             m.hid[tid] = y;
@@ -516,18 +608,29 @@ __device__ void bwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int r, in
             m.vb[d * N + i] += gv[d];
         }
     }
-    for (int o = tid; o < O; o += BT) {
-        T ab = T(0);
+    // actuator adjoints: one wavefront per actuator, its springs strided over
+    // the lanes and summed in a fixed order (a large robot's actuator owns
+    // hundreds of springs)
+    {
+        const int lane = tid & 63, NW = BT >> 6;
+        for (int o = tid >> 6; o < O; o += NW) {
+            T ab = T(0);
+            const int e1 = R.aptr[o + 1];
 #pragma unroll 4
-        for (int e = R.aptr[o]; e < R.aptr[o + 1]; ++e) ab += G[6 * S + R.aidx[e]];
-        const T z = ab * m.dout[o];
-        m.z2b[o] = z;
-        R.sz2[(size_t)slot * O + o] = z;
+            for (int e = R.aptr[o] + lane; e < e1; e += 64) ab += G[6 * S + R.aidx[e]];
+            ab = wave_sum(ab);
+            if (lane == 0) {
+                const T z = ab * m.dout[o];
+                m.z2b[o] = z;
+                R.sz2[(size_t)slot * O + o] = z;
+            }
+        }
     }
     __syncthreads();
     // MLP adjoint: hidden
     for (int j = tid; j < H; j += BT) {
         T acc = T(0);
+#pragma unroll 8
         for (int o = 0; o < O; ++o) acc += R.W2[(size_t)o * H + j] * m.z2b[o];
         const T z = acc * m.dhid[j];
         m.z1b[j] = z;
@@ -535,7 +638,9 @@ __device__ void bwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int r, in
     }
     __syncthreads();
     // input adjoint: in_b = W1^T z1b (W1 [H][I] rows coalesced across i)
-    for (int i = tid; i < I; i += BT) {
+    if (w1_rows16(R, H)) {
+        if constexpr (std::is_same_v<T, float>) input_adj_rows16(R.W1T, m.z1b, I, m.inbar);
+    } else for (int i = tid; i < I; i += BT) {
         T acc = T(0);
         if (R.WL) {
             const T *row = R.WL + i * (HMAX + 1);
@@ -590,8 +695,10 @@ __device__ void bwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int r, in
 
 // Fold the segment's weight gradients into the robot's partial:
 // W1g += sum_t z1b_t in_t^T, b1g += sum_t z1b_t, W2g += sum_t z2b_t h_t^T, ...
+// The step loops are unrolled so that several steps' row loads are in flight
+// (each iteration's loads are independent; the sums keep the step order).
 template <typename T>
-__device__ void seg_wgrad(const RArgs<T> &A, const Rob<T> &R, int n) {
+__device__ void seg_wgrad(const RArgs<T> &A, const Rob<T> &R, int n, T *zs, int zcap) {
     const int I = R.I, O = R.O, H = A.H, tid = threadIdx.x, BT = blockDim.x;
     for (int i0 = 0; i0 < I; i0 += BT) {
         const int i = i0 + tid;
@@ -599,14 +706,47 @@ __device__ void seg_wgrad(const RArgs<T> &A, const Rob<T> &R, int n) {
         T acc[HMAX];
 #pragma unroll
         for (int j = 0; j < HMAX; ++j) acc[j] = T(0);
-        for (int s = 0; s < n; ++s) {
-            const T xin = ok ? R.sin[(size_t)s * I + i] : T(0);
-            const T *z = R.sz1 + (size_t)s * H;
+        if (H == HMAX && zcap > 0) {
+            // the z rows staged in LDS, zcap steps at a time (one cooperative
+            // load, then broadcast reads): read straight from global memory,
+            // each guarded entry would wait for its own load
+            for (int s0 = 0; s0 < n; s0 += zcap) {
+                const int c = min(zcap, n - s0);
+                __syncthreads();
+                for (int q = tid; q < c * HMAX; q += BT) zs[q] = R.sz1[(size_t)s0 * HMAX + q];
+                __syncthreads();
+                for (int s = 0; s < c; ++s) {
+                    const T xin = ok ? R.sin[(size_t)(s0 + s) * I + i] : T(0);
+                    // 16 entries per group, the groups not interleaved by the
+                    // scheduler (acc already holds 64 registers)
 #pragma unroll
-            for (int j = 0; j < HMAX; ++j)
-                if (j < H) acc[j] += z[j] * xin;
+                    for (int j0 = 0; j0 < HMAX; j0 += 16) {
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int j = j0; j < j0 + 16; ++j) acc[j] += zs[s * HMAX + j] * xin;
+                    }
+                }
+            }
+        } else {
+            for (int s = 0; s < n; ++s) {
+                const T xin = ok ? R.sin[(size_t)s * I + i] : T(0);
+                const T *z = R.sz1 + (size_t)s * H;
+#pragma unroll
+                for (int j = 0; j < HMAX; ++j)
+                    if (j < H) acc[j] += z[j] * xin;
+            }
         }
-        if (ok) {
+        if (ok && H == HMAX) {
+            // read-modify-write in groups of 16 independent loads
+#pragma unroll
+            for (int j0 = 0; j0 < HMAX; j0 += 16) {
+                T g[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) g[j] = R.Wg1[(size_t)(j0 + j) * I + i];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) R.Wg1[(size_t)(j0 + j) * I + i] = g[j] + acc[j0 + j];
+            }
+        } else if (ok) {
 #pragma unroll
             for (int j = 0; j < HMAX; ++j)
                 if (j < H) R.Wg1[(size_t)j * I + i] += acc[j];
@@ -614,17 +754,20 @@ __device__ void seg_wgrad(const RArgs<T> &A, const Rob<T> &R, int n) {
     }
     for (int j = tid; j < H; j += BT) {
         T acc = T(0);
+#pragma unroll 8
         for (int s = 0; s < n; ++s) acc += R.sz1[(size_t)s * H + j];
         R.bg1[j] += acc;
     }
     for (int q = tid; q < O * H; q += BT) {
         const int o = q / H, j = q - o * H;
         T acc = T(0);
+#pragma unroll 8
         for (int s = 0; s < n; ++s) acc += R.sz2[(size_t)s * O + o] * R.shid[(size_t)s * H + j];
         R.Wg2[q] += acc;
     }
     for (int o = tid; o < O; o += BT) {
         T acc = T(0);
+#pragma unroll 8
         for (int s = 0; s < n; ++s) acc += R.sz2[(size_t)s * O + o];
         R.bg2[o] += acc;
     }
@@ -704,7 +847,9 @@ __global__ void __launch_bounds__(LB) k_rollout_bwd(RArgs<T> A) {
             Rs.sz1 = R.sz1 + (size_t)off * A.H;
             Rs.sz2 = R.sz2 + (size_t)off * R.O;
             Rs.shid = R.shid + (size_t)off * A.H;
-            seg_wgrad(A, Rs, tb - lo);
+            // the resident positions / velocities (6 max_n reals) are dead
+            // until the next segment's recompute: they hold the z rows
+            seg_wgrad(A, Rs, tb - lo, m.xs, 6 * A.max_n / HMAX);
         }
         __syncthreads();
         budget -= tb - lo;
@@ -2040,7 +2185,7 @@ static int launch_rollout(const dl_rollout *r, bool bwd, hipStream_t st) {
 extern "C" {
 
 int64_t dl_weights_size(int32_t I, int32_t H, int32_t O) {
-    return 2 * (int64_t)H * I + H + 2 * (int64_t)O * H + O;
+    return (2 * (int64_t)H * I + H + 2 * (int64_t)O * H + O + 15) / 16 * 16;
 }
 
 int64_t dl_wgrad_size(int32_t I, int32_t H, int32_t O) {

@@ -279,7 +279,10 @@ typedef struct dl_robot {
 } dl_robot;
 
 /* Weight block of a design: W1 [H][I], W1T [I][H], b1 [H], W2 [O][H],
- * W2T [H][O], b2 [O] -- dl_weights_size() elements.  Gradient partial
+ * W2T [H][O], b2 [O] -- dl_weights_size() elements, rounded up to a
+ * multiple of 16, so blocks packed back to back from an aligned base keep
+ * W1 and W1T 64-byte aligned (FP32 robots with W1 outside LDS read W1T as
+ * 16-byte rows when H = 64 and it is aligned).  Gradient partial
  * block: W1 [H][I], b1 [H], W2 [O][H], b2 [O] -- dl_wgrad_size() elements. */
 int64_t dl_weights_size(int32_t I, int32_t H, int32_t O);
 int64_t dl_wgrad_size(int32_t I, int32_t H, int32_t O);
