@@ -105,7 +105,11 @@ __device__ __forceinline__ Rob<T> make_rob(const RArgs<T> &A, int r) {
     if (s) {
         carve_scratch(q, s, K, H);
         q.spill = q.scv + (size_t)K * 6;
-        q.scr2 = q.spill + (size_t)10 * q.S;
+        q.scr2 = q.spill + (size_t)10 * q.S + 4;
+        // the spilled spring rows are 16-byte rows (32 bytes in FP64): the
+        // area (10 S + 4 reals) holds 8 S of them after aligning its start
+        const uintptr_t al = 4 * sizeof(T);
+        q.spill = reinterpret_cast<T *>((reinterpret_cast<uintptr_t>(q.spill) + al - 1) / al * al);
     } else {
         q.sx = q.svi = q.sin = q.shid = q.sdhid = q.sz1 = q.sout = q.sdout = q.sz2 = q.scv = q.spill = nullptr;
         q.scr2 = nullptr;
@@ -176,6 +180,120 @@ __host__ size_t smem_bytes(int max_n, int max_s, int max_i, int max_o, bool bwd,
     size_t b = n * sizeof(T);
     if (topo) b += sizeof(int) * (6 * (size_t)max_s + max_n + max_o + 2);
     return b;
+}
+
+// Per-spring rows of the step: impulses J (3 reals) in the forward and the
+// spring adjoint terms G (dL/d dx, dL/d dv, dL/d a: 7 reals) in the adjoint.
+// In LDS they are channel-major, [c][S].  Spilled to global scratch (robots
+// whose springs do not fit in LDS) they are spring-major with 16-byte rows
+// (J: 4 reals per spring, G: 8), so a vertex's gather costs one (J) or two
+// (G) vector loads per incident spring instead of 3 or 6 scattered ones.
+template <typename T> struct V4 { T a, b, c, d; };
+template <typename T>
+__device__ __forceinline__ V4<T> ld_v4(const T *p) {
+    if constexpr (std::is_same_v<T, float>) {
+        const float4 v = *reinterpret_cast<const float4 *>(p);
+        return {v.x, v.y, v.z, v.w};
+    } else {
+        const double2 u = *reinterpret_cast<const double2 *>(p);
+        const double2 v = *reinterpret_cast<const double2 *>(p + 2);
+        return {u.x, u.y, v.x, v.y};
+    }
+}
+template <typename T>
+__device__ __forceinline__ void st_v4(T *p, T a, T b, T c, T d) {
+    if constexpr (std::is_same_v<T, float>) {
+        *reinterpret_cast<float4 *>(p) = make_float4(a, b, c, d);
+    } else {
+        *reinterpret_cast<double2 *>(p) = make_double2(a, b);
+        *reinterpret_cast<double2 *>(p + 2) = make_double2(c, d);
+    }
+}
+template <typename T>
+__device__ __forceinline__ void j_store(T *J, int S, bool rows, int s, const T (&j)[3]) {
+    if (rows) st_v4(J + 4 * (size_t)s, j[0], j[1], j[2], T(0));
+    else { J[s] = j[0]; J[S + s] = j[1]; J[2 * S + s] = j[2]; }
+}
+template <bool ROWS_, typename T>
+__device__ __forceinline__ void j_load(const T *J, int S, int s, T (&j)[3]) {
+    if constexpr (ROWS_) {
+        const V4<T> v = ld_v4(J + 4 * (size_t)s);
+        j[0] = v.a; j[1] = v.b; j[2] = v.c;
+    } else {
+        j[0] = J[s]; j[1] = J[S + s]; j[2] = J[2 * S + s];
+    }
+}
+template <typename T>
+__device__ __forceinline__ void g_store(T *G, int S, bool rows, int s, const T (&gd)[3],
+                                        const T (&gw)[3], T ga) {
+    if (rows) {
+        st_v4(G + 8 * (size_t)s, gd[0], gd[1], gd[2], gw[0]);
+        st_v4(G + 8 * (size_t)s + 4, gw[1], gw[2], ga, T(0));
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            G[c * S + s] = gd[c];
+            G[(3 + c) * S + s] = gw[c];
+        }
+        G[6 * S + s] = ga;
+    }
+}
+template <bool ROWS_, typename T>
+__device__ __forceinline__ void g_load(const T *G, int S, int s, T (&gd)[3], T (&gw)[3]) {
+    if constexpr (ROWS_) {
+        const V4<T> u = ld_v4(G + 8 * (size_t)s), v = ld_v4(G + 8 * (size_t)s + 4);
+        gd[0] = u.a; gd[1] = u.b; gd[2] = u.c; gw[0] = u.d; gw[1] = v.a; gw[2] = v.b;
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            gd[c] = G[c * S + s];
+            gw[c] = G[(3 + c) * S + s];
+        }
+    }
+}
+template <bool ROWS_, typename T>
+__device__ __forceinline__ T g_act(const T *G, int S, int s) {
+    if constexpr (ROWS_) return G[8 * (size_t)s + 6];
+    return G[6 * S + s];
+}
+
+// Signed sum of the impulses of vertex i's incident springs (CSR vptr/vidx:
+// entry 2 s + 1 when i is the spring's first end) and of their adjoint terms.
+template <bool ROWS_, typename T>
+__device__ __forceinline__ void gather_j(const Rob<T> &R, const T *J, int S, int i, T (&vi)[3]) {
+#pragma unroll 4
+    for (int e = R.vptr[i]; e < R.vptr[i + 1]; ++e) {
+        const int ent = R.vidx[e], s = ent >> 1;
+        const T sg = (ent & 1) ? T(1) : T(-1);
+        T j[3];
+        j_load<ROWS_>(J, S, s, j);
+        vi[0] += sg * j[0]; vi[1] += sg * j[1]; vi[2] += sg * j[2];
+    }
+}
+template <bool ROWS_, typename T>
+__device__ __forceinline__ void gather_g(const Rob<T> &R, const T *G, int S, int i, T (&gx)[3],
+                                         T (&gv)[3]) {
+#pragma unroll 4
+    for (int e = R.vptr[i]; e < R.vptr[i + 1]; ++e) {
+        const int ent = R.vidx[e], s = ent >> 1;
+        const T sg = (ent & 1) ? T(-1) : T(1);
+        T gd[3], gw[3];
+        g_load<ROWS_>(G, S, s, gd, gw);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            gx[c] += sg * gd[c];
+            gv[c] += sg * gw[c];
+        }
+    }
+}
+// Sum of the actuation-adjoint terms of springs e = e0, e0 + 64, ... < e1
+// of an actuator's spring list (one lane's share).
+template <bool ROWS_, typename T>
+__device__ __forceinline__ T act_share(const Rob<T> &R, const T *G, int S, int e0, int e1) {
+    T ab = T(0);
+#pragma unroll 4
+    for (int e = e0; e < e1; e += 64) ab += g_act<ROWS_>(G, S, R.aidx[e]);
+    return ab;
 }
 
 // Copy the robot's design topology (and W1^T when requested) into LDS and
@@ -440,18 +558,14 @@ __device__ void fwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int t, in
         const T w[3] = {m.vs[a] - m.vs[b], m.vs[N + a] - m.vs[N + b], m.vs[2 * N + a] - m.vs[2 * N + b]};
         T J[3];
         spring_impulse<T>(d, w, R.L0[s], R.k[s], R.coef[s], m.outb[R.sact[s]], A.P, J);
-        m.J[s] = J[0]; m.J[S + s] = J[1]; m.J[2 * S + s] = J[2];
+        j_store(m.J, S, !A.springs_in_lds, s, J);
     }
     __syncthreads();
     // gather + integrate (vertex owned by one thread, in place)
     for (int i = tid; i < N; i += BT) {
         T vi[3] = {0, 0, 0};
-#pragma unroll 4
-        for (int e = R.vptr[i]; e < R.vptr[i + 1]; ++e) {
-            const int ent = R.vidx[e], s = ent >> 1;
-            const T sg = (ent & 1) ? T(1) : T(-1);
-            vi[0] += sg * m.J[s]; vi[1] += sg * m.J[S + s]; vi[2] += sg * m.J[2 * S + s];
-        }
+        if (A.springs_in_lds) gather_j<false>(R, m.J, S, i, vi);
+        else gather_j<true>(R, m.J, S, i, vi);
         if constexpr (ROWS) {
             // kept for the adjoint, which then skips its spring pass
             T *sv = R.svi + (size_t)slot * 3 * N;
@@ -534,7 +648,7 @@ __device__ void bwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int r, in
     }
     if (tid < 6) m.cv[tid] = R.scv[slot * 6 + tid];
     __syncthreads();
-    T *G = m.J;  // [7][S], overwrites the impulses once the integrator adjoint consumed them
+    T *G = m.J;  // overwrites the impulses once the integrator adjoint consumed them
     if constexpr (!STORED) {
         // recompute impulses
         for (int s = tid; s < S; s += BT) {
@@ -543,7 +657,7 @@ __device__ void bwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int r, in
             const T w[3] = {m.vs[a] - m.vs[b], m.vs[N + a] - m.vs[N + b], m.vs[2 * N + a] - m.vs[2 * N + b]};
             T J[3];
             spring_impulse<T>(d, w, R.L0[s], R.k[s], R.coef[s], m.outb[R.sact[s]], A.P, J);
-            m.J[s] = J[0]; m.J[S + s] = J[1]; m.J[2 * S + s] = J[2];
+            j_store(m.J, S, !A.springs_in_lds, s, J);
         }
         __syncthreads();
     }
@@ -554,11 +668,8 @@ __device__ void bwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int r, in
             const T *sv = R.svi + (size_t)slot * 3 * N;
             vi[0] = sv[i]; vi[1] = sv[N + i]; vi[2] = sv[2 * N + i];
         } else {
-            for (int e = R.vptr[i]; e < R.vptr[i + 1]; ++e) {
-                const int ent = R.vidx[e], s = ent >> 1;
-                const T sg = (ent & 1) ? T(1) : T(-1);
-                vi[0] += sg * m.J[s]; vi[1] += sg * m.J[S + s]; vi[2] += sg * m.J[2 * S + s];
-            }
+            if (A.springs_in_lds) gather_j<false>(R, m.J, S, i, vi);
+            else gather_j<true>(R, m.J, S, i, vi);
         }
         const T x[3] = {m.xs[i], m.xs[N + i], m.xs[2 * N + i]};
         const T v[3] = {m.vs[i], m.vs[N + i], m.vs[2 * N + i]};
@@ -582,26 +693,13 @@ __device__ void bwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int r, in
         const T Jb[3] = {m.vib[b] - m.vib[a], m.vib[N + b] - m.vib[N + a], m.vib[2 * N + b] - m.vib[2 * N + a]};
         T gd[3], gw[3], ga;
         spring_impulse_adj<T>(d, w, R.L0[s], R.k[s], R.coef[s], m.outb[R.sact[s]], A.P, Jb, gd, gw, ga);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            G[c * S + s] = gd[c];
-            G[(3 + c) * S + s] = gw[c];
-        }
-        G[6 * S + s] = ga;
+        g_store(G, S, !A.springs_in_lds, s, gd, gw, ga);
     }
     __syncthreads();
     for (int i = tid; i < N; i += BT) {
         T gx[3] = {0, 0, 0}, gv[3] = {0, 0, 0};
-#pragma unroll 4
-        for (int e = R.vptr[i]; e < R.vptr[i + 1]; ++e) {
-            const int ent = R.vidx[e], s = ent >> 1;
-            const T sg = (ent & 1) ? T(-1) : T(1);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                gx[c] += sg * G[c * S + s];
-                gv[c] += sg * G[(3 + c) * S + s];
-            }
-        }
+        if (A.springs_in_lds) gather_g<false>(R, G, S, i, gx, gv);
+        else gather_g<true>(R, G, S, i, gx, gv);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             m.xb[d * N + i] += gx[d];
@@ -614,10 +712,9 @@ __device__ void bwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int r, in
     {
         const int lane = tid & 63, NW = BT >> 6;
         for (int o = tid >> 6; o < O; o += NW) {
-            T ab = T(0);
-            const int e1 = R.aptr[o + 1];
-#pragma unroll 4
-            for (int e = R.aptr[o] + lane; e < e1; e += 64) ab += G[6 * S + R.aidx[e]];
+            const int e0 = R.aptr[o] + lane, e1 = R.aptr[o + 1];
+            T ab = A.springs_in_lds ? act_share<false>(R, G, S, e0, e1)
+                                    : act_share<true>(R, G, S, e0, e1);
             ab = wave_sum(ab);
             if (lane == 0) {
                 const T z = ab * m.dout[o];
@@ -627,16 +724,24 @@ __device__ void bwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int r, in
         }
     }
     __syncthreads();
-    // MLP adjoint: hidden
-    for (int j = tid; j < H; j += BT) {
+    // MLP adjoint: hidden; wavefront g sums the outputs o = g, g + NW, ...
+    // (W2 rows coalesced across the hidden units), the partials in LDS
+    {
+        const int j = tid & (HMAX - 1), g = tid / HMAX, NW = BT / HMAX;
         T acc = T(0);
-#pragma unroll 8
-        for (int o = 0; o < O; ++o) acc += R.W2[(size_t)o * H + j] * m.z2b[o];
-        const T z = acc * m.dhid[j];
-        m.z1b[j] = z;
-        R.sz1[(size_t)slot * H + j] = z;
+        if (j < H)
+            for (int o = g; o < O; o += NW) acc += R.W2[(size_t)o * H + j] * m.z2b[o];
+        m.red[g * HMAX + j] = acc;
+        __syncthreads();
+        if (tid < H) {
+            T zs = T(0);
+            for (int w = 0; w < NW; ++w) zs += m.red[w * HMAX + tid];
+            const T z = zs * m.dhid[tid];
+            m.z1b[tid] = z;
+            R.sz1[(size_t)slot * H + tid] = z;
+        }
+        __syncthreads();
     }
-    __syncthreads();
     // input adjoint: in_b = W1^T z1b (W1 [H][I] rows coalesced across i)
     if (w1_rows16(R, H)) {
         if constexpr (std::is_same_v<T, float>) input_adj_rows16(R.W1T, m.z1b, I, m.inbar);
@@ -2194,7 +2299,7 @@ int64_t dl_wgrad_size(int32_t I, int32_t H, int32_t O) {
 
 int64_t dl_scratch_size(int32_t N, int32_t S, int32_t I, int32_t H, int32_t O, int32_t K) {
     return 2 * (int64_t)K * (9 * (int64_t)N + I + 3 * (int64_t)H + 3 * (int64_t)O + 6) +
-           10 * (int64_t)S;
+           10 * (int64_t)S + 4;
 }
 
 int32_t dl_ms_rollout_forward(const dl_rollout *r, void *stream) {

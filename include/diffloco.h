@@ -291,7 +291,8 @@ int64_t dl_wgrad_size(int32_t I, int32_t H, int32_t O);
  * the stored-rows forward only), network input (I), hidden values,
  * derivatives and adjoints (3H), outputs (3O), COM row (6); the pipelined
  * adjoint fills one buffer while it reverses the other -- and a spring spill
- * area (10S). */
+ * area (10S + 4: the block kernels keep spilled spring rows as 16-byte rows,
+ * 8 reals per spring, from an aligned start). */
 int64_t dl_scratch_size(int32_t N, int32_t S, int32_t I, int32_t H, int32_t O, int32_t K);
 
 typedef struct dl_rollout {

@@ -77,7 +77,8 @@ def test_size_helpers():
     assert L.dl_weights_size(179, 64, 4) == (n + 15) // 16 * 16
     assert all(L.dl_weights_size(I, 64, O) % 16 == 0 for I in (17, 179, 2384) for O in (1, 4, 7))
     assert L.dl_wgrad_size(179, 64, 4) == 64 * 179 + 64 + 4 * 64 + 4
-    assert L.dl_scratch_size(54, 323, 179, 64, 4, 50) == 2 * 50 * (9 * 54 + 179 + 3 * 64 + 3 * 4 + 6) + 10 * 323
+    assert L.dl_scratch_size(54, 323, 179, 64, 4, 50) == (2 * 50 * (9 * 54 + 179 + 3 * 64 + 3 * 4 + 6)
+                                                        + 10 * 323 + 4)
 
 
 def test_invalid_arguments_are_rejected_without_a_gpu():

@@ -470,7 +470,7 @@ def test_wave_recompute_reproduces_forward(variant):
     batch.forward()
     batch.backward(loss_grad=np.full(B, 1.0 / B))
     torch.cuda.synchronize()
-    buf_len = K * (9 * N + I + 3 * 64 + 3 * O + 6) + 10 * S   # buffer 0 rows + spill
+    buf_len = K * (9 * N + I + 3 * 64 + 3 * O + 6) + 10 * S + 4   # buffer 0 rows + spill
     for r in range(B):
         o = int(batch.table[r]["traj"])
         fwd = batch.traj[o:o + K * (3 * N + 3)].view(K, 3 * N + 3)[:, :3 * N]
