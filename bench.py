@@ -364,6 +364,99 @@ def main():
         torch.cuda.empty_cache()
     mlp = None if args.skip_mlp else mlp_ab(R, I, H, O)
 
+    # ---- uneven batch: 512 seeded random voxel robots, size classes, HIP graphs
+    # (run before the MPM leg: measured after the MPM leg's 27-65 GB of
+    # allocations were freed, the same iteration took 1.5x as long)
+    uneven = None
+    if not args.skip_uneven:
+        from diffloco import robot_design as rdm
+        from diffloco.uneven import UnevenBatch, n_inputs, tune_schedule
+        designs_u, hz = rdm.random_robot_batch(args.uneven_robots, seed=3)
+        Ru = len(designs_u)
+        rngu = np.random.default_rng(4)
+        tgu = np.zeros((Ru, 5))
+        tgu[:, 0] = rngu.uniform(-0.08, 0.08, Ru)
+        tgu[:, 2] = rngu.uniform(-0.08, 0.08, Ru)
+        tgu[:, 3] = rngu.uniform(0.1, 0.25, Ru)
+        # dashpot impulse 0.03: the no-dt impulse c (va - vb).d d of the
+        # reference integrator diverges at c = 0.12 for vertices with many
+        # springs (dense voxel bodies), also in the FP64 oracle
+        phys_u = _physics(qc)
+        phys_u.dashpot = UNEVEN_DASHPOT
+
+        wu = [init_weights(n_inputs(lay, d.n_vert), 64, d.n_act, seed=1000 + r)
+              for r, d in enumerate(designs_u)]
+
+        def load_weights(ub):
+            for r in range(Ru):
+                ub.set_weights(r, *wu[r])
+
+        def uneven_run(stored):
+            # robot-to-launch schedule (size-class boundaries), then each
+            # class's launch shape, both by the autotuner
+            t_tune = time.perf_counter()
+            bounds, sched = tune_schedule(designs_u, hz, phys_u, lay, tgu, set_weights=load_weights,
+                                          ckpt_every=50, loss_window=100, stored=stored)
+            ub = UnevenBatch(designs_u, hz, phys_u, lay, tgu, ckpt_every=50, loss_window=100,
+                             stored=stored, bounds=bounds)
+            load_weights(ub)
+            ub.tune()
+            t_tune = time.perf_counter() - t_tune
+            ub.prepare(np.full(Ru, 1.0 / Ru))
+            ub.run_iteration()
+            torch.cuda.synchronize()
+            t3 = time.perf_counter()
+            ub.run_iteration()
+            torch.cuda.synchronize()
+            eager_ms = (time.perf_counter() - t3) * 1e3
+            ub.capture()
+            ub.replay()
+            torch.cuda.synchronize()
+            t3 = time.perf_counter()
+            for _ in range(args.uneven_iters):
+                ub.replay()
+            torch.cuda.synchronize()
+            graph_ms = (time.perf_counter() - t3) * 1e3 / args.uneven_iters
+            ub.prepare(np.full(Ru, 1.0 / Ru))
+            per_class = ub.class_times()
+            rs = ub.robot_steps
+            ubytes = 0.0
+            for d, T_r in zip(designs_u, hz):
+                I_r = n_inputs(lay, d.n_vert)
+                f_r, b_r = algorithmic_bytes(d.n_vert, d.n_spring, I_r, 64, d.n_act,
+                                             T_r if ub.stored else 50, 1, stored=ub.stored)
+                ubytes += (f_r + b_r) * T_r
+            uval = rs / (graph_ms * 1e-3)
+            nv = np.array([d.n_vert for d in designs_u])
+            out = {"value": uval, "unit": "robot-steps/s", "robots": Ru, "robot_steps": rs,
+                      "n_vert": [int(nv.min()), float(nv.mean()), int(nv.max())],
+                      "horizon": [int(min(hz)), int(max(hz))], "dtype": "f32",
+                      "ms_per_iteration_graph": graph_ms, "ms_per_iteration_eager": eager_ms,
+                      "autotune_s": t_tune,
+                      "classes": [{"name": bk["name"], "robots": len(bk["ids"]), "launch": bk["config"],
+                                   "ms_alone": per_class[bk["name"]]}
+                                  for bk in ub.buckets],
+                      "schedule": {"bounds": list(bounds),
+                                   "candidates_ms": [(t["candidate"]["bounds"], t.get("ms"))
+                                                     for t in (sched or [])]},
+                      "roofline": {"bound": "hbm", "achieved": ubytes / (graph_ms * 1e-3) / 1e9,
+                                   "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                   "frac": ubytes / (graph_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                   "algorithmic_bytes_per_iteration": ubytes, "traffic": None},
+                      "dashpot_impulse": UNEVEN_DASHPOT,
+                      "nonfinite_robots": int((~torch.isfinite(ub.losses())).sum().item()),
+                      "mean_loss": float(ub.losses().double().mean().item()), "stored": ub.stored}
+            del ub
+            torch.cuda.empty_cache()
+            return out
+
+        uneven = uneven_run(bool(args.uneven_stored))
+        if not args.skip_alt:
+            ua = uneven_run(not uneven["stored"])
+            uneven["checkpointed" if uneven["stored"] else "stored_rows"] = {
+                k: ua[k] for k in ("value", "stored", "ms_per_iteration_graph", "autotune_s",
+                                   "classes", "schedule", "nonfinite_robots", "mean_loss")}
+
     # ---- MLS-MPM: 8 soft quadrupeds, each on its own 64^3 grid
     mpm = None
     if not args.skip_mpm:
@@ -486,97 +579,6 @@ def main():
                 "mean_loss": float(mr.loss.item()), "device_gb": mr.nbytes() / 1e9}
             del mr
             torch.cuda.empty_cache()
-
-    # ---- uneven batch: 512 seeded random voxel robots, size classes, HIP graphs
-    uneven = None
-    if not args.skip_uneven:
-        from diffloco import robot_design as rdm
-        from diffloco.uneven import UnevenBatch, n_inputs, tune_schedule
-        designs_u, hz = rdm.random_robot_batch(args.uneven_robots, seed=3)
-        Ru = len(designs_u)
-        rngu = np.random.default_rng(4)
-        tgu = np.zeros((Ru, 5))
-        tgu[:, 0] = rngu.uniform(-0.08, 0.08, Ru)
-        tgu[:, 2] = rngu.uniform(-0.08, 0.08, Ru)
-        tgu[:, 3] = rngu.uniform(0.1, 0.25, Ru)
-        # dashpot impulse 0.03: the no-dt impulse c (va - vb).d d of the
-        # reference integrator diverges at c = 0.12 for vertices with many
-        # springs (dense voxel bodies), also in the FP64 oracle
-        phys_u = _physics(qc)
-        phys_u.dashpot = UNEVEN_DASHPOT
-
-        wu = [init_weights(n_inputs(lay, d.n_vert), 64, d.n_act, seed=1000 + r)
-              for r, d in enumerate(designs_u)]
-
-        def load_weights(ub):
-            for r in range(Ru):
-                ub.set_weights(r, *wu[r])
-
-        def uneven_run(stored):
-            # robot-to-launch schedule (size-class boundaries), then each
-            # class's launch shape, both by the autotuner
-            t_tune = time.perf_counter()
-            bounds, sched = tune_schedule(designs_u, hz, phys_u, lay, tgu, set_weights=load_weights,
-                                          ckpt_every=50, loss_window=100, stored=stored)
-            ub = UnevenBatch(designs_u, hz, phys_u, lay, tgu, ckpt_every=50, loss_window=100,
-                             stored=stored, bounds=bounds)
-            load_weights(ub)
-            ub.tune()
-            t_tune = time.perf_counter() - t_tune
-            ub.prepare(np.full(Ru, 1.0 / Ru))
-            ub.run_iteration()
-            torch.cuda.synchronize()
-            t3 = time.perf_counter()
-            ub.run_iteration()
-            torch.cuda.synchronize()
-            eager_ms = (time.perf_counter() - t3) * 1e3
-            ub.capture()
-            ub.replay()
-            torch.cuda.synchronize()
-            t3 = time.perf_counter()
-            for _ in range(args.uneven_iters):
-                ub.replay()
-            torch.cuda.synchronize()
-            graph_ms = (time.perf_counter() - t3) * 1e3 / args.uneven_iters
-            ub.prepare(np.full(Ru, 1.0 / Ru))
-            per_class = ub.class_times()
-            rs = ub.robot_steps
-            ubytes = 0.0
-            for d, T_r in zip(designs_u, hz):
-                I_r = n_inputs(lay, d.n_vert)
-                f_r, b_r = algorithmic_bytes(d.n_vert, d.n_spring, I_r, 64, d.n_act,
-                                             T_r if ub.stored else 50, 1, stored=ub.stored)
-                ubytes += (f_r + b_r) * T_r
-            uval = rs / (graph_ms * 1e-3)
-            nv = np.array([d.n_vert for d in designs_u])
-            out = {"value": uval, "unit": "robot-steps/s", "robots": Ru, "robot_steps": rs,
-                      "n_vert": [int(nv.min()), float(nv.mean()), int(nv.max())],
-                      "horizon": [int(min(hz)), int(max(hz))], "dtype": "f32",
-                      "ms_per_iteration_graph": graph_ms, "ms_per_iteration_eager": eager_ms,
-                      "autotune_s": t_tune,
-                      "classes": [{"name": bk["name"], "robots": len(bk["ids"]), "launch": bk["config"],
-                                   "ms_alone": per_class[bk["name"]]}
-                                  for bk in ub.buckets],
-                      "schedule": {"bounds": list(bounds),
-                                   "candidates_ms": [(t["candidate"]["bounds"], t.get("ms"))
-                                                     for t in (sched or [])]},
-                      "roofline": {"bound": "hbm", "achieved": ubytes / (graph_ms * 1e-3) / 1e9,
-                                   "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                   "frac": ubytes / (graph_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                                   "algorithmic_bytes_per_iteration": ubytes, "traffic": None},
-                      "dashpot_impulse": UNEVEN_DASHPOT,
-                      "nonfinite_robots": int((~torch.isfinite(ub.losses())).sum().item()),
-                      "mean_loss": float(ub.losses().double().mean().item()), "stored": ub.stored}
-            del ub
-            torch.cuda.empty_cache()
-            return out
-
-        uneven = uneven_run(bool(args.uneven_stored))
-        if not args.skip_alt:
-            ua = uneven_run(not uneven["stored"])
-            uneven["checkpointed" if uneven["stored"] else "stored_rows"] = {
-                k: ua[k] for k in ("value", "stored", "ms_per_iteration_graph", "autotune_s",
-                                   "classes", "schedule", "nonfinite_robots", "mean_loss")}
 
     # ---- request loop: Poisson arrivals, 70 % random mass-spring robots, 30 % MPM bodies
     serve = None
