@@ -188,31 +188,55 @@ __host__ size_t smem_bytes(int max_n, int max_s, int max_i, int max_o, bool bwd,
 // whose springs do not fit in LDS) they are spring-major with 16-byte rows
 // (J: 4 reals per spring, G: 8), so a vertex's gather costs one (J) or two
 // (G) vector loads per incident spring instead of 3 or 6 scattered ones.
+//
+// Placement: the topology arrays (sa, sb, sact, L0, k, coef, vptr, vidx,
+// aptr, aidx) are in LDS when staged and in global memory otherwise, and the
+// spring rows in LDS or spilled; decided per launch.  Through a generic
+// pointer every access would be a flat instruction (which waits on both the
+// vector-memory and the LDS counters), so the spring passes and gathers are
+// instantiated per placement -- ROWS_: rows spilled to global memory, TL:
+// topology in LDS -- with address-space-qualified pointers, and the caller
+// picks the instance once per pass (DL_PLACEMENT).
+#define DL_AS_LDS __attribute__((address_space(3)))
+#define DL_AS_GLB __attribute__((address_space(1)))
+template <bool LDS_, typename T> struct AsPtr { using type = DL_AS_GLB T *; };
+template <typename T> struct AsPtr<true, T> { using type = DL_AS_LDS T *; };
+template <bool LDS_, typename T>
+__device__ __forceinline__ typename AsPtr<LDS_, T>::type as_ptr(T *p) {
+    return (typename AsPtr<LDS_, T>::type)p;
+}
+
 template <typename T> struct V4 { T a, b, c, d; };
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
 template <typename T>
 __device__ __forceinline__ V4<T> ld_v4(const T *p) {
     if constexpr (std::is_same_v<T, float>) {
-        const float4 v = *reinterpret_cast<const float4 *>(p);
+        const f32x4 v = *as_ptr<false>(reinterpret_cast<const f32x4 *>(p));
         return {v.x, v.y, v.z, v.w};
     } else {
-        const double2 u = *reinterpret_cast<const double2 *>(p);
-        const double2 v = *reinterpret_cast<const double2 *>(p + 2);
+        const f64x2 u = *as_ptr<false>(reinterpret_cast<const f64x2 *>(p));
+        const f64x2 v = *as_ptr<false>(reinterpret_cast<const f64x2 *>(p + 2));
         return {u.x, u.y, v.x, v.y};
     }
 }
 template <typename T>
 __device__ __forceinline__ void st_v4(T *p, T a, T b, T c, T d) {
     if constexpr (std::is_same_v<T, float>) {
-        *reinterpret_cast<float4 *>(p) = make_float4(a, b, c, d);
+        *as_ptr<false>(reinterpret_cast<f32x4 *>(p)) = f32x4{a, b, c, d};
     } else {
-        *reinterpret_cast<double2 *>(p) = make_double2(a, b);
-        *reinterpret_cast<double2 *>(p + 2) = make_double2(c, d);
+        *as_ptr<false>(reinterpret_cast<f64x2 *>(p)) = f64x2{a, b};
+        *as_ptr<false>(reinterpret_cast<f64x2 *>(p + 2)) = f64x2{c, d};
     }
 }
-template <typename T>
-__device__ __forceinline__ void j_store(T *J, int S, bool rows, int s, const T (&j)[3]) {
-    if (rows) st_v4(J + 4 * (size_t)s, j[0], j[1], j[2], T(0));
-    else { J[s] = j[0]; J[S + s] = j[1]; J[2 * S + s] = j[2]; }
+template <bool ROWS_, typename T>
+__device__ __forceinline__ void j_store(T *J, int S, int s, const T (&j)[3]) {
+    if constexpr (ROWS_) {
+        st_v4(J + 4 * (size_t)s, j[0], j[1], j[2], T(0));
+    } else {
+        const auto L = as_ptr<true>(J);
+        L[s] = j[0]; L[S + s] = j[1]; L[2 * S + s] = j[2];
+    }
 }
 template <bool ROWS_, typename T>
 __device__ __forceinline__ void j_load(const T *J, int S, int s, T (&j)[3]) {
@@ -220,22 +244,24 @@ __device__ __forceinline__ void j_load(const T *J, int S, int s, T (&j)[3]) {
         const V4<T> v = ld_v4(J + 4 * (size_t)s);
         j[0] = v.a; j[1] = v.b; j[2] = v.c;
     } else {
-        j[0] = J[s]; j[1] = J[S + s]; j[2] = J[2 * S + s];
+        const auto L = as_ptr<true>(J);
+        j[0] = L[s]; j[1] = L[S + s]; j[2] = L[2 * S + s];
     }
 }
-template <typename T>
-__device__ __forceinline__ void g_store(T *G, int S, bool rows, int s, const T (&gd)[3],
-                                        const T (&gw)[3], T ga) {
-    if (rows) {
+template <bool ROWS_, typename T>
+__device__ __forceinline__ void g_store(T *G, int S, int s, const T (&gd)[3], const T (&gw)[3],
+                                        T ga) {
+    if constexpr (ROWS_) {
         st_v4(G + 8 * (size_t)s, gd[0], gd[1], gd[2], gw[0]);
         st_v4(G + 8 * (size_t)s + 4, gw[1], gw[2], ga, T(0));
     } else {
+        const auto L = as_ptr<true>(G);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            G[c * S + s] = gd[c];
-            G[(3 + c) * S + s] = gw[c];
+            L[c * S + s] = gd[c];
+            L[(3 + c) * S + s] = gw[c];
         }
-        G[6 * S + s] = ga;
+        L[6 * S + s] = ga;
     }
 }
 template <bool ROWS_, typename T>
@@ -244,38 +270,88 @@ __device__ __forceinline__ void g_load(const T *G, int S, int s, T (&gd)[3], T (
         const V4<T> u = ld_v4(G + 8 * (size_t)s), v = ld_v4(G + 8 * (size_t)s + 4);
         gd[0] = u.a; gd[1] = u.b; gd[2] = u.c; gw[0] = u.d; gw[1] = v.a; gw[2] = v.b;
     } else {
+        const auto L = as_ptr<true>(G);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            gd[c] = G[c * S + s];
-            gw[c] = G[(3 + c) * S + s];
+            gd[c] = L[c * S + s];
+            gw[c] = L[(3 + c) * S + s];
         }
     }
 }
 template <bool ROWS_, typename T>
 __device__ __forceinline__ T g_act(const T *G, int S, int s) {
-    if constexpr (ROWS_) return G[8 * (size_t)s + 6];
-    return G[6 * S + s];
+    if constexpr (ROWS_) return *as_ptr<false>(G + 8 * (size_t)s + 6);
+    return as_ptr<true>(G)[6 * S + s];
+}
+
+#define DL_PLACEMENT(A_, FN_, ...)                                                       \
+    do {                                                                                 \
+        if ((A_).springs_in_lds) {                                                       \
+            if ((A_).topo_in_lds) FN_<false, true>(__VA_ARGS__);                         \
+            else FN_<false, false>(__VA_ARGS__);                                         \
+        } else {                                                                         \
+            if ((A_).topo_in_lds) FN_<true, true>(__VA_ARGS__);                          \
+            else FN_<true, false>(__VA_ARGS__);                                          \
+        }                                                                                \
+    } while (0)
+
+// Spring impulses of the resident state into the step's rows (J).
+template <bool ROWS_, bool TL, typename T>
+__device__ void impulse_pass(const RArgs<T> &A, const Rob<T> &R, const Sm<T> &m) {
+    const int N = R.N, S = R.S, tid = threadIdx.x, BT = blockDim.x;
+    const auto sa = as_ptr<TL>(R.sa), sb = as_ptr<TL>(R.sb), sact = as_ptr<TL>(R.sact);
+    const auto L0 = as_ptr<TL>(R.L0), kk = as_ptr<TL>(R.k), coef = as_ptr<TL>(R.coef);
+    const auto xs = as_ptr<true>(m.xs), vs = as_ptr<true>(m.vs), outb = as_ptr<true>(m.outb);
+    for (int s = tid; s < S; s += BT) {
+        const int a = sa[s], b = sb[s];
+        const T d[3] = {xs[a] - xs[b], xs[N + a] - xs[N + b], xs[2 * N + a] - xs[2 * N + b]};
+        const T w[3] = {vs[a] - vs[b], vs[N + a] - vs[N + b], vs[2 * N + a] - vs[2 * N + b]};
+        T J[3];
+        spring_impulse<T>(d, w, L0[s], kk[s], coef[s], outb[sact[s]], A.P, J);
+        j_store<ROWS_>(m.J, S, s, J);
+    }
+}
+
+// Spring adjoint terms (G) from the impulse adjoints (vib) of the step.
+template <bool ROWS_, bool TL, typename T>
+__device__ void impulse_adj_pass(const RArgs<T> &A, const Rob<T> &R, const Sm<T> &m) {
+    const int N = R.N, S = R.S, tid = threadIdx.x, BT = blockDim.x;
+    const auto sa = as_ptr<TL>(R.sa), sb = as_ptr<TL>(R.sb), sact = as_ptr<TL>(R.sact);
+    const auto L0 = as_ptr<TL>(R.L0), kk = as_ptr<TL>(R.k), coef = as_ptr<TL>(R.coef);
+    const auto xs = as_ptr<true>(m.xs), vs = as_ptr<true>(m.vs), vib = as_ptr<true>(m.vib);
+    const auto outb = as_ptr<true>(m.outb);
+    for (int s = tid; s < S; s += BT) {
+        const int a = sa[s], b = sb[s];
+        const T d[3] = {xs[a] - xs[b], xs[N + a] - xs[N + b], xs[2 * N + a] - xs[2 * N + b]};
+        const T w[3] = {vs[a] - vs[b], vs[N + a] - vs[N + b], vs[2 * N + a] - vs[2 * N + b]};
+        const T Jb[3] = {vib[b] - vib[a], vib[N + b] - vib[N + a], vib[2 * N + b] - vib[2 * N + a]};
+        T gd[3], gw[3], ga;
+        spring_impulse_adj<T>(d, w, L0[s], kk[s], coef[s], outb[sact[s]], A.P, Jb, gd, gw, ga);
+        g_store<ROWS_>(m.J, S, s, gd, gw, ga);
+    }
 }
 
 // Signed sum of the impulses of vertex i's incident springs (CSR vptr/vidx:
 // entry 2 s + 1 when i is the spring's first end) and of their adjoint terms.
-template <bool ROWS_, typename T>
+template <bool ROWS_, bool TL, typename T>
 __device__ __forceinline__ void gather_j(const Rob<T> &R, const T *J, int S, int i, T (&vi)[3]) {
+    const auto vptr = as_ptr<TL>(R.vptr), vidx = as_ptr<TL>(R.vidx);
 #pragma unroll 4
-    for (int e = R.vptr[i]; e < R.vptr[i + 1]; ++e) {
-        const int ent = R.vidx[e], s = ent >> 1;
+    for (int e = vptr[i]; e < vptr[i + 1]; ++e) {
+        const int ent = vidx[e], s = ent >> 1;
         const T sg = (ent & 1) ? T(1) : T(-1);
         T j[3];
         j_load<ROWS_>(J, S, s, j);
         vi[0] += sg * j[0]; vi[1] += sg * j[1]; vi[2] += sg * j[2];
     }
 }
-template <bool ROWS_, typename T>
+template <bool ROWS_, bool TL, typename T>
 __device__ __forceinline__ void gather_g(const Rob<T> &R, const T *G, int S, int i, T (&gx)[3],
                                          T (&gv)[3]) {
+    const auto vptr = as_ptr<TL>(R.vptr), vidx = as_ptr<TL>(R.vidx);
 #pragma unroll 4
-    for (int e = R.vptr[i]; e < R.vptr[i + 1]; ++e) {
-        const int ent = R.vidx[e], s = ent >> 1;
+    for (int e = vptr[i]; e < vptr[i + 1]; ++e) {
+        const int ent = vidx[e], s = ent >> 1;
         const T sg = (ent & 1) ? T(-1) : T(1);
         T gd[3], gw[3];
         g_load<ROWS_>(G, S, s, gd, gw);
@@ -288,11 +364,12 @@ __device__ __forceinline__ void gather_g(const Rob<T> &R, const T *G, int S, int
 }
 // Sum of the actuation-adjoint terms of springs e = e0, e0 + 64, ... < e1
 // of an actuator's spring list (one lane's share).
-template <bool ROWS_, typename T>
+template <bool ROWS_, bool TL, typename T>
 __device__ __forceinline__ T act_share(const Rob<T> &R, const T *G, int S, int e0, int e1) {
+    const auto aidx = as_ptr<TL>(R.aidx);
     T ab = T(0);
 #pragma unroll 4
-    for (int e = e0; e < e1; e += 64) ab += g_act<ROWS_>(G, S, R.aidx[e]);
+    for (int e = e0; e < e1; e += 64) ab += g_act<ROWS_>(G, S, aidx[e]);
     return ab;
 }
 
@@ -382,14 +459,26 @@ __device__ __forceinline__ bool w1_rows16(const Rob<T> &R, int H) {
 // threads tid < 64 return z for hidden unit tid (bias included).
 __device__ __forceinline__ float layer1_rows16(const float *W1T, const float *b1, const float *x,
                                                int I, float *red) {
-    const int tid = threadIdx.x, q = tid & 15, C = blockDim.x >> 4;
+    constexpr int U = 8;
+    const int tid = threadIdx.x, q = tid & 15, g = tid >> 4, C = blockDim.x >> 4;
     const float4 *w4 = reinterpret_cast<const float4 *>(W1T) + q;
     float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
-#pragma unroll 8
-    for (int i = tid >> 4; i < I; i += C) {
-        const float4 w = w4[(size_t)i * (HMAX / 4)];
-        const float xi = x[i];
-        a0 += w.x * xi; a1 += w.y * xi; a2 += w.z * xi; a3 += w.w * xi;
+    // U rows' loads issued before their products (held in distinct
+    // registers, so the loads overlap even where the kernel's register
+    // budget is tight); rows past I contribute nothing
+    for (int i0 = g; i0 < I; i0 += U * C) {
+        float4 w[U];
+        float xi[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + u * C;
+            w[u] = i < I ? w4[(size_t)i * (HMAX / 4)] : make_float4(0.f, 0.f, 0.f, 0.f);
+            xi[u] = i < I ? x[i] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            a0 += w[u].x * xi[u]; a1 += w[u].y * xi[u]; a2 += w[u].z * xi[u]; a3 += w[u].w * xi[u];
+        }
     }
 #pragma unroll
     for (int off = 16; off < 64; off <<= 1) {
@@ -552,20 +641,12 @@ __device__ void fwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int t, in
         __syncthreads();
     }
     // spring impulses
-    for (int s = tid; s < S; s += BT) {
-        const int a = R.sa[s], b = R.sb[s];
-        const T d[3] = {m.xs[a] - m.xs[b], m.xs[N + a] - m.xs[N + b], m.xs[2 * N + a] - m.xs[2 * N + b]};
-        const T w[3] = {m.vs[a] - m.vs[b], m.vs[N + a] - m.vs[N + b], m.vs[2 * N + a] - m.vs[2 * N + b]};
-        T J[3];
-        spring_impulse<T>(d, w, R.L0[s], R.k[s], R.coef[s], m.outb[R.sact[s]], A.P, J);
-        j_store(m.J, S, !A.springs_in_lds, s, J);
-    }
+    DL_PLACEMENT(A, impulse_pass, A, R, m);
     __syncthreads();
     // gather + integrate (vertex owned by one thread, in place)
     for (int i = tid; i < N; i += BT) {
         T vi[3] = {0, 0, 0};
-        if (A.springs_in_lds) gather_j<false>(R, m.J, S, i, vi);
-        else gather_j<true>(R, m.J, S, i, vi);
+        DL_PLACEMENT(A, gather_j, R, m.J, S, i, vi);
         if constexpr (ROWS) {
             // kept for the adjoint, which then skips its spring pass
             T *sv = R.svi + (size_t)slot * 3 * N;
@@ -651,14 +732,7 @@ __device__ void bwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int r, in
     T *G = m.J;  // overwrites the impulses once the integrator adjoint consumed them
     if constexpr (!STORED) {
         // recompute impulses
-        for (int s = tid; s < S; s += BT) {
-            const int a = R.sa[s], b = R.sb[s];
-            const T d[3] = {m.xs[a] - m.xs[b], m.xs[N + a] - m.xs[N + b], m.xs[2 * N + a] - m.xs[2 * N + b]};
-            const T w[3] = {m.vs[a] - m.vs[b], m.vs[N + a] - m.vs[N + b], m.vs[2 * N + a] - m.vs[2 * N + b]};
-            T J[3];
-            spring_impulse<T>(d, w, R.L0[s], R.k[s], R.coef[s], m.outb[R.sact[s]], A.P, J);
-            j_store(m.J, S, !A.springs_in_lds, s, J);
-        }
+        DL_PLACEMENT(A, impulse_pass, A, R, m);
         __syncthreads();
     }
     // integrate adjoint
@@ -668,8 +742,7 @@ __device__ void bwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int r, in
             const T *sv = R.svi + (size_t)slot * 3 * N;
             vi[0] = sv[i]; vi[1] = sv[N + i]; vi[2] = sv[2 * N + i];
         } else {
-            if (A.springs_in_lds) gather_j<false>(R, m.J, S, i, vi);
-            else gather_j<true>(R, m.J, S, i, vi);
+            DL_PLACEMENT(A, gather_j, R, m.J, S, i, vi);
         }
         const T x[3] = {m.xs[i], m.xs[N + i], m.xs[2 * N + i]};
         const T v[3] = {m.vs[i], m.vs[N + i], m.vs[2 * N + i]};
@@ -686,20 +759,11 @@ __device__ void bwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int r, in
     }
     __syncthreads();
     // spring adjoint
-    for (int s = tid; s < S; s += BT) {
-        const int a = R.sa[s], b = R.sb[s];
-        const T d[3] = {m.xs[a] - m.xs[b], m.xs[N + a] - m.xs[N + b], m.xs[2 * N + a] - m.xs[2 * N + b]};
-        const T w[3] = {m.vs[a] - m.vs[b], m.vs[N + a] - m.vs[N + b], m.vs[2 * N + a] - m.vs[2 * N + b]};
-        const T Jb[3] = {m.vib[b] - m.vib[a], m.vib[N + b] - m.vib[N + a], m.vib[2 * N + b] - m.vib[2 * N + a]};
-        T gd[3], gw[3], ga;
-        spring_impulse_adj<T>(d, w, R.L0[s], R.k[s], R.coef[s], m.outb[R.sact[s]], A.P, Jb, gd, gw, ga);
-        g_store(G, S, !A.springs_in_lds, s, gd, gw, ga);
-    }
+    DL_PLACEMENT(A, impulse_adj_pass, A, R, m);
     __syncthreads();
     for (int i = tid; i < N; i += BT) {
         T gx[3] = {0, 0, 0}, gv[3] = {0, 0, 0};
-        if (A.springs_in_lds) gather_g<false>(R, G, S, i, gx, gv);
-        else gather_g<true>(R, G, S, i, gx, gv);
+        DL_PLACEMENT(A, gather_g, R, G, S, i, gx, gv);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             m.xb[d * N + i] += gx[d];
@@ -713,8 +777,8 @@ __device__ void bwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int r, in
         const int lane = tid & 63, NW = BT >> 6;
         for (int o = tid >> 6; o < O; o += NW) {
             const int e0 = R.aptr[o] + lane, e1 = R.aptr[o + 1];
-            T ab = A.springs_in_lds ? act_share<false>(R, G, S, e0, e1)
-                                    : act_share<true>(R, G, S, e0, e1);
+            T ab = T(0);
+            DL_PLACEMENT(A, ab = act_share, R, G, S, e0, e1);
             ab = wave_sum(ab);
             if (lane == 0) {
                 const T z = ab * m.dout[o];
