@@ -39,6 +39,8 @@ struct RArgs {
     int max_n, max_s, max_i, max_o;
     int springs_in_lds;   // per-step spring buffers in LDS (else scratch)
     int topo_in_lds;      // design topology staged in LDS at launch start
+    int csr_in_lds;       // vertex / actuator spring lists in LDS (with the topology, or
+                          // alone when only they fit)
     int w_in_lds;         // W1 staged in LDS as [I][H+1] (serves W1 x and W1^T x)
 };
 
@@ -171,7 +173,7 @@ __device__ __forceinline__ Sm<T> carve(const RArgs<T> &A, unsigned char *raw, T 
 
 template <typename T>
 __host__ size_t smem_bytes(int max_n, int max_s, int max_i, int max_o, bool bwd, bool springs,
-                           bool topo, bool wl) {
+                           bool topo, bool wl, bool csr = false) {
     size_t n = 6 * (size_t)max_n + (bwd ? 9 * (size_t)max_n : 0);
     if (springs) n += (bwd ? 7 : 3) * (size_t)max_s;
     n += (bwd ? 2 : 1) * (size_t)max_i + 3 * HMAX + 3 * (size_t)max_o + 16 + 16 * HMAX + 96;
@@ -179,6 +181,7 @@ __host__ size_t smem_bytes(int max_n, int max_s, int max_i, int max_o, bool bwd,
     if (wl) n += (size_t)max_i * (HMAX + 1);
     size_t b = n * sizeof(T);
     if (topo) b += sizeof(int) * (6 * (size_t)max_s + max_n + max_o + 2);
+    else if (csr) b += sizeof(int) * (3 * (size_t)max_s + max_n + max_o + 2);
     return b;
 }
 
@@ -284,13 +287,15 @@ __device__ __forceinline__ T g_act(const T *G, int S, int s) {
     return as_ptr<true>(G)[6 * S + s];
 }
 
-#define DL_PLACEMENT(A_, FN_, ...)                                                       \
+// SPR_: spring rows in LDS; TL_: the arrays the pass indexes with are in LDS
+// (the spring arrays for the spring passes, the spring lists for the gathers)
+#define DL_PLACEMENT(SPR_, TL_, FN_, ...)                                                \
     do {                                                                                 \
-        if ((A_).springs_in_lds) {                                                       \
-            if ((A_).topo_in_lds) FN_<false, true>(__VA_ARGS__);                         \
+        if (SPR_) {                                                                      \
+            if (TL_) FN_<false, true>(__VA_ARGS__);                                      \
             else FN_<false, false>(__VA_ARGS__);                                         \
         } else {                                                                         \
-            if ((A_).topo_in_lds) FN_<true, true>(__VA_ARGS__);                          \
+            if (TL_) FN_<true, true>(__VA_ARGS__);                                       \
             else FN_<true, false>(__VA_ARGS__);                                          \
         }                                                                                \
     } while (0)
@@ -375,7 +380,8 @@ __device__ __forceinline__ T act_share(const Rob<T> &R, const T *G, int S, int e
 
 // Copy the robot's design topology (and W1^T when requested) into LDS and
 // repoint the robot view at the copies.  CSR rows are rebased to the copies.
-template <typename T>
+// CSR_: the launch may stage the spring lists alone (adjoint launches only).
+template <bool CSR_ = false, typename T>
 __device__ void stage(const RArgs<T> &A, Rob<T> &R, Sm<T> &m) {
     const int tid = threadIdx.x, BT = blockDim.x, S = R.S, N = R.N, O = R.O;
     if (A.topo_in_lds) {
@@ -395,6 +401,17 @@ __device__ void stage(const RArgs<T> &A, Rob<T> &R, Sm<T> &m) {
         for (int q = tid; q < na; q += BT) ax[q] = R.aidx[a0 + q];
         R.L0 = tr; R.k = tr + S; R.coef = tr + 2 * S;
         R.sa = ti; R.sb = ti + S; R.sact = ti + 2 * S;
+        R.vptr = vp; R.vidx = vx; R.aptr = ap; R.aidx = ax;
+    } else if (CSR_ && A.csr_in_lds) {
+        // only the vertex and actuator spring lists (the gathers' indices)
+        const int v0 = R.vptr[0], nv = R.vptr[N] - v0;
+        int *vp = m.topo_i, *vx = vp + N + 1;
+        for (int q = tid; q <= N; q += BT) vp[q] = R.vptr[q] - v0;
+        for (int q = tid; q < nv; q += BT) vx[q] = R.vidx[v0 + q];
+        const int a0 = R.aptr[0], na = R.aptr[O] - a0;
+        int *ap = vx + nv, *ax = ap + O + 1;
+        for (int q = tid; q <= O; q += BT) ap[q] = R.aptr[q] - a0;
+        for (int q = tid; q < na; q += BT) ax[q] = R.aidx[a0 + q];
         R.vptr = vp; R.vidx = vx; R.aptr = ap; R.aidx = ax;
     }
     if (A.w_in_lds) {
@@ -641,12 +658,12 @@ __device__ void fwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int t, in
         __syncthreads();
     }
     // spring impulses
-    DL_PLACEMENT(A, impulse_pass, A, R, m);
+    DL_PLACEMENT(A.springs_in_lds, A.topo_in_lds, impulse_pass, A, R, m);
     __syncthreads();
     // gather + integrate (vertex owned by one thread, in place)
     for (int i = tid; i < N; i += BT) {
         T vi[3] = {0, 0, 0};
-        DL_PLACEMENT(A, gather_j, R, m.J, S, i, vi);
+        DL_PLACEMENT(A.springs_in_lds, A.csr_in_lds, gather_j, R, m.J, S, i, vi);
         if constexpr (ROWS) {
             // kept for the adjoint, which then skips its spring pass
             T *sv = R.svi + (size_t)slot * 3 * N;
@@ -732,7 +749,7 @@ __device__ void bwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int r, in
     T *G = m.J;  // overwrites the impulses once the integrator adjoint consumed them
     if constexpr (!STORED) {
         // recompute impulses
-        DL_PLACEMENT(A, impulse_pass, A, R, m);
+        DL_PLACEMENT(A.springs_in_lds, A.topo_in_lds, impulse_pass, A, R, m);
         __syncthreads();
     }
     // integrate adjoint
@@ -742,7 +759,7 @@ __device__ void bwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int r, in
             const T *sv = R.svi + (size_t)slot * 3 * N;
             vi[0] = sv[i]; vi[1] = sv[N + i]; vi[2] = sv[2 * N + i];
         } else {
-            DL_PLACEMENT(A, gather_j, R, m.J, S, i, vi);
+            DL_PLACEMENT(A.springs_in_lds, A.csr_in_lds, gather_j, R, m.J, S, i, vi);
         }
         const T x[3] = {m.xs[i], m.xs[N + i], m.xs[2 * N + i]};
         const T v[3] = {m.vs[i], m.vs[N + i], m.vs[2 * N + i]};
@@ -759,11 +776,11 @@ __device__ void bwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int r, in
     }
     __syncthreads();
     // spring adjoint
-    DL_PLACEMENT(A, impulse_adj_pass, A, R, m);
+    DL_PLACEMENT(A.springs_in_lds, A.topo_in_lds, impulse_adj_pass, A, R, m);
     __syncthreads();
     for (int i = tid; i < N; i += BT) {
         T gx[3] = {0, 0, 0}, gv[3] = {0, 0, 0};
-        DL_PLACEMENT(A, gather_g, R, G, S, i, gx, gv);
+        DL_PLACEMENT(A.springs_in_lds, A.csr_in_lds, gather_g, R, G, S, i, gx, gv);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             m.xb[d * N + i] += gx[d];
@@ -778,7 +795,7 @@ __device__ void bwd_step(const RArgs<T> &A, const Rob<T> &R, Sm<T> &m, int r, in
         for (int o = tid >> 6; o < O; o += NW) {
             const int e0 = R.aptr[o] + lane, e1 = R.aptr[o + 1];
             T ab = T(0);
-            DL_PLACEMENT(A, ab = act_share, R, G, S, e0, e1);
+            DL_PLACEMENT(A.springs_in_lds, A.csr_in_lds, ab = act_share, R, G, S, e0, e1);
             ab = wave_sum(ab);
             if (lane == 0) {
                 const T z = ab * m.dout[o];
@@ -951,7 +968,7 @@ __global__ void __launch_bounds__(LB) k_rollout_bwd(RArgs<T> A) {
     const int r = blockIdx.x;
     Rob<T> R = make_rob(A, r);
     Sm<T> m = carve(A, smem_raw, R.spill, true);
-    stage(A, R, m);
+    stage<true>(A, R, m);
     const int N = R.N, tid = threadIdx.x, BT = blockDim.x, K = A.K;
     int tb = A.robots[r].t_bwd;
     if (tb <= 0) return;
@@ -2185,6 +2202,7 @@ static RArgs<T> make_rargs(const dl_rollout *r) {
     A.max_n = r->max_n; A.max_s = r->max_s; A.max_i = r->max_i; A.max_o = r->max_o;
     A.springs_in_lds = 1;
     A.topo_in_lds = 0;
+    A.csr_in_lds = 0;
     A.w_in_lds = 0;
     return A;
 }
@@ -2272,17 +2290,20 @@ static int launch_rollout(const dl_rollout *r, bool bwd, hipStream_t st) {
     // when the total stays within 80 KiB), 1 = topology only, 2 = topology +
     // W1, 3 = nothing staged, 4 = as much as fits in the whole LDS (spring
     // buffers, then topology, then W1; one workgroup per CU for big robots).
+    // Under policies 0 and 4 a robot whose topology does not fit keeps its
+    // vertex / actuator spring lists in LDS when they fit.
     const int pol = r->lds_policy;
-    auto bytes = [&](bool springs, bool topo, bool wl) {
-        return smem_bytes<T>(r->max_n, r->max_s, r->max_i, r->max_o, bwd, springs, topo, wl);
+    auto bytes = [&](bool springs, bool topo, bool wl, bool csr = false) {
+        return smem_bytes<T>(r->max_n, r->max_s, r->max_i, r->max_o, bwd, springs, topo, wl, csr);
     };
     const size_t cap = pol == 4 ? kLdsLimit : 64 * 1024;
     bool springs = bytes(true, false, false) <= cap;
-    bool topo = pol != 3, wl = pol == 2 || pol == 0;
+    bool topo = pol != 3, wl = pol == 2 || pol == 0, csr = false;
     if (pol == 0 || pol == 4) {
         topo = bytes(springs, true, false) <= cap;
         wl = topo && r->n_hidden == HMAX &&
              bytes(springs, true, true) <= (pol == 4 ? kLdsLimit : 80 * 1024);
+        csr = bwd && !topo && bytes(springs, false, false, true) <= cap;
     }
     // wave-per-robot kernel: FP32, hidden width 64, inputs <= 192, outputs <= 8
     const int variant = r->variant;
@@ -2307,12 +2328,13 @@ static int launch_rollout(const dl_rollout *r, bool bwd, hipStream_t st) {
     DL_CHECK_ARG(variant != 5 || r->scratch != nullptr, "rollout: stored rows need the scratch buffer");
     if (variant == 3 || (variant == 0 && pipe_ok)) return launch_wave<T>(r, A, bwd, true, false, st);
     if (variant == 2 || (variant == 0 && wave_ok)) return launch_wave<T>(r, A, bwd, false, false, st);
-    size_t sh = bytes(springs, topo, wl);
+    size_t sh = bytes(springs, topo, wl, csr);
     DL_CHECK_ARG(sh <= kLdsLimit, "rollout: robot too large for LDS (%zu bytes)", sh);
     DL_CHECK_ARG(springs || r->scratch != nullptr,
                  "rollout: robot spills springs to scratch; scratch required");
     A.springs_in_lds = springs;
     A.topo_in_lds = topo;
+    A.csr_in_lds = topo || csr;
     A.w_in_lds = wl;
     const bool big = r->block > 256, rows = variant == 5;
     const dim3 grid(r->n_robots), block(r->block);
