@@ -702,6 +702,102 @@ __device__ __forceinline__ void sense_block(bool valid, int g, const T nx[3], co
         if (acc[i] != T(0)) gadd(sens_k + i, acc[i]);
 }
 
+// ---------------------------------------------------------------- controller state
+template <typename T> struct Ctrl {
+    int P, Gs, n_sin, run_period, I, H, O, act;
+    T scale, max_speed;
+    const T *inv_count, *targets, *W1, *b1, *W2, *b2;
+    const int32_t *np, *hz;   // uneven batches (see MP); nullptr: even
+    int per_robot_weights;    // robot k's W1, b1, W2, b2 at k (H I), k H, k (O H), k O
+    int t;                    // the step (adjoint launches)
+    // robot k's weights
+    __device__ __forceinline__ const T *w1(int k) const { return W1 + (per_robot_weights ? (size_t)k * H * I : 0); }
+    __device__ __forceinline__ const T *bb1(int k) const { return b1 + (per_robot_weights ? (size_t)k * H : 0); }
+    __device__ __forceinline__ const T *w2(int k) const { return W2 + (per_robot_weights ? (size_t)k * O * H : 0); }
+    __device__ __forceinline__ const T *bb2(int k) const { return b2 + (per_robot_weights ? (size_t)k * O : 0); }
+};
+
+// The controller of step t for robot k, run by every thread of a launch of
+// nt = blockDim.x threads (a multiple of 64): the network input from the
+// sensor sums (taichi_env.py:149-183 order for the MPM quadruped), hidden and
+// output values (nn.py:130-150).  Layer 1 takes G = 4 threads per hidden unit
+// when nt >= 4 H (partial sums over every G-th input, combined by lane
+// shuffles), layer 2 one wavefront per output.  The outputs land in `so`
+// (LDS, O values); with `rows` the center, COM velocity, input, hidden and
+// output rows of step t are written too.  si, shd: LDS for I and H values.
+// For a robot that takes step t (t < hz[k]); every thread must call it.
+template <typename T>
+__device__ __forceinline__ void ctrl_step(const Ctrl<T> &c, int k, int t, const T *sens, T *center,
+                                          T *vcom, T *inp, T *hid, T *out, bool rows, T *si,
+                                          T *shd, T *so) {
+    const int tid = threadIdx.x, nt = blockDim.x, NS = 3 * c.Gs + 6;
+    const T *sk = sens + (size_t)k * NS;
+    const T invP = T(1) / T(c.np ? min(c.np[k], c.P) : c.P);
+    if (rows && tid < 3) {
+        center[k * 3 + tid] = sk[3 * c.Gs + tid] * invP;
+        vcom[k * 3 + tid] = sk[3 * c.Gs + 3 + tid] * invP;
+    }
+    const T *W1 = c.w1(k), *b1 = c.bb1(k), *W2 = c.w2(k), *b2 = c.bb2(k);
+    const T *tg = c.targets + (size_t)k * 5;
+    for (int j = tid; j < c.I; j += nt) {
+        T val;
+        if (j < c.n_sin) {
+            val = T(::sin(2.0 * M_PI / c.run_period * (double)t + 2.0 * M_PI / c.n_sin * (double)j));
+        } else if (j < c.n_sin + 3) {
+            val = sk[3 * c.Gs + 3 + (j - c.n_sin)] * invP;
+        } else if (j < c.n_sin + 3 + 3 * c.Gs) {
+            const int q = j - c.n_sin - 3, g = q / 3, d = q - 3 * g;
+            val = (sk[q] * c.inv_count[(size_t)k * c.Gs + g] - sk[3 * c.Gs + d] * invP) / c.scale;
+        } else if (j < c.n_sin + 6 + 3 * c.Gs) {
+            val = tg[j - c.n_sin - 3 - 3 * c.Gs] / c.max_speed;
+        } else {
+            val = tg[3];
+        }
+        si[j] = val;
+        if (rows) inp[(size_t)k * c.I + j] = val;
+    }
+    __syncthreads();
+    const int G = nt >= 4 * c.H ? 4 : 1;
+    for (int h0 = 0; h0 < c.H; h0 += nt / G) {
+        const int h = h0 + tid / G, part = tid % G;
+        T z = T(0);
+        if (h < c.H) {
+            const T *w = W1 + (size_t)h * c.I;
+            for (int i = part; i < c.I; i += G) z += w[i] * si[i];
+        }
+        if (G == 4) {
+            z += __shfl_xor(z, 1, 64);
+            z += __shfl_xor(z, 2, 64);
+        }
+        if (h < c.H && part == 0) {
+            T y, dy;
+            activate<T>(c.act, z + b1[h], y, dy);
+            shd[h] = y;
+            if (rows) hid[(size_t)k * c.H + h] = y;
+        }
+    }
+    __syncthreads();
+    const int lane = tid & 63;
+    for (int o = tid >> 6; o < c.O; o += nt >> 6) {
+        T z = T(0);
+        for (int h = lane; h < c.H; h += 64) z += W2[(size_t)o * c.H + h] * shd[h];
+        z = wave_sum(z);
+        if (lane == 0) {
+            T y, dy;
+            activate<T>(c.act, z + b2[o], y, dy);
+            so[o] = y;
+            if (rows) out[(size_t)k * c.O + o] = y;
+        }
+    }
+    __syncthreads();
+}
+
+// Controller rows of one step (CTRL launches of P2G): row pointers at step t.
+template <typename T> struct CtrlRows {
+    const T *sens;
+    T *center, *vcom, *inp, *hid, *out;
+};
+
 // ---------------------------------------------------------------- P2G
 // Particle range of this workgroup: window blockIdx.x of the window table,
 // or the blockIdx.x-th chunk of PB particles without one.
@@ -713,30 +809,58 @@ __device__ __forceinline__ Window window_of(const int32_t *win, int P) {
     return {win[2 * blockIdx.x], min(win[2 * blockIdx.x + 1], P)};
 }
 
-template <int D, typename T, bool STORE_F, int U>
+// CTRL: the step's controller runs at the start of every workgroup (after the
+// particle loads are issued, so their latency overlaps it) instead of in a
+// launch of its own; workgroup 0 of each robot writes the controller rows
+// (and the center / COM velocity of the final state of a robot whose horizon
+// ends at this step).  Otherwise the actuation comes from act.
+template <int D, typename T, bool STORE_F, int U, bool CTRL = false>
 __global__ __launch_bounds__(PB) void k_p2g_b(MP<T> p, PView<D, T> pv, const T *act,
                                               const int32_t *act_id, int n_act, T *F_out,
                                               T *grid, Blk bk, int32_t *err, TileBuf<T> tb,
-                                              const int32_t *win) {
+                                              const int32_t *win, Ctrl<T> cc = {},
+                                              CtrlRows<T> cr = {}) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *tile = reinterpret_cast<T *>(smem_raw);     // [tb.tn][4]
     __shared__ int red[2 * D * NWAVE];
-    if (idle_at(p, blockIdx.y, p.t)) return;
-    const Window wn = window_of(win, n_part(p, blockIdx.y, pv.P));
+    const int k = blockIdx.y;
+    if (idle_at(p, k, p.t)) {
+        if constexpr (CTRL) {
+            if (p.t == cc.hz[k] && blockIdx.x == 0 && threadIdx.x < 3) {
+                const T *sk = cr.sens + (size_t)k * (3 * cc.Gs + 6);
+                const T invP = T(1) / T(cc.np ? min(cc.np[k], cc.P) : cc.P);
+                cr.center[k * 3 + threadIdx.x] = sk[3 * cc.Gs + threadIdx.x] * invP;
+                cr.vcom[k * 3 + threadIdx.x] = sk[3 * cc.Gs + 3 + threadIdx.x] * invP;
+            }
+        }
+        return;
+    }
+    const Window wn = window_of(win, n_part(p, k, pv.P));
     if (threadIdx.x == 0 && wn.q1 - wn.q0 > PB) atomicOr(err, 2);
-    const int k = blockIdx.y, q = wn.q0 + (int)threadIdx.x, P = pv.P;
+    const int q = wn.q0 + (int)threadIdx.x, P = pv.P;
     bool valid = q >= 0 && q < wn.q1;
     T mv[D];
     Mat<D, T> aff;
     Stencil<D, T> s;
 #pragma unroll
     for (int d = 0; d < D; ++d) s.base[d] = 0;
+    T xs[D], vs[D];
+    Mat<D, T> Cm, Fm;
+    int id = -1;
     if (valid) {
-        T xs[D], vs[D];
-        Mat<D, T> Cm, Fm;
         pv.load(k, q, xs, vs, Cm, Fm);
-        const int id = act_id[(size_t)k * P + q];
-        const T a = id >= 0 ? act[(size_t)k * n_act + id] : T(0);
+        id = act_id[(size_t)k * P + q];
+    }
+    T a = T(0);
+    if constexpr (CTRL) {
+        T *si = tile, *shd = si + cc.I, *so = shd + cc.H;
+        ctrl_step<T>(cc, k, p.t, cr.sens, cr.center, cr.vcom, cr.inp, cr.hid, cr.out,
+                     blockIdx.x == 0, si, shd, so);
+        if (valid && id >= 0) a = so[id];
+    } else {
+        if (valid && id >= 0) a = act[(size_t)k * n_act + id];
+    }
+    if (valid) {
         const Consti<D, T> c = constitutive<D, T>(Cm, Fm, a, p);
         if (STORE_F) store_mat<D, T>(F_out, k, q, P, c.Fn, false);
         aff = c.affine;
@@ -1002,20 +1126,6 @@ __global__ __launch_bounds__(PB) void k_sense(int P, const int32_t *np, const T 
 }
 
 // ---------------------------------------------------------------- controller
-template <typename T> struct Ctrl {
-    int P, Gs, n_sin, run_period, I, H, O, act;
-    T scale, max_speed;
-    const T *inv_count, *targets, *W1, *b1, *W2, *b2;
-    const int32_t *np, *hz;   // uneven batches (see MP); nullptr: even
-    int per_robot_weights;    // robot k's W1, b1, W2, b2 at k (H I), k H, k (O H), k O
-    int t;                    // the step (adjoint launches)
-    // robot k's weights
-    __device__ __forceinline__ const T *w1(int k) const { return W1 + (per_robot_weights ? (size_t)k * H * I : 0); }
-    __device__ __forceinline__ const T *bb1(int k) const { return b1 + (per_robot_weights ? (size_t)k * H : 0); }
-    __device__ __forceinline__ const T *w2(int k) const { return W2 + (per_robot_weights ? (size_t)k * O * H : 0); }
-    __device__ __forceinline__ const T *bb2(int k) const { return b2 + (per_robot_weights ? (size_t)k * O : 0); }
-};
-
 // Block per robot: COM / COM velocity of step t from the sensor sums, and
 // with `mlp` the network input, hidden and output rows (taichi_env.py:149-183
 // sensor order for the MPM quadruped; nn.py:130-150).
@@ -1621,6 +1731,15 @@ int forward_impl(const dl_mpm_batch *b, const dl_mpm_slots *st, int t0, int n, i
     const int unroll = b->scatter_unroll;
     DL_UNROLL_DISPATCH(U_, DL_CHECK_HIP(ensure_dynamic_lds((const void *)k_p2g_b<D, T, true, U_>,
                                                            tile_bytes)));
+    const Ctrl<T> ctrl = make_ctrl<T>(b);
+    if constexpr (D == 3) {
+        if (control) {
+            DL_CHECK_ARG((size_t)(I + H + O) * sizeof(T) <= tile_bytes,
+                         "MPM batch: the controller's LDS rows exceed the scatter tile");
+            DL_UNROLL_DISPATCH(U_, DL_CHECK_HIP(ensure_dynamic_lds(
+                                       (const void *)k_p2g_b<D, T, true, U_, true>, tile_bytes)));
+        }
+    }
     if (control && (flags & DL_MPM_SENSE_FIRST)) {
         const Slot<D, T> s0 = slot<D, T>(st, 0, B, P);
         if constexpr (D == 3)
@@ -1642,17 +1761,30 @@ int forward_impl(const dl_mpm_batch *b, const dl_mpm_slots *st, int t0, int n, i
                                (const T *)a.F, c.F);
             continue;
         }
-        if (control)
-            hipLaunchKernelGGL(k_ctrl<T>, dim3(B), dim3(64), (I + H) * sizeof(T), s,
-                               make_ctrl<T>(b), t, (const T *)b->sens + (size_t)t * B * NS,
-                               (T *)b->center + (size_t)t * B * 3, (T *)b->vcom + (size_t)t * B * 3,
-                               (T *)b->inp + (size_t)t * B * I, (T *)b->hid + (size_t)t * B * H,
-                               (T *)b->act_out + (size_t)t * B * O, 1);
         PView<D, T> pv{a.x, a.v, a.C, a.F, P};
-        DL_UNROLL_DISPATCH(U_, hipLaunchKernelGGL((k_p2g_b<D, T, true, U_>), pw, dim3(PB),
-                                                  tile_bytes, s, mp, pv, act + (size_t)t * B * O,
-                                                  b->act_id, O, c.F, (T *)b->grid, bk, b->err,
-                                                  tb, b->windows));
+        bool fused = false;
+        if constexpr (D == 3) {
+            if (control) {
+                // the step's controller runs inside P2G (no launch of its own)
+                const CtrlRows<T> cr{(const T *)b->sens + (size_t)t * B * NS,
+                                     (T *)b->center + (size_t)t * B * 3,
+                                     (T *)b->vcom + (size_t)t * B * 3,
+                                     (T *)b->inp + (size_t)t * B * I,
+                                     (T *)b->hid + (size_t)t * B * H,
+                                     (T *)b->act_out + (size_t)t * B * O};
+                DL_UNROLL_DISPATCH(U_, hipLaunchKernelGGL((k_p2g_b<D, T, true, U_, true>), pw,
+                                                          dim3(PB), tile_bytes, s, mp, pv,
+                                                          act + (size_t)t * B * O, b->act_id, O,
+                                                          c.F, (T *)b->grid, bk, b->err, tb,
+                                                          b->windows, ctrl, cr));
+                fused = true;
+            }
+        }
+        if (!fused)
+            DL_UNROLL_DISPATCH(U_, hipLaunchKernelGGL((k_p2g_b<D, T, true, U_>), pw, dim3(PB),
+                                                      tile_bytes, s, mp, pv, act + (size_t)t * B * O,
+                                                      b->act_id, O, c.F, (T *)b->grid, bk, b->err,
+                                                      tb, b->windows));
         const GridSave<T> sv = (flags & DL_MPM_SAVE_GRID)      ? save_at<D, T>(b, j)
                                : (flags & DL_MPM_SAVE_HORIZON) ? hsave_at<T>(b, t)
                                                                : GridSave<T>{nullptr, nullptr, nullptr, nullptr, 0, nullptr};
