@@ -24,6 +24,24 @@ from .rollout import GraphedIteration, MassSpringBatch
 WAVE_MAX_IN, WAVE_MAX_OUT, WAVE_MAX_VERT = 192, 8, 64
 
 
+_CLASS_STREAMS = {}
+
+
+def _class_streams(n, device):
+    """The first n of a process-wide list of streams for size classes.  The
+    HIP runtime binds a stream to one of its hardware queues (four by
+    default) when the stream is first used, in round-robin order; streams
+    created per batch, and so handed out by torch's pool in whatever order,
+    can land two classes on one queue, where their launches run one after
+    the other.  Reused streams, first used together (classes are issued
+    largest first), keep the mapping the same for every batch."""
+    key = str(torch.device(device))
+    lst = _CLASS_STREAMS.setdefault(key, [])
+    while len(lst) < n:
+        lst.append(torch.cuda.Stream(device=device))
+    return lst[:n]
+
+
 def n_inputs(layout, n_vert):
     """Network input width of a robot with n_vert vertices (taichi_env.py:149-183 order)."""
     lay = layout
@@ -76,7 +94,9 @@ class UnevenBatch:
                 self.slot[r] = (len(self.buckets), j)
             self.buckets.append({"name": name, "ids": np.asarray(ids), "batch": b, "config": None})
         self.graphs = None
-        self._streams = [torch.cuda.Stream(device=device) for _ in self.buckets] if streams else None
+        n = len(self.buckets)
+        # the largest class on the first shared stream, the next on the second ...
+        self._streams = _class_streams(n, device)[::-1] if streams else None
 
     # ------------------------------------------------------------ weights
     def set_weights(self, r, W1, b1, W2, b2):
@@ -128,7 +148,9 @@ class UnevenBatch:
 
     def tune(self, tuner=None, quick=False):
         """Pick each size class's launch shape by timing a full iteration of
-        that class with every candidate (cached by the autotuner)."""
+        that class with every candidate (cached by the autotuner); without
+        ``quick``, keep these picks only if the concurrent iteration is faster
+        with them than with the quick candidates' picks."""
         tuner = tuner or Autotuner()
         for bk in self.buckets:
             b = bk["batch"]
@@ -152,7 +174,37 @@ class UnevenBatch:
 
             self._apply(bk, tuner.pick(key + f"|quick={int(quick)}",
                                        self.candidates(bk["name"], quick), measure))
+        if not quick:
+            # each class was timed alone; concurrently a shape that wins alone
+            # can lose (a wider workgroup waiting for a CU the other classes
+            # hold), so the whole iteration is timed with these shapes and with
+            # the quick candidates' picks, and the faster set is kept
+            full = [bk["config"] for bk in self.buckets]
+            quick_set = self.tune(tuner, quick=True)
+            if full != quick_set:
+                t_quick = self._iteration_ms()
+                for bk, cfg in zip(self.buckets, full):
+                    self._apply(bk, cfg)
+                # the classes' placement on the CUs varies from run to run, so
+                # the per-class picks must win clearly to be kept
+                if self._iteration_ms() > 0.97 * t_quick:
+                    for bk, cfg in zip(self.buckets, quick_set):
+                        self._apply(bk, cfg)
         return [bk["config"] for bk in self.buckets]
+
+    def _iteration_ms(self, reps=3):
+        """Median of ``reps`` concurrent iterations (after one warm-up), ms."""
+        self.prepare()
+        self.run_iteration()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ts = []
+        for _ in range(reps):
+            e0.record()
+            self.run_iteration()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return float(np.median(ts))
 
     def class_times(self):
         """Milliseconds of one forward + adjoint iteration of each size class
@@ -237,13 +289,16 @@ def tune_schedule(designs, horizons, physics, layout, targets, set_weights=None,
         ub.prepare()
         ub.run_iteration()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        ub.run_iteration()
-        e1.record()
-        torch.cuda.synchronize()
+        best = float("inf")
+        for _ in range(2):          # the better of two iterations (concurrent classes vary)
+            e0.record()
+            ub.run_iteration()
+            e1.record()
+            torch.cuda.synchronize()
+            best = min(best, e0.elapsed_time(e1))
         del ub
         torch.cuda.empty_cache()
-        return e0.elapsed_time(e1)
+        return best
 
     pick = tuner.pick(key, [{"bounds": list(b)} for b in schedules], measure, repeat=1)
     return tuple(pick["bounds"]), tuner.cache.get(key, {}).get("timings")
