@@ -365,8 +365,6 @@ def main():
     mlp = None if args.skip_mlp else mlp_ab(R, I, H, O)
 
     # ---- uneven batch: 512 seeded random voxel robots, size classes, HIP graphs
-    # (run before the MPM leg: measured after the MPM leg's 27-65 GB of
-    # allocations were freed, the same iteration took 1.5x as long)
     uneven = None
     if not args.skip_uneven:
         from diffloco import robot_design as rdm
