@@ -1126,68 +1126,24 @@ __global__ __launch_bounds__(PB) void k_sense(int P, const int32_t *np, const T 
 }
 
 // ---------------------------------------------------------------- controller
-// Block per robot: COM / COM velocity of step t from the sensor sums, and
-// with `mlp` the network input, hidden and output rows (taichi_env.py:149-183
-// sensor order for the MPM quadruped; nn.py:130-150).
+// COM and COM velocity of the final state (step t = horizon) from its sensor
+// sums, one 64-thread block per robot; the steps before it compute theirs
+// with the controller inside P2G (ctrl_step).
 template <typename T>
-__global__ __launch_bounds__(64) void k_ctrl(Ctrl<T> c, int t, const T *sens, T *center, T *vcom,
-                                             T *inp, T *hid, T *out, int mlp) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T *si = reinterpret_cast<T *>(smem_raw);
-    T *shd = si + c.I;
+__global__ __launch_bounds__(64) void k_center(Ctrl<T> c, int t, const T *sens, T *center,
+                                               T *vcom) {
     const int k = blockIdx.x, tid = threadIdx.x;
-    const int NS = 3 * c.Gs + 6;
-    // robot k: state t exists for t <= hz[k]; the controller runs for t < hz[k]
-    if (c.hz && t > c.hz[k]) return;
-    if (c.hz && t == c.hz[k]) mlp = 0;
-    const T *sk = sens + (size_t)k * NS;
+    // robot k: state t exists for t <= hz[k] (its last one written in P2G)
+    if (c.hz && t != c.hz[k]) return;
+    const T *sk = sens + (size_t)k * (3 * c.Gs + 6);
     const T invP = T(1) / T(c.np ? min(c.np[k], c.P) : c.P);
     if (tid < 3) {
         center[k * 3 + tid] = sk[3 * c.Gs + tid] * invP;
         vcom[k * 3 + tid] = sk[3 * c.Gs + 3 + tid] * invP;
     }
-    if (!mlp) return;
-    const T *W1 = c.w1(k), *b1 = c.bb1(k), *W2 = c.w2(k), *b2 = c.bb2(k);
-    const T *tg = c.targets + (size_t)k * 5;
-    for (int j = tid; j < c.I; j += 64) {
-        T val;
-        if (j < c.n_sin) {
-            val = T(::sin(2.0 * M_PI / c.run_period * (double)t + 2.0 * M_PI / c.n_sin * (double)j));
-        } else if (j < c.n_sin + 3) {
-            val = sk[3 * c.Gs + 3 + (j - c.n_sin)] * invP;
-        } else if (j < c.n_sin + 3 + 3 * c.Gs) {
-            const int q = j - c.n_sin - 3, g = q / 3, d = q - 3 * g;
-            val = (sk[q] * c.inv_count[(size_t)k * c.Gs + g] - sk[3 * c.Gs + d] * invP) / c.scale;
-        } else if (j < c.n_sin + 6 + 3 * c.Gs) {
-            val = tg[j - c.n_sin - 3 - 3 * c.Gs] / c.max_speed;
-        } else {
-            val = tg[3];
-        }
-        si[j] = val;
-        inp[(size_t)k * c.I + j] = val;
-    }
-    __syncthreads();
-    for (int h = tid; h < c.H; h += 64) {
-        T z = b1[h];
-        const T *w = W1 + (size_t)h * c.I;
-        for (int i = 0; i < c.I; ++i) z += w[i] * si[i];
-        T y, dy;
-        activate<T>(c.act, z, y, dy);
-        shd[h] = y;
-        hid[(size_t)k * c.H + h] = y;
-    }
-    __syncthreads();
-    for (int o = tid; o < c.O; o += 64) {
-        T z = b2[o];
-        const T *w = W2 + (size_t)o * c.H;
-        for (int h = 0; h < c.H; ++h) z += w[h] * shd[h];
-        T y, dy;
-        activate<T>(c.act, z, y, dy);
-        out[(size_t)k * c.O + o] = y;
-    }
 }
 
-// Adjoint of k_ctrl (with mlp) at step t for robot k, by the nt threads of
+// Adjoint of the controller (ctrl_step) at step t for robot k, by the nt threads of
 // a workgroup (each entry computed by one thread, in the same order for any
 // nt): consumes (and zeroes) act_b, writes in_b row t and the pre-activation
 // adjoints of the step (zh: hidden [B][H], zo: output [B][O]); k_wgrad_time
@@ -1801,11 +1757,10 @@ int forward_impl(const dl_mpm_batch *b, const dl_mpm_slots *st, int t0, int n, i
                                SenseOut<T>{nullptr, nullptr, 0});
     }
     if (control && t0 + n == b->horizon)
-        hipLaunchKernelGGL(k_ctrl<T>, dim3(B), dim3(64), (I + H) * sizeof(T), s, make_ctrl<T>(b),
-                           t0 + n, (const T *)b->sens + (size_t)(t0 + n) * B * NS,
+        hipLaunchKernelGGL(k_center<T>, dim3(B), dim3(64), 0, s, ctrl, t0 + n,
+                           (const T *)b->sens + (size_t)(t0 + n) * B * NS,
                            (T *)b->center + (size_t)(t0 + n) * B * 3,
-                           (T *)b->vcom + (size_t)(t0 + n) * B * 3, (T *)nullptr, (T *)nullptr,
-                           (T *)nullptr, 0);
+                           (T *)b->vcom + (size_t)(t0 + n) * B * 3);
     DL_CHECK_LAUNCH();
     return 0;
 }

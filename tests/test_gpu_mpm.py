@@ -296,7 +296,10 @@ def test_mpm_uneven_batch_matches_robots_alone(prec):
     grads = ub.backward(loss_grad=torch.ones(3, dtype=torch.float64))
     torch.cuda.synchronize()
     ub.check()
-    tl, tg = (1e-11, 1e-9) if prec == "f64" else (1e-5, 1e-4)
+    # FP32 gradients: two identical runs of the 343-particle robot differ by
+    # up to 1.5e-4 relative (P2G atomic order over 13 steps), so the bound is
+    # the 1e-3 gradient tolerance; FP64 agrees to rounding.
+    tl, tg = (1e-11, 1e-9) if prec == "f64" else (1e-5, 1e-3)
     for k in range(3):
         one = dm.MPMBatch(params, bodies[k][0], bodies[k][1], 4, 1, hz[k], K=K,
                           window=min(W, hz[k]), dtype=dt, save_horizon=False)
