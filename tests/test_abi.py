@@ -101,3 +101,19 @@ def test_invalid_arguments_are_rejected_without_a_gpu():
                       params=dm.quadruped3d_params(n_grid=30))
     assert L.dl_mpm_forward(ctypes.byref(mb), None, 0, 1, 0, None) == -1
     assert "multiple of 4" in nat.last_error()
+
+
+def test_empty_batches_are_argument_errors():
+    """A batch with no robots, or an MPM batch with no steps, is rejected
+    with DL_E_ARG before any launch (not run as an empty grid)."""
+    L = nat.lib()
+    D = nat.Rollout()
+    D.dtype, D.act, D.n_robots = nat.F32, nat.ACT_TANH, 0
+    for fn in (L.dl_ms_rollout_forward, L.dl_ms_rollout_backward):
+        assert fn(ctypes.byref(D), None) == -1
+        assert "robot count" in nat.last_error()
+    for B, horizon in ((0, 10), (1, 0)):
+        mb = dm.MPMBatchC(dtype=0, B=B, P=64, n_act=1, horizon=horizon, tile_nodes=2048,
+                          scatter=dm.SCATTER_ATOMIC, params=dm.quadruped3d_params(n_grid=32))
+        assert L.dl_mpm_forward(ctypes.byref(mb), None, 0, 1, 0, None) == -1
+        assert "bad sizes" in nat.last_error()
