@@ -3,8 +3,8 @@
 TEST INFRASTRUCTURE ONLY.  Nothing in the product package may import, call
 or execute anything under this directory; only ``tests/``,
 ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` use
-it, and only as the checker (or as the timed CPU baseline), never as the thing
-measured or shipped.
+it, and only as the reference results are compared against (or as the timed
+CPU baseline), never as the thing measured or shipped.
 
 The oracle restates the reference's algorithm (Python + Taichi, FP64
 everywhere: ``multitask/utils.py:12-15``) in FP64 on the CPU:

@@ -3,7 +3,7 @@
  * recomputed in the backward, one Adam step.
  *
  * TEST INFRASTRUCTURE AND CPU BASELINE ONLY.  Nothing in the product package
- * loads this library; only tests/ (as a checker against the torch oracle)
+ * loads this library; only tests/ (compared against the torch oracle)
  * and the cpu_baseline leg of bench.py (as the timed CPU figure) do.
  *
  * Built twice by oracle/build_cpu.py: REAL = float (the baseline) and
