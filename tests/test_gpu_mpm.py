@@ -614,3 +614,50 @@ def test_mpm_rollout_autograd_function():
     assert abs(loss.item() - lo.item()) < 1e-10 * abs(lo.item())
     for a, b in zip(wd, wr):
         assert rel_err(a.grad.cpu(), b.grad) < 1e-7
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_mpm_sensor_entry_points_match_oracle(prec):
+    """dl_mpm_sensor / dl_mpm_sensor_adj (an MPM robot's controller input,
+    COM and COM velocity, and the input adjoint) against the oracle's sensor
+    vector and torch autograd of it; one actuator group is left empty (its
+    offset is -COM / scale)."""
+    from diffloco import _native as nat
+    _say(f"mpm sensor entry points {prec}")
+    dt, f64 = DT[prec], torch.float64
+    g = torch.Generator().manual_seed(12)
+    B, P, G, n_sin, period, t, scale, vmax = 3, 700, 5, 10, 100, 37, 0.2, 0.08
+    x = (0.2 + 0.5 * torch.rand(B, P, 3, generator=g, dtype=f64)).requires_grad_()
+    v = (0.1 * torch.randn(B, P, 3, generator=g, dtype=f64)).requires_grad_()
+    group = torch.randint(0, G - 1, (P,), generator=g)          # group G - 1 empty
+    tv = 0.08 * torch.rand(B, 3, generator=g, dtype=f64)
+    th = 0.1 + 0.1 * torch.rand(B, generator=g, dtype=f64)
+    inp, c, vc = omp.sensors(t, x, v, group, G, n_sin, period, scale, vmax, tv, th)
+    I = inp.shape[1]
+    gin = torch.randn(B, I, generator=g, dtype=f64)
+    xg, vg = torch.autograd.grad((inp * gin).sum(), (x, v))
+    cnt = torch.bincount(group, minlength=G).to(f64)
+    inv = torch.where(cnt > 0, 1.0 / cnt.clamp_min(1), torch.zeros_like(cnt))
+
+    def dev(a):
+        return a.detach().to("cuda", dt).contiguous()
+
+    xd, vd = dev(x.transpose(1, 2)), dev(v.transpose(1, 2))         # [B][3][P]
+    gd = group.to(torch.int32)[None].expand(B, -1).contiguous().cuda()
+    tgd = dev(torch.cat([tv, th[:, None], torch.zeros(B, 1, dtype=f64)], 1))
+    invd = dev(inv[None].expand(B, -1))
+    ind = torch.empty(B, I, device="cuda", dtype=dt)
+    cd = torch.empty(B, 3, device="cuda", dtype=dt)
+    vcd = torch.empty(B, 3, device="cuda", dtype=dt)
+    dc = nat.dtype_code(dt)
+    nat.call("dl_mpm_sensor", dc, B, P, G, n_sin, period, t, scale, vmax,
+             *map(nat.ptr, (xd, vd, gd, tgd, invd, ind, cd, vcd)), nat.stream())
+    xgd, vgd = torch.zeros_like(xd), torch.zeros_like(vd)
+    nat.call("dl_mpm_sensor_adj", dc, B, P, G, n_sin, scale, nat.ptr(gd), nat.ptr(invd),
+             nat.ptr(dev(gin)), nat.ptr(xgd), nat.ptr(vgd), nat.stream())
+    torch.cuda.synchronize()
+    tol = 1e-12 if prec == "f64" else 1e-5
+    assert rel_err(ind, inp.detach()) < tol
+    assert rel_err(cd, c.detach()) < tol and rel_err(vcd, vc.detach()) < tol
+    assert rel_err(xgd.transpose(1, 2), xg) < tol
+    assert rel_err(vgd.transpose(1, 2), vg) < tol
