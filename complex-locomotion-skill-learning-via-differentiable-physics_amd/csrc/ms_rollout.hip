@@ -18,6 +18,11 @@ namespace dl {
 
 constexpr int HMAX = 64;
 
+// Instruction classes of __builtin_amdgcn_sched_group_barrier (LLVM AMDGPU):
+// the wave kernels pin "all LDS reads, then the arithmetic" in their LDS-bound
+// inner loops, which the compiler otherwise serializes under register pressure.
+constexpr int kSchedValu = 0x002, kSchedDsRead = 0x100;
+
 template <typename T>
 struct RArgs {
     const dl_robot *robots;
@@ -1135,7 +1140,7 @@ struct WShared {
 
 // `regions`: reals taken by the wavefronts' own regions at the start of LDS.
 template <typename T>
-__device__ WShared<T> wave_stage(const RArgs<T> &A, const Rob<T> &R0, unsigned char *raw, bool bwd,
+__device__ __forceinline__ WShared<T> wave_stage(const RArgs<T> &A, const Rob<T> &R0, unsigned char *raw, bool bwd,
                                  size_t regions) {
     WShared<T> sh;
     T *p = reinterpret_cast<T *>(raw) + regions;
@@ -1435,10 +1440,25 @@ __device__ __forceinline__ void wave_fwd_step(const RArgs<T> &A, const Rob<T> &R
     // recompute reproduces the forward's hidden values bit for bit.
     T acc = w.b1;
     if (R.WL) {
+        // 16 inputs per round: their 32 LDS reads are issued before the FMAs
+        // (the scheduling groups pin that order; left to itself the compiler
+        // reuses one register pair and waits on every read once the kernel's
+        // register pressure grows)
         T a[4] = {acc, T(0), T(0), T(0)};
         const T *wl = R.WL + lane;
-#pragma unroll 16
-        for (int i = 0; i < I; ++i) a[i & 3] += wl[i * (HMAX + 1)] * m.inb[i];
+        int i = 0;
+        for (; i + 16 <= I; i += 16) {
+            T wv[16], xv[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) wv[q] = wl[(i + q) * (HMAX + 1)];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) xv[q] = m.inb[i + q];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) a[q & 3] += wv[q] * xv[q];
+            __builtin_amdgcn_sched_group_barrier(kSchedDsRead, 32, 0);
+            __builtin_amdgcn_sched_group_barrier(kSchedValu, 16, 0);
+        }
+        for (; i < I; ++i) a[i & 3] += wl[i * (HMAX + 1)] * m.inb[i];
         acc = (a[0] + a[1]) + (a[2] + a[3]);
     } else if (lane < H) {
 #pragma unroll 8
@@ -1592,15 +1612,23 @@ __device__ __forceinline__ void pref_load(const Rob<T> &R, int H, int lane, int 
 }
 
 // acc[p] += rows[p][j0 + k] * z_k in order k = 0..15, z_k from lane k of the
-// lane's 16-lane row of z.
+// lane's 16-lane row of z.  The 16 NP row entries are read from LDS into
+// registers before the FMAs (order pinned by scheduling groups).
 template <int NP, int... K>
 __device__ __forceinline__ void in_adj_rows(float (&acc)[NP], const float *const (&rows)[NP], int j0,
                                             float z, std::integer_sequence<int, K...>) {
+    float rv[NP][16];
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+#pragma unroll
+        for (int k = 0; k < 16; ++k) rv[p][k] = rows[p][j0 + k];
     auto one = [&](float zk, int k) {
 #pragma unroll
-        for (int p = 0; p < NP; ++p) acc[p] += rows[p][j0 + k] * zk;
+        for (int p = 0; p < NP; ++p) acc[p] += rv[p][k] * zk;
     };
     (one(row_bcast<K>(z), K), ...);
+    __builtin_amdgcn_sched_group_barrier(kSchedDsRead, 16 * NP, 0);
+    __builtin_amdgcn_sched_group_barrier(kSchedValu, 16 * (NP + 1), 0);
 }
 
 // VI: the step's impulse sums come from the rows the forward kept (no spring
@@ -1819,58 +1847,140 @@ __device__ __forceinline__ void wave_bwd_step(const RArgs<T> &A, const Rob<T> &R
 }
 
 // Weight-gradient fold of n steps starting at scratch slot `off`:
-// W1g[j][i] += sum_s z1[s][j] in[s][i].  Lanes run over inputs i with 64
-// accumulators (one per hidden unit); z1 rows are staged through LDS (`zl`,
-// the free spring buffer) in chunks so the inner loop reads LDS broadcasts.
-template <typename T>
+// W1g[j][i] += sum_s z1[s][j] in[s][i], then the b1, W2 and b2 gradients.
+// Lanes run over inputs i with 64 accumulators (one per hidden unit).  The
+// steps go in chunks: a chunk's rows (z1 and the lane's inputs) are loaded
+// with every load issued before any is used, staged in LDS (`zl`, zcap
+// reals that are dead while the fold runs), then summed from LDS (z1 as
+// broadcasts).  The rows were written a segment earlier and come from HBM,
+// so one load in flight at a time would cost a memory round trip per step.
+// The loads walk per-lane pointers (the hidden rows are HMAX wide in the
+// wave kernels, so their offsets are immediates): the kernel has no scalar
+// registers to spare for a row address per load.  Steps are summed in
+// ascending order.
+constexpr int kFoldChunk = 16;
+// LDS reals per staged step: z1 and input rows (first pass); z1, hidden and
+// z2 rows (second pass, z2 at most 8 wide)
+constexpr int kFoldStepReals = 2 * HMAX + 8;
+
+// Reals of a wave region free while the fold runs: its spring rows and its
+// state rows (`bwd`: the adjoint region, else the recompute region), which
+// lie back to back at the start of the region.
+__host__ __device__ inline int wave_fold_cap(int max_n, int max_s, bool bwd) {
+    return (bwd ? 8 : 4) * max_s + 8 * max_n;
+}
+
+template <int OMAX, typename T>
 __device__ __forceinline__ void wave_seg_wgrad(const RArgs<T> &A, const Rob<T> &R, T *zl, int zcap,
                                                int lane, int off, int n) {
-    const int I = R.I, O = R.O, H = A.H;
-    const int chunk = zcap / HMAX;
+    static_assert(OMAX <= kFoldStepReals - 2 * HMAX, "z2 rows wider than the fold's staging");
+    const int I = R.I, O = R.O;
+    constexpr int CB = kFoldChunk;
+    const int chunk = min(zcap / (2 * HMAX), CB);
+    T *const xl = zl + chunk * HMAX;
     for (int i0 = 0; i0 < I; i0 += 64) {
         const int i = i0 + lane;
         const bool ok = i < I;
         T acc[HMAX];
 #pragma unroll
         for (int j = 0; j < HMAX; ++j) acc[j] = T(0);
+        const T *zp = R.sz1 + (size_t)off * HMAX + lane;
+        const T *xp = R.sin + (size_t)off * I + (ok ? i : 0);
         for (int c0 = 0; c0 < n; c0 += chunk) {
             const int cn = min(chunk, n - c0);
-            wave_sync();
-            for (int q = lane; q < cn * HMAX; q += 64) {
-                const int ss = q / HMAX, j = q - ss * HMAX;
-                zl[q] = j < H ? R.sz1[(size_t)(off + c0 + ss) * H + j] : T(0);
+            T zr[CB], xr[CB];
+            const T *xq = xp;
+#pragma unroll
+            for (int u = 0; u < CB; ++u) {
+                zr[u] = xr[u] = T(0);
+                if (u < cn) {
+                    zr[u] = zp[u * HMAX];
+                    xr[u] = *xq;
+                }
+                xq += I;
+            }
+            zp += cn * HMAX;
+            xp += (size_t)cn * I;
+            wave_sync();   // the previous chunk's reads of the stage are done
+#pragma unroll
+            for (int u = 0; u < CB; ++u) {
+                if (u < cn) {
+                    zl[u * HMAX + lane] = zr[u];
+                    xl[u * HMAX + lane] = xr[u];
+                }
             }
             wave_sync();
-            for (int ss = 0; ss < cn; ++ss) {
-                const T xin = ok ? R.sin[(size_t)(off + c0 + ss) * I + i] : T(0);
-                const T *z = zl + ss * HMAX;
+            for (int u = 0; u < cn; ++u) {
+                const T x = xl[u * HMAX + lane];
+                const T *z = zl + u * HMAX;
 #pragma unroll
-                for (int j = 0; j < HMAX; ++j) acc[j] += z[j] * xin;
+                for (int j = 0; j < HMAX; ++j) acc[j] += z[j] * x;
             }
         }
         if (ok) {
+            // 16 entries' loads in flight, then their stores (entry by entry,
+            // each load would wait behind the store before it: rows j I + i
+            // may alias for all the compiler knows)
+            T *const wp = R.Wg1 + i;
 #pragma unroll
-            for (int j = 0; j < HMAX; ++j)
-                if (j < H) R.Wg1[(size_t)j * I + i] += acc[j];
+            for (int j0 = 0; j0 < HMAX; j0 += 16) {
+                T g[16];
+#pragma unroll
+                for (int q = 0; q < 16; ++q) g[q] = wp[(size_t)(j0 + q) * I];
+#pragma unroll
+                for (int q = 0; q < 16; ++q) wp[(size_t)(j0 + q) * I] = g[q] + acc[j0 + q];
+            }
         }
     }
-    wave_sync();
-    if (lane < H) {
-        T b = T(0);
-        for (int ss = 0; ss < n; ++ss) b += R.sz1[(size_t)(off + ss) * H + lane];
-        R.bg1[lane] += b;
-        for (int o = 0; o < O; ++o) {
-            T a = T(0);
-            for (int ss = 0; ss < n; ++ss)
-                a += R.sz2[(size_t)(off + ss) * O + o] * R.shid[(size_t)(off + ss) * H + lane];
-            R.Wg2[(size_t)o * H + lane] += a;
+    // lane j sums z1[s][j] and z2[s][o] hid[s][j]; lane o < O sums z2[s][o]
+    const int chunk2 = min(zcap / kFoldStepReals, CB);
+    T *const hl = zl + chunk2 * HMAX, *const ql = hl + chunk2 * HMAX;
+    T b = T(0), c = T(0), a[OMAX];
+#pragma unroll
+    for (int o = 0; o < OMAX; ++o) a[o] = T(0);
+    const T *zp = R.sz1 + (size_t)off * HMAX + lane;
+    const T *hp = R.shid + (size_t)off * HMAX + lane;
+    const T *qp = R.sz2 + (size_t)off * O + (lane < O ? lane : 0);
+    for (int c0 = 0; c0 < n; c0 += chunk2) {
+        const int cn = min(chunk2, n - c0);
+        T zr[CB], hr[CB], qr[CB];
+#pragma unroll
+        for (int u = 0; u < CB; ++u) {
+            zr[u] = hr[u] = qr[u] = T(0);
+            if (u < cn) {
+                zr[u] = zp[u * HMAX];
+                hr[u] = hp[u * HMAX];
+                qr[u] = qp[u * O];
+            }
+        }
+        zp += cn * HMAX;
+        hp += cn * HMAX;
+        qp += cn * O;
+        wave_sync();
+#pragma unroll
+        for (int u = 0; u < CB; ++u) {
+            if (u < cn) {
+                zl[u * HMAX + lane] = zr[u];
+                hl[u * HMAX + lane] = hr[u];
+                if (lane < OMAX) ql[u * OMAX + lane] = lane < O ? qr[u] : T(0);
+            }
+        }
+        wave_sync();
+        for (int u = 0; u < cn; ++u) {
+            const T h = hl[u * HMAX + lane];
+            const T *q = ql + u * OMAX;
+            b += zl[u * HMAX + lane];
+            c += q[lane < OMAX ? lane : 0];
+#pragma unroll
+            for (int o = 0; o < OMAX; ++o)
+                if (o < O) a[o] += q[o] * h;
         }
     }
-    for (int o = lane; o < O; o += 64) {
-        T a = T(0);
-        for (int ss = 0; ss < n; ++ss) a += R.sz2[(size_t)(off + ss) * O + o];
-        R.bg2[o] += a;
-    }
+    R.bg1[lane] += b;
+#pragma unroll
+    for (int o = 0; o < OMAX; ++o)
+        if (o < O) R.Wg2[(size_t)o * HMAX + lane] += a[o];
+    if (lane < O) R.bg2[lane] += c;
 }
 
 // Adjoint state of the robot's last state (t_bwd) into LDS, with the loss
@@ -1950,7 +2060,8 @@ __device__ __forceinline__ void wave_bwd_segment(const RArgs<T> &A, const Rob<T>
                                                  const WRegs<T, 1, OMAX> &w, const LaneTopo<T> &tp,
                                                  int r, int lane, int s0, int lo, int tb) {
     wave_bwd_steps<T, IMAX, OMAX>(A, R, m, w, tp, r, lane, s0, lo, tb);
-    wave_seg_wgrad(A, R, m.J, 7 * A.max_s, lane, lo - s0, tb - lo);
+    wave_seg_wgrad<OMAX>(A, R, m.J, wave_fold_cap(A.max_n, A.max_s, true), lane, lo - s0,
+                         tb - lo);
     wave_sync();
 }
 
@@ -2102,7 +2213,8 @@ __global__ void __launch_bounds__(512) k_wave_bwd_pipe(RArgs<T> A, int n_robots)
             if (role == 0) {
                 if (go) wave_bwd_steps<T, IMAX, OMAX, true>(A, R, m, w, tp, r, lane, 0, lo, tb);
             } else if (ftb > flo) {
-                wave_seg_wgrad(A, R, m.J, 3 * A.max_s, lane, flo, ftb - flo);
+                wave_seg_wgrad<OMAX>(A, R, m.J, wave_fold_cap(A.max_n, A.max_s, false), lane,
+                                     flo, ftb - flo);
             }
         } else if (role == 0) {
             if (go) {
@@ -2112,7 +2224,8 @@ __global__ void __launch_bounds__(512) k_wave_bwd_pipe(RArgs<T> A, int n_robots)
         } else {
             carve_scratch(R, buf ? scr0 : scr1, K, A.H);
             if (ftb > flo) {
-                wave_seg_wgrad(A, R, m.J, 3 * A.max_s, lane, flo - fs0, ftb - flo);
+                wave_seg_wgrad<OMAX>(A, R, m.J, wave_fold_cap(A.max_n, A.max_s, false), lane,
+                                     flo - fs0, ftb - flo);
                 // the fold's reads of this buffer complete before it is rebuilt
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
                 wave_sync();
@@ -2134,10 +2247,12 @@ __global__ void __launch_bounds__(512) k_wave_bwd_pipe(RArgs<T> A, int n_robots)
     if (role == 1 && ftb > flo) {
         if constexpr (STORED) {
             carve_scratch(R, scr0, A.K, A.H);
-            wave_seg_wgrad(A, R, m.J, 3 * A.max_s, lane, flo, ftb - flo);
+            wave_seg_wgrad<OMAX>(A, R, m.J, wave_fold_cap(A.max_n, A.max_s, false), lane,
+                                 flo, ftb - flo);
         } else {
             carve_scratch(R, buf ? scr0 : scr1, K, A.H);
-            wave_seg_wgrad(A, R, m.J, 3 * A.max_s, lane, flo - fs0, ftb - flo);
+            wave_seg_wgrad<OMAX>(A, R, m.J, wave_fold_cap(A.max_n, A.max_s, false), lane,
+                                 flo - fs0, ftb - flo);
         }
     }
 }
@@ -2309,7 +2424,8 @@ static int launch_rollout(const dl_rollout *r, bool bwd, hipStream_t st) {
     const int variant = r->variant;
     const int n_tab = r->lay.run_period * r->lay.n_sin;
     const bool wave_ok = sizeof(T) == 4 && r->n_hidden == HMAX && r->max_i <= 192 &&
-                         r->max_o <= 8 && r->max_n <= 64 && 7 * r->max_s >= HMAX && r->lay.n_tv >= 1 &&
+                         r->max_o <= 8 && r->max_n <= 64 && 7 * r->max_s >= HMAX &&
+                         wave_fold_cap(r->max_n, r->max_s, true) >= kFoldStepReals && r->lay.n_tv >= 1 &&
                          (r->block == 64 || r->block == 256) &&
                          wave_smem_bytes(r->max_n, r->max_s, r->max_i, r->max_o, bwd, n_tab,
                                          r->block / 64) <= kLdsLimit;
@@ -2318,6 +2434,7 @@ static int launch_rollout(const dl_rollout *r, bool bwd, hipStream_t st) {
     // pipelined adjoint: a second wavefront per robot (the scratch holds two
     // segment buffers, see dl_scratch_size)
     const bool pipe_ok = wave_ok && r->scratch != nullptr && 3 * r->max_s >= HMAX &&
+                         wave_fold_cap(r->max_n, r->max_s, false) >= kFoldStepReals &&
                          wave_pipe_smem_bytes(r->max_n, r->max_s, r->max_i, r->max_o, n_tab,
                                               r->block / 64) <= kLdsLimit;
     DL_CHECK_ARG((variant != 3 && variant != 4) || pipe_ok,
