@@ -1913,8 +1913,30 @@ __device__ __forceinline__ void wave_seg_wgrad(const RArgs<T> &A, const Rob<T> &
             for (int u = 0; u < cn; ++u) {
                 const T x = xl[u * HMAX + lane];
                 const T *z = zl + u * HMAX;
+                if constexpr (std::is_same<T, float>::value) {
+                    // z1 row as 16-byte broadcasts, 16 entries per pinned
+                    // round (64 reads in flight beside the 64 accumulators
+                    // would not fit; scalar reads each wait on the last)
 #pragma unroll
-                for (int j = 0; j < HMAX; ++j) acc[j] += z[j] * x;
+                    for (int j0 = 0; j0 < HMAX; j0 += 16) {
+                        float4 zz[4];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            zz[q] = *reinterpret_cast<const float4 *>(z + j0 + 4 * q);
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            acc[j0 + 4 * q + 0] += zz[q].x * x;
+                            acc[j0 + 4 * q + 1] += zz[q].y * x;
+                            acc[j0 + 4 * q + 2] += zz[q].z * x;
+                            acc[j0 + 4 * q + 3] += zz[q].w * x;
+                        }
+                        __builtin_amdgcn_sched_group_barrier(kSchedDsRead, 4, 0);
+                        __builtin_amdgcn_sched_group_barrier(kSchedValu, 16, 0);
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < HMAX; ++j) acc[j] += z[j] * x;
+                }
             }
         }
         if (ok) {
