@@ -1338,8 +1338,29 @@ __device__ __forceinline__ void gather_J(const Rob<T> &R, const WSm<T> &m, const
         vi[1] += sg * j1;
         vi[2] += sg * j2;
     }
-#pragma unroll 4
-    for (int e = R.vptr[i] + DK; e < R.vptr[i + 1]; ++e) {
+    // the rest of the list four entries at a time: the four indices, then
+    // their rows, then the sums (groups pinned), so each round costs two
+    // LDS round trips instead of one per index and one per row
+    const int e1 = R.vptr[i + 1];
+    int e = R.vptr[i] + DK;
+    for (; e + 4 <= e1; e += 4) {
+        int ent[4];
+        T jr[4][3];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) ent[q] = R.vidx[e + q];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) row3_get(m.J, ent[q] >> 1, jr[q][0], jr[q][1], jr[q][2]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const T sg = (ent[q] & 1) ? T(1) : T(-1);
+            vi[0] += sg * jr[q][0]; vi[1] += sg * jr[q][1]; vi[2] += sg * jr[q][2];
+        }
+        __builtin_amdgcn_sched_group_barrier(kSchedDsRead, 4, 0);
+        __builtin_amdgcn_sched_group_barrier(kSchedValu, 12, 0);
+        __builtin_amdgcn_sched_group_barrier(kSchedDsRead, 4, 0);
+        __builtin_amdgcn_sched_group_barrier(kSchedValu, 24, 0);
+    }
+    for (; e < e1; ++e) {
         const int ent = R.vidx[e], s = ent >> 1;
         const T sg = (ent & 1) ? T(1) : T(-1);
         T j0, j1, j2;
@@ -1731,8 +1752,35 @@ __device__ __forceinline__ void wave_bwd_step(const RArgs<T> &A, const Rob<T> &R
                 gv[c] += sg * g[3 + c];
             }
         }
-#pragma unroll 4
-        for (int e = R.vptr[i] + DK; e < R.vptr[i + 1]; ++e) {
+        // four entries per round (indices, then rows, then sums)
+        const int e1 = R.vptr[i + 1];
+        int e = R.vptr[i] + DK;
+        for (; e + 4 <= e1; e += 4) {
+            int ent[4];
+            T gr[4][6];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) ent[q] = R.vidx[e + q];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int s = ent[q] >> 1;
+                row3_get(G, 2 * s, gr[q][0], gr[q][1], gr[q][2]);
+                row3_get(G, 2 * s + 1, gr[q][3], gr[q][4], gr[q][5]);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const T sg = (ent[q] & 1) ? T(-1) : T(1);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    gx[c] += sg * gr[q][c];
+                    gv[c] += sg * gr[q][3 + c];
+                }
+            }
+            __builtin_amdgcn_sched_group_barrier(kSchedDsRead, 4, 0);
+            __builtin_amdgcn_sched_group_barrier(kSchedValu, 16, 0);
+            __builtin_amdgcn_sched_group_barrier(kSchedDsRead, 8, 0);
+            __builtin_amdgcn_sched_group_barrier(kSchedValu, 40, 0);
+        }
+        for (; e < e1; ++e) {
             const int ent = R.vidx[e], s = ent >> 1;
             const T sg = (ent & 1) ? T(-1) : T(1);
             T g[6];
