@@ -21,7 +21,7 @@ constexpr int HMAX = 64;
 // Instruction classes of __builtin_amdgcn_sched_group_barrier (LLVM AMDGPU):
 // the wave kernels pin "all LDS reads, then the arithmetic" in their LDS-bound
 // inner loops, which the compiler otherwise serializes under register pressure.
-constexpr int kSchedValu = 0x002, kSchedDsRead = 0x100;
+constexpr int kSchedValu = 0x002, kSchedDsRead = 0x100, kSchedRead = 0x120;   // DS or VMEM reads
 
 template <typename T>
 struct RArgs {
@@ -346,8 +346,28 @@ __device__ void impulse_adj_pass(const RArgs<T> &A, const Rob<T> &R, const Sm<T>
 template <bool ROWS_, bool TL, typename T>
 __device__ __forceinline__ void gather_j(const Rob<T> &R, const T *J, int S, int i, T (&vi)[3]) {
     const auto vptr = as_ptr<TL>(R.vptr), vidx = as_ptr<TL>(R.vidx);
-#pragma unroll 4
-    for (int e = vptr[i]; e < vptr[i + 1]; ++e) {
+    // four entries per pinned round: the indices, then their rows (LDS or
+    // global), then the sums in list order
+    const int e1 = vptr[i + 1];
+    int e = vptr[i];
+    for (; e + 4 <= e1; e += 4) {
+        int ent[4];
+        T jr[4][3];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) ent[q] = vidx[e + q];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) j_load<ROWS_>(J, S, ent[q] >> 1, jr[q]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const T sg = (ent[q] & 1) ? T(1) : T(-1);
+            vi[0] += sg * jr[q][0]; vi[1] += sg * jr[q][1]; vi[2] += sg * jr[q][2];
+        }
+        __builtin_amdgcn_sched_group_barrier(kSchedRead, 4, 0);
+        __builtin_amdgcn_sched_group_barrier(kSchedValu, 12, 0);
+        __builtin_amdgcn_sched_group_barrier(kSchedRead, ROWS_ ? 4 : 12, 0);
+        __builtin_amdgcn_sched_group_barrier(kSchedValu, 24, 0);
+    }
+    for (; e < e1; ++e) {
         const int ent = vidx[e], s = ent >> 1;
         const T sg = (ent & 1) ? T(1) : T(-1);
         T j[3];
@@ -359,8 +379,31 @@ template <bool ROWS_, bool TL, typename T>
 __device__ __forceinline__ void gather_g(const Rob<T> &R, const T *G, int S, int i, T (&gx)[3],
                                          T (&gv)[3]) {
     const auto vptr = as_ptr<TL>(R.vptr), vidx = as_ptr<TL>(R.vidx);
-#pragma unroll 4
-    for (int e = vptr[i]; e < vptr[i + 1]; ++e) {
+    // four entries per pinned round (indices, rows, sums in list order)
+    const int e1 = vptr[i + 1];
+    int e = vptr[i];
+    for (; e + 4 <= e1; e += 4) {
+        int ent[4];
+        T gd[4][3], gw[4][3];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) ent[q] = vidx[e + q];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) g_load<ROWS_>(G, S, ent[q] >> 1, gd[q], gw[q]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const T sg = (ent[q] & 1) ? T(-1) : T(1);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                gx[c] += sg * gd[q][c];
+                gv[c] += sg * gw[q][c];
+            }
+        }
+        __builtin_amdgcn_sched_group_barrier(kSchedRead, 4, 0);
+        __builtin_amdgcn_sched_group_barrier(kSchedValu, 12, 0);
+        __builtin_amdgcn_sched_group_barrier(kSchedRead, ROWS_ ? 8 : 24, 0);
+        __builtin_amdgcn_sched_group_barrier(kSchedValu, 48, 0);
+    }
+    for (; e < e1; ++e) {
         const int ent = vidx[e], s = ent >> 1;
         const T sg = (ent & 1) ? T(-1) : T(1);
         T gd[3], gw[3];
