@@ -1948,7 +1948,9 @@ __device__ __forceinline__ void wave_bwd_step(const RArgs<T> &A, const Rob<T> &R
 // The loads walk per-lane pointers (the hidden rows are HMAX wide in the
 // wave kernels, so their offsets are immediates): the kernel has no scalar
 // registers to spare for a row address per load.  Steps are summed in
-// ascending order.
+// ascending order.  Only the W1 rows of inputs [ia, ib) are folded, and the
+// b1, W2 and b2 gradients only with `part2`, so two wavefronts can fold one
+// segment between them, each into entries of its own.
 constexpr int kFoldChunk = 16;
 // LDS reals per staged step: z1 and input rows (first pass); z1, hidden and
 // z2 rows (second pass, z2 at most 8 wide)
@@ -1963,13 +1965,14 @@ __host__ __device__ inline int wave_fold_cap(int max_n, int max_s, bool bwd) {
 
 template <int OMAX, typename T>
 __device__ __forceinline__ void wave_seg_wgrad(const RArgs<T> &A, const Rob<T> &R, T *zl, int zcap,
-                                               int lane, int off, int n) {
+                                               int lane, int off, int n, int ia = 0,
+                                               int ib = 1 << 30, bool part2 = true) {
     static_assert(OMAX <= kFoldStepReals - 2 * HMAX, "z2 rows wider than the fold's staging");
     const int I = R.I, O = R.O;
     constexpr int CB = kFoldChunk;
     const int chunk = min(zcap / (2 * HMAX), CB);
     T *const xl = zl + chunk * HMAX;
-    for (int i0 = 0; i0 < I; i0 += 64) {
+    for (int i0 = ia; i0 < min(I, ib); i0 += 64) {
         const int i = i0 + lane;
         const bool ok = i < I;
         T acc[HMAX];
@@ -2045,6 +2048,7 @@ __device__ __forceinline__ void wave_seg_wgrad(const RArgs<T> &A, const Rob<T> &
             }
         }
     }
+    if (!part2) return;
     // lane j sums z1[s][j] and z2[s][o] hid[s][j]; lane o < O sums z2[s][o]
     const int chunk2 = min(zcap / kFoldStepReals, CB);
     T *const hl = zl + chunk2 * HMAX, *const ql = hl + chunk2 * HMAX;
@@ -2298,6 +2302,12 @@ __global__ void __launch_bounds__(512) k_wave_bwd_pipe(RArgs<T> A, int n_robots)
     Rob<T> R = make_rob(A, active ? r : r0);
     wave_point_at_shared(R, sh);
     T *const scr0 = R.sx, *const scr1 = R.scr2;   // scratch buffers 0 and 1
+    // Outside the stored mode the reversing wavefront folds the W1 rows of the
+    // last input block and the b1, W2, b2 sums of each segment as soon as it
+    // has reversed it (its state rows are dead then), and the recomputing
+    // wavefront folds the other input blocks one segment later.  The two write
+    // disjoint entries, and every entry still sums its steps in order.
+    const int isplit = ((R.I + 63) / 64 - 1) * 64;
     T *region = reinterpret_cast<T *>(smem_raw) + (size_t)rl * (Rb + Rf);
     WSm<T> m = role == 0 ? wave_carve_at(A, region, true) : wave_carve_at(A, region + Rb, false);
     int tb = active ? A.robots[r].t_bwd : 0, budget = budget0;
@@ -2333,12 +2343,14 @@ __global__ void __launch_bounds__(512) k_wave_bwd_pipe(RArgs<T> A, int n_robots)
             if (go) {
                 carve_scratch(R, buf ? scr1 : scr0, K, A.H);
                 wave_bwd_steps<T, IMAX, OMAX, true>(A, R, m, w, tp, r, lane, s0, lo, tb);
+                wave_seg_wgrad<OMAX>(A, R, m.J, wave_fold_cap(A.max_n, A.max_s, true), lane,
+                                     lo - s0, tb - lo, isplit, 1 << 30, true);
             }
         } else {
             carve_scratch(R, buf ? scr0 : scr1, K, A.H);
             if (ftb > flo) {
                 wave_seg_wgrad<OMAX>(A, R, m.J, wave_fold_cap(A.max_n, A.max_s, false), lane,
-                                     flo - fs0, ftb - flo);
+                                     flo - fs0, ftb - flo, 0, isplit, false);
                 // the fold's reads of this buffer complete before it is rebuilt
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
                 wave_sync();
@@ -2365,7 +2377,7 @@ __global__ void __launch_bounds__(512) k_wave_bwd_pipe(RArgs<T> A, int n_robots)
         } else {
             carve_scratch(R, buf ? scr0 : scr1, K, A.H);
             wave_seg_wgrad<OMAX>(A, R, m.J, wave_fold_cap(A.max_n, A.max_s, false), lane,
-                                 flo - fs0, ftb - flo);
+                                 flo - fs0, ftb - flo, 0, isplit, false);
         }
     }
 }
