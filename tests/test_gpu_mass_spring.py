@@ -480,6 +480,46 @@ def test_wave_recompute_reproduces_forward(variant):
         assert hits[0] if variant == 2 else any(hits), (r, hits)
 
 
+@pytest.mark.parametrize("n_cubes", [2, 6])
+def test_pipelined_weight_fold_split_by_input_width(n_cubes):
+    """The pipelined adjoint (variant 3) splits each segment's weight-gradient
+    fold between its two wavefronts by input block (the reversing one takes
+    the last block of 64 inputs and the b1, W2, b2 sums).  A chain of 2 cubes
+    has 53 inputs (one block: the reversing wavefront folds everything), a
+    chain of 6 has 101 (two blocks); both give the sequential adjoint's
+    (variant 2) gradients and adjoint state bit for bit.  The robots start
+    0.1 above the ground and land after about 80 steps: the horizons run past
+    that, or the gradients would be zero (internal forces do not move the
+    centre of mass in free fall)."""
+    from diffloco import robot_design as rd
+    _say(f"pipelined fold split, chain of {n_cubes} cubes")
+    design = rd.build_voxels([((i, 0, 0), True, i % 2) for i in range(n_cubes)], n_act=2)
+    I = om.TaskConfig().layout.n_input(design.n_vert)
+    assert I == {2: 53, 6: 101}[n_cubes]
+    horizons = [160, 121, 144, 133]
+    B, K, W = len(horizons), 16, 60
+    tv, th = task_targets(B, 29)
+    w64 = [w.to("cuda", torch.float32) for w in om.init_weights(I, 64, 2, seed=30)]
+    qc = dl.quadruped_config(batch_size=B, steps=max(horizons), window=W).get_config()
+    x0 = design.objects + np.array([0.4, 0.0, 0.0])
+    tg = np.concatenate([tv, th[:, None], np.zeros((B, 1))], 1)
+    out = []
+    for variant in (2, 3):
+        batch = MassSpringBatch([design], [0] * B, horizons, _physics(qc), sensor_layout(qc), tg,
+                                [x0] * B, act="tanh", ckpt_every=K, dtype=torch.float32,
+                                loss_window=W, variant=variant)
+        batch.set_weights(0, *w64)
+        batch.forward()
+        batch.backward(loss_grad=np.linspace(0.5, 1.5, B))
+        torch.cuda.synchronize()
+        out.append(([g.clone() for g in batch.weight_grads(0)], batch.adjoint.clone()))
+    (ga, aa), (gb, ab) = out
+    assert all(torch.isfinite(g).all() and g.abs().max() > 0 for g in ga)
+    for x, y in zip(ga, gb):
+        assert torch.equal(x, y)
+    assert torch.equal(aa, ab)
+
+
 def _uneven(prec, variant, seed=31, block=256, lds_policy=0):
     from diffloco import robot_design as rd
     rng = np.random.default_rng(seed)
