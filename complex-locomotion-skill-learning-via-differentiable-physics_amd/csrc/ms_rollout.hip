@@ -1125,10 +1125,17 @@ __host__ __device__ inline size_t wave_region(int max_n, int max_s, int max_i, i
     return (n + 3) & ~(size_t)3;
 }
 
+// Reals of the forward kernel's W1 copy: inputs in groups of four, group g
+// holding W1[j][4g .. 4g+3] at 4 (g HMAX + j), so that hidden unit j reads
+// one 16-byte entry per group; the last group padded.
+__host__ __device__ inline size_t wave_wl4_reals(int max_i) {
+    return (size_t)((max_i + 3) / 4) * 4 * HMAX;
+}
+
 __host__ inline size_t wave_smem_bytes(int max_n, int max_s, int max_i, int max_o, bool bwd,
                                        int n_tab, int waves) {
     size_t n = waves * wave_region(max_n, max_s, max_i, max_o, bwd) + 3 * (size_t)max_s + n_tab +
-               (size_t)max_i * (HMAX + 1);
+               (bwd ? (size_t)max_i * (HMAX + 1) : wave_wl4_reals(max_i));
     return n * sizeof(float) + sizeof(int) * (6 * (size_t)max_s + max_n + max_o + 2);
 }
 
@@ -1173,8 +1180,13 @@ __device__ __forceinline__ WSm<T> wave_carve(const RArgs<T> &A, T *base, int wid
                          bwd);
 }
 
-// Block-wide: stage the topology of robot r0's design and its W1 (rows of
-// pitch HMAX + 1) into the shared region; returns pointers through `sh`.
+// Block-wide: stage the topology of robot r0's design and its W1 into the
+// shared region; returns pointers through `sh`.  The adjoint kernels keep W1
+// as input rows of pitch HMAX + 1 (lane j reads W1[j][i] with 4-byte reads
+// in layer 1, lane i reads row i in the input adjoint); the forward kernel
+// (`bwd` false), which has no input adjoint, keeps it in groups of four
+// inputs (wave_wl4_reals), first in the shared region so that the groups are
+// 16-byte aligned.
 template <typename T>
 struct WShared {
     T *topo_r, *wl, *sin_tab;
@@ -1187,9 +1199,13 @@ __device__ __forceinline__ WShared<T> wave_stage(const RArgs<T> &A, const Rob<T>
                                  size_t regions) {
     WShared<T> sh;
     T *p = reinterpret_cast<T *>(raw) + regions;
-    sh.topo_r = p; p += 3 * A.max_s;
-    sh.wl = p;
-    p += (size_t)A.max_i * (HMAX + 1);
+    if (bwd) {
+        sh.topo_r = p; p += 3 * A.max_s;
+        sh.wl = p; p += (size_t)A.max_i * (HMAX + 1);
+    } else {
+        sh.wl = p; p += wave_wl4_reals(A.max_i);
+        sh.topo_r = p; p += 3 * A.max_s;
+    }
     // sinusoid table over one period: phase 2 pi (t mod P) / P + 2 pi j / n_sin
     // evaluated in double precision (periodic in t, so equal to the direct form)
     const int n_tab = A.L.run_period * A.L.n_sin;
@@ -1217,7 +1233,8 @@ __device__ __forceinline__ WShared<T> wave_stage(const RArgs<T> &A, const Rob<T>
         const int H = A.H, I = R0.I;
         for (int q = tid; q < I * H; q += BT) {
             const int i = q / H, j = q - i * H;
-            sh.wl[i * (HMAX + 1) + j] = R0.W1T[q];
+            if (bwd) sh.wl[i * (HMAX + 1) + j] = R0.W1T[q];
+            else sh.wl[4 * ((i >> 2) * HMAX + j) + (i & 3)] = R0.W1T[q];
         }
     }
     return sh;
@@ -1451,8 +1468,9 @@ __device__ __forceinline__ void wave_sensors(const RArgs<T> &A, const Rob<T> &R,
 // one forward step of the wave's robot (MODE_FWD / MODE_REC like fwd_step).
 // ROWS: a forward step also writes the step's adjoint rows (as MODE_REC does)
 // at scratch slot `slot`, for the adjoint that reads them instead of
-// recomputing.
-template <typename T, int OMAX, int SK, int DK, bool ROWS = false>
+// recomputing.  WL4: the LDS copy of W1 is in groups of four inputs (the
+// forward kernel's layout), else in rows of pitch HMAX + 1.
+template <typename T, int OMAX, int SK, int DK, bool ROWS = false, bool WL4 = false>
 __device__ __forceinline__ void wave_fwd_step(const RArgs<T> &A, const Rob<T> &R, WSm<T> &m,
                               const WRegs<T, 1, OMAX> &w, const LaneTopo<T> &tp,
                               const T *sin_tab, int lane, int t, int mode, int slot, T &lsum) {
@@ -1511,18 +1529,44 @@ __device__ __forceinline__ void wave_fwd_step(const RArgs<T> &A, const Rob<T> &R
         T a[4] = {acc, T(0), T(0), T(0)};
         const T *wl = R.WL + lane;
         int i = 0;
-        for (; i + 16 <= I; i += 16) {
-            T wv[16], xv[16];
+        if constexpr (WL4) {
+            static_assert(std::is_same<T, float>::value, "grouped W1 copy: FP32 wave kernels");
+            // one 16-byte read per lane and group of four inputs, the inputs
+            // as 16-byte broadcasts (m.inb is 16-byte aligned); input i
+            // still goes into a[i % 4], in the same order as below
+            const float4 *w4 = reinterpret_cast<const float4 *>(R.WL) + lane;
+            const float4 *x4 = reinterpret_cast<const float4 *>(m.inb);
+            for (; i + 16 <= I; i += 16) {
+                float4 wv[4], xv[4];
 #pragma unroll
-            for (int q = 0; q < 16; ++q) wv[q] = wl[(i + q) * (HMAX + 1)];
+                for (int q = 0; q < 4; ++q) wv[q] = w4[((i >> 2) + q) * HMAX];
 #pragma unroll
-            for (int q = 0; q < 16; ++q) xv[q] = m.inb[i + q];
+                for (int q = 0; q < 4; ++q) xv[q] = x4[(i >> 2) + q];
 #pragma unroll
-            for (int q = 0; q < 16; ++q) a[q & 3] += wv[q] * xv[q];
-            __builtin_amdgcn_sched_group_barrier(kSchedDsRead, 32, 0);
-            __builtin_amdgcn_sched_group_barrier(kSchedValu, 16, 0);
+                for (int q = 0; q < 4; ++q) {
+                    a[0] += wv[q].x * xv[q].x;
+                    a[1] += wv[q].y * xv[q].y;
+                    a[2] += wv[q].z * xv[q].z;
+                    a[3] += wv[q].w * xv[q].w;
+                }
+                __builtin_amdgcn_sched_group_barrier(kSchedDsRead, 8, 0);
+                __builtin_amdgcn_sched_group_barrier(kSchedValu, 16, 0);
+            }
+            for (; i < I; ++i) a[i & 3] += R.WL[4 * ((i >> 2) * HMAX + lane) + (i & 3)] * m.inb[i];
+        } else {
+            for (; i + 16 <= I; i += 16) {
+                T wv[16], xv[16];
+#pragma unroll
+                for (int q = 0; q < 16; ++q) wv[q] = wl[(i + q) * (HMAX + 1)];
+#pragma unroll
+                for (int q = 0; q < 16; ++q) xv[q] = m.inb[i + q];
+#pragma unroll
+                for (int q = 0; q < 16; ++q) a[q & 3] += wv[q] * xv[q];
+                __builtin_amdgcn_sched_group_barrier(kSchedDsRead, 32, 0);
+                __builtin_amdgcn_sched_group_barrier(kSchedValu, 16, 0);
+            }
+            for (; i < I; ++i) a[i & 3] += wl[i * (HMAX + 1)] * m.inb[i];
         }
-        for (; i < I; ++i) a[i & 3] += wl[i * (HMAX + 1)] * m.inb[i];
         acc = (a[0] + a[1]) + (a[2] + a[3]);
     } else if (lane < H) {
 #pragma unroll 8
@@ -1622,7 +1666,8 @@ __global__ void __launch_bounds__(256) k_wave_fwd(RArgs<T> A, int n_robots) {
     wave_sync();
     T lsum = T(0);
     for (int t = t0; t < t1; ++t)
-        wave_fwd_step<T, OMAX, FSK, FDK, ROWS>(A, R, m, w, tp, sh.sin_tab, lane, t, MODE_FWD, t, lsum);
+        wave_fwd_step<T, OMAX, FSK, FDK, ROWS, true>(A, R, m, w, tp, sh.sin_tab, lane, t, MODE_FWD, t,
+                                                    lsum);
     if (t1 == R.Th) {
         T s6[6] = {0, 0, 0, 0, 0, 0};
         const T invN = T(1) / T(N);
