@@ -1125,17 +1125,20 @@ __host__ __device__ inline size_t wave_region(int max_n, int max_s, int max_i, i
     return (n + 3) & ~(size_t)3;
 }
 
-// Reals of the forward kernel's W1 copy: inputs in groups of four, group g
-// holding W1[j][4g .. 4g+3] at 4 (g HMAX + j), so that hidden unit j reads
-// one 16-byte entry per group; the last group padded.
+// The wave kernels' LDS copy of W1: inputs in groups of four, entry (g, j)
+// = W1[j][4g .. 4g+3] at 4 (g (HMAX + 1) + j).  Hidden unit j reads one
+// 16-byte entry per group (layer 1); input i reads W1[j][i] for j = 0, 1, ..
+// at a stride of 4 reals (input adjoint), and the group pitch of HMAX + 1
+// entries puts the 64 lanes' reads in distinct banks.  The last group padded.
 __host__ __device__ inline size_t wave_wl4_reals(int max_i) {
-    return (size_t)((max_i + 3) / 4) * 4 * HMAX;
+    return (size_t)((max_i + 3) / 4) * 4 * (HMAX + 1);
 }
+__device__ __forceinline__ int wl4_at(int i, int j) { return 4 * ((i >> 2) * (HMAX + 1) + j) + (i & 3); }
 
 __host__ inline size_t wave_smem_bytes(int max_n, int max_s, int max_i, int max_o, bool bwd,
                                        int n_tab, int waves) {
     size_t n = waves * wave_region(max_n, max_s, max_i, max_o, bwd) + 3 * (size_t)max_s + n_tab +
-               (bwd ? (size_t)max_i * (HMAX + 1) : wave_wl4_reals(max_i));
+               wave_wl4_reals(max_i);
     return n * sizeof(float) + sizeof(int) * (6 * (size_t)max_s + max_n + max_o + 2);
 }
 
@@ -1144,7 +1147,7 @@ __host__ inline size_t wave_pipe_smem_bytes(int max_n, int max_s, int max_i, int
                                             int robots) {
     size_t n = robots * (wave_region(max_n, max_s, max_i, max_o, true) +
                          wave_region(max_n, max_s, max_i, max_o, false)) +
-               3 * (size_t)max_s + n_tab + (size_t)max_i * (HMAX + 1);
+               3 * (size_t)max_s + n_tab + wave_wl4_reals(max_i);
     return n * sizeof(float) + sizeof(int) * (6 * (size_t)max_s + max_n + max_o + 2);
 }
 
@@ -1180,13 +1183,9 @@ __device__ __forceinline__ WSm<T> wave_carve(const RArgs<T> &A, T *base, int wid
                          bwd);
 }
 
-// Block-wide: stage the topology of robot r0's design and its W1 into the
-// shared region; returns pointers through `sh`.  The adjoint kernels keep W1
-// as input rows of pitch HMAX + 1 (lane j reads W1[j][i] with 4-byte reads
-// in layer 1, lane i reads row i in the input adjoint); the forward kernel
-// (`bwd` false), which has no input adjoint, keeps it in groups of four
-// inputs (wave_wl4_reals), first in the shared region so that the groups are
-// 16-byte aligned.
+// Block-wide: stage the topology of robot r0's design and its W1 (groups of
+// four inputs, wave_wl4_reals; first in the shared region, so 16-byte
+// aligned) into the shared region; returns pointers through `sh`.
 template <typename T>
 struct WShared {
     T *topo_r, *wl, *sin_tab;
@@ -1199,13 +1198,8 @@ __device__ __forceinline__ WShared<T> wave_stage(const RArgs<T> &A, const Rob<T>
                                  size_t regions) {
     WShared<T> sh;
     T *p = reinterpret_cast<T *>(raw) + regions;
-    if (bwd) {
-        sh.topo_r = p; p += 3 * A.max_s;
-        sh.wl = p; p += (size_t)A.max_i * (HMAX + 1);
-    } else {
-        sh.wl = p; p += wave_wl4_reals(A.max_i);
-        sh.topo_r = p; p += 3 * A.max_s;
-    }
+    sh.wl = p; p += wave_wl4_reals(A.max_i);
+    sh.topo_r = p; p += 3 * A.max_s;
     // sinusoid table over one period: phase 2 pi (t mod P) / P + 2 pi j / n_sin
     // evaluated in double precision (periodic in t, so equal to the direct form)
     const int n_tab = A.L.run_period * A.L.n_sin;
@@ -1233,8 +1227,7 @@ __device__ __forceinline__ WShared<T> wave_stage(const RArgs<T> &A, const Rob<T>
         const int H = A.H, I = R0.I;
         for (int q = tid; q < I * H; q += BT) {
             const int i = q / H, j = q - i * H;
-            if (bwd) sh.wl[i * (HMAX + 1) + j] = R0.W1T[q];
-            else sh.wl[4 * ((i >> 2) * HMAX + j) + (i & 3)] = R0.W1T[q];
+            sh.wl[wl4_at(i, j)] = R0.W1T[q];
         }
     }
     return sh;
@@ -1468,9 +1461,8 @@ __device__ __forceinline__ void wave_sensors(const RArgs<T> &A, const Rob<T> &R,
 // one forward step of the wave's robot (MODE_FWD / MODE_REC like fwd_step).
 // ROWS: a forward step also writes the step's adjoint rows (as MODE_REC does)
 // at scratch slot `slot`, for the adjoint that reads them instead of
-// recomputing.  WL4: the LDS copy of W1 is in groups of four inputs (the
-// forward kernel's layout), else in rows of pitch HMAX + 1.
-template <typename T, int OMAX, int SK, int DK, bool ROWS = false, bool WL4 = false>
+// recomputing.
+template <typename T, int OMAX, int SK, int DK, bool ROWS = false>
 __device__ __forceinline__ void wave_fwd_step(const RArgs<T> &A, const Rob<T> &R, WSm<T> &m,
                               const WRegs<T, 1, OMAX> &w, const LaneTopo<T> &tp,
                               const T *sin_tab, int lane, int t, int mode, int slot, T &lsum) {
@@ -1522,51 +1514,34 @@ __device__ __forceinline__ void wave_fwd_step(const RArgs<T> &A, const Rob<T> &R
     // recompute reproduces the forward's hidden values bit for bit.
     T acc = w.b1;
     if (R.WL) {
-        // 16 inputs per round: their 32 LDS reads are issued before the FMAs
-        // (the scheduling groups pin that order; left to itself the compiler
-        // reuses one register pair and waits on every read once the kernel's
-        // register pressure grows)
+        // 16 inputs per round: one 16-byte read of W1 per lane and group of
+        // four inputs (the LDS copy's grouped layout), the inputs as 16-byte
+        // broadcasts (m.inb is 16-byte aligned); input i goes into a[i % 4].
+        // The scheduling groups keep a round's 8 reads ahead of its 16 FMAs
+        // (left to itself the compiler reuses registers and waits on every
+        // read once the kernel's register pressure grows).
+        static_assert(std::is_same<T, float>::value, "LDS copy of W1: FP32 wave kernels");
         T a[4] = {acc, T(0), T(0), T(0)};
-        const T *wl = R.WL + lane;
+        const float4 *w4 = reinterpret_cast<const float4 *>(R.WL) + lane;
+        const float4 *x4 = reinterpret_cast<const float4 *>(m.inb);
         int i = 0;
-        if constexpr (WL4) {
-            static_assert(std::is_same<T, float>::value, "grouped W1 copy: FP32 wave kernels");
-            // one 16-byte read per lane and group of four inputs, the inputs
-            // as 16-byte broadcasts (m.inb is 16-byte aligned); input i
-            // still goes into a[i % 4], in the same order as below
-            const float4 *w4 = reinterpret_cast<const float4 *>(R.WL) + lane;
-            const float4 *x4 = reinterpret_cast<const float4 *>(m.inb);
-            for (; i + 16 <= I; i += 16) {
-                float4 wv[4], xv[4];
+        for (; i + 16 <= I; i += 16) {
+            float4 wv[4], xv[4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) wv[q] = w4[((i >> 2) + q) * HMAX];
+            for (int q = 0; q < 4; ++q) wv[q] = w4[((i >> 2) + q) * (HMAX + 1)];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) xv[q] = x4[(i >> 2) + q];
+            for (int q = 0; q < 4; ++q) xv[q] = x4[(i >> 2) + q];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    a[0] += wv[q].x * xv[q].x;
-                    a[1] += wv[q].y * xv[q].y;
-                    a[2] += wv[q].z * xv[q].z;
-                    a[3] += wv[q].w * xv[q].w;
-                }
-                __builtin_amdgcn_sched_group_barrier(kSchedDsRead, 8, 0);
-                __builtin_amdgcn_sched_group_barrier(kSchedValu, 16, 0);
+            for (int q = 0; q < 4; ++q) {
+                a[0] += wv[q].x * xv[q].x;
+                a[1] += wv[q].y * xv[q].y;
+                a[2] += wv[q].z * xv[q].z;
+                a[3] += wv[q].w * xv[q].w;
             }
-            for (; i < I; ++i) a[i & 3] += R.WL[4 * ((i >> 2) * HMAX + lane) + (i & 3)] * m.inb[i];
-        } else {
-            for (; i + 16 <= I; i += 16) {
-                T wv[16], xv[16];
-#pragma unroll
-                for (int q = 0; q < 16; ++q) wv[q] = wl[(i + q) * (HMAX + 1)];
-#pragma unroll
-                for (int q = 0; q < 16; ++q) xv[q] = m.inb[i + q];
-#pragma unroll
-                for (int q = 0; q < 16; ++q) a[q & 3] += wv[q] * xv[q];
-                __builtin_amdgcn_sched_group_barrier(kSchedDsRead, 32, 0);
-                __builtin_amdgcn_sched_group_barrier(kSchedValu, 16, 0);
-            }
-            for (; i < I; ++i) a[i & 3] += wl[i * (HMAX + 1)] * m.inb[i];
+            __builtin_amdgcn_sched_group_barrier(kSchedDsRead, 8, 0);
+            __builtin_amdgcn_sched_group_barrier(kSchedValu, 16, 0);
         }
+        for (; i < I; ++i) a[i & 3] += R.WL[wl4_at(i, lane)] * m.inb[i];
         acc = (a[0] + a[1]) + (a[2] + a[3]);
     } else if (lane < H) {
 #pragma unroll 8
@@ -1666,8 +1641,7 @@ __global__ void __launch_bounds__(256) k_wave_fwd(RArgs<T> A, int n_robots) {
     wave_sync();
     T lsum = T(0);
     for (int t = t0; t < t1; ++t)
-        wave_fwd_step<T, OMAX, FSK, FDK, ROWS, true>(A, R, m, w, tp, sh.sin_tab, lane, t, MODE_FWD, t,
-                                                    lsum);
+        wave_fwd_step<T, OMAX, FSK, FDK, ROWS>(A, R, m, w, tp, sh.sin_tab, lane, t, MODE_FWD, t, lsum);
     if (t1 == R.Th) {
         T s6[6] = {0, 0, 0, 0, 0, 0};
         const T invN = T(1) / T(N);
@@ -1720,9 +1694,11 @@ __device__ __forceinline__ void pref_load(const Rob<T> &R, int H, int lane, int 
     p.cv = lane < 6 ? R.scv[slot * 6 + lane] : T(0);
 }
 
-// acc[p] += rows[p][j0 + k] * z_k in order k = 0..15, z_k from lane k of the
-// lane's 16-lane row of z.  The 16 NP row entries are read from LDS into
-// registers before the FMAs (order pinned by scheduling groups).
+// acc[p] += rows[p][4 (j0 + k)] * z_k in order k = 0..15 (rows[p] points at
+// W1[0][i] of the lane's input i in the grouped LDS copy, where W1[j][i] is
+// 4 j further), z_k from lane k of the lane's 16-lane row of z.  The 16 NP
+// entries are read from LDS into registers before the FMAs (order pinned by
+// scheduling groups).
 template <int NP, int... K>
 __device__ __forceinline__ void in_adj_rows(float (&acc)[NP], const float *const (&rows)[NP], int j0,
                                             float z, std::integer_sequence<int, K...>) {
@@ -1730,7 +1706,7 @@ __device__ __forceinline__ void in_adj_rows(float (&acc)[NP], const float *const
 #pragma unroll
     for (int p = 0; p < NP; ++p)
 #pragma unroll
-        for (int k = 0; k < 16; ++k) rv[p][k] = rows[p][j0 + k];
+        for (int k = 0; k < 16; ++k) rv[p][k] = rows[p][4 * (j0 + k)];
     auto one = [&](float zk, int k) {
 #pragma unroll
         for (int p = 0; p < NP; ++p) acc[p] += rv[p][k] * zk;
@@ -1922,7 +1898,7 @@ __device__ __forceinline__ void wave_bwd_step(const RArgs<T> &A, const Rob<T> &R
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
             const int i = lane + 64 * p;
-            rows[p] = reinterpret_cast<const float *>(R.WL) + (i < I ? i : I - 1) * (HMAX + 1);
+            rows[p] = reinterpret_cast<const float *>(R.WL) + wl4_at(i < I ? i : I - 1, 0);
             acc[p] = 0.f;
         }
 #pragma unroll 1
