@@ -43,8 +43,8 @@ UNEVEN_DASHPOT = 0.03
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 MEASURED_KIB = {
     # (kernel, robots, horizon, checkpoint interval): (FETCH_SIZE, WRITE_SIZE)
-    ("k_wave_bwd_pipe", 1024, 1000, 50): (3020123.4, 4624119.3),
-    ("k_wave_fwd", 1024, 1000, 50): (1243.1, 27251.2),
+    ("k_wave_bwd_pipe", 1024, 1000, 50): (3029464.6, 4624975.7),
+    ("k_wave_fwd", 1024, 1000, 50): (1250.6, 27251.4),
 }
 # MLS-MPM, 8 robots x 27 648 particles on 64^3 grids, checkpoints every 100
 # steps, replayed segments: FETCH_SIZE and WRITE_SIZE (KiB) of one step of
