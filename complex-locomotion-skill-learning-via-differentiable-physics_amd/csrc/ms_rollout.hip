@@ -1100,10 +1100,11 @@ __global__ void __launch_bounds__(LB) k_rollout_bwd(RArgs<T> A) {
 //
 // A workgroup holds WAVES robots, one wavefront each, so every SIMD of a CU
 // advances its own robot with no workgroup barrier inside the step loop.
-// Lane j keeps column j of W1^T (the weights of hidden unit j) in VGPRs for
-// the forward GEMV; the adjoint GEMV W1^T z reads one LDS copy of W1 shared
-// by the workgroup's robots (robots of one design are packed together).  The
-// design topology is staged into LDS once per workgroup.
+// Both GEMVs read one LDS copy of W1 shared by the workgroup's robots
+// (robots of one design are packed together; groups of four inputs, see
+// wave_wl4_reals): the forward's lane j reads hidden unit j's weights 16
+// bytes at a time, the adjoint's W1^T z has lane i read input i's weights.
+// The design topology is staged into LDS once per workgroup.
 // ===================================================================
 
 
